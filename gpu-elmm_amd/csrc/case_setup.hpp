@@ -1,0 +1,64 @@
+// case_setup.hpp - host-side case description: grid coordinates, masks and
+// point lists in the reference's conventions (1-based i,j,k; ghosted arrays
+// with lower bound -2 and extents n+6, Fortran order).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "elmm_gpu.h"
+
+namespace elmm {
+
+// 1-D grid coordinate arrays of InitBoundaryConditions (src/initial.f90:
+// 4395-4502), stored with offset 3 so that index -3 is element 0.
+struct Grid1D {
+  int nx = 0, ny = 0, nz = 0;
+  double dxmin = 0, dymin = 0, dzmin = 0;
+  std::vector<double> xU, yV, zW, xPr, yPr, zPr, dxPr, dyPr, dzPr, dxU, dyV, dzW;
+  static double &at(std::vector<double> &v, int i) { return v[i + 3]; }
+  static double at(const std::vector<double> &v, int i) { return v[i + 3]; }
+};
+
+struct WMList {
+  std::vector<int32_t> ijk;  // 3 per point
+  std::vector<double> dist;  // 3 per point
+  std::vector<double> z0;
+};
+
+struct IBList {
+  std::vector<int32_t> ijk;      // target, 3 per point
+  std::vector<int32_t> ptr{0};   // n+1 offsets
+  std::vector<int32_t> src;      // interpolation sources, 3 per tap
+  std::vector<double> coef;      // coefficient per tap
+};
+
+struct CaseData {
+  elmm_config cfg{};
+  int nx = 0, ny = 0, nz = 0;
+  int n[3][3] = {};  // component extents n[c][d]
+  long sy = 0, sz = 0, ntot = 0;
+  std::vector<int32_t> prtype, utype, vtype, wtype;
+  std::vector<uint8_t> scfl_mask[3];
+  WMList wm[18];
+  WMList wmp;
+  std::vector<int32_t> scfl[3];
+  IBList ib[4];
+  std::vector<int32_t> null_ijk[3];
+  std::vector<int32_t> src_ijk, src_sc;
+  std::vector<double> src_flux;
+  long lin(int i, int j, int k) const { return (long)(i + 2) + (long)(j + 2) * sy + (long)(k + 2) * sz; }
+};
+
+Grid1D make_grid(const elmm_config &c);
+void component_extents(const elmm_config &c, int n[3][3]);
+std::string validate_config(const elmm_config &c);
+void remove_duplicate_wm_points(const elmm_config &c, WMList &L);
+int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc, const elmm_point_source *src,
+               CaseData &cd);
+void view_case(const CaseData &cd, elmm_lists &v);
+// Solid bodies (axis-aligned cuboids): masks, IB lists, solid-body WM points,
+// nullified points.  Defined in solids.cpp.
+int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cuboid *cub, CaseData &cd);
+
+}  // namespace elmm

@@ -1,0 +1,66 @@
+// device_common.hpp - HBM layout and shared device helpers of the gfx950
+// LES kernels.
+//
+// Every 3-D field lives in one layout: index space -2..n+3 in each direction
+// (n = Prnx/Prny/Prnz), x fastest, with a front pad so that interior element
+// i=1 of every row starts on a 128-byte boundary for FP64 (64 bytes for FP32)
+// and the row pitch a multiple of 16 elements.  A wavefront of 64 lanes
+// sweeping i=1..64 of a row therefore issues aligned, fully coalesced loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace elmm {
+
+constexpr int kFrontPad = 13;  // elements before i=-2 in a row
+constexpr int kRowAlign = 16;  // row pitch multiple (elements)
+
+struct Geo {
+  int nx, ny, nz;     // Prnx, Prny, Prnz
+  int n[3][3];        // component extents: n[c][d]
+  long sy, sz;        // strides in elements
+  long base;          // element offset of (0,0,0)
+  long ntot;          // elements per field allocation
+  double dxmin, dymin, dzmin;
+  int variable_z;
+  int bt[6];          // velocity boundary types
+  int sbt[6];         // scalar boundary types
+  double side_scal[6];
+  // 1-D coordinate tables (device pointers already offset so index i works
+  // directly for the reference's index i)
+  const double *dxPr, *dyPr, *dzPr, *dzW, *zPr, *zW;
+};
+
+__host__ __device__ inline long gix(const Geo &g, int i, int j, int k) {
+  return g.base + (long)i + (long)j * g.sy + (long)k * g.sz;
+}
+
+// Per-cell code bytes used by the bulk kernels.
+enum : uint8_t {
+  CODE_ADV_MASK = 3,   // 0: 4th order, 1: 2nd order (Xtype<0), 2: none (Xtype>0)
+  CODE_CORRECTED = 4,  // cell handled by the ordered-correction kernel
+  CODE_SOLID = 8       // Prtype>0 (scalar code only)
+};
+
+// Stage constants shared by the momentum and scalar kernels.
+struct StageConst {
+  int stage;          // 0,1,2
+  double dt;
+  double rho_dt;      // RK_rho(stage)*dt   (scalar RK uses the product)
+  double beta_dt;     // RK_beta(stage)*dt
+  double rho, beta;   // momentum RK uses (Ustar*rho)*dt, (Ustar*beta)*dt
+  double coef;        // 2*RK_alpha(stage)*dt
+  double A, Ax, Ay, Az;     // -coef, -coef/dxmin, ...
+  double prgx, prgy;  // uniform pressure gradient (0 = off)
+};
+
+// Ordered per-cell corrections (built on the host in the reference's scatter
+// order).  Cells are listed once; terms[ptr[c]..ptr[c+1]) apply in order.
+struct TermList {
+  int n;                  // number of cells
+  const int32_t *cell;    // 3 per cell: i,j,k
+  const int32_t *ptr;     // n+1
+  const int32_t *term;    // 4 per term, meaning depends on the family
+};
+
+}  // namespace elmm

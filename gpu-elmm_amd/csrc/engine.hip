@@ -1,0 +1,880 @@
+// engine.hip - device-resident ELMM LES step: buffers, set-up of ordered
+// correction lists, stage sequencing and the C ABI.
+//
+// One elmm_handle = one case with all fields resident in HBM.  A step is the
+// reference's TMarchRK3 (src/tsteps.f90:24-243) issued as a fixed sequence
+// of kernels on the handle's stream; nothing returns to the host between
+// stages.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "case_setup.hpp"
+#include "device_common.hpp"
+#include "elmm_gpu.h"
+#include "errors.hpp"
+#include "kernels.hpp"
+
+namespace elmm {
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct HipError {
+  std::string msg;
+  explicit HipError(std::string m) : msg(std::move(m)) {}
+};
+struct ArgError {
+  std::string msg;
+  explicit ArgError(std::string m) : msg(std::move(m)) {}
+};
+
+static const double RK_alpha[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+static const double RK_beta[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double RK_rho[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+// Device allocation owned by the engine.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    bytes = b ? b : 16;
+    HIPCHK(hipMalloc(&p, bytes));
+    HIPCHK(hipMemset(p, 0, bytes));
+  }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+template <typename T>
+static void upload_vec(DevBuf &b, const std::vector<T> &v) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+struct DevTerms {
+  DevBuf cell, ptr, term;
+  int n = 0;
+  TermList view() const { return TermList{n, cell.as<int32_t>(), ptr.as<int32_t>(), term.as<int32_t>()}; }
+};
+
+// Host-side accumulation of ordered per-cell terms, keyed by target cell.
+struct TermBuilder {
+  std::vector<std::array<int, 3>> cells;
+  std::vector<std::vector<std::array<int32_t, 4>>> terms;
+  std::unordered_map<long, int> index;
+  int cell_of(long key, int i, int j, int k) {
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    int c = (int)cells.size();
+    index.emplace(key, c);
+    cells.push_back({i, j, k});
+    terms.emplace_back();
+    return c;
+  }
+  void add(long key, int i, int j, int k, std::array<int32_t, 4> t) { terms[cell_of(key, i, j, k)].push_back(t); }
+  void upload(DevTerms &d) const {
+    std::vector<int32_t> c, p{0}, t;
+    for (size_t q = 0; q < cells.size(); q++) {
+      c.insert(c.end(), cells[q].begin(), cells[q].end());
+      for (auto &x : terms[q]) t.insert(t.end(), x.begin(), x.end());
+      p.push_back((int32_t)(t.size() / 4));
+    }
+    d.n = (int)cells.size();
+    upload_vec(d.cell, c);
+    upload_vec(d.ptr, p);
+    upload_vec(d.term, t);
+  }
+};
+
+struct Engine {
+  elmm_config cfg{};
+  std::vector<double> zw_copy;
+  Geo g{};
+  Grid1D grid;
+  hipStream_t stream = nullptr;
+  int nsc = 0;
+  // fields
+  DevBuf U, V, W, U2, V2, W2, Us, Vs, Ws, Pr, Visc, TDiff, Q, Phi, RHS;
+  std::vector<std::unique_ptr<DevBuf>> S, Snew, Sadv;
+  double *u, *v, *w, *u2, *v2, *w2;  // current rotation of U/U2
+  std::vector<double *> s_cur, s_nxt;
+  // masks and codes
+  DevBuf code[3], code_s, mask[3], prtype;
+  // coordinates
+  DevBuf coords;
+  // wall model
+  DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_logdz0;
+  DevBuf width_k;
+  bool have_width_k = false;
+  int wm_n = 0;
+  DevBuf wmp_ijk, wmp_div;
+  int wmp_n = 0;
+  DevTerms mom_terms[3];
+  std::vector<std::unique_ptr<DevTerms>> sc_terms;
+  DevBuf src_flux;
+  // immersed boundary
+  DevBuf ib_tgt[4], ib_ptr[4], ib_src[4], ib_coef[4], ib_tmp;
+  int ib_n[4] = {0, 0, 0, 0};
+  std::vector<int32_t> visc_levels;
+  DevTerms q_terms;
+  DevBuf null_ijk[3];
+  int null_n[3] = {0, 0, 0};
+  // pressure
+  DevBuf X, twx, twy, lamx, lamy, az, cz;
+  PoissonPlan pp{};
+  DevBuf partial, cfl_partial, div_partial;
+  int top_blocks = 0, red_blocks = 0, cfl_blocks = 0;
+  bool initialized = false, first_step_done = false;
+  double last_div = 0, last_cfl = 0;
+  bool want_diag = false;
+
+  IBDev ibdev(int c) const {
+    return IBDev{ib_n[c], ib_tgt[c].as<int32_t>(), ib_ptr[c].as<int32_t>(), ib_src[c].as<int32_t>(),
+                 ib_coef[c].as<double>()};
+  }
+  WMPoints wmdev() {
+    return WMPoints{wm_n,
+                    wm_ijk.as<int32_t>(),
+                    wm_list.as<int32_t>(),
+                    wm_dist.as<double>(),
+                    wm_z0.as<double>(),
+                    wm_ustar.as<double>(),
+                    wm_fluxp.as<double>(),
+                    wm_logdz0.as<double>()};
+  }
+  bool has_ib() const { return ib_n[0] + ib_n[1] + ib_n[2] > 0; }
+
+  void create(const elmm_config &c, const elmm_lists &L);
+  void setup_geo();
+  void setup_lists(const elmm_lists &L);
+  void setup_poisson();
+  void xfer(int field, int lb, const int32_t ext[3], double *host, bool up);
+  void bound_uvw(double *a, double *b, double *c) { launch_bound_uvw<double>(g, a, b, c, stream); }
+  void ib_momentum(double *a, double *b, double *c);
+  void pressure_correction(double *a, double *b, double *c, bool use_q, double coef);
+  void subgrid_stresses(bool cfl);
+  void stage(int st, bool last);
+  void init();
+  void step(int nsteps);
+  double max_div(double *a, double *b, double *c, bool use_q);
+  double reduce_max(DevBuf &buf, int n);
+};
+
+void Engine::setup_geo() {
+  const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  component_extents(cfg, g.n);
+  long row = kFrontPad + nx + 6;
+  g.sy = (row + kRowAlign - 1) / kRowAlign * kRowAlign;
+  g.sz = g.sy * (ny + 6);
+  g.base = kFrontPad + 2 + 2 * g.sy + 2 * g.sz;
+  g.ntot = g.sz * (nz + 6) + kRowAlign;
+  grid = make_grid(cfg);
+  g.dxmin = grid.dxmin;
+  g.dymin = grid.dymin;
+  g.dzmin = grid.dzmin;
+  g.variable_z = cfg.stretched_z != 0;
+  for (int s = 0; s < 6; s++) {
+    g.bt[s] = cfg.btype[s];
+    g.sbt[s] = cfg.scal_btype[s];
+    g.side_scal[s] = cfg.side_scal[s];
+  }
+  // coordinate tables: dxPr dyPr dzPr dzW zPr zW, each n+8 with offset 3
+  std::vector<double> tab;
+  size_t off[6];
+  auto push = [&](const std::vector<double> &v, int k) {
+    off[k] = tab.size();
+    tab.insert(tab.end(), v.begin(), v.end());
+  };
+  push(grid.dxPr, 0);
+  push(grid.dyPr, 1);
+  push(grid.dzPr, 2);
+  push(grid.dzW, 3);
+  push(grid.zPr, 4);
+  push(grid.zW, 5);
+  upload_vec(coords, tab);
+  const double *base = coords.as<double>();
+  g.dxPr = base + off[0] + 3;
+  g.dyPr = base + off[1] + 3;
+  g.dzPr = base + off[2] + 3;
+  g.dzW = base + off[3] + 3;
+  g.zPr = base + off[4] + 3;
+  g.zW = base + off[5] + 3;
+  // SGS filter width per level (src/subgrid.f90:62): filter_ratio *
+  // (dxPr*dyPr*dzPr)**(1/3), tabulated when dxPr(i)*dyPr(j) is uniform
+  const double dxy = Grid1D::at(grid.dxPr, 1) * Grid1D::at(grid.dyPr, 1);
+  bool uni = true;
+  for (int i = 1; i <= nx && uni; i++)
+    for (int j = 1; j <= ny && uni; j++) uni = Grid1D::at(grid.dxPr, i) * Grid1D::at(grid.dyPr, j) == dxy;
+  if (uni) {
+    std::vector<double> wk(nz + 2, 0.0);
+    for (int k = 1; k <= nz; k++) wk[k] = cfg.filter_ratio * std::pow(dxy * Grid1D::at(grid.dzPr, k), 1.0 / 3.0);
+    upload_vec(width_k, wk);
+    have_width_k = true;
+  }
+}
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+void Engine::setup_poisson() {
+  const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+  if (!is_pow2(nx) || !is_pow2(ny)) throw ArgError("FFT pressure solver needs power-of-two nx and ny");
+  if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
+  if (nz > 256) throw ArgError("z solver supports nz <= 256");
+  const int M = nx / 2 + 1;
+  X.alloc((size_t)M * ny * nz * sizeof(double2));
+  auto tw = [](int n) {
+    std::vector<double2> t(std::max(1, n / 2));
+    for (int q = 0; q < n / 2; q++) {
+      // exp(-2 pi i q/n) from the first octant by symmetry for accuracy
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * q / n;
+      t[q] = make_double2((double)cosl(a), (double)-sinl(a));
+    }
+    return t;
+  };
+  upload_vec(twx, tw(nx));
+  upload_vec(twy, tw(ny));
+  std::vector<double> lx(M), ly(ny), a(nz), c(nz);
+  for (int m = 0; m < M; m++) {
+    double s = std::sin(M_PI * m / nx);
+    lx[m] = -4 * s * s / (grid.dxmin * grid.dxmin);
+  }
+  for (int j = 0; j < ny; j++) {
+    double s = std::sin(M_PI * j / ny);
+    ly[j] = -4 * s * s / (grid.dymin * grid.dymin);
+  }
+  for (int k = 1; k <= nz; k++) {
+    a[k - 1] = k > 1 ? 1 / (Grid1D::at(grid.dzW, k - 1) * Grid1D::at(grid.dzPr, k)) : 0;
+    c[k - 1] = k < nz ? 1 / (Grid1D::at(grid.dzW, k) * Grid1D::at(grid.dzPr, k)) : 0;
+  }
+  upload_vec(lamx, lx);
+  upload_vec(lamy, ly);
+  upload_vec(az, a);
+  upload_vec(cz, c);
+  pp = PoissonPlan{nx, ny, nz, M, X.as<double2>(), twx.as<double2>(), twy.as<double2>(), lamx.as<double>(),
+                   lamy.as<double>(), az.as<double>(), cz.as<double>(), g.dzPr, g.dzW};
+  top_blocks = (int)((nx * ny + 255) / 256);
+  red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
+  cfl_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
+  partial.alloc(std::max(top_blocks, 1) * sizeof(double));
+  div_partial.alloc(std::max(red_blocks, 1) * sizeof(double));
+  cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));
+}
+
+void Engine::setup_lists(const elmm_lists &L) {
+  const long ntot = g.ntot;
+  auto lin = [&](int i, int j, int k) { return gix(g, i, j, k); };
+  // host copies of masks in the device layout
+  auto remap_i32 = [&](const int32_t *src) {
+    std::vector<int32_t> out(ntot, 0);
+    const long hy = cfg.nx + 6, hz = (long)(cfg.nx + 6) * (cfg.ny + 6);
+    for (int k = -2; k <= cfg.nz + 3; k++)
+      for (int j = -2; j <= cfg.ny + 3; j++)
+        for (int i = -2; i <= cfg.nx + 3; i++) out[lin(i, j, k)] = src[(i + 2) + (j + 2) * hy + (k + 2) * hz];
+    return out;
+  };
+  auto remap_u8 = [&](const uint8_t *src) {
+    std::vector<uint8_t> out(ntot, 0);
+    const long hy = cfg.nx + 6, hz = (long)(cfg.nx + 6) * (cfg.ny + 6);
+    for (int k = -2; k <= cfg.nz + 3; k++)
+      for (int j = -2; j <= cfg.ny + 3; j++)
+        for (int i = -2; i <= cfg.nx + 3; i++) out[lin(i, j, k)] = src[(i + 2) + (j + 2) * hy + (k + 2) * hz];
+    return out;
+  };
+  std::vector<int32_t> types[3] = {remap_i32(L.utype), remap_i32(L.vtype), remap_i32(L.wtype)};
+  std::vector<int32_t> pt = remap_i32(L.prtype);
+  std::vector<uint8_t> codes[3];
+  for (int c = 0; c < 3; c++) {
+    codes[c].assign(ntot, 0);
+    for (long p = 0; p < ntot; p++) {
+      int t = types[c][p];
+      codes[c][p] = t == 0 ? 0 : t < 0 ? 1 : 2;
+    }
+  }
+  std::vector<uint8_t> scode(ntot, 0);
+  for (long p = 0; p < ntot; p++)
+    if (pt[p] > 0) scode[p] = CODE_SOLID;
+  upload_vec(prtype, pt);
+  for (int d = 0; d < 3; d++) upload_vec(mask[d], remap_u8(L.scfl_mask[d]));
+
+  // wall-model points of the 18 momentum lists, concatenated in list order
+  std::vector<int32_t> wijk, wlist;
+  std::vector<double> wdist, wz0, wustar;
+  std::vector<int> list_off(19, 0);
+  for (int l = 0; l < 18; l++) {
+    list_off[l] = (int)wlist.size();
+    for (int p = 0; p < L.wm_n[l]; p++) {
+      wijk.insert(wijk.end(), {L.wm_ijk[l][3 * p], L.wm_ijk[l][3 * p + 1], L.wm_ijk[l][3 * p + 2]});
+      wlist.push_back(l);
+      wdist.insert(wdist.end(), {L.wm_dist[l][3 * p], L.wm_dist[l][3 * p + 1], L.wm_dist[l][3 * p + 2]});
+      wz0.push_back(L.wm_z0[l][p]);
+      wustar.push_back(1.0);  // WMpointUVW%ustar = 1 (src/wallmodels.f90:79)
+    }
+  }
+  list_off[18] = (int)wlist.size();
+  wm_n = (int)wlist.size();
+  std::vector<double> wlog(wm_n, 0.0);
+  for (int p = 0; p < wm_n; p++) {
+    const double *d = &wdist[3 * p];
+    double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (wz0[p] > 0) wlog[p] = std::log(dist / wz0[p]);
+  }
+  upload_vec(wm_logdz0, wlog);
+  upload_vec(wm_ijk, wijk);
+  upload_vec(wm_list, wlist);
+  upload_vec(wm_dist, wdist);
+  upload_vec(wm_z0, wz0);
+  upload_vec(wm_ustar, wustar);
+  wm_fluxp.alloc(std::max(wm_n, 1) * sizeof(double));
+
+  // ordered momentum corrections (wmfluxes-nobranch-inc.f90:49-208)
+  if (cfg.wall_model) {
+    for (int c = 0; c < 3; c++) {
+      TermBuilder tb;
+      const int *n = g.n[c];
+      for (int dir = 0; dir < 6; dir++) {
+        const int l = c * 6 + dir, ax = dir / 2, plus = dir % 2;
+        for (int pass = 0; pass < 2; pass++)
+          for (int p = 0; p < L.wm_n[l]; p++) {
+            int q[3] = {L.wm_ijk[l][3 * p], L.wm_ijk[l][3 * p + 1], L.wm_ijk[l][3 * p + 2]};
+            int f[3] = {q[0], q[1], q[2]}, t[3] = {q[0], q[1], q[2]};
+            if (!plus) {
+              f[ax] -= 1;
+              if (pass == 1) t[ax] -= 1;
+            } else if (pass == 0) {
+              t[ax] += 1;
+            }
+            const int neg = pass == 1;
+            if (t[0] < 1 || t[0] > n[0] || t[1] < 1 || t[1] > n[1] || t[2] < 1 || t[2] > n[2]) continue;
+            long key = lin(t[0], t[1], t[2]);
+            tb.add(key, t[0], t[1], t[2],
+                   {ax | (neg << 2), list_off[l] + p, (int32_t)lin(f[0], f[1], f[2]), 0});
+            codes[c][key] |= CODE_CORRECTED;
+          }
+      }
+      tb.upload(mom_terms[c]);
+    }
+  }
+  for (int c = 0; c < 3; c++) upload_vec(code[c], codes[c]);
+
+  // scalar wall-model points and ordered scalar terms
+  wmp_n = cfg.wall_model ? L.wmp_n : 0;
+  std::vector<int32_t> pijk;
+  for (int p = 0; p < wmp_n; p++) pijk.insert(pijk.end(), {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]});
+  upload_vec(wmp_ijk, pijk);
+  wmp_div.alloc(std::max(wmp_n, 1) * sizeof(double));
+  std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
+  upload_vec(src_flux, sflux);
+  for (int s = 0; s < nsc; s++) sc_terms.emplace_back(new DevTerms);
+  if (nsc > 0) {
+    std::vector<TermBuilder> tbs(nsc);
+    auto interior = [&](int i, int j, int k) {
+      return i >= 1 && i <= cfg.nx && j >= 1 && j <= cfg.ny && k >= 1 && k <= cfg.nz;
+    };
+    for (int s = 0; s < nsc; s++) {
+      TermBuilder &tb = tbs[s];
+      for (int d = 0; d < 3; d++)
+        for (int pass = 0; pass < 2; pass++)
+          for (int p = 0; p < L.scfl_n[d]; p++) {
+            int q[3] = {L.scfl_ijk[d][3 * p], L.scfl_ijk[d][3 * p + 1], L.scfl_ijk[d][3 * p + 2]};
+            int t[3] = {q[0], q[1], q[2]};
+            if (pass == 1) t[d] += 1;
+            if (!interior(t[0], t[1], t[2])) continue;
+            tb.add(lin(t[0], t[1], t[2]), t[0], t[1], t[2],
+                   {(pass == 0 ? 0 : 1) | (d << 2), (int32_t)lin(q[0], q[1], q[2]), 0, 0});
+          }
+      for (int p = 0; p < L.src_n; p++) {
+        if (L.src_scalar[p] != s) continue;
+        int q[3] = {L.src_ijk[3 * p], L.src_ijk[3 * p + 1], L.src_ijk[3 * p + 2]};
+        tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {2, p, 0, 0});
+      }
+      for (int p = 0; p < wmp_n; p++) {
+        int q[3] = {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]};
+        tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {3, p, 0, 0});
+      }
+    }
+    // every scalar lists the union of corrected cells (same cell order)
+    TermBuilder uni;
+    for (int s = 0; s < nsc; s++)
+      for (size_t q = 0; q < tbs[s].cells.size(); q++) {
+        auto &cc = tbs[s].cells[q];
+        uni.cell_of(lin(cc[0], cc[1], cc[2]), cc[0], cc[1], cc[2]);
+      }
+    for (int s = 0; s < nsc; s++) {
+      TermBuilder out;
+      for (auto &cc : uni.cells) {
+        long key = lin(cc[0], cc[1], cc[2]);
+        out.cell_of(key, cc[0], cc[1], cc[2]);
+        auto it = tbs[s].index.find(key);
+        if (it != tbs[s].index.end()) out.terms[out.index[key]] = tbs[s].terms[it->second];
+      }
+      out.upload(*sc_terms[s]);
+    }
+    for (auto &cc : uni.cells) scode[lin(cc[0], cc[1], cc[2])] |= CODE_CORRECTED;
+  }
+  upload_vec(code_s, scode);
+
+  // immersed-boundary lists
+  for (int c = 0; c < 4; c++) {
+    ib_n[c] = L.ib_n[c];
+    std::vector<int32_t> tg(L.ib_ijk[c], L.ib_ijk[c] + 3 * (size_t)ib_n[c]);
+    std::vector<int32_t> pt(L.ib_ptr[c], L.ib_ptr[c] + (ib_n[c] ? ib_n[c] + 1 : 0));
+    if (pt.empty()) pt.push_back(0);
+    int nt = pt.back();
+    std::vector<int32_t> sr(L.ib_src_ijk[c], L.ib_src_ijk[c] + 3 * (size_t)nt);
+    std::vector<double> cf(L.ib_coef[c], L.ib_coef[c] + nt);
+    if (c == 3 && ib_n[3] > 0) {
+      // dependency levels of the sequential ScalFlIBPoints loop
+      std::unordered_map<long, int> wlev, rlev;
+      std::vector<int> lev(ib_n[3]);
+      for (int p = 0; p < ib_n[3]; p++) {
+        long tk = lin(tg[3 * p], tg[3 * p + 1], tg[3 * p + 2]);
+        int lv = 0;
+        auto bump = [&](std::unordered_map<long, int> &m, long key) {
+          auto it = m.find(key);
+          if (it != m.end()) lv = std::max(lv, it->second + 1);
+        };
+        for (int t = pt[p]; t < pt[p + 1]; t++) bump(wlev, lin(sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]));
+        bump(rlev, tk);
+        bump(wlev, tk);
+        lev[p] = lv;
+        wlev[tk] = std::max(wlev.count(tk) ? wlev[tk] : 0, lv);
+        for (int t = pt[p]; t < pt[p + 1]; t++) {
+          long sk = lin(sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]);
+          rlev[sk] = std::max(rlev.count(sk) ? rlev[sk] : 0, lv);
+        }
+      }
+      int nl = *std::max_element(lev.begin(), lev.end()) + 1;
+      std::vector<int> order(ib_n[3]);
+      for (int p = 0; p < ib_n[3]; p++) order[p] = p;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
+      std::vector<int32_t> tg2, pt2{0}, sr2;
+      std::vector<double> cf2;
+      visc_levels.assign(nl + 1, 0);
+      for (int q = 0; q < ib_n[3]; q++) {
+        int p = order[q];
+        tg2.insert(tg2.end(), {tg[3 * p], tg[3 * p + 1], tg[3 * p + 2]});
+        for (int t = pt[p]; t < pt[p + 1]; t++) {
+          sr2.insert(sr2.end(), {sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]});
+          cf2.push_back(cf[t]);
+        }
+        pt2.push_back((int32_t)cf2.size());
+        visc_levels[lev[p] + 1] = q + 1;
+      }
+      for (int l = 1; l <= nl; l++) visc_levels[l] = std::max(visc_levels[l], visc_levels[l - 1]);
+      tg = tg2;
+      pt = pt2;
+      sr = sr2;
+      cf = cf2;
+    }
+    upload_vec(ib_tgt[c], tg);
+    upload_vec(ib_ptr[c], pt);
+    upload_vec(ib_src[c], sr);
+    upload_vec(ib_coef[c], cf);
+  }
+  ib_tmp.alloc((size_t)std::max({ib_n[0], ib_n[1], ib_n[2], 1}) * sizeof(double));
+  // IBMassSources gather terms (src/tsteps.f90:490-531) + Bound_Q fold targets
+  if (cfg.ib_mass_sources && has_ib()) {
+    TermBuilder tb;
+    for (int c = 0; c < 3; c++)
+      for (int p = 0; p < ib_n[c]; p++) {
+        int q[3] = {L.ib_ijk[c][3 * p], L.ib_ijk[c][3 * p + 1], L.ib_ijk[c][3 * p + 2]};
+        int32_t src = (int32_t)lin(q[0], q[1], q[2]);
+        tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {c, src, q[c], 0});
+        int r[3] = {q[0], q[1], q[2]};
+        r[c] += 1;
+        tb.add(lin(r[0], r[1], r[2]), r[0], r[1], r[2], {c | (1 << 2), src, q[c] + 1, 0});
+      }
+    // fold destinations must be rewritten every stage (their pre-fold value
+    // is their own direct sum, zero when they have none)
+    const int N[3] = {cfg.nx, cfg.ny, cfg.nz};
+    std::vector<std::array<int, 3>> targets(tb.cells.begin(), tb.cells.end());
+    // Bound_Q reads a ghost cell only when exactly one of its coordinates is
+    // outside the interior (src/boundaries.f90:2094-2192)
+    for (auto &q : targets) {
+      int nghost = 0, a = -1;
+      for (int b = 0; b < 3; b++)
+        if (q[b] < 1 || q[b] > N[b]) {
+          nghost++;
+          a = b;
+        }
+      if (nghost != 1) continue;
+      bool per = cfg.btype[2 * a] == ELMM_BC_PERIODIC;
+      std::array<int, 3> d = q;
+      if (q[a] <= 0) d[a] = per ? N[a] : 1;
+      else d[a] = per ? 1 : N[a];
+      tb.cell_of(lin(d[0], d[1], d[2]), d[0], d[1], d[2]);
+    }
+    tb.upload(q_terms);
+  }
+  for (int c = 0; c < 3; c++) {
+    null_n[c] = L.null_n[c];
+    std::vector<int32_t> v(L.null_ijk[c], L.null_ijk[c] + 3 * (size_t)null_n[c]);
+    upload_vec(null_ijk[c], v);
+  }
+}
+
+void Engine::create(const elmm_config &c, const elmm_lists &L) {
+  cfg = c;
+  std::string err = validate_config(cfg);
+  if (!err.empty()) throw ArgError(err);
+  if (cfg.precision != ELMM_FP64) throw ArgError("this build of the step implements precision ELMM_FP64");
+  if (cfg.poisson_solver != ELMM_POISSON_FFT) throw ArgError("poisson_solver must be ELMM_POISSON_FFT");
+  for (int s = 4; s < 6; s++)
+    if (cfg.nscalars > 0 && cfg.scal_btype[s] == ELMM_BC_PERIODIC) throw ArgError("periodic scalar z boundaries");
+  if (cfg.stretched_z) {
+    zw_copy.assign(cfg.zw, cfg.zw + cfg.nz + 1);
+    cfg.zw = zw_copy.data();
+  }
+  nsc = cfg.nscalars;
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  setup_geo();
+  const size_t fb = g.ntot * sizeof(double);
+  for (DevBuf *b : {&U, &V, &W, &U2, &V2, &W2, &Us, &Vs, &Ws, &Pr, &Visc, &TDiff, &Q, &Phi, &RHS}) b->alloc(fb);
+  for (int s = 0; s < nsc; s++) {
+    S.emplace_back(new DevBuf);
+    Snew.emplace_back(new DevBuf);
+    Sadv.emplace_back(new DevBuf);
+    S.back()->alloc(fb);
+    Snew.back()->alloc(fb);
+    Sadv.back()->alloc(fb);
+    s_cur.push_back(S.back()->as<double>());
+    s_nxt.push_back(Snew.back()->as<double>());
+  }
+  u = U.as<double>();
+  v = V.as<double>();
+  w = W.as<double>();
+  u2 = U2.as<double>();
+  v2 = V2.as<double>();
+  w2 = W2.as<double>();
+  setup_lists(L);
+  setup_poisson();
+  HIPCHK(hipDeviceSynchronize());
+}
+
+void Engine::xfer(int field, int lb, const int32_t ext[3], double *host, bool up) {
+  double *d = nullptr;
+  switch (field) {
+    case ELMM_F_U: d = u; break;
+    case ELMM_F_V: d = v; break;
+    case ELMM_F_W: d = w; break;
+    case ELMM_F_PR: d = Pr.as<double>(); break;
+    case ELMM_F_VISC: d = Visc.as<double>(); break;
+    case ELMM_F_TDIFF: d = TDiff.as<double>(); break;
+    case ELMM_F_PHI: d = Phi.as<double>(); break;
+    case ELMM_F_Q: d = Q.as<double>(); break;
+    default:
+      if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + nsc) d = s_cur[field - ELMM_F_SCALAR];
+  }
+  if (!d) throw ArgError("unknown field id " + std::to_string(field));
+  if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) throw ArgError("extents must be positive");
+  HIPCHK(hipStreamSynchronize(stream));
+  std::vector<double> stage(g.ntot);
+  HIPCHK(hipMemcpy(stage.data(), d, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
+  const int n[3] = {cfg.nx, cfg.ny, cfg.nz};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = std::max(lb, -2);
+    hi[a] = std::min(lb + ext[a] - 1, n[a] + 3);
+  }
+  const long hy = ext[0], hz = (long)ext[0] * ext[1];
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        long h = (i - lb) + (j - lb) * hy + (k - lb) * hz;
+        long p = gix(g, i, j, k);
+        if (up)
+          stage[p] = host[h];
+        else
+          host[h] = stage[p];
+      }
+  if (up) {
+    HIPCHK(hipMemcpy(d, stage.data(), g.ntot * sizeof(double), hipMemcpyHostToDevice));
+    // scalar buffers rotate: both copies start from the uploaded state so
+    // ghost cells that no fill touches keep the uploaded value
+    if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + nsc)
+      HIPCHK(hipMemcpy(s_nxt[field - ELMM_F_SCALAR], stage.data(), g.ntot * sizeof(double), hipMemcpyHostToDevice));
+  }
+}
+
+void Engine::ib_momentum(double *a, double *b, double *c) {
+  double *F[3] = {a, b, c};
+  for (int comp = 0; comp < 3; comp++) {
+    if (!ib_n[comp]) continue;
+    launch_ib_interp(g, ibdev(comp), F[comp], ib_tmp.as<double>(), stream);
+    launch_ib_scatter(g, ibdev(comp), ib_tmp.as<double>(), F[comp], stream);
+  }
+}
+
+double Engine::reduce_max(DevBuf &buf, int n) {
+  std::vector<double> h(n);
+  HIPCHK(hipMemcpyAsync(h.data(), buf.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  double m = 0;
+  for (double x : h) m = std::max(m, x);
+  return m;
+}
+
+double Engine::max_div(double *a, double *b, double *c, bool use_q) {
+  launch_max_div(g, a, b, c, use_q ? Q.as<double>() : nullptr, div_partial.as<double>(), red_blocks, stream);
+  return reduce_max(div_partial, red_blocks);
+}
+
+// PressureCorrection, src/pressure.f90:169-266.  PrePoisson's BoundUVW
+// (pressure.f90:290) repeats the fill the caller has just done and is
+// omitted; the zeroing of open-boundary faces inside solids
+// (pressure.f90:301-342) only acts on non-periodic, non-noslip sides, where
+// the normal ghost is already 0 for every supported boundary type.
+void Engine::pressure_correction(double *a, double *b, double *c, bool use_q, double coef) {
+  const double dt2 = coef, dt2_rec = 1.0 / dt2;
+  launch_poisson_fwd(g, pp, a, b, c, use_q ? Q.as<double>() : nullptr, nullptr, dt2_rec, nullptr, stream);
+  launch_poisson_solve_z(g, pp, stream);
+  launch_poisson_inv(g, pp, Phi.as<double>(), stream);
+  launch_top_sum(g, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, stream);
+  launch_projection(g, a, b, c, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, dt2, stream);
+  launch_bound_pr<double>(g, Pr.as<double>(), stream);
+}
+
+// SubgridStresses, src/dynamics.f90:869-908 (ComputeViscsWM only writes
+// values that SubgridModel overwrites plus the diagnostic WMPoints%ustar;
+// BoundViscosity(TDiff) is idempotent on the frozen TDiff and done at init).
+void Engine::subgrid_stresses(bool cfl) {
+  launch_sgs(g, u, v, w, Visc.as<double>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
+             have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model, cfl ? cfl_partial.as<double>() : nullptr,
+             cfl_blocks, stream);
+  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
+  launch_bound_visc<double>(g, Visc.as<double>(), stream);
+  if (ib_n[3]) launch_ib_visc_serial(g, ibdev(3), Visc.as<double>(), visc_levels.data(), (int)visc_levels.size() - 1, stream);
+}
+
+void Engine::stage(int st, bool last) {
+  const double dt = cfg.dt;
+  StageConst sc{};
+  sc.stage = st;
+  sc.dt = dt;
+  sc.rho = RK_rho[st];
+  sc.beta = RK_beta[st];
+  sc.rho_dt = RK_rho[st] * dt;
+  sc.beta_dt = RK_beta[st] * dt;
+  sc.coef = 2 * RK_alpha[st] * dt;
+  sc.A = -sc.coef;
+  sc.Ax = -sc.coef / g.dxmin;
+  sc.Ay = -sc.coef / g.dymin;
+  sc.Az = -sc.coef / g.dzmin;
+  sc.prgx = cfg.pr_gradient_x;
+  sc.prgy = cfg.pr_gradient_y;
+
+  subgrid_stresses(st == 0 && cfg.cfl_check);
+  double *ustar[3] = {Us.as<double>(), Vs.as<double>(), Ws.as<double>()};
+  double *U2p[3] = {u2, v2, w2};
+  for (int c = 0; c < 3; c++) {
+    launch_momentum(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c], code[c].as<uint8_t>(), sc,
+                    stream);
+    launch_momentum_corr(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c],
+                         code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(), stream);
+  }
+  if (nsc > 0) {
+    // ScalarRK3 (src/scalars.f90:88-299) with the stage-start velocity;
+    // the stage-start BoundScalar repeats the previous stage's final fill.
+    launch_wm_div(g, WMScalarPoints{wmp_n, wmp_ijk.as<int32_t>(), wmp_div.as<double>()}, prtype.as<int32_t>(), u, v,
+                  w, stream);
+    for (int s = 0; s < nsc; s++) {
+      launch_scalar(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(), code_s.as<uint8_t>(),
+                    mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, stream);
+      launch_scalar_corr(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(),
+                         code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(),
+                         sc, sc_terms[s]->view(), src_flux.as<double>(), wmp_div.as<double>(), stream);
+      launch_bound_scalar<double>(g, s_nxt[s], stream);
+      std::swap(s_cur[s], s_nxt[s]);
+    }
+  }
+  bound_uvw(u2, v2, w2);
+  const bool use_q = cfg.ib_mass_sources && has_ib();
+  if (has_ib()) {
+    ib_momentum(u2, v2, w2);
+    bound_uvw(u2, v2, w2);
+  }
+  if (use_q) launch_mass_sources(g, q_terms.view(), u2, v2, w2, Q.as<double>(), stream);
+  pressure_correction(u2, v2, w2, use_q, sc.coef);
+  if (last && want_diag) {
+    bound_uvw(u2, v2, w2);
+    last_div = max_div(u2, v2, w2, use_q);
+  }
+  std::swap(u, u2);
+  std::swap(v, v2);
+  std::swap(w, w2);
+  for (int c = 0; c < 3; c++) launch_null(g, null_n[c], null_ijk[c].as<int32_t>(), c == 0 ? u : c == 1 ? v : w, stream);
+  bound_uvw(u, v, w);
+  if (has_ib()) {
+    ib_momentum(u, v, w);
+    bound_uvw(u, v, w);
+  }
+}
+
+void Engine::init() {
+  // src/initial.f90:4284-4342
+  bound_uvw(u, v, w);
+  launch_bound_pr<double>(g, Pr.as<double>(), stream);
+  pressure_correction(u, v, w, false, 1.0);
+  launch_sgs(g, u, v, w, Visc.as<double>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
+             have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model, nullptr, 0, stream);
+  launch_bound_visc<double>(g, Visc.as<double>(), stream);
+  if (nsc > 0) {
+    launch_tdiff_init(g, Visc.as<double>(), TDiff.as<double>(), cfg.nu, cfg.nu_scalar, stream);
+    launch_bound_visc<double>(g, TDiff.as<double>(), stream);
+  }
+  for (int s = 0; s < nsc; s++) launch_bound_scalar<double>(g, s_cur[s], stream);
+  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
+  launch_bound_visc<double>(g, Visc.as<double>(), stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  initialized = true;
+}
+
+void Engine::step(int nsteps) {
+  for (int s = 0; s < nsteps; s++) {
+    if (!first_step_done) {
+      // first call of TMarchRK3 (src/tsteps.f90:54-77)
+      bound_uvw(u, v, w);
+      if (has_ib()) ib_momentum(u, v, w);
+      first_step_done = true;
+    }
+    for (int st = 0; st < 3; st++) stage(st, s == nsteps - 1 && st == 2);
+    if (cfg.cfl_check && s == nsteps - 1) last_cfl = reduce_max(cfl_partial, cfl_blocks) * cfg.dt;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+}
+
+}  // namespace elmm
+
+using elmm::ArgError;
+using elmm::HipError;
+
+struct elmm_handle {
+  elmm::Engine e;
+};
+
+template <typename F>
+static int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const ArgError &a) {
+    return elmm::fail(ELMM_E_ARG, a.msg);
+  } catch (const HipError &h) {
+    return elmm::fail(ELMM_E_HIP, h.msg);
+  } catch (const std::bad_alloc &) {
+    return elmm::fail(ELMM_E_NOMEM, "out of host memory");
+  }
+}
+
+extern "C" int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm_handle **out) {
+  if (!cfg || !lists || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_create");
+  *out = nullptr;
+  auto h = std::make_unique<elmm_handle>();
+  int rc = guarded([&] { h->e.create(*cfg, *lists); });
+  if (rc == 0) *out = h.release();
+  return rc;
+}
+
+extern "C" int elmm_upload(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], const double *src) {
+  if (!h || !ext || !src) return elmm::fail(ELMM_E_ARG, "null argument to elmm_upload");
+  return guarded([&] { h->e.xfer(field, lb, ext, const_cast<double *>(src), true); });
+}
+
+extern "C" int elmm_download(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], double *dst) {
+  if (!h || !ext || !dst) return elmm::fail(ELMM_E_ARG, "null argument to elmm_download");
+  return guarded([&] { h->e.xfer(field, lb, ext, dst, false); });
+}
+
+extern "C" int elmm_init(elmm_handle *h) {
+  if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  return guarded([&] { h->e.init(); });
+}
+
+static int check_diverged(elmm_handle *h) {
+  const elmm_config &c = h->e.cfg;
+  if (c.cfl_check && c.cfl_max > 0 && !(h->e.last_cfl <= c.cfl_max))
+    return elmm::fail(ELMM_E_DIVERGED, "Solution diverged. 2 (CFL above CFL_max or NaN)");
+  return 0;
+}
+
+extern "C" int elmm_step(elmm_handle *h, int32_t nsteps, elmm_diag *diag) {
+  if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  if (nsteps < 0) return elmm::fail(ELMM_E_ARG, "nsteps must be >= 0");
+  if (!h->e.initialized) return elmm::fail(ELMM_E_STATE, "elmm_step before elmm_init");
+  h->e.want_diag = diag != nullptr;
+  int rc = guarded([&] { h->e.step(nsteps); });
+  if (rc) return rc;
+  if (diag) {
+    std::memset(diag, 0, sizeof *diag);
+    diag->cfl = h->e.last_cfl;
+    diag->max_div = h->e.last_div;
+    diag->steps_done = nsteps;
+  }
+  return check_diverged(h);
+}
+
+extern "C" int elmm_step_async(elmm_handle *h, int32_t nsteps) {
+  if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  if (!h->e.initialized) return elmm::fail(ELMM_E_STATE, "elmm_step_async before elmm_init");
+  h->e.want_diag = false;
+  return guarded([&] {
+    for (int s = 0; s < nsteps; s++) {
+      if (!h->e.first_step_done) {
+        h->e.bound_uvw(h->e.u, h->e.v, h->e.w);
+        if (h->e.has_ib()) h->e.ib_momentum(h->e.u, h->e.v, h->e.w);
+        h->e.first_step_done = true;
+      }
+      for (int st = 0; st < 3; st++) h->e.stage(st, false);
+    }
+    HIPCHK(hipGetLastError());
+  });
+}
+
+extern "C" int elmm_wait(elmm_handle *h, elmm_diag *diag) {
+  if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  int rc = guarded([&] { HIPCHK(hipStreamSynchronize(h->e.stream)); });
+  if (rc) return rc;
+  if (diag) std::memset(diag, 0, sizeof *diag);
+  return 0;
+}
+
+extern "C" int elmm_divergence(elmm_handle *h, double *max_div) {
+  if (!h || !max_div) return elmm::fail(ELMM_E_ARG, "null argument to elmm_divergence");
+  return guarded([&] {
+    h->e.bound_uvw(h->e.u, h->e.v, h->e.w);
+    bool use_q = h->e.cfg.ib_mass_sources && h->e.has_ib() && h->e.first_step_done;
+    *max_div = h->e.max_div(h->e.u, h->e.v, h->e.w, use_q);
+  });
+}
+
+extern "C" int elmm_destroy(elmm_handle *h) {
+  if (!h) return 0;
+  if (h->e.stream) {
+    (void)hipStreamSynchronize(h->e.stream);
+    (void)hipStreamDestroy(h->e.stream);
+  }
+  delete h;
+  return 0;
+}

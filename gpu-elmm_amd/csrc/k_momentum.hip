@@ -1,0 +1,508 @@
+// k_momentum.hip - subgrid viscosity, wall-model fluxes, fused momentum
+// right-hand side + RK3 combine + pressure gradient, ordered wall-model
+// corrections, immersed-boundary gathers.
+//
+// Fusion (one HBM pass per component instead of the reference's separate
+// passes): Convection's advection (CD4divU, src/advection.f90:334-491),
+// explicit diffusion (MomentumDiffusion_nobranch_2ord,
+// src/momentum_diffusion.f90:169-288), the RK combine (src/dynamics.f90:
+// 266-295, 450-478), PressureGrad (src/dynamics.f90:58-199) and
+// add(U2,U) (src/tsteps.f90:350-355).  The arithmetic of each value is the
+// reference's, operation by operation (built with -ffp-contract=off).
+#include <cmath>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace elmm {
+
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------- Smagorinsky
+// NuSmag/StrainIJ/Strainu, src/subgrid.f90:55-67, 170-209
+__global__ void __launch_bounds__(256) k_sgs(Geo g, const double *__restrict__ U, const double *__restrict__ V,
+                                             const double *__restrict__ W, double *__restrict__ Visc, double nu,
+                                             double c_smag, double filter_ratio, const double *__restrict__ width_k,
+                                             int sgs_model, double *__restrict__ cfl_partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  double cfl = 0;
+  if (i <= g.nx && j <= g.ny) {
+    const long p = gix(g, i, j, k);
+    const long sy = g.sy, sz = g.sz;
+    double v;
+    if (sgs_model == 1) {
+      const double dx = g.dxmin, dy = g.dymin;
+      const double zp1 = g.zPr[k + 1], zm1 = g.zPr[k - 1];
+      double D00 = (U[p] - U[p - 1]) / dx;
+      double D11 = (V[p] - V[p - sy]) / dy;
+      double D22 = (W[p] - W[p - sz]) / g.dzPr[k];
+      double D01 = (U[p + sy] + U[p - 1 + sy] - U[p - sy] - U[p - 1 - sy]) / (4 * dy);
+      double D02 = (U[p + sz] + U[p - 1 + sz] - U[p - sz] - U[p - 1 - sz]) / (2 * (zp1 - zm1));
+      double D10 = (V[p + 1] + V[p + 1 - sy] - V[p - 1] - V[p - 1 - sy]) / (4 * dx);
+      double D12 = (V[p + sz] + V[p - sy + sz] - V[p - sz] - V[p - sy - sz]) / (2 * (zp1 - zm1));
+      double D20 = (W[p + 1] + W[p + 1 - sz] - W[p - 1] - W[p - 1 - sz]) / (4 * dx);
+      double D21 = (W[p + sy] + W[p + sy - sz] - W[p - sy] - W[p - sy - sz]) / (4 * dy);
+      const double D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
+      double s = 0;
+#pragma unroll
+      for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) {
+          double Sij = (D[ii][jj] + D[jj][ii]) / 2;
+          s = s + Sij * Sij;
+        }
+      double Sbar = sqrt(2.0 * s);
+      // filter width: per-level table from the host libm when dxPr*dyPr is
+      // uniform (the cube root of a static geometric factor)
+      double width = width_k ? width_k[k] : filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0);
+      double wc = width * c_smag;
+      v = Sbar * (wc * wc);
+      if (nu > 0) v = v + nu;
+    } else {
+      v = nu;
+    }
+    Visc[p] = v;
+    if (cfl_partial) {
+      // TimeStepLength's U_dx_max, src/dynamics.f90:556-594 (NaN -> +inf)
+      double a1 = fabs(U[p]), a0 = fabs(U[p - 1]);
+      double b1 = fabs(V[p]), b0 = fabs(V[p - sy]);
+      double c1 = fabs(W[p]), c0 = fabs(W[p - sz]);
+      double dxk = g.variable_z ? g.dxPr[i] : g.dxmin;
+      double dyk = g.variable_z ? g.dyPr[j] : g.dymin;
+      double dzk = g.variable_z ? g.dzPr[k] : g.dzmin;
+      double q = (a1 > a0 ? a1 : a0) / dxk;
+      q = q + (b1 > b0 ? b1 : b0) / dyk;
+      q = q + (c1 > c0 ? c1 : c0) / dzk;
+      cfl = (q != q) ? INFINITY : q;
+    }
+  }
+  if (cfl_partial) {
+    // wavefront max via cross-lane shuffles, then LDS across the 4 waves
+    for (int off = 32; off > 0; off >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, off));
+    __shared__ double red[4];
+    const int lane = (threadIdx.y * blockDim.x + threadIdx.x);
+    if ((lane & 63) == 0) red[lane >> 6] = cfl;
+    __syncthreads();
+    if (lane == 0) {
+      double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      cfl_partial[(long)blockIdx.z * gridDim.y * gridDim.x + blockIdx.y * gridDim.x + blockIdx.x] = m;
+    }
+  }
+}
+
+void launch_sgs(const Geo &g, const double *U, const double *V, const double *W, double *Visc, double nu,
+                double c_smag, double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial, int,
+                hipStream_t st) {
+  dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
+  hipLaunchKernelGGL(k_sgs, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width_k, sgs_model,
+                     cfl_partial);
+}
+
+// ------------------------------------------------------------- wall model
+// vel_and_dist, WMRoughUstar, WMFlatUstar, fluxes:
+// src/wallmodels.f90:1190-1240, 1312-1391, 2159-2337, 2356-2399
+__device__ inline double wm_flat_ustar(double ustar, double vel, double dist, double nu) {
+  const double eps = 1e-4;
+  if (nu <= 0) return 0;
+  double ustar_lam = sqrt(nu * vel / dist);
+  if ((dist * ustar_lam / nu) < 5.0) return ustar_lam;
+  for (int i = 1;; i++) {
+    double ustar2 = ustar;
+    double z_pl = dist * ustar2 / nu;
+    if (z_pl <= 5.0)
+      ustar = sqrt(nu * vel / dist);
+    else if (z_pl < 30.0)
+      ustar = vel / (5 * log(z_pl) - (double)3.05f);
+    else
+      ustar = vel / (log(z_pl) / 0.4 + 5.5);
+    if (fabs(ustar - ustar2) / fabs(ustar) < eps) break;
+    if (i >= 50) {
+      ustar = 0;
+      break;
+    }
+  }
+  return ustar;
+}
+
+__device__ inline double wm_rough_ustar(double ustar, double vel, double dist, double z0, double nu,
+                                       double logdz0) {
+  const double yplcrit = 11.225;
+  if (nu > 0) {
+    ustar = sqrt(nu * vel / dist);
+    if (dist * ustar / nu < yplcrit * 1.1) return ustar;
+  }
+  if (dist <= z0 * 2.0) {
+    ustar = nu > 0 ? wm_flat_ustar(ustar, vel, dist, nu) : 0.0;
+  } else {
+    ustar = vel * 0.41 / logdz0;  // log(dist/z0), precomputed per point
+  }
+  return ustar;
+}
+
+__global__ void k_wm_fluxes(Geo g, const double *__restrict__ U, const double *__restrict__ V,
+                            const double *__restrict__ W, WMPoints wm, double nu, int variable_z) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= wm.n) return;
+  const int l = wm.list[p];
+  const int comp = l / 6, dir = l % 6;
+  const int xi = wm.ijk[3 * p], yj = wm.ijk[3 * p + 1], zk = wm.ijk[3 * p + 2];
+  const long q = gix(g, xi, yj, zk);
+  const long sy = g.sy, sz = g.sz;
+  double v[3];
+  if (comp == 0) {
+    v[0] = U[q];
+    v[1] = (V[q + 1] + V[q] + V[q + 1 - sy] + V[q - sy]) / 4.0;
+    v[2] = (W[q + 1] + W[q] + W[q + 1 - sz] + W[q - sz]) / 4.0;
+  } else if (comp == 1) {
+    v[0] = (U[q + sy] + U[q] + U[q - 1 + sy] + U[q - 1]) / 4.0;
+    v[1] = V[q];
+    v[2] = (W[q + sy] + W[q] + W[q + sy - sz] + W[q - sz]) / 4.0;
+  } else {
+    v[0] = (U[q + sz] + U[q] + U[q - 1 + sz] + U[q - 1]) / 4.0;
+    v[1] = (V[q + sz] + V[q] + V[q - sy + sz] + V[q - sy]) / 4.0;
+    v[2] = W[q];
+  }
+  const double d0 = wm.dist[3 * p], d1 = wm.dist[3 * p + 1], d2 = wm.dist[3 * p + 2];
+  const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  double t0 = v[0] - 0.0, t1 = v[1] - 0.0, t2 = v[2] - 0.0;
+  const double dot = t0 * d0 + t1 * d1 + t2 * d2;
+  const double dd = dist * dist;
+  t0 = t0 - dot * d0 / dd;
+  t1 = t1 - dot * d1 / dd;
+  t2 = t2 - dot * d2 / dd;
+  const double vel = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+  double ustar = wm.ustar[p];
+  const double z0 = wm.z0[p];
+  if (vel != 0)
+    ustar = z0 > 0 ? wm_rough_ustar(ustar, vel, dist, z0, nu, wm.logdz0[p]) : wm_flat_ustar(ustar, vel, dist, nu);
+  if (ustar < 0) ustar = 0;
+  wm.ustar[p] = ustar;
+  const double eps = 1.4901161193847656e-08;  // sqrt(epsilon(1._knd))
+  double fluxp = 0;
+  if (vel > eps) {
+    const double tc = comp == 0 ? t0 : comp == 1 ? t1 : t2;
+    if (variable_z && dir == 4) {
+      fluxp = (tc / vel) * (ustar * ustar);
+      fluxp = comp < 2 ? fluxp / g.dzPr[zk] : fluxp / g.dzW[zk];
+    } else if (variable_z && dir == 5) {
+      fluxp = (tc / vel) * (ustar * ustar);
+      fluxp = comp < 2 ? fluxp / g.dzPr[zk + 1] : fluxp / g.dzW[zk + 1];
+    } else {
+      const double drec = dir < 2 ? 1 / g.dxmin : dir < 4 ? 1 / g.dymin : 1 / g.dzmin;
+      fluxp = (tc / vel) * (ustar * ustar) * drec;
+    }
+    if ((dir & 1) == 0) fluxp = -fluxp;
+  }
+  wm.fluxp[p] = fluxp;
+}
+
+void launch_wm_fluxes(const Geo &g, const double *U, const double *V, const double *W, WMPoints wm, double nu,
+                      int variable_z, hipStream_t st) {
+  if (wm.n == 0) return;
+  hipLaunchKernelGGL(k_wm_fluxes, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, wm, nu, variable_z);
+}
+
+// ------------------------------------------------------ CD4 advection
+__device__ inline double mask4(double a, double b, double c, double d) { return ((-a + 9 * b) + 9 * c) + -d; }
+
+// Advection of component C at linear index p (CD4divU/V/W,
+// src/advection.f90:334-834).  cls: 0 4th order, 1 2nd order, 2 none.
+template <int C>
+__device__ inline double cd4(const Geo &g, const double *__restrict__ U, const double *__restrict__ V,
+                             const double *__restrict__ W, long p, int cls) {
+  const long sy = g.sy, sz = g.sz;
+  double acc = 0, d;
+  if (cls == 2) return acc * (-1.0 / 256);
+  // F: advected component; A: advecting velocity per direction (U, V, W)
+  const double *F = C == 0 ? U : C == 1 ? V : W;
+  const long st[3] = {1, sy, sz};
+  const double inv[3] = {g.dxmin, g.dymin, g.dzmin};
+#pragma unroll
+  for (int dd = 0; dd < 3; dd++) {
+    const long s = st[dd];
+    if (cls == 0) {
+      double uv1b, uv1a, uv3b, uv3a;
+      if (dd == C) {
+        // self-advection: UV1(m)=Uint(m)*(F(m-1)+F(m)), Uint over F(m-2..m+1)
+        double Fi;
+        Fi = mask4(F[p - s], F[p], F[p + s], F[p + 2 * s]);
+        uv1b = Fi * (F[p] + F[p + s]);
+        Fi = mask4(F[p - 2 * s], F[p - s], F[p], F[p + s]);
+        uv1a = Fi * (F[p - s] + F[p]);
+        Fi = mask4(F[p], F[p + s], F[p + 2 * s], F[p + 3 * s]);
+        uv3b = Fi * (F[p] + F[p + 3 * s]);
+        Fi = mask4(F[p - 3 * s], F[p - 2 * s], F[p - s], F[p]);
+        uv3a = Fi * (F[p - 3 * s] + F[p]);
+      } else {
+        // cross term: advecting component A = (U,V,W)[dd] interpolated along
+        // axis C: Aint(m) = mask4(A(m, C-1..C+2)); UV1(m) = Aint(m)*(F(m)+F(m+1))
+        const double *A = dd == 0 ? U : dd == 1 ? V : W;
+        const long c = st[C];
+#define AI(o) mask4(A[p + (o) * s - c], A[p + (o) * s], A[p + (o) * s + c], A[p + (o) * s + 2 * c])
+        uv1b = AI(0) * (F[p] + F[p + s]);
+        uv1a = AI(-1) * (F[p - s] + F[p]);
+        uv3b = AI(1) * (F[p] + F[p + 3 * s]);
+        uv3a = AI(-2) * (F[p - 3 * s] + F[p]);
+#undef AI
+      }
+      d = 9 * (uv1b - uv1a);
+      d = d - (uv3b - uv3a) / 3;
+      acc = acc + d / inv[dd];
+    } else {
+      // 2nd order near boundaries (coef2ord = 64)
+      if (dd == C) {
+        d = 64 * ((F[p + s] + F[p]) * (F[p + s] + F[p]) - (F[p] + F[p - s]) * (F[p] + F[p - s]));
+      } else {
+        const double *A = dd == 0 ? U : dd == 1 ? V : W;
+        const long c = st[C];
+        // (F(+s)+F)*(A(+c)+A) - (F+F(-s))*(A(+c-s)+A(-s)); the reference's
+        // W/y term lists the last pair as (V(j-1)+V(j-1,k+1)): a+b = b+a.
+        d = 64 * ((F[p + s] + F[p]) * (A[p + c] + A[p]) - (F[p] + F[p - s]) * (A[p + c - s] + A[p - s]));
+      }
+      acc = acc + d / inv[dd];
+    }
+  }
+  return acc * (-1.0 / 256);
+}
+
+// ------------------------------------------------------ diffusion fluxes
+// Flux on the upper face of cell p in direction d for component C
+// (MomentumDiffusion_nobranch_2ord, src/momentum_diffusion.f90:169-288).
+template <int C>
+__device__ inline double dflux(const Geo &g, const double *__restrict__ nu, const double *__restrict__ F, long p,
+                               int d) {
+  const long sy = g.sy, sz = g.sz;
+  const double rx = 1.0 / (g.dxmin * g.dxmin), ry = 1.0 / (g.dymin * g.dymin), rz = 1.0 / (g.dzmin * g.dzmin);
+  if (C == 0) {
+    if (d == 0) return nu[p + 1] * (F[p + 1] - F[p]) * rx;
+    if (d == 1) return 0.25 * (nu[p + 1 + sy] + nu[p + 1] + nu[p + sy] + nu[p]) * (F[p + sy] - F[p]) * ry;
+    return 0.25 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) * rz;
+  } else if (C == 1) {
+    if (d == 0) return 0.25 * (nu[p + 1 + sy] + nu[p + 1] + nu[p + sy] + nu[p]) * (F[p + 1] - F[p]) * rx;
+    if (d == 1) return nu[p + sy] * (F[p + sy] - F[p]) * ry;
+    return 0.25 * (nu[p + sy + sz] + nu[p + sy] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) * rz;
+  } else {
+    if (d == 0) return 0.25 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + 1] - F[p]) * rx;
+    if (d == 1) return 0.25 * (nu[p + sy + sz] + nu[p + sz] + nu[p + sy] + nu[p]) * (F[p + sy] - F[p]) * ry;
+    return nu[p + sz] * (F[p + sz] - F[p]) * rz;
+  }
+}
+
+template <int C>
+__device__ inline double diffusion(const Geo &g, const double *__restrict__ nu, const double *__restrict__ F, long p,
+                                   double a) {
+  a = a + dflux<C>(g, nu, F, p, 0);
+  a = a - dflux<C>(g, nu, F, p - 1, 0);
+  a = a + dflux<C>(g, nu, F, p, 1);
+  a = a - dflux<C>(g, nu, F, p - g.sy, 1);
+  a = a + dflux<C>(g, nu, F, p, 2);
+  a = a - dflux<C>(g, nu, F, p - g.sz, 2);
+  return a;
+}
+
+// RK combine + PressureGrad + add(U2,U) for one cell
+template <int C>
+__device__ inline void finish(const Geo &g, const StageConst &sc, const double *__restrict__ F,
+                              const double *__restrict__ Pr, double *__restrict__ Ustar, double *__restrict__ U2,
+                              long p, int k, double w) {
+  double u2 = sc.stage > 0 ? Ustar[p] * sc.rho * sc.dt : 0.0;
+  u2 = u2 + w * sc.beta * sc.dt;
+  if (C == 0 && sc.prgx != 0) u2 = u2 + sc.A * sc.prgx;
+  if (C == 1 && sc.prgy != 0) u2 = u2 + sc.A * sc.prgy;
+  if (C == 0) u2 = u2 + sc.Ax * (Pr[p + 1] - Pr[p]);
+  if (C == 1) u2 = u2 + sc.Ay * (Pr[p + g.sy] - Pr[p]);
+  if (C == 2) {
+    if (g.variable_z)
+      u2 = u2 - sc.coef * (Pr[p + g.sz] - Pr[p]) / g.dzW[k];
+    else
+      u2 = u2 + sc.Az * (Pr[p + g.sz] - Pr[p]);
+  }
+  u2 = u2 + F[p];
+  Ustar[p] = w;
+  U2[p] = u2;
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+    k_momentum(Geo g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
+               const double *__restrict__ Visc, const double *__restrict__ Pr, double *__restrict__ Ustar,
+               double *__restrict__ U2, const uint8_t *__restrict__ code, StageConst sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.n[C][0] || j > g.n[C][1]) return;
+  const long p = gix(g, i, j, k);
+  const uint8_t cd = code[p];
+  if (cd & CODE_CORRECTED) return;
+  const double *F = C == 0 ? U : C == 1 ? V : W;
+  double w = cd4<C>(g, U, V, W, p, cd & CODE_ADV_MASK);
+  w = diffusion<C>(g, Visc, F, p, w);
+  finish<C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+}
+
+void launch_momentum(const Geo &g, int comp, const double *U, const double *V, const double *W, const double *Visc,
+                     const double *Pr, double *Ustar, double *U2, const uint8_t *code, StageConst sc,
+                     hipStream_t st) {
+  const int *n = g.n[comp];
+  dim3 b(64, 4, 1), gr(nblk(n[0], 64), nblk(n[1], 4), n[2]);
+  if (comp == 0)
+    hipLaunchKernelGGL(k_momentum<0>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+  else if (comp == 1)
+    hipLaunchKernelGGL(k_momentum<1>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+  else
+    hipLaunchKernelGGL(k_momentum<2>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+}
+
+// Cells with wall-model face-flux replacements (wmfluxes-nobranch-inc.f90:
+// 49-208): the bulk value, then the terms in the reference's scatter order.
+// term = {axis | neg<<2, WM point index, face-cell linear index, 0}
+template <int C>
+__global__ void k_momentum_corr(Geo g, const double *__restrict__ U, const double *__restrict__ V,
+                                const double *__restrict__ W, const double *__restrict__ Visc,
+                                const double *__restrict__ Pr, double *__restrict__ Ustar, double *__restrict__ U2,
+                                const uint8_t *__restrict__ code, StageConst sc, TermList tl,
+                                const double *__restrict__ fluxp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= tl.n) return;
+  const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
+  const long p = gix(g, i, j, k);
+  const double *F = C == 0 ? U : C == 1 ? V : W;
+  double w = cd4<C>(g, U, V, W, p, code[p] & CODE_ADV_MASK);
+  w = diffusion<C>(g, Visc, F, p, w);
+  for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
+    const int32_t *tt = &tl.term[4 * t];
+    const int ax = tt[0] & 3;
+    const bool neg = (tt[0] >> 2) & 1;
+    const double fl = dflux<C>(g, Visc, F, (long)tt[2], ax);
+    const double fp = fluxp[tt[1]];
+    if (neg) {
+      w = w - fl;
+      w = w - fp;
+    } else {
+      w = w + fl;
+      w = w + fp;
+    }
+  }
+  finish<C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+}
+
+void launch_momentum_corr(const Geo &g, int comp, const double *U, const double *V, const double *W,
+                          const double *Visc, const double *Pr, double *Ustar, double *U2, const uint8_t *code,
+                          StageConst sc, TermList tl, const double *fluxp, hipStream_t st) {
+  if (tl.n == 0) return;
+  dim3 b(128), gr(nblk(tl.n, 128));
+  if (comp == 0)
+    hipLaunchKernelGGL(k_momentum_corr<0>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+  else if (comp == 1)
+    hipLaunchKernelGGL(k_momentum_corr<1>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+  else
+    hipLaunchKernelGGL(k_momentum_corr<2>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+}
+
+// ------------------------------------------------------------- IB gathers
+// TIBPoint_Interpolate, src/immersedboundary.f90:637-651; IBMomentum is
+// gather-all-then-scatter (src/tsteps.f90:368-418).
+__global__ void k_ib_interp(Geo g, IBDev ib, const double *__restrict__ F, double *__restrict__ tmp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ib.n) return;
+  double s = 0;
+  for (int t = ib.ptr[p]; t < ib.ptr[p + 1]; t++)
+    s = s + ib.coef[t] * F[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
+  tmp[p] = s;
+}
+__global__ void k_ib_scatter(Geo g, IBDev ib, const double *__restrict__ tmp, double *__restrict__ F) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ib.n) return;
+  F[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = tmp[p];
+}
+void launch_ib_interp(const Geo &g, IBDev ib, const double *F, double *tmp, hipStream_t st) {
+  if (ib.n) hipLaunchKernelGGL(k_ib_interp, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, F, tmp);
+}
+void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, double *F, hipStream_t st) {
+  if (ib.n) hipLaunchKernelGGL(k_ib_scatter, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, tmp, F);
+}
+
+// ScalFlIBPoints viscosity loop (src/dynamics.f90:899-902) is sequential:
+// a point may read a value written by an earlier point.  The host groups the
+// points into dependency levels; points of one level are independent.
+__global__ void k_ib_visc_level(Geo g, IBDev ib, double *__restrict__ Visc, int p0, int p1) {
+  const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  double s = 0;
+  for (int t = ib.ptr[p]; t < ib.ptr[p + 1]; t++)
+    s = s + ib.coef[t] * Visc[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
+  Visc[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = s;
+}
+void launch_ib_visc_serial(const Geo &g, IBDev ib, double *Visc, const int32_t *level_ptr, int nlevels,
+                           hipStream_t st) {
+  for (int l = 0; l < nlevels; l++) {
+    int p0 = level_ptr[l], p1 = level_ptr[l + 1];
+    if (p1 > p0) hipLaunchKernelGGL(k_ib_visc_level, dim3(nblk(p1 - p0, 256)), dim3(256), 0, st, g, ib, Visc, p0, p1);
+  }
+}
+
+// NullInterior, src/tsteps.f90:543-567
+__global__ void k_null(Geo g, int n, const int32_t *__restrict__ ijk, double *__restrict__ F) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) F[gix(g, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2])] = 0;
+}
+void launch_null(const Geo &g, int n, const int32_t *ijk, double *F, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_null, dim3(nblk(n, 256)), dim3(256), 0, st, g, n, ijk, F);
+}
+
+// IBMassSources (2nd order), src/tsteps.f90:490-531, as a gather: every
+// written Q cell sums its contributions in the reference's single-thread
+// order (the reference scatters with omp atomic).  Cells that only receive a
+// Bound_Q fold are listed with no terms, so their pre-fold value is 0.
+// term = {component | neg<<2, source linear index, spacing index, 0}
+__global__ void k_mass_sources(Geo g, TermList tl, const double *__restrict__ U, const double *__restrict__ V,
+                               const double *__restrict__ W, double *__restrict__ Q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= tl.n) return;
+  double q = 0;
+  for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
+    const int32_t *tt = &tl.term[4 * t];
+    const int comp = tt[0] & 3;
+    const bool neg = (tt[0] >> 2) & 1;
+    const double *F = comp == 0 ? U : comp == 1 ? V : W;
+    const double *dl = comp == 0 ? g.dxPr : comp == 1 ? g.dyPr : g.dzPr;
+    const double val = F[(long)tt[1]] / dl[tt[2]];
+    q = neg ? q - val : q + val;
+  }
+  Q[gix(g, tl.cell[3 * c], tl.cell[3 * c + 1], tl.cell[3 * c + 2])] = q;
+}
+
+// Bound_Q, src/boundaries.f90:2094-2192: fold ghost layers into the interior,
+// one axis per launch (y reads the x-folded values, z the y-folded ones).
+__global__ void k_fold_q(Geo g, double *__restrict__ Q, int axis, int periodic) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int b = blockIdx.y + 1;
+  const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  const int na = axis == 0 ? g.ny : g.nx;
+  if (a > na) return;
+  auto at = [&](int m) -> double & {
+    if (axis == 0) return Q[gix(g, m, a, b)];
+    if (axis == 1) return Q[gix(g, a, m, b)];
+    return Q[gix(g, a, b, m)];
+  };
+  if (periodic) {
+    at(n) = at(0) + at(n);
+    at(1) = at(1) + at(n + 1);
+  } else {
+    at(1) = at(1) + at(0);
+    at(n) = at(n) + at(n + 1);
+  }
+}
+
+void launch_mass_sources(const Geo &g, TermList tl, const double *U, const double *V, const double *W, double *Q,
+                         hipStream_t st) {
+  if (!tl.n) return;
+  hipLaunchKernelGGL(k_mass_sources, dim3(nblk(tl.n, 256)), dim3(256), 0, st, g, tl, U, V, W, Q);
+  hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.ny, 64), g.nz), dim3(64), 0, st, g, Q, 0, g.bt[0] == 3);
+  hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.nx, 64), g.nz), dim3(64), 0, st, g, Q, 1, g.bt[2] == 3);
+  hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.nx, 64), g.ny), dim3(64), 0, st, g, Q, 2, g.bt[4] == 3);
+}
+
+}  // namespace elmm

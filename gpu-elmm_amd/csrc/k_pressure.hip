@@ -1,0 +1,455 @@
+// k_pressure.hip - pressure projection on the device: divergence -> FFT
+// Poisson solve -> velocity correction + pressure update + reference fix.
+//
+// Replaces PrePoisson/PostPoisson/FixPressureToReference
+// (src/pressure.f90:272-723, 728-991, 994-1229) and the external FFT
+// library behind Poiss_PoisSolver (src/poisson.f90:76-210), solving the
+// FD2 operator of Residue_variable_z (src/poisson.f90:23-60):
+//   periodic x, y : discrete Fourier modes, eigenvalue -4 sin^2(pi m/n)/d^2
+//   z (walls)     : tridiagonal per mode with the staggered-Neumann rows,
+//                   solved by parallel cyclic reduction in LDS.
+// Passes (each one read + one write of the spectral array in HBM):
+//   1 divergence + x FFT of each (j,k) row  (RHS never goes to HBM)
+//   2 y FFT of tiles of 8 x-modes
+//   3 z tridiagonal solve of tiles of 8 x-modes
+//   4 inverse y FFT
+//   5 inverse x FFT, real part -> Phi
+#include <cmath>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace elmm {
+
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+__device__ inline int bitrev(int v, int logn) { return (int)(__brev((unsigned)v) >> (32 - logn)); }
+
+// In-place radix-2 decimation-in-time FFT of `cnt` independent lines stored
+// in LDS, line l at buf[l*stride + m].  Input must already be in
+// bit-reversed order.  tw[t] = exp(-2 pi i t / n); inverse uses conj.
+__device__ void lds_fft(double2 *buf, int n, int logn, int cnt, int stride, const double2 *__restrict__ tw,
+                        bool inverse) {
+  const int nbf = (n / 2) * cnt;
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, tstep = n / len;
+    for (int b = threadIdx.x; b < nbf; b += blockDim.x) {
+      const int l = b / (n / 2), r = b % (n / 2);
+      const int grp = r / half, pos = r % half;
+      const int i0 = l * stride + grp * len + pos, i1 = i0 + half;
+      double2 w = tw[pos * tstep];
+      if (inverse) w.y = -w.y;
+      const double2 a = buf[i0], c = cmul(buf[i1], w);
+      buf[i0] = cadd(a, c);
+      buf[i1] = csub(a, c);
+    }
+    __syncthreads();
+  }
+}
+
+// Pass 1: RHS = ((dU/dx + dV/dy + dW/dz) - Q) / coef for the rows of this
+// block, then forward FFT in x.  One block handles ROWS rows.
+__global__ void k_fft_x_fwd(Geo g, PoissonPlan pp, const double *__restrict__ U, const double *__restrict__ V,
+                            const double *__restrict__ W, const double *__restrict__ Q,
+                            const double *__restrict__ rhs_in, double dt2_rec, double *__restrict__ rhs_out,
+                            int rows_per_block, int logn) {
+  extern __shared__ double2 sh[];
+  const int nx = pp.nx;
+  const int row0 = blockIdx.x * rows_per_block;
+  const int nrows = (long)pp.ny * pp.nz;
+  for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
+    const int l = t / nx, m = t % nx;
+    const int row = row0 + l;
+    double r = 0;
+    if (row < nrows) {
+      const int j = row % pp.ny + 1, k = row / pp.ny + 1, i = m + 1;
+      const long p = gix(g, i, j, k);
+      if (rhs_in) {
+        r = rhs_in[p];
+      } else {
+        const double dz = g.variable_z ? g.dzPr[k] : g.dzmin;
+        r = (U[p] - U[p - 1]) / g.dxmin + (V[p] - V[p - g.sy]) / g.dymin + (W[p] - W[p - g.sz]) / dz;
+        if (Q) r = r - Q[p];
+        r = r * dt2_rec;
+        if (rhs_out) rhs_out[p] = r;
+      }
+    }
+    sh[l * nx + bitrev(m, logn)] = make_double2(r, 0.0);
+  }
+  __syncthreads();
+  lds_fft(sh, nx, logn, rows_per_block, nx, pp.twx, false);
+  const int M = pp.M;
+  for (int t = threadIdx.x; t < rows_per_block * M; t += blockDim.x) {
+    const int l = t / M, m = t % M;
+    const int row = row0 + l;
+    if (row < nrows) pp.X[(long)row * M + m] = sh[l * nx + m];
+  }
+}
+
+// Passes 2 and 4: FFT along y of TX consecutive x-modes for one z-plane.
+template <int TX>
+__global__ void k_fft_y(PoissonPlan pp, int logn, int inverse) {
+  extern __shared__ double2 sh[];
+  const int ny = pp.ny, M = pp.M;
+  const int m0 = blockIdx.x * TX, k = blockIdx.y;
+  double2 *X = pp.X + (long)k * ny * M;
+  for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
+    const int c = t % TX, j = t / TX;
+    const int m = m0 + c;
+    double2 v = m < M ? X[(long)j * M + m] : make_double2(0, 0);
+    sh[c * ny + bitrev(j, logn)] = v;
+  }
+  __syncthreads();
+  lds_fft(sh, ny, logn, TX, ny, pp.twy, inverse != 0);
+  for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
+    const int c = t % TX, j = t / TX;
+    const int m = m0 + c;
+    if (m < M) X[(long)j * M + m] = sh[c * ny + j];
+  }
+}
+
+// Pass 3: per (m, j) mode, a_k x_{k-1} + (lam - a_k - c_k) x_k + c_k x_{k+1}
+// = d_k, solved by parallel cyclic reduction in LDS.  The singular mode
+// (m=j=0) pins x_1 = 0 and is integrated exactly (see below).
+template <int TX>
+__global__ void k_solve_z(PoissonPlan pp) {
+  extern __shared__ double smem[];
+  const int nz = pp.nz, ny = pp.ny, M = pp.M;
+  const int m0 = blockIdx.x * TX, j = blockIdx.y;
+  const int N = TX * nz;
+  double *A = smem, *B = A + N, *C = B + N;
+  double2 *D = (double2 *)(C + N);
+  const long plane = (long)ny * M;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const int c = t % TX, k = t / TX;
+    const int m = m0 + c;
+    const int s = c * nz + k;
+    double2 d = make_double2(0, 0);
+    double lam = 0;
+    if (m < M) {
+      d = pp.X[(long)k * plane + (long)j * M + m];
+      lam = pp.lamx[m] + pp.lamy[j];
+    }
+    double a = pp.az[k], cc = pp.cz[k];
+    double b = lam - a - cc;
+    if ((m == 0 && j == 0) || m >= M) {
+      // identity rows: padding columns, and the singular mean mode, which is
+      // integrated exactly after the reduction (its D survives unchanged)
+      a = 0;
+      b = 1;
+      cc = 0;
+      if (m >= M) d = make_double2(0, 0);
+    }
+    A[s] = a;
+    B[s] = b;
+    C[s] = cc;
+    D[s] = d;
+  }
+  __syncthreads();
+  for (int h = 1; h < nz; h <<= 1) {
+    double na[8], nb[8], nc[8];
+    double2 nd[8];
+    int cnt = 0;
+    for (int t = threadIdx.x; t < N; t += blockDim.x, cnt++) {
+      const int c = t / nz, k = t % nz;
+      const int s = c * nz + k;
+      double a = A[s], b = B[s], cc = C[s];
+      double2 d = D[s];
+      double a2 = 0, c2 = 0;
+      double2 d2 = d;
+      double b2 = b;
+      if (k - h >= 0) {
+        const double k1 = a / B[s - h];
+        a2 = -A[s - h] * k1;
+        b2 = b2 - C[s - h] * k1;
+        d2 = make_double2(d2.x - D[s - h].x * k1, d2.y - D[s - h].y * k1);
+      }
+      if (k + h < nz) {
+        const double k2 = cc / B[s + h];
+        c2 = -C[s + h] * k2;
+        b2 = b2 - A[s + h] * k2;
+        d2 = make_double2(d2.x - D[s + h].x * k2, d2.y - D[s + h].y * k2);
+      }
+      na[cnt] = a2;
+      nb[cnt] = b2;
+      nc[cnt] = c2;
+      nd[cnt] = d2;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int t = threadIdx.x; t < N; t += blockDim.x, cnt++) {
+      const int c = t / nz, k = t % nz;
+      const int s = c * nz + k;
+      A[s] = na[cnt];
+      B[s] = nb[cnt];
+      C[s] = nc[cnt];
+      D[s] = nd[cnt];
+    }
+    __syncthreads();
+  }
+  if (m0 == 0 && j == 0 && threadIdx.x == 0) {
+    // Horizontal-mean mode with x_1 pinned to 0 (rows 2..nz of the Neumann
+    // problem): G_{k-1} = G_k - d_k dzPr(k) from G_nz = 0 downwards, then
+    // x_{k+1} = x_k + G_k dzW(k).  Both sums are compensated (Kahan), so the
+    // mode is exact to rounding instead of carrying the nz^2 growth of an
+    // elimination on the singular operator.
+    double2 G = make_double2(0, 0), Gc = G;
+    for (int k = nz; k >= 2; k--) {
+      const double dz = pp.dzPr[k];
+      double2 t = make_double2(-D[k - 1].x * dz, -D[k - 1].y * dz);
+      double2 y = make_double2(t.x - Gc.x, t.y - Gc.y);
+      double2 s2 = make_double2(G.x + y.x, G.y + y.y);
+      Gc = make_double2((s2.x - G.x) - y.x, (s2.y - G.y) - y.y);
+      G = s2;
+      A[k - 2] = G.x;  // column 0 of A and C is free after the reduction
+      C[k - 2] = G.y;
+    }
+    double2 x = make_double2(0, 0), xc = x;
+    D[0] = x;
+    for (int k = 1; k <= nz - 1; k++) {
+      const double dz = pp.dzW[k];
+      double2 t = make_double2(A[k - 1] * dz, C[k - 1] * dz);
+      double2 y = make_double2(t.x - xc.x, t.y - xc.y);
+      double2 s2 = make_double2(x.x + y.x, x.y + y.y);
+      xc = make_double2((s2.x - x.x) - y.x, (s2.y - x.y) - y.y);
+      x = s2;
+      D[k] = x;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const int c = t % TX, k = t / TX;
+    const int m = m0 + c;
+    const int s = c * nz + k;
+    if (m < M) {
+      const double b = B[s];
+      pp.X[(long)k * plane + (long)j * M + m] = make_double2(D[s].x / b, D[s].y / b);
+    }
+  }
+}
+
+// Pass 5: inverse x FFT from the half spectrum (Hermitian symmetry), real
+// part scaled by 1/(nx*ny) -> Phi interior.
+__global__ void k_fft_x_inv(Geo g, PoissonPlan pp, double *__restrict__ Phi, int rows_per_block, int logn) {
+  extern __shared__ double2 sh[];
+  const int nx = pp.nx, M = pp.M;
+  const int row0 = blockIdx.x * rows_per_block;
+  const int nrows = pp.ny * pp.nz;
+  for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
+    const int l = t / nx, m = t % nx;
+    const int row = row0 + l;
+    double2 v = make_double2(0, 0);
+    if (row < nrows) {
+      if (m < M) {
+        v = pp.X[(long)row * M + m];
+      } else {
+        v = pp.X[(long)row * M + (nx - m)];
+        v.y = -v.y;
+      }
+    }
+    sh[l * nx + bitrev(m, logn)] = v;
+  }
+  __syncthreads();
+  lds_fft(sh, nx, logn, rows_per_block, nx, pp.twx, true);
+  const double scale = 1.0 / ((double)pp.nx * pp.ny);
+  for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
+    const int l = t / nx, m = t % nx;
+    const int row = row0 + l;
+    if (row < nrows) {
+      const int j = row % pp.ny + 1, k = row / pp.ny + 1;
+      Phi[gix(g, m + 1, j, k)] = sh[l * nx + m].x * scale;
+    }
+  }
+}
+
+static int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return l;
+}
+
+static int rows_for(int nx) { return std::max(1, 1024 / nx); }
+
+void launch_poisson_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
+                        const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
+  const int rows = rows_for(pp.nx);
+  const int nrows = pp.ny * pp.nz;
+  size_t sh = (size_t)rows * pp.nx * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, dt2_rec,
+                     rhs_out, rows, ilog2(pp.nx));
+  constexpr int TX = 8;
+  hipLaunchKernelGGL(k_fft_y<TX>, dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(double2), st,
+                     pp, ilog2(pp.ny), 0);
+}
+
+void launch_poisson_solve_z(const Geo &, const PoissonPlan &pp, hipStream_t st) {
+  constexpr int TX = 8;
+  size_t sh = (size_t)TX * pp.nz * (3 * sizeof(double) + sizeof(double2));
+  hipLaunchKernelGGL(k_solve_z<TX>, dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
+}
+
+void launch_poisson_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st) {
+  constexpr int TX = 8;
+  hipLaunchKernelGGL(k_fft_y<TX>, dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(double2), st,
+                     pp, ilog2(pp.ny), 1);
+  const int rows = rows_for(pp.nx);
+  const int nrows = pp.ny * pp.nz;
+  size_t sh = (size_t)rows * pp.nx * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, Phi, rows, ilog2(pp.nx));
+}
+
+// ------------------------------------------------------------ reductions
+// Block partial sums/maxima, combined by the consumer in block order so the
+// result does not depend on scheduling.
+__device__ inline double block_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+  return s;
+}
+__device__ inline double block_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s = fmax(s, red[w]);
+  return s;
+}
+
+// Sum over the top layer of (Pr + Phi): FixPressureToReference with the
+// default reference boundary To (src/pressure.f90:994-1229).
+__global__ void k_top_sum(Geo g, const double *__restrict__ Pr, const double *__restrict__ Phi,
+                          double *__restrict__ partial) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (t < (long)g.nx * g.ny) {
+    const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
+    const long p = gix(g, i, j, g.nz);
+    v = Pr[p] + Phi[p];
+  }
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+void launch_top_sum(const Geo &g, const double *Pr, const double *Phi, double *partial, int nblocks,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_top_sum, dim3(nblocks), dim3(256), 0, st, g, Pr, Phi, partial);
+}
+
+// PostPoisson (2nd order) + Pr = Pr + Phi - ref:
+// U -= coef*(Phi(i+1)-Phi(i))/dx etc. (src/pressure.f90:789-853).  x and y
+// are periodic, so Phi(nx+1) = Phi(1) is read by index wrap (Bound_Phi).
+__global__ void __launch_bounds__(256)
+    k_projection(Geo g, double *__restrict__ U, double *__restrict__ V, double *__restrict__ W,
+                 double *__restrict__ Pr, const double *__restrict__ Phi, const double *__restrict__ partial,
+                 int nblocks, double dt2) {
+  __shared__ double ref_s;
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    double s = 0;
+    for (int b = 0; b < nblocks; b++) s += partial[b];
+    ref_s = s / (double)(g.nx * g.ny);
+  }
+  __syncthreads();
+  const double ref = ref_s;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long p = gix(g, i, j, k);
+  const double ph = Phi[p];
+  const double phx = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
+  const double phy = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
+  const double Au = dt2 / g.dxmin, Av = dt2 / g.dymin, Aw = dt2 / g.dzmin;
+  if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) U[p] = U[p] - Au * (phx - ph);
+  if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) V[p] = V[p] - Av * (phy - ph);
+  if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
+    const double phz = Phi[p + g.sz];
+    if (g.variable_z)
+      W[p] = W[p] - dt2 * (phz - ph) / g.dzW[k];
+    else
+      W[p] = W[p] - Aw * (phz - ph);
+  }
+  double pr = Pr[p] + ph;
+  if (ref != 0) pr = pr - ref;
+  Pr[p] = pr;
+}
+
+void launch_projection(const Geo &g, double *U, double *V, double *W, double *Pr, const double *Phi,
+                       const double *partial, int nblocks, double dt2, hipStream_t st) {
+  dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
+  hipLaunchKernelGGL(k_projection, gr, b, 0, st, g, U, V, W, Pr, Phi, partial, nblocks, dt2);
+}
+
+// max |div U - Q| (check_divergence, src/pressure.f90:937-981)
+__global__ void k_max_div(Geo g, const double *__restrict__ U, const double *__restrict__ V,
+                          const double *__restrict__ W, const double *__restrict__ Q, double *__restrict__ partial) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (t < (long)g.nx * g.ny * g.nz) {
+    const int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
+    const long p = gix(g, i, j, k);
+    const double dz = g.variable_z ? g.dzPr[k] : g.dzmin;
+    double d = (U[p] - U[p - 1]) / (g.dxmin) + (V[p] - V[p - g.sy]) / (g.dymin) + (W[p] - W[p - g.sz]) / (dz);
+    if (Q) d = d - Q[p];
+    v = fabs(d);
+  }
+  const double m = block_max(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+void launch_max_div(const Geo &g, const double *U, const double *V, const double *W, const double *Q,
+                    double *partial, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_max_div, dim3(nblocks), dim3(256), 0, st, g, U, V, W, Q, partial);
+}
+
+// Residue_variable_z, src/poisson.f90:23-60 (Phi ghosts: periodic x/y by
+// index wrap, mirror in z)
+__global__ void k_residual(Geo g, const double *__restrict__ Phi, const double *__restrict__ RHS,
+                           double *__restrict__ partial) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (t < (long)g.nx * g.ny * g.nz) {
+    const int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
+    const long p = gix(g, i, j, k);
+    const double Aw = 1 / (g.dxmin * g.dxmin), Ae = Aw, As = 1 / (g.dymin * g.dymin), An = As;
+    const double Ab = 1 / ((g.zPr[k] - g.zPr[k - 1]) * (g.zW[k] - g.zW[k - 1]));
+    const double At = 1 / ((g.zPr[k + 1] - g.zPr[k]) * (g.zW[k] - g.zW[k - 1]));
+    const double Ap = Aw + Ae + As + An + Ab + At;
+    const double pw = Phi[i > 1 ? p - 1 : p - 1 + g.nx], pe = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
+    const double ps = Phi[j > 1 ? p - g.sy : p - g.sy + (long)g.ny * g.sy];
+    const double pn = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
+    const double pb = Phi[k > 1 ? p - g.sz : p], pt = Phi[k < g.nz ? p + g.sz : p];
+    double q = 0;
+    q = q + pw * Aw;
+    q = q + pe * Ae;
+    q = q + ps * As;
+    q = q + pn * An;
+    q = q + pb * Ab;
+    q = q + pt * At;
+    q = q - RHS[p];
+    v = fabs(-q + Ap * Phi[p]);
+  }
+  const double m = block_max(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+void launch_residual(const Geo &g, const double *Phi, const double *RHS, double *partial, int nblocks,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_residual, dim3(nblocks), dim3(256), 0, st, g, Phi, RHS, partial);
+}
+
+}  // namespace elmm

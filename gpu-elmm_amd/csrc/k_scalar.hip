@@ -1,0 +1,199 @@
+// k_scalar.hip - passive-scalar RK3 stage as one fused pass per cell:
+// kappa-scheme advection with flux limiter (KappaScalar,
+// src/scalar_advection.f90:102-269), zeroing in solids (src/scalars.f90:
+// 261-269), explicit diffusion (ScalarDiffusion_nobranch,
+// src/scalar_diffusion.f90:404-449), RK combine and update
+// (src/scalars.f90:279-283).
+//
+// The reference scatters each face flux into the two cells it separates
+// (x, y passes in increasing index order; z pass over even faces first, then
+// odd faces).  Here every cell gathers its two fluxes per direction and adds
+// them in exactly that order, so the result is the reference's value bit for
+// bit and the pass needs no atomics and no Scal2 zero-fill.
+#include <cmath>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace elmm {
+
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__device__ inline double flux_limiter(double r) {
+  double a = (1 + 2 * r) / 3;
+  double b = 2.0 < a ? 2.0 : a;
+  double c = 2 * r < b ? 2 * r : b;
+  return 0.0 > c ? 0.0 : c;
+}
+
+// Flux through the upper face of cell p along stride s with face velocity u.
+__device__ inline double kappa_flux(const double *__restrict__ S, long p, long s, double u) {
+  const double eps = (double)1e-8f;  // default-real literal in the reference
+  double sr, sl;
+  if (u > 0) {
+    sr = S[p + s] - S[p];
+    sl = S[p] - S[p - s];
+  } else {
+    sr = S[p] - S[p + s];
+    sl = S[p + s] - S[p + 2 * s];
+  }
+  const double sg = copysign(1.0, sl);
+  const double slope = flux_limiter((sr + eps * sg) / (sl + eps * sg));
+  if (u > 0) return u * (S[p] + (S[p] - S[p - s]) * slope / 2.0);
+  return u * (S[p + s] + (S[p + s] - S[p + 2 * s]) * slope / 2.0);
+}
+
+__device__ inline double scalar_cell(const Geo &g, const double *__restrict__ U, const double *__restrict__ V,
+                                     const double *__restrict__ W, const double *__restrict__ T,
+                                     const double *__restrict__ S, const uint8_t *__restrict__ mx,
+                                     const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, long p, int k,
+                                     uint8_t code) {
+  const long sy = g.sy, sz = g.sz;
+  const double Ax = 1 / g.dxmin, Ay = 1 / g.dymin, Az = 1 / g.dzmin;
+  double a = 0;
+  // x pass: face i-1 adds, face i subtracts
+  if (mx[p - 1]) a = a + Ax * kappa_flux(S, p - 1, 1, U[p - 1]);
+  if (mx[p]) a = a - Ax * kappa_flux(S, p, 1, U[p]);
+  if (my[p - sy]) a = a + Ay * kappa_flux(S, p - sy, sy, V[p - sy]);
+  if (my[p]) a = a - Ay * kappa_flux(S, p, sy, V[p]);
+  // z pass: faces of even index are processed before faces of odd index
+  const bool mlo = mz[p - sz] != 0, mhi = mz[p] != 0;
+  if ((k & 1) == 0) {
+    if (mhi) a = a - Az * kappa_flux(S, p, sz, W[p]);
+    if (mlo) a = a + Az * kappa_flux(S, p - sz, sz, W[p - sz]);
+  } else {
+    if (mlo) a = a + Az * kappa_flux(S, p - sz, sz, W[p - sz]);
+    if (mhi) a = a - Az * kappa_flux(S, p, sz, W[p]);
+  }
+  if (code & CODE_SOLID) a = 0;
+  const double Dx = 1 / (2 * (g.dxmin * g.dxmin)), Dy = 1 / (2 * (g.dymin * g.dymin)),
+               Dz = 1 / (2 * (g.dzmin * g.dzmin));
+  a = a + (T[p + 1] + T[p]) * (S[p + 1] - S[p]) * Dx;
+  a = a - (T[p] + T[p - 1]) * (S[p] - S[p - 1]) * Dx;
+  a = a + (T[p + sy] + T[p]) * (S[p + sy] - S[p]) * Dy;
+  a = a - (T[p] + T[p - sy]) * (S[p] - S[p - sy]) * Dy;
+  a = a + (T[p + sz] + T[p]) * (S[p + sz] - S[p]) * Dz;
+  a = a - (T[p] + T[p - sz]) * (S[p] - S[p - sz]) * Dz;
+  return a;
+}
+
+__device__ inline void scalar_finish(const StageConst &sc, const double *__restrict__ S, double *__restrict__ Snew,
+                                     double *__restrict__ Sadv, long p, double a) {
+  double s2 = sc.stage > 0 ? Sadv[p] * sc.rho_dt : 0.0;
+  s2 = s2 + a * sc.beta_dt;
+  Sadv[p] = a;
+  Snew[p] = S[p] + s2;
+}
+
+__global__ void __launch_bounds__(256)
+    k_scalar(Geo g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
+             const double *__restrict__ T, const double *__restrict__ S, double *__restrict__ Snew,
+             double *__restrict__ Sadv, const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
+             const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long p = gix(g, i, j, k);
+  const uint8_t cd = code[p];
+  if (cd & CODE_CORRECTED) return;
+  double a = scalar_cell(g, U, V, W, T, S, mx, my, mz, p, k, cd);
+  scalar_finish(sc, S, Snew, Sadv, p, a);
+}
+
+void launch_scalar(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
+                   const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
+                   const uint8_t *my, const uint8_t *mz, StageConst sc, hipStream_t st) {
+  dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
+  hipLaunchKernelGGL(k_scalar, gr, b, 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code, mx, my, mz, sc);
+}
+
+// Cells with ordered extra terms: scalar wall-face flux removal
+// (src/scalar_diffusion.f90:451-505), point/volume sources (FluxKernel,
+// src/volumesources.f90:757-771) and the wall-model divergence term
+// (ScalarExtra, src/scalars.f90:364-379).
+// term = {kind | axis<<2, a, b, 0}
+//   kind 0: subtract face flux, face lower cell a (linear), axis
+//   kind 1: add face flux, face lower cell a (linear), axis
+//   kind 2: add src_flux[a]
+//   kind 3: add wm_div[a] * S(cell)
+__global__ void k_scalar_corr(Geo g, const double *__restrict__ U, const double *__restrict__ V,
+                              const double *__restrict__ W, const double *__restrict__ T,
+                              const double *__restrict__ S, double *__restrict__ Snew, double *__restrict__ Sadv,
+                              const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
+                              const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc,
+                              TermList tl, const double *__restrict__ src_flux, const double *__restrict__ wm_div) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= tl.n) return;
+  const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
+  const long p = gix(g, i, j, k);
+  double a = scalar_cell(g, U, V, W, T, S, mx, my, mz, p, k, code[p]);
+  const double Dd[3] = {1 / (2 * (g.dxmin * g.dxmin)), 1 / (2 * (g.dymin * g.dymin)), 1 / (2 * (g.dzmin * g.dzmin))};
+  const long st[3] = {1, g.sy, g.sz};
+  for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
+    const int32_t *tt = &tl.term[4 * t];
+    const int kind = tt[0] & 3, axis = (tt[0] >> 2) & 3;
+    if (kind <= 1) {
+      const long q = tt[1], r = q + st[axis];
+      const double fl = (T[r] + T[q]) * (S[r] - S[q]) * Dd[axis];
+      a = kind == 0 ? a - fl : a + fl;
+    } else if (kind == 2) {
+      a = a + src_flux[tt[1]];
+    } else {
+      a = a + wm_div[tt[1]] * S[p];
+    }
+  }
+  scalar_finish(sc, S, Snew, Sadv, p, a);
+}
+
+void launch_scalar_corr(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
+                        const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
+                        const uint8_t *my, const uint8_t *mz, StageConst sc, TermList tl, const double *src_flux,
+                        const double *wm_div, hipStream_t st) {
+  if (!tl.n) return;
+  hipLaunchKernelGGL(k_scalar_corr, dim3(nblk(tl.n, 128)), dim3(128), 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code,
+                     mx, my, mz, sc, tl, src_flux, wm_div);
+}
+
+// DivergenceWM (2nd order), src/wallmodels.f90:1153-1166
+__global__ void k_wm_div(Geo g, WMScalarPoints wp, const int32_t *__restrict__ pt, const double *__restrict__ U,
+                         const double *__restrict__ V, const double *__restrict__ W) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= wp.n) return;
+  const int xi = wp.ijk[3 * n], yj = wp.ijk[3 * n + 1], zk = wp.ijk[3 * n + 2];
+  const long p = gix(g, xi, yj, zk);
+  const long sy = g.sy, sz = g.sz;
+  double div = 0;
+  if (pt[p + 1] <= 0) div = div + U[p] / g.dxmin;
+  if (pt[p - 1] <= 0) div = div - U[p - 1] / g.dxmin;
+  if (pt[p + sy] <= 0) div = div + V[p] / g.dymin;
+  if (pt[p - sy] <= 0) div = div - V[p - sy] / g.dymin;
+  if (pt[p + sz] <= 0) div = div + W[p] / g.dzPr[zk];
+  if (pt[p - sz] <= 0) div = div - W[p - sz] / g.dzPr[zk];
+  wp.div[n] = div;
+}
+
+void launch_wm_div(const Geo &g, WMScalarPoints p, const int32_t *prtype, const double *U, const double *V,
+                   const double *W, hipStream_t st) {
+  if (p.n) hipLaunchKernelGGL(k_wm_div, dim3(nblk(p.n, 256)), dim3(256), 0, st, g, p, prtype, U, V, W);
+}
+
+// TDiff = (Viscosity - molecular_viscosity)/constPr_sgs + molecular_diffusivity,
+// src/initial.f90:4302-4313 (constPr_sgs = 0.6 as a default-real literal)
+__global__ void k_tdiff_init(Geo g, const double *__restrict__ Visc, double *__restrict__ TDiff, double nu,
+                             double nu_scalar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long p = gix(g, i, j, k);
+  TDiff[p] = (Visc[p] - nu) / (double)0.6f + nu_scalar;
+}
+
+void launch_tdiff_init(const Geo &g, const double *Visc, double *TDiff, double nu, double nu_scalar,
+                       hipStream_t st) {
+  dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
+  hipLaunchKernelGGL(k_tdiff_init, gr, b, 0, st, g, Visc, TDiff, nu, nu_scalar);
+}
+
+}  // namespace elmm

@@ -1,0 +1,100 @@
+// kernels.hpp - host-side launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "device_common.hpp"
+
+namespace elmm {
+
+// ghost fills (k_ghost.hip)
+template <typename T> void launch_bound_uvw(const Geo &g, T *U, T *V, T *W, hipStream_t st);
+template <typename T> void launch_bound_visc(const Geo &g, T *F, hipStream_t st);
+template <typename T> void launch_bound_scalar(const Geo &g, T *F, hipStream_t st);
+template <typename T> void launch_bound_pr(const Geo &g, T *P, hipStream_t st);
+
+// Wall-model points of the 18 momentum lists, concatenated (SoA).
+struct WMPoints {
+  int n;
+  const int32_t *ijk;      // 3 per point
+  const int32_t *list;     // list id 0..17 = comp*6+dir
+  const double *dist;      // 3 per point
+  const double *z0;
+  double *ustar;           // in/out state
+  double *fluxp;           // out
+  const double *logdz0;    // log(|dist|/z0) per point (static, from the host libm)
+};
+
+// Scalar wall-model points (WMPoints of the reference).
+struct WMScalarPoints {
+  int n;
+  const int32_t *ijk;
+  double *div;
+};
+
+// IB list on the device (CSR).
+struct IBDev {
+  int n;
+  const int32_t *tgt;      // 3 per point
+  const int32_t *ptr;      // n+1
+  const int32_t *src;      // 3 per tap
+  const double *coef;
+};
+
+// momentum (k_momentum.hip)
+void launch_sgs(const Geo &g, const double *U, const double *V, const double *W, double *Visc, double nu,
+                double c_smag, double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial,
+                int cfl_blocks, hipStream_t st);
+void launch_wm_fluxes(const Geo &g, const double *U, const double *V, const double *W, WMPoints wm, double nu,
+                      int variable_z, hipStream_t st);
+void launch_momentum(const Geo &g, int comp, const double *U, const double *V, const double *W, const double *Visc,
+                     const double *Pr, double *Ustar, double *U2, const uint8_t *code, StageConst sc,
+                     hipStream_t st);
+void launch_momentum_corr(const Geo &g, int comp, const double *U, const double *V, const double *W,
+                          const double *Visc, const double *Pr, double *Ustar, double *U2, const uint8_t *code,
+                          StageConst sc, TermList tl, const double *fluxp, hipStream_t st);
+void launch_ib_interp(const Geo &g, IBDev ib, const double *F, double *tmp, hipStream_t st);
+void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, double *F, hipStream_t st);
+void launch_ib_visc_serial(const Geo &g, IBDev ib, double *Visc, const int32_t *level_ptr, int nlevels,
+                           hipStream_t st);
+void launch_null(const Geo &g, int n, const int32_t *ijk, double *F, hipStream_t st);
+void launch_mass_sources(const Geo &g, TermList tl, const double *U, const double *V, const double *W, double *Q,
+                         hipStream_t st);
+
+// scalar (k_scalar.hip)
+void launch_wm_div(const Geo &g, WMScalarPoints p, const int32_t *prtype, const double *U, const double *V,
+                   const double *W, hipStream_t st);
+void launch_scalar(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
+                   const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
+                   const uint8_t *my, const uint8_t *mz, StageConst sc, hipStream_t st);
+void launch_scalar_corr(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
+                        const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
+                        const uint8_t *my, const uint8_t *mz, StageConst sc, TermList tl, const double *src_flux,
+                        const double *wm_div, hipStream_t st);
+void launch_tdiff_init(const Geo &g, const double *Visc, double *TDiff, double nu, double nu_scalar,
+                       hipStream_t st);
+
+// pressure (k_pressure.hip)
+struct PoissonPlan {
+  int nx, ny, nz, M;            // M = nx/2+1
+  double2 *X;                   // spectral work array M*ny*nz
+  const double2 *twx, *twy;     // exp(-2 pi i t/N), t < N/2
+  const double *lamx, *lamy;    // eigenvalues of the periodic directions
+  const double *az, *cz;        // z coupling coefficients per k (0-based)
+  const double *dzPr, *dzW;     // grid spacings indexed like the reference (k = 1..nz)
+};
+void launch_poisson_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
+                        const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st);
+void launch_poisson_solve_z(const Geo &g, const PoissonPlan &pp, hipStream_t st);
+void launch_poisson_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st);
+void launch_top_sum(const Geo &g, const double *Pr, const double *Phi, double *partial, int nblocks,
+                    hipStream_t st);
+void launch_projection(const Geo &g, double *U, double *V, double *W, double *Pr, const double *Phi,
+                       const double *partial, int nblocks, double dt2, hipStream_t st);
+void launch_max_div(const Geo &g, const double *U, const double *V, const double *W, const double *Q,
+                    double *partial, int nblocks, hipStream_t st);
+void launch_residual(const Geo &g, const double *Phi, const double *RHS, double *partial, int nblocks,
+                     hipStream_t st);
+
+}  // namespace elmm

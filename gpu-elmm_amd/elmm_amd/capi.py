@@ -1,0 +1,277 @@
+"""ctypes binding of the C ABI in include/elmm_gpu.h.
+
+This is the Python host-side mirror of the reference's time-step interface:
+``Solver.step`` plays the role of ``TMarchRK3`` (src/tsteps.f90:24-243),
+``Solver.init`` the initial-condition tail (src/initial.f90:4284-4342) and
+``Case`` the list set-up of InitBoundaryConditions (src/initial.f90:
+4507-4681).  Arrays cross the boundary in Fortran order with the lower bound
+-2 in every direction (the reference's U/V/W/Scalar allocation).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libelmm_hip.so")
+
+# sides and boundary codes (src/parameters.f90:233, 260-261)
+WE, EA, SO, NO, BO, TO = range(6)
+NOSLIP, FREESLIP, PERIODIC, DIRICHLET, NEUMANN = 1, 2, 3, 4, 5
+FP64, FP32_REFINED = 0, 1
+POISSON_FFT, POISSON_MG = 2, 3
+F_U, F_V, F_W, F_PR, F_VISC, F_TDIFF, F_PHI, F_Q, F_SCALAR = 0, 1, 2, 3, 4, 5, 6, 7, 16
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+        ("stretched_z", C.c_int32),
+        ("zw", C.POINTER(C.c_double)),
+        ("btype", C.c_int32 * 6),
+        ("z0_wall", C.c_double * 6),
+        ("scal_btype", C.c_int32 * 6),
+        ("side_scal", C.c_double * 6),
+        ("nu", C.c_double), ("nu_scalar", C.c_double), ("c_smag", C.c_double),
+        ("filter_ratio", C.c_double), ("dt", C.c_double),
+        ("pr_gradient_x", C.c_double), ("pr_gradient_y", C.c_double),
+        ("advection_method", C.c_int32), ("sgs_model", C.c_int32), ("wall_model", C.c_int32),
+        ("nscalars", C.c_int32), ("poisson_solver", C.c_int32),
+        ("poisson_tol", C.c_double),
+        ("cfl_check", C.c_int32),
+        ("cfl_max", C.c_double),
+        ("precision", C.c_int32), ("ib_mass_sources", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+_pi32 = C.POINTER(C.c_int32)
+_pu8 = C.POINTER(C.c_uint8)
+_pd = C.POINTER(C.c_double)
+
+
+class Lists(C.Structure):
+    _fields_ = [
+        ("prtype", _pi32), ("utype", _pi32), ("vtype", _pi32), ("wtype", _pi32),
+        ("scfl_mask", _pu8 * 3),
+        ("wm_n", C.c_int32 * 18), ("wm_ijk", _pi32 * 18), ("wm_dist", _pd * 18), ("wm_z0", _pd * 18),
+        ("wmp_n", C.c_int32), ("wmp_ijk", _pi32), ("wmp_dist", _pd), ("wmp_z0", _pd),
+        ("scfl_n", C.c_int32 * 3), ("scfl_ijk", _pi32 * 3),
+        ("ib_n", C.c_int32 * 4), ("ib_ijk", _pi32 * 4), ("ib_ptr", _pi32 * 4),
+        ("ib_src_ijk", _pi32 * 4), ("ib_coef", _pd * 4),
+        ("null_n", C.c_int32 * 3), ("null_ijk", _pi32 * 3),
+        ("src_n", C.c_int32), ("src_ijk", _pi32), ("src_scalar", _pi32), ("src_flux", _pd),
+    ]
+
+
+class Diag(C.Structure):
+    _fields_ = [("delta", C.c_double), ("cfl", C.c_double), ("max_div", C.c_double),
+                ("poisson_res", C.c_double), ("refinements", C.c_int32), ("steps_done", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+class Cuboid(C.Structure):
+    _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
+                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double)]
+
+
+class PointSource(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("flux", C.c_double),
+                ("scalar", C.c_int32), ("pad", C.c_int32)]
+
+
+# Every symbol the header declares, with its ctypes signature.
+EXPORTS = {
+    "elmm_case_build": (C.c_int, [C.POINTER(Config), C.c_int32, C.POINTER(Cuboid), C.c_int32,
+                                  C.POINTER(PointSource), C.POINTER(C.c_void_p)]),
+    "elmm_case_view": (C.c_int, [C.c_void_p, C.POINTER(Lists)]),
+    "elmm_case_free": (None, [C.c_void_p]),
+    "elmm_create": (C.c_int, [C.POINTER(Config), C.POINTER(Lists), C.POINTER(C.c_void_p)]),
+    "elmm_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _pd]),
+    "elmm_download": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _pd]),
+    "elmm_init": (C.c_int, [C.c_void_p]),
+    "elmm_step": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Diag)]),
+    "elmm_step_async": (C.c_int, [C.c_void_p, C.c_int32]),
+    "elmm_wait": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
+    "elmm_divergence": (C.c_int, [C.c_void_p, _pd]),
+    "elmm_destroy": (C.c_int, [C.c_void_p]),
+    "elmm_version": (C.c_char_p, []),
+    "elmm_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+class ElmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"elmm error {code}: {msg}")
+        self.code = code
+
+
+def load(path=LIB_PATH):
+    """Load libelmm_hip.so; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not found: build the HIP library first (python __graft_entry__.py build)")
+    lib = C.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise ElmmError(rc, _lib.elmm_last_error().decode())
+
+
+def make_config(**kw):
+    c = Config()
+    c.filter_ratio = 1.0
+    c.c_smag = 0.122
+    c.advection_method = 4
+    c.sgs_model = 1
+    c.wall_model = 1
+    c.poisson_solver = POISSON_FFT
+    c.poisson_tol = 1e-10
+    c.ib_mass_sources = 1
+    zw = kw.pop("zw", None)
+    for k, v in kw.items():
+        if isinstance(getattr(c, k), C.Array):
+            arr = getattr(c, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(c, k, v)
+    if zw is not None:
+        zw = np.ascontiguousarray(zw, dtype=np.float64)
+        c._zw_keep = zw
+        c.zw = zw.ctypes.data_as(_pd)
+        c.stretched_z = 1
+    return c
+
+
+def _np(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Case:
+    """Masks and point lists of one case (elmm_case_build)."""
+
+    def __init__(self, cfg, cuboids=(), sources=()):
+        lib = load()
+        self.cfg = cfg
+        cub = (Cuboid * max(1, len(cuboids)))(*[Cuboid(*c) for c in cuboids])
+        src = (PointSource * max(1, len(sources)))(
+            *[PointSource(s[0], s[1], s[2], s[3], int(s[4]) if len(s) > 4 else 0, 0) for s in sources])
+        h = C.c_void_p()
+        _check(lib.elmm_case_build(C.byref(cfg), len(cuboids), cub, len(sources), src, C.byref(h)))
+        self._h = h
+        self.view = Lists()
+        _check(lib.elmm_case_view(h, C.byref(self.view)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.elmm_case_free(self._h)
+            self._h = None
+
+    def shape(self):
+        return (self.cfg.nx + 6, self.cfg.ny + 6, self.cfg.nz + 6)
+
+    def masks(self):
+        """Prtype, Utype, Vtype, Wtype as (nx+6, ny+6, nz+6) Fortran arrays."""
+        n = int(np.prod(self.shape()))
+        out = []
+        for name in ("prtype", "utype", "vtype", "wtype"):
+            a = _np(getattr(self.view, name), n, np.int32)
+            out.append(a.reshape(self.shape(), order="F"))
+        return out
+
+    def wm_list(self, l):
+        v = self.view
+        n = v.wm_n[l]
+        return (_np(v.wm_ijk[l], 3 * n, np.int32).reshape(n, 3), _np(v.wm_dist[l], 3 * n, np.float64).reshape(n, 3),
+                _np(v.wm_z0[l], n, np.float64))
+
+    def wmp_list(self):
+        v = self.view
+        n = v.wmp_n
+        return (_np(v.wmp_ijk, 3 * n, np.int32).reshape(n, 3), _np(v.wmp_dist, 3 * n, np.float64).reshape(n, 3),
+                _np(v.wmp_z0, n, np.float64))
+
+    def ib_list(self, c):
+        v = self.view
+        n = v.ib_n[c]
+        ptr = _np(v.ib_ptr[c], n + 1, np.int32) if n else np.zeros(1, np.int32)
+        nt = int(ptr[-1])
+        return (_np(v.ib_ijk[c], 3 * n, np.int32).reshape(n, 3), ptr,
+                _np(v.ib_src_ijk[c], 3 * nt, np.int32).reshape(nt, 3), _np(v.ib_coef[c], nt, np.float64))
+
+    def counts(self):
+        v = self.view
+        return {"wm": [v.wm_n[l] for l in range(18)], "wmp": v.wmp_n, "scfl": [v.scfl_n[d] for d in range(3)],
+                "ib": [v.ib_n[c] for c in range(4)], "null": [v.null_n[c] for c in range(3)], "src": v.src_n}
+
+
+def ghosted_shape(cfg):
+    return (cfg.nx + 6, cfg.ny + 6, cfg.nz + 6)
+
+
+class Solver:
+    """Device-resident LES case (elmm_create ... elmm_destroy)."""
+
+    def __init__(self, cfg, case):
+        lib = load()
+        self.cfg = cfg
+        self.case = case
+        h = C.c_void_p()
+        _check(lib.elmm_create(C.byref(cfg), C.byref(case.view), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.elmm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, field, arr, lb=-2):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        ext = (C.c_int32 * 3)(*a.shape)
+        _check(_lib.elmm_upload(self._h, field, lb, ext, a.ctypes.data_as(_pd)))
+
+    def download(self, field, lb=-2, shape=None):
+        shape = shape or ghosted_shape(self.cfg)
+        a = np.zeros(shape, dtype=np.float64, order="F")
+        ext = (C.c_int32 * 3)(*shape)
+        _check(_lib.elmm_download(self._h, field, lb, ext, a.ctypes.data_as(_pd)))
+        return a
+
+    def init(self):
+        _check(_lib.elmm_init(self._h))
+
+    def step(self, n=1, diag=True):
+        d = Diag()
+        _check(_lib.elmm_step(self._h, n, C.byref(d) if diag else None))
+        return d
+
+    def step_async(self, n=1):
+        _check(_lib.elmm_step_async(self._h, n))
+
+    def wait(self):
+        _check(_lib.elmm_wait(self._h, None))
+
+    def divergence(self):
+        x = C.c_double()
+        _check(_lib.elmm_divergence(self._h, C.byref(x)))
+        return x.value

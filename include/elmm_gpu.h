@@ -1,0 +1,227 @@
+/*
+ * elmm_gpu.h - C ABI of the MI355X-native ELMM LES time step.
+ *
+ * The library advances ELMM's staggered-grid LES state on one GPU.  The
+ * entry points replace the reference's CPU dynamics loop:
+ *
+ *   elmm_step      <-> TMarchRK3 (src/tsteps.f90:24-243) called from the
+ *                      time loop in src/main.f90:112-128
+ *   elmm_init      <-> the initial-condition tail of InitialConditions
+ *                      (src/initial.f90:4284-4342)
+ *   elmm_case_*    <-> the list/mask set-up of InitBoundaryConditions
+ *                      (src/initial.f90:4507-4681, src/wallmodels.f90:449-1113,
+ *                       src/immersedboundary.f90:1475-1768)
+ *   elmm_upload /  <-> the Fortran allocatables U,V,W,Pr,Scalar,Viscosity,TDiff
+ *   elmm_download      owned by the main program (src/main.f90:229-285)
+ *
+ * All functions return 0 on success and a negative ELMM_E* code on error;
+ * elmm_last_error() returns the message of the last failure on the calling
+ * thread.  Arrays cross the boundary in Fortran order (x fastest), with the
+ * lower bound and extents stated by the caller, so a Fortran host passes its
+ * own allocatables through iso_c_binding without copies of its own.
+ */
+#ifndef ELMM_GPU_H
+#define ELMM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sides, 0-based (src/parameters.f90:233: We=1, Ea=2, So=3, No=4, Bo=5, To=6). */
+enum { ELMM_WE = 0, ELMM_EA = 1, ELMM_SO = 2, ELMM_NO = 3, ELMM_BO = 4, ELMM_TO = 5 };
+
+/* Boundary codes: identical numbers to src/parameters.f90:260-261 BC_*. */
+enum {
+  ELMM_BC_NOSLIP = 1,
+  ELMM_BC_FREESLIP = 2,
+  ELMM_BC_PERIODIC = 3,
+  ELMM_BC_DIRICHLET = 4,
+  ELMM_BC_NEUMANN = 5
+};
+
+/* Precision forms of the step. */
+enum {
+  ELMM_FP64 = 0,          /* every field and every operation in FP64          */
+  ELMM_FP32_REFINED = 1   /* FP32 stencils, FP64 state/projection, FP32
+                             pressure solves inside FP64 iterative refinement */
+};
+
+/* Pressure solvers (src/pressure.f90:17-20 numbering). */
+enum { ELMM_POISSON_FFT = 2, ELMM_POISSON_MG = 3 };
+
+/* Field ids for upload/download. */
+enum {
+  ELMM_F_U = 0, ELMM_F_V = 1, ELMM_F_W = 2, ELMM_F_PR = 3,
+  ELMM_F_VISC = 4, ELMM_F_TDIFF = 5, ELMM_F_PHI = 6, ELMM_F_Q = 7,
+  ELMM_F_SCALAR = 16      /* + scalar index (0-based) */
+};
+
+/* Error codes. */
+enum {
+  ELMM_OK = 0,
+  ELMM_E_ARG = -1,        /* invalid argument or unsupported configuration   */
+  ELMM_E_HIP = -2,        /* HIP runtime failure                             */
+  ELMM_E_NOMEM = -3,      /* allocation failed                               */
+  ELMM_E_DIVERGED = -4,   /* NaN detected or CFL above the limit (main.f90:154-169) */
+  ELMM_E_STATE = -5       /* call out of order (e.g. step before init)       */
+};
+
+/*
+ * Case configuration: the subset of ELMM's namelists/.conf inputs that the
+ * time step consumes (src/parameters.f90:34-287, src/initial.f90:56-1193).
+ */
+typedef struct elmm_config {
+  int32_t nx, ny, nz;        /* Prnx, Prny, Prnz                              */
+  double  dx, dy, dz;        /* dxmin, dymin, dzmin (dz unused if stretched)  */
+  double  x0, y0, z0;        /* domain origin (im_xmin, im_ymin, im_zmin)     */
+  int32_t stretched_z;       /* 0 GRID_UNIFORM, 1 GRID_VARIABLE_Z             */
+  const double *zw;          /* zW(0:nz) when stretched_z (nz+1 values)       */
+  int32_t btype[6];          /* Btype: velocity boundary per side             */
+  double  z0_wall[6];        /* z0W..z0T roughness lengths                    */
+  int32_t scal_btype[6];     /* ScalBtype                                     */
+  double  side_scal[6];      /* sideScal (flux for Neumann, value Dirichlet)  */
+  double  nu;                /* molecular_viscosity                           */
+  double  nu_scalar;         /* molecular_diffusivity                         */
+  double  c_smag;            /* C_Smagorinsky                                 */
+  double  filter_ratio;      /* filter_ratios(filtertype)                     */
+  double  dt;                /* time_stepping%dt_constant                     */
+  double  pr_gradient_x;     /* uniform mean pressure gradient, 0 = off       */
+  double  pr_gradient_y;
+  int32_t advection_method;  /* 4: CD4div (Morinishi), 2: CD2 variable z      */
+  int32_t sgs_model;         /* 0 none, 1 Smagorinsky                         */
+  int32_t wall_model;        /* wallmodeltype: 0 off, 1 log law               */
+  int32_t nscalars;          /* num_of_scalars (passive)                      */
+  int32_t poisson_solver;    /* ELMM_POISSON_FFT / ELMM_POISSON_MG            */
+  double  poisson_tol;       /* residual target of iterative solves           */
+  int32_t cfl_check;         /* time_stepping%enable_CFL_check                */
+  double  cfl_max;           /* CFL_max, 0 = no check                         */
+  int32_t precision;         /* ELMM_FP64 / ELMM_FP32_REFINED                 */
+  int32_t ib_mass_sources;   /* enable_ibm_mass_sources                       */
+  int32_t reserved[8];
+} elmm_config;
+
+/*
+ * Masks and integer point lists consumed by the step (read-only view).
+ * 3D masks are full ghosted arrays (lower bound -2, extents n+6 in each
+ * direction, Fortran order).  Points are 1-based (i,j,k) triples as in the
+ * reference.  Momentum wall-model lists are indexed [component*6 + dir] with
+ * component 0=U,1=V,2=W and dir 0..5 = xm,xp,ym,yp,zm,zp
+ * (src/wallmodels.f90:146-149).
+ */
+typedef struct elmm_lists {
+  const int32_t *prtype, *utype, *vtype, *wtype;  /* Prtype/Utype/Vtype/Wtype */
+  const uint8_t *scfl_mask[3];                    /* Scfl{x,y,z}_mask, 1=flux */
+
+  int32_t        wm_n[18];
+  const int32_t *wm_ijk[18];     /* 3 per point                               */
+  const double  *wm_dist[18];    /* distx,disty,distz per point               */
+  const double  *wm_z0[18];
+
+  int32_t        wmp_n;          /* WMPoints (scalar points)                  */
+  const int32_t *wmp_ijk;
+  const double  *wmp_dist;
+  const double  *wmp_z0;
+
+  int32_t        scfl_n[3];      /* Scfl{x,y,z}_points                        */
+  const int32_t *scfl_ijk[3];
+
+  int32_t        ib_n[4];        /* U,V,W,ScalFl IB points (CSR)              */
+  const int32_t *ib_ijk[4];
+  const int32_t *ib_ptr[4];      /* n+1 offsets into src/coef                 */
+  const int32_t *ib_src_ijk[4];
+  const double  *ib_coef[4];
+
+  int32_t        null_n[3];      /* Unull, Vnull, Wnull                       */
+  const int32_t *null_ijk[3];
+
+  int32_t        src_n;          /* scalar volume (point) sources             */
+  const int32_t *src_ijk;
+  const int32_t *src_scalar;     /* 0-based scalar index                      */
+  const double  *src_flux;       /* flux / cell volume                        */
+} elmm_lists;
+
+/* Per-call diagnostics of elmm_step. */
+typedef struct elmm_diag {
+  double  delta;          /* sum of mean |U-U2| over components / dt (tsteps.f90:188-209) */
+  double  cfl;            /* CFL of the last step when cfl_check           */
+  double  max_div;        /* max |div U| after the last projection         */
+  double  poisson_res;    /* max residual of the last pressure solve       */
+  int32_t refinements;    /* refinement sweeps of the last solve           */
+  int32_t steps_done;
+  int32_t reserved[6];
+} elmm_diag;
+
+typedef struct elmm_case elmm_case;
+typedef struct elmm_handle elmm_handle;
+
+/* Axis-aligned solid cuboid, metres: xmin,xmax,ymin,ymax,zmin,zmax. */
+typedef struct elmm_cuboid { double xmin, xmax, ymin, ymax, zmin, zmax; } elmm_cuboid;
+
+/* Scalar point source: position (m), flux (units/s), scalar index. */
+typedef struct elmm_point_source { double x, y, z, flux; int32_t scalar; int32_t pad; } elmm_point_source;
+
+/* ---------------------------------------------------------------------------
+ * Case set-up (host only, no GPU needed).
+ * Builds Prtype/Utype/Vtype/Wtype, the wall-model lists, the scalar
+ * wall-face lists, the IB interpolation lists, the nullified points and the
+ * point sources for the given configuration and solids.
+ *   replaces: InitBoundaryConditions obstacle/list section
+ *             (src/initial.f90:4646-4705) with GetSolidBodiesBC
+ *             (src/immersedboundary.f90:1750-1768), GetOutsideBoundariesWM
+ *             (src/wallmodels.f90:449-904), InitWMMasks (911-1113),
+ *             SetNullifiedPoints (src/initial.f90:4861-4962),
+ *             InitPointSources (src/pointsources.f90:44-66)
+ * ------------------------------------------------------------------------- */
+int  elmm_case_build(const elmm_config *cfg,
+                     int32_t n_cuboids, const elmm_cuboid *cuboids,
+                     int32_t n_sources, const elmm_point_source *sources,
+                     elmm_case **out);
+int  elmm_case_view(const elmm_case *c, elmm_lists *view);
+void elmm_case_free(elmm_case *c);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident solver.
+ * ------------------------------------------------------------------------- */
+
+/* Allocate the device state of one case and copy masks/lists to HBM.
+ * replaces: AllocateGlobals (src/main.f90:229-285) and the first-call
+ * allocations of TMarchRK3 (src/tsteps.f90:54-77). */
+int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm_handle **out);
+
+/* Copy one field between a host array and HBM.  The host array has lower
+ * bound lb in every direction and extents ext[3]; the overlap with the
+ * field's ghosted device extent (-2..n+3) is copied. */
+int elmm_upload(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], const double *src);
+int elmm_download(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], double *dst);
+
+/* Initial-condition tail: ghost fills, initial projection with coef=1,
+ * SGS viscosity, frozen TDiff, wall-model seed.
+ * replaces: src/initial.f90:4284-4342 */
+int elmm_init(elmm_handle *h);
+
+/* Advance nsteps RK3 steps with constant dt; synchronous at return.
+ * replaces: TMarchRK3 (src/tsteps.f90:24-243) + main loop time update
+ * (src/main.f90:112-128). diag may be NULL. */
+int elmm_step(elmm_handle *h, int32_t nsteps, elmm_diag *diag);
+
+/* Enqueue nsteps without waiting (device stream order); elmm_wait joins. */
+int elmm_step_async(elmm_handle *h, int32_t nsteps);
+int elmm_wait(elmm_handle *h, elmm_diag *diag);
+
+/* Maximum |div U| of the current state (PostPoisson check_divergence,
+ * src/pressure.f90:884-989), computed in FP64 on the device. */
+int elmm_divergence(elmm_handle *h, double *max_div);
+
+int elmm_destroy(elmm_handle *h);
+
+/* Version and last error of the calling thread. */
+const char *elmm_version(void);
+const char *elmm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ELMM_GPU_H */

@@ -1,0 +1,2093 @@
+/*
+ * elmm_oracle.c - CPU FP64 restatement of ELMM's TMarchRK3 step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see elmm_oracle.h).  Compile with
+ *   gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math
+ * so the arithmetic is the reference's, operation by operation.
+ *
+ * Storage: every 3D field uses the index space -2..n+3 in each direction
+ * (n = Prnx/Prny/Prnz), Fortran order.  This contains each reference array's
+ * own bounds (U: -2:Unx+3, Pr/Phi/Viscosity: -1:Prn+2, RHS/Q: 0:Prn+1).
+ */
+#include "elmm_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static char g_err[512];
+static void set_err(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+const char *oracle_last_error(void) { return g_err; }
+
+/* RK3 coefficients, src/parameters.f90:312-322. */
+static const double RK_alpha[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+static const double RK_beta[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double RK_rho[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+enum { We = 0, Ea = 1, So = 2, No = 3, Bo = 4, To = 5 };
+enum { NOSLIP = 1, FREESLIP = 2, PERIODIC = 3, DIRICHLET = 4, NEUMANN = 5 };
+
+typedef struct {
+  int n;
+  int32_t *ijk;
+  double *dist, *z0, *ustar, *fluxp;
+} wm_list;
+
+struct oracle {
+  elmm_config c;
+  int nx, ny, nz;
+  int Un[3], Vn[3], Wn[3];
+  long sy, sz, ntot;
+  int variable_z;
+  double dxmin, dymin, dzmin;
+  /* 1-D coordinate arrays, stored with offset 3 (index -3 at position 0) */
+  double *xU, *yV, *zW, *xPr, *yPr, *zPr, *dxPr, *dyPr, *dzPr, *dxU, *dyV, *dzW;
+  double *U, *V, *W, *U2, *V2, *W2, *Us, *Vs, *Ws;
+  double *Pr, *Visc, *TDiff, *Q, *Phi, *RHS;
+  double **S, **S2, **Sadv;
+  int32_t *Prtype, *Utype, *Vtype, *Wtype;
+  uint8_t *scflmask[3];
+  wm_list wm[18];
+  wm_list wmp;
+  double *wmp_div;
+  int scfl_n[3];
+  int32_t *scfl_ijk[3];
+  int ib_n[4];
+  int32_t *ib_ijk[4], *ib_ptr[4], *ib_src[4];
+  double *ib_coef[4];
+  int null_n[3];
+  int32_t *null_ijk[3];
+  int src_n;
+  int32_t *src_ijk, *src_sc;
+  double *src_flux;
+  int initialized, first_step_done;
+  double last_res, last_div;
+};
+
+#define C1D(a, i) ((a)[(i) + 3])
+#define IX(o, i, j, k) ((long)((i) + 2) + (long)((j) + 2) * (o)->sy + (long)((k) + 2) * (o)->sz)
+#define A3(o, a, i, j, k) ((a)[IX(o, i, j, k)])
+
+static void *xcalloc(size_t n, size_t s) {
+  void *p = calloc(n ? n : 1, s);
+  if (!p) {
+    fprintf(stderr, "oracle: out of memory\n");
+    abort();
+  }
+  return p;
+}
+
+/* ---------------------------------------------------------------- grid --- */
+/* Coordinates as in InitBoundaryConditions, src/initial.f90:4395-4502. */
+static void init_grid(oracle *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  o->xU = xcalloc(nx + 8, sizeof(double));
+  o->yV = xcalloc(ny + 8, sizeof(double));
+  o->zW = xcalloc(nz + 8, sizeof(double));
+  o->xPr = xcalloc(nx + 8, sizeof(double));
+  o->yPr = xcalloc(ny + 8, sizeof(double));
+  o->zPr = xcalloc(nz + 8, sizeof(double));
+  o->dxPr = xcalloc(nx + 8, sizeof(double));
+  o->dyPr = xcalloc(ny + 8, sizeof(double));
+  o->dzPr = xcalloc(nz + 8, sizeof(double));
+  o->dxU = xcalloc(nx + 8, sizeof(double));
+  o->dyV = xcalloc(ny + 8, sizeof(double));
+  o->dzW = xcalloc(nz + 8, sizeof(double));
+  o->dxmin = o->c.dx;
+  o->dymin = o->c.dy;
+  for (int i = -3; i <= nx + 3; i++) C1D(o->xU, i) = i * o->dxmin + o->c.x0;
+  for (int j = -3; j <= ny + 3; j++) C1D(o->yV, j) = j * o->dymin + o->c.y0;
+  if (o->variable_z) {
+    for (int k = 0; k <= nz; k++) C1D(o->zW, k) = o->c.zw[k];
+    /* bottom extension (src/initial.f90:4418-4421 for a non-periodic bottom) */
+    if (o->c.btype[Bo] == PERIODIC) {
+      for (int j = -1; j >= -3; j--)
+        C1D(o->zW, j) = C1D(o->zW, 0) - (C1D(o->zW, nz - 1) - C1D(o->zW, nz - 1 + j));
+    } else {
+      for (int j = -1; j >= -3; j--) C1D(o->zW, j) = C1D(o->zW, 0) - (C1D(o->zW, -j) - C1D(o->zW, 0));
+    }
+    if (o->c.btype[To] == PERIODIC) {
+      for (int j = nz + 1; j <= nz + 3; j++)
+        C1D(o->zW, j) = C1D(o->zW, nz - 1) + (C1D(o->zW, j - nz + 1) - C1D(o->zW, 0));
+    } else {
+      for (int j = nz + 1; j <= nz + 3; j++)
+        C1D(o->zW, j) = C1D(o->zW, nz - 1) + (C1D(o->zW, nz - 1) - C1D(o->zW, nz - 1 - (j - nz + 1)));
+    }
+  } else {
+    o->dzmin = o->c.dz;
+    for (int k = -3; k <= nz + 3; k++) C1D(o->zW, k) = k * o->dzmin + o->c.z0;
+  }
+  for (int i = -2; i <= nx + 3; i++) {
+    C1D(o->xPr, i) = (C1D(o->xU, i - 1) + C1D(o->xU, i)) / 2.0;
+    C1D(o->dxPr, i) = C1D(o->xU, i) - C1D(o->xU, i - 1);
+  }
+  for (int j = -2; j <= ny + 3; j++) {
+    C1D(o->yPr, j) = (C1D(o->yV, j - 1) + C1D(o->yV, j)) / 2.0;
+    C1D(o->dyPr, j) = C1D(o->yV, j) - C1D(o->yV, j - 1);
+  }
+  for (int k = -2; k <= nz + 3; k++) {
+    C1D(o->zPr, k) = (C1D(o->zW, k - 1) + C1D(o->zW, k)) / 2.0;
+    C1D(o->dzPr, k) = C1D(o->zW, k) - C1D(o->zW, k - 1);
+  }
+  for (int i = -2; i <= nx + 2; i++) C1D(o->dxU, i) = C1D(o->xPr, i + 1) - C1D(o->xPr, i);
+  for (int j = -2; j <= ny + 2; j++) C1D(o->dyV, j) = C1D(o->yPr, j + 1) - C1D(o->yPr, j);
+  for (int k = -2; k <= nz + 2; k++) C1D(o->dzW, k) = C1D(o->zPr, k + 1) - C1D(o->zPr, k);
+  if (o->variable_z) {
+    double m = C1D(o->dzPr, -2);
+    for (int k = -2; k <= nz + 3; k++)
+      if (C1D(o->dzPr, k) < m) m = C1D(o->dzPr, k);
+    o->dzmin = m;
+  }
+}
+
+/* ------------------------------------------------------------ creation --- */
+static double *field(oracle *o) { return xcalloc(o->ntot, sizeof(double)); }
+
+static void copy_wm(wm_list *d, int n, const int32_t *ijk, const double *dist, const double *z0) {
+  d->n = n;
+  d->ijk = xcalloc(3 * (size_t)n, sizeof(int32_t));
+  d->dist = xcalloc(3 * (size_t)n, sizeof(double));
+  d->z0 = xcalloc((size_t)n, sizeof(double));
+  d->ustar = xcalloc((size_t)n, sizeof(double));
+  d->fluxp = xcalloc((size_t)n, sizeof(double));
+  if (n) {
+    memcpy(d->ijk, ijk, 3 * (size_t)n * sizeof(int32_t));
+    memcpy(d->dist, dist, 3 * (size_t)n * sizeof(double));
+    memcpy(d->z0, z0, (size_t)n * sizeof(double));
+  }
+  for (int p = 0; p < n; p++) d->ustar[p] = 1.0; /* WMpointUVW%ustar = 1 (wallmodels.f90:79) */
+}
+
+oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
+  if (!cfg || !L) {
+    set_err("null argument");
+    return NULL;
+  }
+  if (cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4) {
+    set_err("grid too small");
+    return NULL;
+  }
+  for (int s = 0; s < 6; s++) {
+    int b = cfg->btype[s];
+    if (b != NOSLIP && b != FREESLIP && b != PERIODIC) {
+      set_err("velocity boundary type %d unsupported on side %d", b, s);
+      return NULL;
+    }
+  }
+  if ((cfg->btype[We] == PERIODIC) != (cfg->btype[Ea] == PERIODIC) ||
+      (cfg->btype[So] == PERIODIC) != (cfg->btype[No] == PERIODIC) ||
+      (cfg->btype[Bo] == PERIODIC) != (cfg->btype[To] == PERIODIC)) {
+    set_err("periodic boundaries must be paired");
+    return NULL;
+  }
+  if (cfg->btype[We] != PERIODIC || cfg->btype[So] != PERIODIC || cfg->btype[Bo] == PERIODIC) {
+    set_err("pressure solver supports periodic x/y with walls in z");
+    return NULL;
+  }
+  oracle *o = xcalloc(1, sizeof(oracle));
+  o->c = *cfg;
+  o->nx = cfg->nx;
+  o->ny = cfg->ny;
+  o->nz = cfg->nz;
+  o->variable_z = cfg->stretched_z != 0;
+  if (o->variable_z) {
+    double *zw = xcalloc(cfg->nz + 1, sizeof(double));
+    memcpy(zw, cfg->zw, (cfg->nz + 1) * sizeof(double));
+    o->c.zw = zw;
+  }
+  /* extents, src/initial.f90:871-896 */
+  o->Un[0] = cfg->btype[Ea] == PERIODIC ? o->nx : o->nx - 1;
+  o->Un[1] = o->ny;
+  o->Un[2] = o->nz;
+  o->Vn[0] = o->nx;
+  o->Vn[1] = cfg->btype[No] == PERIODIC ? o->ny : o->ny - 1;
+  o->Vn[2] = o->nz;
+  o->Wn[0] = o->nx;
+  o->Wn[1] = o->ny;
+  o->Wn[2] = cfg->btype[To] == PERIODIC ? o->nz : o->nz - 1;
+  o->sy = o->nx + 6;
+  o->sz = (long)(o->nx + 6) * (o->ny + 6);
+  o->ntot = o->sz * (o->nz + 6);
+  init_grid(o);
+
+  o->U = field(o);
+  o->V = field(o);
+  o->W = field(o);
+  o->U2 = field(o);
+  o->V2 = field(o);
+  o->W2 = field(o);
+  o->Us = field(o);
+  o->Vs = field(o);
+  o->Ws = field(o);
+  o->Pr = field(o);
+  o->Visc = field(o);
+  o->TDiff = field(o);
+  o->Q = field(o);
+  o->Phi = field(o);
+  o->RHS = field(o);
+  int ns = cfg->nscalars;
+  o->S = xcalloc(ns + 1, sizeof(double *));
+  o->S2 = xcalloc(ns + 1, sizeof(double *));
+  o->Sadv = xcalloc(ns + 1, sizeof(double *));
+  for (int s = 0; s < ns; s++) {
+    o->S[s] = field(o);
+    o->S2[s] = field(o);
+    o->Sadv[s] = field(o);
+  }
+  o->Prtype = xcalloc(o->ntot, sizeof(int32_t));
+  o->Utype = xcalloc(o->ntot, sizeof(int32_t));
+  o->Vtype = xcalloc(o->ntot, sizeof(int32_t));
+  o->Wtype = xcalloc(o->ntot, sizeof(int32_t));
+  memcpy(o->Prtype, L->prtype, o->ntot * sizeof(int32_t));
+  memcpy(o->Utype, L->utype, o->ntot * sizeof(int32_t));
+  memcpy(o->Vtype, L->vtype, o->ntot * sizeof(int32_t));
+  memcpy(o->Wtype, L->wtype, o->ntot * sizeof(int32_t));
+  for (int d = 0; d < 3; d++) {
+    o->scflmask[d] = xcalloc(o->ntot, 1);
+    memcpy(o->scflmask[d], L->scfl_mask[d], o->ntot);
+  }
+  for (int l = 0; l < 18; l++) copy_wm(&o->wm[l], L->wm_n[l], L->wm_ijk[l], L->wm_dist[l], L->wm_z0[l]);
+  copy_wm(&o->wmp, L->wmp_n, L->wmp_ijk, L->wmp_dist, L->wmp_z0);
+  o->wmp_div = xcalloc(L->wmp_n + 1, sizeof(double));
+  for (int d = 0; d < 3; d++) {
+    o->scfl_n[d] = L->scfl_n[d];
+    o->scfl_ijk[d] = xcalloc(3 * (size_t)L->scfl_n[d] + 1, sizeof(int32_t));
+    if (L->scfl_n[d]) memcpy(o->scfl_ijk[d], L->scfl_ijk[d], 3 * (size_t)L->scfl_n[d] * sizeof(int32_t));
+  }
+  for (int c = 0; c < 4; c++) {
+    int n = L->ib_n[c];
+    o->ib_n[c] = n;
+    o->ib_ijk[c] = xcalloc(3 * (size_t)n + 1, sizeof(int32_t));
+    o->ib_ptr[c] = xcalloc((size_t)n + 1, sizeof(int32_t));
+    int nt = n ? L->ib_ptr[c][n] : 0;
+    o->ib_src[c] = xcalloc(3 * (size_t)nt + 1, sizeof(int32_t));
+    o->ib_coef[c] = xcalloc((size_t)nt + 1, sizeof(double));
+    if (n) {
+      memcpy(o->ib_ijk[c], L->ib_ijk[c], 3 * (size_t)n * sizeof(int32_t));
+      memcpy(o->ib_ptr[c], L->ib_ptr[c], ((size_t)n + 1) * sizeof(int32_t));
+      memcpy(o->ib_src[c], L->ib_src_ijk[c], 3 * (size_t)nt * sizeof(int32_t));
+      memcpy(o->ib_coef[c], L->ib_coef[c], (size_t)nt * sizeof(double));
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    o->null_n[c] = L->null_n[c];
+    o->null_ijk[c] = xcalloc(3 * (size_t)L->null_n[c] + 1, sizeof(int32_t));
+    if (L->null_n[c]) memcpy(o->null_ijk[c], L->null_ijk[c], 3 * (size_t)L->null_n[c] * sizeof(int32_t));
+  }
+  o->src_n = L->src_n;
+  o->src_ijk = xcalloc(3 * (size_t)L->src_n + 1, sizeof(int32_t));
+  o->src_sc = xcalloc((size_t)L->src_n + 1, sizeof(int32_t));
+  o->src_flux = xcalloc((size_t)L->src_n + 1, sizeof(double));
+  if (L->src_n) {
+    memcpy(o->src_ijk, L->src_ijk, 3 * (size_t)L->src_n * sizeof(int32_t));
+    memcpy(o->src_sc, L->src_scalar, (size_t)L->src_n * sizeof(int32_t));
+    memcpy(o->src_flux, L->src_flux, (size_t)L->src_n * sizeof(double));
+  }
+  return o;
+}
+
+static void free_wm(wm_list *l) {
+  free(l->ijk);
+  free(l->dist);
+  free(l->z0);
+  free(l->ustar);
+  free(l->fluxp);
+}
+
+void oracle_destroy(oracle *o) {
+  if (!o) return;
+  double *arrs[] = {o->xU, o->yV, o->zW, o->xPr, o->yPr, o->zPr, o->dxPr, o->dyPr, o->dzPr, o->dxU, o->dyV,
+                    o->dzW, o->U, o->V, o->W, o->U2, o->V2, o->W2, o->Us, o->Vs, o->Ws, o->Pr, o->Visc,
+                    o->TDiff, o->Q, o->Phi, o->RHS};
+  for (size_t i = 0; i < sizeof arrs / sizeof arrs[0]; i++) free(arrs[i]);
+  for (int s = 0; s < o->c.nscalars; s++) {
+    free(o->S[s]);
+    free(o->S2[s]);
+    free(o->Sadv[s]);
+  }
+  free(o->S);
+  free(o->S2);
+  free(o->Sadv);
+  free(o->Prtype);
+  free(o->Utype);
+  free(o->Vtype);
+  free(o->Wtype);
+  for (int d = 0; d < 3; d++) {
+    free(o->scflmask[d]);
+    free(o->scfl_ijk[d]);
+    free(o->null_ijk[d]);
+  }
+  for (int l = 0; l < 18; l++) free_wm(&o->wm[l]);
+  free_wm(&o->wmp);
+  free(o->wmp_div);
+  for (int c = 0; c < 4; c++) {
+    free(o->ib_ijk[c]);
+    free(o->ib_ptr[c]);
+    free(o->ib_src[c]);
+    free(o->ib_coef[c]);
+  }
+  free(o->src_ijk);
+  free(o->src_sc);
+  free(o->src_flux);
+  if (o->variable_z) free((void *)o->c.zw);
+  free(o);
+}
+
+static double *field_ptr(oracle *o, int32_t f) {
+  switch (f) {
+    case ELMM_F_U: return o->U;
+    case ELMM_F_V: return o->V;
+    case ELMM_F_W: return o->W;
+    case ELMM_F_PR: return o->Pr;
+    case ELMM_F_VISC: return o->Visc;
+    case ELMM_F_TDIFF: return o->TDiff;
+    case ELMM_F_PHI: return o->Phi;
+    case ELMM_F_Q: return o->Q;
+    case ORACLE_F_USTAR: return o->Us;
+    case ORACLE_F_VSTAR: return o->Vs;
+    case ORACLE_F_WSTAR: return o->Ws;
+    case ORACLE_F_RHS: return o->RHS;
+    default:
+      if (f >= ELMM_F_SCALAR && f < ELMM_F_SCALAR + o->c.nscalars) return o->S[f - ELMM_F_SCALAR];
+      if (f >= ORACLE_F_SADV && f < ORACLE_F_SADV + o->c.nscalars) return o->Sadv[f - ORACLE_F_SADV];
+  }
+  return NULL;
+}
+
+static int xfer(oracle *o, int32_t f, int32_t lb, const int32_t ext[3], double *host, int up) {
+  double *a = field_ptr(o, f);
+  if (!a) {
+    set_err("bad field %d", f);
+    return ELMM_E_ARG;
+  }
+  int n[3] = {o->nx, o->ny, o->nz};
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = lb > -2 ? lb : -2;
+    int top = lb + ext[d] - 1;
+    hi[d] = top < n[d] + 3 ? top : n[d] + 3;
+  }
+  long hsy = ext[0], hsz = (long)ext[0] * ext[1];
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        long h = (i - lb) + (j - lb) * hsy + (k - lb) * hsz;
+        if (up)
+          A3(o, a, i, j, k) = host[h];
+        else
+          host[h] = A3(o, a, i, j, k);
+      }
+  return 0;
+}
+int oracle_upload(oracle *o, int32_t f, int32_t lb, const int32_t ext[3], const double *src) {
+  return xfer(o, f, lb, ext, (double *)src, 1);
+}
+int oracle_download(oracle *o, int32_t f, int32_t lb, const int32_t ext[3], double *dst) {
+  return xfer(o, f, lb, ext, dst, 0);
+}
+
+/* ------------------------------------------------------- ghost fills --- */
+
+/* Fills of one velocity component along one axis; the reference's rules per
+ * boundary type (src/boundaries.f90:461-572 west, 783-894 east, and the same
+ * pattern for y 1122-1233/1323-1434 and z 1533-1647/1676-1789). */
+static void vel_low(double *a, long base, long st, int n, int bt, int normal) {
+#define E(m) a[base + (long)(m) * st]
+  if (bt == PERIODIC) {
+    E(0) = E(n);
+    E(-1) = E(n - 1);
+    E(-2) = E(n - 2);
+  } else if (normal) { /* noslip and freeslip: normal component */
+    E(0) = 0;
+    E(-1) = -E(1);
+    E(-2) = -E(2);
+  } else if (bt == NOSLIP) {
+    E(0) = -E(1);
+    E(-1) = -E(2);
+    E(-2) = -E(3);
+  } else { /* FREESLIP tangential */
+    E(0) = E(1);
+    E(-1) = E(1);
+    E(-2) = E(1);
+  }
+}
+static void vel_high(double *a, long base, long st, int n, int bt, int normal) {
+  if (bt == PERIODIC) {
+    E(n + 1) = E(1);
+    E(n + 2) = E(2);
+    E(n + 3) = E(3);
+  } else if (normal) {
+    E(n + 1) = 0;
+    E(n + 2) = -E(n);
+    E(n + 3) = -E(n - 1);
+  } else if (bt == NOSLIP) {
+    E(n + 1) = -E(n);
+    E(n + 2) = -E(n - 1);
+    E(n + 3) = -E(n - 2);
+  } else {
+    E(n + 1) = E(n);
+    E(n + 2) = E(n);
+    E(n + 3) = E(n);
+  }
+#undef E
+}
+
+/* periodic_corners_edges, src/boundaries.f90:189-394 */
+static void periodic_corners_edges(oracle *o, double *U, const int *n) {
+  int nx = n[0], ny = n[1], nz = n[2];
+  int px = o->c.btype[Ea] == PERIODIC, py = o->c.btype[No] == PERIODIC, pz = o->c.btype[To] == PERIODIC;
+  if (px && py && pz) {
+    for (int sk = 0; sk < 2; sk++)
+      for (int sj = 0; sj < 2; sj++)
+        for (int si = 0; si < 2; si++) {
+          int k0 = sk ? nz + 1 : -2, j0 = sj ? ny + 1 : -2, i0 = si ? nx + 1 : -2;
+          int dk = sk ? -nz : nz, dj = sj ? -ny : ny, di = si ? -nx : nx;
+          for (int k = k0; k < k0 + 3; k++)
+            for (int j = j0; j < j0 + 3; j++)
+              for (int i = i0; i < i0 + 3; i++) A3(o, U, i, j, k) = A3(o, U, i + di, j + dj, k + dk);
+        }
+  }
+  if (px && py) {
+    for (int sj = 0; sj < 2; sj++)
+      for (int si = 0; si < 2; si++) {
+        int j0 = sj ? ny + 1 : -2, i0 = si ? nx + 1 : -2;
+        int dj = sj ? -ny : ny, di = si ? -nx : nx;
+        for (int k = 1; k <= nz; k++)
+          for (int j = j0; j < j0 + 3; j++)
+            for (int i = i0; i < i0 + 3; i++) A3(o, U, i, j, k) = A3(o, U, i + di, j + dj, k);
+      }
+  }
+  if (px && pz) {
+    for (int sk = 0; sk < 2; sk++)
+      for (int si = 0; si < 2; si++) {
+        int k0 = sk ? nz + 1 : -2, i0 = si ? nx + 1 : -2;
+        int dk = sk ? -nz : nz, di = si ? -nx : nx;
+        for (int k = k0; k < k0 + 3; k++)
+          for (int j = 1; j <= ny; j++)
+            for (int i = i0; i < i0 + 3; i++) A3(o, U, i, j, k) = A3(o, U, i + di, j, k + dk);
+      }
+  }
+  if (py && pz) {
+    for (int sk = 0; sk < 2; sk++)
+      for (int sj = 0; sj < 2; sj++) {
+        int k0 = sk ? nz + 1 : -2, j0 = sj ? ny + 1 : -2;
+        int dk = sk ? -nz : nz, dj = sj ? -ny : ny;
+        for (int k = k0; k < k0 + 3; k++)
+          for (int j = j0; j < j0 + 3; j++)
+            for (int i = 1; i <= nx; i++) A3(o, U, i, j, k) = A3(o, U, i, j + dj, k + dk);
+      }
+  }
+}
+
+/* BoundUVW_do_BC, src/boundaries.f90:398-1791 (periodic, noslip, freeslip). */
+static void bound_uvw(oracle *o, double *U, double *V, double *W) {
+  double *F[3] = {U, V, W};
+  const int *N[3] = {o->Un, o->Vn, o->Wn};
+  for (int c = 0; c < 3; c++) periodic_corners_edges(o, F[c], N[c]);
+  /* x faces: j,k interior */
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    for (int k = 1; k <= n[2]; k++)
+      for (int j = 1; j <= n[1]; j++) {
+        long b = IX(o, 0, j, k);
+        vel_low(F[c], b, 1, n[0], o->c.btype[We], c == 0);
+        vel_high(F[c], b, 1, n[0], o->c.btype[Ea], c == 0);
+      }
+  }
+  /* y faces: i = -2..n+3, k interior */
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    for (int k = 1; k <= n[2]; k++)
+      for (int i = -2; i <= n[0] + 3; i++) {
+        long b = IX(o, i, 0, k);
+        vel_low(F[c], b, o->sy, n[1], o->c.btype[So], c == 1);
+        vel_high(F[c], b, o->sy, n[1], o->c.btype[No], c == 1);
+      }
+  }
+  /* z faces: i = -2..n+3, j = -2..n+3 */
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    for (int j = -2; j <= n[1] + 3; j++)
+      for (int i = -2; i <= n[0] + 3; i++) {
+        long b = IX(o, i, j, 0);
+        vel_low(F[c], b, o->sz, n[2], o->c.btype[Bo], c == 2);
+        vel_high(F[c], b, o->sz, n[2], o->c.btype[To], c == 2);
+      }
+  }
+}
+
+/* CommonBase, src/scalarboundaries.f90:18-279 (Dirichlet, periodic, Neumann). */
+static void bound_scalar(oracle *o, double *a) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  const int *bt = o->c.scal_btype;
+  const double *side = o->c.side_scal;
+#define S_(i, j, k) A3(o, a, i, j, k)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      if (bt[We] == DIRICHLET) {
+        S_(-2, j, k) = side[We] - (S_(3, j, k) - side[We]);
+        S_(-1, j, k) = side[We] - (S_(2, j, k) - side[We]);
+        S_(0, j, k) = side[We] - (S_(1, j, k) - side[We]);
+      } else if (bt[We] == PERIODIC) {
+        S_(-2, j, k) = S_(nx - 2, j, k);
+        S_(-1, j, k) = S_(nx - 1, j, k);
+        S_(0, j, k) = S_(nx, j, k);
+      } else if (bt[We] == NEUMANN) {
+        S_(0, j, k) = S_(1, j, k) - side[We] * o->dxmin;
+        S_(-1, j, k) = S_(1, j, k) - side[We] * 2 * o->dxmin;
+        S_(-2, j, k) = S_(1, j, k) - side[We] * 3 * o->dxmin;
+      }
+      if (bt[Ea] == DIRICHLET) {
+        S_(nx + 1, j, k) = side[Ea] - (S_(nx, j, k) - side[Ea]);
+        S_(nx + 2, j, k) = side[Ea] - (S_(nx - 1, j, k) - side[Ea]);
+        S_(nx + 3, j, k) = side[Ea] - (S_(nx - 2, j, k) - side[Ea]);
+      } else if (bt[Ea] == PERIODIC) {
+        S_(nx + 1, j, k) = S_(1, j, k);
+        S_(nx + 2, j, k) = S_(2, j, k);
+        S_(nx + 3, j, k) = S_(3, j, k);
+      } else if (bt[Ea] == NEUMANN) {
+        S_(nx + 1, j, k) = S_(nx, j, k) + side[Ea] * o->dxmin;
+        S_(nx + 2, j, k) = S_(nx, j, k) + side[Ea] * 2 * o->dxmin;
+        S_(nx + 3, j, k) = S_(nx, j, k) + side[Ea] * 3 * o->dxmin;
+      }
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = -1; i <= nx + 2; i++) {
+      if (bt[So] == DIRICHLET) {
+        S_(i, -2, k) = side[So] - (S_(i, 3, k) - side[So]);
+        S_(i, -1, k) = side[So] - (S_(i, 2, k) - side[So]);
+        S_(i, 0, k) = side[So] - (S_(i, 1, k) - side[So]);
+      } else if (bt[So] == PERIODIC) {
+        S_(i, -2, k) = S_(i, ny - 2, k);
+        S_(i, -1, k) = S_(i, ny - 1, k);
+        S_(i, 0, k) = S_(i, ny, k);
+      } else if (bt[So] == NEUMANN) {
+        S_(i, 0, k) = S_(i, 1, k) - side[So] * o->dymin;
+        S_(i, -1, k) = S_(i, 1, k) - side[So] * 2 * o->dymin;
+        S_(i, -2, k) = S_(i, 1, k) - side[So] * 3 * o->dymin;
+      }
+      if (bt[No] == DIRICHLET) {
+        S_(i, ny + 1, k) = side[No] - (S_(i, ny, k) - side[No]);
+        S_(i, ny + 2, k) = side[No] - (S_(i, ny - 1, k) - side[No]);
+        S_(i, ny + 3, k) = side[No] - (S_(i, ny - 2, k) - side[No]);
+      } else if (bt[No] == PERIODIC) {
+        S_(i, ny + 1, k) = S_(i, 1, k);
+        S_(i, ny + 2, k) = S_(i, 2, k);
+        S_(i, ny + 3, k) = S_(i, 3, k);
+      } else if (bt[No] == NEUMANN) {
+        S_(i, ny + 1, k) = S_(i, ny, k) + side[No] * o->dymin;
+        S_(i, ny + 2, k) = S_(i, ny, k) + side[No] * 2 * o->dymin;
+        S_(i, ny + 3, k) = S_(i, ny, k) + side[No] * 3 * o->dymin;
+      }
+    }
+  for (int j = -1; j <= ny + 2; j++)
+    for (int i = -1; i <= nx + 2; i++) {
+      if (bt[Bo] == DIRICHLET) {
+        S_(i, j, -2) = side[Bo] - (S_(i, j, 3) - side[Bo]);
+        S_(i, j, -1) = side[Bo] - (S_(i, j, 2) - side[Bo]);
+        S_(i, j, 0) = side[Bo] - (S_(i, j, 1) - side[Bo]);
+      } else if (bt[Bo] == PERIODIC) { /* reversed order as written (scalarboundaries.f90:213) */
+        S_(i, j, -2) = S_(i, j, nz);
+        S_(i, j, -1) = S_(i, j, nz - 1);
+        S_(i, j, 0) = S_(i, j, nz - 2);
+      } else if (bt[Bo] == NEUMANN) {
+        S_(i, j, 0) = S_(i, j, 1) - side[Bo] * C1D(o->dzW, 0);
+        S_(i, j, -1) = S_(i, j, 1) - side[Bo] * (C1D(o->zPr, 1) - C1D(o->zPr, -1));
+        S_(i, j, -2) = S_(i, j, 1) - side[Bo] * (C1D(o->zPr, 1) - C1D(o->zPr, -2));
+      }
+      if (bt[To] == DIRICHLET) {
+        S_(i, j, nz + 1) = side[To] - (S_(i, j, nz) - side[To]);
+        S_(i, j, nz + 2) = side[To] - (S_(i, j, nz - 1) - side[To]);
+        S_(i, j, nz + 3) = side[To] - (S_(i, j, nz - 2) - side[To]);
+      } else if (bt[To] == PERIODIC) {
+        /* arr(i,j,nz+1:3) = arr(i,j,1:3) is an empty section for nz > 2
+           (scalarboundaries.f90:251) */
+      } else if (bt[To] == NEUMANN) {
+        S_(i, j, nz + 1) = S_(i, j, nz) + side[To] * C1D(o->dzW, nz + 1);
+        S_(i, j, nz + 2) = S_(i, j, nz) + side[To] * (C1D(o->zPr, nz + 2) - C1D(o->zPr, nz));
+        S_(i, j, nz + 3) = S_(i, j, nz) + side[To] * (C1D(o->zPr, nz + 3) - C1D(o->zPr, nz));
+      }
+    }
+#undef S_
+}
+
+/* BoundViscosity, src/scalarboundaries.f90:326-431 */
+static void bound_visc(oracle *o, double *a) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  int px = o->c.btype[We] == PERIODIC, py = o->c.btype[So] == PERIODIC, pz = o->c.btype[Bo] == PERIODIC;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      A3(o, a, 0, j, k) = px ? A3(o, a, nx, j, k) : A3(o, a, 1, j, k);
+      A3(o, a, nx + 1, j, k) = px ? A3(o, a, 1, j, k) : A3(o, a, nx, j, k);
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 0; i <= nx + 1; i++) {
+      A3(o, a, i, 0, k) = py ? A3(o, a, i, ny, k) : A3(o, a, i, 1, k);
+      A3(o, a, i, ny + 1, k) = py ? A3(o, a, i, 1, k) : A3(o, a, i, ny, k);
+    }
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      A3(o, a, i, j, 0) = pz ? A3(o, a, i, j, nz) : A3(o, a, i, j, 1);
+      A3(o, a, i, j, nz + 1) = pz ? A3(o, a, i, j, 1) : A3(o, a, i, j, nz);
+    }
+}
+
+/* Bound_Phi, src/boundaries.f90:1817-1948 (PrBtype periodic or Neumann). */
+static void bound_phi(oracle *o, double *a) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  int px = o->c.btype[We] == PERIODIC, py = o->c.btype[So] == PERIODIC, pz = o->c.btype[Bo] == PERIODIC;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      if (px) {
+        A3(o, a, -1, j, k) = A3(o, a, nx - 1, j, k);
+        A3(o, a, 0, j, k) = A3(o, a, nx, j, k);
+        A3(o, a, nx + 1, j, k) = A3(o, a, 1, j, k);
+        A3(o, a, nx + 2, j, k) = A3(o, a, 2, j, k);
+      } else {
+        A3(o, a, -1, j, k) = A3(o, a, 2, j, k);
+        A3(o, a, 0, j, k) = A3(o, a, 1, j, k);
+        A3(o, a, nx + 1, j, k) = A3(o, a, nx, j, k);
+        A3(o, a, nx + 2, j, k) = A3(o, a, nx - 1, j, k);
+      }
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) {
+      if (py) {
+        A3(o, a, i, -1, k) = A3(o, a, i, ny - 1, k);
+        A3(o, a, i, 0, k) = A3(o, a, i, ny, k);
+        A3(o, a, i, ny + 1, k) = A3(o, a, i, 1, k);
+        A3(o, a, i, ny + 2, k) = A3(o, a, i, 2, k);
+      } else {
+        A3(o, a, i, -1, k) = A3(o, a, i, 2, k);
+        A3(o, a, i, 0, k) = A3(o, a, i, 1, k);
+        A3(o, a, i, ny + 1, k) = A3(o, a, i, ny, k);
+        A3(o, a, i, ny + 2, k) = A3(o, a, i, ny - 1, k);
+      }
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      if (pz) {
+        A3(o, a, i, j, -1) = A3(o, a, i, j, nz - 1);
+        A3(o, a, i, j, 0) = A3(o, a, i, j, nz);
+        A3(o, a, i, j, nz + 1) = A3(o, a, i, j, 1);
+        A3(o, a, i, j, nz + 2) = A3(o, a, i, j, 2);
+      } else {
+        A3(o, a, i, j, -1) = A3(o, a, i, j, 2);
+        A3(o, a, i, j, 0) = A3(o, a, i, j, 1);
+        A3(o, a, i, j, nz + 1) = A3(o, a, i, j, nz);
+        A3(o, a, i, j, nz + 2) = A3(o, a, i, j, nz - 1);
+      }
+    }
+}
+
+/* Bound_Pr, src/boundaries.f90:1951-2090 (periodic or extrapolation). */
+static void bound_pr(oracle *o, double *a) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  int px = o->c.btype[We] == PERIODIC, py = o->c.btype[So] == PERIODIC, pz = o->c.btype[Bo] == PERIODIC;
+#define P_(i, j, k) A3(o, a, i, j, k)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      if (px) {
+        P_(-1, j, k) = P_(nx - 1, j, k);
+        P_(0, j, k) = P_(nx, j, k);
+        P_(nx + 1, j, k) = P_(1, j, k);
+        P_(nx + 2, j, k) = P_(2, j, k);
+      } else {
+        P_(0, j, k) = P_(3, j, k) - 3 * (P_(2, j, k) - P_(1, j, k));
+        P_(-1, j, k) = P_(2, j, k);
+        P_(nx + 1, j, k) = P_(nx - 2, j, k) - 3 * (P_(nx - 1, j, k) - P_(nx, j, k));
+        P_(nx + 2, j, k) = P_(nx - 1, j, k);
+      }
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) {
+      if (py) {
+        P_(i, -1, k) = P_(i, ny - 1, k);
+        P_(i, 0, k) = P_(i, ny, k);
+        P_(i, ny + 1, k) = P_(i, 1, k);
+        P_(i, ny + 2, k) = P_(i, 2, k);
+      } else {
+        P_(i, 0, k) = P_(i, 3, k) - 3 * (P_(i, 2, k) - P_(i, 1, k));
+        P_(i, -1, k) = P_(i, 2, k);
+        P_(i, ny + 1, k) = P_(i, ny - 2, k) - 3 * (P_(i, ny - 1, k) - P_(i, ny, k));
+        P_(i, ny + 2, k) = P_(i, ny - 1, k);
+      }
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      if (pz) {
+        P_(i, j, -1) = P_(i, j, nz - 1);
+        P_(i, j, 0) = P_(i, j, nz);
+        P_(i, j, nz + 1) = P_(i, j, 1);
+        P_(i, j, nz + 2) = P_(i, j, 2);
+      } else {
+        P_(i, j, 0) = P_(i, j, 3) - 3 * (P_(i, j, 2) - P_(i, j, 1));
+        P_(i, j, -1) = P_(i, j, 2);
+        P_(i, j, nz + 1) = P_(i, j, nz - 2) - 3 * (P_(i, j, nz - 1) - P_(i, j, nz));
+        P_(i, j, nz + 2) = P_(i, j, nz - 1);
+      }
+    }
+#undef P_
+}
+
+/* Bound_Q, src/boundaries.f90:2094-2192 */
+static void bound_q(oracle *o, double *q) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  int px = o->c.btype[We] == PERIODIC, py = o->c.btype[So] == PERIODIC, pz = o->c.btype[Bo] == PERIODIC;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      if (px)
+        A3(o, q, nx, j, k) = A3(o, q, 0, j, k) + A3(o, q, nx, j, k);
+      else
+        A3(o, q, 1, j, k) = A3(o, q, 1, j, k) + A3(o, q, 0, j, k);
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      if (px)
+        A3(o, q, 1, j, k) = A3(o, q, 1, j, k) + A3(o, q, nx + 1, j, k);
+      else
+        A3(o, q, nx, j, k) = A3(o, q, nx, j, k) + A3(o, q, nx + 1, j, k);
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) {
+      if (py)
+        A3(o, q, i, ny, k) = A3(o, q, i, ny, k) + A3(o, q, i, 0, k);
+      else
+        A3(o, q, i, 1, k) = A3(o, q, i, 1, k) + A3(o, q, i, 0, k);
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) {
+      if (py)
+        A3(o, q, i, 1, k) = A3(o, q, i, 1, k) + A3(o, q, i, ny + 1, k);
+      else
+        A3(o, q, i, ny, k) = A3(o, q, i, ny, k) + A3(o, q, i, ny + 1, k);
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      if (pz)
+        A3(o, q, i, j, nz) = A3(o, q, i, j, nz) + A3(o, q, i, j, 0);
+      else
+        A3(o, q, i, j, 1) = A3(o, q, i, j, 1) + A3(o, q, i, j, 0);
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      if (pz)
+        A3(o, q, i, j, 1) = A3(o, q, i, j, 1) + A3(o, q, i, j, nz + 1);
+      else
+        A3(o, q, i, j, nz) = A3(o, q, i, j, nz) + A3(o, q, i, j, nz + 1);
+    }
+}
+
+/* ------------------------------------------------------------- SGS ------ */
+/* NuSmag + StrainIJ + Strainu, src/subgrid.f90:55-67, 170-209 */
+static double nu_smag(oracle *o, int i, int j, int k, const double *U, const double *V, const double *W) {
+  double D[3][3] = {{0}};
+  double dx = o->dxmin, dy = o->dymin;
+  double zp1 = C1D(o->zPr, k + 1), zm1 = C1D(o->zPr, k - 1);
+  D[0][0] = (A3(o, U, i, j, k) - A3(o, U, i - 1, j, k)) / dx;
+  D[1][1] = (A3(o, V, i, j, k) - A3(o, V, i, j - 1, k)) / dy;
+  D[2][2] = (A3(o, W, i, j, k) - A3(o, W, i, j, k - 1)) / C1D(o->dzPr, k);
+  D[0][1] = (A3(o, U, i, j + 1, k) + A3(o, U, i - 1, j + 1, k) - A3(o, U, i, j - 1, k) - A3(o, U, i - 1, j - 1, k)) /
+            (4 * dy);
+  D[0][2] = (A3(o, U, i, j, k + 1) + A3(o, U, i - 1, j, k + 1) - A3(o, U, i, j, k - 1) - A3(o, U, i - 1, j, k - 1)) /
+            (2 * (zp1 - zm1));
+  D[1][0] = (A3(o, V, i + 1, j, k) + A3(o, V, i + 1, j - 1, k) - A3(o, V, i - 1, j, k) - A3(o, V, i - 1, j - 1, k)) /
+            (4 * dx);
+  D[1][2] = (A3(o, V, i, j, k + 1) + A3(o, V, i, j - 1, k + 1) - A3(o, V, i, j, k - 1) - A3(o, V, i, j - 1, k - 1)) /
+            (2 * (zp1 - zm1));
+  D[2][0] = (A3(o, W, i + 1, j, k) + A3(o, W, i + 1, j, k - 1) - A3(o, W, i - 1, j, k) - A3(o, W, i - 1, j, k - 1)) /
+            (4 * dx);
+  D[2][1] = (A3(o, W, i, j + 1, k) + A3(o, W, i, j + 1, k - 1) - A3(o, W, i, j - 1, k) - A3(o, W, i, j - 1, k - 1)) /
+            (4 * dy);
+  /* Strainu: sum over jj outer, ii inner of S(ii,jj)^2 with S = (D + D^T)/2.
+     D is indexed here D[row=ii][col=jj]. */
+  double s = 0;
+  for (int jj = 0; jj < 3; jj++)
+    for (int ii = 0; ii < 3; ii++) {
+      double Sij = (D[ii][jj] + D[jj][ii]) / 2;
+      s = s + Sij * Sij;
+    }
+  double Sbar = sqrt(2.0 * s);
+  double width = o->c.filter_ratio * pow(C1D(o->dxPr, i) * C1D(o->dyPr, j) * C1D(o->dzPr, k), 1.0 / 3.0);
+  double wc = width * o->c.c_smag;
+  return Sbar * (wc * wc);
+}
+
+/* SGS_Smag, src/subgrid.f90:30-50 (incl. add of molecular_viscosity to the
+   whole array) */
+static void sgs_smag(oracle *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  if (o->c.sgs_model == 1) {
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) A3(o, o->Visc, i, j, k) = nu_smag(o, i, j, k, o->U, o->V, o->W);
+    if (o->c.nu > 0)
+      for (long p = 0; p < o->ntot; p++) o->Visc[p] = o->Visc[p] + o->c.nu;
+  } else {
+    for (long p = 0; p < o->ntot; p++) o->Visc[p] = o->c.nu;
+  }
+}
+
+/* ------------------------------------------------------ wall model ------ */
+static double norm3(const double v[3]) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* vel_and_dist, src/wallmodels.f90:1382-1391 */
+static void vel_and_dist(double tan[3], double *dist, const double u[3], const double wallu[3], const double d[3]) {
+  *dist = norm3(d);
+  for (int c = 0; c < 3; c++) tan[c] = u[c] - wallu[c];
+  double dot = tan[0] * d[0] + tan[1] * d[1] + tan[2] * d[2];
+  double d2 = (*dist) * (*dist);
+  for (int c = 0; c < 3; c++) tan[c] = tan[c] - dot * d[c] / d2;
+}
+
+/* WMFlatUstar, src/wallmodels.f90:1190-1240 */
+static double wm_flat_ustar(double ustar, double vel, double dist, double nu) {
+  const double eps = 1e-4;
+  const double zpl_lam = 5.0, zpl_turb = 30.0;
+  if (nu <= 0) return 0;
+  double ustar_lam = sqrt(nu * vel / dist);
+  if ((dist * ustar_lam / nu) < zpl_lam) return ustar_lam;
+  int i = 0;
+  for (;;) {
+    i = i + 1;
+    double ustar2 = ustar;
+    double z_pl = dist * ustar2 / nu;
+    if (z_pl <= zpl_lam)
+      ustar = sqrt(nu * vel / dist);
+    else if (z_pl < zpl_turb)
+      ustar = vel / (5 * log(z_pl) - (double)3.05f);
+    else
+      ustar = vel / (log(z_pl) / 0.4 + 5.5);
+    if (fabs(ustar - ustar2) / fabs(ustar) < eps) break;
+    if (i >= 50) {
+      ustar = 0;
+      break;
+    }
+  }
+  return ustar;
+}
+
+/* WMRoughUstar, src/wallmodels.f90:1312-1333 */
+static double wm_rough_ustar(double ustar, double vel, double dist, double z0, double nu) {
+  const double yplcrit = 11.225;
+  if (nu > 0) {
+    ustar = sqrt(nu * vel / dist);
+    if (dist * ustar / nu < yplcrit * 1.1) return ustar;
+  }
+  if (dist <= z0 * 2.0) {
+    if (nu > 0)
+      ustar = wm_flat_ustar(ustar, vel, dist, nu);
+    else
+      ustar = 0;
+  } else {
+    ustar = vel * 0.41 / log(dist / z0);
+  }
+  return ustar;
+}
+
+/* local_velocity, src/wallmodels.f90:2356-2399 */
+static void local_velocity(oracle *o, int comp, int i, int j, int k, double v[3]) {
+  const double *U = o->U, *V = o->V, *W = o->W;
+  if (comp == 0) {
+    v[0] = A3(o, U, i, j, k);
+    v[1] = (A3(o, V, i + 1, j, k) + A3(o, V, i, j, k) + A3(o, V, i + 1, j - 1, k) + A3(o, V, i, j - 1, k)) / 4.0;
+    v[2] = (A3(o, W, i + 1, j, k) + A3(o, W, i, j, k) + A3(o, W, i + 1, j, k - 1) + A3(o, W, i, j, k - 1)) / 4.0;
+  } else if (comp == 1) {
+    v[0] = (A3(o, U, i, j + 1, k) + A3(o, U, i, j, k) + A3(o, U, i - 1, j + 1, k) + A3(o, U, i - 1, j, k)) / 4.0;
+    v[1] = A3(o, V, i, j, k);
+    v[2] = (A3(o, W, i, j + 1, k) + A3(o, W, i, j, k) + A3(o, W, i, j + 1, k - 1) + A3(o, W, i, j, k - 1)) / 4.0;
+  } else {
+    v[0] = (A3(o, U, i, j, k + 1) + A3(o, U, i, j, k) + A3(o, U, i - 1, j, k + 1) + A3(o, U, i - 1, j, k)) / 4.0;
+    v[1] = (A3(o, V, i, j, k + 1) + A3(o, V, i, j, k) + A3(o, V, i, j - 1, k + 1) + A3(o, V, i, j - 1, k)) / 4.0;
+    v[2] = A3(o, W, i, j, k);
+  }
+}
+
+/* ComputeUVWFluxesWM + fluxes, src/wallmodels.f90:2159-2337 (neutral,
+   no prescribed ustar, wall velocity 0). */
+static void wm_uvw_fluxes(oracle *o) {
+  if (!o->c.wall_model) return;
+  const double nu = o->c.nu;
+  const double eps = sqrt(2.220446049250313e-16);
+  double drec[6] = {1 / o->dxmin, 1 / o->dxmin, 1 / o->dymin, 1 / o->dymin, 1 / o->dzmin, 1 / o->dzmin};
+  for (int l = 0; l < 18; l++) {
+    wm_list *L = &o->wm[l];
+    int comp = l / 6, dir = l % 6; /* dir 0..5 = MINUSX..PLUSZ - 1 */
+    for (int p = 0; p < L->n; p++) {
+      int xi = L->ijk[3 * p], yj = L->ijk[3 * p + 1], zk = L->ijk[3 * p + 2];
+      double d[3] = {L->dist[3 * p], L->dist[3 * p + 1], L->dist[3 * p + 2]};
+      double vel[3], tan[3], dist, wallu[3] = {0, 0, 0};
+      local_velocity(o, comp, xi, yj, zk, vel);
+      vel_and_dist(tan, &dist, vel, wallu, d);
+      double v = norm3(tan);
+      double ustar = L->ustar[p];
+      if (L->z0[p] > 0) {
+        if (v != 0) ustar = wm_rough_ustar(ustar, v, dist, L->z0[p], nu);
+      } else {
+        if (v != 0) ustar = wm_flat_ustar(ustar, v, dist, nu);
+      }
+      if (ustar < 0) ustar = 0;
+      L->ustar[p] = ustar;
+      double mag = v;
+      double fluxp;
+      if (mag > eps) {
+        if (o->variable_z && dir == 4) {
+          fluxp = (tan[comp] / mag) * (ustar * ustar);
+          fluxp = comp < 2 ? fluxp / C1D(o->dzPr, zk) : fluxp / C1D(o->dzW, zk);
+        } else if (o->variable_z && dir == 5) {
+          fluxp = (tan[comp] / mag) * (ustar * ustar);
+          fluxp = comp < 2 ? fluxp / C1D(o->dzPr, zk + 1) : fluxp / C1D(o->dzW, zk + 1);
+        } else {
+          fluxp = (tan[comp] / mag) * (ustar * ustar) * drec[dir];
+        }
+        if ((dir + 1) % 2 == 1) fluxp = -fluxp;
+      } else {
+        fluxp = 0;
+      }
+      L->fluxp[p] = fluxp;
+    }
+  }
+}
+
+/* DivergenceWM (2nd order), src/wallmodels.f90:1153-1166 */
+static void wm_divergence(oracle *o) {
+  for (int p = 0; p < o->wmp.n; p++) {
+    int xi = o->wmp.ijk[3 * p], yj = o->wmp.ijk[3 * p + 1], zk = o->wmp.ijk[3 * p + 2];
+    double div = 0;
+    if (A3(o, o->Prtype, xi + 1, yj, zk) <= 0) div = div + A3(o, o->U, xi, yj, zk) / o->dxmin;
+    if (A3(o, o->Prtype, xi - 1, yj, zk) <= 0) div = div - A3(o, o->U, xi - 1, yj, zk) / o->dxmin;
+    if (A3(o, o->Prtype, xi, yj + 1, zk) <= 0) div = div + A3(o, o->V, xi, yj, zk) / o->dymin;
+    if (A3(o, o->Prtype, xi, yj - 1, zk) <= 0) div = div - A3(o, o->V, xi, yj - 1, zk) / o->dymin;
+    if (A3(o, o->Prtype, xi, yj, zk + 1) <= 0) div = div + A3(o, o->W, xi, yj, zk) / C1D(o->dzPr, zk);
+    if (A3(o, o->Prtype, xi, yj, zk - 1) <= 0) div = div - A3(o, o->W, xi, yj, zk - 1) / C1D(o->dzPr, zk);
+    o->wmp_div[p] = div;
+  }
+}
+
+/* --------------------------------------------------------- IB gathers --- */
+/* TIBPoint_Interpolate, src/immersedboundary.f90:637-651 */
+static double ib_interp(oracle *o, int c, int p, const double *F) {
+  double s = 0;
+  for (int t = o->ib_ptr[c][p]; t < o->ib_ptr[c][p + 1]; t++) {
+    const int32_t *q = &o->ib_src[c][3 * t];
+    s = s + o->ib_coef[c][t] * A3(o, F, q[0], q[1], q[2]);
+  }
+  return s;
+}
+
+/* IBMomentum, src/tsteps.f90:368-418: gather all, then scatter, per component */
+static void ib_momentum(oracle *o, double *U, double *V, double *W) {
+  double *F[3] = {U, V, W};
+  for (int c = 0; c < 3; c++) {
+    int n = o->ib_n[c];
+    if (!n) continue;
+    double *tmp = xcalloc(n, sizeof(double));
+    for (int p = 0; p < n; p++) tmp[p] = ib_interp(o, c, p, F[c]);
+    for (int p = 0; p < n; p++) {
+      const int32_t *q = &o->ib_ijk[c][3 * p];
+      A3(o, F[c], q[0], q[1], q[2]) = tmp[p];
+    }
+    free(tmp);
+  }
+}
+
+/* IBMassSources (2nd order), src/tsteps.f90:425-538, sequential order */
+static void ib_mass_sources(oracle *o, const double *U, const double *V, const double *W) {
+  for (long p = 0; p < o->ntot; p++) o->Q[p] = 0;
+  const double *F[3] = {U, V, W};
+  const double *dl[3] = {o->dxPr, o->dyPr, o->dzPr};
+  for (int c = 0; c < 3; c++)
+    for (int p = 0; p < o->ib_n[c]; p++) {
+      int q[3] = {o->ib_ijk[c][3 * p], o->ib_ijk[c][3 * p + 1], o->ib_ijk[c][3 * p + 2]};
+      double val = A3(o, F[c], q[0], q[1], q[2]);
+      A3(o, o->Q, q[0], q[1], q[2]) = A3(o, o->Q, q[0], q[1], q[2]) + val / C1D(dl[c], q[c]);
+      int r[3] = {q[0], q[1], q[2]};
+      r[c] += 1;
+      A3(o, o->Q, r[0], r[1], r[2]) = A3(o, o->Q, r[0], r[1], r[2]) - val / C1D(dl[c], q[c] + 1);
+    }
+  bound_q(o, o->Q);
+}
+
+/* NullInterior, src/tsteps.f90:543-567 */
+static void null_interior(oracle *o) {
+  double *F[3] = {o->U, o->V, o->W};
+  for (int c = 0; c < 3; c++)
+    for (int p = 0; p < o->null_n[c]; p++) {
+      const int32_t *q = &o->null_ijk[c][3 * p];
+      A3(o, F[c], q[0], q[1], q[2]) = 0;
+    }
+}
+
+/* SubgridStresses, src/dynamics.f90:869-908 (neutral; ComputeViscsWM
+   only writes Viscosity that SubgridModel overwrites and the diagnostic
+   WMPoints%ustar, so it is not restated). */
+static void subgrid_stresses(oracle *o) {
+  sgs_smag(o);
+  wm_uvw_fluxes(o);
+  bound_visc(o, o->Visc);
+  /* serial loop over ScalFlIBPoints (dynamics.f90:899-902) */
+  for (int p = 0; p < o->ib_n[3]; p++) {
+    const int32_t *q = &o->ib_ijk[3][3 * p];
+    A3(o, o->Visc, q[0], q[1], q[2]) = ib_interp(o, 3, p, o->Visc);
+  }
+  if (o->c.nscalars > 0) bound_visc(o, o->TDiff);
+}
+
+/* ------------------------------------------------- momentum advection --- */
+static inline double mask4(double a, double b, double c, double d) {
+  /* sum(i_mask4 * X(m-2:m+1)) with i_mask4 = [-1, 9, 9, -1], left to right */
+  return ((-a + 9 * b) + 9 * c) + -d;
+}
+
+/* CD4divU, src/advection.f90:334-491 */
+static double cd4_u(oracle *o, int i, int j, int k) {
+  const double *U = o->U, *V = o->V, *W = o->W;
+  double acc = 0, dU;
+  int t = A3(o, o->Utype, i, j, k);
+#define Ux(m) A3(o, U, m, j, k)
+  if (t == 0) {
+    double uv1a, uv1b, uv3a, uv3b, Ui;
+    Ui = mask4(Ux(i - 1), Ux(i), Ux(i + 1), Ux(i + 2)); /* face i+1 */
+    uv1b = Ui * (Ux(i) + Ux(i + 1));
+    Ui = mask4(Ux(i - 2), Ux(i - 1), Ux(i), Ux(i + 1)); /* face i */
+    uv1a = Ui * (Ux(i - 1) + Ux(i));
+    Ui = mask4(Ux(i), Ux(i + 1), Ux(i + 2), Ux(i + 3)); /* face i+2 */
+    uv3b = Ui * (Ux(i) + Ux(i + 3));
+    Ui = mask4(Ux(i - 3), Ux(i - 2), Ux(i - 1), Ux(i)); /* face i-1 */
+    uv3a = Ui * (Ux(i - 3) + Ux(i));
+    dU = 9 * (uv1b - uv1a);
+    dU = dU - (uv3b - uv3a) / 3;
+    acc = acc + dU / o->dxmin;
+  } else if (t < 0) {
+    dU = 64 * ((Ux(i + 1) + Ux(i)) * (Ux(i + 1) + Ux(i)) - (Ux(i) + Ux(i - 1)) * (Ux(i) + Ux(i - 1)));
+    acc = acc + dU / o->dxmin;
+  }
+#undef Ux
+  /* y: Vint(m) = mask4(V(i-1:i+2, m)), UV1(m) = Vint(m)*(U(j=m)+U(m+1)) */
+#define VI(m) mask4(A3(o, V, i - 1, m, k), A3(o, V, i, m, k), A3(o, V, i + 1, m, k), A3(o, V, i + 2, m, k))
+#define Uy(m) A3(o, U, i, m, k)
+  if (t == 0) {
+    double uv1b = VI(j) * (Uy(j) + Uy(j + 1));
+    double uv1a = VI(j - 1) * (Uy(j - 1) + Uy(j));
+    double uv3b = VI(j + 1) * (Uy(j) + Uy(j + 3));
+    double uv3a = VI(j - 2) * (Uy(j - 3) + Uy(j));
+    dU = 9 * (uv1b - uv1a);
+    dU = dU - (uv3b - uv3a) / 3;
+    acc = acc + dU / o->dymin;
+  } else if (t < 0) {
+    dU = 64 * ((Uy(j + 1) + Uy(j)) * (A3(o, V, i + 1, j, k) + A3(o, V, i, j, k)) -
+               (Uy(j) + Uy(j - 1)) * (A3(o, V, i + 1, j - 1, k) + A3(o, V, i, j - 1, k)));
+    acc = acc + dU / o->dymin;
+  }
+#undef VI
+#undef Uy
+#define WI(m) mask4(A3(o, W, i - 1, j, m), A3(o, W, i, j, m), A3(o, W, i + 1, j, m), A3(o, W, i + 2, j, m))
+#define Uz(m) A3(o, U, i, j, m)
+  if (t == 0) {
+    double uv1b = WI(k) * (Uz(k) + Uz(k + 1));
+    double uv1a = WI(k - 1) * (Uz(k - 1) + Uz(k));
+    double uv3b = WI(k + 1) * (Uz(k) + Uz(k + 3));
+    double uv3a = WI(k - 2) * (Uz(k - 3) + Uz(k));
+    dU = 9 * (uv1b - uv1a);
+    dU = dU - (uv3b - uv3a) / 3;
+    acc = acc + dU / o->dzmin;
+  } else if (t < 0) {
+    dU = 64 * ((Uz(k + 1) + Uz(k)) * (A3(o, W, i + 1, j, k) + A3(o, W, i, j, k)) -
+               (Uz(k) + Uz(k - 1)) * (A3(o, W, i + 1, j, k - 1) + A3(o, W, i, j, k - 1)));
+    acc = acc + dU / o->dzmin;
+  }
+#undef WI
+#undef Uz
+  return acc * (-1.0 / 256);
+}
+
+/* CD4divV, src/advection.f90:500-658 */
+static double cd4_v(oracle *o, int i, int j, int k) {
+  const double *U = o->U, *V = o->V, *W = o->W;
+  double acc = 0, dV;
+  int t = A3(o, o->Vtype, i, j, k);
+#define UI(m) mask4(A3(o, U, m, j - 1, k), A3(o, U, m, j, k), A3(o, U, m, j + 1, k), A3(o, U, m, j + 2, k))
+#define Vx(m) A3(o, V, m, j, k)
+  if (t == 0) {
+    double uv1b = UI(i) * (Vx(i) + Vx(i + 1));
+    double uv1a = UI(i - 1) * (Vx(i - 1) + Vx(i));
+    double uv3b = UI(i + 1) * (Vx(i) + Vx(i + 3));
+    double uv3a = UI(i - 2) * (Vx(i - 3) + Vx(i));
+    dV = 9 * (uv1b - uv1a);
+    dV = dV - (uv3b - uv3a) / 3;
+    acc = acc + dV / o->dxmin;
+  } else if (t < 0) {
+    dV = 64 * ((Vx(i + 1) + Vx(i)) * (A3(o, U, i, j + 1, k) + A3(o, U, i, j, k)) -
+               (Vx(i) + Vx(i - 1)) * (A3(o, U, i - 1, j + 1, k) + A3(o, U, i - 1, j, k)));
+    acc = acc + dV / o->dxmin;
+  }
+#undef UI
+#undef Vx
+#define Vy(m) A3(o, V, i, m, k)
+  if (t == 0) {
+    double Vi;
+    Vi = mask4(Vy(j - 1), Vy(j), Vy(j + 1), Vy(j + 2));
+    double uv1b = Vi * (Vy(j) + Vy(j + 1));
+    Vi = mask4(Vy(j - 2), Vy(j - 1), Vy(j), Vy(j + 1));
+    double uv1a = Vi * (Vy(j - 1) + Vy(j));
+    Vi = mask4(Vy(j), Vy(j + 1), Vy(j + 2), Vy(j + 3));
+    double uv3b = Vi * (Vy(j) + Vy(j + 3));
+    Vi = mask4(Vy(j - 3), Vy(j - 2), Vy(j - 1), Vy(j));
+    double uv3a = Vi * (Vy(j - 3) + Vy(j));
+    dV = 9 * (uv1b - uv1a);
+    dV = dV - (uv3b - uv3a) / 3;
+    acc = acc + dV / o->dymin;
+  } else if (t < 0) {
+    dV = 64 * ((Vy(j + 1) + Vy(j)) * (Vy(j + 1) + Vy(j)) - (Vy(j) + Vy(j - 1)) * (Vy(j) + Vy(j - 1)));
+    acc = acc + dV / o->dymin;
+  }
+#undef Vy
+#define WI(m) mask4(A3(o, W, i, j - 1, m), A3(o, W, i, j, m), A3(o, W, i, j + 1, m), A3(o, W, i, j + 2, m))
+#define Vz(m) A3(o, V, i, j, m)
+  if (t == 0) {
+    double uv1b = WI(k) * (Vz(k) + Vz(k + 1));
+    double uv1a = WI(k - 1) * (Vz(k - 1) + Vz(k));
+    double uv3b = WI(k + 1) * (Vz(k) + Vz(k + 3));
+    double uv3a = WI(k - 2) * (Vz(k - 3) + Vz(k));
+    dV = 9 * (uv1b - uv1a);
+    dV = dV - (uv3b - uv3a) / 3;
+    acc = acc + dV / o->dzmin;
+  } else if (t < 0) {
+    dV = 64 * ((Vz(k + 1) + Vz(k)) * (A3(o, W, i, j + 1, k) + A3(o, W, i, j, k)) -
+               (Vz(k) + Vz(k - 1)) * (A3(o, W, i, j + 1, k - 1) + A3(o, W, i, j, k - 1)));
+    acc = acc + dV / o->dzmin;
+  }
+#undef WI
+#undef Vz
+  return acc * (-1.0 / 256);
+}
+
+/* CD4divW, src/advection.f90:675-832 */
+static double cd4_w(oracle *o, int i, int j, int k) {
+  const double *U = o->U, *V = o->V, *W = o->W;
+  double acc = 0, dW;
+  int t = A3(o, o->Wtype, i, j, k);
+#define UI(m) mask4(A3(o, U, m, j, k - 1), A3(o, U, m, j, k), A3(o, U, m, j, k + 1), A3(o, U, m, j, k + 2))
+#define Wx(m) A3(o, W, m, j, k)
+  if (t == 0) {
+    double uv1b = UI(i) * (Wx(i) + Wx(i + 1));
+    double uv1a = UI(i - 1) * (Wx(i - 1) + Wx(i));
+    double uv3b = UI(i + 1) * (Wx(i) + Wx(i + 3));
+    double uv3a = UI(i - 2) * (Wx(i - 3) + Wx(i));
+    dW = 9 * (uv1b - uv1a);
+    dW = dW - (uv3b - uv3a) / 3;
+    acc = acc + dW / o->dxmin;
+  } else if (t < 0) {
+    dW = 64 * ((Wx(i + 1) + Wx(i)) * (A3(o, U, i, j, k + 1) + A3(o, U, i, j, k)) -
+               (Wx(i) + Wx(i - 1)) * (A3(o, U, i - 1, j, k + 1) + A3(o, U, i - 1, j, k)));
+    acc = acc + dW / o->dxmin;
+  }
+#undef UI
+#undef Wx
+#define VI(m) mask4(A3(o, V, i, m, k - 1), A3(o, V, i, m, k), A3(o, V, i, m, k + 1), A3(o, V, i, m, k + 2))
+#define Wy(m) A3(o, W, i, m, k)
+  if (t == 0) {
+    double uv1b = VI(j) * (Wy(j) + Wy(j + 1));
+    double uv1a = VI(j - 1) * (Wy(j - 1) + Wy(j));
+    double uv3b = VI(j + 1) * (Wy(j) + Wy(j + 3));
+    double uv3a = VI(j - 2) * (Wy(j - 3) + Wy(j));
+    dW = 9 * (uv1b - uv1a);
+    dW = dW - (uv3b - uv3a) / 3;
+    acc = acc + dW / o->dymin;
+  } else if (t < 0) {
+    dW = 64 * ((Wy(j + 1) + Wy(j)) * (A3(o, V, i, j, k + 1) + A3(o, V, i, j, k)) -
+               (Wy(j) + Wy(j - 1)) * (A3(o, V, i, j - 1, k) + A3(o, V, i, j - 1, k + 1)));
+    acc = acc + dW / o->dymin;
+  }
+#undef VI
+#undef Wy
+#define Wz(m) A3(o, W, i, j, m)
+  if (t == 0) {
+    double Wi;
+    Wi = mask4(Wz(k - 1), Wz(k), Wz(k + 1), Wz(k + 2));
+    double uv1b = Wi * (Wz(k) + Wz(k + 1));
+    Wi = mask4(Wz(k - 2), Wz(k - 1), Wz(k), Wz(k + 1));
+    double uv1a = Wi * (Wz(k - 1) + Wz(k));
+    Wi = mask4(Wz(k), Wz(k + 1), Wz(k + 2), Wz(k + 3));
+    double uv3b = Wi * (Wz(k) + Wz(k + 3));
+    Wi = mask4(Wz(k - 3), Wz(k - 2), Wz(k - 1), Wz(k));
+    double uv3a = Wi * (Wz(k - 3) + Wz(k));
+    dW = 9 * (uv1b - uv1a);
+    dW = dW - (uv3b - uv3a) / 3;
+    acc = acc + dW / o->dzmin;
+  } else if (t < 0) {
+    dW = 64 * ((Wz(k + 1) + Wz(k)) * (Wz(k + 1) + Wz(k)) - (Wz(k) + Wz(k - 1)) * (Wz(k) + Wz(k - 1)));
+    acc = acc + dW / o->dzmin;
+  }
+#undef Wz
+  return acc * (-1.0 / 256);
+}
+
+/* ------------------------------------------------ momentum diffusion --- */
+/* Face fluxes of MomentumDiffusion_nobranch_2ord (momentum_diffusion.f90:
+   169-288); flux F_d(c) sits on the upper face of c in direction d and the
+   bulk adds +F_d(c) - F_d(c - e_d), d = x, y, z. */
+static double flux_u(oracle *o, int d, int i, int j, int k) {
+  const double *nu = o->Visc, *U = o->U;
+  double rx = 1.0 / (o->dxmin * o->dxmin), ry = 1.0 / (o->dymin * o->dymin), rz = 1.0 / (o->dzmin * o->dzmin);
+  if (d == 0) return A3(o, nu, i + 1, j, k) * (A3(o, U, i + 1, j, k) - A3(o, U, i, j, k)) * rx;
+  if (d == 1)
+    return 0.25 * (A3(o, nu, i + 1, j + 1, k) + A3(o, nu, i + 1, j, k) + A3(o, nu, i, j + 1, k) + A3(o, nu, i, j, k)) *
+           (A3(o, U, i, j + 1, k) - A3(o, U, i, j, k)) * ry;
+  return 0.25 * (A3(o, nu, i + 1, j, k + 1) + A3(o, nu, i + 1, j, k) + A3(o, nu, i, j, k + 1) + A3(o, nu, i, j, k)) *
+         (A3(o, U, i, j, k + 1) - A3(o, U, i, j, k)) * rz;
+}
+static double flux_v(oracle *o, int d, int i, int j, int k) {
+  const double *nu = o->Visc, *V = o->V;
+  double rx = 1.0 / (o->dxmin * o->dxmin), ry = 1.0 / (o->dymin * o->dymin), rz = 1.0 / (o->dzmin * o->dzmin);
+  if (d == 0)
+    return 0.25 * (A3(o, nu, i + 1, j + 1, k) + A3(o, nu, i + 1, j, k) + A3(o, nu, i, j + 1, k) + A3(o, nu, i, j, k)) *
+           (A3(o, V, i + 1, j, k) - A3(o, V, i, j, k)) * rx;
+  if (d == 1) return A3(o, nu, i, j + 1, k) * (A3(o, V, i, j + 1, k) - A3(o, V, i, j, k)) * ry;
+  return 0.25 * (A3(o, nu, i, j + 1, k + 1) + A3(o, nu, i, j + 1, k) + A3(o, nu, i, j, k + 1) + A3(o, nu, i, j, k)) *
+         (A3(o, V, i, j, k + 1) - A3(o, V, i, j, k)) * rz;
+}
+static double flux_w(oracle *o, int d, int i, int j, int k) {
+  const double *nu = o->Visc, *W = o->W;
+  double rx = 1.0 / (o->dxmin * o->dxmin), ry = 1.0 / (o->dymin * o->dymin), rz = 1.0 / (o->dzmin * o->dzmin);
+  if (d == 0)
+    return 0.25 * (A3(o, nu, i + 1, j, k + 1) + A3(o, nu, i + 1, j, k) + A3(o, nu, i, j, k + 1) + A3(o, nu, i, j, k)) *
+           (A3(o, W, i + 1, j, k) - A3(o, W, i, j, k)) * rx;
+  if (d == 1)
+    return 0.25 * (A3(o, nu, i, j + 1, k + 1) + A3(o, nu, i, j, k + 1) + A3(o, nu, i, j + 1, k) + A3(o, nu, i, j, k)) *
+           (A3(o, W, i, j + 1, k) - A3(o, W, i, j, k)) * ry;
+  return A3(o, nu, i, j, k + 1) * (A3(o, W, i, j, k + 1) - A3(o, W, i, j, k)) * rz;
+}
+typedef double (*flux_fn)(oracle *, int, int, int, int);
+static const flux_fn FLUX[3] = {flux_u, flux_v, flux_w};
+
+static void momentum_diffusion(oracle *o) {
+  double *S[3] = {o->Us, o->Vs, o->Ws};
+  const int *N[3] = {o->Un, o->Vn, o->Wn};
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    flux_fn F = FLUX[c];
+    double *s = S[c];
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= n[2]; k++)
+      for (int j = 1; j <= n[1]; j++)
+        for (int i = 1; i <= n[0]; i++) {
+          double a = A3(o, s, i, j, k);
+          a = a + F(o, 0, i, j, k);
+          a = a - F(o, 0, i - 1, j, k);
+          a = a + F(o, 1, i, j, k);
+          a = a - F(o, 1, i, j - 1, k);
+          a = a + F(o, 2, i, j, k);
+          a = a - F(o, 2, i, j, k - 1);
+          A3(o, s, i, j, k) = a;
+        }
+    if (!o->c.wall_model) continue;
+    /* wall-model face-flux replacement, wmfluxes-nobranch-inc.f90:49-208:
+       per list (xm, xp, ym, yp, zm, zp) two passes over the points */
+    for (int dir = 0; dir < 6; dir++) {
+      wm_list *L = &o->wm[c * 6 + dir];
+      int ax = dir / 2, plus = dir % 2;
+      for (int pass = 0; pass < 2; pass++)
+        for (int p = 0; p < L->n; p++) {
+          int q[3] = {L->ijk[3 * p], L->ijk[3 * p + 1], L->ijk[3 * p + 2]};
+          int f[3] = {q[0], q[1], q[2]}; /* lower cell of the face */
+          int t[3] = {q[0], q[1], q[2]}; /* written cell */
+          if (!plus) f[ax] -= 1;
+          double sign;
+          if (!plus) {
+            if (pass == 1) t[ax] -= 1;
+            sign = pass == 0 ? 1.0 : -1.0;
+          } else {
+            if (pass == 0) t[ax] += 1;
+            sign = pass == 0 ? 1.0 : -1.0;
+          }
+          double fl = F(o, ax, f[0], f[1], f[2]);
+          double *w = &A3(o, s, t[0], t[1], t[2]);
+          if (sign > 0) {
+            *w = *w + fl;
+            *w = *w + L->fluxp[p];
+          } else {
+            *w = *w - fl;
+            *w = *w - L->fluxp[p];
+          }
+        }
+    }
+  }
+}
+
+/* Convection (advection_method 4, explicit diffusion), src/dynamics.f90:230-499 */
+static void convection(oracle *o, int stage, double dt) {
+  double *U2[3] = {o->U2, o->V2, o->W2};
+  double *S[3] = {o->Us, o->Vs, o->Ws};
+  const int *N[3] = {o->Un, o->Vn, o->Wn};
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    for (int k = 1; k <= n[2]; k++)
+      for (int j = 1; j <= n[1]; j++)
+        for (int i = 1; i <= n[0]; i++) {
+          if (stage > 0)
+            A3(o, U2[c], i, j, k) = A3(o, S[c], i, j, k) * RK_rho[stage] * dt;
+          else {
+            A3(o, S[c], i, j, k) = 0;
+            A3(o, U2[c], i, j, k) = 0;
+          }
+        }
+  }
+  /* CD4div*: set(U2, 0) on the whole array, accumulate, multiply by -1/256 */
+  for (int c = 0; c < 3; c++) {
+    double *s = S[c];
+    /* entries outside the interior end as 0 * (-1/256) = -0.0 */
+    for (long p = 0; p < o->ntot; p++) s[p] = 0.0 * (-1.0 / 256);
+    const int *n = N[c];
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= n[2]; k++)
+      for (int j = 1; j <= n[1]; j++)
+        for (int i = 1; i <= n[0]; i++)
+          A3(o, s, i, j, k) = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
+  }
+  momentum_diffusion(o);
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    for (int k = 1; k <= n[2]; k++)
+      for (int j = 1; j <= n[1]; j++)
+        for (int i = 1; i <= n[0]; i++)
+          A3(o, U2[c], i, j, k) = A3(o, U2[c], i, j, k) + A3(o, S[c], i, j, k) * RK_beta[stage] * dt;
+  }
+}
+
+/* OtherTerms with explicit diffusion: PressureGrad (2nd order) + add(U2,U),
+   src/tsteps.f90:263-358, src/dynamics.f90:58-199 */
+static void other_terms(oracle *o, double coef) {
+  double A = -coef, Ax = -coef / o->dxmin, Ay = -coef / o->dymin, Az = -coef / o->dzmin;
+  const double *Pr = o->Pr;
+  if (o->c.pr_gradient_x != 0)
+    for (int k = 1; k <= o->Un[2]; k++)
+      for (int j = 1; j <= o->Un[1]; j++)
+        for (int i = 1; i <= o->Un[0]; i++)
+          A3(o, o->U2, i, j, k) = A3(o, o->U2, i, j, k) + A * o->c.pr_gradient_x;
+  if (o->c.pr_gradient_y != 0)
+    for (int k = 1; k <= o->Vn[2]; k++)
+      for (int j = 1; j <= o->Vn[1]; j++)
+        for (int i = 1; i <= o->Vn[0]; i++)
+          A3(o, o->V2, i, j, k) = A3(o, o->V2, i, j, k) + A * o->c.pr_gradient_y;
+  for (int k = 1; k <= o->Un[2]; k++)
+    for (int j = 1; j <= o->Un[1]; j++)
+      for (int i = 1; i <= o->Un[0]; i++)
+        A3(o, o->U2, i, j, k) = A3(o, o->U2, i, j, k) + Ax * (A3(o, Pr, i + 1, j, k) - A3(o, Pr, i, j, k));
+  for (int k = 1; k <= o->Vn[2]; k++)
+    for (int j = 1; j <= o->Vn[1]; j++)
+      for (int i = 1; i <= o->Vn[0]; i++)
+        A3(o, o->V2, i, j, k) = A3(o, o->V2, i, j, k) + Ay * (A3(o, Pr, i, j + 1, k) - A3(o, Pr, i, j, k));
+  for (int k = 1; k <= o->Wn[2]; k++)
+    for (int j = 1; j <= o->Wn[1]; j++)
+      for (int i = 1; i <= o->Wn[0]; i++) {
+        if (o->variable_z)
+          A3(o, o->W2, i, j, k) =
+              A3(o, o->W2, i, j, k) - coef * (A3(o, Pr, i, j, k + 1) - A3(o, Pr, i, j, k)) / C1D(o->dzW, k);
+        else
+          A3(o, o->W2, i, j, k) = A3(o, o->W2, i, j, k) + Az * (A3(o, Pr, i, j, k + 1) - A3(o, Pr, i, j, k));
+      }
+  for (long p = 0; p < o->ntot; p++) {
+    o->U2[p] = o->U2[p] + o->U[p];
+    o->V2[p] = o->V2[p] + o->V[p];
+    o->W2[p] = o->W2[p] + o->W[p];
+  }
+}
+
+/* ---------------------------------------------------------- scalars ----- */
+static inline double flux_limiter(double r) {
+  /* src/scalar_advection.f90:260-267 */
+  double a = (1 + 2 * r) / 3;
+  double b = 2.0 < a ? 2.0 : a;
+  double c2 = 2 * r < b ? 2 * r : b;
+  return 0.0 > c2 ? 0.0 : c2;
+}
+static inline double sgn1(double x) { return copysign(1.0, x); }
+
+/* KappaScalar, src/scalar_advection.f90:102-269, restated per cell with the
+   reference's accumulation order (x pass, y pass, z pass with odd/even
+   ordering of the k faces). */
+static void kappa_scalar(oracle *o, double *Sadv, const double *S) {
+  const double eps = (double)1e-8f; /* real(knd), parameter :: eps = 1e-8 (default-real literal) */
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  const double Ax = 1 / o->dxmin, Ay = 1 / o->dymin, Az = 1 / o->dzmin;
+  for (long p = 0; p < o->ntot; p++) Sadv[p] = 0;
+  const double *U = o->U, *V = o->V, *W = o->W;
+  /* x pass (faces i = 0..nx) */
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 0; i <= nx; i++) {
+        double u = A3(o, U, i, j, k), sr, sl;
+        if (u > 0) {
+          sr = A3(o, S, i + 1, j, k) - A3(o, S, i, j, k);
+          sl = A3(o, S, i, j, k) - A3(o, S, i - 1, j, k);
+        } else {
+          sr = A3(o, S, i, j, k) - A3(o, S, i + 1, j, k);
+          sl = A3(o, S, i + 1, j, k) - A3(o, S, i + 2, j, k);
+        }
+        double slope = flux_limiter((sr + eps * sgn1(sl)) / (sl + eps * sgn1(sl)));
+        if (A3(o, o->scflmask[0], i, j, k)) {
+          double flux;
+          if (u > 0)
+            flux = u * (A3(o, S, i, j, k) + (A3(o, S, i, j, k) - A3(o, S, i - 1, j, k)) * slope / 2.0);
+          else
+            flux = u * (A3(o, S, i + 1, j, k) + (A3(o, S, i + 1, j, k) - A3(o, S, i + 2, j, k)) * slope / 2.0);
+          A3(o, Sadv, i, j, k) = A3(o, Sadv, i, j, k) - Ax * flux;
+          A3(o, Sadv, i + 1, j, k) = A3(o, Sadv, i + 1, j, k) + Ax * flux;
+        }
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 0; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double v = A3(o, V, i, j, k), sr, sl;
+        if (v > 0) {
+          sr = A3(o, S, i, j + 1, k) - A3(o, S, i, j, k);
+          sl = A3(o, S, i, j, k) - A3(o, S, i, j - 1, k);
+        } else {
+          sr = A3(o, S, i, j, k) - A3(o, S, i, j + 1, k);
+          sl = A3(o, S, i, j + 1, k) - A3(o, S, i, j + 2, k);
+        }
+        double slope = flux_limiter((sr + eps * sgn1(sl)) / (sl + eps * sgn1(sl)));
+        if (A3(o, o->scflmask[1], i, j, k)) {
+          double flux;
+          if (v > 0)
+            flux = v * (A3(o, S, i, j, k) + (A3(o, S, i, j, k) - A3(o, S, i, j - 1, k)) * slope / 2.0);
+          else
+            flux = v * (A3(o, S, i, j + 1, k) + (A3(o, S, i, j + 1, k) - A3(o, S, i, j + 2, k)) * slope / 2.0);
+          A3(o, Sadv, i, j, k) = A3(o, Sadv, i, j, k) - Ay * flux;
+          A3(o, Sadv, i, j + 1, k) = A3(o, Sadv, i, j + 1, k) + Ay * flux;
+        }
+      }
+  for (int l = 0; l <= 1; l++)
+    for (int k = l; k <= nz; k += 2)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          double w = A3(o, W, i, j, k), sr, sl;
+          if (w > 0) {
+            sr = A3(o, S, i, j, k + 1) - A3(o, S, i, j, k);
+            sl = A3(o, S, i, j, k) - A3(o, S, i, j, k - 1);
+          } else {
+            sr = A3(o, S, i, j, k) - A3(o, S, i, j, k + 1);
+            sl = A3(o, S, i, j, k + 1) - A3(o, S, i, j, k + 2);
+          }
+          double slope = flux_limiter((sr + eps * sgn1(sl)) / (sl + eps * sgn1(sl)));
+          if (A3(o, o->scflmask[2], i, j, k)) {
+            double flux;
+            if (w > 0)
+              flux = w * (A3(o, S, i, j, k) + (A3(o, S, i, j, k) - A3(o, S, i, j, k - 1)) * slope / 2.0);
+            else
+              flux = w * (A3(o, S, i, j, k + 1) + (A3(o, S, i, j, k + 1) - A3(o, S, i, j, k + 2)) * slope / 2.0);
+            A3(o, Sadv, i, j, k) = A3(o, Sadv, i, j, k) - Az * flux;
+            A3(o, Sadv, i, j, k + 1) = A3(o, Sadv, i, j, k + 1) + Az * flux;
+          }
+        }
+}
+
+/* ScalarDiffusion_nobranch, src/scalar_diffusion.f90:404-507 */
+static void scalar_diffusion(oracle *o, double *Sadv, const double *S) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  const double *T = o->TDiff;
+  const double Ax = 1 / (2 * (o->dxmin * o->dxmin)), Ay = 1 / (2 * (o->dymin * o->dymin)),
+               Az = 1 / (2 * (o->dzmin * o->dzmin));
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double a = A3(o, Sadv, i, j, k);
+        a = a + (A3(o, T, i + 1, j, k) + A3(o, T, i, j, k)) * (A3(o, S, i + 1, j, k) - A3(o, S, i, j, k)) * Ax;
+        a = a - (A3(o, T, i, j, k) + A3(o, T, i - 1, j, k)) * (A3(o, S, i, j, k) - A3(o, S, i - 1, j, k)) * Ax;
+        a = a + (A3(o, T, i, j + 1, k) + A3(o, T, i, j, k)) * (A3(o, S, i, j + 1, k) - A3(o, S, i, j, k)) * Ay;
+        a = a - (A3(o, T, i, j, k) + A3(o, T, i, j - 1, k)) * (A3(o, S, i, j, k) - A3(o, S, i, j - 1, k)) * Ay;
+        a = a + (A3(o, T, i, j, k + 1) + A3(o, T, i, j, k)) * (A3(o, S, i, j, k + 1) - A3(o, S, i, j, k)) * Az;
+        a = a - (A3(o, T, i, j, k) + A3(o, T, i, j, k - 1)) * (A3(o, S, i, j, k) - A3(o, S, i, j, k - 1)) * Az;
+        A3(o, Sadv, i, j, k) = a;
+      }
+  const double Ad[3] = {Ax, Ay, Az};
+  for (int d = 0; d < 3; d++)
+    for (int pass = 0; pass < 2; pass++)
+      for (int p = 0; p < o->scfl_n[d]; p++) {
+        int q[3] = {o->scfl_ijk[d][3 * p], o->scfl_ijk[d][3 * p + 1], o->scfl_ijk[d][3 * p + 2]};
+        int r[3] = {q[0], q[1], q[2]};
+        r[d] += 1;
+        double fl = (A3(o, T, r[0], r[1], r[2]) + A3(o, T, q[0], q[1], q[2])) *
+                    (A3(o, S, r[0], r[1], r[2]) - A3(o, S, q[0], q[1], q[2])) * Ad[d];
+        if (pass == 0)
+          A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) - fl;
+        else
+          A3(o, Sadv, r[0], r[1], r[2]) = A3(o, Sadv, r[0], r[1], r[2]) + fl;
+      }
+}
+
+/* ScalarRK3 stage for passive scalars, src/scalars.f90:88-299, 364-379 */
+static void scalar_rk3(oracle *o, int stage, double dt) {
+  if (o->c.nscalars == 0) return;
+  wm_divergence(o);
+  for (int sc = 0; sc < o->c.nscalars; sc++) {
+    double *S = o->S[sc], *S2 = o->S2[sc], *Sadv = o->Sadv[sc];
+    bound_scalar(o, S);
+    if (stage > 0) {
+      double a = RK_rho[stage] * dt;
+      for (long p = 0; p < o->ntot; p++) S2[p] = Sadv[p] * a;
+    } else {
+      for (long p = 0; p < o->ntot; p++) S2[p] = 0;
+    }
+    kappa_scalar(o, Sadv, S);
+    for (int k = 0; k <= o->nz + 1; k++)
+      for (int j = 0; j <= o->ny + 1; j++)
+        for (int i = 0; i <= o->nx + 1; i++)
+          if (A3(o, o->Prtype, i, j, k) > 0) A3(o, Sadv, i, j, k) = 0;
+    scalar_diffusion(o, Sadv, S);
+    /* ScalarExtra: volume (point) sources, then WM divergence term */
+    for (int p = 0; p < o->src_n; p++) {
+      if (o->src_sc[p] != sc) continue;
+      const int32_t *q = &o->src_ijk[3 * p];
+      A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->src_flux[p];
+    }
+    for (int p = 0; p < o->wmp.n; p++) {
+      const int32_t *q = &o->wmp.ijk[3 * p];
+      A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
+    }
+    double b = RK_beta[stage] * dt;
+    for (long p = 0; p < o->ntot; p++) S2[p] = S2[p] + Sadv[p] * b;
+    for (long p = 0; p < o->ntot; p++) S[p] = S[p] + S2[p];
+    bound_scalar(o, S);
+  }
+}
+
+/* ------------------------------------------------------------ FFT ------- */
+/* The reference calls an external FFT Poisson library (PoisSolver,
+   src/poisson.f90:76-210) that is not part of the reference tree.  This
+   restatement solves the same discrete operator (Residue_variable_z,
+   src/poisson.f90:23-60) directly: DFT in the periodic x and y directions,
+   a tridiagonal system per mode in z.  The transform is an iterative
+   radix-2 decimation-in-time FFT with twiddles exp(-2 pi i q/n) taken from a
+   long-double table, and the tridiagonal systems are reduced by parallel
+   cyclic reduction (executed serially here); the singular horizontal-mean
+   mode is integrated exactly with compensated sums.  The operation order is
+   stated so that results can be compared bit for bit. */
+typedef struct {
+  double re, im;
+} c2;
+
+static c2 *twiddles(int n) {
+  c2 *t = xcalloc(n / 2 + 1, sizeof(c2));
+  for (int q = 0; q < n / 2; q++) {
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * q / n;
+    t[q].re = (double)cosl(a);
+    t[q].im = (double)-sinl(a);
+  }
+  return t;
+}
+
+static int ilog2i(int n) {
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return l;
+}
+static int bitrev_i(int v, int logn) {
+  int r = 0;
+  for (int b = 0; b < logn; b++)
+    if (v & (1 << b)) r |= 1 << (logn - 1 - b);
+  return r;
+}
+
+/* In-place FFT of a strided line of length n (power of two). */
+static void fft_line(c2 *x, int n, long stride, const c2 *tw, int inverse, c2 *work) {
+  int logn = ilog2i(n);
+  for (int m = 0; m < n; m++) work[bitrev_i(m, logn)] = x[m * stride];
+  for (int len = 2; len <= n; len <<= 1) {
+    int half = len / 2, tstep = n / len;
+    for (int g0 = 0; g0 < n; g0 += len)
+      for (int pos = 0; pos < half; pos++) {
+        c2 w = tw[pos * tstep];
+        if (inverse) w.im = -w.im;
+        c2 a = work[g0 + pos], b = work[g0 + pos + half], c;
+        c.re = b.re * w.re - b.im * w.im;
+        c.im = b.re * w.im + b.im * w.re;
+        work[g0 + pos].re = a.re + c.re;
+        work[g0 + pos].im = a.im + c.im;
+        work[g0 + pos + half].re = a.re - c.re;
+        work[g0 + pos + half].im = a.im - c.im;
+      }
+  }
+  for (int m = 0; m < n; m++) x[m * stride] = work[m];
+}
+
+/* Parallel cyclic reduction of one tridiagonal system (serial execution of
+   the parallel steps): a_k x_{k-h} + b_k x_k + c_k x_{k+h} = d_k. */
+static void pcr_solve(int nz, double *a, double *b, double *c, c2 *d, double *na, double *nb, double *nc, c2 *nd) {
+  for (int h = 1; h < nz; h <<= 1) {
+    for (int k = 0; k < nz; k++) {
+      double a2 = 0, c2v = 0, b2 = b[k];
+      c2 d2 = d[k];
+      if (k - h >= 0) {
+        double k1 = a[k] / b[k - h];
+        a2 = -a[k - h] * k1;
+        b2 = b2 - c[k - h] * k1;
+        d2.re = d2.re - d[k - h].re * k1;
+        d2.im = d2.im - d[k - h].im * k1;
+      }
+      if (k + h < nz) {
+        double k2 = c[k] / b[k + h];
+        c2v = -c[k + h] * k2;
+        b2 = b2 - a[k + h] * k2;
+        d2.re = d2.re - d[k + h].re * k2;
+        d2.im = d2.im - d[k + h].im * k2;
+      }
+      na[k] = a2;
+      nb[k] = b2;
+      nc[k] = c2v;
+      nd[k] = d2;
+    }
+    memcpy(a, na, nz * sizeof(double));
+    memcpy(b, nb, nz * sizeof(double));
+    memcpy(c, nc, nz * sizeof(double));
+    memcpy(d, nd, nz * sizeof(c2));
+  }
+  for (int k = 0; k < nz; k++) {
+    d[k].re = d[k].re / b[k];
+    d[k].im = d[k].im / b[k];
+  }
+}
+
+/* Kahan step: s += t with compensation cmp */
+static void kahan(double *s, double *cmp, double t) {
+  double y = t - *cmp, s2 = *s + y;
+  *cmp = (s2 - *s) - y;
+  *s = s2;
+}
+
+/* Direct solve; phi and rhs are interior arrays nx*ny*nz, x fastest.
+   The singular mode pins Phi_hat(k=1) = 0. */
+static void poisson_direct(oracle *o, const double *rhs, double *phi) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  if ((nx & (nx - 1)) || (ny & (ny - 1))) {
+    fprintf(stderr, "oracle: power-of-two nx, ny required by the FFT solver\n");
+    abort();
+  }
+  const int M = nx / 2 + 1;
+  long plane = (long)M * ny;
+  c2 *X = xcalloc(plane * nz, sizeof(c2));
+  c2 *twx = twiddles(nx), *twy = twiddles(ny);
+  int nmax = nx > ny ? nx : ny;
+  nmax = nmax > nz ? nmax : nz;
+  /* x transform of each real row, keep modes 0..nx/2 */
+#pragma omp parallel
+  {
+    c2 *work = xcalloc(nmax, sizeof(c2)), *line = xcalloc(nx, sizeof(c2));
+#pragma omp for collapse(2)
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++) {
+        for (int i = 0; i < nx; i++) {
+          line[i].re = rhs[i + (long)j * nx + (long)k * nx * ny];
+          line[i].im = 0;
+        }
+        fft_line(line, nx, 1, twx, 0, work);
+        for (int m = 0; m < M; m++) X[(long)k * plane + (long)j * M + m] = line[m];
+      }
+#pragma omp for collapse(2)
+    for (int k = 0; k < nz; k++)
+      for (int m = 0; m < M; m++) fft_line(X + (long)k * plane + m, ny, M, twy, 0, work);
+    free(work);
+    free(line);
+  }
+  /* z systems */
+  double *az = xcalloc(nz, sizeof(double)), *cz = xcalloc(nz, sizeof(double));
+  for (int k = 1; k <= nz; k++) {
+    az[k - 1] = k > 1 ? 1 / (C1D(o->dzW, k - 1) * C1D(o->dzPr, k)) : 0;
+    cz[k - 1] = k < nz ? 1 / (C1D(o->dzW, k) * C1D(o->dzPr, k)) : 0;
+  }
+  double *lamx = xcalloc(M, sizeof(double)), *lamy = xcalloc(ny, sizeof(double));
+  for (int m = 0; m < M; m++) {
+    double sv = sin(M_PI * m / nx);
+    lamx[m] = -4 * sv * sv / (o->dxmin * o->dxmin);
+  }
+  for (int j = 0; j < ny; j++) {
+    double sv = sin(M_PI * j / ny);
+    lamy[j] = -4 * sv * sv / (o->dymin * o->dymin);
+  }
+#pragma omp parallel
+  {
+    double *a = xcalloc(nz, sizeof(double)), *b = xcalloc(nz, sizeof(double)), *c = xcalloc(nz, sizeof(double));
+    double *na = xcalloc(nz, sizeof(double)), *nb = xcalloc(nz, sizeof(double)), *nc = xcalloc(nz, sizeof(double));
+    c2 *d = xcalloc(nz, sizeof(c2)), *nd = xcalloc(nz, sizeof(c2));
+#pragma omp for collapse(2)
+    for (int j = 0; j < ny; j++)
+      for (int m = 0; m < M; m++) {
+        for (int k = 0; k < nz; k++) d[k] = X[(long)k * plane + (long)j * M + m];
+        if (m == 0 && j == 0) {
+          /* horizontal-mean mode with x_1 = 0: G_{k-1} = G_k - d_k dzPr(k)
+             from G_nz = 0, then x_{k+1} = x_k + G_k dzW(k); compensated */
+          double Gr = 0, Gi = 0, cr = 0, ci = 0;
+          for (int k = nz; k >= 2; k--) {
+            double dz = C1D(o->dzPr, k);
+            kahan(&Gr, &cr, -d[k - 1].re * dz);
+            kahan(&Gi, &ci, -d[k - 1].im * dz);
+            a[k - 2] = Gr;
+            c[k - 2] = Gi;
+          }
+          double xr = 0, xi = 0;
+          cr = ci = 0;
+          d[0].re = 0;
+          d[0].im = 0;
+          for (int k = 1; k <= nz - 1; k++) {
+            double dz = C1D(o->dzW, k);
+            kahan(&xr, &cr, a[k - 1] * dz);
+            kahan(&xi, &ci, c[k - 1] * dz);
+            d[k].re = xr;
+            d[k].im = xi;
+          }
+        } else {
+          double lam = lamx[m] + lamy[j];
+          for (int k = 0; k < nz; k++) {
+            a[k] = az[k];
+            c[k] = cz[k];
+            b[k] = lam - a[k] - c[k];
+          }
+          pcr_solve(nz, a, b, c, d, na, nb, nc, nd);
+        }
+        for (int k = 0; k < nz; k++) X[(long)k * plane + (long)j * M + m] = d[k];
+      }
+    free(a); free(b); free(c); free(na); free(nb); free(nc); free(d); free(nd);
+  }
+  free(az); free(cz); free(lamx); free(lamy);
+  /* inverse y, then inverse x from the half spectrum (Hermitian symmetry) */
+  double scale = 1.0 / ((double)nx * ny);
+#pragma omp parallel
+  {
+    c2 *work = xcalloc(nmax, sizeof(c2)), *line = xcalloc(nx, sizeof(c2));
+#pragma omp for collapse(2)
+    for (int k = 0; k < nz; k++)
+      for (int m = 0; m < M; m++) fft_line(X + (long)k * plane + m, ny, M, twy, 1, work);
+#pragma omp for collapse(2)
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++) {
+        const c2 *row = X + (long)k * plane + (long)j * M;
+        for (int i = 0; i < nx; i++) {
+          if (i < M) {
+            line[i] = row[i];
+          } else {
+            line[i].re = row[nx - i].re;
+            line[i].im = -row[nx - i].im;
+          }
+        }
+        fft_line(line, nx, 1, twx, 1, work);
+        for (int i = 0; i < nx; i++) phi[i + (long)j * nx + (long)k * nx * ny] = line[i].re * scale;
+      }
+    free(work);
+    free(line);
+  }
+  free(X);
+  free(twx);
+  free(twy);
+}
+
+/* Residue_variable_z, src/poisson.f90:23-60 */
+static double residue(oracle *o, const double *Phi, const double *RHS) {
+  double Aw = 1 / (o->dxmin * o->dxmin), Ae = Aw, As = 1 / (o->dymin * o->dymin), An = As;
+  double res = 0;
+  for (int k = 1; k <= o->nz; k++) {
+    double Ab = 1 / ((C1D(o->zPr, k) - C1D(o->zPr, k - 1)) * (C1D(o->zW, k) - C1D(o->zW, k - 1)));
+    double At = 1 / ((C1D(o->zPr, k + 1) - C1D(o->zPr, k)) * (C1D(o->zW, k) - C1D(o->zW, k - 1)));
+    double Ap = Aw + Ae + As + An + Ab + At;
+    for (int j = 1; j <= o->ny; j++)
+      for (int i = 1; i <= o->nx; i++) {
+        double p = 0;
+        p = p + A3(o, Phi, i - 1, j, k) * Aw;
+        p = p + A3(o, Phi, i + 1, j, k) * Ae;
+        p = p + A3(o, Phi, i, j - 1, k) * As;
+        p = p + A3(o, Phi, i, j + 1, k) * An;
+        p = p + A3(o, Phi, i, j, k - 1) * Ab;
+        p = p + A3(o, Phi, i, j, k + 1) * At;
+        p = p - A3(o, RHS, i, j, k);
+        p = fabs(-p + Ap * A3(o, Phi, i, j, k));
+        if (p > res) res = p;
+      }
+  }
+  return res;
+}
+
+/* Sum in the device's order: blocks of 256 values, each the in-order sum of
+   four 64-lane xor-butterfly sums; block sums added in order. */
+static double tree_sum(const double *v, long n) {
+  double total = 0;
+  for (long b0 = 0; b0 < n; b0 += 256) {
+    double blk = 0;
+    for (int w = 0; w < 4; w++) {
+      double lane[64];
+      for (int l = 0; l < 64; l++) {
+        long t = b0 + 64 * w + l;
+        lane[l] = t < n ? v[t] : 0.0;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        double nl[64];
+        for (int l = 0; l < 64; l++) nl[l] = lane[l] + lane[l ^ off];
+        memcpy(lane, nl, sizeof lane);
+      }
+      blk += lane[0];
+    }
+    total += blk;
+  }
+  return total;
+}
+
+/* ------------------------------------------------ pressure correction --- */
+/* PressureCorrection -> PrePoisson -> solver -> Bound_Phi -> PostPoisson,
+   src/pressure.f90:169-266, 272-723, 728-991, 994-1229 */
+static void pressure_correction(oracle *o, double *U, double *V, double *W, int use_q, double coef) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  double dt2 = coef;
+  double dt2_rec = 1.0 / dt2;
+  bound_uvw(o, U, V, W);
+  /* zero boundary-face velocity inside solids on open sides (pressure.f90:301-342) */
+  if (o->c.btype[To] != NOSLIP && o->c.btype[To] != PERIODIC)
+    for (int j = 1; j <= o->Wn[1]; j++)
+      for (int i = 1; i <= o->Wn[0]; i++)
+        if (A3(o, o->Wtype, i, j, nz) > 0) A3(o, W, i, j, nz) = 0;
+  if (o->c.btype[Bo] != NOSLIP && o->c.btype[Bo] != PERIODIC)
+    for (int j = 1; j <= o->Wn[1]; j++)
+      for (int i = 1; i <= o->Wn[0]; i++)
+        if (A3(o, o->Wtype, i, j, 0) > 0) A3(o, W, i, j, 0) = 0;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double dz = o->variable_z ? C1D(o->dzPr, k) : o->dzmin;
+        double r = (A3(o, U, i, j, k) - A3(o, U, i - 1, j, k)) / o->dxmin +
+                   (A3(o, V, i, j, k) - A3(o, V, i, j - 1, k)) / o->dymin +
+                   (A3(o, W, i, j, k) - A3(o, W, i, j, k - 1)) / dz;
+        if (use_q) r = r - A3(o, o->Q, i, j, k);
+        A3(o, o->RHS, i, j, k) = r * dt2_rec;
+      }
+  long plane = (long)nx * ny;
+  double *rhs = xcalloc(plane * nz, sizeof(double)), *phi = xcalloc(plane * nz, sizeof(double));
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) rhs[(i - 1) + (j - 1) * nx + (k - 1) * plane] = A3(o, o->RHS, i, j, k);
+  poisson_direct(o, rhs, phi);
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) A3(o, o->Phi, i, j, k) = phi[(i - 1) + (j - 1) * nx + (k - 1) * plane];
+  free(rhs);
+  free(phi);
+  bound_phi(o, o->Phi);
+  o->last_res = residue(o, o->Phi, o->RHS);
+  /* PostPoisson (2nd order) */
+  double Au = dt2 / o->dxmin, Av = dt2 / o->dymin, Aw = dt2 / o->dzmin;
+  const double *Phi = o->Phi;
+  for (int k = 1; k <= o->Un[2]; k++)
+    for (int j = 1; j <= o->Un[1]; j++)
+      for (int i = 1; i <= o->Un[0]; i++)
+        A3(o, U, i, j, k) = A3(o, U, i, j, k) - Au * (A3(o, Phi, i + 1, j, k) - A3(o, Phi, i, j, k));
+  for (int k = 1; k <= o->Vn[2]; k++)
+    for (int j = 1; j <= o->Vn[1]; j++)
+      for (int i = 1; i <= o->Vn[0]; i++)
+        A3(o, V, i, j, k) = A3(o, V, i, j, k) - Av * (A3(o, Phi, i, j + 1, k) - A3(o, Phi, i, j, k));
+  for (int k = 1; k <= o->Wn[2]; k++)
+    for (int j = 1; j <= o->Wn[1]; j++)
+      for (int i = 1; i <= o->Wn[0]; i++) {
+        if (o->variable_z)
+          A3(o, W, i, j, k) =
+              A3(o, W, i, j, k) - dt2 * (A3(o, Phi, i, j, k + 1) - A3(o, Phi, i, j, k)) / C1D(o->dzW, k);
+        else
+          A3(o, W, i, j, k) = A3(o, W, i, j, k) - Aw * (A3(o, Phi, i, j, k + 1) - A3(o, Phi, i, j, k));
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) A3(o, o->Pr, i, j, k) = A3(o, o->Pr, i, j, k) + A3(o, Phi, i, j, k);
+  /* FixPressureToReference: mean of the top layer (reference boundary To).
+     The reference sums with an OpenMP reduction whose order is unspecified;
+     the order used here is the device's: blocks of 256 consecutive cells
+     (i fastest), 64-lane butterfly sums per wavefront, wavefront sums in
+     order, block sums in order. */
+  double *top = xcalloc((long)nx * ny, sizeof(double));
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) top[(i - 1) + (long)(j - 1) * nx] = A3(o, o->Pr, i, j, nz);
+  double ref = tree_sum(top, (long)nx * ny);
+  free(top);
+  ref = ref / (double)(nx * ny);
+  if (ref != 0)
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) A3(o, o->Pr, i, j, k) = A3(o, o->Pr, i, j, k) - ref;
+  bound_pr(o, o->Pr);
+  /* check_divergence (pressure.f90:884-989) computes on refreshed ghosts;
+     the diagnostic works on copies so the state keeps the reference's
+     (stale) ghosts, which the next SubgridModel call reads at init */
+  double *Uc = xcalloc(o->ntot, sizeof(double)), *Vc = xcalloc(o->ntot, sizeof(double)),
+         *Wc = xcalloc(o->ntot, sizeof(double));
+  memcpy(Uc, U, o->ntot * sizeof(double));
+  memcpy(Vc, V, o->ntot * sizeof(double));
+  memcpy(Wc, W, o->ntot * sizeof(double));
+  bound_uvw(o, Uc, Vc, Wc);
+  double S = 0;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double dz = o->variable_z ? C1D(o->dzPr, k) : o->dzmin;
+        double p = (A3(o, Uc, i, j, k) - A3(o, Uc, i - 1, j, k)) / (o->dxmin) +
+                   (A3(o, Vc, i, j, k) - A3(o, Vc, i, j - 1, k)) / (o->dymin) +
+                   (A3(o, Wc, i, j, k) - A3(o, Wc, i, j, k - 1)) / (dz);
+        if (use_q) p = p - A3(o, o->Q, i, j, k);
+        if (fabs(p) > S) S = fabs(p);
+      }
+  free(Uc);
+  free(Vc);
+  free(Wc);
+  o->last_div = S;
+}
+
+/* TimeStepLength with a constant dt and optional CFL check,
+   src/dynamics.f90:511-600 */
+static double cfl_max(oracle *o) {
+  double m = 0;
+  for (int k = 1; k <= o->nz; k++)
+    for (int j = 1; j <= o->ny; j++)
+      for (int i = 1; i <= o->nx; i++) {
+        double a1 = fabs(A3(o, o->U, i, j, k)), a0 = fabs(A3(o, o->U, i - 1, j, k));
+        double b1 = fabs(A3(o, o->V, i, j, k)), b0 = fabs(A3(o, o->V, i, j - 1, k));
+        double c1 = fabs(A3(o, o->W, i, j, k)), c0 = fabs(A3(o, o->W, i, j, k - 1));
+        double dzk = o->variable_z ? C1D(o->dzPr, k) : o->dzmin;
+        double dxk = o->variable_z ? C1D(o->dxPr, i) : o->dxmin;
+        double dyk = o->variable_z ? C1D(o->dyPr, j) : o->dymin;
+        double p = (a1 > a0 ? a1 : a0) / dxk;
+        p = p + (b1 > b0 ? b1 : b0) / dyk;
+        p = p + (c1 > c0 ? c1 : c0) / dzk;
+        if (p != p) return INFINITY;
+        if (p > m) m = p;
+      }
+  return m;
+}
+
+/* ----------------------------------------------------- public driver --- */
+int oracle_init(oracle *o) {
+  /* InitialConditions tail, src/initial.f90:4284-4342 */
+  bound_uvw(o, o->U, o->V, o->W);
+  bound_pr(o, o->Pr);
+  pressure_correction(o, o->U, o->V, o->W, 0, 1.0);
+  sgs_smag(o);
+  bound_visc(o, o->Visc);
+  if (o->c.nscalars > 0) {
+    for (int k = 1; k <= o->nz; k++)
+      for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++)
+          A3(o, o->TDiff, i, j, k) = (A3(o, o->Visc, i, j, k) - o->c.nu) / (double)0.6f + o->c.nu_scalar;
+    bound_visc(o, o->TDiff);
+  }
+  for (int s = 0; s < o->c.nscalars; s++) bound_scalar(o, o->S[s]);
+  wm_uvw_fluxes(o);
+  bound_visc(o, o->Visc);
+  o->initialized = 1;
+  return 0;
+}
+
+static void swap(double **a, double **b) {
+  double *t = *a;
+  *a = *b;
+  *b = t;
+}
+
+int oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag) {
+  if (!o->initialized) {
+    set_err("oracle_step before oracle_init");
+    return ELMM_E_STATE;
+  }
+  double dt = o->c.dt;
+  double cfl = 0;
+  for (int s = 0; s < nsteps; s++) {
+    if (!o->first_step_done) {
+      /* first call of TMarchRK3, src/tsteps.f90:54-77 */
+      memcpy(o->U2, o->U, o->ntot * sizeof(double));
+      memcpy(o->V2, o->V, o->ntot * sizeof(double));
+      memcpy(o->W2, o->W, o->ntot * sizeof(double));
+      memcpy(o->Us, o->U, o->ntot * sizeof(double));
+      memcpy(o->Vs, o->V, o->ntot * sizeof(double));
+      memcpy(o->Ws, o->W, o->ntot * sizeof(double));
+      bound_uvw(o, o->U, o->V, o->W);
+      ib_momentum(o, o->U, o->V, o->W);
+      o->first_step_done = 1;
+    }
+    if (o->c.cfl_check) {
+      double m = cfl_max(o);
+      cfl = m * dt;
+      if (o->c.cfl_max > 0 && !(cfl <= o->c.cfl_max)) {
+        set_err("Solution diverged. 2");
+        return ELMM_E_DIVERGED;
+      }
+    }
+    for (int st = 0; st < 3; st++) {
+      double coef = 2 * RK_alpha[st] * dt;
+      subgrid_stresses(o);
+      convection(o, st, dt);
+      scalar_rk3(o, st, dt);
+      other_terms(o, coef);
+      bound_uvw(o, o->U2, o->V2, o->W2);
+      ib_momentum(o, o->U2, o->V2, o->W2);
+      bound_uvw(o, o->U2, o->V2, o->W2);
+      int use_q = o->c.ib_mass_sources != 0;
+      if (use_q) ib_mass_sources(o, o->U2, o->V2, o->W2);
+      pressure_correction(o, o->U2, o->V2, o->W2, use_q, coef);
+      swap(&o->U, &o->U2);
+      swap(&o->V, &o->V2);
+      swap(&o->W, &o->W2);
+      null_interior(o);
+      bound_uvw(o, o->U, o->V, o->W);
+      ib_momentum(o, o->U, o->V, o->W);
+      bound_uvw(o, o->U, o->V, o->W);
+    }
+  }
+  if (diag) {
+    memset(diag, 0, sizeof *diag);
+    diag->cfl = cfl;
+    diag->poisson_res = o->last_res;
+    diag->max_div = o->last_div;
+    diag->steps_done = nsteps;
+  }
+  return 0;
+}
+
+/* max |div U - Q| as in PostPoisson check_divergence (2nd order),
+   src/pressure.f90:937-981 (without the Q term: the state after a step) */
+double oracle_divergence(oracle *o) {
+  double S = 0;
+  for (int k = 1; k <= o->nz; k++)
+    for (int j = 1; j <= o->ny; j++)
+      for (int i = 1; i <= o->nx; i++) {
+        double dz = o->variable_z ? C1D(o->dzPr, k) : o->dzmin;
+        double p = (A3(o, o->U, i, j, k) - A3(o, o->U, i - 1, j, k)) / (o->dxmin) +
+                   (A3(o, o->V, i, j, k) - A3(o, o->V, i, j - 1, k)) / (o->dymin) +
+                   (A3(o, o->W, i, j, k) - A3(o, o->W, i, j, k - 1)) / (dz);
+        if (o->c.ib_mass_sources && o->first_step_done) p = p - A3(o, o->Q, i, j, k);
+        if (fabs(p) > S) S = fabs(p);
+      }
+  return S;
+}
+
+/* ------------------------------------------------------ operator hooks --- */
+void oracle_op_bound_uvw(oracle *o) { bound_uvw(o, o->U, o->V, o->W); }
+void oracle_op_advection(oracle *o) {
+  double *S[3] = {o->Us, o->Vs, o->Ws};
+  const int *N[3] = {o->Un, o->Vn, o->Wn};
+  for (int c = 0; c < 3; c++) {
+    for (long p = 0; p < o->ntot; p++) S[c][p] = 0;
+    for (int k = 1; k <= N[c][2]; k++)
+      for (int j = 1; j <= N[c][1]; j++)
+        for (int i = 1; i <= N[c][0]; i++)
+          A3(o, S[c], i, j, k) = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
+  }
+}
+void oracle_op_diffusion(oracle *o) {
+  for (long p = 0; p < o->ntot; p++) o->Us[p] = o->Vs[p] = o->Ws[p] = 0;
+  momentum_diffusion(o);
+}
+void oracle_op_kappa(oracle *o, int32_t sc) { kappa_scalar(o, o->Sadv[sc], o->S[sc]); }
+void oracle_op_sgs(oracle *o) { subgrid_stresses(o); }
+void oracle_op_poisson(oracle *o, const double *rhs, double *phi) { poisson_direct(o, rhs, phi); }
+double oracle_op_residual(oracle *o, const double *phi, const double *rhs) { return residue(o, phi, rhs); }

@@ -1,0 +1,53 @@
+/*
+ * elmm_oracle.h - CPU FP64 restatement of ELMM's LES time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the correctness oracle for the HIP
+ * library: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker or the timed CPU baseline.  The
+ * product path never links or calls it.
+ *
+ * The functions restate the reference's Fortran procedures in their
+ * operation order (each function cites the file:line it follows), compiled
+ * with -ffp-contract=off so that no multiply-add is fused.  Scatter loops that
+ * the reference runs under OpenMP with atomics or races are executed in the
+ * single-thread order (OMP_NUM_THREADS=1 semantics).
+ *
+ * Parity pins: see tests/test_oracle_*.py.  Functions without an independent
+ * pin are listed as "parity unpinned" in oracle/README.md.
+ */
+#ifndef ELMM_ORACLE_H
+#define ELMM_ORACLE_H
+
+#include "../include/elmm_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle oracle;
+
+oracle *oracle_create(const elmm_config *cfg, const elmm_lists *lists);
+void    oracle_destroy(oracle *o);
+int     oracle_upload(oracle *o, int32_t field, int32_t lb, const int32_t ext[3], const double *src);
+int     oracle_download(oracle *o, int32_t field, int32_t lb, const int32_t ext[3], double *dst);
+int     oracle_init(oracle *o);
+int     oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag);
+double  oracle_divergence(oracle *o);
+const char *oracle_last_error(void);
+
+/* Oracle-only field ids (intermediate arrays). */
+enum { ORACLE_F_USTAR = 8, ORACLE_F_VSTAR = 9, ORACLE_F_WSTAR = 10, ORACLE_F_RHS = 11, ORACLE_F_SADV = 32 };
+
+/* Single operators on the current state, for operator-level tests. */
+void oracle_op_bound_uvw(oracle *o);           /* BoundUVW(U,V,W)              */
+void oracle_op_advection(oracle *o);           /* CD4div* -> Ustar,Vstar,Wstar */
+void oracle_op_diffusion(oracle *o);           /* diffusion incl. WM -> Ustar.. */
+void oracle_op_kappa(oracle *o, int32_t sc);   /* KappaScalar -> Sadv(sc)       */
+void oracle_op_sgs(oracle *o);                 /* SubgridStresses              */
+void oracle_op_poisson(oracle *o, const double *rhs_interior, double *phi_interior);
+double oracle_op_residual(oracle *o, const double *phi_ghosted, const double *rhs_ghosted);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

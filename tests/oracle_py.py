@@ -1,0 +1,125 @@
+"""ctypes wrapper of the CPU oracle (oracle/lib/libelmm_oracle.so).
+
+Test infrastructure: used only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed baseline.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "lib", "libelmm_oracle.so")
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        import sys
+        sys.path.insert(0, os.path.join(ROOT, "gpu-elmm_amd"))
+        from elmm_amd import capi
+        lib = C.CDLL(ORACLE_LIB)
+        pd = C.POINTER(C.c_double)
+        lib.oracle_create.restype = C.c_void_p
+        lib.oracle_create.argtypes = [C.POINTER(capi.Config), C.POINTER(capi.Lists)]
+        lib.oracle_destroy.argtypes = [C.c_void_p]
+        lib.oracle_upload.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), pd]
+        lib.oracle_download.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), pd]
+        lib.oracle_init.argtypes = [C.c_void_p]
+        lib.oracle_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(capi.Diag)]
+        lib.oracle_divergence.argtypes = [C.c_void_p]
+        lib.oracle_divergence.restype = C.c_double
+        lib.oracle_last_error.restype = C.c_char_p
+        lib.oracle_op_bound_uvw.argtypes = [C.c_void_p]
+        lib.oracle_op_sgs.argtypes = [C.c_void_p]
+        lib.oracle_op_advection.argtypes = [C.c_void_p]
+        lib.oracle_op_diffusion.argtypes = [C.c_void_p]
+        lib.oracle_op_kappa.argtypes = [C.c_void_p, C.c_int32]
+        lib.oracle_op_poisson.argtypes = [C.c_void_p, pd, pd]
+        lib.oracle_op_residual.argtypes = [C.c_void_p, pd, pd]
+        lib.oracle_op_residual.restype = C.c_double
+        _lib = lib
+    return _lib
+
+
+F_USTAR, F_VSTAR, F_WSTAR, F_RHS, F_SADV = 8, 9, 10, 11, 32
+
+
+class Oracle:
+    def __init__(self, cfg, case):
+        lib = _load()
+        self.cfg = cfg
+        self.case = case
+        self._h = lib.oracle_create(C.byref(cfg), C.byref(case.view))
+        if not self._h:
+            raise RuntimeError("oracle_create: " + lib.oracle_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.oracle_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def shape(self):
+        return (self.cfg.nx + 6, self.cfg.ny + 6, self.cfg.nz + 6)
+
+    def upload(self, field, arr, lb=-2):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        ext = (C.c_int32 * 3)(*a.shape)
+        if _lib.oracle_upload(self._h, field, lb, ext, a.ctypes.data_as(C.POINTER(C.c_double))):
+            raise RuntimeError(_lib.oracle_last_error().decode())
+
+    def download(self, field, lb=-2, shape=None):
+        shape = shape or self.shape()
+        a = np.zeros(shape, dtype=np.float64, order="F")
+        ext = (C.c_int32 * 3)(*shape)
+        if _lib.oracle_download(self._h, field, lb, ext, a.ctypes.data_as(C.POINTER(C.c_double))):
+            raise RuntimeError(_lib.oracle_last_error().decode())
+        return a
+
+    def init(self):
+        if _lib.oracle_init(self._h):
+            raise RuntimeError(_lib.oracle_last_error().decode())
+
+    def step(self, n=1):
+        from elmm_amd import capi
+        d = capi.Diag()
+        rc = _lib.oracle_step(self._h, n, C.byref(d))
+        if rc:
+            raise RuntimeError(_lib.oracle_last_error().decode())
+        return d
+
+    def divergence(self):
+        return _lib.oracle_divergence(self._h)
+
+    def op_bound_uvw(self):
+        _lib.oracle_op_bound_uvw(self._h)
+
+    def op_sgs(self):
+        _lib.oracle_op_sgs(self._h)
+
+    def op_advection(self):
+        _lib.oracle_op_advection(self._h)
+
+    def op_diffusion(self):
+        _lib.oracle_op_diffusion(self._h)
+
+    def op_kappa(self, sc=0):
+        _lib.oracle_op_kappa(self._h, sc)
+
+    def op_poisson(self, rhs):
+        """Direct solve of the FD2 Poisson problem; rhs, result: (nx,ny,nz) Fortran order."""
+        r = np.asfortranarray(rhs, dtype=np.float64)
+        out = np.zeros_like(r, order="F")
+        pd = C.POINTER(C.c_double)
+        _lib.oracle_op_poisson(self._h, r.ctypes.data_as(pd), out.ctypes.data_as(pd))
+        return out
+
+    def op_residual(self, phi_ghosted, rhs_ghosted):
+        pd = C.POINTER(C.c_double)
+        p = np.asfortranarray(phi_ghosted, dtype=np.float64)
+        r = np.asfortranarray(rhs_ghosted, dtype=np.float64)
+        return _lib.oracle_op_residual(self._h, p.ctypes.data_as(pd), r.ctypes.data_as(pd))
