@@ -14,6 +14,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -144,6 +145,49 @@ struct Engine {
   bool initialized = false, first_step_done = false;
   double last_div = 0, last_cfl = 0;
   bool want_diag = false;
+  // kernel timers (HIP events on the stream, read after each sync)
+  bool timers_on = false;
+  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> event_pool;
+  double tms[ELMM_T_NGROUPS] = {};
+  int64_t tlaunch[ELMM_T_NGROUPS] = {};
+
+  hipEvent_t get_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  // Runs f (which enqueues exactly one kernel) between two events when the
+  // timers are on.
+  template <typename F> void timed(int grp, F &&f) {
+    if (!timers_on) {
+      f();
+      return;
+    }
+    hipEvent_t a = get_event(), b = get_event();
+    HIPCHK(hipEventRecord(a, stream));
+    f();
+    HIPCHK(hipEventRecord(b, stream));
+    pending.emplace_back(grp, a, b);
+  }
+  void collect_timers() {
+    for (auto &t : pending) {
+      hipEvent_t a = std::get<1>(t), b = std::get<2>(t);
+      HIPCHK(hipEventSynchronize(b));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      tms[std::get<0>(t)] += ms;
+      tlaunch[std::get<0>(t)] += 1;
+      event_pool.push_back(a);
+      event_pool.push_back(b);
+    }
+    pending.clear();
+  }
 
   IBDev ibdev(int c) const {
     return IBDev{ib_n[c], ib_tgt[c].as<int32_t>(), ib_ptr[c].as<int32_t>(), ib_src[c].as<int32_t>(),
@@ -166,7 +210,9 @@ struct Engine {
   void setup_lists(const elmm_lists &L);
   void setup_poisson();
   void xfer(int field, int lb, const int32_t ext[3], double *host, bool up);
-  void bound_uvw(double *a, double *b, double *c) { launch_bound_uvw<double>(g, a, b, c, stream); }
+  void bound_uvw(double *a, double *b, double *c) {
+    timed(ELMM_T_GHOST, [&] { launch_bound_uvw<double>(g, a, b, c, stream); });
+  }
   void ib_momentum(double *a, double *b, double *c);
   void pressure_correction(double *a, double *b, double *c, bool use_q, double coef);
   void subgrid_stresses(bool cfl);
@@ -275,7 +321,7 @@ void Engine::setup_poisson() {
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
   cfl_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
-  partial.alloc(std::max(top_blocks, 1) * sizeof(double));
+  partial.alloc((std::max(top_blocks, 1) + 1) * sizeof(double));
   div_partial.alloc(std::max(red_blocks, 1) * sizeof(double));
   cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));
 }
@@ -620,8 +666,8 @@ void Engine::ib_momentum(double *a, double *b, double *c) {
   double *F[3] = {a, b, c};
   for (int comp = 0; comp < 3; comp++) {
     if (!ib_n[comp]) continue;
-    launch_ib_interp(g, ibdev(comp), F[comp], ib_tmp.as<double>(), stream);
-    launch_ib_scatter(g, ibdev(comp), ib_tmp.as<double>(), F[comp], stream);
+    timed(ELMM_T_IB, [&] { launch_ib_interp(g, ibdev(comp), F[comp], ib_tmp.as<double>(), stream); });
+    timed(ELMM_T_IB, [&] { launch_ib_scatter(g, ibdev(comp), ib_tmp.as<double>(), F[comp], stream); });
   }
 }
 
@@ -646,24 +692,36 @@ double Engine::max_div(double *a, double *b, double *c, bool use_q) {
 // the normal ghost is already 0 for every supported boundary type.
 void Engine::pressure_correction(double *a, double *b, double *c, bool use_q, double coef) {
   const double dt2 = coef, dt2_rec = 1.0 / dt2;
-  launch_poisson_fwd(g, pp, a, b, c, use_q ? Q.as<double>() : nullptr, nullptr, dt2_rec, nullptr, stream);
-  launch_poisson_solve_z(g, pp, stream);
-  launch_poisson_inv(g, pp, Phi.as<double>(), stream);
-  launch_top_sum(g, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, stream);
-  launch_projection(g, a, b, c, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, dt2, stream);
-  launch_bound_pr<double>(g, Pr.as<double>(), stream);
+  double *q = use_q ? Q.as<double>() : nullptr;
+  timed(ELMM_T_POISSON_X_FWD, [&] { launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, dt2_rec, nullptr, stream); });
+  timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(pp, 0, stream); });
+  timed(ELMM_T_POISSON_Z, [&] { launch_poisson_solve_z(g, pp, stream); });
+  timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(pp, 1, stream); });
+  timed(ELMM_T_POISSON_X_INV, [&] { launch_fft_x_inv(g, pp, Phi.as<double>(), stream); });
+  timed(ELMM_T_REDUCE, [&] {
+    launch_top_sum(g, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, stream);
+  });
+  timed(ELMM_T_PROJECTION, [&] {
+    launch_projection(g, a, b, c, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, dt2, stream);
+  });
+  timed(ELMM_T_GHOST, [&] { launch_bound_pr<double>(g, Pr.as<double>(), stream); });
 }
 
 // SubgridStresses, src/dynamics.f90:869-908 (ComputeViscsWM only writes
 // values that SubgridModel overwrites plus the diagnostic WMPoints%ustar;
 // BoundViscosity(TDiff) is idempotent on the frozen TDiff and done at init).
 void Engine::subgrid_stresses(bool cfl) {
-  launch_sgs(g, u, v, w, Visc.as<double>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
-             have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model, cfl ? cfl_partial.as<double>() : nullptr,
-             cfl_blocks, stream);
-  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
-  launch_bound_visc<double>(g, Visc.as<double>(), stream);
-  if (ib_n[3]) launch_ib_visc_serial(g, ibdev(3), Visc.as<double>(), visc_levels.data(), (int)visc_levels.size() - 1, stream);
+  timed(ELMM_T_SGS, [&] {
+    launch_sgs(g, u, v, w, Visc.as<double>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
+               have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model,
+               cfl ? cfl_partial.as<double>() : nullptr, cfl_blocks, stream);
+  });
+  if (cfg.wall_model && wm_n) timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream); });
+  timed(ELMM_T_GHOST, [&] { launch_bound_visc<double>(g, Visc.as<double>(), stream); });
+  if (ib_n[3])
+    timed(ELMM_T_IB, [&] {
+      launch_ib_visc_serial(g, ibdev(3), Visc.as<double>(), visc_levels.data(), (int)visc_levels.size() - 1, stream);
+    });
 }
 
 void Engine::stage(int st, bool last) {
@@ -687,23 +745,38 @@ void Engine::stage(int st, bool last) {
   double *ustar[3] = {Us.as<double>(), Vs.as<double>(), Ws.as<double>()};
   double *U2p[3] = {u2, v2, w2};
   for (int c = 0; c < 3; c++) {
-    launch_momentum(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c], code[c].as<uint8_t>(), sc,
-                    stream);
-    launch_momentum_corr(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c],
-                         code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(), stream);
+    timed(ELMM_T_MOMENTUM, [&] {
+      launch_momentum(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c], code[c].as<uint8_t>(),
+                      sc, stream);
+    });
+    if (mom_terms[c].n)
+      timed(ELMM_T_MOMENTUM_CORR, [&] {
+        launch_momentum_corr(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c],
+                             code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(), stream);
+      });
   }
   if (nsc > 0) {
     // ScalarRK3 (src/scalars.f90:88-299) with the stage-start velocity;
     // the stage-start BoundScalar repeats the previous stage's final fill.
-    launch_wm_div(g, WMScalarPoints{wmp_n, wmp_ijk.as<int32_t>(), wmp_div.as<double>()}, prtype.as<int32_t>(), u, v,
-                  w, stream);
+    if (wmp_n)
+      timed(ELMM_T_WALL, [&] {
+        launch_wm_div(g, WMScalarPoints{wmp_n, wmp_ijk.as<int32_t>(), wmp_div.as<double>()}, prtype.as<int32_t>(), u,
+                      v, w, stream);
+      });
     for (int s = 0; s < nsc; s++) {
-      launch_scalar(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(), code_s.as<uint8_t>(),
-                    mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, stream);
-      launch_scalar_corr(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(),
-                         code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(),
-                         sc, sc_terms[s]->view(), src_flux.as<double>(), wmp_div.as<double>(), stream);
-      launch_bound_scalar<double>(g, s_nxt[s], stream);
+      timed(ELMM_T_SCALAR, [&] {
+        launch_scalar(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(),
+                      code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc,
+                      stream);
+      });
+      if (sc_terms[s]->n)
+        timed(ELMM_T_SCALAR_CORR, [&] {
+          launch_scalar_corr(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(),
+                             code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(),
+                             mask[2].as<uint8_t>(), sc, sc_terms[s]->view(), src_flux.as<double>(),
+                             wmp_div.as<double>(), stream);
+        });
+      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<double>(g, s_nxt[s], stream); });
       std::swap(s_cur[s], s_nxt[s]);
     }
   }
@@ -713,7 +786,7 @@ void Engine::stage(int st, bool last) {
     ib_momentum(u2, v2, w2);
     bound_uvw(u2, v2, w2);
   }
-  if (use_q) launch_mass_sources(g, q_terms.view(), u2, v2, w2, Q.as<double>(), stream);
+  if (use_q) timed(ELMM_T_IB, [&] { launch_mass_sources(g, q_terms.view(), u2, v2, w2, Q.as<double>(), stream); });
   pressure_correction(u2, v2, w2, use_q, sc.coef);
   if (last && want_diag) {
     bound_uvw(u2, v2, w2);
@@ -722,7 +795,9 @@ void Engine::stage(int st, bool last) {
   std::swap(u, u2);
   std::swap(v, v2);
   std::swap(w, w2);
-  for (int c = 0; c < 3; c++) launch_null(g, null_n[c], null_ijk[c].as<int32_t>(), c == 0 ? u : c == 1 ? v : w, stream);
+  for (int c = 0; c < 3; c++)
+    if (null_n[c])
+      timed(ELMM_T_IB, [&] { launch_null(g, null_n[c], null_ijk[c].as<int32_t>(), c == 0 ? u : c == 1 ? v : w, stream); });
   bound_uvw(u, v, w);
   if (has_ib()) {
     ib_momentum(u, v, w);
@@ -763,6 +838,7 @@ void Engine::step(int nsteps) {
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
+  collect_timers();
 }
 
 }  // namespace elmm
@@ -869,10 +945,47 @@ extern "C" int elmm_divergence(elmm_handle *h, double *max_div) {
   });
 }
 
+extern "C" int elmm_timers_enable(elmm_handle *h, int32_t enable) {
+  if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  return guarded([&] {
+    HIPCHK(hipStreamSynchronize(h->e.stream));
+    h->e.collect_timers();
+    h->e.timers_on = enable != 0;
+  });
+}
+
+extern "C" int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset) {
+  if (!h || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_timers_read");
+  return guarded([&] {
+    HIPCHK(hipStreamSynchronize(h->e.stream));
+    h->e.collect_timers();
+    for (int g = 0; g < ELMM_T_NGROUPS; g++) {
+      out->ms[g] = h->e.tms[g];
+      out->launches[g] = h->e.tlaunch[g];
+      if (reset) {
+        h->e.tms[g] = 0;
+        h->e.tlaunch[g] = 0;
+      }
+    }
+  });
+}
+
+extern "C" const char *elmm_timer_name(int32_t g) {
+  static const char *names[ELMM_T_NGROUPS] = {
+      "sgs", "wall_model", "momentum", "momentum_corr", "scalar", "scalar_corr", "ghost", "immersed_boundary",
+      "poisson_x_fwd", "poisson_y", "poisson_z", "poisson_x_inv", "projection", "reduce", "other", "unused"};
+  return (g >= 0 && g < ELMM_T_NGROUPS) ? names[g] : "invalid";
+}
+
 extern "C" int elmm_destroy(elmm_handle *h) {
   if (!h) return 0;
   if (h->e.stream) {
     (void)hipStreamSynchronize(h->e.stream);
+    for (auto &t : h->e.pending) {
+      (void)hipEventDestroy(std::get<1>(t));
+      (void)hipEventDestroy(std::get<2>(t));
+    }
+    for (auto e : h->e.event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->e.stream);
   }
   delete h;

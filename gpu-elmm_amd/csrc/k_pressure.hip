@@ -276,16 +276,19 @@ static int ilog2(int n) {
 
 static int rows_for(int nx) { return std::max(1, 1024 / nx); }
 
-void launch_poisson_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
-                        const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
+void launch_fft_x_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
+                      const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
   size_t sh = (size_t)rows * pp.nx * sizeof(double2);
   hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, dt2_rec,
                      rhs_out, rows, ilog2(pp.nx));
+}
+
+void launch_fft_y(const PoissonPlan &pp, int inverse, hipStream_t st) {
   constexpr int TX = 8;
   hipLaunchKernelGGL(k_fft_y<TX>, dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(double2), st,
-                     pp, ilog2(pp.ny), 0);
+                     pp, ilog2(pp.ny), inverse);
 }
 
 void launch_poisson_solve_z(const Geo &, const PoissonPlan &pp, hipStream_t st) {
@@ -294,10 +297,7 @@ void launch_poisson_solve_z(const Geo &, const PoissonPlan &pp, hipStream_t st) 
   hipLaunchKernelGGL(k_solve_z<TX>, dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
 }
 
-void launch_poisson_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st) {
-  constexpr int TX = 8;
-  hipLaunchKernelGGL(k_fft_y<TX>, dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(double2), st,
-                     pp, ilog2(pp.ny), 1);
+void launch_fft_x_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
   size_t sh = (size_t)rows * pp.nx * sizeof(double2);
@@ -345,9 +345,18 @@ __global__ void k_top_sum(Geo g, const double *__restrict__ Pr, const double *__
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Block sums combined in block order by one thread: partial[nblocks] = mean.
+__global__ void k_top_final(Geo g, double *__restrict__ partial, int nblocks) {
+  if (threadIdx.x != 0) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; b++) s += partial[b];
+  partial[nblocks] = s / (double)(g.nx * g.ny);
+}
+
 void launch_top_sum(const Geo &g, const double *Pr, const double *Phi, double *partial, int nblocks,
                     hipStream_t st) {
   hipLaunchKernelGGL(k_top_sum, dim3(nblocks), dim3(256), 0, st, g, Pr, Phi, partial);
+  hipLaunchKernelGGL(k_top_final, dim3(1), dim3(64), 0, st, g, partial, nblocks);
 }
 
 // PostPoisson (2nd order) + Pr = Pr + Phi - ref:
@@ -357,14 +366,7 @@ __global__ void __launch_bounds__(256)
     k_projection(Geo g, double *__restrict__ U, double *__restrict__ V, double *__restrict__ W,
                  double *__restrict__ Pr, const double *__restrict__ Phi, const double *__restrict__ partial,
                  int nblocks, double dt2) {
-  __shared__ double ref_s;
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    double s = 0;
-    for (int b = 0; b < nblocks; b++) s += partial[b];
-    ref_s = s / (double)(g.nx * g.ny);
-  }
-  __syncthreads();
-  const double ref = ref_s;
+  const double ref = partial[nblocks];
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int k = blockIdx.z + 1;

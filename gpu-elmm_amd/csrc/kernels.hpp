@@ -84,10 +84,11 @@ struct PoissonPlan {
   const double *az, *cz;        // z coupling coefficients per k (0-based)
   const double *dzPr, *dzW;     // grid spacings indexed like the reference (k = 1..nz)
 };
-void launch_poisson_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
-                        const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st);
+void launch_fft_x_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
+                      const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st);
+void launch_fft_y(const PoissonPlan &pp, int inverse, hipStream_t st);
 void launch_poisson_solve_z(const Geo &g, const PoissonPlan &pp, hipStream_t st);
-void launch_poisson_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st);
+void launch_fft_x_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st);
 void launch_top_sum(const Geo &g, const double *Pr, const double *Phi, double *partial, int nblocks,
                     hipStream_t st);
 void launch_projection(const Geo &g, double *U, double *V, double *W, double *Pr, const double *Phi,

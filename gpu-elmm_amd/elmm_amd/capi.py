@@ -72,6 +72,13 @@ class Diag(C.Structure):
                 ("reserved", C.c_int32 * 6)]
 
 
+T_NGROUPS = 16
+
+
+class Timers(C.Structure):
+    _fields_ = [("ms", C.c_double * T_NGROUPS), ("launches", C.c_int64 * T_NGROUPS)]
+
+
 class Cuboid(C.Structure):
     _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
                 ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double)]
@@ -97,6 +104,9 @@ EXPORTS = {
     "elmm_wait": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
     "elmm_divergence": (C.c_int, [C.c_void_p, _pd]),
     "elmm_destroy": (C.c_int, [C.c_void_p]),
+    "elmm_timers_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "elmm_timers_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "elmm_timer_name": (C.c_char_p, [C.c_int32]),
     "elmm_version": (C.c_char_p, []),
     "elmm_last_error": (C.c_char_p, []),
 }
@@ -270,6 +280,16 @@ class Solver:
 
     def wait(self):
         _check(_lib.elmm_wait(self._h, None))
+
+    def timers(self, enable=None, reset=False):
+        """Per kernel group: (name, milliseconds, launches)."""
+        if enable is not None:
+            _check(_lib.elmm_timers_enable(self._h, 1 if enable else 0))
+            return None
+        t = Timers()
+        _check(_lib.elmm_timers_read(self._h, C.byref(t), 1 if reset else 0))
+        return {_lib.elmm_timer_name(g).decode(): (t.ms[g], t.launches[g]) for g in range(T_NGROUPS)
+                if t.launches[g] > 0}
 
     def divergence(self):
         x = C.c_double()

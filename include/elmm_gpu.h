@@ -216,6 +216,27 @@ int elmm_divergence(elmm_handle *h, double *max_div);
 
 int elmm_destroy(elmm_handle *h);
 
+/* ---------------------------------------------------------------------------
+ * Kernel timers: HIP events around each kernel group on the handle's
+ * stream, accumulated per group while enabled.
+ *   replaces: the stage timers printed with debugparam>1
+ *             (src/tsteps.f90:234-238, src/pressure.f90:252-264) and
+ *             poisson_solver_time (src/poisson.f90:125-138)
+ * ------------------------------------------------------------------------- */
+enum {
+  ELMM_T_SGS = 0, ELMM_T_WALL = 1, ELMM_T_MOMENTUM = 2, ELMM_T_MOMENTUM_CORR = 3,
+  ELMM_T_SCALAR = 4, ELMM_T_SCALAR_CORR = 5, ELMM_T_GHOST = 6, ELMM_T_IB = 7,
+  ELMM_T_POISSON_X_FWD = 8, ELMM_T_POISSON_Y = 9, ELMM_T_POISSON_Z = 10, ELMM_T_POISSON_X_INV = 11,
+  ELMM_T_PROJECTION = 12, ELMM_T_REDUCE = 13, ELMM_T_OTHER = 14, ELMM_T_NGROUPS = 16
+};
+typedef struct elmm_timers {
+  double  ms[ELMM_T_NGROUPS];        /* summed event time per group          */
+  int64_t launches[ELMM_T_NGROUPS];  /* kernel launches per group            */
+} elmm_timers;
+int elmm_timers_enable(elmm_handle *h, int32_t enable);
+int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset);
+const char *elmm_timer_name(int32_t group);
+
 /* Version and last error of the calling thread. */
 const char *elmm_version(void);
 const char *elmm_last_error(void);
