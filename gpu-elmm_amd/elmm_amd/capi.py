@@ -81,7 +81,7 @@ class Timers(C.Structure):
 
 class Cuboid(C.Structure):
     _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
-                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double)]
+                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double), ("z0", C.c_double)]
 
 
 class PointSource(C.Structure):
@@ -225,6 +225,11 @@ class Case:
         nt = int(ptr[-1])
         return (_np(v.ib_ijk[c], 3 * n, np.int32).reshape(n, 3), ptr,
                 _np(v.ib_src_ijk[c], 3 * nt, np.int32).reshape(nt, 3), _np(v.ib_coef[c], nt, np.float64))
+
+    def null_list(self, c):
+        v = self.view
+        n = v.null_n[c]
+        return _np(v.null_ijk[c], 3 * n, np.int32).reshape(n, 3)
 
     def counts(self):
         v = self.view

@@ -14,9 +14,10 @@ from oracle_py import Oracle
 pytestmark = pytest.mark.gpu
 
 
-def pair(cfg, scalar_amp=None, seed=1):
-    case = capi.Case(cfg)
-    f = cases.initial_fields(cfg, seed=seed)
+def pair(cfg, scalar_amp=None, seed=1, cuboids=(), sources=(), u0=None):
+    case = capi.Case(cfg, cuboids=cuboids, sources=sources)
+    f = cases.initial_fields(cfg, seed=seed) if u0 is None else cases.initial_fields(cfg, u0=u0, amp=0.1 * u0,
+                                                                                      seed=seed)
     if scalar_amp:
         n = (cfg.nx, cfg.ny, cfg.nz)
         f[4][0][3:3 + n[0], 3:3 + n[1], 3:3 + n[2]] = 1.0 + cases.perturbation(n, scalar_amp, seed, 7)
@@ -101,3 +102,24 @@ def test_step_before_init_is_an_error(require_gpu):
     with pytest.raises(capi.ElmmError) as e:
         s.step(1)
     assert e.value.code == -5
+
+
+OBSTACLES = [(8.0, 16.0, 8.0, 16.0, 0.0, 6.0), (20.0, 26.0, 18.0, 30.0, 0.0, 10.0, 0.01)]
+
+
+def test_immersed_boundary_case_parity(require_gpu):
+    """Two cuboids (one smooth, one rough, one wrapping the periodic y edge),
+    a street-level point source: IB forcing, IB mass sources, solid-body wall
+    models, nullified points and the scalar wall faces, 1 and 10 steps."""
+    cfg = cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1)
+    o, s = pair(cfg, seed=3, cuboids=OBSTACLES, sources=[(4.5, 4.5, 0.5, 1.0, 0)], u0=3.0)
+    err = compare_state(s, o, cfg)
+    assert max(err.values()) <= 1e-12, err
+    do, ds = o.step(1), s.step(1)
+    err = compare_state(s, o, cfg)
+    assert max(err.values()) <= 1e-12, err
+    assert ds.max_div <= max(do.max_div, 1e-15) * 1.5
+    o.step(9)
+    s.step(9)
+    err = compare_state(s, o, cfg)
+    assert max(err.values()) <= 1e-11, err
