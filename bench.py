@@ -22,29 +22,30 @@ from elmm_amd import capi, cases, workloads  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(group, cfg, launches_by_stage):
-    """Algorithmic HBM bytes of one kernel group over the timed launches
-    (each field a kernel must read or write, counted once per launch)."""
-    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
-    cells = nx * ny * nz
-    M = nx // 2 + 1
-    spec = M * ny * nz
-    e = 8  # FP64
-    per = {
-        # U,V,W stencil + Visc + Pr + code byte, Ustar read (stages 2-3), Ustar + U2 written
-        "momentum": lambda st: cells * (3 * e + e + e + 1 + (e if st > 0 else 0) + 2 * e),
-        # U,V,W + TDiff + S + Sadv_prev (stages 2-3) + code + 3 mask bytes, Sadv + S_new written
-        "scalar": lambda st: cells * (3 * e + e + e + (e if st > 0 else 0) + 4 + 2 * e),
-        "sgs": lambda st: cells * (3 * e + e),
-        "poisson_x_fwd": lambda st: cells * 3 * e + spec * 16,
-        "poisson_y": lambda st: spec * 32,
-        "poisson_z": lambda st: spec * 32,
-        "poisson_x_inv": lambda st: spec * 16 + cells * e,
-        "projection": lambda st: cells * (5 * e + 4 * e),
-    }
-    if group not in per:
+# Canonical algorithmic bytes per interior cell for one RK stage and kernel
+# group (each field a group must read or write, counted once; halos, masks and
+# surface lists excluded).  FP64 element = 8 B.  (stage 1, stages 2-3)
+CANONICAL_FP64 = {
+    "sgs": (32, 32),                 # R U,V,W; W Visc
+    "momentum": (88, 112),           # R U,V,W,Visc,Pr,[Ustar x3]; W Ustar x3, U2 x3
+    "scalar": (56, 64),              # R U,V,W,S,TDiff,[Sadv]; W Sadv,S   (per scalar)
+    "poisson_x_fwd": (48, 48),       # divergence of U2,V2,W2 -> RHS (32) + 1st transform pass (16)
+    "poisson_y": (16, 16),           # one transform pass (R + W)
+    "poisson_z": (16, 16),           # tridiagonal solve pass (R + W)
+    "poisson_x_inv": (16, 16),       # last transform pass (R + W)
+    "projection": (72, 72),          # R U2,V2,W2,Phi,Pr; W U,V,W,Pr
+}
+
+
+def algorithmic_bytes(group, cfg, steps):
+    """Algorithmic HBM bytes of one kernel group over the timed steps."""
+    if group not in CANONICAL_FP64:
         return None
-    return sum(per[group](st) * n for st, n in launches_by_stage.items())
+    cells = cfg.nx * cfg.ny * cfg.nz
+    b1, b23 = CANONICAL_FP64[group]
+    per_step = b1 + 2 * b23
+    mult = cfg.nscalars if group == "scalar" else (2 if group == "poisson_y" else 1)
+    return cells * per_step * mult * steps
 
 
 def main():
@@ -86,14 +87,8 @@ def main():
     ms_per_step = dt_s * 1e3 / args.steps
 
     # dominant kernel group by summed event time
-    dom = max(timers.items(), key=lambda kv: kv[1][0])
-    dom_name, (dom_ms, dom_launches) = dom
-    launches_by_stage = {st: args.steps * (3 if dom_name in ("momentum",) else 1) for st in range(3)}
-    if dom_name in ("poisson_y",):
-        launches_by_stage = {st: args.steps * 2 for st in range(3)}
-    if dom_name == "scalar":
-        launches_by_stage = {st: args.steps * cfg.nscalars for st in range(3)}
-    nbytes = algorithmic_bytes(dom_name, cfg, launches_by_stage)
+    dom_name, (dom_ms, dom_launches) = max(timers.items(), key=lambda kv: kv[1][0])
+    nbytes = algorithmic_bytes(dom_name, cfg, args.steps)
     roofline = None
     if nbytes is not None:
         avg_s = dom_ms / 1e3 / dom_launches

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -145,6 +146,11 @@ struct Engine {
   bool initialized = false, first_step_done = false;
   double last_div = 0, last_cfl = 0;
   bool want_diag = false;
+  // momentum kernel selection: the fused z-marching kernel on uniform grids
+  // (ELMM_MOMENTUM=percomp selects the per-component kernels), its z chunk
+  // length (ELMM_ZCHUNK, default 64 levels)
+  bool fused_momentum = false;
+  int zchunk = 64;
   // kernel timers (HIP events on the stream, read after each sync)
   bool timers_on = false;
   std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
@@ -239,6 +245,12 @@ void Engine::setup_geo() {
   g.dymin = grid.dymin;
   g.dzmin = grid.dzmin;
   g.variable_z = cfg.stretched_z != 0;
+  {
+    const char *m = std::getenv("ELMM_MOMENTUM");
+    fused_momentum = momentum_zm_supported(g) && !(m && std::string(m) == "percomp");
+    const char *zc = std::getenv("ELMM_ZCHUNK");
+    if (zc && std::atoi(zc) > 0) zchunk = std::atoi(zc);
+  }
   for (int s = 0; s < 6; s++) {
     g.bt[s] = cfg.btype[s];
     g.sbt[s] = cfg.scal_btype[s];
@@ -744,11 +756,19 @@ void Engine::stage(int st, bool last) {
   subgrid_stresses(st == 0 && cfg.cfl_check);
   double *ustar[3] = {Us.as<double>(), Vs.as<double>(), Ws.as<double>()};
   double *U2p[3] = {u2, v2, w2};
-  for (int c = 0; c < 3; c++) {
+  if (fused_momentum) {
+    const uint8_t *codes[3] = {code[0].as<uint8_t>(), code[1].as<uint8_t>(), code[2].as<uint8_t>()};
     timed(ELMM_T_MOMENTUM, [&] {
-      launch_momentum(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c], code[c].as<uint8_t>(),
-                      sc, stream);
+      launch_momentum_zm<double>(g, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar, U2p, codes, sc, zchunk,
+                                 stream);
     });
+  }
+  for (int c = 0; c < 3; c++) {
+    if (!fused_momentum)
+      timed(ELMM_T_MOMENTUM, [&] {
+        launch_momentum(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c], code[c].as<uint8_t>(),
+                        sc, stream);
+      });
     if (mom_terms[c].n)
       timed(ELMM_T_MOMENTUM_CORR, [&] {
         launch_momentum_corr(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c],

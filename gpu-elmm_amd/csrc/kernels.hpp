@@ -51,6 +51,12 @@ void launch_wm_fluxes(const Geo &g, const double *U, const double *V, const doub
 void launch_momentum(const Geo &g, int comp, const double *U, const double *V, const double *W, const double *Visc,
                      const double *Pr, double *Ustar, double *U2, const uint8_t *code, StageConst sc,
                      hipStream_t st);
+// fused three-component momentum kernel, z-marching (k_momentum_zm.hip)
+bool momentum_zm_supported(const Geo &g);
+size_t momentum_zm_lds_bytes(int elem);
+template <typename T>
+void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *const Us[3],
+                        T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, hipStream_t st);
 void launch_momentum_corr(const Geo &g, int comp, const double *U, const double *V, const double *W,
                           const double *Visc, const double *Pr, double *Ustar, double *U2, const uint8_t *code,
                           StageConst sc, TermList tl, const double *fluxp, hipStream_t st);

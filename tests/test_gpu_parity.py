@@ -123,3 +123,18 @@ def test_immersed_boundary_case_parity(require_gpu):
     s.step(9)
     err = compare_state(s, o, cfg)
     assert max(err.values()) <= 1e-11, err
+
+
+@pytest.mark.parametrize("env", [{"ELMM_MOMENTUM": "percomp"}, {"ELMM_ZCHUNK": "5"}])
+def test_momentum_kernel_variants_bit_identical(require_gpu, monkeypatch, env):
+    """The per-component momentum kernels and the fused z-marching kernel
+    with short z chunks (chunk boundaries inside the domain) give the same
+    bits as the oracle on the obstacle case."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    cfg = cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1)
+    o, s = pair(cfg, seed=4, cuboids=OBSTACLES, sources=[(4.5, 4.5, 0.5, 1.0, 0)], u0=3.0)
+    o.step(2)
+    s.step(2)
+    err = compare_state(s, o, cfg)
+    assert max(err.values()) == 0.0, err
