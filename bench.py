@@ -24,28 +24,76 @@ HBM_PEAK_GBS = 8000.0
 
 # Canonical algorithmic bytes per interior cell for one RK stage and kernel
 # group (each field a group must read or write, counted once; halos, masks and
-# surface lists excluded).  FP64 element = 8 B.  (stage 1, stages 2-3)
-CANONICAL_FP64 = {
-    "sgs": (32, 32),                 # R U,V,W; W Visc
-    "momentum": (88, 112),           # R U,V,W,Visc,Pr,[Ustar x3]; W Ustar x3, U2 x3
-    "scalar": (56, 64),              # R U,V,W,S,TDiff,[Sadv]; W Sadv,S   (per scalar)
-    "poisson_x_fwd": (48, 48),       # divergence of U2,V2,W2 -> RHS (32) + 1st transform pass (16)
-    "poisson_y": (16, 16),           # one transform pass (R + W)
-    "poisson_z": (16, 16),           # tridiagonal solve pass (R + W)
-    "poisson_x_inv": (16, 16),       # last transform pass (R + W)
-    "projection": (72, 72),          # R U2,V2,W2,Phi,Pr; W U,V,W,Pr
+# surface lists excluded), per precision form: (stage 1, stages 2-3).
+CANONICAL = {
+    "fp64": {
+        "sgs": (32, 32),                 # R U,V,W; W Visc
+        "momentum": (88, 112),           # R U,V,W,Visc,Pr,[Ustar x3]; W Ustar x3, U2 x3
+        "scalar": (56, 64),              # R U,V,W,S,TDiff,[Sadv]; W Sadv,S   (per scalar)
+        "poisson_x_fwd": (48, 48),       # divergence of U2,V2,W2 -> RHS (32) + first transform pass (16)
+        "poisson_y": (16, 16),           # one transform pass (R + W)
+        "poisson_z": (16, 16),           # tridiagonal solve pass (R + W)
+        "poisson_x_inv": (16, 16),       # last transform pass (R + W)
+        "projection": (72, 72),          # R U2,V2,W2,Phi,Pr; W U,V,W,Pr
+    },
+    "fp32r": {
+        "sgs": (16, 16),
+        "momentum": (44, 56),
+        "scalar": (28, 32),
+        "projection": (36, 36),
+    },
 }
+PRECISION = {"fp64": capi.FP64, "fp32r": capi.FP32_REFINED}
 
 
-def algorithmic_bytes(group, cfg, steps):
+def algorithmic_bytes(group, cfg, steps, prec):
     """Algorithmic HBM bytes of one kernel group over the timed steps."""
-    if group not in CANONICAL_FP64:
+    table = CANONICAL[prec]
+    if group not in table:
         return None
     cells = cfg.nx * cfg.ny * cfg.nz
-    b1, b23 = CANONICAL_FP64[group]
+    b1, b23 = table[group]
     per_step = b1 + 2 * b23
     mult = cfg.nscalars if group == "scalar" else (2 if group == "poisson_y" else 1)
     return cells * per_step * mult * steps
+
+
+def run_one(wl, cfg, case, fields, prec, steps, warmup):
+    """Time `steps` steps of one precision form with the state resident in
+    HBM; returns throughput, per-group kernel times and diagnostics."""
+    c = capi.with_precision(cfg, PRECISION[prec])
+    t0 = time.perf_counter()
+    solver = capi.Solver(c, case)
+    cases.upload_initial(solver, c, fields)
+    solver.init()
+    setup_s = time.perf_counter() - t0
+    solver.step(warmup, diag=False)
+    solver.timers(enable=True)
+    solver.timers(reset=True)
+    t0 = time.perf_counter()
+    diag = solver.step(steps)
+    dt_s = time.perf_counter() - t0
+    timers = solver.timers()
+    solver.timers(enable=False)
+    solver.close()
+    cells = cfg.nx * cfg.ny * cfg.nz
+    return {"value": cells * steps / dt_s / 1e6, "ms_per_step": dt_s * 1e3 / steps, "timers": timers,
+            "max_div": diag.max_div, "stored_div": diag.stored_div, "poisson_res": diag.poisson_res,
+            "setup_s": setup_s}
+
+
+def roofline_of(run, cfg, steps, prec, wl):
+    dom_name, (dom_ms, dom_launches) = max(run["timers"].items(), key=lambda kv: kv[1][0])
+    nbytes = algorithmic_bytes(dom_name, cfg, steps, prec)
+    if nbytes is None:
+        return None
+    avg_s = dom_ms / 1e3 / dom_launches
+    per_launch = nbytes / dom_launches
+    achieved = per_launch / avg_s / 1e9
+    traffic = workloads.measured_traffic(dom_name, f"{wl.name}_{prec}")
+    return {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
 
 
 def main():
@@ -54,7 +102,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default=workloads.DEFAULT)
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="both", choices=["both", "fp32r", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     args = ap.parse_args()
@@ -67,78 +115,59 @@ def main():
     cfg, cuboids, sources = wl.build()
     t0 = time.perf_counter()
     case = capi.Case(cfg, cuboids=cuboids, sources=sources)
+    case_s = time.perf_counter() - t0
     fields = wl.fields(cfg)
-    solver = capi.Solver(cfg, case)
-    cases.upload_initial(solver, cfg, fields)
-    solver.init()
-    setup_s = time.perf_counter() - t0
-
-    solver.step(args.warmup, diag=False)
-    solver.timers(enable=True)
-    solver.timers(reset=True)
-    t0 = time.perf_counter()
-    diag = solver.step(args.steps)
-    dt_s = time.perf_counter() - t0
-    timers = solver.timers()
-    solver.timers(enable=False)
-
-    cells = cfg.nx * cfg.ny * cfg.nz
-    value = cells * args.steps / dt_s / 1e6
-    ms_per_step = dt_s * 1e3 / args.steps
-
-    # dominant kernel group by summed event time
-    dom_name, (dom_ms, dom_launches) = max(timers.items(), key=lambda kv: kv[1][0])
-    nbytes = algorithmic_bytes(dom_name, cfg, args.steps)
-    roofline = None
-    if nbytes is not None:
-        avg_s = dom_ms / 1e3 / dom_launches
-        per_launch = nbytes / dom_launches
-        achieved = per_launch / avg_s / 1e9
-        traffic = workloads.measured_traffic(dom_name, wl.name)
-        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+    precs = ["fp64", "fp32r"] if args.precision == "both" else [args.precision]
+    runs = {p: run_one(wl, cfg, case, fields, p, args.steps, args.warmup) for p in precs}
+    prim = "fp32r" if "fp32r" in runs else "fp64"
+    r = runs[prim]
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, cfg, cuboids, sources, args.cpu_threads)
+        cpu = cpu_baseline(wl, cfg, case, fields, args.cpu_threads)
 
-    step_model = workloads.step_bytes_model(cfg, has_ib=bool(cuboids), precision=args.precision)
+    cells = cfg.nx * cfg.ny * cfg.nz
     out = {
         "metric": "Mcell-steps/s, FP32+refine vs FP64 on 1 GPU; p50/p99 scenario latency",
-        "value": round(value, 2),
+        "value": round(r["value"], 2),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 stencils, f64 refined pressure" if prim == "fp32r" else "f64",
         "data": "synthetic (seeded perturbations, float32-representable)",
         "config": {"workload": wl.name, "grid": [cfg.nx, cfg.ny, cfg.nz], "cells": cells, "dt": cfg.dt,
-                   "precision": args.precision, "obstacles": len(cuboids), "scalars": cfg.nscalars},
-        "roofline": roofline,
+                   "precision": prim, "obstacles": len(cuboids), "scalars": cfg.nscalars},
+        "roofline": roofline_of(r, cfg, args.steps, prim, wl),
         "cpu_baseline": cpu,
-        "step_model": {"bytes_per_cell_step": step_model,
-                       "effective_GBs": round(step_model * cells * args.steps / dt_s / 1e9, 1)},
-        "max_div": diag.max_div,
-        "setup_s": round(setup_s, 2),
-        "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in sorted(timers.items())},
+        "max_div": r["max_div"],
+        "stored_div": r["stored_div"],
+        "poisson_res": r["poisson_res"],
+        "case_build_s": round(case_s, 2),
     }
+    if "fp64" in runs and prim != "fp64":
+        f = runs["fp64"]
+        out["fp64"] = {"value": round(f["value"], 2), "ms_per_step": round(f["ms_per_step"], 4),
+                       "max_div": f["max_div"], "roofline": roofline_of(f, cfg, args.steps, "fp64", wl),
+                       "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in sorted(f["timers"].items())}}
+        out["speedup_vs_fp64"] = round(r["value"] / f["value"], 3)
+    out["kernel_ms"] = {k: [round(v[0], 3), v[1]] for k, v in sorted(r["timers"].items())}
     print(json.dumps(out))
 
 
-def cpu_baseline(wl, cfg, cuboids, sources, threads):
-    """The oracle (C restatement of the reference step, OpenMP) timed on a
-    bounded sample: one step of the same case after its initial projection."""
+def cpu_baseline(wl, cfg, case, fields, threads):
+    """The oracle (C restatement of the reference step, OpenMP FP64) timed on
+    a bounded sample: one step of the same case after its initial
+    projection."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import Oracle
-    case = capi.Case(cfg, cuboids=cuboids, sources=sources)
     o = Oracle(cfg, case)
-    cases.upload_initial(o, cfg, wl.fields(cfg))
+    cases.upload_initial(o, cfg, fields)
     o.init()
     nsteps = 1
     t0 = time.perf_counter()

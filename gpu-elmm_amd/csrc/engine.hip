@@ -11,6 +11,7 @@
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -105,58 +106,29 @@ struct TermBuilder {
   }
 };
 
-struct Engine {
+// Type-independent part of an engine: configuration, stream, state flags
+// and the per-group kernel timers.  The C ABI talks to this interface.
+struct EngineIface {
   elmm_config cfg{};
-  std::vector<double> zw_copy;
-  Geo g{};
-  Grid1D grid;
   hipStream_t stream = nullptr;
-  int nsc = 0;
-  // fields
-  DevBuf U, V, W, U2, V2, W2, Us, Vs, Ws, Pr, Visc, TDiff, Q, Phi, RHS;
-  std::vector<std::unique_ptr<DevBuf>> S, Snew, Sadv;
-  double *u, *v, *w, *u2, *v2, *w2;  // current rotation of U/U2
-  std::vector<double *> s_cur, s_nxt;
-  // masks and codes
-  DevBuf code[3], code_s, mask[3], prtype;
-  // coordinates
-  DevBuf coords;
-  // wall model
-  DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_logdz0;
-  DevBuf width_k;
-  bool have_width_k = false;
-  int wm_n = 0;
-  DevBuf wmp_ijk, wmp_div;
-  int wmp_n = 0;
-  DevTerms mom_terms[3];
-  std::vector<std::unique_ptr<DevTerms>> sc_terms;
-  DevBuf src_flux;
-  // immersed boundary
-  DevBuf ib_tgt[4], ib_ptr[4], ib_src[4], ib_coef[4], ib_tmp;
-  int ib_n[4] = {0, 0, 0, 0};
-  std::vector<int32_t> visc_levels;
-  DevTerms q_terms;
-  DevBuf null_ijk[3];
-  int null_n[3] = {0, 0, 0};
-  // pressure
-  DevBuf X, twx, twy, lamx, lamy, az, cz;
-  PoissonPlan pp{};
-  DevBuf partial, cfl_partial, div_partial;
-  int top_blocks = 0, red_blocks = 0, cfl_blocks = 0;
   bool initialized = false, first_step_done = false;
-  double last_div = 0, last_cfl = 0;
+  double last_div = 0, last_cfl = 0, last_res = 0, last_stored_div = 0;
+  int last_refinements = 0;
   bool want_diag = false;
-  // momentum kernel selection: the fused z-marching kernel on uniform grids
-  // (ELMM_MOMENTUM=percomp selects the per-component kernels), its z chunk
-  // length (ELMM_ZCHUNK, default 64 levels)
-  bool fused_momentum = false;
-  int zchunk = 64;
   // kernel timers (HIP events on the stream, read after each sync)
   bool timers_on = false;
   std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
   std::vector<hipEvent_t> event_pool;
   double tms[ELMM_T_NGROUPS] = {};
   int64_t tlaunch[ELMM_T_NGROUPS] = {};
+
+  virtual ~EngineIface() = default;
+  virtual void create(const elmm_config &c, const elmm_lists &L) = 0;
+  virtual void xfer(int field, int lb, const int32_t ext[3], double *host, bool up) = 0;
+  virtual void init() = 0;
+  virtual void step(int nsteps) = 0;
+  virtual void enqueue_steps(int nsteps) = 0;
+  virtual double divergence() = 0;
 
   hipEvent_t get_event() {
     if (!event_pool.empty()) {
@@ -194,6 +166,55 @@ struct Engine {
     }
     pending.clear();
   }
+};
+
+// T: field storage and stencil arithmetic type.  double: the FP64 form (the
+// reference's arithmetic); float: the precision form (FP32 fields and
+// stencils, FP64 divergence/RHS, Phi, residual, reductions and corrections,
+// FP32 spectral solves inside FP64 iterative refinement).
+template <typename T>
+struct Engine : EngineIface {
+  std::vector<double> zw_copy;
+  Geo g{};
+  Grid1D grid;
+  int nsc = 0;
+  // fields
+  DevBuf U, V, W, U2, V2, W2, Us, Vs, Ws, Pr, Visc, TDiff, Q, Phi, RHS, Rres;
+  std::vector<std::unique_ptr<DevBuf>> S, Snew, Sadv;
+  T *u, *v, *w, *u2, *v2, *w2;  // current rotation of U/U2
+  std::vector<T *> s_cur, s_nxt;
+  // masks and codes
+  DevBuf code[3], code_s, mask[3], prtype;
+  // coordinates
+  DevBuf coords;
+  // wall model
+  DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_logdz0;
+  DevBuf width_k;
+  bool have_width_k = false;
+  int wm_n = 0;
+  DevBuf wmp_ijk, wmp_div;
+  int wmp_n = 0;
+  DevTerms mom_terms[3];
+  std::vector<std::unique_ptr<DevTerms>> sc_terms;
+  DevBuf src_flux;
+  // immersed boundary
+  DevBuf ib_tgt[4], ib_ptr[4], ib_src[4], ib_coef[4], ib_tmp;
+  int ib_n[4] = {0, 0, 0, 0};
+  std::vector<int32_t> visc_levels;
+  DevTerms q_terms;
+  DevBuf null_ijk[3];
+  int null_n[3] = {0, 0, 0};
+  // pressure
+  DevBuf X, twx, twy, lamx, lamy, az, cz, az64, cz64;
+  PoissonPlan<T> pp{};
+  DevBuf partial, cfl_partial, div_partial;
+  int top_blocks = 0, red_blocks = 0, cfl_blocks = 0;
+  int refine_iters = 0;
+  // momentum kernel selection: the fused z-marching kernel on uniform grids
+  // (ELMM_MOMENTUM=percomp selects the per-component kernels), its z chunk
+  // length (ELMM_ZCHUNK, default 64 levels)
+  bool fused_momentum = false;
+  int zchunk = 64;
 
   IBDev ibdev(int c) const {
     return IBDev{ib_n[c], ib_tgt[c].as<int32_t>(), ib_ptr[c].as<int32_t>(), ib_src[c].as<int32_t>(),
@@ -211,25 +232,29 @@ struct Engine {
   }
   bool has_ib() const { return ib_n[0] + ib_n[1] + ib_n[2] > 0; }
 
-  void create(const elmm_config &c, const elmm_lists &L);
+  void create(const elmm_config &c, const elmm_lists &L) override;
   void setup_geo();
   void setup_lists(const elmm_lists &L);
   void setup_poisson();
-  void xfer(int field, int lb, const int32_t ext[3], double *host, bool up);
-  void bound_uvw(double *a, double *b, double *c) {
-    timed(ELMM_T_GHOST, [&] { launch_bound_uvw<double>(g, a, b, c, stream); });
+  void xfer(int field, int lb, const int32_t ext[3], double *host, bool up) override;
+  void bound_uvw(T *a, T *b, T *c) {
+    timed(ELMM_T_GHOST, [&] { launch_bound_uvw<T>(g, a, b, c, stream); });
   }
-  void ib_momentum(double *a, double *b, double *c);
-  void pressure_correction(double *a, double *b, double *c, bool use_q, double coef);
+  void ib_momentum(T *a, T *b, T *c);
+  void pressure_correction(T *a, T *b, T *c, bool use_q, double coef, bool diag = false);
+  void poisson_solve(int accumulate);
   void subgrid_stresses(bool cfl);
   void stage(int st, bool last);
-  void init();
-  void step(int nsteps);
-  double max_div(double *a, double *b, double *c, bool use_q);
+  void init() override;
+  void step(int nsteps) override;
+  void enqueue_steps(int nsteps) override;
+  double divergence() override;
+  double max_div(T *a, T *b, T *c, bool use_q);
   double reduce_max(DevBuf &buf, int n);
 };
 
-void Engine::setup_geo() {
+template <typename T>
+void Engine<T>::setup_geo() {
   const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
   g.nx = nx;
   g.ny = ny;
@@ -293,19 +318,22 @@ void Engine::setup_geo() {
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-void Engine::setup_poisson() {
+template <typename T>
+void Engine<T>::setup_poisson() {
   const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
   if (!is_pow2(nx) || !is_pow2(ny)) throw ArgError("FFT pressure solver needs power-of-two nx and ny");
   if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
   if (nz > 256) throw ArgError("z solver supports nz <= 256");
   const int M = nx / 2 + 1;
-  X.alloc((size_t)M * ny * nz * sizeof(double2));
+  using V2 = C2<T>;
+  X.alloc((size_t)M * ny * nz * sizeof(V2));
   auto tw = [](int n) {
-    std::vector<double2> t(std::max(1, n / 2));
+    std::vector<V2> t(std::max(1, n / 2));
     for (int q = 0; q < n / 2; q++) {
-      // exp(-2 pi i q/n) from the first octant by symmetry for accuracy
+      // exp(-2 pi i q/n), evaluated in long double and rounded once
       long double a = 2.0L * 3.14159265358979323846264338327950288L * q / n;
-      t[q] = make_double2((double)cosl(a), (double)-sinl(a));
+      t[q].x = (T)(double)cosl(a);
+      t[q].y = (T)(double)-sinl(a);
     }
     return t;
   };
@@ -324,12 +352,15 @@ void Engine::setup_poisson() {
     a[k - 1] = k > 1 ? 1 / (Grid1D::at(grid.dzW, k - 1) * Grid1D::at(grid.dzPr, k)) : 0;
     c[k - 1] = k < nz ? 1 / (Grid1D::at(grid.dzW, k) * Grid1D::at(grid.dzPr, k)) : 0;
   }
-  upload_vec(lamx, lx);
-  upload_vec(lamy, ly);
-  upload_vec(az, a);
-  upload_vec(cz, c);
-  pp = PoissonPlan{nx, ny, nz, M, X.as<double2>(), twx.as<double2>(), twy.as<double2>(), lamx.as<double>(),
-                   lamy.as<double>(), az.as<double>(), cz.as<double>(), g.dzPr, g.dzW};
+  auto cast = [](const std::vector<double> &v) { return std::vector<T>(v.begin(), v.end()); };
+  upload_vec(lamx, cast(lx));
+  upload_vec(lamy, cast(ly));
+  upload_vec(az, cast(a));
+  upload_vec(cz, cast(c));
+  upload_vec(az64, a);
+  upload_vec(cz64, c);
+  pp = PoissonPlan<T>{nx, ny, nz, M, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<T>(), lamy.as<T>(), az.as<T>(),
+                      cz.as<T>(), g.dzPr, g.dzW};
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
   cfl_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
@@ -338,7 +369,8 @@ void Engine::setup_poisson() {
   cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));
 }
 
-void Engine::setup_lists(const elmm_lists &L) {
+template <typename T>
+void Engine<T>::setup_lists(const elmm_lists &L) {
   const long ntot = g.ntot;
   auto lin = [&](int i, int j, int k) { return gix(g, i, j, k); };
   // host copies of masks in the device layout
@@ -591,11 +623,14 @@ void Engine::setup_lists(const elmm_lists &L) {
   }
 }
 
-void Engine::create(const elmm_config &c, const elmm_lists &L) {
+template <typename T>
+void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   cfg = c;
   std::string err = validate_config(cfg);
   if (!err.empty()) throw ArgError(err);
-  if (cfg.precision != ELMM_FP64) throw ArgError("this build of the step implements precision ELMM_FP64");
+  const int want = std::is_same<T, double>::value ? ELMM_FP64 : ELMM_FP32_REFINED;
+  if (cfg.precision != want) throw ArgError("engine/precision mismatch");
+  refine_iters = want == ELMM_FP32_REFINED ? (cfg.refine_iters > 0 ? cfg.refine_iters : 1) : 0;
   if (cfg.poisson_solver != ELMM_POISSON_FFT) throw ArgError("poisson_solver must be ELMM_POISSON_FFT");
   for (int s = 4; s < 6; s++)
     if (cfg.nscalars > 0 && cfg.scal_btype[s] == ELMM_BC_PERIODIC) throw ArgError("periodic scalar z boundaries");
@@ -606,8 +641,10 @@ void Engine::create(const elmm_config &c, const elmm_lists &L) {
   nsc = cfg.nscalars;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   setup_geo();
-  const size_t fb = g.ntot * sizeof(double);
-  for (DevBuf *b : {&U, &V, &W, &U2, &V2, &W2, &Us, &Vs, &Ws, &Pr, &Visc, &TDiff, &Q, &Phi, &RHS}) b->alloc(fb);
+  const size_t fb = g.ntot * sizeof(T), fb64 = g.ntot * sizeof(double);
+  for (DevBuf *b : {&U, &V, &W, &U2, &V2, &W2, &Us, &Vs, &Ws, &Pr, &Visc, &TDiff}) b->alloc(fb);
+  for (DevBuf *b : {&Q, &Phi, &RHS}) b->alloc(fb64);
+  if (refine_iters > 0) Rres.alloc(fb64);
   for (int s = 0; s < nsc; s++) {
     S.emplace_back(new DevBuf);
     Snew.emplace_back(new DevBuf);
@@ -615,39 +652,48 @@ void Engine::create(const elmm_config &c, const elmm_lists &L) {
     S.back()->alloc(fb);
     Snew.back()->alloc(fb);
     Sadv.back()->alloc(fb);
-    s_cur.push_back(S.back()->as<double>());
-    s_nxt.push_back(Snew.back()->as<double>());
+    s_cur.push_back(S.back()->as<T>());
+    s_nxt.push_back(Snew.back()->as<T>());
   }
-  u = U.as<double>();
-  v = V.as<double>();
-  w = W.as<double>();
-  u2 = U2.as<double>();
-  v2 = V2.as<double>();
-  w2 = W2.as<double>();
+  u = U.as<T>();
+  v = V.as<T>();
+  w = W.as<T>();
+  u2 = U2.as<T>();
+  v2 = V2.as<T>();
+  w2 = W2.as<T>();
   setup_lists(L);
   setup_poisson();
   HIPCHK(hipDeviceSynchronize());
 }
 
-void Engine::xfer(int field, int lb, const int32_t ext[3], double *host, bool up) {
-  double *d = nullptr;
+template <typename T>
+void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool up) {
+  void *d = nullptr;
+  bool is64 = false;  // Phi and Q are FP64 in every form
   switch (field) {
     case ELMM_F_U: d = u; break;
     case ELMM_F_V: d = v; break;
     case ELMM_F_W: d = w; break;
-    case ELMM_F_PR: d = Pr.as<double>(); break;
-    case ELMM_F_VISC: d = Visc.as<double>(); break;
-    case ELMM_F_TDIFF: d = TDiff.as<double>(); break;
-    case ELMM_F_PHI: d = Phi.as<double>(); break;
-    case ELMM_F_Q: d = Q.as<double>(); break;
+    case ELMM_F_PR: d = Pr.p; break;
+    case ELMM_F_VISC: d = Visc.p; break;
+    case ELMM_F_TDIFF: d = TDiff.p; break;
+    case ELMM_F_PHI: d = Phi.p; is64 = true; break;
+    case ELMM_F_Q: d = Q.p; is64 = true; break;
     default:
       if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + nsc) d = s_cur[field - ELMM_F_SCALAR];
   }
   if (!d) throw ArgError("unknown field id " + std::to_string(field));
   if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) throw ArgError("extents must be positive");
   HIPCHK(hipStreamSynchronize(stream));
+  const size_t es = is64 ? sizeof(double) : sizeof(T);
   std::vector<double> stage(g.ntot);
-  HIPCHK(hipMemcpy(stage.data(), d, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<T> staget(is64 ? 0 : g.ntot);
+  if (is64) {
+    HIPCHK(hipMemcpy(stage.data(), d, g.ntot * es, hipMemcpyDeviceToHost));
+  } else {
+    HIPCHK(hipMemcpy(staget.data(), d, g.ntot * es, hipMemcpyDeviceToHost));
+    for (long q = 0; q < g.ntot; q++) stage[q] = (double)staget[q];
+  }
   const int n[3] = {cfg.nx, cfg.ny, cfg.nz};
   int lo[3], hi[3];
   for (int a = 0; a < 3; a++) {
@@ -666,16 +712,22 @@ void Engine::xfer(int field, int lb, const int32_t ext[3], double *host, bool up
           host[h] = stage[p];
       }
   if (up) {
-    HIPCHK(hipMemcpy(d, stage.data(), g.ntot * sizeof(double), hipMemcpyHostToDevice));
+    const void *src = stage.data();
+    if (!is64) {
+      for (long q = 0; q < g.ntot; q++) staget[q] = (T)stage[q];
+      src = staget.data();
+    }
+    HIPCHK(hipMemcpy(d, src, g.ntot * es, hipMemcpyHostToDevice));
     // scalar buffers rotate: both copies start from the uploaded state so
     // ghost cells that no fill touches keep the uploaded value
     if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + nsc)
-      HIPCHK(hipMemcpy(s_nxt[field - ELMM_F_SCALAR], stage.data(), g.ntot * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(s_nxt[field - ELMM_F_SCALAR], src, g.ntot * es, hipMemcpyHostToDevice));
   }
 }
 
-void Engine::ib_momentum(double *a, double *b, double *c) {
-  double *F[3] = {a, b, c};
+template <typename T>
+void Engine<T>::ib_momentum(T *a, T *b, T *c) {
+  T *F[3] = {a, b, c};
   for (int comp = 0; comp < 3; comp++) {
     if (!ib_n[comp]) continue;
     timed(ELMM_T_IB, [&] { launch_ib_interp(g, ibdev(comp), F[comp], ib_tmp.as<double>(), stream); });
@@ -683,7 +735,8 @@ void Engine::ib_momentum(double *a, double *b, double *c) {
   }
 }
 
-double Engine::reduce_max(DevBuf &buf, int n) {
+template <typename T>
+double Engine<T>::reduce_max(DevBuf &buf, int n) {
   std::vector<double> h(n);
   HIPCHK(hipMemcpyAsync(h.data(), buf.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
@@ -692,7 +745,8 @@ double Engine::reduce_max(DevBuf &buf, int n) {
   return m;
 }
 
-double Engine::max_div(double *a, double *b, double *c, bool use_q) {
+template <typename T>
+double Engine<T>::max_div(T *a, T *b, T *c, bool use_q) {
   launch_max_div(g, a, b, c, use_q ? Q.as<double>() : nullptr, div_partial.as<double>(), red_blocks, stream);
   return reduce_max(div_partial, red_blocks);
 }
@@ -702,41 +756,73 @@ double Engine::max_div(double *a, double *b, double *c, bool use_q) {
 // omitted; the zeroing of open-boundary faces inside solids
 // (pressure.f90:301-342) only acts on non-periodic, non-noslip sides, where
 // the normal ghost is already 0 for every supported boundary type.
-void Engine::pressure_correction(double *a, double *b, double *c, bool use_q, double coef) {
-  const double dt2 = coef, dt2_rec = 1.0 / dt2;
-  double *q = use_q ? Q.as<double>() : nullptr;
-  timed(ELMM_T_POISSON_X_FWD, [&] { launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, dt2_rec, nullptr, stream); });
+// Spectral passes 2-5 of a solve whose x-transformed right-hand side is in
+// X: y FFT, z tridiagonal, inverse y, inverse x into Phi (or added to Phi).
+template <typename T>
+void Engine<T>::poisson_solve(int accumulate) {
   timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(pp, 0, stream); });
   timed(ELMM_T_POISSON_Z, [&] { launch_poisson_solve_z(g, pp, stream); });
   timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(pp, 1, stream); });
-  timed(ELMM_T_POISSON_X_INV, [&] { launch_fft_x_inv(g, pp, Phi.as<double>(), stream); });
+  timed(ELMM_T_POISSON_X_INV, [&] { launch_fft_x_inv(g, pp, Phi.as<double>(), accumulate, stream); });
+}
+
+template <typename T>
+void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, bool diag) {
+  const double dt2 = coef, dt2_rec = 1.0 / dt2;
+  double *q = use_q ? Q.as<double>() : nullptr;
+  // first solve; the precision form keeps the FP64 right-hand side
+  double *rhs_keep = refine_iters > 0 ? RHS.as<double>() : nullptr;
+  timed(ELMM_T_POISSON_X_FWD, [&] { launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, dt2_rec, rhs_keep, stream); });
+  poisson_solve(0);
+  // iterative refinement: r = RHS - A Phi in FP64, FP32 solve of A e = r,
+  // Phi += e (FP64)
+  for (int it = 0; it < refine_iters; it++) {
+    timed(ELMM_T_REFINE, [&] {
+      launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(),
+                             Rres.as<double>(), div_partial.as<double>(), red_blocks, stream);
+    });
+    timed(ELMM_T_POISSON_X_FWD,
+          [&] { launch_fft_x_fwd(g, pp, a, b, c, nullptr, Rres.as<double>(), 1.0, nullptr, stream); });
+    poisson_solve(1);
+  }
+  if (diag && refine_iters > 0) {
+    // FP64 residual of the final Phi; the divergence of the FP64-corrected
+    // velocity equals coef * r before the correction is rounded to T
+    launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(),
+                           Rres.as<double>(), div_partial.as<double>(), red_blocks, stream);
+    last_res = reduce_max(div_partial, red_blocks);
+    last_div = dt2 * last_res;
+    last_refinements = refine_iters;
+  }
   timed(ELMM_T_REDUCE, [&] {
-    launch_top_sum(g, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, stream);
+    launch_top_sum(g, Pr.as<T>(), Phi.as<double>(), partial.as<double>(), top_blocks, stream);
   });
   timed(ELMM_T_PROJECTION, [&] {
-    launch_projection(g, a, b, c, Pr.as<double>(), Phi.as<double>(), partial.as<double>(), top_blocks, dt2, stream);
+    launch_projection(g, a, b, c, Pr.as<T>(), Phi.as<double>(), partial.as<double>(), top_blocks, dt2, stream);
   });
-  timed(ELMM_T_GHOST, [&] { launch_bound_pr<double>(g, Pr.as<double>(), stream); });
+  timed(ELMM_T_GHOST, [&] { launch_bound_pr<T>(g, Pr.as<T>(), stream); });
 }
 
 // SubgridStresses, src/dynamics.f90:869-908 (ComputeViscsWM only writes
 // values that SubgridModel overwrites plus the diagnostic WMPoints%ustar;
 // BoundViscosity(TDiff) is idempotent on the frozen TDiff and done at init).
-void Engine::subgrid_stresses(bool cfl) {
+template <typename T>
+void Engine<T>::subgrid_stresses(bool cfl) {
   timed(ELMM_T_SGS, [&] {
-    launch_sgs(g, u, v, w, Visc.as<double>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
+    launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
                have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model,
                cfl ? cfl_partial.as<double>() : nullptr, cfl_blocks, stream);
   });
   if (cfg.wall_model && wm_n) timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream); });
-  timed(ELMM_T_GHOST, [&] { launch_bound_visc<double>(g, Visc.as<double>(), stream); });
+  timed(ELMM_T_GHOST, [&] { launch_bound_visc<T>(g, Visc.as<T>(), stream); });
   if (ib_n[3])
     timed(ELMM_T_IB, [&] {
-      launch_ib_visc_serial(g, ibdev(3), Visc.as<double>(), visc_levels.data(), (int)visc_levels.size() - 1, stream);
+      launch_ib_visc_serial(g, ibdev(3), Visc.as<T>(), visc_levels.data(), (int)visc_levels.size() - 1, stream);
     });
 }
 
-void Engine::stage(int st, bool last) {
+template <typename T>
+void Engine<T>::stage(int st, bool last) {
   const double dt = cfg.dt;
   StageConst sc{};
   sc.stage = st;
@@ -754,24 +840,24 @@ void Engine::stage(int st, bool last) {
   sc.prgy = cfg.pr_gradient_y;
 
   subgrid_stresses(st == 0 && cfg.cfl_check);
-  double *ustar[3] = {Us.as<double>(), Vs.as<double>(), Ws.as<double>()};
-  double *U2p[3] = {u2, v2, w2};
+  T *ustar[3] = {Us.as<T>(), Vs.as<T>(), Ws.as<T>()};
+  T *U2p[3] = {u2, v2, w2};
   if (fused_momentum) {
     const uint8_t *codes[3] = {code[0].as<uint8_t>(), code[1].as<uint8_t>(), code[2].as<uint8_t>()};
     timed(ELMM_T_MOMENTUM, [&] {
-      launch_momentum_zm<double>(g, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar, U2p, codes, sc, zchunk,
+      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, codes, sc, zchunk,
                                  stream);
     });
   }
   for (int c = 0; c < 3; c++) {
     if (!fused_momentum)
       timed(ELMM_T_MOMENTUM, [&] {
-        launch_momentum(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c], code[c].as<uint8_t>(),
+        launch_momentum(g, c, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar[c], U2p[c], code[c].as<uint8_t>(),
                         sc, stream);
       });
     if (mom_terms[c].n)
       timed(ELMM_T_MOMENTUM_CORR, [&] {
-        launch_momentum_corr(g, c, u, v, w, Visc.as<double>(), Pr.as<double>(), ustar[c], U2p[c],
+        launch_momentum_corr(g, c, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar[c], U2p[c],
                              code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(), stream);
       });
   }
@@ -785,18 +871,18 @@ void Engine::stage(int st, bool last) {
       });
     for (int s = 0; s < nsc; s++) {
       timed(ELMM_T_SCALAR, [&] {
-        launch_scalar(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(),
+        launch_scalar(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(),
                       code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc,
                       stream);
       });
       if (sc_terms[s]->n)
         timed(ELMM_T_SCALAR_CORR, [&] {
-          launch_scalar_corr(g, u, v, w, TDiff.as<double>(), s_cur[s], s_nxt[s], Sadv[s]->as<double>(),
+          launch_scalar_corr(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(),
                              code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(),
                              mask[2].as<uint8_t>(), sc, sc_terms[s]->view(), src_flux.as<double>(),
                              wmp_div.as<double>(), stream);
         });
-      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<double>(g, s_nxt[s], stream); });
+      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<T>(g, s_nxt[s], stream); });
       std::swap(s_cur[s], s_nxt[s]);
     }
   }
@@ -807,10 +893,11 @@ void Engine::stage(int st, bool last) {
     bound_uvw(u2, v2, w2);
   }
   if (use_q) timed(ELMM_T_IB, [&] { launch_mass_sources(g, q_terms.view(), u2, v2, w2, Q.as<double>(), stream); });
-  pressure_correction(u2, v2, w2, use_q, sc.coef);
+  pressure_correction(u2, v2, w2, use_q, sc.coef, last && want_diag);
   if (last && want_diag) {
     bound_uvw(u2, v2, w2);
-    last_div = max_div(u2, v2, w2, use_q);
+    last_stored_div = max_div(u2, v2, w2, use_q);
+    if (refine_iters == 0) last_div = last_stored_div;
   }
   std::swap(u, u2);
   std::swap(v, v2);
@@ -825,27 +912,29 @@ void Engine::stage(int st, bool last) {
   }
 }
 
-void Engine::init() {
+template <typename T>
+void Engine<T>::init() {
   // src/initial.f90:4284-4342
   bound_uvw(u, v, w);
-  launch_bound_pr<double>(g, Pr.as<double>(), stream);
+  launch_bound_pr<T>(g, Pr.as<T>(), stream);
   pressure_correction(u, v, w, false, 1.0);
-  launch_sgs(g, u, v, w, Visc.as<double>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
+  launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
              have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model, nullptr, 0, stream);
-  launch_bound_visc<double>(g, Visc.as<double>(), stream);
+  launch_bound_visc<T>(g, Visc.as<T>(), stream);
   if (nsc > 0) {
-    launch_tdiff_init(g, Visc.as<double>(), TDiff.as<double>(), cfg.nu, cfg.nu_scalar, stream);
-    launch_bound_visc<double>(g, TDiff.as<double>(), stream);
+    launch_tdiff_init(g, Visc.as<T>(), TDiff.as<T>(), cfg.nu, cfg.nu_scalar, stream);
+    launch_bound_visc<T>(g, TDiff.as<T>(), stream);
   }
-  for (int s = 0; s < nsc; s++) launch_bound_scalar<double>(g, s_cur[s], stream);
+  for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(g, s_cur[s], stream);
   if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
-  launch_bound_visc<double>(g, Visc.as<double>(), stream);
+  launch_bound_visc<T>(g, Visc.as<T>(), stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
   initialized = true;
 }
 
-void Engine::step(int nsteps) {
+template <typename T>
+void Engine<T>::step(int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     if (!first_step_done) {
       // first call of TMarchRK3 (src/tsteps.f90:54-77)
@@ -861,13 +950,38 @@ void Engine::step(int nsteps) {
   collect_timers();
 }
 
+// Steps enqueued on the engine's stream without a host synchronisation.
+template <typename T>
+void Engine<T>::enqueue_steps(int nsteps) {
+  want_diag = false;
+  for (int s = 0; s < nsteps; s++) {
+    if (!first_step_done) {
+      bound_uvw(u, v, w);
+      if (has_ib()) ib_momentum(u, v, w);
+      first_step_done = true;
+    }
+    for (int st = 0; st < 3; st++) stage(st, false);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+double Engine<T>::divergence() {
+  bound_uvw(u, v, w);
+  const bool use_q = cfg.ib_mass_sources && has_ib() && first_step_done;
+  return max_div(u, v, w, use_q);
+}
+
+template struct Engine<double>;
+template struct Engine<float>;
+
 }  // namespace elmm
 
 using elmm::ArgError;
 using elmm::HipError;
 
 struct elmm_handle {
-  elmm::Engine e;
+  std::unique_ptr<elmm::EngineIface> e;
 };
 
 template <typename F>
@@ -884,33 +998,56 @@ static int guarded(F &&f) {
   }
 }
 
+static void release_engine(elmm::EngineIface *e) {
+  if (!e || !e->stream) return;
+  (void)hipStreamSynchronize(e->stream);
+  for (auto &t : e->pending) {
+    (void)hipEventDestroy(std::get<1>(t));
+    (void)hipEventDestroy(std::get<2>(t));
+  }
+  e->pending.clear();
+  for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
+  e->event_pool.clear();
+  (void)hipStreamDestroy(e->stream);
+  e->stream = nullptr;
+}
+
 extern "C" int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm_handle **out) {
   if (!cfg || !lists || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_create");
   *out = nullptr;
+  if (cfg->precision != ELMM_FP64 && cfg->precision != ELMM_FP32_REFINED)
+    return elmm::fail(ELMM_E_ARG, "precision must be ELMM_FP64 or ELMM_FP32_REFINED");
   auto h = std::make_unique<elmm_handle>();
-  int rc = guarded([&] { h->e.create(*cfg, *lists); });
-  if (rc == 0) *out = h.release();
+  if (cfg->precision == ELMM_FP64)
+    h->e.reset(new elmm::Engine<double>());
+  else
+    h->e.reset(new elmm::Engine<float>());
+  int rc = guarded([&] { h->e->create(*cfg, *lists); });
+  if (rc == 0)
+    *out = h.release();
+  else
+    release_engine(h->e.get());
   return rc;
 }
 
 extern "C" int elmm_upload(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], const double *src) {
   if (!h || !ext || !src) return elmm::fail(ELMM_E_ARG, "null argument to elmm_upload");
-  return guarded([&] { h->e.xfer(field, lb, ext, const_cast<double *>(src), true); });
+  return guarded([&] { h->e->xfer(field, lb, ext, const_cast<double *>(src), true); });
 }
 
 extern "C" int elmm_download(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], double *dst) {
   if (!h || !ext || !dst) return elmm::fail(ELMM_E_ARG, "null argument to elmm_download");
-  return guarded([&] { h->e.xfer(field, lb, ext, dst, false); });
+  return guarded([&] { h->e->xfer(field, lb, ext, dst, false); });
 }
 
 extern "C" int elmm_init(elmm_handle *h) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
-  return guarded([&] { h->e.init(); });
+  return guarded([&] { h->e->init(); });
 }
 
 static int check_diverged(elmm_handle *h) {
-  const elmm_config &c = h->e.cfg;
-  if (c.cfl_check && c.cfl_max > 0 && !(h->e.last_cfl <= c.cfl_max))
+  const elmm_config &c = h->e->cfg;
+  if (c.cfl_check && c.cfl_max > 0 && !(h->e->last_cfl <= c.cfl_max))
     return elmm::fail(ELMM_E_DIVERGED, "Solution diverged. 2 (CFL above CFL_max or NaN)");
   return 0;
 }
@@ -918,14 +1055,17 @@ static int check_diverged(elmm_handle *h) {
 extern "C" int elmm_step(elmm_handle *h, int32_t nsteps, elmm_diag *diag) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
   if (nsteps < 0) return elmm::fail(ELMM_E_ARG, "nsteps must be >= 0");
-  if (!h->e.initialized) return elmm::fail(ELMM_E_STATE, "elmm_step before elmm_init");
-  h->e.want_diag = diag != nullptr;
-  int rc = guarded([&] { h->e.step(nsteps); });
+  if (!h->e->initialized) return elmm::fail(ELMM_E_STATE, "elmm_step before elmm_init");
+  h->e->want_diag = diag != nullptr;
+  int rc = guarded([&] { h->e->step(nsteps); });
   if (rc) return rc;
   if (diag) {
     std::memset(diag, 0, sizeof *diag);
-    diag->cfl = h->e.last_cfl;
-    diag->max_div = h->e.last_div;
+    diag->cfl = h->e->last_cfl;
+    diag->max_div = h->e->last_div;
+    diag->stored_div = h->e->last_stored_div;
+    diag->poisson_res = h->e->last_res;
+    diag->refinements = h->e->last_refinements;
     diag->steps_done = nsteps;
   }
   return check_diverged(h);
@@ -933,24 +1073,15 @@ extern "C" int elmm_step(elmm_handle *h, int32_t nsteps, elmm_diag *diag) {
 
 extern "C" int elmm_step_async(elmm_handle *h, int32_t nsteps) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
-  if (!h->e.initialized) return elmm::fail(ELMM_E_STATE, "elmm_step_async before elmm_init");
-  h->e.want_diag = false;
-  return guarded([&] {
-    for (int s = 0; s < nsteps; s++) {
-      if (!h->e.first_step_done) {
-        h->e.bound_uvw(h->e.u, h->e.v, h->e.w);
-        if (h->e.has_ib()) h->e.ib_momentum(h->e.u, h->e.v, h->e.w);
-        h->e.first_step_done = true;
-      }
-      for (int st = 0; st < 3; st++) h->e.stage(st, false);
-    }
-    HIPCHK(hipGetLastError());
-  });
+  if (!h->e->initialized) return elmm::fail(ELMM_E_STATE, "elmm_step_async before elmm_init");
+  h->e->want_diag = false;
+  if (nsteps < 0) return elmm::fail(ELMM_E_ARG, "nsteps must be >= 0");
+  return guarded([&] { h->e->enqueue_steps(nsteps); });
 }
 
 extern "C" int elmm_wait(elmm_handle *h, elmm_diag *diag) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
-  int rc = guarded([&] { HIPCHK(hipStreamSynchronize(h->e.stream)); });
+  int rc = guarded([&] { HIPCHK(hipStreamSynchronize(h->e->stream)); });
   if (rc) return rc;
   if (diag) std::memset(diag, 0, sizeof *diag);
   return 0;
@@ -958,33 +1089,29 @@ extern "C" int elmm_wait(elmm_handle *h, elmm_diag *diag) {
 
 extern "C" int elmm_divergence(elmm_handle *h, double *max_div) {
   if (!h || !max_div) return elmm::fail(ELMM_E_ARG, "null argument to elmm_divergence");
-  return guarded([&] {
-    h->e.bound_uvw(h->e.u, h->e.v, h->e.w);
-    bool use_q = h->e.cfg.ib_mass_sources && h->e.has_ib() && h->e.first_step_done;
-    *max_div = h->e.max_div(h->e.u, h->e.v, h->e.w, use_q);
-  });
+  return guarded([&] { *max_div = h->e->divergence(); });
 }
 
 extern "C" int elmm_timers_enable(elmm_handle *h, int32_t enable) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
   return guarded([&] {
-    HIPCHK(hipStreamSynchronize(h->e.stream));
-    h->e.collect_timers();
-    h->e.timers_on = enable != 0;
+    HIPCHK(hipStreamSynchronize(h->e->stream));
+    h->e->collect_timers();
+    h->e->timers_on = enable != 0;
   });
 }
 
 extern "C" int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset) {
   if (!h || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_timers_read");
   return guarded([&] {
-    HIPCHK(hipStreamSynchronize(h->e.stream));
-    h->e.collect_timers();
+    HIPCHK(hipStreamSynchronize(h->e->stream));
+    h->e->collect_timers();
     for (int g = 0; g < ELMM_T_NGROUPS; g++) {
-      out->ms[g] = h->e.tms[g];
-      out->launches[g] = h->e.tlaunch[g];
+      out->ms[g] = h->e->tms[g];
+      out->launches[g] = h->e->tlaunch[g];
       if (reset) {
-        h->e.tms[g] = 0;
-        h->e.tlaunch[g] = 0;
+        h->e->tms[g] = 0;
+        h->e->tlaunch[g] = 0;
       }
     }
   });
@@ -993,21 +1120,13 @@ extern "C" int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset)
 extern "C" const char *elmm_timer_name(int32_t g) {
   static const char *names[ELMM_T_NGROUPS] = {
       "sgs", "wall_model", "momentum", "momentum_corr", "scalar", "scalar_corr", "ghost", "immersed_boundary",
-      "poisson_x_fwd", "poisson_y", "poisson_z", "poisson_x_inv", "projection", "reduce", "other", "unused"};
+      "poisson_x_fwd", "poisson_y", "poisson_z", "poisson_x_inv", "projection", "reduce", "other", "refine"};
   return (g >= 0 && g < ELMM_T_NGROUPS) ? names[g] : "invalid";
 }
 
 extern "C" int elmm_destroy(elmm_handle *h) {
   if (!h) return 0;
-  if (h->e.stream) {
-    (void)hipStreamSynchronize(h->e.stream);
-    for (auto &t : h->e.pending) {
-      (void)hipEventDestroy(std::get<1>(t));
-      (void)hipEventDestroy(std::get<2>(t));
-    }
-    for (auto e : h->e.event_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(h->e.stream);
-  }
+  release_engine(h->e.get());
   delete h;
   return 0;
 }

@@ -19,45 +19,48 @@ namespace elmm {
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------- Smagorinsky
-// NuSmag/StrainIJ/Strainu, src/subgrid.f90:55-67, 170-209
-__global__ void __launch_bounds__(256) k_sgs(Geo g, const double *__restrict__ U, const double *__restrict__ V,
-                                             const double *__restrict__ W, double *__restrict__ Visc, double nu,
-                                             double c_smag, double filter_ratio, const double *__restrict__ width_k,
+// NuSmag/StrainIJ/Strainu, src/subgrid.f90:55-67, 170-209.  T is the field
+// and arithmetic type (double: the reference's FP64; float: precision form).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, const T *__restrict__ V,
+                                             const T *__restrict__ W, T *__restrict__ Visc, double nu_,
+                                             double c_smag_, double filter_ratio, const double *__restrict__ width_k,
                                              int sgs_model, double *__restrict__ cfl_partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
+  const T nu = (T)nu_, c_smag = (T)c_smag_;
   double cfl = 0;
   if (i <= g.nx && j <= g.ny) {
     const long p = gix(g, i, j, k);
     const long sy = g.sy, sz = g.sz;
-    double v;
+    T v;
     if (sgs_model == 1) {
-      const double dx = g.dxmin, dy = g.dymin;
-      const double zp1 = g.zPr[k + 1], zm1 = g.zPr[k - 1];
-      double D00 = (U[p] - U[p - 1]) / dx;
-      double D11 = (V[p] - V[p - sy]) / dy;
-      double D22 = (W[p] - W[p - sz]) / g.dzPr[k];
-      double D01 = (U[p + sy] + U[p - 1 + sy] - U[p - sy] - U[p - 1 - sy]) / (4 * dy);
-      double D02 = (U[p + sz] + U[p - 1 + sz] - U[p - sz] - U[p - 1 - sz]) / (2 * (zp1 - zm1));
-      double D10 = (V[p + 1] + V[p + 1 - sy] - V[p - 1] - V[p - 1 - sy]) / (4 * dx);
-      double D12 = (V[p + sz] + V[p - sy + sz] - V[p - sz] - V[p - sy - sz]) / (2 * (zp1 - zm1));
-      double D20 = (W[p + 1] + W[p + 1 - sz] - W[p - 1] - W[p - 1 - sz]) / (4 * dx);
-      double D21 = (W[p + sy] + W[p + sy - sz] - W[p - sy] - W[p - sy - sz]) / (4 * dy);
-      const double D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
-      double s = 0;
+      const T dx = (T)g.dxmin, dy = (T)g.dymin;
+      const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
+      T D00 = (U[p] - U[p - 1]) / dx;
+      T D11 = (V[p] - V[p - sy]) / dy;
+      T D22 = (W[p] - W[p - sz]) / (T)g.dzPr[k];
+      T D01 = (U[p + sy] + U[p - 1 + sy] - U[p - sy] - U[p - 1 - sy]) / (4 * dy);
+      T D02 = (U[p + sz] + U[p - 1 + sz] - U[p - sz] - U[p - 1 - sz]) / (2 * (zp1 - zm1));
+      T D10 = (V[p + 1] + V[p + 1 - sy] - V[p - 1] - V[p - 1 - sy]) / (4 * dx);
+      T D12 = (V[p + sz] + V[p - sy + sz] - V[p - sz] - V[p - sy - sz]) / (2 * (zp1 - zm1));
+      T D20 = (W[p + 1] + W[p + 1 - sz] - W[p - 1] - W[p - 1 - sz]) / (4 * dx);
+      T D21 = (W[p + sy] + W[p + sy - sz] - W[p - sy] - W[p - sy - sz]) / (4 * dy);
+      const T D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
+      T s = 0;
 #pragma unroll
       for (int jj = 0; jj < 3; jj++)
 #pragma unroll
         for (int ii = 0; ii < 3; ii++) {
-          double Sij = (D[ii][jj] + D[jj][ii]) / 2;
+          T Sij = (D[ii][jj] + D[jj][ii]) / 2;
           s = s + Sij * Sij;
         }
-      double Sbar = sqrt(2.0 * s);
+      T Sbar = sqrt((T)2.0 * s);
       // filter width: per-level table from the host libm when dxPr*dyPr is
       // uniform (the cube root of a static geometric factor)
-      double width = width_k ? width_k[k] : filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0);
-      double wc = width * c_smag;
+      T width = width_k ? (T)width_k[k] : (T)(filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0));
+      T wc = width * c_smag;
       v = Sbar * (wc * wc);
       if (nu > 0) v = v + nu;
     } else {
@@ -66,9 +69,9 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const double *__restrict__ U
     Visc[p] = v;
     if (cfl_partial) {
       // TimeStepLength's U_dx_max, src/dynamics.f90:556-594 (NaN -> +inf)
-      double a1 = fabs(U[p]), a0 = fabs(U[p - 1]);
-      double b1 = fabs(V[p]), b0 = fabs(V[p - sy]);
-      double c1 = fabs(W[p]), c0 = fabs(W[p - sz]);
+      double a1 = fabs((double)U[p]), a0 = fabs((double)U[p - 1]);
+      double b1 = fabs((double)V[p]), b0 = fabs((double)V[p - sy]);
+      double c1 = fabs((double)W[p]), c0 = fabs((double)W[p - sz]);
       double dxk = g.variable_z ? g.dxPr[i] : g.dxmin;
       double dyk = g.variable_z ? g.dyPr[j] : g.dymin;
       double dzk = g.variable_z ? g.dzPr[k] : g.dzmin;
@@ -92,11 +95,11 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const double *__restrict__ U
   }
 }
 
-void launch_sgs(const Geo &g, const double *U, const double *V, const double *W, double *Visc, double nu,
-                double c_smag, double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial, int,
-                hipStream_t st) {
+template <typename T>
+void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
+                double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial, int, hipStream_t st) {
   dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
-  hipLaunchKernelGGL(k_sgs, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width_k, sgs_model,
+  hipLaunchKernelGGL(k_sgs<T>, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width_k, sgs_model,
                      cfl_partial);
 }
 
@@ -141,8 +144,13 @@ __device__ inline double wm_rough_ustar(double ustar, double vel, double dist, d
   return ustar;
 }
 
-__global__ void k_wm_fluxes(Geo g, const double *__restrict__ U, const double *__restrict__ V,
-                            const double *__restrict__ W, WMPoints wm, double nu, int variable_z) {
+template <typename T>
+__global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict__ V_, const T *__restrict__ W_,
+                            WMPoints wm, double nu, int variable_z) {
+  // surface lists: evaluated in FP64 whatever the field type
+  auto U = [&](long q) { return (double)U_[q]; };
+  auto V = [&](long q) { return (double)V_[q]; };
+  auto W = [&](long q) { return (double)W_[q]; };
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wm.n) return;
   const int l = wm.list[p];
@@ -152,17 +160,17 @@ __global__ void k_wm_fluxes(Geo g, const double *__restrict__ U, const double *_
   const long sy = g.sy, sz = g.sz;
   double v[3];
   if (comp == 0) {
-    v[0] = U[q];
-    v[1] = (V[q + 1] + V[q] + V[q + 1 - sy] + V[q - sy]) / 4.0;
-    v[2] = (W[q + 1] + W[q] + W[q + 1 - sz] + W[q - sz]) / 4.0;
+    v[0] = U(q);
+    v[1] = (V(q + 1) + V(q) + V(q + 1 - sy) + V(q - sy)) / 4.0;
+    v[2] = (W(q + 1) + W(q) + W(q + 1 - sz) + W(q - sz)) / 4.0;
   } else if (comp == 1) {
-    v[0] = (U[q + sy] + U[q] + U[q - 1 + sy] + U[q - 1]) / 4.0;
-    v[1] = V[q];
-    v[2] = (W[q + sy] + W[q] + W[q + sy - sz] + W[q - sz]) / 4.0;
+    v[0] = (U(q + sy) + U(q) + U(q - 1 + sy) + U(q - 1)) / 4.0;
+    v[1] = V(q);
+    v[2] = (W(q + sy) + W(q) + W(q + sy - sz) + W(q - sz)) / 4.0;
   } else {
-    v[0] = (U[q + sz] + U[q] + U[q - 1 + sz] + U[q - 1]) / 4.0;
-    v[1] = (V[q + sz] + V[q] + V[q - sy + sz] + V[q - sy]) / 4.0;
-    v[2] = W[q];
+    v[0] = (U(q + sz) + U(q) + U(q - 1 + sz) + U(q - 1)) / 4.0;
+    v[1] = (V(q + sz) + V(q) + V(q - sy + sz) + V(q - sy)) / 4.0;
+    v[2] = W(q);
   }
   const double d0 = wm.dist[3 * p], d1 = wm.dist[3 * p + 1], d2 = wm.dist[3 * p + 2];
   const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
@@ -198,49 +206,51 @@ __global__ void k_wm_fluxes(Geo g, const double *__restrict__ U, const double *_
   wm.fluxp[p] = fluxp;
 }
 
-void launch_wm_fluxes(const Geo &g, const double *U, const double *V, const double *W, WMPoints wm, double nu,
-                      int variable_z, hipStream_t st) {
+template <typename T>
+void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,
+                      hipStream_t st) {
   if (wm.n == 0) return;
-  hipLaunchKernelGGL(k_wm_fluxes, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, wm, nu, variable_z);
+  hipLaunchKernelGGL(k_wm_fluxes<T>, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, wm, nu, variable_z);
 }
 
 // ------------------------------------------------------ CD4 advection
-__device__ inline double mask4(double a, double b, double c, double d) { return ((-a + 9 * b) + 9 * c) + -d; }
+template <typename T>
+__device__ inline T mask4(T a, T b, T c, T d) { return ((-a + 9 * b) + 9 * c) + -d; }
 
 // Advection of component C at linear index p (CD4divU/V/W,
 // src/advection.f90:334-834).  cls: 0 4th order, 1 2nd order, 2 none.
-template <int C>
-__device__ inline double cd4(const Geo &g, const double *__restrict__ U, const double *__restrict__ V,
-                             const double *__restrict__ W, long p, int cls) {
+template <typename T, int C>
+__device__ inline double cd4(const Geo &g, const T *__restrict__ U, const T *__restrict__ V,
+                             const T *__restrict__ W, long p, int cls) {
   const long sy = g.sy, sz = g.sz;
-  double acc = 0, d;
-  if (cls == 2) return acc * (-1.0 / 256);
+  T acc = 0, d;
+  if (cls == 2) return acc * (T)(-1.0 / 256);
   // F: advected component; A: advecting velocity per direction (U, V, W)
-  const double *F = C == 0 ? U : C == 1 ? V : W;
+  const T *F = C == 0 ? U : C == 1 ? V : W;
   const long st[3] = {1, sy, sz};
-  const double inv[3] = {g.dxmin, g.dymin, g.dzmin};
+  const T inv[3] = {(T)g.dxmin, (T)g.dymin, (T)g.dzmin};
 #pragma unroll
   for (int dd = 0; dd < 3; dd++) {
     const long s = st[dd];
     if (cls == 0) {
-      double uv1b, uv1a, uv3b, uv3a;
+      T uv1b, uv1a, uv3b, uv3a;
       if (dd == C) {
         // self-advection: UV1(m)=Uint(m)*(F(m-1)+F(m)), Uint over F(m-2..m+1)
-        double Fi;
-        Fi = mask4(F[p - s], F[p], F[p + s], F[p + 2 * s]);
+        T Fi;
+        Fi = mask4<T>(F[p - s], F[p], F[p + s], F[p + 2 * s]);
         uv1b = Fi * (F[p] + F[p + s]);
-        Fi = mask4(F[p - 2 * s], F[p - s], F[p], F[p + s]);
+        Fi = mask4<T>(F[p - 2 * s], F[p - s], F[p], F[p + s]);
         uv1a = Fi * (F[p - s] + F[p]);
-        Fi = mask4(F[p], F[p + s], F[p + 2 * s], F[p + 3 * s]);
+        Fi = mask4<T>(F[p], F[p + s], F[p + 2 * s], F[p + 3 * s]);
         uv3b = Fi * (F[p] + F[p + 3 * s]);
-        Fi = mask4(F[p - 3 * s], F[p - 2 * s], F[p - s], F[p]);
+        Fi = mask4<T>(F[p - 3 * s], F[p - 2 * s], F[p - s], F[p]);
         uv3a = Fi * (F[p - 3 * s] + F[p]);
       } else {
         // cross term: advecting component A = (U,V,W)[dd] interpolated along
-        // axis C: Aint(m) = mask4(A(m, C-1..C+2)); UV1(m) = Aint(m)*(F(m)+F(m+1))
-        const double *A = dd == 0 ? U : dd == 1 ? V : W;
+        // axis C: Aint(m) = mask4<T>(A(m, C-1..C+2)); UV1(m) = Aint(m)*(F(m)+F(m+1))
+        const T *A = dd == 0 ? U : dd == 1 ? V : W;
         const long c = st[C];
-#define AI(o) mask4(A[p + (o) * s - c], A[p + (o) * s], A[p + (o) * s + c], A[p + (o) * s + 2 * c])
+#define AI(o) mask4<T>(A[p + (o) * s - c], A[p + (o) * s], A[p + (o) * s + c], A[p + (o) * s + 2 * c])
         uv1b = AI(0) * (F[p] + F[p + s]);
         uv1a = AI(-1) * (F[p - s] + F[p]);
         uv3b = AI(1) * (F[p] + F[p + 3 * s]);
@@ -255,7 +265,7 @@ __device__ inline double cd4(const Geo &g, const double *__restrict__ U, const d
       if (dd == C) {
         d = 64 * ((F[p + s] + F[p]) * (F[p + s] + F[p]) - (F[p] + F[p - s]) * (F[p] + F[p - s]));
       } else {
-        const double *A = dd == 0 ? U : dd == 1 ? V : W;
+        const T *A = dd == 0 ? U : dd == 1 ? V : W;
         const long c = st[C];
         // (F(+s)+F)*(A(+c)+A) - (F+F(-s))*(A(+c-s)+A(-s)); the reference's
         // W/y term lists the last pair as (V(j-1)+V(j-1,k+1)): a+b = b+a.
@@ -264,71 +274,73 @@ __device__ inline double cd4(const Geo &g, const double *__restrict__ U, const d
       acc = acc + d / inv[dd];
     }
   }
-  return acc * (-1.0 / 256);
+  return acc * (T)(-1.0 / 256);
 }
 
 // ------------------------------------------------------ diffusion fluxes
 // Flux on the upper face of cell p in direction d for component C
 // (MomentumDiffusion_nobranch_2ord, src/momentum_diffusion.f90:169-288).
-template <int C>
-__device__ inline double dflux(const Geo &g, const double *__restrict__ nu, const double *__restrict__ F, long p,
+template <typename T, int C>
+__device__ inline T dflux(const Geo &g, const T *__restrict__ nu, const T *__restrict__ F, long p,
                                int d) {
   const long sy = g.sy, sz = g.sz;
-  const double rx = 1.0 / (g.dxmin * g.dxmin), ry = 1.0 / (g.dymin * g.dymin), rz = 1.0 / (g.dzmin * g.dzmin);
+  const T rx = (T)1.0 / ((T)g.dxmin * (T)g.dxmin), ry = (T)1.0 / ((T)g.dymin * (T)g.dymin),
+          rz = (T)1.0 / ((T)g.dzmin * (T)g.dzmin);
+  const T q4 = (T)0.25;
   if (C == 0) {
     if (d == 0) return nu[p + 1] * (F[p + 1] - F[p]) * rx;
-    if (d == 1) return 0.25 * (nu[p + 1 + sy] + nu[p + 1] + nu[p + sy] + nu[p]) * (F[p + sy] - F[p]) * ry;
-    return 0.25 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) * rz;
+    if (d == 1) return q4 * (nu[p + 1 + sy] + nu[p + 1] + nu[p + sy] + nu[p]) * (F[p + sy] - F[p]) * ry;
+    return q4 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) * rz;
   } else if (C == 1) {
-    if (d == 0) return 0.25 * (nu[p + 1 + sy] + nu[p + 1] + nu[p + sy] + nu[p]) * (F[p + 1] - F[p]) * rx;
+    if (d == 0) return q4 * (nu[p + 1 + sy] + nu[p + 1] + nu[p + sy] + nu[p]) * (F[p + 1] - F[p]) * rx;
     if (d == 1) return nu[p + sy] * (F[p + sy] - F[p]) * ry;
-    return 0.25 * (nu[p + sy + sz] + nu[p + sy] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) * rz;
+    return q4 * (nu[p + sy + sz] + nu[p + sy] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) * rz;
   } else {
-    if (d == 0) return 0.25 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + 1] - F[p]) * rx;
-    if (d == 1) return 0.25 * (nu[p + sy + sz] + nu[p + sz] + nu[p + sy] + nu[p]) * (F[p + sy] - F[p]) * ry;
+    if (d == 0) return q4 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + 1] - F[p]) * rx;
+    if (d == 1) return q4 * (nu[p + sy + sz] + nu[p + sz] + nu[p + sy] + nu[p]) * (F[p + sy] - F[p]) * ry;
     return nu[p + sz] * (F[p + sz] - F[p]) * rz;
   }
 }
 
-template <int C>
-__device__ inline double diffusion(const Geo &g, const double *__restrict__ nu, const double *__restrict__ F, long p,
-                                   double a) {
-  a = a + dflux<C>(g, nu, F, p, 0);
-  a = a - dflux<C>(g, nu, F, p - 1, 0);
-  a = a + dflux<C>(g, nu, F, p, 1);
-  a = a - dflux<C>(g, nu, F, p - g.sy, 1);
-  a = a + dflux<C>(g, nu, F, p, 2);
-  a = a - dflux<C>(g, nu, F, p - g.sz, 2);
+template <typename T, int C>
+__device__ inline T diffusion(const Geo &g, const T *__restrict__ nu, const T *__restrict__ F, long p,
+                                   T a) {
+  a = a + dflux<T, C>(g, nu, F, p, 0);
+  a = a - dflux<T, C>(g, nu, F, p - 1, 0);
+  a = a + dflux<T, C>(g, nu, F, p, 1);
+  a = a - dflux<T, C>(g, nu, F, p - g.sy, 1);
+  a = a + dflux<T, C>(g, nu, F, p, 2);
+  a = a - dflux<T, C>(g, nu, F, p - g.sz, 2);
   return a;
 }
 
 // RK combine + PressureGrad + add(U2,U) for one cell
-template <int C>
-__device__ inline void finish(const Geo &g, const StageConst &sc, const double *__restrict__ F,
-                              const double *__restrict__ Pr, double *__restrict__ Ustar, double *__restrict__ U2,
-                              long p, int k, double w) {
-  double u2 = sc.stage > 0 ? Ustar[p] * sc.rho * sc.dt : 0.0;
-  u2 = u2 + w * sc.beta * sc.dt;
-  if (C == 0 && sc.prgx != 0) u2 = u2 + sc.A * sc.prgx;
-  if (C == 1 && sc.prgy != 0) u2 = u2 + sc.A * sc.prgy;
-  if (C == 0) u2 = u2 + sc.Ax * (Pr[p + 1] - Pr[p]);
-  if (C == 1) u2 = u2 + sc.Ay * (Pr[p + g.sy] - Pr[p]);
+template <typename T, int C>
+__device__ inline void finish(const Geo &g, const StageConst &sc, const T *__restrict__ F,
+                              const T *__restrict__ Pr, T *__restrict__ Ustar, T *__restrict__ U2,
+                              long p, int k, T w) {
+  T u2 = sc.stage > 0 ? Ustar[p] * (T)sc.rho * (T)sc.dt : (T)0.0;
+  u2 = u2 + w * (T)sc.beta * (T)sc.dt;
+  if (C == 0 && sc.prgx != 0) u2 = u2 + (T)sc.A * (T)sc.prgx;
+  if (C == 1 && sc.prgy != 0) u2 = u2 + (T)sc.A * (T)sc.prgy;
+  if (C == 0) u2 = u2 + (T)sc.Ax * (Pr[p + 1] - Pr[p]);
+  if (C == 1) u2 = u2 + (T)sc.Ay * (Pr[p + g.sy] - Pr[p]);
   if (C == 2) {
     if (g.variable_z)
-      u2 = u2 - sc.coef * (Pr[p + g.sz] - Pr[p]) / g.dzW[k];
+      u2 = u2 - (T)sc.coef * (Pr[p + g.sz] - Pr[p]) / (T)g.dzW[k];
     else
-      u2 = u2 + sc.Az * (Pr[p + g.sz] - Pr[p]);
+      u2 = u2 + (T)sc.Az * (Pr[p + g.sz] - Pr[p]);
   }
   u2 = u2 + F[p];
   Ustar[p] = w;
   U2[p] = u2;
 }
 
-template <int C>
+template <typename T, int C>
 __global__ void __launch_bounds__(256)
-    k_momentum(Geo g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
-               const double *__restrict__ Visc, const double *__restrict__ Pr, double *__restrict__ Ustar,
-               double *__restrict__ U2, const uint8_t *__restrict__ code, StageConst sc) {
+    k_momentum(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+               const T *__restrict__ Visc, const T *__restrict__ Pr, T *__restrict__ Ustar,
+               T *__restrict__ U2, const uint8_t *__restrict__ code, StageConst sc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
@@ -336,47 +348,47 @@ __global__ void __launch_bounds__(256)
   const long p = gix(g, i, j, k);
   const uint8_t cd = code[p];
   if (cd & CODE_CORRECTED) return;
-  const double *F = C == 0 ? U : C == 1 ? V : W;
-  double w = cd4<C>(g, U, V, W, p, cd & CODE_ADV_MASK);
-  w = diffusion<C>(g, Visc, F, p, w);
-  finish<C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+  const T *F = C == 0 ? U : C == 1 ? V : W;
+  T w = cd4<T, C>(g, U, V, W, p, cd & CODE_ADV_MASK);
+  w = diffusion<T, C>(g, Visc, F, p, w);
+  finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w);
 }
 
-void launch_momentum(const Geo &g, int comp, const double *U, const double *V, const double *W, const double *Visc,
-                     const double *Pr, double *Ustar, double *U2, const uint8_t *code, StageConst sc,
-                     hipStream_t st) {
+template <typename T>
+void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *Ustar,
+                     T *U2, const uint8_t *code, StageConst sc, hipStream_t st) {
   const int *n = g.n[comp];
   dim3 b(64, 4, 1), gr(nblk(n[0], 64), nblk(n[1], 4), n[2]);
   if (comp == 0)
-    hipLaunchKernelGGL(k_momentum<0>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+    hipLaunchKernelGGL((k_momentum<T, 0>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
   else if (comp == 1)
-    hipLaunchKernelGGL(k_momentum<1>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+    hipLaunchKernelGGL((k_momentum<T, 1>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
   else
-    hipLaunchKernelGGL(k_momentum<2>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+    hipLaunchKernelGGL((k_momentum<T, 2>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
 }
 
 // Cells with wall-model face-flux replacements (wmfluxes-nobranch-inc.f90:
 // 49-208): the bulk value, then the terms in the reference's scatter order.
 // term = {axis | neg<<2, WM point index, face-cell linear index, 0}
-template <int C>
-__global__ void k_momentum_corr(Geo g, const double *__restrict__ U, const double *__restrict__ V,
-                                const double *__restrict__ W, const double *__restrict__ Visc,
-                                const double *__restrict__ Pr, double *__restrict__ Ustar, double *__restrict__ U2,
+template <typename T, int C>
+__global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restrict__ V,
+                                const T *__restrict__ W, const T *__restrict__ Visc,
+                                const T *__restrict__ Pr, T *__restrict__ Ustar, T *__restrict__ U2,
                                 const uint8_t *__restrict__ code, StageConst sc, TermList tl,
                                 const double *__restrict__ fluxp) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= tl.n) return;
   const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
   const long p = gix(g, i, j, k);
-  const double *F = C == 0 ? U : C == 1 ? V : W;
-  double w = cd4<C>(g, U, V, W, p, code[p] & CODE_ADV_MASK);
-  w = diffusion<C>(g, Visc, F, p, w);
+  const T *F = C == 0 ? U : C == 1 ? V : W;
+  T w = cd4<T, C>(g, U, V, W, p, code[p] & CODE_ADV_MASK);
+  w = diffusion<T, C>(g, Visc, F, p, w);
   for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
     const int32_t *tt = &tl.term[4 * t];
     const int ax = tt[0] & 3;
     const bool neg = (tt[0] >> 2) & 1;
-    const double fl = dflux<C>(g, Visc, F, (long)tt[2], ax);
-    const double fp = fluxp[tt[1]];
+    const T fl = dflux<T, C>(g, Visc, F, (long)tt[2], ax);
+    const T fp = (T)fluxp[tt[1]];
     if (neg) {
       w = w - fl;
       w = w - fp;
@@ -385,71 +397,82 @@ __global__ void k_momentum_corr(Geo g, const double *__restrict__ U, const doubl
       w = w + fp;
     }
   }
-  finish<C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+  finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w);
 }
 
-void launch_momentum_corr(const Geo &g, int comp, const double *U, const double *V, const double *W,
-                          const double *Visc, const double *Pr, double *Ustar, double *U2, const uint8_t *code,
-                          StageConst sc, TermList tl, const double *fluxp, hipStream_t st) {
+template <typename T>
+void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
+                          T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
+                          hipStream_t st) {
   if (tl.n == 0) return;
   dim3 b(128), gr(nblk(tl.n, 128));
   if (comp == 0)
-    hipLaunchKernelGGL(k_momentum_corr<0>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+    hipLaunchKernelGGL((k_momentum_corr<T, 0>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
   else if (comp == 1)
-    hipLaunchKernelGGL(k_momentum_corr<1>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+    hipLaunchKernelGGL((k_momentum_corr<T, 1>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
   else
-    hipLaunchKernelGGL(k_momentum_corr<2>, gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+    hipLaunchKernelGGL((k_momentum_corr<T, 2>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
 }
 
 // ------------------------------------------------------------- IB gathers
 // TIBPoint_Interpolate, src/immersedboundary.f90:637-651; IBMomentum is
 // gather-all-then-scatter (src/tsteps.f90:368-418).
-__global__ void k_ib_interp(Geo g, IBDev ib, const double *__restrict__ F, double *__restrict__ tmp) {
+// The interpolation is evaluated in FP64 (coefficients are FP64) and the
+// result rounded to the field type.
+template <typename T>
+__global__ void k_ib_interp(Geo g, IBDev ib, const T *__restrict__ F, double *__restrict__ tmp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ib.n) return;
   double s = 0;
   for (int t = ib.ptr[p]; t < ib.ptr[p + 1]; t++)
-    s = s + ib.coef[t] * F[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
+    s = s + ib.coef[t] * (double)F[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
   tmp[p] = s;
 }
-__global__ void k_ib_scatter(Geo g, IBDev ib, const double *__restrict__ tmp, double *__restrict__ F) {
+template <typename T>
+__global__ void k_ib_scatter(Geo g, IBDev ib, const double *__restrict__ tmp, T *__restrict__ F) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ib.n) return;
-  F[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = tmp[p];
+  F[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = (T)tmp[p];
 }
-void launch_ib_interp(const Geo &g, IBDev ib, const double *F, double *tmp, hipStream_t st) {
-  if (ib.n) hipLaunchKernelGGL(k_ib_interp, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, F, tmp);
+template <typename T>
+void launch_ib_interp(const Geo &g, IBDev ib, const T *F, double *tmp, hipStream_t st) {
+  if (ib.n) hipLaunchKernelGGL(k_ib_interp<T>, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, F, tmp);
 }
-void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, double *F, hipStream_t st) {
-  if (ib.n) hipLaunchKernelGGL(k_ib_scatter, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, tmp, F);
+template <typename T>
+void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, T *F, hipStream_t st) {
+  if (ib.n) hipLaunchKernelGGL(k_ib_scatter<T>, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, tmp, F);
 }
 
 // ScalFlIBPoints viscosity loop (src/dynamics.f90:899-902) is sequential:
 // a point may read a value written by an earlier point.  The host groups the
 // points into dependency levels; points of one level are independent.
-__global__ void k_ib_visc_level(Geo g, IBDev ib, double *__restrict__ Visc, int p0, int p1) {
+template <typename T>
+__global__ void k_ib_visc_level(Geo g, IBDev ib, T *__restrict__ Visc, int p0, int p1) {
   const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   double s = 0;
   for (int t = ib.ptr[p]; t < ib.ptr[p + 1]; t++)
-    s = s + ib.coef[t] * Visc[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
-  Visc[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = s;
+    s = s + ib.coef[t] * (double)Visc[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
+  Visc[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = (T)s;
 }
-void launch_ib_visc_serial(const Geo &g, IBDev ib, double *Visc, const int32_t *level_ptr, int nlevels,
-                           hipStream_t st) {
+template <typename T>
+void launch_ib_visc_serial(const Geo &g, IBDev ib, T *Visc, const int32_t *level_ptr, int nlevels, hipStream_t st) {
   for (int l = 0; l < nlevels; l++) {
     int p0 = level_ptr[l], p1 = level_ptr[l + 1];
-    if (p1 > p0) hipLaunchKernelGGL(k_ib_visc_level, dim3(nblk(p1 - p0, 256)), dim3(256), 0, st, g, ib, Visc, p0, p1);
+    if (p1 > p0)
+      hipLaunchKernelGGL(k_ib_visc_level<T>, dim3(nblk(p1 - p0, 256)), dim3(256), 0, st, g, ib, Visc, p0, p1);
   }
 }
 
 // NullInterior, src/tsteps.f90:543-567
-__global__ void k_null(Geo g, int n, const int32_t *__restrict__ ijk, double *__restrict__ F) {
+template <typename T>
+__global__ void k_null(Geo g, int n, const int32_t *__restrict__ ijk, T *__restrict__ F) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) F[gix(g, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2])] = 0;
 }
-void launch_null(const Geo &g, int n, const int32_t *ijk, double *F, hipStream_t st) {
-  if (n) hipLaunchKernelGGL(k_null, dim3(nblk(n, 256)), dim3(256), 0, st, g, n, ijk, F);
+template <typename T>
+void launch_null(const Geo &g, int n, const int32_t *ijk, T *F, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_null<T>, dim3(nblk(n, 256)), dim3(256), 0, st, g, n, ijk, F);
 }
 
 // IBMassSources (2nd order), src/tsteps.f90:490-531, as a gather: every
@@ -457,8 +480,9 @@ void launch_null(const Geo &g, int n, const int32_t *ijk, double *F, hipStream_t
 // order (the reference scatters with omp atomic).  Cells that only receive a
 // Bound_Q fold are listed with no terms, so their pre-fold value is 0.
 // term = {component | neg<<2, source linear index, spacing index, 0}
-__global__ void k_mass_sources(Geo g, TermList tl, const double *__restrict__ U, const double *__restrict__ V,
-                               const double *__restrict__ W, double *__restrict__ Q) {
+template <typename T>
+__global__ void k_mass_sources(Geo g, TermList tl, const T *__restrict__ U, const T *__restrict__ V,
+                               const T *__restrict__ W, double *__restrict__ Q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= tl.n) return;
   double q = 0;
@@ -466,9 +490,9 @@ __global__ void k_mass_sources(Geo g, TermList tl, const double *__restrict__ U,
     const int32_t *tt = &tl.term[4 * t];
     const int comp = tt[0] & 3;
     const bool neg = (tt[0] >> 2) & 1;
-    const double *F = comp == 0 ? U : comp == 1 ? V : W;
+    const T *F = comp == 0 ? U : comp == 1 ? V : W;
     const double *dl = comp == 0 ? g.dxPr : comp == 1 ? g.dyPr : g.dzPr;
-    const double val = F[(long)tt[1]] / dl[tt[2]];
+    const double val = (double)F[(long)tt[1]] / dl[tt[2]];
     q = neg ? q - val : q + val;
   }
   Q[gix(g, tl.cell[3 * c], tl.cell[3 * c + 1], tl.cell[3 * c + 2])] = q;
@@ -496,13 +520,33 @@ __global__ void k_fold_q(Geo g, double *__restrict__ Q, int axis, int periodic) 
   }
 }
 
-void launch_mass_sources(const Geo &g, TermList tl, const double *U, const double *V, const double *W, double *Q,
-                         hipStream_t st) {
+template <typename T>
+void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, const T *W, double *Q, hipStream_t st) {
   if (!tl.n) return;
-  hipLaunchKernelGGL(k_mass_sources, dim3(nblk(tl.n, 256)), dim3(256), 0, st, g, tl, U, V, W, Q);
+  hipLaunchKernelGGL(k_mass_sources<T>, dim3(nblk(tl.n, 256)), dim3(256), 0, st, g, tl, U, V, W, Q);
   hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.ny, 64), g.nz), dim3(64), 0, st, g, Q, 0, g.bt[0] == 3);
   hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.nx, 64), g.nz), dim3(64), 0, st, g, Q, 1, g.bt[2] == 3);
   hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.nx, 64), g.ny), dim3(64), 0, st, g, Q, 2, g.bt[4] == 3);
 }
+
+// explicit instantiations: FP64 (reference arithmetic) and FP32 (precision form)
+#define ELMM_INST_MOMENTUM(T)                                                                                       \
+  template void launch_sgs<T>(const Geo &, const T *, const T *, const T *, T *, double, double, double,           \
+                              const double *, int, double *, int, hipStream_t);                                    \
+  template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, WMPoints, double, int,           \
+                                    hipStream_t);                                                                  \
+  template void launch_momentum<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *, T *,   \
+                                   T *, const uint8_t *, StageConst, hipStream_t);                                 \
+  template void launch_momentum_corr<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *,   \
+                                        T *, T *, const uint8_t *, StageConst, TermList, const double *,           \
+                                        hipStream_t);                                                              \
+  template void launch_ib_interp<T>(const Geo &, IBDev, const T *, double *, hipStream_t);                         \
+  template void launch_ib_scatter<T>(const Geo &, IBDev, const double *, T *, hipStream_t);                        \
+  template void launch_ib_visc_serial<T>(const Geo &, IBDev, T *, const int32_t *, int, hipStream_t);              \
+  template void launch_null<T>(const Geo &, int, const int32_t *, T *, hipStream_t);                               \
+  template void launch_mass_sources<T>(const Geo &, TermList, const T *, const T *, const T *, double *,           \
+                                       hipStream_t);
+ELMM_INST_MOMENTUM(double)
+ELMM_INST_MOMENTUM(float)
 
 }  // namespace elmm

@@ -384,5 +384,8 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
 template void launch_momentum_zm<double>(const Geo &, const double *, const double *, const double *, const double *,
                                          const double *, double *const[3], double *const[3],
                                          const uint8_t *const[3], StageConst, int, hipStream_t);
+template void launch_momentum_zm<float>(const Geo &, const float *, const float *, const float *, const float *,
+                                        const float *, float *const[3], float *const[3], const uint8_t *const[3],
+                                        StageConst, int, hipStream_t);
 
 }  // namespace elmm

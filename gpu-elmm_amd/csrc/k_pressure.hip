@@ -23,19 +23,30 @@ namespace elmm {
 
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Spectral arithmetic in R (double: FP64 form; float: the inner solves of the
+// precision form).  C2<R> is double2 / float2.
+__device__ inline double2 mk2(double x, double y) { return make_double2(x, y); }
+__device__ inline float2 mk2(float x, float y) { return make_float2(x, y); }
+template <typename V>
+__device__ inline V cmul(V a, V b) {
+  return mk2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <typename V>
+__device__ inline V cadd(V a, V b) {
+  return mk2(a.x + b.x, a.y + b.y);
+}
+template <typename V>
+__device__ inline V csub(V a, V b) {
+  return mk2(a.x - b.x, a.y - b.y);
+}
 
 __device__ inline int bitrev(int v, int logn) { return (int)(__brev((unsigned)v) >> (32 - logn)); }
 
 // In-place radix-2 decimation-in-time FFT of `cnt` independent lines stored
 // in LDS, line l at buf[l*stride + m].  Input must already be in
 // bit-reversed order.  tw[t] = exp(-2 pi i t / n); inverse uses conj.
-__device__ void lds_fft(double2 *buf, int n, int logn, int cnt, int stride, const double2 *__restrict__ tw,
-                        bool inverse) {
+template <typename V>
+__device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *__restrict__ tw, bool inverse) {
   const int nbf = (n / 2) * cnt;
   for (int len = 2; len <= n; len <<= 1) {
     const int half = len >> 1, tstep = n / len;
@@ -43,9 +54,9 @@ __device__ void lds_fft(double2 *buf, int n, int logn, int cnt, int stride, cons
       const int l = b / (n / 2), r = b % (n / 2);
       const int grp = r / half, pos = r % half;
       const int i0 = l * stride + grp * len + pos, i1 = i0 + half;
-      double2 w = tw[pos * tstep];
+      V w = tw[pos * tstep];
       if (inverse) w.y = -w.y;
-      const double2 a = buf[i0], c = cmul(buf[i1], w);
+      const V a = buf[i0], c = cmul(buf[i1], w);
       buf[i0] = cadd(a, c);
       buf[i1] = csub(a, c);
     }
@@ -55,11 +66,14 @@ __device__ void lds_fft(double2 *buf, int n, int logn, int cnt, int stride, cons
 
 // Pass 1: RHS = ((dU/dx + dV/dy + dW/dz) - Q) / coef for the rows of this
 // block, then forward FFT in x.  One block handles ROWS rows.
-__global__ void k_fft_x_fwd(Geo g, PoissonPlan pp, const double *__restrict__ U, const double *__restrict__ V,
-                            const double *__restrict__ W, const double *__restrict__ Q,
+// The divergence is evaluated in FP64 whatever the field type T.
+template <typename T, typename R>
+__global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, const T *__restrict__ V,
+                            const T *__restrict__ W, const double *__restrict__ Q,
                             const double *__restrict__ rhs_in, double dt2_rec, double *__restrict__ rhs_out,
                             int rows_per_block, int logn) {
-  extern __shared__ double2 sh[];
+  extern __shared__ __align__(16) unsigned char sh_raw[];
+  C2<R> *sh = reinterpret_cast<C2<R> *>(sh_raw);
   const int nx = pp.nx;
   const int row0 = blockIdx.x * rows_per_block;
   const int nrows = (long)pp.ny * pp.nz;
@@ -74,13 +88,14 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan pp, const double *__restrict__ U,
         r = rhs_in[p];
       } else {
         const double dz = g.variable_z ? g.dzPr[k] : g.dzmin;
-        r = (U[p] - U[p - 1]) / g.dxmin + (V[p] - V[p - g.sy]) / g.dymin + (W[p] - W[p - g.sz]) / dz;
+        r = ((double)U[p] - (double)U[p - 1]) / g.dxmin + ((double)V[p] - (double)V[p - g.sy]) / g.dymin +
+            ((double)W[p] - (double)W[p - g.sz]) / dz;
         if (Q) r = r - Q[p];
         r = r * dt2_rec;
         if (rhs_out) rhs_out[p] = r;
       }
     }
-    sh[l * nx + bitrev(m, logn)] = make_double2(r, 0.0);
+    sh[l * nx + bitrev(m, logn)] = mk2((R)r, (R)0);
   }
   __syncthreads();
   lds_fft(sh, nx, logn, rows_per_block, nx, pp.twx, false);
@@ -93,16 +108,17 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan pp, const double *__restrict__ U,
 }
 
 // Passes 2 and 4: FFT along y of TX consecutive x-modes for one z-plane.
-template <int TX>
-__global__ void k_fft_y(PoissonPlan pp, int logn, int inverse) {
-  extern __shared__ double2 sh[];
+template <int TX, typename R>
+__global__ void k_fft_y(PoissonPlan<R> pp, int logn, int inverse) {
+  extern __shared__ __align__(16) unsigned char sh_raw[];
+  C2<R> *sh = reinterpret_cast<C2<R> *>(sh_raw);
   const int ny = pp.ny, M = pp.M;
   const int m0 = blockIdx.x * TX, k = blockIdx.y;
-  double2 *X = pp.X + (long)k * ny * M;
+  C2<R> *X = pp.X + (long)k * ny * M;
   for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
     const int c = t % TX, j = t / TX;
     const int m = m0 + c;
-    double2 v = m < M ? X[(long)j * M + m] : make_double2(0, 0);
+    C2<R> v = m < M ? X[(long)j * M + m] : mk2((R)0, (R)0);
     sh[c * ny + bitrev(j, logn)] = v;
   }
   __syncthreads();
@@ -117,34 +133,35 @@ __global__ void k_fft_y(PoissonPlan pp, int logn, int inverse) {
 // Pass 3: per (m, j) mode, a_k x_{k-1} + (lam - a_k - c_k) x_k + c_k x_{k+1}
 // = d_k, solved by parallel cyclic reduction in LDS.  The singular mode
 // (m=j=0) pins x_1 = 0 and is integrated exactly (see below).
-template <int TX>
-__global__ void k_solve_z(PoissonPlan pp) {
-  extern __shared__ double smem[];
+template <int TX, typename R>
+__global__ void k_solve_z(PoissonPlan<R> pp) {
+  extern __shared__ __align__(16) unsigned char sh_raw[];
+  using V = C2<R>;
   const int nz = pp.nz, ny = pp.ny, M = pp.M;
   const int m0 = blockIdx.x * TX, j = blockIdx.y;
   const int N = TX * nz;
-  double *A = smem, *B = A + N, *C = B + N;
-  double2 *D = (double2 *)(C + N);
+  V *D = reinterpret_cast<V *>(sh_raw);
+  R *A = reinterpret_cast<R *>(D + N), *B = A + N, *C = B + N;
   const long plane = (long)ny * M;
   for (int t = threadIdx.x; t < N; t += blockDim.x) {
     const int c = t % TX, k = t / TX;
     const int m = m0 + c;
     const int s = c * nz + k;
-    double2 d = make_double2(0, 0);
-    double lam = 0;
+    V d = mk2((R)0, (R)0);
+    R lam = 0;
     if (m < M) {
       d = pp.X[(long)k * plane + (long)j * M + m];
       lam = pp.lamx[m] + pp.lamy[j];
     }
-    double a = pp.az[k], cc = pp.cz[k];
-    double b = lam - a - cc;
+    R a = pp.az[k], cc = pp.cz[k];
+    R b = lam - a - cc;
     if ((m == 0 && j == 0) || m >= M) {
       // identity rows: padding columns, and the singular mean mode, which is
       // integrated exactly after the reduction (its D survives unchanged)
       a = 0;
       b = 1;
       cc = 0;
-      if (m >= M) d = make_double2(0, 0);
+      if (m >= M) d = mk2((R)0, (R)0);
     }
     A[s] = a;
     B[s] = b;
@@ -153,28 +170,28 @@ __global__ void k_solve_z(PoissonPlan pp) {
   }
   __syncthreads();
   for (int h = 1; h < nz; h <<= 1) {
-    double na[8], nb[8], nc[8];
-    double2 nd[8];
+    R na[8], nb[8], nc[8];
+    V nd[8];
     int cnt = 0;
     for (int t = threadIdx.x; t < N; t += blockDim.x, cnt++) {
       const int c = t / nz, k = t % nz;
       const int s = c * nz + k;
-      double a = A[s], b = B[s], cc = C[s];
-      double2 d = D[s];
-      double a2 = 0, c2 = 0;
-      double2 d2 = d;
-      double b2 = b;
+      R a = A[s], b = B[s], cc = C[s];
+      V d = D[s];
+      R a2 = 0, c2 = 0;
+      V d2 = d;
+      R b2 = b;
       if (k - h >= 0) {
-        const double k1 = a / B[s - h];
+        const R k1 = a / B[s - h];
         a2 = -A[s - h] * k1;
         b2 = b2 - C[s - h] * k1;
-        d2 = make_double2(d2.x - D[s - h].x * k1, d2.y - D[s - h].y * k1);
+        d2 = mk2(d2.x - D[s - h].x * k1, d2.y - D[s - h].y * k1);
       }
       if (k + h < nz) {
-        const double k2 = cc / B[s + h];
+        const R k2 = cc / B[s + h];
         c2 = -C[s + h] * k2;
         b2 = b2 - A[s + h] * k2;
-        d2 = make_double2(d2.x - D[s + h].x * k2, d2.y - D[s + h].y * k2);
+        d2 = mk2(d2.x - D[s + h].x * k2, d2.y - D[s + h].y * k2);
       }
       na[cnt] = a2;
       nb[cnt] = b2;
@@ -199,25 +216,25 @@ __global__ void k_solve_z(PoissonPlan pp) {
     // x_{k+1} = x_k + G_k dzW(k).  Both sums are compensated (Kahan), so the
     // mode is exact to rounding instead of carrying the nz^2 growth of an
     // elimination on the singular operator.
-    double2 G = make_double2(0, 0), Gc = G;
+    V G = mk2((R)0, (R)0), Gc = G;
     for (int k = nz; k >= 2; k--) {
-      const double dz = pp.dzPr[k];
-      double2 t = make_double2(-D[k - 1].x * dz, -D[k - 1].y * dz);
-      double2 y = make_double2(t.x - Gc.x, t.y - Gc.y);
-      double2 s2 = make_double2(G.x + y.x, G.y + y.y);
-      Gc = make_double2((s2.x - G.x) - y.x, (s2.y - G.y) - y.y);
+      const R dz = (R)pp.dzPr[k];
+      V t = mk2(-D[k - 1].x * dz, -D[k - 1].y * dz);
+      V y = mk2(t.x - Gc.x, t.y - Gc.y);
+      V s2 = mk2(G.x + y.x, G.y + y.y);
+      Gc = mk2((s2.x - G.x) - y.x, (s2.y - G.y) - y.y);
       G = s2;
       A[k - 2] = G.x;  // column 0 of A and C is free after the reduction
       C[k - 2] = G.y;
     }
-    double2 x = make_double2(0, 0), xc = x;
+    V x = mk2((R)0, (R)0), xc = x;
     D[0] = x;
     for (int k = 1; k <= nz - 1; k++) {
-      const double dz = pp.dzW[k];
-      double2 t = make_double2(A[k - 1] * dz, C[k - 1] * dz);
-      double2 y = make_double2(t.x - xc.x, t.y - xc.y);
-      double2 s2 = make_double2(x.x + y.x, x.y + y.y);
-      xc = make_double2((s2.x - x.x) - y.x, (s2.y - x.y) - y.y);
+      const R dz = (R)pp.dzW[k];
+      V t = mk2(A[k - 1] * dz, C[k - 1] * dz);
+      V y = mk2(t.x - xc.x, t.y - xc.y);
+      V s2 = mk2(x.x + y.x, x.y + y.y);
+      xc = mk2((s2.x - x.x) - y.x, (s2.y - x.y) - y.y);
       x = s2;
       D[k] = x;
     }
@@ -228,23 +245,27 @@ __global__ void k_solve_z(PoissonPlan pp) {
     const int m = m0 + c;
     const int s = c * nz + k;
     if (m < M) {
-      const double b = B[s];
-      pp.X[(long)k * plane + (long)j * M + m] = make_double2(D[s].x / b, D[s].y / b);
+      const R b = B[s];
+      pp.X[(long)k * plane + (long)j * M + m] = mk2(D[s].x / b, D[s].y / b);
     }
   }
 }
 
 // Pass 5: inverse x FFT from the half spectrum (Hermitian symmetry), real
-// part scaled by 1/(nx*ny) -> Phi interior.
-__global__ void k_fft_x_inv(Geo g, PoissonPlan pp, double *__restrict__ Phi, int rows_per_block, int logn) {
-  extern __shared__ double2 sh[];
+// part scaled by 1/(nx*ny) -> Phi interior (FP64); with accumulate the
+// result is added to Phi (iterative-refinement correction).
+template <typename R>
+__global__ void k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, int accumulate, int rows_per_block,
+                            int logn) {
+  extern __shared__ __align__(16) unsigned char sh_raw[];
+  C2<R> *sh = reinterpret_cast<C2<R> *>(sh_raw);
   const int nx = pp.nx, M = pp.M;
   const int row0 = blockIdx.x * rows_per_block;
   const int nrows = pp.ny * pp.nz;
   for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
     const int l = t / nx, m = t % nx;
     const int row = row0 + l;
-    double2 v = make_double2(0, 0);
+    C2<R> v = mk2((R)0, (R)0);
     if (row < nrows) {
       if (m < M) {
         v = pp.X[(long)row * M + m];
@@ -257,13 +278,15 @@ __global__ void k_fft_x_inv(Geo g, PoissonPlan pp, double *__restrict__ Phi, int
   }
   __syncthreads();
   lds_fft(sh, nx, logn, rows_per_block, nx, pp.twx, true);
-  const double scale = 1.0 / ((double)pp.nx * pp.ny);
+  const R scale = (R)1.0 / ((R)pp.nx * pp.ny);
   for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
     const int l = t / nx, m = t % nx;
     const int row = row0 + l;
     if (row < nrows) {
       const int j = row % pp.ny + 1, k = row / pp.ny + 1;
-      Phi[gix(g, m + 1, j, k)] = sh[l * nx + m].x * scale;
+      const long q = gix(g, m + 1, j, k);
+      const double val = (double)(sh[l * nx + m].x * scale);
+      Phi[q] = accumulate ? Phi[q] + val : val;
     }
   }
 }
@@ -276,32 +299,37 @@ static int ilog2(int n) {
 
 static int rows_for(int nx) { return std::max(1, 1024 / nx); }
 
-void launch_fft_x_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
-                      const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
+template <typename T, typename R>
+void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
+                      const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
-  size_t sh = (size_t)rows * pp.nx * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, dt2_rec,
+  size_t sh = (size_t)rows * pp.nx * sizeof(C2<R>);
+  hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, dt2_rec,
                      rhs_out, rows, ilog2(pp.nx));
 }
 
-void launch_fft_y(const PoissonPlan &pp, int inverse, hipStream_t st) {
+template <typename R>
+void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
   constexpr int TX = 8;
-  hipLaunchKernelGGL(k_fft_y<TX>, dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(double2), st,
-                     pp, ilog2(pp.ny), inverse);
+  hipLaunchKernelGGL((k_fft_y<TX, R>), dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(C2<R>),
+                     st, pp, ilog2(pp.ny), inverse);
 }
 
-void launch_poisson_solve_z(const Geo &, const PoissonPlan &pp, hipStream_t st) {
+template <typename R>
+void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t st) {
   constexpr int TX = 8;
-  size_t sh = (size_t)TX * pp.nz * (3 * sizeof(double) + sizeof(double2));
-  hipLaunchKernelGGL(k_solve_z<TX>, dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
+  size_t sh = (size_t)TX * pp.nz * (3 * sizeof(R) + sizeof(C2<R>));
+  hipLaunchKernelGGL((k_solve_z<TX, R>), dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
 }
 
-void launch_fft_x_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st) {
+template <typename R>
+void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
-  size_t sh = (size_t)rows * pp.nx * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, Phi, rows, ilog2(pp.nx));
+  size_t sh = (size_t)rows * pp.nx * sizeof(C2<R>);
+  hipLaunchKernelGGL(k_fft_x_inv<R>, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, Phi, accumulate, rows,
+                     ilog2(pp.nx));
 }
 
 // ------------------------------------------------------------ reductions
@@ -332,14 +360,15 @@ __device__ inline double block_max(double v) {
 
 // Sum over the top layer of (Pr + Phi): FixPressureToReference with the
 // default reference boundary To (src/pressure.f90:994-1229).
-__global__ void k_top_sum(Geo g, const double *__restrict__ Pr, const double *__restrict__ Phi,
+template <typename T>
+__global__ void k_top_sum(Geo g, const T *__restrict__ Pr, const double *__restrict__ Phi,
                           double *__restrict__ partial) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0;
   if (t < (long)g.nx * g.ny) {
     const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
     const long p = gix(g, i, j, g.nz);
-    v = Pr[p] + Phi[p];
+    v = (double)Pr[p] + Phi[p];
   }
   const double s = block_sum(v);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -353,19 +382,20 @@ __global__ void k_top_final(Geo g, double *__restrict__ partial, int nblocks) {
   partial[nblocks] = s / (double)(g.nx * g.ny);
 }
 
-void launch_top_sum(const Geo &g, const double *Pr, const double *Phi, double *partial, int nblocks,
-                    hipStream_t st) {
-  hipLaunchKernelGGL(k_top_sum, dim3(nblocks), dim3(256), 0, st, g, Pr, Phi, partial);
+template <typename T>
+void launch_top_sum(const Geo &g, const T *Pr, const double *Phi, double *partial, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_top_sum<T>, dim3(nblocks), dim3(256), 0, st, g, Pr, Phi, partial);
   hipLaunchKernelGGL(k_top_final, dim3(1), dim3(64), 0, st, g, partial, nblocks);
 }
 
 // PostPoisson (2nd order) + Pr = Pr + Phi - ref:
 // U -= coef*(Phi(i+1)-Phi(i))/dx etc. (src/pressure.f90:789-853).  x and y
 // are periodic, so Phi(nx+1) = Phi(1) is read by index wrap (Bound_Phi).
+// The correction is evaluated in FP64 and rounded to the field type T.
+template <typename T>
 __global__ void __launch_bounds__(256)
-    k_projection(Geo g, double *__restrict__ U, double *__restrict__ V, double *__restrict__ W,
-                 double *__restrict__ Pr, const double *__restrict__ Phi, const double *__restrict__ partial,
-                 int nblocks, double dt2) {
+    k_projection(Geo g, T *__restrict__ U, T *__restrict__ V, T *__restrict__ W, T *__restrict__ Pr,
+                 const double *__restrict__ Phi, const double *__restrict__ partial, int nblocks, double dt2) {
   const double ref = partial[nblocks];
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -376,36 +406,39 @@ __global__ void __launch_bounds__(256)
   const double phx = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
   const double phy = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
   const double Au = dt2 / g.dxmin, Av = dt2 / g.dymin, Aw = dt2 / g.dzmin;
-  if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) U[p] = U[p] - Au * (phx - ph);
-  if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) V[p] = V[p] - Av * (phy - ph);
+  if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) U[p] = (T)((double)U[p] - Au * (phx - ph));
+  if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) V[p] = (T)((double)V[p] - Av * (phy - ph));
   if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
     const double phz = Phi[p + g.sz];
     if (g.variable_z)
-      W[p] = W[p] - dt2 * (phz - ph) / g.dzW[k];
+      W[p] = (T)((double)W[p] - dt2 * (phz - ph) / g.dzW[k]);
     else
-      W[p] = W[p] - Aw * (phz - ph);
+      W[p] = (T)((double)W[p] - Aw * (phz - ph));
   }
-  double pr = Pr[p] + ph;
+  double pr = (double)Pr[p] + ph;
   if (ref != 0) pr = pr - ref;
-  Pr[p] = pr;
+  Pr[p] = (T)pr;
 }
 
-void launch_projection(const Geo &g, double *U, double *V, double *W, double *Pr, const double *Phi,
-                       const double *partial, int nblocks, double dt2, hipStream_t st) {
+template <typename T>
+void launch_projection(const Geo &g, T *U, T *V, T *W, T *Pr, const double *Phi, const double *partial, int nblocks,
+                       double dt2, hipStream_t st) {
   dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
-  hipLaunchKernelGGL(k_projection, gr, b, 0, st, g, U, V, W, Pr, Phi, partial, nblocks, dt2);
+  hipLaunchKernelGGL(k_projection<T>, gr, b, 0, st, g, U, V, W, Pr, Phi, partial, nblocks, dt2);
 }
 
 // max |div U - Q| (check_divergence, src/pressure.f90:937-981)
-__global__ void k_max_div(Geo g, const double *__restrict__ U, const double *__restrict__ V,
-                          const double *__restrict__ W, const double *__restrict__ Q, double *__restrict__ partial) {
+template <typename T>
+__global__ void k_max_div(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+                          const double *__restrict__ Q, double *__restrict__ partial) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0;
   if (t < (long)g.nx * g.ny * g.nz) {
     const int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
     const long p = gix(g, i, j, k);
     const double dz = g.variable_z ? g.dzPr[k] : g.dzmin;
-    double d = (U[p] - U[p - 1]) / (g.dxmin) + (V[p] - V[p - g.sy]) / (g.dymin) + (W[p] - W[p - g.sz]) / (dz);
+    double d = ((double)U[p] - (double)U[p - 1]) / (g.dxmin) + ((double)V[p] - (double)V[p - g.sy]) / (g.dymin) +
+               ((double)W[p] - (double)W[p - g.sz]) / (dz);
     if (Q) d = d - Q[p];
     v = fabs(d);
   }
@@ -413,9 +446,10 @@ __global__ void k_max_div(Geo g, const double *__restrict__ U, const double *__r
   if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
-void launch_max_div(const Geo &g, const double *U, const double *V, const double *W, const double *Q,
-                    double *partial, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_max_div, dim3(nblocks), dim3(256), 0, st, g, U, V, W, Q, partial);
+template <typename T>
+void launch_max_div(const Geo &g, const T *U, const T *V, const T *W, const double *Q, double *partial, int nblocks,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_max_div<T>, dim3(nblocks), dim3(256), 0, st, g, U, V, W, Q, partial);
 }
 
 // Residue_variable_z, src/poisson.f90:23-60 (Phi ghosts: periodic x/y by
@@ -453,5 +487,59 @@ void launch_residual(const Geo &g, const double *Phi, const double *RHS, double 
                      hipStream_t st) {
   hipLaunchKernelGGL(k_residual, dim3(nblocks), dim3(256), 0, st, g, Phi, RHS, partial);
 }
+
+// Residual of the solved FD2 operator in FP64 for iterative refinement:
+// r = RHS - A Phi with A the operator the spectral solve inverts (periodic x
+// and y by index wrap; z rows a_k (Phi_{k-1} - Phi_k) + c_k (Phi_{k+1} -
+// Phi_k), a_1 = c_nz = 0).  Block maxima of |r| go to partial.
+__global__ void k_refine_residual(Geo g, const double *__restrict__ Phi, const double *__restrict__ RHS,
+                                  const double *__restrict__ az, const double *__restrict__ cz,
+                                  double *__restrict__ r_out, double *__restrict__ partial) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (t < (long)g.nx * g.ny * g.nz) {
+    const int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
+    const long p = gix(g, i, j, k);
+    const double rx = 1 / (g.dxmin * g.dxmin), ry = 1 / (g.dymin * g.dymin);
+    const double ph = Phi[p];
+    const double pw = Phi[i > 1 ? p - 1 : p - 1 + g.nx], pe = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
+    const double ps = Phi[j > 1 ? p - g.sy : p - g.sy + (long)g.ny * g.sy];
+    const double pn = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
+    double a = ((pw - ph) + (pe - ph)) * rx + ((ps - ph) + (pn - ph)) * ry;
+    if (k > 1) a = a + az[k - 1] * (Phi[p - g.sz] - ph);
+    if (k < g.nz) a = a + cz[k - 1] * (Phi[p + g.sz] - ph);
+    const double r = RHS[p] - a;
+    r_out[p] = r;
+    v = fabs(r);
+  }
+  const double m = block_max(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+void launch_refine_residual(const Geo &g, const double *Phi, const double *RHS, const double *az, const double *cz,
+                            double *r_out, double *partial, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_refine_residual, dim3(nblocks), dim3(256), 0, st, g, Phi, RHS, az, cz, r_out, partial);
+}
+
+#define ELMM_INST_PRESSURE_R(R)                                                                                    \
+  template void launch_fft_y<R>(const PoissonPlan<R> &, int, hipStream_t);                                        \
+  template void launch_poisson_solve_z<R>(const Geo &, const PoissonPlan<R> &, hipStream_t);                       \
+  template void launch_fft_x_inv<R>(const Geo &, const PoissonPlan<R> &, double *, int, hipStream_t);
+#define ELMM_INST_PRESSURE_T(T)                                                                                    \
+  template void launch_top_sum<T>(const Geo &, const T *, const double *, double *, int, hipStream_t);             \
+  template void launch_projection<T>(const Geo &, T *, T *, T *, T *, const double *, const double *, int, double,  \
+                                     hipStream_t);                                                                 \
+  template void launch_max_div<T>(const Geo &, const T *, const T *, const T *, const double *, double *, int,      \
+                                  hipStream_t);
+ELMM_INST_PRESSURE_R(double)
+ELMM_INST_PRESSURE_R(float)
+ELMM_INST_PRESSURE_T(double)
+ELMM_INST_PRESSURE_T(float)
+template void launch_fft_x_fwd<double, double>(const Geo &, const PoissonPlan<double> &, const double *,
+                                               const double *, const double *, const double *, const double *,
+                                               double, double *, hipStream_t);
+template void launch_fft_x_fwd<float, float>(const Geo &, const PoissonPlan<float> &, const float *, const float *,
+                                             const float *, const double *, const double *, double, double *,
+                                             hipStream_t);
 
 }  // namespace elmm

@@ -19,17 +19,20 @@ namespace elmm {
 
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-__device__ inline double flux_limiter(double r) {
-  double a = (1 + 2 * r) / 3;
-  double b = 2.0 < a ? 2.0 : a;
-  double c = 2 * r < b ? 2 * r : b;
-  return 0.0 > c ? 0.0 : c;
+// Real = field and arithmetic type (double: FP64 form, float: precision form)
+template <typename Real>
+__device__ inline Real flux_limiter(Real r) {
+  Real a = (1 + 2 * r) / 3;
+  Real b = (Real)2.0 < a ? (Real)2.0 : a;
+  Real c = 2 * r < b ? 2 * r : b;
+  return (Real)0.0 > c ? (Real)0.0 : c;
 }
 
 // Flux through the upper face of cell p along stride s with face velocity u.
-__device__ inline double kappa_flux(const double *__restrict__ S, long p, long s, double u) {
-  const double eps = (double)1e-8f;  // default-real literal in the reference
-  double sr, sl;
+template <typename Real>
+__device__ inline Real kappa_flux(const Real *__restrict__ S, long p, long s, Real u) {
+  const Real eps = (Real)1e-8f;  // default-real literal in the reference
+  Real sr, sl;
   if (u > 0) {
     sr = S[p + s] - S[p];
     sl = S[p] - S[p - s];
@@ -37,20 +40,21 @@ __device__ inline double kappa_flux(const double *__restrict__ S, long p, long s
     sr = S[p] - S[p + s];
     sl = S[p + s] - S[p + 2 * s];
   }
-  const double sg = copysign(1.0, sl);
-  const double slope = flux_limiter((sr + eps * sg) / (sl + eps * sg));
-  if (u > 0) return u * (S[p] + (S[p] - S[p - s]) * slope / 2.0);
-  return u * (S[p + s] + (S[p + s] - S[p + 2 * s]) * slope / 2.0);
+  const Real sg = copysign((Real)1.0, sl);
+  const Real slope = flux_limiter<Real>((sr + eps * sg) / (sl + eps * sg));
+  if (u > 0) return u * (S[p] + (S[p] - S[p - s]) * slope / (Real)2.0);
+  return u * (S[p + s] + (S[p + s] - S[p + 2 * s]) * slope / (Real)2.0);
 }
 
-__device__ inline double scalar_cell(const Geo &g, const double *__restrict__ U, const double *__restrict__ V,
-                                     const double *__restrict__ W, const double *__restrict__ T,
-                                     const double *__restrict__ S, const uint8_t *__restrict__ mx,
+template <typename Real>
+__device__ inline Real scalar_cell(const Geo &g, const Real *__restrict__ U, const Real *__restrict__ V,
+                                   const Real *__restrict__ W, const Real *__restrict__ T,
+                                   const Real *__restrict__ S, const uint8_t *__restrict__ mx,
                                      const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, long p, int k,
                                      uint8_t code) {
   const long sy = g.sy, sz = g.sz;
-  const double Ax = 1 / g.dxmin, Ay = 1 / g.dymin, Az = 1 / g.dzmin;
-  double a = 0;
+  const Real Ax = 1 / (Real)g.dxmin, Ay = 1 / (Real)g.dymin, Az = 1 / (Real)g.dzmin;
+  Real a = 0;
   // x pass: face i-1 adds, face i subtracts
   if (mx[p - 1]) a = a + Ax * kappa_flux(S, p - 1, 1, U[p - 1]);
   if (mx[p]) a = a - Ax * kappa_flux(S, p, 1, U[p]);
@@ -66,8 +70,8 @@ __device__ inline double scalar_cell(const Geo &g, const double *__restrict__ U,
     if (mhi) a = a - Az * kappa_flux(S, p, sz, W[p]);
   }
   if (code & CODE_SOLID) a = 0;
-  const double Dx = 1 / (2 * (g.dxmin * g.dxmin)), Dy = 1 / (2 * (g.dymin * g.dymin)),
-               Dz = 1 / (2 * (g.dzmin * g.dzmin));
+  const Real Dx = 1 / (2 * ((Real)g.dxmin * (Real)g.dxmin)), Dy = 1 / (2 * ((Real)g.dymin * (Real)g.dymin)),
+             Dz = 1 / (2 * ((Real)g.dzmin * (Real)g.dzmin));
   a = a + (T[p + 1] + T[p]) * (S[p + 1] - S[p]) * Dx;
   a = a - (T[p] + T[p - 1]) * (S[p] - S[p - 1]) * Dx;
   a = a + (T[p + sy] + T[p]) * (S[p + sy] - S[p]) * Dy;
@@ -77,18 +81,20 @@ __device__ inline double scalar_cell(const Geo &g, const double *__restrict__ U,
   return a;
 }
 
-__device__ inline void scalar_finish(const StageConst &sc, const double *__restrict__ S, double *__restrict__ Snew,
-                                     double *__restrict__ Sadv, long p, double a) {
-  double s2 = sc.stage > 0 ? Sadv[p] * sc.rho_dt : 0.0;
-  s2 = s2 + a * sc.beta_dt;
+template <typename Real>
+__device__ inline void scalar_finish(const StageConst &sc, const Real *__restrict__ S, Real *__restrict__ Snew,
+                                     Real *__restrict__ Sadv, long p, Real a) {
+  Real s2 = sc.stage > 0 ? Sadv[p] * (Real)sc.rho_dt : (Real)0.0;
+  s2 = s2 + a * (Real)sc.beta_dt;
   Sadv[p] = a;
   Snew[p] = S[p] + s2;
 }
 
+template <typename Real>
 __global__ void __launch_bounds__(256)
-    k_scalar(Geo g, const double *__restrict__ U, const double *__restrict__ V, const double *__restrict__ W,
-             const double *__restrict__ T, const double *__restrict__ S, double *__restrict__ Snew,
-             double *__restrict__ Sadv, const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
+    k_scalar(Geo g, const Real *__restrict__ U, const Real *__restrict__ V, const Real *__restrict__ W,
+             const Real *__restrict__ T, const Real *__restrict__ S, Real *__restrict__ Snew,
+             Real *__restrict__ Sadv, const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
              const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -97,15 +103,16 @@ __global__ void __launch_bounds__(256)
   const long p = gix(g, i, j, k);
   const uint8_t cd = code[p];
   if (cd & CODE_CORRECTED) return;
-  double a = scalar_cell(g, U, V, W, T, S, mx, my, mz, p, k, cd);
-  scalar_finish(sc, S, Snew, Sadv, p, a);
+  Real a = scalar_cell<Real>(g, U, V, W, T, S, mx, my, mz, p, k, cd);
+  scalar_finish<Real>(sc, S, Snew, Sadv, p, a);
 }
 
-void launch_scalar(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
-                   const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
-                   const uint8_t *my, const uint8_t *mz, StageConst sc, hipStream_t st) {
+template <typename Real>
+void launch_scalar(const Geo &g, const Real *U, const Real *V, const Real *W, const Real *TDiff, const Real *S,
+                   Real *Snew, Real *Sadv, const uint8_t *code, const uint8_t *mx, const uint8_t *my,
+                   const uint8_t *mz, StageConst sc, hipStream_t st) {
   dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
-  hipLaunchKernelGGL(k_scalar, gr, b, 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code, mx, my, mz, sc);
+  hipLaunchKernelGGL(k_scalar<Real>, gr, b, 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code, mx, my, mz, sc);
 }
 
 // Cells with ordered extra terms: scalar wall-face flux removal
@@ -117,9 +124,10 @@ void launch_scalar(const Geo &g, const double *U, const double *V, const double 
 //   kind 1: add face flux, face lower cell a (linear), axis
 //   kind 2: add src_flux[a]
 //   kind 3: add wm_div[a] * S(cell)
-__global__ void k_scalar_corr(Geo g, const double *__restrict__ U, const double *__restrict__ V,
-                              const double *__restrict__ W, const double *__restrict__ T,
-                              const double *__restrict__ S, double *__restrict__ Snew, double *__restrict__ Sadv,
+template <typename Real>
+__global__ void k_scalar_corr(Geo g, const Real *__restrict__ U, const Real *__restrict__ V,
+                              const Real *__restrict__ W, const Real *__restrict__ T,
+                              const Real *__restrict__ S, Real *__restrict__ Snew, Real *__restrict__ Sadv,
                               const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
                               const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc,
                               TermList tl, const double *__restrict__ src_flux, const double *__restrict__ wm_div) {
@@ -127,73 +135,94 @@ __global__ void k_scalar_corr(Geo g, const double *__restrict__ U, const double 
   if (c >= tl.n) return;
   const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
   const long p = gix(g, i, j, k);
-  double a = scalar_cell(g, U, V, W, T, S, mx, my, mz, p, k, code[p]);
-  const double Dd[3] = {1 / (2 * (g.dxmin * g.dxmin)), 1 / (2 * (g.dymin * g.dymin)), 1 / (2 * (g.dzmin * g.dzmin))};
+  Real a = scalar_cell<Real>(g, U, V, W, T, S, mx, my, mz, p, k, code[p]);
+  const Real Dd[3] = {1 / (2 * ((Real)g.dxmin * (Real)g.dxmin)), 1 / (2 * ((Real)g.dymin * (Real)g.dymin)),
+                      1 / (2 * ((Real)g.dzmin * (Real)g.dzmin))};
   const long st[3] = {1, g.sy, g.sz};
   for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
     const int32_t *tt = &tl.term[4 * t];
     const int kind = tt[0] & 3, axis = (tt[0] >> 2) & 3;
     if (kind <= 1) {
       const long q = tt[1], r = q + st[axis];
-      const double fl = (T[r] + T[q]) * (S[r] - S[q]) * Dd[axis];
+      const Real fl = (T[r] + T[q]) * (S[r] - S[q]) * Dd[axis];
       a = kind == 0 ? a - fl : a + fl;
     } else if (kind == 2) {
-      a = a + src_flux[tt[1]];
+      a = a + (Real)src_flux[tt[1]];
     } else {
-      a = a + wm_div[tt[1]] * S[p];
+      a = a + (Real)wm_div[tt[1]] * S[p];
     }
   }
-  scalar_finish(sc, S, Snew, Sadv, p, a);
+  scalar_finish<Real>(sc, S, Snew, Sadv, p, a);
 }
 
-void launch_scalar_corr(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
-                        const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
+template <typename Real>
+void launch_scalar_corr(const Geo &g, const Real *U, const Real *V, const Real *W, const Real *TDiff,
+                        const Real *S, Real *Snew, Real *Sadv, const uint8_t *code, const uint8_t *mx,
                         const uint8_t *my, const uint8_t *mz, StageConst sc, TermList tl, const double *src_flux,
                         const double *wm_div, hipStream_t st) {
   if (!tl.n) return;
-  hipLaunchKernelGGL(k_scalar_corr, dim3(nblk(tl.n, 128)), dim3(128), 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code,
+  hipLaunchKernelGGL(k_scalar_corr<Real>, dim3(nblk(tl.n, 128)), dim3(128), 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code,
                      mx, my, mz, sc, tl, src_flux, wm_div);
 }
 
 // DivergenceWM (2nd order), src/wallmodels.f90:1153-1166
-__global__ void k_wm_div(Geo g, WMScalarPoints wp, const int32_t *__restrict__ pt, const double *__restrict__ U,
-                         const double *__restrict__ V, const double *__restrict__ W) {
+template <typename Real>
+__global__ void k_wm_div(Geo g, WMScalarPoints wp, const int32_t *__restrict__ pt, const Real *__restrict__ U_,
+                         const Real *__restrict__ V_, const Real *__restrict__ W_) {
+  auto U = [&](long q) { return (double)U_[q]; };
+  auto V = [&](long q) { return (double)V_[q]; };
+  auto W = [&](long q) { return (double)W_[q]; };
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= wp.n) return;
   const int xi = wp.ijk[3 * n], yj = wp.ijk[3 * n + 1], zk = wp.ijk[3 * n + 2];
   const long p = gix(g, xi, yj, zk);
   const long sy = g.sy, sz = g.sz;
   double div = 0;
-  if (pt[p + 1] <= 0) div = div + U[p] / g.dxmin;
-  if (pt[p - 1] <= 0) div = div - U[p - 1] / g.dxmin;
-  if (pt[p + sy] <= 0) div = div + V[p] / g.dymin;
-  if (pt[p - sy] <= 0) div = div - V[p - sy] / g.dymin;
-  if (pt[p + sz] <= 0) div = div + W[p] / g.dzPr[zk];
-  if (pt[p - sz] <= 0) div = div - W[p - sz] / g.dzPr[zk];
+  if (pt[p + 1] <= 0) div = div + U(p) / g.dxmin;
+  if (pt[p - 1] <= 0) div = div - U(p - 1) / g.dxmin;
+  if (pt[p + sy] <= 0) div = div + V(p) / g.dymin;
+  if (pt[p - sy] <= 0) div = div - V(p - sy) / g.dymin;
+  if (pt[p + sz] <= 0) div = div + W(p) / g.dzPr[zk];
+  if (pt[p - sz] <= 0) div = div - W(p - sz) / g.dzPr[zk];
   wp.div[n] = div;
 }
 
-void launch_wm_div(const Geo &g, WMScalarPoints p, const int32_t *prtype, const double *U, const double *V,
-                   const double *W, hipStream_t st) {
-  if (p.n) hipLaunchKernelGGL(k_wm_div, dim3(nblk(p.n, 256)), dim3(256), 0, st, g, p, prtype, U, V, W);
+template <typename Real>
+void launch_wm_div(const Geo &g, WMScalarPoints p, const int32_t *prtype, const Real *U, const Real *V, const Real *W,
+                   hipStream_t st) {
+  if (p.n) hipLaunchKernelGGL(k_wm_div<Real>, dim3(nblk(p.n, 256)), dim3(256), 0, st, g, p, prtype, U, V, W);
 }
 
 // TDiff = (Viscosity - molecular_viscosity)/constPr_sgs + molecular_diffusivity,
 // src/initial.f90:4302-4313 (constPr_sgs = 0.6 as a default-real literal)
-__global__ void k_tdiff_init(Geo g, const double *__restrict__ Visc, double *__restrict__ TDiff, double nu,
+template <typename Real>
+__global__ void k_tdiff_init(Geo g, const Real *__restrict__ Visc, Real *__restrict__ TDiff, double nu,
                              double nu_scalar) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
   if (i > g.nx || j > g.ny) return;
   const long p = gix(g, i, j, k);
-  TDiff[p] = (Visc[p] - nu) / (double)0.6f + nu_scalar;
+  TDiff[p] = (Visc[p] - (Real)nu) / (Real)(double)0.6f + (Real)nu_scalar;
 }
 
-void launch_tdiff_init(const Geo &g, const double *Visc, double *TDiff, double nu, double nu_scalar,
-                       hipStream_t st) {
+template <typename Real>
+void launch_tdiff_init(const Geo &g, const Real *Visc, Real *TDiff, double nu, double nu_scalar, hipStream_t st) {
   dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
-  hipLaunchKernelGGL(k_tdiff_init, gr, b, 0, st, g, Visc, TDiff, nu, nu_scalar);
+  hipLaunchKernelGGL(k_tdiff_init<Real>, gr, b, 0, st, g, Visc, TDiff, nu, nu_scalar);
 }
+
+#define ELMM_INST_SCALAR(R)                                                                                        \
+  template void launch_scalar<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *,     \
+                                 const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *, StageConst,   \
+                                 hipStream_t);                                                                    \
+  template void launch_scalar_corr<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *, \
+                                      const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *,          \
+                                      StageConst, TermList, const double *, const double *, hipStream_t);          \
+  template void launch_wm_div<R>(const Geo &, WMScalarPoints, const int32_t *, const R *, const R *, const R *,    \
+                                 hipStream_t);                                                                    \
+  template void launch_tdiff_init<R>(const Geo &, const R *, R *, double, double, hipStream_t);
+ELMM_INST_SCALAR(double)
+ELMM_INST_SCALAR(float)
 
 }  // namespace elmm

@@ -42,66 +42,87 @@ struct IBDev {
   const double *coef;
 };
 
-// momentum (k_momentum.hip)
-void launch_sgs(const Geo &g, const double *U, const double *V, const double *W, double *Visc, double nu,
-                double c_smag, double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial,
-                int cfl_blocks, hipStream_t st);
-void launch_wm_fluxes(const Geo &g, const double *U, const double *V, const double *W, WMPoints wm, double nu,
-                      int variable_z, hipStream_t st);
-void launch_momentum(const Geo &g, int comp, const double *U, const double *V, const double *W, const double *Visc,
-                     const double *Pr, double *Ustar, double *U2, const uint8_t *code, StageConst sc,
-                     hipStream_t st);
+// momentum (k_momentum.hip); T = field type (double or float)
+template <typename T>
+void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
+                double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial, int cfl_blocks,
+                hipStream_t st);
+template <typename T>
+void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,
+                      hipStream_t st);
+template <typename T>
+void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *Ustar,
+                     T *U2, const uint8_t *code, StageConst sc, hipStream_t st);
 // fused three-component momentum kernel, z-marching (k_momentum_zm.hip)
 bool momentum_zm_supported(const Geo &g);
 size_t momentum_zm_lds_bytes(int elem);
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *const Us[3],
                         T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, hipStream_t st);
-void launch_momentum_corr(const Geo &g, int comp, const double *U, const double *V, const double *W,
-                          const double *Visc, const double *Pr, double *Ustar, double *U2, const uint8_t *code,
-                          StageConst sc, TermList tl, const double *fluxp, hipStream_t st);
-void launch_ib_interp(const Geo &g, IBDev ib, const double *F, double *tmp, hipStream_t st);
-void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, double *F, hipStream_t st);
-void launch_ib_visc_serial(const Geo &g, IBDev ib, double *Visc, const int32_t *level_ptr, int nlevels,
-                           hipStream_t st);
-void launch_null(const Geo &g, int n, const int32_t *ijk, double *F, hipStream_t st);
-void launch_mass_sources(const Geo &g, TermList tl, const double *U, const double *V, const double *W, double *Q,
-                         hipStream_t st);
+template <typename T>
+void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
+                          T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
+                          hipStream_t st);
+template <typename T>
+void launch_ib_interp(const Geo &g, IBDev ib, const T *F, double *tmp, hipStream_t st);
+template <typename T>
+void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, T *F, hipStream_t st);
+template <typename T>
+void launch_ib_visc_serial(const Geo &g, IBDev ib, T *Visc, const int32_t *level_ptr, int nlevels, hipStream_t st);
+template <typename T>
+void launch_null(const Geo &g, int n, const int32_t *ijk, T *F, hipStream_t st);
+template <typename T>
+void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, const T *W, double *Q, hipStream_t st);
 
-// scalar (k_scalar.hip)
-void launch_wm_div(const Geo &g, WMScalarPoints p, const int32_t *prtype, const double *U, const double *V,
-                   const double *W, hipStream_t st);
-void launch_scalar(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
-                   const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
-                   const uint8_t *my, const uint8_t *mz, StageConst sc, hipStream_t st);
-void launch_scalar_corr(const Geo &g, const double *U, const double *V, const double *W, const double *TDiff,
-                        const double *S, double *Snew, double *Sadv, const uint8_t *code, const uint8_t *mx,
-                        const uint8_t *my, const uint8_t *mz, StageConst sc, TermList tl, const double *src_flux,
-                        const double *wm_div, hipStream_t st);
-void launch_tdiff_init(const Geo &g, const double *Visc, double *TDiff, double nu, double nu_scalar,
-                       hipStream_t st);
+// scalar (k_scalar.hip); R = field type (double or float)
+template <typename R>
+void launch_wm_div(const Geo &g, WMScalarPoints p, const int32_t *prtype, const R *U, const R *V, const R *W,
+                   hipStream_t st);
+template <typename R>
+void launch_scalar(const Geo &g, const R *U, const R *V, const R *W, const R *TDiff, const R *S, R *Snew, R *Sadv,
+                   const uint8_t *code, const uint8_t *mx, const uint8_t *my, const uint8_t *mz, StageConst sc,
+                   hipStream_t st);
+template <typename R>
+void launch_scalar_corr(const Geo &g, const R *U, const R *V, const R *W, const R *TDiff, const R *S, R *Snew,
+                        R *Sadv, const uint8_t *code, const uint8_t *mx, const uint8_t *my, const uint8_t *mz,
+                        StageConst sc, TermList tl, const double *src_flux, const double *wm_div, hipStream_t st);
+template <typename R>
+void launch_tdiff_init(const Geo &g, const R *Visc, R *TDiff, double nu, double nu_scalar, hipStream_t st);
 
-// pressure (k_pressure.hip)
+// pressure (k_pressure.hip).  R = spectral type of the solve (double: FP64
+// form; float: inner solves of the precision form), T = field type.
+template <typename R> struct cplx_of;
+template <> struct cplx_of<double> { using type = double2; };
+template <> struct cplx_of<float> { using type = float2; };
+template <typename R> using C2 = typename cplx_of<R>::type;
+
+template <typename R>
 struct PoissonPlan {
   int nx, ny, nz, M;            // M = nx/2+1
-  double2 *X;                   // spectral work array M*ny*nz
-  const double2 *twx, *twy;     // exp(-2 pi i t/N), t < N/2
-  const double *lamx, *lamy;    // eigenvalues of the periodic directions
-  const double *az, *cz;        // z coupling coefficients per k (0-based)
+  C2<R> *X;                     // spectral work array M*ny*nz
+  const C2<R> *twx, *twy;       // exp(-2 pi i t/N), t < N/2
+  const R *lamx, *lamy;         // eigenvalues of the periodic directions
+  const R *az, *cz;             // z coupling coefficients per k (0-based)
   const double *dzPr, *dzW;     // grid spacings indexed like the reference (k = 1..nz)
 };
-void launch_fft_x_fwd(const Geo &g, const PoissonPlan &pp, const double *U, const double *V, const double *W,
-                      const double *Q, const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st);
-void launch_fft_y(const PoissonPlan &pp, int inverse, hipStream_t st);
-void launch_poisson_solve_z(const Geo &g, const PoissonPlan &pp, hipStream_t st);
-void launch_fft_x_inv(const Geo &g, const PoissonPlan &pp, double *Phi, hipStream_t st);
-void launch_top_sum(const Geo &g, const double *Pr, const double *Phi, double *partial, int nblocks,
+template <typename T, typename R>
+void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
+                      const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st);
+template <typename R> void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st);
+template <typename R> void launch_poisson_solve_z(const Geo &g, const PoissonPlan<R> &pp, hipStream_t st);
+template <typename R>
+void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st);
+template <typename T>
+void launch_top_sum(const Geo &g, const T *Pr, const double *Phi, double *partial, int nblocks, hipStream_t st);
+template <typename T>
+void launch_projection(const Geo &g, T *U, T *V, T *W, T *Pr, const double *Phi, const double *partial, int nblocks,
+                       double dt2, hipStream_t st);
+template <typename T>
+void launch_max_div(const Geo &g, const T *U, const T *V, const T *W, const double *Q, double *partial, int nblocks,
                     hipStream_t st);
-void launch_projection(const Geo &g, double *U, double *V, double *W, double *Pr, const double *Phi,
-                       const double *partial, int nblocks, double dt2, hipStream_t st);
-void launch_max_div(const Geo &g, const double *U, const double *V, const double *W, const double *Q,
-                    double *partial, int nblocks, hipStream_t st);
 void launch_residual(const Geo &g, const double *Phi, const double *RHS, double *partial, int nblocks,
                      hipStream_t st);
+void launch_refine_residual(const Geo &g, const double *Phi, const double *RHS, const double *az, const double *cz,
+                            double *r_out, double *partial, int nblocks, hipStream_t st);
 
 }  // namespace elmm

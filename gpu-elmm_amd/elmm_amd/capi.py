@@ -43,7 +43,7 @@ class Config(C.Structure):
         ("cfl_check", C.c_int32),
         ("cfl_max", C.c_double),
         ("precision", C.c_int32), ("ib_mass_sources", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("refine_iters", C.c_int32), ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -69,7 +69,7 @@ class Lists(C.Structure):
 class Diag(C.Structure):
     _fields_ = [("delta", C.c_double), ("cfl", C.c_double), ("max_div", C.c_double),
                 ("poisson_res", C.c_double), ("refinements", C.c_int32), ("steps_done", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("stored_div", C.c_double), ("reserved", C.c_int32 * 4)]
 
 
 T_NGROUPS = 16
@@ -165,6 +165,17 @@ def make_config(**kw):
         c._zw_keep = zw
         c.zw = zw.ctypes.data_as(_pd)
         c.stretched_z = 1
+    return c
+
+
+def with_precision(cfg, precision, refine_iters=0):
+    """Copy of a configuration with another precision form."""
+    c = Config()
+    C.memmove(C.byref(c), C.byref(cfg), C.sizeof(Config))
+    if hasattr(cfg, "_zw_keep"):
+        c._zw_keep = cfg._zw_keep
+    c.precision = precision
+    c.refine_iters = refine_iters
     return c
 
 

@@ -17,7 +17,7 @@ import numpy as np
 from . import capi, cases
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-DEFAULT = "flat512"
+DEFAULT = "urban512"
 
 
 class Workload:

@@ -44,8 +44,9 @@ enum {
 /* Precision forms of the step. */
 enum {
   ELMM_FP64 = 0,          /* every field and every operation in FP64          */
-  ELMM_FP32_REFINED = 1   /* FP32 stencils, FP64 state/projection, FP32
-                             pressure solves inside FP64 iterative refinement */
+  ELMM_FP32_REFINED = 1   /* FP32 fields and stencils; FP64 divergence/RHS,
+                             Phi, residual, corrections and global sums; FP32
+                             spectral solves inside FP64 iterative refinement */
 };
 
 /* Pressure solvers (src/pressure.f90:17-20 numbering). */
@@ -99,7 +100,9 @@ typedef struct elmm_config {
   double  cfl_max;           /* CFL_max, 0 = no check                         */
   int32_t precision;         /* ELMM_FP64 / ELMM_FP32_REFINED                 */
   int32_t ib_mass_sources;   /* enable_ibm_mass_sources                       */
-  int32_t reserved[8];
+  int32_t refine_iters;      /* ELMM_FP32_REFINED: FP32 solves of the FP64
+                                residual after the first solve (0 = default 1) */
+  int32_t reserved[7];
 } elmm_config;
 
 /*
@@ -146,11 +149,15 @@ typedef struct elmm_lists {
 typedef struct elmm_diag {
   double  delta;          /* sum of mean |U-U2| over components / dt (tsteps.f90:188-209) */
   double  cfl;            /* CFL of the last step when cfl_check           */
-  double  max_div;        /* max |div U| after the last projection         */
+  double  max_div;        /* max |div U - Q| after the last projection,
+                             evaluated in FP64 on the corrected velocity
+                             before it is stored in the field type         */
   double  poisson_res;    /* max residual of the last pressure solve       */
   int32_t refinements;    /* refinement sweeps of the last solve           */
   int32_t steps_done;
-  int32_t reserved[6];
+  double  stored_div;     /* max |div U - Q| of the stored field (equals
+                             max_div in the FP64 form)                      */
+  int32_t reserved[4];
 } elmm_diag;
 
 typedef struct elmm_case elmm_case;
@@ -231,7 +238,7 @@ enum {
   ELMM_T_SGS = 0, ELMM_T_WALL = 1, ELMM_T_MOMENTUM = 2, ELMM_T_MOMENTUM_CORR = 3,
   ELMM_T_SCALAR = 4, ELMM_T_SCALAR_CORR = 5, ELMM_T_GHOST = 6, ELMM_T_IB = 7,
   ELMM_T_POISSON_X_FWD = 8, ELMM_T_POISSON_Y = 9, ELMM_T_POISSON_Z = 10, ELMM_T_POISSON_X_INV = 11,
-  ELMM_T_PROJECTION = 12, ELMM_T_REDUCE = 13, ELMM_T_OTHER = 14, ELMM_T_NGROUPS = 16
+  ELMM_T_PROJECTION = 12, ELMM_T_REDUCE = 13, ELMM_T_OTHER = 14, ELMM_T_REFINE = 15, ELMM_T_NGROUPS = 16
 };
 typedef struct elmm_timers {
   double  ms[ELMM_T_NGROUPS];        /* summed event time per group          */

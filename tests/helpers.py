@@ -47,3 +47,16 @@ def compare_state(a, b, cfg, fields=None):
             out[name] = rel_err(interior(a.download(capi.F_SCALAR + s), pr_n),
                                 interior(b.download(capi.F_SCALAR + s), pr_n))
     return out
+
+
+def velocity_rel_l2(a, b, cfg):
+    """Relative L2 error of the velocity vector (U, V, W interiors) of a
+    against b: ||u_a - u_b||_2 / ||u_b||_2."""
+    ext = component_extents(cfg)
+    num = den = 0.0
+    for c in range(3):
+        fa = interior(a.download(c), ext[c])
+        fb = interior(b.download(c), ext[c])
+        num += float(np.sum((fa - fb) ** 2))
+        den += float(np.sum(fb ** 2))
+    return (num / den) ** 0.5
