@@ -43,25 +43,58 @@ __device__ inline V csub(V a, V b) {
 __device__ inline int bitrev(int v, int logn) { return (int)(__brev((unsigned)v) >> (32 - logn)); }
 
 // In-place radix-2 decimation-in-time FFT of `cnt` independent lines stored
-// in LDS, line l at buf[l*stride + m].  Input must already be in
-// bit-reversed order.  tw[t] = exp(-2 pi i t / n); inverse uses conj.
+// in LDS, line l at buf[l*stride + pad(m)], pad(m) = m + m/8 (one pad slot per
+// 8 elements keeps the 8-element register sets free of bank conflicts).  Input
+// must already be in bit-reversed order.  tw[t] = exp(-2 pi i t / n) (in LDS);
+// inverse uses conj.  Stages are executed three at a time on 8-element sets
+// held in registers (the sets are closed under three consecutive stages), so
+// every butterfly is the same operation on the same operands as a
+// stage-by-stage radix-2 sweep.
+__device__ inline int fpad(int m) { return m + (m >> 3); }
+
 template <typename V>
-__device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *__restrict__ tw, bool inverse) {
-  const int nbf = (n / 2) * cnt;
-  for (int len = 2; len <= n; len <<= 1) {
-    const int half = len >> 1, tstep = n / len;
-    for (int b = threadIdx.x; b < nbf; b += blockDim.x) {
-      const int l = b / (n / 2), r = b % (n / 2);
-      const int grp = r / half, pos = r % half;
-      const int i0 = l * stride + grp * len + pos, i1 = i0 + half;
-      V w = tw[pos * tstep];
-      if (inverse) w.y = -w.y;
-      const V a = buf[i0], c = cmul(buf[i1], w);
-      buf[i0] = cadd(a, c);
-      buf[i1] = csub(a, c);
+__device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *tw, bool inverse) {
+  for (int s0 = 0; s0 < logn; s0 += 3) {
+    const int nb = min(3, logn - s0);
+    const int ne = 1 << nb;
+    const int per_line = n >> nb;
+    const int nsets = per_line * cnt;
+    for (int q = threadIdx.x; q < nsets; q += blockDim.x) {
+      const int l = q / per_line, r = q % per_line;
+      const int lo = r & ((1 << s0) - 1), hi = r >> s0;
+      const int base = (hi << (s0 + nb)) | lo;
+      V x[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < ne) x[e] = buf[l * stride + fpad(base + (e << s0))];
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        if (b >= nb) break;
+        const int half = 1 << (s0 + b), tstep = n / (2 * half);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          if (e >= ne || (e >> b) & 1) continue;
+          const int e1 = e | (1 << b);
+          const int pos = (base + (e << s0)) & (half - 1);
+          V w = tw[pos * tstep];
+          if (inverse) w.y = -w.y;
+          const V a = x[e], c = cmul(x[e1], w);
+          x[e] = cadd(a, c);
+          x[e1] = csub(a, c);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < ne) buf[l * stride + fpad(base + (e << s0))] = x[e];
     }
     __syncthreads();
   }
+}
+
+// twiddle table of length n/2 from global memory into LDS
+template <typename V>
+__device__ inline void load_twiddles(V *dst, const V *__restrict__ src, int n) {
+  for (int t = threadIdx.x; t < n / 2; t += blockDim.x) dst[t] = src[t];
 }
 
 // Pass 1: RHS = ((dU/dx + dV/dy + dW/dz) - Q) / coef for the rows of this
@@ -73,8 +106,10 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, c
                             const double *__restrict__ rhs_in, double dt2_rec, double *__restrict__ rhs_out,
                             int rows_per_block, int logn) {
   extern __shared__ __align__(16) unsigned char sh_raw[];
-  C2<R> *sh = reinterpret_cast<C2<R> *>(sh_raw);
-  const int nx = pp.nx;
+  C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
+  const int nx = pp.nx, ls = fpad(nx);
+  C2<R> *sh = tw + nx / 2;
+  load_twiddles(tw, pp.twx, nx);
   const int row0 = blockIdx.x * rows_per_block;
   const int nrows = (long)pp.ny * pp.nz;
   for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
@@ -95,15 +130,15 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, c
         if (rhs_out) rhs_out[p] = r;
       }
     }
-    sh[l * nx + bitrev(m, logn)] = mk2((R)r, (R)0);
+    sh[l * ls + fpad(bitrev(m, logn))] = mk2((R)r, (R)0);
   }
   __syncthreads();
-  lds_fft(sh, nx, logn, rows_per_block, nx, pp.twx, false);
+  lds_fft(sh, nx, logn, rows_per_block, ls, tw, false);
   const int M = pp.M;
   for (int t = threadIdx.x; t < rows_per_block * M; t += blockDim.x) {
     const int l = t / M, m = t % M;
     const int row = row0 + l;
-    if (row < nrows) pp.X[(long)row * M + m] = sh[l * nx + m];
+    if (row < nrows) pp.X[(long)row * M + m] = sh[l * ls + fpad(m)];
   }
 }
 
@@ -111,22 +146,24 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, c
 template <int TX, typename R>
 __global__ void k_fft_y(PoissonPlan<R> pp, int logn, int inverse) {
   extern __shared__ __align__(16) unsigned char sh_raw[];
-  C2<R> *sh = reinterpret_cast<C2<R> *>(sh_raw);
-  const int ny = pp.ny, M = pp.M;
+  C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
+  const int ny = pp.ny, M = pp.M, ls = fpad(ny);
+  C2<R> *sh = tw + ny / 2;
+  load_twiddles(tw, pp.twy, ny);
   const int m0 = blockIdx.x * TX, k = blockIdx.y;
   C2<R> *X = pp.X + (long)k * ny * M;
   for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
     const int c = t % TX, j = t / TX;
     const int m = m0 + c;
     C2<R> v = m < M ? X[(long)j * M + m] : mk2((R)0, (R)0);
-    sh[c * ny + bitrev(j, logn)] = v;
+    sh[c * ls + fpad(bitrev(j, logn))] = v;
   }
   __syncthreads();
-  lds_fft(sh, ny, logn, TX, ny, pp.twy, inverse != 0);
+  lds_fft(sh, ny, logn, TX, ls, tw, inverse != 0);
   for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
     const int c = t % TX, j = t / TX;
     const int m = m0 + c;
-    if (m < M) X[(long)j * M + m] = sh[c * ny + j];
+    if (m < M) X[(long)j * M + m] = sh[c * ls + fpad(j)];
   }
 }
 
@@ -258,8 +295,10 @@ template <typename R>
 __global__ void k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, int accumulate, int rows_per_block,
                             int logn) {
   extern __shared__ __align__(16) unsigned char sh_raw[];
-  C2<R> *sh = reinterpret_cast<C2<R> *>(sh_raw);
-  const int nx = pp.nx, M = pp.M;
+  C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
+  const int nx = pp.nx, M = pp.M, ls = fpad(nx);
+  C2<R> *sh = tw + nx / 2;
+  load_twiddles(tw, pp.twx, nx);
   const int row0 = blockIdx.x * rows_per_block;
   const int nrows = pp.ny * pp.nz;
   for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
@@ -274,10 +313,10 @@ __global__ void k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, 
         v.y = -v.y;
       }
     }
-    sh[l * nx + bitrev(m, logn)] = v;
+    sh[l * ls + fpad(bitrev(m, logn))] = v;
   }
   __syncthreads();
-  lds_fft(sh, nx, logn, rows_per_block, nx, pp.twx, true);
+  lds_fft(sh, nx, logn, rows_per_block, ls, tw, true);
   const R scale = (R)1.0 / ((R)pp.nx * pp.ny);
   for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
     const int l = t / nx, m = t % nx;
@@ -285,7 +324,7 @@ __global__ void k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, 
     if (row < nrows) {
       const int j = row % pp.ny + 1, k = row / pp.ny + 1;
       const long q = gix(g, m + 1, j, k);
-      const double val = (double)(sh[l * nx + m].x * scale);
+      const double val = (double)(sh[l * ls + fpad(m)].x * scale);
       Phi[q] = accumulate ? Phi[q] + val : val;
     }
   }
@@ -297,14 +336,22 @@ static int ilog2(int n) {
   return l;
 }
 
-static int rows_for(int nx) { return std::max(1, 1024 / nx); }
+static int rows_for(int nx) { return std::max(1, 2048 / nx); }
+static int padded(int n) { return n + n / 8; }
+
+// Dynamic LDS above 64 KiB must be opted into per kernel (gfx950 has 160 KiB
+// per workgroup).
+static void allow_lds(const void *fn, size_t bytes) {
+  if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
 
 template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
                       const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
-  size_t sh = (size_t)rows * pp.nx * sizeof(C2<R>);
+  size_t sh = ((size_t)rows * padded(pp.nx) + pp.nx / 2) * sizeof(C2<R>);
+  allow_lds((const void *)k_fft_x_fwd<T, R>, sh);
   hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, dt2_rec,
                      rhs_out, rows, ilog2(pp.nx));
 }
@@ -312,8 +359,9 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
 template <typename R>
 void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
   constexpr int TX = 8;
-  hipLaunchKernelGGL((k_fft_y<TX, R>), dim3(nblk(pp.M, TX), pp.nz), dim3(256), (size_t)TX * pp.ny * sizeof(C2<R>),
-                     st, pp, ilog2(pp.ny), inverse);
+  const size_t sh = ((size_t)TX * padded(pp.ny) + pp.ny / 2) * sizeof(C2<R>);
+  allow_lds((const void *)k_fft_y<TX, R>, sh);
+  hipLaunchKernelGGL((k_fft_y<TX, R>), dim3(nblk(pp.M, TX), pp.nz), dim3(256), sh, st, pp, ilog2(pp.ny), inverse);
 }
 
 template <typename R>
@@ -327,7 +375,8 @@ template <typename R>
 void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
-  size_t sh = (size_t)rows * pp.nx * sizeof(C2<R>);
+  size_t sh = ((size_t)rows * padded(pp.nx) + pp.nx / 2) * sizeof(C2<R>);
+  allow_lds((const void *)k_fft_x_inv<R>, sh);
   hipLaunchKernelGGL(k_fft_x_inv<R>, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, Phi, accumulate, rows,
                      ilog2(pp.nx));
 }
