@@ -12,8 +12,20 @@
  * the reference runs under OpenMP with atomics or races are executed in the
  * single-thread order (OMP_NUM_THREADS=1 semantics).
  *
- * Parity pins: see tests/test_oracle_*.py.  Functions without an independent
- * pin are listed as "parity unpinned" in oracle/README.md.
+ * Parity pins (tests/test_oracle_pins.py): Poisson solve (FD2 residual,
+ * eigenfunction, dense solve), Smagorinsky (linear shear, constant field),
+ * CD4 advection (Morinishi's textbook form on periodic grids, constant
+ * field), diffusion (Fourier mode), rough- and smooth-wall laws (closed form,
+ * fixed point), kappa limiter (closed form, constant field), ghost fills
+ * (mirror rules, idempotence), projection (divergence), pressure reference
+ * (top-layer mean), scalar budget (conservation, point source), RK3
+ * (coefficient identities, uniform forcing).
+ *
+ * Parity unpinned (no independent check; agreement with the GPU library is
+ * the only evidence): TimeStepLength's CFL scan, the immersed-boundary
+ * runtime (IBMomentum gathers, IBMassSources + Bound_Q fold, NullInterior,
+ * the ScalFl viscosity gather), DivergenceWM / ScalarExtra, and the
+ * even/odd ordering of the z pass of KappaScalar.
  */
 #ifndef ELMM_ORACLE_H
 #define ELMM_ORACLE_H

@@ -354,3 +354,107 @@ def test_rk3_low_storage_consistency():
     rho = np.array([0, -17 / 60, -5 / 12])
     assert np.allclose(alpha, (beta + rho) / 2)
     assert abs(2 * alpha.sum() - 1) < 1e-15
+
+
+def test_wall_model_smooth_wall_law_fixed_point():
+    """Smooth wall (z0 = 0, the law used on obstacle walls): the u* behind
+    the replaced bottom flux satisfies the log law u/u* = ln(d u*/nu)/0.4 +
+    5.5 to the iteration tolerance 1e-4 (WMFlatUstar, src/wallmodels.f90:
+    1190-1240)."""
+    cfg = small_cfg(8, 8, nscalars=0, z0_bottom=0.0, nu=1e-5)
+    o, _ = make(cfg)
+    U = ghosted(cfg)
+    U[3:11, 3:11, 3:11] = 2.0
+    o.upload(capi.F_U, U)
+    o.upload(capi.F_V, ghosted(cfg))
+    o.upload(capi.F_W, ghosted(cfg))
+    o.op_bound_uvw()
+    o.op_sgs()
+    o.upload(capi.F_VISC, ghosted(cfg) + 1e-3)
+    o.op_diffusion()
+    us2 = -o.download(F_USTAR)[3:11, 3:11, 3]      # u*^2 / dz with dz = 1
+    ustar = np.sqrt(us2)
+    zplus = 0.5 * ustar / 1e-5
+    assert np.all(zplus > 30)
+    law = np.log(zplus) / 0.4 + 5.5
+    assert np.allclose(2.0 / ustar, law, rtol=3e-4, atol=0)
+
+
+def _bound_uvw_random(cfg, seed):
+    o, _ = make(cfg)
+    rng = np.random.default_rng(seed)
+    arrs = []
+    for c in range(3):
+        a = ghosted(cfg)
+        a[3:3 + cfg.nx, 3:3 + cfg.ny, 3:3 + cfg.nz] = rng.standard_normal((cfg.nx, cfg.ny, cfg.nz))
+        if c == 2:
+            a[:, :, 3 + cfg.nz - 1] = 0.0    # W interior is 1..nz-1 with walls
+        arrs.append(a)
+        o.upload(c, a)
+    o.op_bound_uvw()
+    return [o.download(c) for c in range(3)], arrs
+
+
+def test_ghost_fill_mirror_rules():
+    """BoundUVW on random interiors (src/boundaries.f90:398-1791): periodic
+    x/y copies; no-slip bottom: tangential odd mirror U(1-m) = -U(m), wall
+    normal W(0) = 0 and W(-m) = -W(m); free-slip top: the tangential ghosts
+    repeat the top value U(nz+m) = U(nz) (src/boundaries.f90:1743-1750),
+    normal W(nz) = 0, W(nz+m) = -W(nz-m)."""
+    cfg = small_cfg(8, 6, nscalars=0)
+    (U, V, W), _ = _bound_uvw_random(cfg, 7)
+    n, nz = 8, 6
+    I = lambda a, k: a[3:3 + n, 3:3 + n, 2 + k]   # lower bound -2
+    for A in (U, V, W):
+        assert np.array_equal(A[0:3, 3:3 + n, 3:3 + nz], A[n:n + 3, 3:3 + n, 3:3 + nz])
+        assert np.array_equal(A[3 + n:6 + n, 3:3 + n, 3:3 + nz], A[3:6, 3:3 + n, 3:3 + nz])
+        assert np.array_equal(A[3:3 + n, 0:3, 3:3 + nz], A[3:3 + n, n:n + 3, 3:3 + nz])
+    for A in (U, V):
+        for m in (1, 2, 3):
+            assert np.array_equal(I(A, 1 - m), -I(A, m))            # bottom no-slip
+            assert np.array_equal(I(A, nz + m), I(A, nz))            # top free-slip
+    assert not I(W, 0).any() and not I(W, nz).any()
+    for m in (1, 2):
+        assert np.array_equal(I(W, -m), -I(W, m))
+        assert np.array_equal(I(W, nz + m), -I(W, nz - m))
+
+
+def test_ghost_fill_idempotent():
+    cfg = small_cfg(8, 6, nscalars=0)
+    o, _ = make(cfg)
+    (U, V, W), _ = _bound_uvw_random(cfg, 8)
+    for c, a in enumerate((U, V, W)):
+        o.upload(c, a)
+    o.op_bound_uvw()
+    for c, a in enumerate((U, V, W)):
+        assert np.array_equal(o.download(c), a)
+
+
+def test_point_source_mass_budget():
+    """Fluid at rest, zero-flux walls, one point source of F units/s: the
+    scalar content grows by exactly F dt per step because the three RK stage
+    weights add to one (point_source, src/pointsources.f90:20-66; FluxKernel,
+    src/volumesources.f90:757-771; scalar RK, src/scalars.f90:279-283)."""
+    cfg = small_cfg(8, 6, wall_model=0, sgs_model=0)
+    cfg.btype[4] = capi.FREESLIP
+    case = capi.Case(cfg, sources=[(3.3, 4.7, 2.2, 0.75, 0)])
+    o = Oracle(cfg, case)
+    z = ghosted(cfg)
+    for c in range(3):
+        o.upload(c, z)
+    o.upload(capi.F_PR, z)
+    o.upload(capi.F_SCALAR, z)
+    o.init()
+    o.step(4)
+    s = o.download(capi.F_SCALAR)[3:11, 3:11, 3:9]
+    assert abs(s.sum() - 4 * 0.75 * 0.05) < 1e-14
+
+
+def test_pressure_reference_top_mean_zero():
+    """FixPressureToReference with the top boundary as reference
+    (src/pressure.f90:994-1229): after every step the mean of Pr over the
+    top layer is zero to rounding."""
+    cfg = small_cfg(16, 12)
+    o, _ = run_steps(cfg, 2)
+    pr = o.download(capi.F_PR)[3:19, 3:19, 3 + 11]
+    assert abs(pr.mean()) < 1e-15 * max(1.0, np.abs(pr).max())
