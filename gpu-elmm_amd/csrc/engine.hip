@@ -1030,6 +1030,14 @@ extern "C" int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm
   return rc;
 }
 
+extern "C" int elmm_create_case(const elmm_config *cfg, const elmm_case *c, elmm_handle **out) {
+  if (!cfg || !c || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_create_case");
+  elmm_lists v;
+  int rc = elmm_case_view(c, &v);
+  if (rc) return rc;
+  return elmm_create(cfg, &v, out);
+}
+
 extern "C" int elmm_upload(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], const double *src) {
   if (!h || !ext || !src) return elmm::fail(ELMM_E_ARG, "null argument to elmm_upload");
   return guarded([&] { h->e->xfer(field, lb, ext, const_cast<double *>(src), true); });

@@ -65,8 +65,35 @@ def build(verbose=False, jobs=None):
     tmp = lib_path() + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, lib_path())
+    build_fortran_host()
     shutil.rmtree(OBJDIR, ignore_errors=True)
     return lib_path()
+
+
+FORTRAN = os.path.join(PKG, "fortran")
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+
+def host_path():
+    return os.path.join(LIBDIR, "elmm_host")
+
+
+def build_fortran_host():
+    """The iso_c_binding module and the namelist host (gpu-elmm_amd/fortran),
+    linked against the library; skipped when amdflang is absent."""
+    if not os.path.exists(FC):
+        return None
+    mod = os.path.join(OBJDIR, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for f in ("elmm_gpu.f90", "elmm_host.f90"):
+        obj = os.path.join(OBJDIR, f + ".o")
+        _run([FC, "-O2", "-module-dir", mod, "-I", mod, "-c", os.path.join(FORTRAN, f), "-o", obj])
+        objs.append(obj)
+    tmp = host_path() + ".tmp"
+    _run([FC, "-o", tmp] + objs + ["-L", LIBDIR, "-lelmm_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, host_path())
+    return host_path()
 
 
 if __name__ == "__main__":

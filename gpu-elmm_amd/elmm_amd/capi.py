@@ -96,6 +96,7 @@ EXPORTS = {
     "elmm_case_view": (C.c_int, [C.c_void_p, C.POINTER(Lists)]),
     "elmm_case_free": (None, [C.c_void_p]),
     "elmm_create": (C.c_int, [C.POINTER(Config), C.POINTER(Lists), C.POINTER(C.c_void_p)]),
+    "elmm_create_case": (C.c_int, [C.POINTER(Config), C.c_void_p, C.POINTER(C.c_void_p)]),
     "elmm_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _pd]),
     "elmm_download": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _pd]),
     "elmm_init": (C.c_int, [C.c_void_p]),

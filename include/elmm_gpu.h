@@ -201,6 +201,11 @@ void elmm_case_free(elmm_case *c);
  * allocations of TMarchRK3 (src/tsteps.f90:54-77). */
 int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm_handle **out);
 
+/* elmm_create with the lists of a case from elmm_case_build (the case may be
+ * freed afterwards); the form a Fortran host binds, which then never needs
+ * the layout of elmm_lists.  replaces: as elmm_create. */
+int elmm_create_case(const elmm_config *cfg, const elmm_case *c, elmm_handle **out);
+
 /* Copy one field between a host array and HBM.  The host array has lower
  * bound lb in every direction and extents ext[3]; the overlap with the
  * field's ghosted device extent (-2..n+3) is copied. */
