@@ -113,9 +113,9 @@ std::string validate_config(const elmm_config &c) {
     if (c.nscalars > 0 && b != ELMM_BC_PERIODIC && b != ELMM_BC_NEUMANN && b != ELMM_BC_DIRICHLET)
       return "scalar boundary type must be periodic, Neumann or Dirichlet";
   }
-  if (c.advection_method != 4 && c.advection_method != 2) return "advection_method must be 4 or 2";
-  if (c.advection_method == 4 && c.stretched_z) return "CD4 advection needs a uniform z grid";
-  if (c.advection_method == 2 && !c.stretched_z) return "advection_method 2 is implemented for stretched z only";
+  if (c.stretched_z)
+    return "stretched z grids (CDU/V/W_variable_z, variable-z diffusion) are not built in this version";
+  if (c.advection_method != 4) return "advection_method must be 4 (CD4div) on uniform grids";
   if (c.sgs_model != 0 && c.sgs_model != 1) return "sgs_model must be 0 or 1";
   if (c.nscalars < 0 || c.nscalars > 8) return "nscalars must be in 0..8";
   if (c.precision != ELMM_FP64 && c.precision != ELMM_FP32_REFINED) return "unknown precision";
