@@ -10,7 +10,7 @@ import numpy as np
 
 from . import capi
 
-NU_DEFAULT = 1.0 / 70000                      # molecular_viscosity, src/parameters.f90:98
+NU_DEFAULT = 1.0 / 70000                      # molecular_viscosity, src/parameters.f90:128
 NU_SCALAR_DEFAULT = (1.0 / 70000) / float(np.float32(0.7))  # 1._knd/70000/0.7 (default-real 0.7)
 
 

@@ -23,7 +23,10 @@ program elmm_host
 
   integer :: nx = 32, ny = 32, nz = 16, nscalars = 1, precision = 0, top_bc = 2
   real(c_double) :: dx = 1, dy = 1, dz = 1, x0 = 0, y0 = 0, z0 = 0
-  real(c_double) :: nu = 1.0_c_double / 70000, nu_scalar = 1.0_c_double / 70000, c_smag = 0.122_c_double
+  ! molecular_viscosity and molecular_diffusivity defaults (src/parameters.f90:128-129: 1/70000 and
+  ! 1/70000/0.7 with 0.7 a default-real literal)
+  real(c_double) :: nu = 1.0_c_double / 70000, nu_scalar = (1.0_c_double / 70000) / real(0.7, c_double)
+  real(c_double) :: c_smag = 0.122_c_double
   real(c_double) :: z0_bottom = 0.01_c_double, dt = 0.05_c_double, u0 = 1
   integer :: nsteps = 10, report_every = 5, n_cuboids = 0, n_sources = 0
   integer, parameter :: maxobj = 256
