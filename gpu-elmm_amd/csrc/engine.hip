@@ -205,7 +205,7 @@ struct Engine : EngineIface {
   DevBuf null_ijk[3];
   int null_n[3] = {0, 0, 0};
   // pressure
-  DevBuf X, twx, twy, lamx, lamy, az, cz, az64, cz64;
+  DevBuf X, twx, twy, lamx, lamy, az64, cz64;
   PoissonPlan<T> pp{};
   DevBuf partial, cfl_partial, div_partial;
   int top_blocks = 0, red_blocks = 0, cfl_blocks = 0;
@@ -215,6 +215,9 @@ struct Engine : EngineIface {
   // length (ELMM_ZCHUNK, default 64 levels)
   bool fused_momentum = false;
   int zchunk = 64;
+  // scalar kernel: z-marching on uniform grids (ELMM_SCALAR=cellwise selects
+  // the one-thread-per-cell kernel)
+  bool zmarch_scalar = false;
 
   IBDev ibdev(int c) const {
     return IBDev{ib_n[c], ib_tgt[c].as<int32_t>(), ib_ptr[c].as<int32_t>(), ib_src[c].as<int32_t>(),
@@ -273,6 +276,8 @@ void Engine<T>::setup_geo() {
   {
     const char *m = std::getenv("ELMM_MOMENTUM");
     fused_momentum = momentum_zm_supported(g) && !(m && std::string(m) == "percomp");
+    const char *sk = std::getenv("ELMM_SCALAR");
+    zmarch_scalar = !g.variable_z && !(sk && std::string(sk) == "cellwise");
     const char *zc = std::getenv("ELMM_ZCHUNK");
     if (zc && std::atoi(zc) > 0) zchunk = std::atoi(zc);
   }
@@ -352,15 +357,12 @@ void Engine<T>::setup_poisson() {
     a[k - 1] = k > 1 ? 1 / (Grid1D::at(grid.dzW, k - 1) * Grid1D::at(grid.dzPr, k)) : 0;
     c[k - 1] = k < nz ? 1 / (Grid1D::at(grid.dzW, k) * Grid1D::at(grid.dzPr, k)) : 0;
   }
-  auto cast = [](const std::vector<double> &v) { return std::vector<T>(v.begin(), v.end()); };
-  upload_vec(lamx, cast(lx));
-  upload_vec(lamy, cast(ly));
-  upload_vec(az, cast(a));
-  upload_vec(cz, cast(c));
+  upload_vec(lamx, lx);
+  upload_vec(lamy, ly);
   upload_vec(az64, a);
   upload_vec(cz64, c);
-  pp = PoissonPlan<T>{nx, ny, nz, M, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<T>(), lamy.as<T>(), az.as<T>(),
-                      cz.as<T>(), g.dzPr, g.dzW};
+  pp = PoissonPlan<T>{nx, ny, nz, M, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<double>(), lamy.as<double>(),
+                      az64.as<double>(), cz64.as<double>(), g.dzPr, g.dzW};
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
   cfl_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
@@ -871,9 +873,12 @@ void Engine<T>::stage(int st, bool last) {
       });
     for (int s = 0; s < nsc; s++) {
       timed(ELMM_T_SCALAR, [&] {
-        launch_scalar(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(),
-                      code_s.as<uint8_t>(), mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc,
-                      stream);
+        if (zmarch_scalar)
+          launch_scalar_zm(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(), code_s.as<uint8_t>(),
+                           mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, zchunk, stream);
+        else
+          launch_scalar(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(), code_s.as<uint8_t>(),
+                        mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, stream);
       });
       if (sc_terms[s]->n)
         timed(ELMM_T_SCALAR_CORR, [&] {

@@ -170,35 +170,39 @@ __global__ void k_fft_y(PoissonPlan<R> pp, int logn, int inverse) {
 // Pass 3: per (m, j) mode, a_k x_{k-1} + (lam - a_k - c_k) x_k + c_k x_{k+1}
 // = d_k, solved by parallel cyclic reduction in LDS.  The singular mode
 // (m=j=0) pins x_1 = 0 and is integrated exactly (see below).
-template <int TX, typename R>
+// R: storage type of the spectral array; A: arithmetic of the elimination
+// (double for both forms: the lowest modes' tridiagonals are too
+// ill-conditioned for FP32 elimination inside a one-sweep refinement).
+template <int TX, typename R, typename A_t = double>
 __global__ void k_solve_z(PoissonPlan<R> pp) {
   extern __shared__ __align__(16) unsigned char sh_raw[];
-  using V = C2<R>;
+  using V = C2<A_t>;
   const int nz = pp.nz, ny = pp.ny, M = pp.M;
   const int m0 = blockIdx.x * TX, j = blockIdx.y;
   const int N = TX * nz;
   V *D = reinterpret_cast<V *>(sh_raw);
-  R *A = reinterpret_cast<R *>(D + N), *B = A + N, *C = B + N;
+  A_t *A = reinterpret_cast<A_t *>(D + N), *B = A + N, *C = B + N;
   const long plane = (long)ny * M;
   for (int t = threadIdx.x; t < N; t += blockDim.x) {
     const int c = t % TX, k = t / TX;
     const int m = m0 + c;
     const int s = c * nz + k;
-    V d = mk2((R)0, (R)0);
-    R lam = 0;
+    V d = mk2((A_t)0, (A_t)0);
+    A_t lam = 0;
     if (m < M) {
-      d = pp.X[(long)k * plane + (long)j * M + m];
-      lam = pp.lamx[m] + pp.lamy[j];
+      const C2<R> x = pp.X[(long)k * plane + (long)j * M + m];
+      d = mk2((A_t)x.x, (A_t)x.y);
+      lam = (A_t)pp.lamx[m] + (A_t)pp.lamy[j];
     }
-    R a = pp.az[k], cc = pp.cz[k];
-    R b = lam - a - cc;
+    A_t a = (A_t)pp.az[k], cc = (A_t)pp.cz[k];
+    A_t b = lam - a - cc;
     if ((m == 0 && j == 0) || m >= M) {
       // identity rows: padding columns, and the singular mean mode, which is
       // integrated exactly after the reduction (its D survives unchanged)
       a = 0;
       b = 1;
       cc = 0;
-      if (m >= M) d = mk2((R)0, (R)0);
+      if (m >= M) d = mk2((A_t)0, (A_t)0);
     }
     A[s] = a;
     B[s] = b;
@@ -207,25 +211,25 @@ __global__ void k_solve_z(PoissonPlan<R> pp) {
   }
   __syncthreads();
   for (int h = 1; h < nz; h <<= 1) {
-    R na[8], nb[8], nc[8];
+    A_t na[8], nb[8], nc[8];
     V nd[8];
     int cnt = 0;
     for (int t = threadIdx.x; t < N; t += blockDim.x, cnt++) {
       const int c = t / nz, k = t % nz;
       const int s = c * nz + k;
-      R a = A[s], b = B[s], cc = C[s];
+      A_t a = A[s], b = B[s], cc = C[s];
       V d = D[s];
-      R a2 = 0, c2 = 0;
+      A_t a2 = 0, c2 = 0;
       V d2 = d;
-      R b2 = b;
+      A_t b2 = b;
       if (k - h >= 0) {
-        const R k1 = a / B[s - h];
+        const A_t k1 = a / B[s - h];
         a2 = -A[s - h] * k1;
         b2 = b2 - C[s - h] * k1;
         d2 = mk2(d2.x - D[s - h].x * k1, d2.y - D[s - h].y * k1);
       }
       if (k + h < nz) {
-        const R k2 = cc / B[s + h];
+        const A_t k2 = cc / B[s + h];
         c2 = -C[s + h] * k2;
         b2 = b2 - A[s + h] * k2;
         d2 = mk2(d2.x - D[s + h].x * k2, d2.y - D[s + h].y * k2);
@@ -253,9 +257,9 @@ __global__ void k_solve_z(PoissonPlan<R> pp) {
     // x_{k+1} = x_k + G_k dzW(k).  Both sums are compensated (Kahan), so the
     // mode is exact to rounding instead of carrying the nz^2 growth of an
     // elimination on the singular operator.
-    V G = mk2((R)0, (R)0), Gc = G;
+    V G = mk2((A_t)0, (A_t)0), Gc = G;
     for (int k = nz; k >= 2; k--) {
-      const R dz = (R)pp.dzPr[k];
+      const A_t dz = (A_t)pp.dzPr[k];
       V t = mk2(-D[k - 1].x * dz, -D[k - 1].y * dz);
       V y = mk2(t.x - Gc.x, t.y - Gc.y);
       V s2 = mk2(G.x + y.x, G.y + y.y);
@@ -264,10 +268,10 @@ __global__ void k_solve_z(PoissonPlan<R> pp) {
       A[k - 2] = G.x;  // column 0 of A and C is free after the reduction
       C[k - 2] = G.y;
     }
-    V x = mk2((R)0, (R)0), xc = x;
+    V x = mk2((A_t)0, (A_t)0), xc = x;
     D[0] = x;
     for (int k = 1; k <= nz - 1; k++) {
-      const R dz = (R)pp.dzW[k];
+      const A_t dz = (A_t)pp.dzW[k];
       V t = mk2(A[k - 1] * dz, C[k - 1] * dz);
       V y = mk2(t.x - xc.x, t.y - xc.y);
       V s2 = mk2(x.x + y.x, x.y + y.y);
@@ -282,8 +286,8 @@ __global__ void k_solve_z(PoissonPlan<R> pp) {
     const int m = m0 + c;
     const int s = c * nz + k;
     if (m < M) {
-      const R b = B[s];
-      pp.X[(long)k * plane + (long)j * M + m] = mk2(D[s].x / b, D[s].y / b);
+      const A_t b = B[s];
+      pp.X[(long)k * plane + (long)j * M + m] = mk2((R)(D[s].x / b), (R)(D[s].y / b));
     }
   }
 }
@@ -367,8 +371,8 @@ void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
 template <typename R>
 void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t st) {
   constexpr int TX = 8;
-  size_t sh = (size_t)TX * pp.nz * (3 * sizeof(R) + sizeof(C2<R>));
-  hipLaunchKernelGGL((k_solve_z<TX, R>), dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
+  size_t sh = (size_t)TX * pp.nz * (3 * sizeof(double) + sizeof(double2));
+  hipLaunchKernelGGL((k_solve_z<TX, R, double>), dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
 }
 
 template <typename R>

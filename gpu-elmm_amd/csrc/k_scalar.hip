@@ -28,6 +28,25 @@ __device__ inline Real flux_limiter(Real r) {
   return (Real)0.0 > c ? (Real)0.0 : c;
 }
 
+// Flux through the face between s0 and s1 with face velocity u, from the four
+// values s_m1 = S(-1), s0 = S(0), s1 = S(+1), s2 = S(+2) along the direction.
+template <typename Real>
+__device__ inline Real kappa_flux_v(Real u, Real s_m1, Real s0, Real s1, Real s2) {
+  const Real eps = (Real)1e-8f;  // default-real literal in the reference
+  Real sr, sl;
+  if (u > 0) {
+    sr = s1 - s0;
+    sl = s0 - s_m1;
+  } else {
+    sr = s0 - s1;
+    sl = s1 - s2;
+  }
+  const Real sg = copysign((Real)1.0, sl);
+  const Real slope = flux_limiter<Real>((sr + eps * sg) / (sl + eps * sg));
+  if (u > 0) return u * (s0 + (s0 - s_m1) * slope / (Real)2.0);
+  return u * (s1 + (s1 - s2) * slope / (Real)2.0);
+}
+
 // Flux through the upper face of cell p along stride s with face velocity u.
 template <typename Real>
 __device__ inline Real kappa_flux(const Real *__restrict__ S, long p, long s, Real u) {
@@ -113,6 +132,156 @@ void launch_scalar(const Geo &g, const Real *U, const Real *V, const Real *W, co
                    const uint8_t *mz, StageConst sc, hipStream_t st) {
   dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
   hipLaunchKernelGGL(k_scalar<Real>, gr, b, 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code, mx, my, mz, sc);
+}
+
+// ------------------------------------------------------------------------
+// z-marching form of k_scalar for uniform grids: a block owns a TXS x TYS
+// tile of columns; S at level k (halo 2) and TDiff at level k (halo 1) are
+// LDS planes, double-buffered with the next level fetched into registers
+// while the current one is computed; the column values S(k-2..k+2),
+// TDiff(k-1..k+1), W(k-1..k) and the z masks live in registers.  Per cell the
+// operations are those of scalar_cell + scalar_finish in the same order.
+constexpr int TXS = 64, TYS = 4, NTS = TXS * TYS;
+constexpr int SPW = TXS + 4, SPH = TYS + 4, SPS = SPW * SPH;   // S plane, halo 2
+constexpr int TPW = TXS + 2, TPH = TYS + 2, TPS = TPW * TPH;   // TDiff plane, halo 1
+constexpr int SLD = (SPS + NTS - 1) / NTS, TLD = (TPS + NTS - 1) / NTS;
+
+template <typename Real>
+__global__ void __launch_bounds__(NTS)
+    k_scalar_zm(Geo g, const Real *__restrict__ U, const Real *__restrict__ V, const Real *__restrict__ W,
+                const Real *__restrict__ T, const Real *__restrict__ S, Real *__restrict__ Snew,
+                Real *__restrict__ Sadv, const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
+                const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc, int kchunk) {
+  __shared__ Real sS[2][SPS];
+  __shared__ Real sT[2][TPS];
+  const int tid = threadIdx.x, tx = tid % TXS, ty = tid / TXS;
+  const int i0 = 1 + blockIdx.x * TXS, j0 = 1 + blockIdx.y * TYS;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kbeg = 1 + blockIdx.z * kchunk, kend = min(kbeg + kchunk - 1, g.nz);
+  const long sy = g.sy, sz = g.sz;
+  long soff[SLD], toff[TLD];
+  int sidx[SLD], tidx[TLD];
+#pragma unroll
+  for (int r = 0; r < SLD; r++) {
+    const int e = tid + r * NTS;
+    sidx[r] = e < SPS ? e : -1;
+    soff[r] = gix(g, min(i0 - 2 + e % SPW, g.nx + 3), min(j0 - 2 + e / SPW, g.ny + 3), 0);
+  }
+#pragma unroll
+  for (int r = 0; r < TLD; r++) {
+    const int e = tid + r * NTS;
+    tidx[r] = e < TPS ? e : -1;
+    toff[r] = gix(g, min(i0 - 1 + e % TPW, g.nx + 3), min(j0 - 1 + e / TPW, g.ny + 3), 0);
+  }
+  const long pc = gix(g, min(i, g.nx), min(j, g.ny), 0);
+  const bool in_xy = i <= g.nx && j <= g.ny;
+  const int ls = (ty + 2) * SPW + tx + 2, lt = (ty + 1) * TPW + tx + 1;
+
+  // prologue: planes of level kbeg, column rings
+#pragma unroll
+  for (int r = 0; r < SLD; r++)
+    if (sidx[r] >= 0) sS[kbeg & 1][sidx[r]] = S[soff[r] + (long)kbeg * sz];
+#pragma unroll
+  for (int r = 0; r < TLD; r++)
+    if (tidx[r] >= 0) sT[kbeg & 1][tidx[r]] = T[toff[r] + (long)kbeg * sz];
+  Real cs[5], ct[3], cw[2];
+  uint8_t cm[2];
+#pragma unroll
+  for (int q = 0; q < 5; q++) cs[q] = S[pc + (long)(kbeg - 2 + q) * sz];
+#pragma unroll
+  for (int q = 0; q < 3; q++) ct[q] = T[pc + (long)(kbeg - 1 + q) * sz];
+  cw[0] = W[pc + (long)(kbeg - 1) * sz];
+  cw[1] = W[pc + (long)kbeg * sz];
+  cm[0] = mz[pc + (long)(kbeg - 1) * sz];
+  cm[1] = mz[pc + (long)kbeg * sz];
+  __syncthreads();
+
+  const Real Ax = 1 / (Real)g.dxmin, Ay = 1 / (Real)g.dymin, Az = 1 / (Real)g.dzmin;
+  const Real Dx = 1 / (2 * ((Real)g.dxmin * (Real)g.dxmin)), Dy = 1 / (2 * ((Real)g.dymin * (Real)g.dymin)),
+             Dz = 1 / (2 * ((Real)g.dzmin * (Real)g.dzmin));
+
+  for (int k = kbeg; k <= kend; k++) {
+    const bool more = k < kend;
+    const int b = k & 1;
+    Real ps[SLD], pt[TLD], ns = 0, nt = 0, nw = 0;
+    uint8_t nm = 0;
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < SLD; r++)
+        if (sidx[r] >= 0) ps[r] = S[soff[r] + (long)(k + 1) * sz];
+#pragma unroll
+      for (int r = 0; r < TLD; r++)
+        if (tidx[r] >= 0) pt[r] = T[toff[r] + (long)(k + 1) * sz];
+      ns = S[pc + (long)(k + 3) * sz];
+      nt = T[pc + (long)(k + 2) * sz];
+      nw = W[pc + (long)(k + 1) * sz];
+      nm = mz[pc + (long)(k + 1) * sz];
+    }
+    if (in_xy) {
+      const long p = gix(g, i, j, k);
+      const uint8_t cd = code[p];
+      if (!(cd & CODE_CORRECTED)) {
+        const Real *P = sS[b];
+        auto Sx = [&](int o) { return P[ls + o]; };
+        auto Sy = [&](int o) { return P[ls + o * SPW]; };
+        Real a = 0;
+        if (mx[p - 1]) a = a + Ax * kappa_flux_v<Real>(U[p - 1], Sx(-2), Sx(-1), Sx(0), Sx(1));
+        if (mx[p]) a = a - Ax * kappa_flux_v<Real>(U[p], Sx(-1), Sx(0), Sx(1), Sx(2));
+        if (my[p - sy]) a = a + Ay * kappa_flux_v<Real>(V[p - sy], Sy(-2), Sy(-1), Sy(0), Sy(1));
+        if (my[p]) a = a - Ay * kappa_flux_v<Real>(V[p], Sy(-1), Sy(0), Sy(1), Sy(2));
+        const bool mlo = cm[0] != 0, mhi = cm[1] != 0;
+        if ((k & 1) == 0) {
+          if (mhi) a = a - Az * kappa_flux_v<Real>(cw[1], cs[1], cs[2], cs[3], cs[4]);
+          if (mlo) a = a + Az * kappa_flux_v<Real>(cw[0], cs[0], cs[1], cs[2], cs[3]);
+        } else {
+          if (mlo) a = a + Az * kappa_flux_v<Real>(cw[0], cs[0], cs[1], cs[2], cs[3]);
+          if (mhi) a = a - Az * kappa_flux_v<Real>(cw[1], cs[1], cs[2], cs[3], cs[4]);
+        }
+        if (cd & CODE_SOLID) a = 0;
+        const Real *Q = sT[b];
+        const Real t0 = ct[1], s0 = cs[2];
+        a = a + (Q[lt + 1] + t0) * (Sx(1) - s0) * Dx;
+        a = a - (t0 + Q[lt - 1]) * (s0 - Sx(-1)) * Dx;
+        a = a + (Q[lt + TPW] + t0) * (Sy(1) - s0) * Dy;
+        a = a - (t0 + Q[lt - TPW]) * (s0 - Sy(-1)) * Dy;
+        a = a + (ct[2] + t0) * (cs[3] - s0) * Dz;
+        a = a - (t0 + ct[0]) * (s0 - cs[1]) * Dz;
+        Real s2 = sc.stage > 0 ? Sadv[p] * (Real)sc.rho_dt : (Real)0.0;
+        s2 = s2 + a * (Real)sc.beta_dt;
+        Sadv[p] = a;
+        Snew[p] = s0 + s2;
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < SLD; r++)
+        if (sidx[r] >= 0) sS[b ^ 1][sidx[r]] = ps[r];
+#pragma unroll
+      for (int r = 0; r < TLD; r++)
+        if (tidx[r] >= 0) sT[b ^ 1][tidx[r]] = pt[r];
+#pragma unroll
+      for (int q = 0; q < 4; q++) cs[q] = cs[q + 1];
+      cs[4] = ns;
+      ct[0] = ct[1];
+      ct[1] = ct[2];
+      ct[2] = nt;
+      cw[0] = cw[1];
+      cw[1] = nw;
+      cm[0] = cm[1];
+      cm[1] = nm;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename Real>
+void launch_scalar_zm(const Geo &g, const Real *U, const Real *V, const Real *W, const Real *TDiff, const Real *S,
+                      Real *Snew, Real *Sadv, const uint8_t *code, const uint8_t *mx, const uint8_t *my,
+                      const uint8_t *mz, StageConst sc, int kchunk, hipStream_t st) {
+  if (kchunk <= 0) kchunk = g.nz;
+  dim3 b(NTS), gr(nblk(g.nx, TXS), nblk(g.ny, TYS), nblk(g.nz, kchunk));
+  hipLaunchKernelGGL(k_scalar_zm<Real>, gr, b, 0, st, g, U, V, W, TDiff, S, Snew, Sadv, code, mx, my, mz, sc,
+                     kchunk);
 }
 
 // Cells with ordered extra terms: scalar wall-face flux removal
@@ -221,7 +390,10 @@ void launch_tdiff_init(const Geo &g, const Real *Visc, Real *TDiff, double nu, d
                                       StageConst, TermList, const double *, const double *, hipStream_t);          \
   template void launch_wm_div<R>(const Geo &, WMScalarPoints, const int32_t *, const R *, const R *, const R *,    \
                                  hipStream_t);                                                                    \
-  template void launch_tdiff_init<R>(const Geo &, const R *, R *, double, double, hipStream_t);
+  template void launch_tdiff_init<R>(const Geo &, const R *, R *, double, double, hipStream_t);                 \
+  template void launch_scalar_zm<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *,  \
+                                    const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *, StageConst,\
+                                    int, hipStream_t);
 ELMM_INST_SCALAR(double)
 ELMM_INST_SCALAR(float)
 

@@ -82,6 +82,11 @@ template <typename R>
 void launch_scalar(const Geo &g, const R *U, const R *V, const R *W, const R *TDiff, const R *S, R *Snew, R *Sadv,
                    const uint8_t *code, const uint8_t *mx, const uint8_t *my, const uint8_t *mz, StageConst sc,
                    hipStream_t st);
+// z-marching form for uniform grids (same per-cell arithmetic as launch_scalar)
+template <typename R>
+void launch_scalar_zm(const Geo &g, const R *U, const R *V, const R *W, const R *TDiff, const R *S, R *Snew, R *Sadv,
+                      const uint8_t *code, const uint8_t *mx, const uint8_t *my, const uint8_t *mz, StageConst sc,
+                      int kchunk, hipStream_t st);
 template <typename R>
 void launch_scalar_corr(const Geo &g, const R *U, const R *V, const R *W, const R *TDiff, const R *S, R *Snew,
                         R *Sadv, const uint8_t *code, const uint8_t *mx, const uint8_t *my, const uint8_t *mz,
@@ -101,8 +106,8 @@ struct PoissonPlan {
   int nx, ny, nz, M;            // M = nx/2+1
   C2<R> *X;                     // spectral work array M*ny*nz
   const C2<R> *twx, *twy;       // exp(-2 pi i t/N), t < N/2
-  const R *lamx, *lamy;         // eigenvalues of the periodic directions
-  const R *az, *cz;             // z coupling coefficients per k (0-based)
+  const double *lamx, *lamy;    // eigenvalues of the periodic directions (FP64 in both forms)
+  const double *az, *cz;        // z coupling coefficients per k (0-based)
   const double *dzPr, *dzW;     // grid spacings indexed like the reference (k = 1..nz)
 };
 template <typename T, typename R>
