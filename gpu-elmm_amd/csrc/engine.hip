@@ -179,7 +179,7 @@ struct Engine : EngineIface {
   Grid1D grid;
   int nsc = 0;
   // fields
-  DevBuf U, V, W, U2, V2, W2, Us, Vs, Ws, Pr, Visc, TDiff, Q, Phi, RHS, Rres;
+  DevBuf U, V, W, U2, V2, W2, Us, Vs, Ws, Pr, Visc, TDiff, Q, Phi, RHS;
   std::vector<std::unique_ptr<DevBuf>> S, Snew, Sadv;
   T *u, *v, *w, *u2, *v2, *w2;  // current rotation of U/U2
   std::vector<T *> s_cur, s_nxt;
@@ -646,7 +646,6 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   const size_t fb = g.ntot * sizeof(T), fb64 = g.ntot * sizeof(double);
   for (DevBuf *b : {&U, &V, &W, &U2, &V2, &W2, &Us, &Vs, &Ws, &Pr, &Visc, &TDiff}) b->alloc(fb);
   for (DevBuf *b : {&Q, &Phi, &RHS}) b->alloc(fb64);
-  if (refine_iters > 0) Rres.alloc(fb64);
   for (int s = 0; s < nsc; s++) {
     S.emplace_back(new DevBuf);
     Snew.emplace_back(new DevBuf);
@@ -774,24 +773,22 @@ void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, b
   double *q = use_q ? Q.as<double>() : nullptr;
   // first solve; the precision form keeps the FP64 right-hand side
   double *rhs_keep = refine_iters > 0 ? RHS.as<double>() : nullptr;
-  timed(ELMM_T_POISSON_X_FWD, [&] { launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, dt2_rec, rhs_keep, stream); });
+  timed(ELMM_T_POISSON_X_FWD,
+        [&] { launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, nullptr, dt2_rec, rhs_keep, stream); });
   poisson_solve(0);
-  // iterative refinement: r = RHS - A Phi in FP64, FP32 solve of A e = r,
-  // Phi += e (FP64)
+  // iterative refinement: the x pass evaluates r = RHS - A Phi in FP64 and
+  // transforms it; FP32 solve of A e = r; Phi += e (FP64)
   for (int it = 0; it < refine_iters; it++) {
     timed(ELMM_T_REFINE, [&] {
-      launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(),
-                             Rres.as<double>(), div_partial.as<double>(), red_blocks, stream);
+      launch_fft_x_fwd(g, pp, a, b, c, nullptr, RHS.as<double>(), Phi.as<double>(), 1.0, nullptr, stream);
     });
-    timed(ELMM_T_POISSON_X_FWD,
-          [&] { launch_fft_x_fwd(g, pp, a, b, c, nullptr, Rres.as<double>(), 1.0, nullptr, stream); });
     poisson_solve(1);
   }
   if (diag && refine_iters > 0) {
     // FP64 residual of the final Phi; the divergence of the FP64-corrected
     // velocity equals coef * r before the correction is rounded to T
-    launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(),
-                           Rres.as<double>(), div_partial.as<double>(), red_blocks, stream);
+    launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(), nullptr,
+                           div_partial.as<double>(), red_blocks, stream);
     last_res = reduce_max(div_partial, red_blocks);
     last_div = dt2 * last_res;
     last_refinements = refine_iters;

@@ -18,18 +18,26 @@
 // to the ring slot that the current level no longer needs.  Each field is
 // thus read from HBM about once (plus the tile halo) instead of once per
 // stencil tap.
+#include <cstdlib>
+#include <string>
+
 #include "kernels.hpp"
 
 namespace elmm {
 
 namespace {
 
-constexpr int TX = 32, TY = 8, HALO = 3;
-constexpr int PW = TX + 2 * HALO, PH = TY + 2 * HALO, PS = PW * PH;
-constexpr int NTH = TX * TY;
-constexpr int PLOAD = (PS + NTH - 1) / NTH;
+constexpr int HALO = 3;
 // ring slots: U 0-3, V 4-7, W 8-10, viscosity 11-13
 constexpr int SU = 0, SV = 4, SW = 8, SN = 11, NSLOT = 14;
+
+// tile geometry of a TX x TY block
+template <int TX, int TY>
+struct Tile {
+  static constexpr int PW = TX + 2 * HALO, PH = TY + 2 * HALO, PS = PW * PH;
+  static constexpr int NTH = TX * TY;
+  static constexpr int PLOAD = (PS + NTH - 1) / NTH;
+};
 
 template <typename T>
 __device__ inline T m4(T a, T b, T c, T d) {
@@ -104,9 +112,11 @@ __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr
   U2[p] = u2;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(NTH)
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY)
     k_momentum_zm(Geo g, MomZArgs<T> a, StageConst sc, int kchunk) {
+  constexpr int PW = Tile<TX, TY>::PW, PS = Tile<TX, TY>::PS, NTH = Tile<TX, TY>::NTH;
+  constexpr int PLOAD = Tile<TX, TY>::PLOAD;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
   const int tid = threadIdx.x;
@@ -358,9 +368,34 @@ __global__ void __launch_bounds__(NTH)
 
 }  // namespace
 
-size_t momentum_zm_lds_bytes(int elem) { return (size_t)NSLOT * PS * elem; }
+size_t momentum_zm_lds_bytes(int elem) { return (size_t)NSLOT * Tile<32, 8>::PS * elem; }
 
 bool momentum_zm_supported(const Geo &g) { return !g.variable_z; }
+
+namespace {
+template <typename T, int TX, int TY>
+void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
+  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void *)k_momentum_zm<T, TX, TY>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
+  hipLaunchKernelGGL((k_momentum_zm<T, TX, TY>), gr, b, lds, st, g, a, sc, kchunk);
+}
+}  // namespace
+
+// Tile shape: 32x8 by default; ELMM_MOM_TILE=64x4, 32x4, 64x8 or 32x16 selects
+// another instantiation (the last two only for 4-byte fields: their FP64 LDS
+// rings exceed the 160 KiB of a workgroup).
+static int mom_tile_choice() {
+  static int choice = -1;
+  if (choice < 0) {
+    const char *e = std::getenv("ELMM_MOM_TILE");
+    const std::string v = e ? e : "";
+    choice = v == "64x4" ? 1 : v == "32x4" ? 2 : v == "64x8" ? 3 : v == "32x16" ? 4 : 0;
+  }
+  return choice;
+}
 
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *const Us[3],
@@ -377,8 +412,25 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
     a.code[c] = code[c];
   }
   if (kchunk <= 0) kchunk = g.nz;
-  dim3 b(NTH), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL(k_momentum_zm<T>, gr, b, momentum_zm_lds_bytes(sizeof(T)), st, g, a, sc, kchunk);
+  switch (mom_tile_choice()) {
+    case 1: launch_tile<T, 64, 4>(g, a, sc, kchunk, st); break;
+    case 2: launch_tile<T, 32, 4>(g, a, sc, kchunk, st); break;
+    case 3:
+      if (sizeof(T) == 4) {
+        launch_tile<T, 64, 8>(g, a, sc, kchunk, st);
+        break;
+      }
+      launch_tile<T, 32, 8>(g, a, sc, kchunk, st);
+      break;
+    case 4:
+      if (sizeof(T) == 4) {
+        launch_tile<T, 32, 16>(g, a, sc, kchunk, st);
+        break;
+      }
+      launch_tile<T, 32, 8>(g, a, sc, kchunk, st);
+      break;
+    default: launch_tile<T, 32, 8>(g, a, sc, kchunk, st);
+  }
 }
 
 template void launch_momentum_zm<double>(const Geo &, const double *, const double *, const double *, const double *,

@@ -100,11 +100,32 @@ __device__ inline void load_twiddles(V *dst, const V *__restrict__ src, int n) {
 // Pass 1: RHS = ((dU/dx + dV/dy + dW/dz) - Q) / coef for the rows of this
 // block, then forward FFT in x.  One block handles ROWS rows.
 // The divergence is evaluated in FP64 whatever the field type T.
+// Residual of the solved FD2 operator at one interior cell, in FP64:
+// r = RHS - A Phi with A the operator the spectral solve inverts (periodic x
+// and y by index wrap; z rows a_k (Phi_{k-1} - Phi_k) + c_k (Phi_{k+1} -
+// Phi_k), a_1 = c_nz = 0).
+__device__ inline double fd2_residual(const Geo &g, const double *__restrict__ Phi, const double *__restrict__ RHS,
+                                      const double *__restrict__ az, const double *__restrict__ cz, int i, int j,
+                                      int k, long p) {
+  const double rx = 1 / (g.dxmin * g.dxmin), ry = 1 / (g.dymin * g.dymin);
+  const double ph = Phi[p];
+  const double pw = Phi[i > 1 ? p - 1 : p - 1 + g.nx], pe = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
+  const double ps = Phi[j > 1 ? p - g.sy : p - g.sy + (long)g.ny * g.sy];
+  const double pn = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
+  double a = ((pw - ph) + (pe - ph)) * rx + ((ps - ph) + (pn - ph)) * ry;
+  if (k > 1) a = a + az[k - 1] * (Phi[p - g.sz] - ph);
+  if (k < g.nz) a = a + cz[k - 1] * (Phi[p + g.sz] - ph);
+  return RHS[p] - a;
+}
+
+// Right-hand side of the row: (a) the divergence of (U, V, W) minus Q over
+// coef (first solve); (b) rhs_in (a stored right-hand side); (c) with phi,
+// the FP64 residual rhs_in - A phi (refinement sweeps).
 template <typename T, typename R>
 __global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, const T *__restrict__ V,
                             const T *__restrict__ W, const double *__restrict__ Q,
-                            const double *__restrict__ rhs_in, double dt2_rec, double *__restrict__ rhs_out,
-                            int rows_per_block, int logn) {
+                            const double *__restrict__ rhs_in, const double *__restrict__ phi, double dt2_rec,
+                            double *__restrict__ rhs_out, int rows_per_block, int logn) {
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
   const int nx = pp.nx, ls = fpad(nx);
@@ -119,7 +140,9 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, c
     if (row < nrows) {
       const int j = row % pp.ny + 1, k = row / pp.ny + 1, i = m + 1;
       const long p = gix(g, i, j, k);
-      if (rhs_in) {
+      if (phi) {
+        r = fd2_residual(g, phi, rhs_in, pp.az, pp.cz, i, j, k, p);
+      } else if (rhs_in) {
         r = rhs_in[p];
       } else {
         const double dz = g.variable_z ? g.dzPr[k] : g.dzmin;
@@ -351,18 +374,19 @@ static void allow_lds(const void *fn, size_t bytes) {
 
 template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
-                      const double *rhs, double dt2_rec, double *rhs_out, hipStream_t st) {
+                      const double *rhs, const double *phi, double dt2_rec, double *rhs_out, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
   size_t sh = ((size_t)rows * padded(pp.nx) + pp.nx / 2) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_x_fwd<T, R>, sh);
-  hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, dt2_rec,
+  hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, phi, dt2_rec,
                      rhs_out, rows, ilog2(pp.nx));
 }
 
 template <typename R>
 void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
-  constexpr int TX = 8;
+  // 8 FP64 or 16 FP32 modes per block: 128-byte row segments
+  constexpr int TX = sizeof(R) == 4 ? 16 : 8;
   const size_t sh = ((size_t)TX * padded(pp.ny) + pp.ny / 2) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_y<TX, R>, sh);
   hipLaunchKernelGGL((k_fft_y<TX, R>), dim3(nblk(pp.M, TX), pp.nz), dim3(256), sh, st, pp, ilog2(pp.ny), inverse);
@@ -429,16 +453,19 @@ __global__ void k_top_sum(Geo g, const T *__restrict__ Pr, const double *__restr
 
 // Block sums combined in block order by one thread: partial[nblocks] = mean.
 __global__ void k_top_final(Geo g, double *__restrict__ partial, int nblocks) {
+  extern __shared__ double buf[];
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) buf[b] = partial[b];
+  __syncthreads();
   if (threadIdx.x != 0) return;
   double s = 0;
-  for (int b = 0; b < nblocks; b++) s += partial[b];
+  for (int b = 0; b < nblocks; b++) s += buf[b];
   partial[nblocks] = s / (double)(g.nx * g.ny);
 }
 
 template <typename T>
 void launch_top_sum(const Geo &g, const T *Pr, const double *Phi, double *partial, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL(k_top_sum<T>, dim3(nblocks), dim3(256), 0, st, g, Pr, Phi, partial);
-  hipLaunchKernelGGL(k_top_final, dim3(1), dim3(64), 0, st, g, partial, nblocks);
+  hipLaunchKernelGGL(k_top_final, dim3(1), dim3(256), (size_t)nblocks * sizeof(double), st, g, partial, nblocks);
 }
 
 // PostPoisson (2nd order) + Pr = Pr + Phi - ref:
@@ -541,10 +568,8 @@ void launch_residual(const Geo &g, const double *Phi, const double *RHS, double 
   hipLaunchKernelGGL(k_residual, dim3(nblocks), dim3(256), 0, st, g, Phi, RHS, partial);
 }
 
-// Residual of the solved FD2 operator in FP64 for iterative refinement:
-// r = RHS - A Phi with A the operator the spectral solve inverts (periodic x
-// and y by index wrap; z rows a_k (Phi_{k-1} - Phi_k) + c_k (Phi_{k+1} -
-// Phi_k), a_1 = c_nz = 0).  Block maxima of |r| go to partial.
+// Maximum FP64 residual of the solved operator (diagnostics of the
+// precision form): block maxima of |r| go to partial, r to r_out if given.
 __global__ void k_refine_residual(Geo g, const double *__restrict__ Phi, const double *__restrict__ RHS,
                                   const double *__restrict__ az, const double *__restrict__ cz,
                                   double *__restrict__ r_out, double *__restrict__ partial) {
@@ -553,16 +578,8 @@ __global__ void k_refine_residual(Geo g, const double *__restrict__ Phi, const d
   if (t < (long)g.nx * g.ny * g.nz) {
     const int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
     const long p = gix(g, i, j, k);
-    const double rx = 1 / (g.dxmin * g.dxmin), ry = 1 / (g.dymin * g.dymin);
-    const double ph = Phi[p];
-    const double pw = Phi[i > 1 ? p - 1 : p - 1 + g.nx], pe = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
-    const double ps = Phi[j > 1 ? p - g.sy : p - g.sy + (long)g.ny * g.sy];
-    const double pn = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
-    double a = ((pw - ph) + (pe - ph)) * rx + ((ps - ph) + (pn - ph)) * ry;
-    if (k > 1) a = a + az[k - 1] * (Phi[p - g.sz] - ph);
-    if (k < g.nz) a = a + cz[k - 1] * (Phi[p + g.sz] - ph);
-    const double r = RHS[p] - a;
-    r_out[p] = r;
+    const double r = fd2_residual(g, Phi, RHS, az, cz, i, j, k, p);
+    if (r_out) r_out[p] = r;
     v = fabs(r);
   }
   const double m = block_max(v);
@@ -590,9 +607,9 @@ ELMM_INST_PRESSURE_T(double)
 ELMM_INST_PRESSURE_T(float)
 template void launch_fft_x_fwd<double, double>(const Geo &, const PoissonPlan<double> &, const double *,
                                                const double *, const double *, const double *, const double *,
-                                               double, double *, hipStream_t);
+                                               const double *, double, double *, hipStream_t);
 template void launch_fft_x_fwd<float, float>(const Geo &, const PoissonPlan<float> &, const float *, const float *,
-                                             const float *, const double *, const double *, double, double *,
-                                             hipStream_t);
+                                             const float *, const double *, const double *, const double *, double,
+                                             double *, hipStream_t);
 
 }  // namespace elmm
