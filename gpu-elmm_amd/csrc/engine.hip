@@ -1097,6 +1097,20 @@ extern "C" int elmm_wait(elmm_handle *h, elmm_diag *diag) {
   return 0;
 }
 
+extern "C" int elmm_query(elmm_handle *h, int32_t *idle) {
+  if (!h || !idle) return elmm::fail(ELMM_E_ARG, "null argument to elmm_query");
+  const hipError_t e = hipStreamQuery(h->e->stream);
+  if (e == hipSuccess) {
+    *idle = 1;
+    return 0;
+  }
+  if (e == hipErrorNotReady) {
+    *idle = 0;
+    return 0;
+  }
+  return elmm::fail(ELMM_E_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
 extern "C" int elmm_divergence(elmm_handle *h, double *max_div) {
   if (!h || !max_div) return elmm::fail(ELMM_E_ARG, "null argument to elmm_divergence");
   return guarded([&] { *max_div = h->e->divergence(); });

@@ -1,0 +1,276 @@
+// server.cpp - continuous-batching request loop over device-resident cases.
+//
+// Requests (a case, its initial fields and a run length) are queued and
+// admitted in arrival order while their HBM footprint fits the budget and a
+// slot is free.  Every admitted case keeps its fields resident on its own
+// stream; progress() enqueues the next chunk of steps of every idle case, so
+// the work of all active cases overlaps on the device, retires finished cases
+// (results copied to the caller's buffers, memory returned to the budget) and
+// admits queued requests as memory frees up.  Latency = completion - submit.
+//
+// No counterpart in the reference, which runs one case per process
+// (src/main.f90); this is the serving form of the step.
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "case_setup.hpp"
+#include "elmm_gpu.h"
+#include "errors.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id = 0;
+  elmm_config cfg{};
+  std::vector<double> zw;
+  const elmm_case *c = nullptr;
+  int32_t nsteps = 0;
+  // initial state (copied at submit), ghosted with lower bound -2
+  std::vector<double> U, V, W, Pr;
+  std::vector<std::vector<double>> S;
+  // output buffers owned by the caller
+  double *oU = nullptr, *oV = nullptr, *oW = nullptr, *oPr = nullptr;
+  std::vector<double *> oS;
+  size_t footprint = 0;
+  Clock::time_point submitted;
+  // run state
+  elmm_handle *h = nullptr;
+  int32_t done = 0;
+  bool finished = false;
+  int status = 0;
+  double latency = 0;
+  double max_div = 0;
+};
+
+}  // namespace
+
+struct elmm_server {
+  size_t budget = 0, used = 0;
+  int max_active = 0, chunk = 0;
+  int64_t next_id = 1;
+  std::deque<std::unique_ptr<Request>> queue;
+  std::vector<std::unique_ptr<Request>> active;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> done;
+  int64_t cell_steps = 0;
+};
+
+namespace {
+
+int fail_req(Request &r, int rc) {
+  r.finished = true;
+  r.status = rc;
+  if (r.h) {
+    elmm_destroy(r.h);
+    r.h = nullptr;
+  }
+  return rc;
+}
+
+// create + upload + init, then the first chunk
+int admit(elmm_server *s, Request &r) {
+  int rc = elmm_create_case(&r.cfg, r.c, &r.h);
+  if (rc) return fail_req(r, rc);
+  const int32_t e[3] = {r.cfg.nx + 6, r.cfg.ny + 6, r.cfg.nz + 6};
+  if ((rc = elmm_upload(r.h, ELMM_F_U, -2, e, r.U.data())) || (rc = elmm_upload(r.h, ELMM_F_V, -2, e, r.V.data())) ||
+      (rc = elmm_upload(r.h, ELMM_F_W, -2, e, r.W.data())) || (rc = elmm_upload(r.h, ELMM_F_PR, -2, e, r.Pr.data())))
+    return fail_req(r, rc);
+  for (size_t q = 0; q < r.S.size(); q++)
+    if ((rc = elmm_upload(r.h, ELMM_F_SCALAR + (int)q, -2, e, r.S[q].data()))) return fail_req(r, rc);
+  // the host copies are no longer needed
+  r.U = r.V = r.W = r.Pr = std::vector<double>();
+  r.S.clear();
+  if ((rc = elmm_init(r.h))) return fail_req(r, rc);
+  const int32_t n = std::min(s->chunk, r.nsteps);
+  if (n > 0 && (rc = elmm_step_async(r.h, n))) return fail_req(r, rc);
+  r.done = n;
+  return 0;
+}
+
+int retire(elmm_server *s, Request &r) {
+  const int32_t e[3] = {r.cfg.nx + 6, r.cfg.ny + 6, r.cfg.nz + 6};
+  int rc = elmm_wait(r.h, nullptr);
+  if (!rc && r.oU) rc = elmm_download(r.h, ELMM_F_U, -2, e, r.oU);
+  if (!rc && r.oV) rc = elmm_download(r.h, ELMM_F_V, -2, e, r.oV);
+  if (!rc && r.oW) rc = elmm_download(r.h, ELMM_F_W, -2, e, r.oW);
+  if (!rc && r.oPr) rc = elmm_download(r.h, ELMM_F_PR, -2, e, r.oPr);
+  for (size_t q = 0; !rc && q < r.oS.size(); q++)
+    if (r.oS[q]) rc = elmm_download(r.h, ELMM_F_SCALAR + (int)q, -2, e, r.oS[q]);
+  if (!rc) rc = elmm_divergence(r.h, &r.max_div);
+  r.latency = std::chrono::duration<double>(Clock::now() - r.submitted).count();
+  s->cell_steps += (int64_t)r.cfg.nx * r.cfg.ny * r.cfg.nz * r.nsteps;
+  elmm_destroy(r.h);
+  r.h = nullptr;
+  r.finished = true;
+  r.status = rc;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int elmm_footprint(const elmm_config *cfg, int64_t *bytes) {
+  if (!cfg || !bytes) return elmm::fail(ELMM_E_ARG, "null argument to elmm_footprint");
+  // device layout: rows padded to a multiple of 16 after 13 front elements
+  const long row = 13 + cfg->nx + 6;
+  const long sy = (row + 15) / 16 * 16;
+  const long ntot = sy * (cfg->ny + 6) * (cfg->nz + 6) + 16;
+  const int e = cfg->precision == ELMM_FP32_REFINED ? 4 : 8;
+  // 12 velocity/pressure/viscosity fields of the field type, 3 per scalar,
+  // Q, Phi and RHS in FP64, the spectral array, masks and codes
+  const long M = cfg->nx / 2 + 1;
+  int64_t b = (int64_t)ntot * (12 * e + 3 * e * cfg->nscalars + 3 * 8 + 8);
+  b += (int64_t)M * cfg->ny * cfg->nz * 2 * e;
+  *bytes = b;
+  return 0;
+}
+
+
+
+int elmm_server_create(int64_t hbm_budget, int32_t max_active, int32_t chunk_steps, elmm_server **out) {
+  if (!out || hbm_budget <= 0 || max_active <= 0 || chunk_steps <= 0)
+    return elmm::fail(ELMM_E_ARG, "elmm_server_create: budget, slots and chunk must be positive");
+  auto s = std::make_unique<elmm_server>();
+  s->budget = (size_t)hbm_budget;
+  s->max_active = max_active;
+  s->chunk = chunk_steps;
+  *out = s.release();
+  return 0;
+}
+
+int elmm_server_submit(elmm_server *s, const elmm_request *q, int64_t *id) {
+  if (!s || !q || !q->cfg || !q->c || !q->U || !q->V || !q->W || !q->Pr || q->nsteps < 0)
+    return elmm::fail(ELMM_E_ARG, "elmm_server_submit: incomplete request");
+  auto r = std::make_unique<Request>();
+  r->cfg = *q->cfg;
+  if (r->cfg.stretched_z && r->cfg.zw) {
+    r->zw.assign(r->cfg.zw, r->cfg.zw + r->cfg.nz + 1);
+    r->cfg.zw = r->zw.data();
+  }
+  std::string err = elmm::validate_config(r->cfg);
+  if (!err.empty()) return elmm::fail(ELMM_E_ARG, err);
+  r->c = q->c;
+  r->nsteps = q->nsteps;
+  const size_t n = (size_t)(r->cfg.nx + 6) * (r->cfg.ny + 6) * (r->cfg.nz + 6);
+  r->U.assign(q->U, q->U + n);
+  r->V.assign(q->V, q->V + n);
+  r->W.assign(q->W, q->W + n);
+  r->Pr.assign(q->Pr, q->Pr + n);
+  for (int k = 0; k < r->cfg.nscalars; k++) {
+    if (q->scalars && q->scalars[k])
+      r->S.emplace_back(q->scalars[k], q->scalars[k] + n);
+    else
+      r->S.emplace_back(n, 0.0);
+  }
+  r->oU = q->out_U;
+  r->oV = q->out_V;
+  r->oW = q->out_W;
+  r->oPr = q->out_Pr;
+  for (int k = 0; k < r->cfg.nscalars; k++) r->oS.push_back(q->out_scalars ? q->out_scalars[k] : nullptr);
+  int64_t fb = 0;
+  elmm_footprint(&r->cfg, &fb);
+  r->footprint = (size_t)fb;
+  if (r->footprint > s->budget) return elmm::fail(ELMM_E_NOMEM, "request larger than the server's HBM budget");
+  r->submitted = Clock::now();
+  r->id = s->next_id++;
+  if (id) *id = r->id;
+  s->queue.push_back(std::move(r));
+  return 0;
+}
+
+int elmm_server_progress(elmm_server *s, int32_t *n_active, int32_t *n_queued) {
+  if (!s) return elmm::fail(ELMM_E_ARG, "null server");
+  // advance or retire the active cases
+  for (size_t a = 0; a < s->active.size();) {
+    Request &r = *s->active[a];
+    int32_t idle = 0;
+    int rc = elmm_query(r.h, &idle);
+    if (rc) {
+      fail_req(r, rc);
+    } else if (idle) {
+      if (r.done >= r.nsteps) {
+        retire(s, r);
+      } else {
+        const int32_t k = std::min(s->chunk, r.nsteps - r.done);
+        rc = elmm_step_async(r.h, k);
+        if (rc)
+          fail_req(r, rc);
+        else
+          r.done += k;
+      }
+    }
+    if (r.finished) {
+      s->used -= r.footprint;
+      s->done.emplace(r.id, std::move(s->active[a]));
+      s->active.erase(s->active.begin() + a);
+    } else {
+      a++;
+    }
+  }
+  // admit in arrival order while memory and slots allow
+  while (!s->queue.empty() && (int)s->active.size() < s->max_active &&
+         s->used + s->queue.front()->footprint <= s->budget) {
+    std::unique_ptr<Request> r = std::move(s->queue.front());
+    s->queue.pop_front();
+    s->used += r->footprint;
+    admit(s, *r);
+    if (r->finished) {
+      s->used -= r->footprint;
+      s->done.emplace(r->id, std::move(r));
+    } else {
+      s->active.push_back(std::move(r));
+    }
+  }
+  if (n_active) *n_active = (int32_t)s->active.size();
+  if (n_queued) *n_queued = (int32_t)s->queue.size();
+  return 0;
+}
+
+int elmm_server_status(elmm_server *s, int64_t id, elmm_request_status *st) {
+  if (!s || !st) return elmm::fail(ELMM_E_ARG, "null argument to elmm_server_status");
+  std::memset(st, 0, sizeof *st);
+  auto it = s->done.find(id);
+  if (it == s->done.end()) {
+    st->state = 0;
+    for (auto &r : s->active)
+      if (r->id == id) {
+        st->state = 1;
+        st->steps_done = r->done;
+      }
+    return 0;
+  }
+  const Request &r = *it->second;
+  st->state = r.status ? -1 : 2;
+  st->status = r.status;
+  st->steps_done = r.nsteps;
+  st->latency_s = r.latency;
+  st->max_div = r.max_div;
+  return 0;
+}
+
+int elmm_server_stats(elmm_server *s, int64_t *cell_steps, int64_t *hbm_in_use) {
+  if (!s) return elmm::fail(ELMM_E_ARG, "null server");
+  if (cell_steps) *cell_steps = s->cell_steps;
+  if (hbm_in_use) *hbm_in_use = (int64_t)s->used;
+  return 0;
+}
+
+int elmm_server_destroy(elmm_server *s) {
+  if (!s) return 0;
+  for (auto &r : s->active)
+    if (r->h) {
+      elmm_wait(r->h, nullptr);
+      elmm_destroy(r->h);
+    }
+  delete s;
+  return 0;
+}
+
+}  // extern "C"

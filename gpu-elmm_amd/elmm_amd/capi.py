@@ -75,6 +75,19 @@ class Diag(C.Structure):
 T_NGROUPS = 16
 
 
+class Request(C.Structure):
+    _fields_ = [("cfg", C.c_void_p), ("c", C.c_void_p), ("nsteps", C.c_int32), ("pad", C.c_int32),
+                ("U", C.c_void_p), ("V", C.c_void_p), ("W", C.c_void_p), ("Pr", C.c_void_p),
+                ("scalars", C.c_void_p),
+                ("out_U", C.c_void_p), ("out_V", C.c_void_p), ("out_W", C.c_void_p), ("out_Pr", C.c_void_p),
+                ("out_scalars", C.c_void_p)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int32), ("pad", C.c_int32),
+                ("latency_s", C.c_double), ("max_div", C.c_double)]
+
+
 class Timers(C.Structure):
     _fields_ = [("ms", C.c_double * T_NGROUPS), ("launches", C.c_int64 * T_NGROUPS)]
 
@@ -97,6 +110,14 @@ EXPORTS = {
     "elmm_case_free": (None, [C.c_void_p]),
     "elmm_create": (C.c_int, [C.POINTER(Config), C.POINTER(Lists), C.POINTER(C.c_void_p)]),
     "elmm_create_case": (C.c_int, [C.POINTER(Config), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "elmm_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "elmm_footprint": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_int64)]),
+    "elmm_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "elmm_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "elmm_server_progress": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "elmm_server_status": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestStatus)]),
+    "elmm_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "elmm_server_destroy": (C.c_int, [C.c_void_p]),
     "elmm_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _pd]),
     "elmm_download": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _pd]),
     "elmm_init": (C.c_int, [C.c_void_p]),
@@ -312,3 +333,72 @@ class Solver:
         x = C.c_double()
         _check(_lib.elmm_divergence(self._h, C.byref(x)))
         return x.value
+
+
+def footprint(cfg):
+    """Device bytes of one case (elmm_footprint)."""
+    b = C.c_int64()
+    _check(load().elmm_footprint(C.byref(cfg), C.byref(b)))
+    return b.value
+
+
+class Server:
+    """Continuous-batching request loop (elmm_server_*).  submit() copies the
+    initial fields; the Case and the output arrays are kept referenced here
+    until the request finishes."""
+
+    def __init__(self, hbm_budget, max_active=64, chunk_steps=10):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.elmm_server_create(int(hbm_budget), int(max_active), int(chunk_steps), C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, cfg, case, fields, nsteps):
+        U, V, W, Pr, scal = [np.asfortranarray(a, dtype=np.float64) for a in fields[:4]] + [fields[4]]
+        scal = [np.asfortranarray(a, dtype=np.float64) for a in scal]
+        shape = U.shape
+        outs = [np.zeros(shape, order="F") for _ in range(4)]
+        souts = [np.zeros(shape, order="F") for _ in range(cfg.nscalars)]
+        sin = (C.c_void_p * max(1, len(scal)))(*[a.ctypes.data for a in scal]) if scal else None
+        sout = (C.c_void_p * max(1, len(souts)))(*[a.ctypes.data for a in souts]) if souts else None
+        r = Request(C.cast(C.pointer(cfg), C.c_void_p), case._h, int(nsteps), 0,
+                    U.ctypes.data, V.ctypes.data, W.ctypes.data, Pr.ctypes.data,
+                    C.cast(sin, C.c_void_p) if sin is not None else None,
+                    outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data, outs[3].ctypes.data,
+                    C.cast(sout, C.c_void_p) if sout is not None else None)
+        rid = C.c_int64()
+        _check(_lib.elmm_server_submit(self._h, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = (cfg, case, outs, souts, sout)
+        return rid.value
+
+    def progress(self):
+        a, q = C.c_int32(), C.c_int32()
+        _check(_lib.elmm_server_progress(self._h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def status(self, rid):
+        st = RequestStatus()
+        _check(_lib.elmm_server_status(self._h, int(rid), C.byref(st)))
+        return st
+
+    def result(self, rid):
+        """(U, V, W, Pr, [scalars]) of a finished request (ghosted, lb -2)."""
+        _, _, outs, souts, _ = self._keep[rid]
+        return outs[0], outs[1], outs[2], outs[3], souts
+
+    def release(self, rid):
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        cs, used = C.c_int64(), C.c_int64()
+        _check(_lib.elmm_server_stats(self._h, C.byref(cs), C.byref(used)))
+        return cs.value, used.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.elmm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -232,6 +232,53 @@ int elmm_divergence(elmm_handle *h, double *max_div);
 
 int elmm_destroy(elmm_handle *h);
 
+/* 1 in *idle when every step enqueued on the handle has completed, else 0
+ * (non-blocking).  No counterpart in the reference (synchronous loop). */
+int elmm_query(elmm_handle *h, int32_t *idle);
+
+/* Device bytes elmm_create allocates for a configuration (fields, spectral
+ * work array; surface lists excluded).  Used to admit cases against an HBM
+ * budget. */
+int elmm_footprint(const elmm_config *cfg, int64_t *bytes);
+
+/* ---------------------------------------------------------------------------
+ * Serving form: continuous batching of independent cases on one GPU.  No
+ * counterpart in the reference (one case per process, src/main.f90).
+ * Requests are admitted in arrival order while their footprint fits the HBM
+ * budget and a slot is free; each active case keeps its fields resident on
+ * its own stream and advances chunk_steps per elmm_server_progress call when
+ * its previous chunk has finished, so the cases' kernels overlap on the
+ * device.  Finished cases copy their fields to the caller's output buffers
+ * (full ghosted arrays, lower bound -2, extents n+6) and free their memory.
+ * The case (c) and the output buffers must stay valid until the request has
+ * finished; the initial fields are copied at submit.
+ * ------------------------------------------------------------------------- */
+typedef struct elmm_server elmm_server;
+typedef struct elmm_request {
+  const elmm_config *cfg;
+  const elmm_case   *c;
+  int32_t nsteps;
+  int32_t pad;
+  const double *U, *V, *W, *Pr;        /* initial state, lb -2, extents n+6     */
+  const double *const *scalars;        /* nscalars arrays or NULL (zeros)       */
+  double *out_U, *out_V, *out_W, *out_Pr;
+  double *const *out_scalars;          /* NULL: not returned                    */
+} elmm_request;
+typedef struct elmm_request_status {
+  int32_t state;          /* 0 queued, 1 running, 2 finished, -1 failed       */
+  int32_t status;         /* ELMM_E_* of a failed request                      */
+  int32_t steps_done;
+  int32_t pad;
+  double  latency_s;      /* completion - submit, seconds                      */
+  double  max_div;        /* max |div U - Q| of the final state                */
+} elmm_request_status;
+int elmm_server_create(int64_t hbm_budget, int32_t max_active, int32_t chunk_steps, elmm_server **out);
+int elmm_server_submit(elmm_server *s, const elmm_request *r, int64_t *id);
+int elmm_server_progress(elmm_server *s, int32_t *n_active, int32_t *n_queued);
+int elmm_server_status(elmm_server *s, int64_t id, elmm_request_status *st);
+int elmm_server_stats(elmm_server *s, int64_t *cell_steps, int64_t *hbm_in_use);
+int elmm_server_destroy(elmm_server *s);
+
 /* ---------------------------------------------------------------------------
  * Kernel timers: HIP events around each kernel group on the handle's
  * stream, accumulated per group while enabled.
