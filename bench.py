@@ -17,7 +17,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-elmm_amd"))
 
-from elmm_amd import capi, cases, workloads  # noqa: E402
+from elmm_amd import capi, cases, serving, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -104,6 +104,11 @@ def main():
     ap.add_argument("--workload", default=workloads.DEFAULT)
     ap.add_argument("--precision", default="both", choices=["both", "fp32r", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=48, help="batched-ensemble cases (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--serve", type=int, default=8, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-steps-scale", type=float, default=0.1,
+                    help="run lengths U{200..2000} scaled by this factor to bound the run")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -156,6 +161,13 @@ def main():
                        "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in sorted(f["timers"].items())}}
         out["speedup_vs_fp64"] = round(r["value"] / f["value"], 3)
     out["kernel_ms"] = {k: [round(v[0], 3), v[1]] for k, v in sorted(r["timers"].items())}
+    pr = capi.FP32_REFINED if prim == "fp32r" else capi.FP64
+    if args.ensemble > 0:
+        out["ensemble"] = serving.run_ensemble(n_cases=args.ensemble, steps=args.ensemble_steps, precision=pr)
+    if args.serve > 0:
+        sv = serving.run_serving(n_requests=args.serve, steps_scale=args.serve_steps_scale, precision=pr)
+        sv["serial_fp64"] = serving.run_serial(n_requests=args.serve, steps_scale=args.serve_steps_scale)
+        out["serving"] = sv
     print(json.dumps(out))
 
 

@@ -1,0 +1,177 @@
+"""Dispersion scenarios and the drivers of the batched-ensemble and serving
+forms (both on top of the library's continuous-batching server,
+elmm_server_*).
+
+A scenario is a case of N cells (x/y rounded to powers of two for the
+spectral solver, z to a multiple of 8, Nx = Ny ~ 2 Nz), random cuboid
+obstacles up to a plan-area density, a wind of random direction and speed,
+and a point source at a free street-level cell; dt = 0.05 s.
+"""
+import math
+import queue
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import capi, cases
+
+
+def _rng(seed, salt):
+    return np.random.Generator(np.random.Philox(key=[int(seed), int(salt)]))
+
+
+def grid_for_cells(n):
+    """(nx, ny, nz) with nx = ny a power of two near (4n)^(1/3) and nz a
+    multiple of 8 in [16, 256] so that nx*ny*nz ~ n."""
+    nx = 2 ** int(round(math.log2((4 * n) ** (1.0 / 3.0))))
+    nx = max(32, min(1024, nx))
+    nz = int(round(n / (nx * nx) / 8.0)) * 8
+    return nx, nx, max(16, min(256, nz))
+
+
+def scenario(seed, ncells, precision=capi.FP32_REFINED, density=None, speed=None, theta=None):
+    """Config, cuboids, point sources and initial fields of one scenario."""
+    rng = _rng(seed, 17)
+    nx, ny, nz = grid_for_cells(ncells)
+    cfg = capi.with_precision(cases.uniform_config(nx, ny, nz, dx=1.0, dt=0.05, nscalars=1), precision)
+    density = rng.uniform(0.10, 0.30) if density is None else density
+    # one cuboid per 16x16 tile, in random tile order, until the plan-area
+    # density is reached; footprints 6-12 cells with a free margin
+    tiles = [(a, b) for a in range(nx // 16) for b in range(ny // 16)]
+    order = rng.permutation(len(tiles))
+    cub, area = [], 0.0
+    for t in order:
+        if area >= density * nx * ny:
+            break
+        a, b = tiles[t]
+        fx, fy = int(rng.integers(6, 13)), int(rng.integers(6, 13))
+        x0 = 16 * a + 2 + int(rng.integers(0, 13 - fx + 1))
+        y0 = 16 * b + 2 + int(rng.integers(0, 13 - fy + 1))
+        h = int(rng.integers(4, max(5, nz // 3)))
+        cub.append((float(x0), float(x0 + fx), float(y0), float(y0 + fy), 0.0, float(h)))
+        area += fx * fy
+    theta = rng.uniform(0, 2 * math.pi) if theta is None else theta
+    speed = rng.uniform(1.0, 5.0) if speed is None else speed
+    # source: a free cell at k = 1
+    for _ in range(1000):
+        sx, sy = rng.uniform(0, nx), rng.uniform(0, ny)
+        if not any(c[0] - 1 <= sx <= c[1] + 1 and c[2] - 1 <= sy <= c[3] + 1 for c in cub):
+            break
+    src = [(float(sx), float(sy), 0.5, 1.0, 0)]
+    U, V, W, Pr, S = cases.initial_fields(cfg, u0=speed * math.cos(theta), amp=0.05 * speed, seed=seed,
+                                          v0=speed * math.sin(theta))
+    return cfg, cub, src, (U, V, W, Pr, S)
+
+
+def _prepare(seed, ncells, precision):
+    cfg, cub, src, fields = scenario(seed, ncells, precision)
+    case = capi.Case(cfg, cuboids=cub, sources=src)
+    return cfg, case, fields
+
+
+def _pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(math.ceil(p / 100.0 * len(xs))) - 1))
+    return xs[k]
+
+
+def run_ensemble(n_cases=48, steps=20, seed=4, precision=capi.FP32_REFINED, budget=200e9, chunk=5, workers=8):
+    """Batched ensemble: n_cases scenarios with cell counts log-uniform in
+    [64^3, 192x192x96] advance together (all admitted at once, one stream
+    each); returns aggregate Mcell-steps/s over the device-resident run."""
+    rng = _rng(seed, 1)
+    sizes = np.exp(rng.uniform(math.log(64 ** 3), math.log(192 * 192 * 96), n_cases))
+    with ThreadPoolExecutor(workers) as ex:
+        prepared = list(ex.map(lambda a: _prepare(seed * 1000 + a[0], a[1], precision), enumerate(sizes)))
+    srv = capi.Server(budget, max_active=n_cases, chunk_steps=chunk)
+    t0 = time.perf_counter()
+    ids = [srv.submit(cfg, case, fields, steps) for cfg, case, fields in prepared]
+    while True:
+        a, q = srv.progress()
+        if a == 0 and q == 0:
+            break
+        time.sleep(0.0005)
+    wall = time.perf_counter() - t0
+    cell_steps, _ = srv.stats()
+    fails = [i for i in ids if srv.status(i).state != 2]
+    srv.close()
+    cells = [c.nx * c.ny * c.nz for c, _, _ in prepared]
+    return {"cases": n_cases, "steps": steps, "cells_min": int(min(cells)), "cells_max": int(max(cells)),
+            "cells_total": int(sum(cells)), "wall_s": round(wall, 3),
+            "value": round(cell_steps / wall / 1e6, 2), "unit": "Mcell-steps/s", "failed": len(fails),
+            "note": "wall time from the first submit (includes each case's upload and initial projection)"}
+
+
+def run_serving(n_requests=8, rate=2.0, seed=5, steps_scale=0.1, precision=capi.FP32_REFINED, budget=200e9,
+                chunk=10, workers=4):
+    """Serving loop: Poisson arrivals (rate per second), cell counts
+    lognormal (median 1.4 M, sigma 0.8) truncated to [0.25 M, 8 M], run
+    lengths U{200..2000} x steps_scale.  A request's latency runs from its
+    arrival (case set-up included) to the poll that sees it finished."""
+    rng = _rng(seed, 2)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    cells = np.clip(np.exp(rng.normal(math.log(1.4e6), 0.8, n_requests)), 0.25e6, 8e6)
+    lengths = np.maximum(1, np.round(rng.integers(200, 2001, n_requests) * steps_scale)).astype(int)
+    srv = capi.Server(budget, max_active=64, chunk_steps=chunk)
+    ready = queue.Queue()
+    t0 = time.perf_counter()
+
+    def build(k):
+        # set-up of request k starts at its arrival time
+        delay = arrivals[k] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ready.put((k, _prepare(seed * 1000 + k, cells[k], precision)))
+
+    ex = ThreadPoolExecutor(workers)
+    for k in range(n_requests):
+        ex.submit(build, k)
+    ids, done = {}, {}
+    while len(done) < n_requests:
+        while not ready.empty():
+            k, (cfg, case, fields) = ready.get()
+            ids[k] = (srv.submit(cfg, case, fields, int(lengths[k])), cfg)
+        srv.progress()
+        now = time.perf_counter() - t0
+        for k, (rid, cfg) in list(ids.items()):
+            if k not in done and srv.status(rid).state in (2, -1):
+                done[k] = now - arrivals[k]
+                srv.release(rid)
+        time.sleep(0.0005)
+    wall = time.perf_counter() - t0
+    ex.shutdown()
+    cell_steps, _ = srv.stats()
+    srv.close()
+    lat = list(done.values())
+    return {"requests": n_requests, "rate_per_s": rate, "steps_scale": steps_scale,
+            "cells_median": int(np.median(cells)), "steps_total": int(lengths.sum()),
+            "p50_latency_s": round(_pct(lat, 50), 3), "p99_latency_s": round(_pct(lat, 99), 3),
+            "wall_s": round(wall, 3), "value": round(cell_steps / wall / 1e6, 2), "unit": "Mcell-steps/s"}
+
+
+def run_serial(n_requests=8, seed=5, steps_scale=0.1, precision=capi.FP64):
+    """The same request stream solved one after another (no overlap), for
+    the comparison the serving form is measured against."""
+    rng = _rng(seed, 2)
+    gaps = rng.exponential(1.0 / 2.0, n_requests)
+    cells = np.clip(np.exp(rng.normal(math.log(1.4e6), 0.8, n_requests)), 0.25e6, 8e6)
+    lengths = np.maximum(1, np.round(rng.integers(200, 2001, n_requests) * steps_scale)).astype(int)
+    del gaps
+    total, busy = 0, 0.0
+    for k in range(n_requests):
+        cfg, case, fields = _prepare(seed * 1000 + k, cells[k], precision)
+        s = capi.Solver(cfg, case)
+        cases.upload_initial(s, cfg, fields)
+        s.init()
+        t = time.perf_counter()
+        s.step(int(lengths[k]), diag=False)
+        busy += time.perf_counter() - t
+        total += cfg.nx * cfg.ny * cfg.nz * int(lengths[k])
+        s.close()
+    return {"value": round(total / busy / 1e6, 2), "unit": "Mcell-steps/s", "precision": "fp64",
+            "note": "device time of the steps only, one request at a time"}
