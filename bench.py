@@ -110,7 +110,12 @@ def main():
     ap.add_argument("--serve-steps-scale", type=float, default=0.1,
                     help="run lengths U{200..2000} scaled by this factor to bound the run")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel on BYTES and print its bandwidth")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "GBs": round(capi.copy_probe(args.copy_probe, 5), 1)}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the step has no multi-GPU path in "
               "this build", file=sys.stderr)
@@ -153,6 +158,7 @@ def main():
         "stored_div": r["stored_div"],
         "poisson_res": r["poisson_res"],
         "case_build_s": round(case_s, 2),
+        "hbm_copy_GBs": round(capi.copy_probe(1 << 30, 5), 1),
     }
     if "fp64" in runs and prim != "fp64":
         f = runs["fp64"]

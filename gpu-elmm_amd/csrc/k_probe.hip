@@ -1,0 +1,52 @@
+// k_probe.hip - plain device copy of a known size: the reference point of
+// the HBM traffic counters (FETCH_SIZE/WRITE_SIZE per dispatch) and of the
+// achievable copy bandwidth reported beside the roofline.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "elmm_gpu.h"
+#include "errors.hpp"
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_copy_probe(const double4 *__restrict__ src, double4 *__restrict__ dst,
+                                                    long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
+}  // namespace
+
+extern "C" int elmm_copy_probe(int64_t nbytes, int32_t reps, double *gbps) {
+  if (nbytes < 32 || reps < 1 || !gbps) return elmm::fail(ELMM_E_ARG, "elmm_copy_probe: bad arguments");
+  const long n = (long)(nbytes / 32);
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&] {
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  hipError_t err;
+  if ((err = hipMalloc(&a, n * 32)) != hipSuccess || (err = hipMalloc(&b, n * 32)) != hipSuccess ||
+      (err = hipMemset(a, 0, n * 32)) != hipSuccess || (err = hipEventCreate(&e0)) != hipSuccess ||
+      (err = hipEventCreate(&e1)) != hipSuccess) {
+    cleanup();
+    return elmm::fail(ELMM_E_HIP, std::string("elmm_copy_probe: ") + hipGetErrorString(err));
+  }
+  const int blocks = 256 * 16;
+  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, (const double4 *)a, (double4 *)b, n);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, (const double4 *)a, (double4 *)b, n);
+  (void)hipEventRecord(e1, 0);
+  err = hipEventSynchronize(e1);
+  float ms = 0;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  cleanup();
+  if (err != hipSuccess) return elmm::fail(ELMM_E_HIP, std::string("elmm_copy_probe: ") + hipGetErrorString(err));
+  *gbps = 2.0 * n * 32 * reps / (ms * 1e-3) / 1e9;
+  return 0;
+}

@@ -110,6 +110,7 @@ EXPORTS = {
     "elmm_case_free": (None, [C.c_void_p]),
     "elmm_create": (C.c_int, [C.POINTER(Config), C.POINTER(Lists), C.POINTER(C.c_void_p)]),
     "elmm_create_case": (C.c_int, [C.POINTER(Config), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "elmm_copy_probe": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "elmm_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "elmm_footprint": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_int64)]),
     "elmm_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
@@ -402,3 +403,10 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """Achieved bandwidth (GB/s, read + write) of a plain device copy."""
+    g = C.c_double()
+    _check(load().elmm_copy_probe(int(nbytes), int(reps), C.byref(g)))
+    return g.value

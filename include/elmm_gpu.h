@@ -300,6 +300,12 @@ int elmm_timers_enable(elmm_handle *h, int32_t enable);
 int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset);
 const char *elmm_timer_name(int32_t group);
 
+/* Plain device copy of nbytes, reps times (plus one untimed warm-up), on the
+ * default stream; *gbps = bytes read + written per second.  The reference
+ * point of the HBM traffic counters and of the achievable copy bandwidth.
+ * No counterpart in the reference. */
+int elmm_copy_probe(int64_t nbytes, int32_t reps, double *gbps);
+
 /* Version and last error of the calling thread. */
 const char *elmm_version(void);
 const char *elmm_last_error(void);
