@@ -113,9 +113,16 @@ std::string validate_config(const elmm_config &c) {
     if (c.nscalars > 0 && b != ELMM_BC_PERIODIC && b != ELMM_BC_NEUMANN && b != ELMM_BC_DIRICHLET)
       return "scalar boundary type must be periodic, Neumann or Dirichlet";
   }
-  if (c.stretched_z)
-    return "stretched z grids (CDU/V/W_variable_z, variable-z diffusion) are not built in this version";
-  if (c.advection_method != 4) return "advection_method must be 4 (CD4div) on uniform grids";
+  if (c.stretched_z) {
+    for (int k = 0; k < c.nz; k++)
+      if (!(c.zw[k + 1] > c.zw[k])) return "zw must increase strictly";
+    // CD4div uses the uniform dzmin (src/advection.f90:473); the stretched
+    // grid has the variable-z CD2 of src/advection_variable_z.f90 only
+    if (c.advection_method != 2) return "stretched z grids need advection_method 2 (variable-z CD2)";
+    if (c.nscalars > 0) return "passive scalars on stretched z grids are not built in this version";
+  } else if (c.advection_method != 4) {
+    return "advection_method must be 4 (CD4div) on uniform grids";
+  }
   if (c.sgs_model != 0 && c.sgs_model != 1) return "sgs_model must be 0 or 1";
   if (c.nscalars < 0 || c.nscalars > 8) return "nscalars must be in 0..8";
   if (c.precision != ELMM_FP64 && c.precision != ELMM_FP32_REFINED) return "unknown precision";
@@ -322,6 +329,7 @@ int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc,
                CaseData &cd) {
   std::string err = validate_config(c);
   if (!err.empty()) return fail(ELMM_E_ARG, err);
+  if (c.stretched_z && ncub > 0) return fail(ELMM_E_ARG, "solid bodies on stretched z grids are not built in this version");
   cd.cfg = c;
   cd.nx = c.nx;
   cd.ny = c.ny;

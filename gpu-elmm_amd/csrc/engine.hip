@@ -188,7 +188,7 @@ struct Engine : EngineIface {
   // coordinates
   DevBuf coords;
   // wall model
-  DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_logdz0;
+  DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
   DevBuf width_k;
   bool have_width_k = false;
   int wm_n = 0;
@@ -231,6 +231,7 @@ struct Engine : EngineIface {
                     wm_z0.as<double>(),
                     wm_ustar.as<double>(),
                     wm_fluxp.as<double>(),
+                    wm_fluxm.as<double>(),
                     wm_logdz0.as<double>()};
   }
   bool has_ib() const { return ib_n[0] + ib_n[1] + ib_n[2] > 0; }
@@ -374,6 +375,13 @@ void Engine<T>::setup_poisson() {
 template <typename T>
 void Engine<T>::setup_lists(const elmm_lists &L) {
   const long ntot = g.ntot;
+  if (g.variable_z) {
+    // the stretched-z path carries the z-wall lists only (periodic x and y,
+    // no solid bodies)
+    bool extra = L.ib_n[0] || L.ib_n[1] || L.ib_n[2];
+    for (int l = 0; l < 18; l++) extra = extra || (l % 6 < 4 && L.wm_n[l]);
+    if (extra) throw ArgError("solid bodies on stretched z grids are not built in this version");
+  }
   auto lin = [&](int i, int j, int k) { return gix(g, i, j, k); };
   // host copies of masks in the device layout
   auto remap_i32 = [&](const int32_t *src) {
@@ -437,6 +445,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   upload_vec(wm_z0, wz0);
   upload_vec(wm_ustar, wustar);
   wm_fluxp.alloc(std::max(wm_n, 1) * sizeof(double));
+  wm_fluxm.alloc(std::max(wm_n, 1) * sizeof(double));
 
   // ordered momentum corrections (wmfluxes-nobranch-inc.f90:49-208)
   if (cfg.wall_model) {
@@ -458,8 +467,9 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
             const int neg = pass == 1;
             if (t[0] < 1 || t[0] > n[0] || t[1] < 1 || t[1] > n[1] || t[2] < 1 || t[2] > n[2]) continue;
             long key = lin(t[0], t[1], t[2]);
+            // last slot: level of the face's lower cell (stretched-z terms)
             tb.add(key, t[0], t[1], t[2],
-                   {ax | (neg << 2), list_off[l] + p, (int32_t)lin(f[0], f[1], f[2]), 0});
+                   {ax | (neg << 2), list_off[l] + p, (int32_t)lin(f[0], f[1], f[2]), f[2]});
             codes[c][key] |= CODE_CORRECTED;
           }
       }
@@ -857,7 +867,8 @@ void Engine<T>::stage(int st, bool last) {
     if (mom_terms[c].n)
       timed(ELMM_T_MOMENTUM_CORR, [&] {
         launch_momentum_corr(g, c, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar[c], U2p[c],
-                             code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(), stream);
+                             code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(),
+                             wm_fluxm.as<double>(), stream);
       });
   }
   if (nsc > 0) {

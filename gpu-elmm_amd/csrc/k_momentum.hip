@@ -188,22 +188,27 @@ __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict
   if (ustar < 0) ustar = 0;
   wm.ustar[p] = ustar;
   const double eps = 1.4901161193847656e-08;  // sqrt(epsilon(1._knd))
-  double fluxp = 0;
+  // fluxp: the face flux per volume of the cell beyond the face, fluxm: of
+  // the cell before it (src/wallmodels.f90:2285-2321); equal on uniform z
+  double fluxp = 0, fluxm = 0;
   if (vel > eps) {
     const double tc = comp == 0 ? t0 : comp == 1 ? t1 : t2;
-    if (variable_z && dir == 4) {
-      fluxp = (tc / vel) * (ustar * ustar);
-      fluxp = comp < 2 ? fluxp / g.dzPr[zk] : fluxp / g.dzW[zk];
-    } else if (variable_z && dir == 5) {
-      fluxp = (tc / vel) * (ustar * ustar);
-      fluxp = comp < 2 ? fluxp / g.dzPr[zk + 1] : fluxp / g.dzW[zk + 1];
+    if (variable_z && dir >= 4) {
+      const int kp = dir == 4 ? zk : zk + 1;
+      fluxp = fluxm = (tc / vel) * (ustar * ustar);
+      fluxp = comp < 2 ? fluxp / g.dzPr[kp] : fluxp / g.dzW[kp];
+      fluxm = comp < 2 ? fluxm / g.dzPr[kp - 1] : fluxm / g.dzW[kp - 1];
     } else {
       const double drec = dir < 2 ? 1 / g.dxmin : dir < 4 ? 1 / g.dymin : 1 / g.dzmin;
-      fluxp = (tc / vel) * (ustar * ustar) * drec;
+      fluxp = fluxm = (tc / vel) * (ustar * ustar) * drec;
     }
-    if ((dir & 1) == 0) fluxp = -fluxp;
+    if ((dir & 1) == 0) {
+      fluxp = -fluxp;
+      fluxm = -fluxm;
+    }
   }
   wm.fluxp[p] = fluxp;
+  wm.fluxm[p] = fluxm;
 }
 
 template <typename T>
@@ -314,6 +319,90 @@ __device__ inline T diffusion(const Geo &g, const T *__restrict__ nu, const T *_
   return a;
 }
 
+// ------------------------------------------- stretched z (CD2, 2nd order)
+// CDU/V/W_variable_z (src/advection_variable_z.f90:15-321): the divergence
+// form minus the advective form, times 1/2; z differences over dzPr/dzW and
+// zPr/zW, x and y over dxmin/dymin.
+template <typename T, int C>
+__device__ inline T cd2vz(const Geo &g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+                          long p, int k) {
+  const long sy = g.sy, sz = g.sz;
+  const T ax4 = (T)(0.25 / g.dxmin), ay4 = (T)(0.25 / g.dymin), four = 4;
+  T d;
+  if (C == 0) {
+    d = -(((ax4 * (U[p + 1] + U[p])) * (U[p + 1] + U[p]) - (ax4 * (U[p] + U[p - 1])) * (U[p] + U[p - 1])) +
+          ((ay4 * (U[p + sy] + U[p])) * (V[p + 1] + V[p]) - (ay4 * (U[p] + U[p - sy])) * (V[p + 1 - sy] + V[p - sy])) +
+          ((U[p + sz] + U[p]) * (W[p + 1] + W[p]) - (U[p] + U[p - sz]) * (W[p + 1 - sz] + W[p - sz])) /
+              (T)g.dzPr[k] / four);
+    const T ax = (T)(0.5 / g.dxmin), ay = (T)(0.125 / g.dymin);
+    const T va = ((V[p] + V[p + 1]) + V[p - sy]) + V[p + 1 - sy];
+    const T wa = (((W[p] + W[p + 1]) + W[p - sz]) + W[p + 1 - sz]) / four;
+    d = d - (((ax * (U[p + 1] - U[p - 1])) * U[p] + (ay * (U[p + sy] - U[p - sy])) * va) +
+             ((U[p + sz] - U[p - sz]) / (T)(g.zPr[k + 1] - g.zPr[k - 1])) * wa);
+  } else if (C == 1) {
+    d = -(((ay4 * (V[p + sy] + V[p])) * (V[p + sy] + V[p]) - (ay4 * (V[p] + V[p - sy])) * (V[p] + V[p - sy])) +
+          ((ax4 * (V[p + 1] + V[p])) * (U[p + sy] + U[p]) - (ax4 * (V[p] + V[p - 1])) * (U[p - 1 + sy] + U[p - 1])) +
+          ((V[p + sz] + V[p]) * (W[p + sy] + W[p]) - (V[p] + V[p - sz]) * (W[p + sy - sz] + W[p - sz])) /
+              (T)g.dzPr[k] / four);
+    const T ax = (T)(0.125 / g.dxmin), ay = (T)(0.5 / g.dymin);
+    const T ua = ((U[p] + U[p + sy]) + U[p - 1]) + U[p - 1 + sy];
+    const T wa = (((W[p] + W[p + sy]) + W[p - sz]) + W[p + sy - sz]) / four;
+    d = d - (((ax * (V[p + 1] - V[p - 1])) * ua + (ay * (V[p + sy] - V[p - sy])) * V[p]) +
+             ((V[p + sz] - V[p - sz]) / (T)(g.zPr[k + 1] - g.zPr[k - 1])) * wa);
+  } else {
+    d = -(((W[p + sz] + W[p]) * (W[p + sz] + W[p]) - (W[p] + W[p - sz]) * (W[p] + W[p - sz])) / (T)g.dzW[k] / four +
+          ((ay4 * (W[p + sy] + W[p])) * (V[p + sz] + V[p]) - (ay4 * (W[p] + W[p - sy])) * (V[p - sy + sz] + V[p - sy])) +
+          ((ax4 * (W[p + 1] + W[p])) * (U[p + sz] + U[p]) - (ax4 * (W[p] + W[p - 1])) * (U[p - 1 + sz] + U[p - 1])));
+    const T ax = (T)(0.125 / g.dxmin), ay = (T)(0.125 / g.dymin);
+    const T ua = ((U[p] + U[p + sz]) + U[p - 1]) + U[p - 1 + sz];
+    const T va = ((V[p] + V[p + sz]) + V[p - sy]) + V[p - sy + sz];
+    d = d - (((ax * (W[p + 1] - W[p - 1])) * ua + (ay * (W[p + sy] - W[p - sy])) * va) +
+             ((W[p + sz] - W[p - sz]) / (T)(g.zW[k + 1] - g.zW[k - 1])) * W[p]);
+  }
+  return d * (T)0.5;
+}
+
+// z flux on the upper face of cell p (level f) in
+// MomentumDiffusion_variable_z_nobranch_2ord (src/momentum_diffusion.f90:
+// 298-420): U, V: nu4/4 * dF / dzW(f); W: nu(f+1) * dW / dzPr(f+1).
+template <typename T, int C>
+__device__ inline T zflux_vz(const Geo &g, const T *__restrict__ nu, const T *__restrict__ F, long p, int f) {
+  const long sy = g.sy, sz = g.sz;
+  const T q4 = (T)0.25;
+  if (C == 0) return q4 * (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) / (T)g.dzW[f];
+  if (C == 1) return q4 * (nu[p + sy + sz] + nu[p + sy] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) / (T)g.dzW[f];
+  return nu[p + sz] * (F[p + sz] - F[p]) / (T)g.dzPr[f + 1];
+}
+
+template <typename T, int C>
+__device__ inline T diffusion_vz(const Geo &g, const T *__restrict__ nu, const T *__restrict__ F, long p, int k,
+                                 T a) {
+  a = a + dflux<T, C>(g, nu, F, p, 0);
+  a = a - dflux<T, C>(g, nu, F, p - 1, 0);
+  a = a + dflux<T, C>(g, nu, F, p, 1);
+  a = a - dflux<T, C>(g, nu, F, p - g.sy, 1);
+  T tmp = zflux_vz<T, C>(g, nu, F, p, k);
+  tmp = tmp - zflux_vz<T, C>(g, nu, F, p - g.sz, k - 1);
+  tmp = tmp / (T)(C < 2 ? g.dzPr[k] : g.dzW[k]);
+  return a + tmp;
+}
+
+// Wall-model replacement of a z face on a stretched grid
+// (wmfluxes-variable_z-nobranch-{U,V,W}-inc.f90): the face above level f
+// seen from the written cell at level t.
+template <typename T, int C>
+__device__ inline T zterm_vz(const Geo &g, const T *__restrict__ nu, const T *__restrict__ F, long p, int f, int t) {
+  const long sy = g.sy, sz = g.sz;
+  const T four = 4;
+  if (C == 0)
+    return (nu[p + 1 + sz] + nu[p + 1] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) / (T)g.dzW[f] / four /
+           (T)g.dzPr[t];
+  if (C == 1)
+    return (nu[p + sy + sz] + nu[p + sy] + nu[p + sz] + nu[p]) * (F[p + sz] - F[p]) / (T)g.dzW[f] / four /
+           (T)g.dzPr[t];
+  return nu[p + sz] * (F[p + sz] - F[p]) / (T)g.dzPr[f + 1] / (T)g.dzW[t];
+}
+
 // RK combine + PressureGrad + add(U2,U) for one cell
 template <typename T, int C>
 __device__ inline void finish(const Geo &g, const StageConst &sc, const T *__restrict__ F,
@@ -336,7 +425,21 @@ __device__ inline void finish(const Geo &g, const StageConst &sc, const T *__res
   U2[p] = u2;
 }
 
-template <typename T, int C>
+// bulk advection + diffusion of component C at cell p: CD4 on uniform grids
+// (VZ = 0), variable-z CD2 and diffusion on stretched z grids (VZ = 1)
+template <typename T, int C, int VZ>
+__device__ inline T momentum_rhs(const Geo &g, const T *__restrict__ U, const T *__restrict__ V,
+                                 const T *__restrict__ W, const T *__restrict__ Visc, long p, int k, uint8_t cd) {
+  const T *F = C == 0 ? U : C == 1 ? V : W;
+  if (VZ) {
+    T w = cd2vz<T, C>(g, U, V, W, p, k);
+    return diffusion_vz<T, C>(g, Visc, F, p, k, w);
+  }
+  T w = cd4<T, C>(g, U, V, W, p, cd & CODE_ADV_MASK);
+  return diffusion<T, C>(g, Visc, F, p, w);
+}
+
+template <typename T, int C, int VZ>
 __global__ void __launch_bounds__(256)
     k_momentum(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
                const T *__restrict__ Visc, const T *__restrict__ Pr, T *__restrict__ Ustar,
@@ -349,46 +452,57 @@ __global__ void __launch_bounds__(256)
   const uint8_t cd = code[p];
   if (cd & CODE_CORRECTED) return;
   const T *F = C == 0 ? U : C == 1 ? V : W;
-  T w = cd4<T, C>(g, U, V, W, p, cd & CODE_ADV_MASK);
-  w = diffusion<T, C>(g, Visc, F, p, w);
+  const T w = momentum_rhs<T, C, VZ>(g, U, V, W, Visc, p, k, cd);
   finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+}
+
+template <typename T, int VZ>
+void launch_momentum_vz(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
+                        T *Ustar, T *U2, const uint8_t *code, StageConst sc, hipStream_t st) {
+  const int *n = g.n[comp];
+  dim3 b(64, 4, 1), gr(nblk(n[0], 64), nblk(n[1], 4), n[2]);
+  if (comp == 0)
+    hipLaunchKernelGGL((k_momentum<T, 0, VZ>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+  else if (comp == 1)
+    hipLaunchKernelGGL((k_momentum<T, 1, VZ>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+  else
+    hipLaunchKernelGGL((k_momentum<T, 2, VZ>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
 }
 
 template <typename T>
 void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *Ustar,
                      T *U2, const uint8_t *code, StageConst sc, hipStream_t st) {
-  const int *n = g.n[comp];
-  dim3 b(64, 4, 1), gr(nblk(n[0], 64), nblk(n[1], 4), n[2]);
-  if (comp == 0)
-    hipLaunchKernelGGL((k_momentum<T, 0>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
-  else if (comp == 1)
-    hipLaunchKernelGGL((k_momentum<T, 1>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+  if (g.variable_z)
+    launch_momentum_vz<T, 1>(g, comp, U, V, W, Visc, Pr, Ustar, U2, code, sc, st);
   else
-    hipLaunchKernelGGL((k_momentum<T, 2>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc);
+    launch_momentum_vz<T, 0>(g, comp, U, V, W, Visc, Pr, Ustar, U2, code, sc, st);
 }
 
 // Cells with wall-model face-flux replacements (wmfluxes-nobranch-inc.f90:
-// 49-208): the bulk value, then the terms in the reference's scatter order.
-// term = {axis | neg<<2, WM point index, face-cell linear index, 0}
-template <typename T, int C>
+// 49-208; wmfluxes-variable_z-nobranch-{U,V,W}-inc.f90 on stretched z): the
+// bulk value, then the terms in the reference's scatter order.
+// term = {axis | neg<<2, WM point index, face-cell linear index, face level}
+// A negative term (the cell before the face) carries fluxm, a positive one
+// fluxp.
+template <typename T, int C, int VZ>
 __global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restrict__ V,
                                 const T *__restrict__ W, const T *__restrict__ Visc,
                                 const T *__restrict__ Pr, T *__restrict__ Ustar, T *__restrict__ U2,
                                 const uint8_t *__restrict__ code, StageConst sc, TermList tl,
-                                const double *__restrict__ fluxp) {
+                                const double *__restrict__ fluxp, const double *__restrict__ fluxm) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= tl.n) return;
   const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
   const long p = gix(g, i, j, k);
   const T *F = C == 0 ? U : C == 1 ? V : W;
-  T w = cd4<T, C>(g, U, V, W, p, code[p] & CODE_ADV_MASK);
-  w = diffusion<T, C>(g, Visc, F, p, w);
+  T w = momentum_rhs<T, C, VZ>(g, U, V, W, Visc, p, k, code[p]);
   for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
     const int32_t *tt = &tl.term[4 * t];
     const int ax = tt[0] & 3;
     const bool neg = (tt[0] >> 2) & 1;
-    const T fl = dflux<T, C>(g, Visc, F, (long)tt[2], ax);
-    const T fp = (T)fluxp[tt[1]];
+    const T fl = VZ && ax == 2 ? zterm_vz<T, C>(g, Visc, F, (long)tt[2], tt[3], k)
+                               : dflux<T, C>(g, Visc, F, (long)tt[2], ax);
+    const T fp = (T)(neg ? fluxm[tt[1]] : fluxp[tt[1]]);
     if (neg) {
       w = w - fl;
       w = w - fp;
@@ -400,18 +514,31 @@ __global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restr
   finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w);
 }
 
+template <typename T, int VZ>
+void launch_momentum_corr_vz(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
+                             T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
+                             const double *fluxm, hipStream_t st) {
+  dim3 b(128), gr(nblk(tl.n, 128));
+  if (comp == 0)
+    hipLaunchKernelGGL((k_momentum_corr<T, 0, VZ>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl,
+                       fluxp, fluxm);
+  else if (comp == 1)
+    hipLaunchKernelGGL((k_momentum_corr<T, 1, VZ>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl,
+                       fluxp, fluxm);
+  else
+    hipLaunchKernelGGL((k_momentum_corr<T, 2, VZ>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl,
+                       fluxp, fluxm);
+}
+
 template <typename T>
 void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
                           T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
-                          hipStream_t st) {
+                          const double *fluxm, hipStream_t st) {
   if (tl.n == 0) return;
-  dim3 b(128), gr(nblk(tl.n, 128));
-  if (comp == 0)
-    hipLaunchKernelGGL((k_momentum_corr<T, 0>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
-  else if (comp == 1)
-    hipLaunchKernelGGL((k_momentum_corr<T, 1>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+  if (g.variable_z)
+    launch_momentum_corr_vz<T, 1>(g, comp, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp, fluxm, st);
   else
-    hipLaunchKernelGGL((k_momentum_corr<T, 2>), gr, b, 0, st, g, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp);
+    launch_momentum_corr_vz<T, 0>(g, comp, U, V, W, Visc, Pr, Ustar, U2, code, sc, tl, fluxp, fluxm, st);
 }
 
 // ------------------------------------------------------------- IB gathers
@@ -539,7 +666,7 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
                                    T *, const uint8_t *, StageConst, hipStream_t);                                 \
   template void launch_momentum_corr<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *,   \
                                         T *, T *, const uint8_t *, StageConst, TermList, const double *,           \
-                                        hipStream_t);                                                              \
+                                        const double *, hipStream_t);                                              \
   template void launch_ib_interp<T>(const Geo &, IBDev, const T *, double *, hipStream_t);                         \
   template void launch_ib_scatter<T>(const Geo &, IBDev, const double *, T *, hipStream_t);                        \
   template void launch_ib_visc_serial<T>(const Geo &, IBDev, T *, const int32_t *, int, hipStream_t);              \

@@ -22,7 +22,8 @@ struct WMPoints {
   const double *dist;      // 3 per point
   const double *z0;
   double *ustar;           // in/out state
-  double *fluxp;           // out
+  double *fluxp;           // out: flux scaled to the cell beyond the face
+  double *fluxm;           // out: to the cell before it (= fluxp on uniform z)
   const double *logdz0;    // log(|dist|/z0) per point (static, from the host libm)
 };
 
@@ -62,7 +63,7 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
 template <typename T>
 void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
                           T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
-                          hipStream_t st);
+                          const double *fluxm, hipStream_t st);
 template <typename T>
 void launch_ib_interp(const Geo &g, IBDev ib, const T *F, double *tmp, hipStream_t st);
 template <typename T>

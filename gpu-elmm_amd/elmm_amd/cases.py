@@ -40,6 +40,54 @@ def config1(n=64, **kw):
     return uniform_config(n, n, n, **kw)
 
 
+def channel_zw(nz, lz=2.0, gamma=1.5):
+    """Wall-clustered tanh grid zW_k = h (1 - tanh(gamma (1 - 2k/nz)) / tanh(gamma)),
+    h = lz/2, k = 0..nz (walls at 0 and lz exactly)."""
+    k = np.arange(nz + 1, dtype=np.float64)
+    zw = (lz / 2) * (1.0 - np.tanh(gamma * (1.0 - 2.0 * k / nz)) / np.tanh(gamma))
+    zw[0], zw[-1] = 0.0, lz
+    return zw
+
+
+def channel_config(nx=256, ny=128, nz=128, lx=2 * np.pi, ly=np.pi, lz=2.0, gamma=1.5, nu=1e-3, dt=2e-4, **kw):
+    """Turbulent channel (Re_tau = u_tau h / nu = 1000 for u_tau = 1, h = 1):
+    periodic x/y, smooth-wall (z0 = 0) wall model at both z walls, tanh
+    stretched z, variable-z CD2 advection, driven by pr_gradient_x = -1, no
+    scalars."""
+    args = dict(
+        nx=nx, ny=ny, nz=nz, dx=lx / nx, dy=ly / ny, dz=lz / nz,
+        btype=[capi.PERIODIC, capi.PERIODIC, capi.PERIODIC, capi.PERIODIC, capi.NOSLIP, capi.NOSLIP],
+        z0_wall=[0] * 6,
+        scal_btype=[capi.PERIODIC, capi.PERIODIC, capi.PERIODIC, capi.PERIODIC, capi.NEUMANN, capi.NEUMANN],
+        side_scal=[0] * 6, nu=nu, nu_scalar=nu, dt=dt, nscalars=0, cfl_check=1,
+        advection_method=2, pr_gradient_x=-1.0, zw=channel_zw(nz, lz, gamma))
+    args.update(kw)
+    return capi.make_config(**args)
+
+
+def channel_fields(cfg, seed=2, amp=0.05, utau=1.0):
+    """Law-of-the-wall start: U(d) = u_tau d+ for d+ < 11, else
+    u_tau (ln(d+)/0.41 + 5.5), d the distance to the nearer wall; U, V and W
+    carry uniform noise of amp times the local profile value."""
+    zw = np.asarray(cfg._zw_keep)
+    lz = zw[-1]
+
+    def law(d):
+        dp = d * utau / cfg.nu
+        return utau * np.where(dp < 11.0, dp, np.log(np.maximum(dp, 1e-300)) / 0.41 + 5.5)
+
+    zpr = (zw[:-1] + zw[1:]) / 2
+    prof = law(np.minimum(zpr, lz - zpr))
+    profw = law(np.minimum(zw[1:-1], lz - zw[1:-1]))
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    shape = (nx + 6, ny + 6, nz + 6)
+    U, V, W = (np.zeros(shape, order="F") for _ in range(3))
+    U[3:3 + nx, 3:3 + ny, 3:3 + nz] = prof * (1.0 + perturbation((nx, ny, nz), amp, seed, 0))
+    V[3:3 + nx, 3:3 + ny, 3:3 + nz] = prof * perturbation((nx, ny, nz), amp, seed, 1)
+    W[3:3 + nx, 3:3 + ny, 3:3 + nz - 1] = profw * perturbation((nx, ny, nz - 1), amp, seed, 2)
+    return U, V, W, np.zeros(shape, order="F"), []
+
+
 def initial_fields(cfg, u0=1.0, amp=0.1, seed=1, v0=0.0):
     """U = u0 + d, V = v0 + d, W = d on each component's interior; Pr = 0;
     scalars = 0.  Arrays are ghosted (lower bound -2), Fortran order."""

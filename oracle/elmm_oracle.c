@@ -37,7 +37,7 @@ enum { NOSLIP = 1, FREESLIP = 2, PERIODIC = 3, DIRICHLET = 4, NEUMANN = 5 };
 typedef struct {
   int n;
   int32_t *ijk;
-  double *dist, *z0, *ustar, *fluxp;
+  double *dist, *z0, *ustar, *fluxp, *fluxm;
 } wm_list;
 
 struct oracle {
@@ -157,6 +157,7 @@ static void copy_wm(wm_list *d, int n, const int32_t *ijk, const double *dist, c
   d->z0 = xcalloc((size_t)n, sizeof(double));
   d->ustar = xcalloc((size_t)n, sizeof(double));
   d->fluxp = xcalloc((size_t)n, sizeof(double));
+  d->fluxm = xcalloc((size_t)n, sizeof(double));
   if (n) {
     memcpy(d->ijk, ijk, 3 * (size_t)n * sizeof(int32_t));
     memcpy(d->dist, dist, 3 * (size_t)n * sizeof(double));
@@ -299,6 +300,7 @@ static void free_wm(wm_list *l) {
   free(l->z0);
   free(l->ustar);
   free(l->fluxp);
+  free(l->fluxm);
 }
 
 void oracle_destroy(oracle *o) {
@@ -937,22 +939,30 @@ static void wm_uvw_fluxes(oracle *o) {
       if (ustar < 0) ustar = 0;
       L->ustar[p] = ustar;
       double mag = v;
-      double fluxp;
+      /* fluxp scales the flux to the cell beyond the face, fluxm to the
+         cell before it; they differ only on stretched z grids */
+      double fluxp, fluxm;
       if (mag > eps) {
         if (o->variable_z && dir == 4) {
-          fluxp = (tan[comp] / mag) * (ustar * ustar);
+          fluxp = fluxm = (tan[comp] / mag) * (ustar * ustar);
           fluxp = comp < 2 ? fluxp / C1D(o->dzPr, zk) : fluxp / C1D(o->dzW, zk);
+          fluxm = comp < 2 ? fluxm / C1D(o->dzPr, zk - 1) : fluxm / C1D(o->dzW, zk - 1);
         } else if (o->variable_z && dir == 5) {
-          fluxp = (tan[comp] / mag) * (ustar * ustar);
+          fluxp = fluxm = (tan[comp] / mag) * (ustar * ustar);
           fluxp = comp < 2 ? fluxp / C1D(o->dzPr, zk + 1) : fluxp / C1D(o->dzW, zk + 1);
+          fluxm = comp < 2 ? fluxm / C1D(o->dzPr, zk) : fluxm / C1D(o->dzW, zk);
         } else {
-          fluxp = (tan[comp] / mag) * (ustar * ustar) * drec[dir];
+          fluxp = fluxm = (tan[comp] / mag) * (ustar * ustar) * drec[dir];
         }
-        if ((dir + 1) % 2 == 1) fluxp = -fluxp;
+        if ((dir + 1) % 2 == 1) {
+          fluxp = -fluxp;
+          fluxm = -fluxm;
+        }
       } else {
-        fluxp = 0;
+        fluxp = fluxm = 0;
       }
       L->fluxp[p] = fluxp;
+      L->fluxm[p] = fluxm;
     }
   }
 }
@@ -1323,7 +1333,147 @@ static void momentum_diffusion(oracle *o) {
   }
 }
 
-/* Convection (advection_method 4, explicit diffusion), src/dynamics.f90:230-499 */
+/* ------------------------------------- stretched z (GRID_VARIABLE_Z) --- */
+/* CDU/V/W_variable_z = (CD?div_variable_z + CD?adv_variable_z) * 0.5 on a
+   zeroed array, src/advection_variable_z.f90:15-321 (advection_method 2). */
+#define UU(i, j, k) A3(o, o->U, i, j, k)
+#define VV(i, j, k) A3(o, o->V, i, j, k)
+#define WW(i, j, k) A3(o, o->W, i, j, k)
+static double cd2vz_u(oracle *o, int i, int j, int k) {
+  double Ax = 0.25 / o->dxmin, Ay = 0.25 / o->dymin;
+  double d = -(((Ax * (UU(i + 1, j, k) + UU(i, j, k)) * (UU(i + 1, j, k) + UU(i, j, k)) -
+                 Ax * (UU(i, j, k) + UU(i - 1, j, k)) * (UU(i, j, k) + UU(i - 1, j, k))) +
+                (Ay * (UU(i, j + 1, k) + UU(i, j, k)) * (VV(i + 1, j, k) + VV(i, j, k)) -
+                 Ay * (UU(i, j, k) + UU(i, j - 1, k)) * (VV(i + 1, j - 1, k) + VV(i, j - 1, k)))) +
+               ((UU(i, j, k + 1) + UU(i, j, k)) * (WW(i + 1, j, k) + WW(i, j, k)) -
+                (UU(i, j, k) + UU(i, j, k - 1)) * (WW(i + 1, j, k - 1) + WW(i, j, k - 1))) /
+                   C1D(o->dzPr, k) / 4);
+  Ax = 0.5 / o->dxmin;
+  Ay = 0.125 / o->dymin;
+  double Vadv = VV(i, j, k) + VV(i + 1, j, k) + VV(i, j - 1, k) + VV(i + 1, j - 1, k);
+  double Wadv = (WW(i, j, k) + WW(i + 1, j, k) + WW(i, j, k - 1) + WW(i + 1, j, k - 1)) / 4;
+  d = d - (Ax * (UU(i + 1, j, k) - UU(i - 1, j, k)) * UU(i, j, k) + Ay * (UU(i, j + 1, k) - UU(i, j - 1, k)) * Vadv +
+           (UU(i, j, k + 1) - UU(i, j, k - 1)) / (C1D(o->zPr, k + 1) - C1D(o->zPr, k - 1)) * Wadv);
+  return d * 0.5;
+}
+static double cd2vz_v(oracle *o, int i, int j, int k) {
+  double Ax = 0.25 / o->dxmin, Ay = 0.25 / o->dymin;
+  double d = -(((Ay * (VV(i, j + 1, k) + VV(i, j, k)) * (VV(i, j + 1, k) + VV(i, j, k)) -
+                 Ay * (VV(i, j, k) + VV(i, j - 1, k)) * (VV(i, j, k) + VV(i, j - 1, k))) +
+                (Ax * (VV(i + 1, j, k) + VV(i, j, k)) * (UU(i, j + 1, k) + UU(i, j, k)) -
+                 Ax * (VV(i, j, k) + VV(i - 1, j, k)) * (UU(i - 1, j + 1, k) + UU(i - 1, j, k)))) +
+               ((VV(i, j, k + 1) + VV(i, j, k)) * (WW(i, j + 1, k) + WW(i, j, k)) -
+                (VV(i, j, k) + VV(i, j, k - 1)) * (WW(i, j + 1, k - 1) + WW(i, j, k - 1))) /
+                   C1D(o->dzPr, k) / 4);
+  Ax = 0.125 / o->dxmin;
+  Ay = 0.5 / o->dymin;
+  double Uadv = UU(i, j, k) + UU(i, j + 1, k) + UU(i - 1, j, k) + UU(i - 1, j + 1, k);
+  double Wadv = (WW(i, j, k) + WW(i, j + 1, k) + WW(i, j, k - 1) + WW(i, j + 1, k - 1)) / 4;
+  d = d - (Ax * (VV(i + 1, j, k) - VV(i - 1, j, k)) * Uadv + Ay * (VV(i, j + 1, k) - VV(i, j - 1, k)) * VV(i, j, k) +
+           (VV(i, j, k + 1) - VV(i, j, k - 1)) / (C1D(o->zPr, k + 1) - C1D(o->zPr, k - 1)) * Wadv);
+  return d * 0.5;
+}
+static double cd2vz_w(oracle *o, int i, int j, int k) {
+  double Ax = 0.25 / o->dxmin, Ay = 0.25 / o->dymin;
+  double d = -((((WW(i, j, k + 1) + WW(i, j, k)) * (WW(i, j, k + 1) + WW(i, j, k)) -
+                 (WW(i, j, k) + WW(i, j, k - 1)) * (WW(i, j, k) + WW(i, j, k - 1))) /
+                    C1D(o->dzW, k) / 4 +
+                (Ay * (WW(i, j + 1, k) + WW(i, j, k)) * (VV(i, j, k + 1) + VV(i, j, k)) -
+                 Ay * (WW(i, j, k) + WW(i, j - 1, k)) * (VV(i, j - 1, k + 1) + VV(i, j - 1, k)))) +
+               (Ax * (WW(i + 1, j, k) + WW(i, j, k)) * (UU(i, j, k + 1) + UU(i, j, k)) -
+                Ax * (WW(i, j, k) + WW(i - 1, j, k)) * (UU(i - 1, j, k + 1) + UU(i - 1, j, k))));
+  Ax = 0.125 / o->dxmin;
+  Ay = 0.125 / o->dymin;
+  double Uadv = UU(i, j, k) + UU(i, j, k + 1) + UU(i - 1, j, k) + UU(i - 1, j, k + 1);
+  double Vadv = VV(i, j, k) + VV(i, j, k + 1) + VV(i, j - 1, k) + VV(i, j - 1, k + 1);
+  d = d - (Ax * (WW(i + 1, j, k) - WW(i - 1, j, k)) * Uadv + Ay * (WW(i, j + 1, k) - WW(i, j - 1, k)) * Vadv +
+           (WW(i, j, k + 1) - WW(i, j, k - 1)) / (C1D(o->zW, k + 1) - C1D(o->zW, k - 1)) * WW(i, j, k));
+  return d * 0.5;
+}
+#undef UU
+#undef VV
+#undef WW
+
+/* z-direction face term of MomentumDiffusion_variable_z_nobranch_2ord for
+   component c on the face above level f (momentum_diffusion.f90:298-420):
+   U, V: 0.25 * nu4 * (F(f+1) - F(f)) / dzW(f); W: nu(f+1) * (W(f+1) - W(f)) /
+   dzPr(f+1). */
+static double zflux_vz(oracle *o, int c, int i, int j, int f) {
+  const double *nu = o->Visc;
+  if (c == 0)
+    return 0.25 * (A3(o, nu, i + 1, j, f + 1) + A3(o, nu, i + 1, j, f) + A3(o, nu, i, j, f + 1) + A3(o, nu, i, j, f)) *
+           (A3(o, o->U, i, j, f + 1) - A3(o, o->U, i, j, f)) / C1D(o->dzW, f);
+  if (c == 1)
+    return 0.25 * (A3(o, nu, i, j + 1, f + 1) + A3(o, nu, i, j + 1, f) + A3(o, nu, i, j, f + 1) + A3(o, nu, i, j, f)) *
+           (A3(o, o->V, i, j, f + 1) - A3(o, o->V, i, j, f)) / C1D(o->dzW, f);
+  return A3(o, nu, i, j, f + 1) * (A3(o, o->W, i, j, f + 1) - A3(o, o->W, i, j, f)) / C1D(o->dzPr, f + 1);
+}
+
+/* wall-model replacement term of a z face in the variable-z includes
+   (wmfluxes-variable_z-nobranch-{U,V,W}-inc.f90): the face flux written as
+   nu4 * dF / dzW(f) / 4 / dzPr(t) for U, V and nu * dW / dzPr(f+1) / dzW(t)
+   for W, t the level of the cell written. */
+static double zterm_vz(oracle *o, int c, int i, int j, int f, int t) {
+  const double *nu = o->Visc;
+  if (c == 0)
+    return (A3(o, nu, i + 1, j, f + 1) + A3(o, nu, i + 1, j, f) + A3(o, nu, i, j, f + 1) + A3(o, nu, i, j, f)) *
+           (A3(o, o->U, i, j, f + 1) - A3(o, o->U, i, j, f)) / C1D(o->dzW, f) / 4 / C1D(o->dzPr, t);
+  if (c == 1)
+    return (A3(o, nu, i, j + 1, f + 1) + A3(o, nu, i, j + 1, f) + A3(o, nu, i, j, f + 1) + A3(o, nu, i, j, f)) *
+           (A3(o, o->V, i, j, f + 1) - A3(o, o->V, i, j, f)) / C1D(o->dzW, f) / 4 / C1D(o->dzPr, t);
+  return A3(o, nu, i, j, f + 1) * (A3(o, o->W, i, j, f + 1) - A3(o, o->W, i, j, f)) / C1D(o->dzPr, f + 1) /
+         C1D(o->dzW, t);
+}
+
+static void momentum_diffusion_vz(oracle *o) {
+  double *S[3] = {o->Us, o->Vs, o->Ws};
+  const int *N[3] = {o->Un, o->Vn, o->Wn};
+  for (int c = 0; c < 3; c++) {
+    const int *n = N[c];
+    flux_fn F = FLUX[c];
+    double *s = S[c];
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= n[2]; k++)
+      for (int j = 1; j <= n[1]; j++)
+        for (int i = 1; i <= n[0]; i++) {
+          double a = A3(o, s, i, j, k);
+          a = a + F(o, 0, i, j, k);
+          a = a - F(o, 0, i - 1, j, k);
+          a = a + F(o, 1, i, j, k);
+          a = a - F(o, 1, i, j - 1, k);
+          double tmp = zflux_vz(o, c, i, j, k);
+          tmp = tmp - zflux_vz(o, c, i, j, k - 1);
+          tmp = tmp / (c < 2 ? C1D(o->dzPr, k) : C1D(o->dzW, k));
+          a = a + tmp;
+          A3(o, s, i, j, k) = a;
+        }
+    if (!o->c.wall_model) continue;
+    /* z lists only: stretched grids are accepted with periodic x and y and
+       no solid bodies */
+    for (int dir = 4; dir < 6; dir++) {
+      wm_list *L = &o->wm[c * 6 + dir];
+      int plus = dir % 2;
+      for (int pass = 0; pass < 2; pass++)
+        for (int p = 0; p < L->n; p++) {
+          int q[3] = {L->ijk[3 * p], L->ijk[3 * p + 1], L->ijk[3 * p + 2]};
+          int f = plus ? q[2] : q[2] - 1;
+          int t = plus ? (pass == 0 ? q[2] + 1 : q[2]) : (pass == 0 ? q[2] : q[2] - 1);
+          double fl = zterm_vz(o, c, q[0], q[1], f, t);
+          double *w = &A3(o, s, q[0], q[1], t);
+          if (pass == 0) {
+            *w = *w + fl;
+            *w = *w + L->fluxp[p];
+          } else {
+            *w = *w - fl;
+            *w = *w - L->fluxm[p];
+          }
+        }
+    }
+  }
+}
+
+/* Convection (advection_method 4 on uniform grids, 2 on stretched z; explicit
+   diffusion), src/dynamics.f90:230-499 */
 static void convection(oracle *o, int stage, double dt) {
   double *U2[3] = {o->U2, o->V2, o->W2};
   double *S[3] = {o->Us, o->Vs, o->Ws};
@@ -1341,19 +1491,34 @@ static void convection(oracle *o, int stage, double dt) {
           }
         }
   }
-  /* CD4div*: set(U2, 0) on the whole array, accumulate, multiply by -1/256 */
-  for (int c = 0; c < 3; c++) {
-    double *s = S[c];
-    /* entries outside the interior end as 0 * (-1/256) = -0.0 */
-    for (long p = 0; p < o->ntot; p++) s[p] = 0.0 * (-1.0 / 256);
-    const int *n = N[c];
+  if (o->variable_z) {
+    /* CD?_variable_z: set(U2, 0), div, adv, multiply by 0.5 */
+    for (int c = 0; c < 3; c++) {
+      double *s = S[c];
+      for (long p = 0; p < o->ntot; p++) s[p] = 0.0;
+      const int *n = N[c];
 #pragma omp parallel for collapse(2)
-    for (int k = 1; k <= n[2]; k++)
-      for (int j = 1; j <= n[1]; j++)
-        for (int i = 1; i <= n[0]; i++)
-          A3(o, s, i, j, k) = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
+      for (int k = 1; k <= n[2]; k++)
+        for (int j = 1; j <= n[1]; j++)
+          for (int i = 1; i <= n[0]; i++)
+            A3(o, s, i, j, k) = c == 0 ? cd2vz_u(o, i, j, k) : c == 1 ? cd2vz_v(o, i, j, k) : cd2vz_w(o, i, j, k);
+    }
+    momentum_diffusion_vz(o);
+  } else {
+    /* CD4div*: set(U2, 0) on the whole array, accumulate, multiply by -1/256 */
+    for (int c = 0; c < 3; c++) {
+      double *s = S[c];
+      /* entries outside the interior end as 0 * (-1/256) = -0.0 */
+      for (long p = 0; p < o->ntot; p++) s[p] = 0.0 * (-1.0 / 256);
+      const int *n = N[c];
+#pragma omp parallel for collapse(2)
+      for (int k = 1; k <= n[2]; k++)
+        for (int j = 1; j <= n[1]; j++)
+          for (int i = 1; i <= n[0]; i++)
+            A3(o, s, i, j, k) = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
+    }
+    momentum_diffusion(o);
   }
-  momentum_diffusion(o);
   for (int c = 0; c < 3; c++) {
     const int *n = N[c];
     for (int k = 1; k <= n[2]; k++)
@@ -2079,13 +2244,32 @@ void oracle_op_advection(oracle *o) {
     for (long p = 0; p < o->ntot; p++) S[c][p] = 0;
     for (int k = 1; k <= N[c][2]; k++)
       for (int j = 1; j <= N[c][1]; j++)
-        for (int i = 1; i <= N[c][0]; i++)
-          A3(o, S[c], i, j, k) = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
+        for (int i = 1; i <= N[c][0]; i++) {
+          double v;
+          if (o->variable_z)
+            v = c == 0 ? cd2vz_u(o, i, j, k) : c == 1 ? cd2vz_v(o, i, j, k) : cd2vz_w(o, i, j, k);
+          else
+            v = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
+          A3(o, S[c], i, j, k) = v;
+        }
   }
 }
 void oracle_op_diffusion(oracle *o) {
   for (long p = 0; p < o->ntot; p++) o->Us[p] = o->Vs[p] = o->Ws[p] = 0;
-  momentum_diffusion(o);
+  if (o->variable_z)
+    momentum_diffusion_vz(o);
+  else
+    momentum_diffusion(o);
+}
+void oracle_op_wm_fluxes(oracle *o) { wm_uvw_fluxes(o); }
+int32_t oracle_wm_list(oracle *o, int32_t list, double *fluxp, double *fluxm) {
+  if (list < 0 || list >= 18) return -1;
+  wm_list *L = &o->wm[list];
+  for (int p = 0; p < L->n; p++) {
+    if (fluxp) fluxp[p] = L->fluxp[p];
+    if (fluxm) fluxm[p] = L->fluxm[p];
+  }
+  return L->n;
 }
 void oracle_op_kappa(oracle *o, int32_t sc) { kappa_scalar(o, o->Sadv[sc], o->S[sc]); }
 void oracle_op_sgs(oracle *o) { subgrid_stresses(o); }

@@ -52,12 +52,16 @@ enum { ORACLE_F_USTAR = 8, ORACLE_F_VSTAR = 9, ORACLE_F_WSTAR = 10, ORACLE_F_RHS
 
 /* Single operators on the current state, for operator-level tests. */
 void oracle_op_bound_uvw(oracle *o);           /* BoundUVW(U,V,W)              */
-void oracle_op_advection(oracle *o);           /* CD4div* -> Ustar,Vstar,Wstar */
+void oracle_op_advection(oracle *o);           /* CD4div* (CD?_variable_z) -> Ustar.. */
 void oracle_op_diffusion(oracle *o);           /* diffusion incl. WM -> Ustar.. */
 void oracle_op_kappa(oracle *o, int32_t sc);   /* KappaScalar -> Sadv(sc)       */
 void oracle_op_sgs(oracle *o);                 /* SubgridStresses              */
 void oracle_op_poisson(oracle *o, const double *rhs_interior, double *phi_interior);
 double oracle_op_residual(oracle *o, const double *phi_ghosted, const double *rhs_ghosted);
+void oracle_op_wm_fluxes(oracle *o);            /* ComputeUVWFluxesWM           */
+/* fluxp/fluxm of momentum wall-model list `list` (comp*6+dir); returns the
+   number of points (buffers may be NULL to query it) */
+int32_t oracle_wm_list(oracle *o, int32_t list, double *fluxp, double *fluxm);
 
 #ifdef __cplusplus
 }

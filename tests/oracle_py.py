@@ -40,6 +40,9 @@ def _load():
         lib.oracle_op_poisson.argtypes = [C.c_void_p, pd, pd]
         lib.oracle_op_residual.argtypes = [C.c_void_p, pd, pd]
         lib.oracle_op_residual.restype = C.c_double
+        lib.oracle_op_wm_fluxes.argtypes = [C.c_void_p]
+        lib.oracle_wm_list.argtypes = [C.c_void_p, C.c_int32, pd, pd]
+        lib.oracle_wm_list.restype = C.c_int32
         _lib = lib
     return _lib
 
@@ -106,6 +109,18 @@ class Oracle:
 
     def op_diffusion(self):
         _lib.oracle_op_diffusion(self._h)
+
+    def op_wm_fluxes(self):
+        _lib.oracle_op_wm_fluxes(self._h)
+
+    def wm_list(self, comp, direction):
+        """(fluxp, fluxm) of the momentum wall-model list of component comp
+        (0..2) and direction 0..5 (x-, x+, y-, y+, z-, z+)."""
+        pd = C.POINTER(C.c_double)
+        n = _lib.oracle_wm_list(self._h, comp * 6 + direction, None, None)
+        fp, fm = np.zeros(n), np.zeros(n)
+        _lib.oracle_wm_list(self._h, comp * 6 + direction, fp.ctypes.data_as(pd), fm.ctypes.data_as(pd))
+        return fp, fm
 
     def op_kappa(self, sc=0):
         _lib.oracle_op_kappa(self._h, sc)
