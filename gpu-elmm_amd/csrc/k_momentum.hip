@@ -106,6 +106,43 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
 // ------------------------------------------------------------- wall model
 // vel_and_dist, WMRoughUstar, WMFlatUstar, fluxes:
 // src/wallmodels.f90:1190-1240, 1312-1391, 2159-2337, 2356-2399
+// Natural logarithm of a positive normal x with the same operations as the
+// oracle (so CPU and GPU agree bit for bit; the device and host libm logs
+// may differ by an ulp): x = 2^k m, m in [sqrt(2)/2, sqrt(2)), f = m - 1,
+// s = f / (2 + f), log(1 + f) = 2 atanh(s) = f - (hfsq - s (hfsq + R(s^2))),
+// R(z) = sum_{n=1..11} 2 z^n / (2n + 1); log x = k ln2_hi - ((hfsq -
+// (s (hfsq + R) + k ln2_lo)) - f), k ln2_hi exact (31-bit ln2_hi).  The
+// reduction of P. T. P. Tang (ACM TOMS 16, 1990), Taylor polynomial.
+__device__ inline double det_log(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  int k = (int)((b >> 52) & 0x7ff) - 1023;
+  b = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m = __longlong_as_double((long long)b);
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    k = k + 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double R = 2.0 / 23.0;
+  R = 2.0 / 21.0 + z * R;
+  R = 2.0 / 19.0 + z * R;
+  R = 2.0 / 17.0 + z * R;
+  R = 2.0 / 15.0 + z * R;
+  R = 2.0 / 13.0 + z * R;
+  R = 2.0 / 11.0 + z * R;
+  R = 2.0 / 9.0 + z * R;
+  R = 2.0 / 7.0 + z * R;
+  R = 2.0 / 5.0 + z * R;
+  R = 2.0 / 3.0 + z * R;
+  R = z * R;
+  const double hfsq = 0.5 * f * f;
+  const double dk = (double)k;
+  const double ln2_hi = 0x1.62e42fee00000p-1, ln2_lo = 0x1.a39ef35793c76p-33;
+  return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
 __device__ inline double wm_flat_ustar(double ustar, double vel, double dist, double nu) {
   const double eps = 1e-4;
   if (nu <= 0) return 0;
@@ -117,9 +154,9 @@ __device__ inline double wm_flat_ustar(double ustar, double vel, double dist, do
     if (z_pl <= 5.0)
       ustar = sqrt(nu * vel / dist);
     else if (z_pl < 30.0)
-      ustar = vel / (5 * log(z_pl) - (double)3.05f);
+      ustar = vel / (5 * det_log(z_pl) - (double)3.05f);
     else
-      ustar = vel / (log(z_pl) / 0.4 + 5.5);
+      ustar = vel / (det_log(z_pl) / 0.4 + 5.5);
     if (fabs(ustar - ustar2) / fabs(ustar) < eps) break;
     if (i >= 50) {
       ustar = 0;

@@ -191,126 +191,116 @@ __global__ void k_fft_y(PoissonPlan<R> pp, int logn, int inverse) {
 }
 
 // Pass 3: per (m, j) mode, a_k x_{k-1} + (lam - a_k - c_k) x_k + c_k x_{k+1}
-// = d_k, solved by parallel cyclic reduction in LDS.  The singular mode
-// (m=j=0) pins x_1 = 0 and is integrated exactly (see below).
-// R: storage type of the spectral array; A: arithmetic of the elimination
-// (double for both forms: the lowest modes' tridiagonals are too
-// ill-conditioned for FP32 elimination inside a one-sweep refinement).
-template <int TX, typename R, typename A_t = double>
-__global__ void k_solve_z(PoissonPlan<R> pp) {
-  extern __shared__ __align__(16) unsigned char sh_raw[];
-  using V = C2<A_t>;
-  const int nz = pp.nz, ny = pp.ny, M = pp.M;
-  const int m0 = blockIdx.x * TX, j = blockIdx.y;
-  const int N = TX * nz;
-  V *D = reinterpret_cast<V *>(sh_raw);
-  A_t *A = reinterpret_cast<A_t *>(D + N), *B = A + N, *C = B + N;
-  const long plane = (long)ny * M;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) {
-    const int c = t % TX, k = t / TX;
-    const int m = m0 + c;
-    const int s = c * nz + k;
-    V d = mk2((A_t)0, (A_t)0);
-    A_t lam = 0;
-    if (m < M) {
-      const C2<R> x = pp.X[(long)k * plane + (long)j * M + m];
-      d = mk2((A_t)x.x, (A_t)x.y);
-      lam = (A_t)pp.lamx[m] + (A_t)pp.lamy[j];
-    }
-    A_t a = (A_t)pp.az[k], cc = (A_t)pp.cz[k];
-    A_t b = lam - a - cc;
-    if ((m == 0 && j == 0) || m >= M) {
-      // identity rows: padding columns, and the singular mean mode, which is
-      // integrated exactly after the reduction (its D survives unchanged)
-      a = 0;
-      b = 1;
-      cc = 0;
-      if (m >= M) d = mk2((A_t)0, (A_t)0);
-    }
-    A[s] = a;
-    B[s] = b;
-    C[s] = cc;
-    D[s] = d;
-  }
-  __syncthreads();
-  for (int h = 1; h < nz; h <<= 1) {
-    A_t na[8], nb[8], nc[8];
-    V nd[8];
-    int cnt = 0;
-    for (int t = threadIdx.x; t < N; t += blockDim.x, cnt++) {
-      const int c = t / nz, k = t % nz;
-      const int s = c * nz + k;
-      A_t a = A[s], b = B[s], cc = C[s];
-      V d = D[s];
-      A_t a2 = 0, c2 = 0;
-      V d2 = d;
-      A_t b2 = b;
-      if (k - h >= 0) {
-        const A_t k1 = a / B[s - h];
-        a2 = -A[s - h] * k1;
-        b2 = b2 - C[s - h] * k1;
-        d2 = mk2(d2.x - D[s - h].x * k1, d2.y - D[s - h].y * k1);
-      }
-      if (k + h < nz) {
-        const A_t k2 = cc / B[s + h];
-        c2 = -C[s + h] * k2;
-        b2 = b2 - A[s + h] * k2;
-        d2 = mk2(d2.x - D[s + h].x * k2, d2.y - D[s + h].y * k2);
-      }
-      na[cnt] = a2;
-      nb[cnt] = b2;
-      nc[cnt] = c2;
-      nd[cnt] = d2;
-    }
-    __syncthreads();
-    cnt = 0;
-    for (int t = threadIdx.x; t < N; t += blockDim.x, cnt++) {
-      const int c = t / nz, k = t % nz;
-      const int s = c * nz + k;
-      A[s] = na[cnt];
-      B[s] = nb[cnt];
-      C[s] = nc[cnt];
-      D[s] = nd[cnt];
-    }
-    __syncthreads();
-  }
-  if (m0 == 0 && j == 0 && threadIdx.x == 0) {
+// = d_k (a_1 = c_nz = 0), one thread per mode, Thomas elimination in FP64 for
+// both forms (the lowest modes' tridiagonals are too ill-conditioned for an
+// FP32 elimination inside a one-sweep refinement).  Every mode but the
+// horizontal mean has lam < 0, so the rows are strictly diagonally dominant
+// and need no pivoting.  The forward state (c', d') is checkpointed in LDS at
+// the end of every SEG levels; the back substitution recomputes one segment
+// at a time into registers from its checkpoint, so the spectral array is
+// read twice and written once, coalesced across the modes of a wavefront.
+// The singular mode (m = j = 0) pins x_1 = 0 and is integrated exactly.
+constexpr int ZSEG = 16, ZSEG_MAX = 16;  // nz <= 256
+
+template <typename R>
+__global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
+  __shared__ double ck[3][ZSEG_MAX][64];
+  __shared__ double gm[2][256];
+  const int t = threadIdx.x;
+  const long plane = (long)pp.ny * pp.M;
+  const long q = (long)blockIdx.x * 64 + t;
+  if (q >= plane) return;
+  const int nz = pp.nz;
+  C2<R> *X = pp.X + q;
+  if (q == 0) {
     // Horizontal-mean mode with x_1 pinned to 0 (rows 2..nz of the Neumann
     // problem): G_{k-1} = G_k - d_k dzPr(k) from G_nz = 0 downwards, then
     // x_{k+1} = x_k + G_k dzW(k).  Both sums are compensated (Kahan), so the
     // mode is exact to rounding instead of carrying the nz^2 growth of an
     // elimination on the singular operator.
-    V G = mk2((A_t)0, (A_t)0), Gc = G;
+    double Gr = 0, Gi = 0, cr = 0, ci = 0;
     for (int k = nz; k >= 2; k--) {
-      const A_t dz = (A_t)pp.dzPr[k];
-      V t = mk2(-D[k - 1].x * dz, -D[k - 1].y * dz);
-      V y = mk2(t.x - Gc.x, t.y - Gc.y);
-      V s2 = mk2(G.x + y.x, G.y + y.y);
-      Gc = mk2((s2.x - G.x) - y.x, (s2.y - G.y) - y.y);
-      G = s2;
-      A[k - 2] = G.x;  // column 0 of A and C is free after the reduction
-      C[k - 2] = G.y;
+      const double dz = pp.dzPr[k];
+      const C2<R> d = X[(long)(k - 1) * plane];
+      double tt = -(double)d.x * dz, y = tt - cr, s2 = Gr + y;
+      cr = (s2 - Gr) - y;
+      Gr = s2;
+      tt = -(double)d.y * dz;
+      y = tt - ci;
+      s2 = Gi + y;
+      ci = (s2 - Gi) - y;
+      Gi = s2;
+      gm[0][k - 2] = Gr;
+      gm[1][k - 2] = Gi;
     }
-    V x = mk2((A_t)0, (A_t)0), xc = x;
-    D[0] = x;
+    double xr = 0, xi = 0;
+    cr = ci = 0;
+    X[0] = mk2((R)0, (R)0);
     for (int k = 1; k <= nz - 1; k++) {
-      const A_t dz = (A_t)pp.dzW[k];
-      V t = mk2(A[k - 1] * dz, C[k - 1] * dz);
-      V y = mk2(t.x - xc.x, t.y - xc.y);
-      V s2 = mk2(x.x + y.x, x.y + y.y);
-      xc = mk2((s2.x - x.x) - y.x, (s2.y - x.y) - y.y);
-      x = s2;
-      D[k] = x;
+      const double dz = pp.dzW[k];
+      double tt = gm[0][k - 1] * dz, y = tt - cr, s2 = xr + y;
+      cr = (s2 - xr) - y;
+      xr = s2;
+      tt = gm[1][k - 1] * dz;
+      y = tt - ci;
+      s2 = xi + y;
+      ci = (s2 - xi) - y;
+      xi = s2;
+      X[(long)k * plane] = mk2((R)xr, (R)xi);
+    }
+    return;
+  }
+  const int m = (int)(q % pp.M), j = (int)(q / pp.M);
+  const double lam = pp.lamx[m] + pp.lamy[j];
+  // forward elimination; den = b - a c', every division a multiplication by
+  // 1/den
+  double cp = 0, dr = 0, di = 0;
+  for (int k = 0; k < nz; k++) {
+    const double a = pp.az[k], c = pp.cz[k];
+    const double b = lam - a - c;
+    const C2<R> x = X[(long)k * plane];
+    const double inv = 1.0 / (b - a * cp);
+    cp = c * inv;
+    dr = ((double)x.x - a * dr) * inv;
+    di = ((double)x.y - a * di) * inv;
+    if (k % ZSEG == ZSEG - 1) {
+      ck[0][k / ZSEG][t] = cp;
+      ck[1][k / ZSEG][t] = dr;
+      ck[2][k / ZSEG][t] = di;
     }
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < N; t += blockDim.x) {
-    const int c = t % TX, k = t / TX;
-    const int m = m0 + c;
-    const int s = c * nz + k;
-    if (m < M) {
-      const A_t b = B[s];
-      pp.X[(long)k * plane + (long)j * M + m] = mk2((R)(D[s].x / b), (R)(D[s].y / b));
+  // back substitution x_k = d'_k - c'_k x_{k+1}, segment by segment
+  double xr = 0, xi = 0;
+  for (int s = (nz - 1) / ZSEG; s >= 0; s--) {
+    const int k0 = s * ZSEG;
+    double scp[ZSEG], sdr[ZSEG], sdi[ZSEG];
+    cp = s ? ck[0][s - 1][t] : 0.0;
+    dr = s ? ck[1][s - 1][t] : 0.0;
+    di = s ? ck[2][s - 1][t] : 0.0;
+#pragma unroll
+    for (int r = 0; r < ZSEG; r++) {
+      const int k = k0 + r;
+      if (k < nz) {
+        const double a = pp.az[k], c = pp.cz[k];
+        const double b = lam - a - c;
+        const C2<R> x = X[(long)k * plane];
+        const double inv = 1.0 / (b - a * cp);
+        cp = c * inv;
+        dr = ((double)x.x - a * dr) * inv;
+        di = ((double)x.y - a * di) * inv;
+      }
+      scp[r] = cp;
+      sdr[r] = dr;
+      sdi[r] = di;
+    }
+#pragma unroll
+    for (int r = ZSEG - 1; r >= 0; r--) {
+      const int k = k0 + r;
+      if (k < nz) {
+        xr = sdr[r] - scp[r] * xr;
+        xi = sdi[r] - scp[r] * xi;
+        X[(long)k * plane] = mk2((R)xr, (R)xi);
+      }
     }
   }
 }
@@ -394,9 +384,9 @@ void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
 
 template <typename R>
 void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t st) {
-  constexpr int TX = 8;
-  size_t sh = (size_t)TX * pp.nz * (3 * sizeof(double) + sizeof(double2));
-  hipLaunchKernelGGL((k_solve_z<TX, R, double>), dim3(nblk(pp.M, TX), pp.ny), dim3(256), sh, st, pp);
+  // nz <= ZSEG * ZSEG_MAX (256) is checked when the plan is set up
+  const long modes = (long)pp.M * pp.ny;
+  hipLaunchKernelGGL(k_solve_z<R>, dim3((unsigned)((modes + 63) / 64)), dim3(64), 0, st, pp);
 }
 
 template <typename R>

@@ -851,6 +851,47 @@ static void vel_and_dist(double tan[3], double *dist, const double u[3], const d
 }
 
 /* WMFlatUstar, src/wallmodels.f90:1190-1240 */
+/* Natural logarithm of a positive normal x, evaluated identically by the
+   oracle and the GPU library (libm's log differs between the two by up to an
+   ulp): x = 2^k m with m in [sqrt(2)/2, sqrt(2)), f = m - 1 (exact),
+   s = f / (2 + f), log(1 + f) = 2 atanh(s) = f - (hfsq - s (hfsq + R(s^2)))
+   with hfsq = f^2 / 2 and R(z) = sum_{n=1..11} 2 z^n / (2n + 1), the Taylor
+   series of the atanh expansion (truncation below 1e-19 for |s| <= 0.172);
+   log x = k ln2_hi - ((hfsq - (s (hfsq + R) + k ln2_lo)) - f), ln2_hi holding
+   31 significant bits so that k ln2_hi is exact.  This is the reduction of
+   P. T. P. Tang (ACM TOMS 16, 1990) with a plain Taylor polynomial. */
+double oracle_det_log(double x) {
+  uint64_t b;
+  memcpy(&b, &x, sizeof b);
+  int k = (int)((b >> 52) & 0x7ff) - 1023;
+  b = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m;
+  memcpy(&m, &b, sizeof m);
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    k = k + 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double R = 2.0 / 23.0;
+  R = 2.0 / 21.0 + z * R;
+  R = 2.0 / 19.0 + z * R;
+  R = 2.0 / 17.0 + z * R;
+  R = 2.0 / 15.0 + z * R;
+  R = 2.0 / 13.0 + z * R;
+  R = 2.0 / 11.0 + z * R;
+  R = 2.0 / 9.0 + z * R;
+  R = 2.0 / 7.0 + z * R;
+  R = 2.0 / 5.0 + z * R;
+  R = 2.0 / 3.0 + z * R;
+  R = z * R;
+  const double hfsq = 0.5 * f * f;
+  const double dk = (double)k;
+  const double ln2_hi = 0x1.62e42fee00000p-1, ln2_lo = 0x1.a39ef35793c76p-33;
+  return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
 static double wm_flat_ustar(double ustar, double vel, double dist, double nu) {
   const double eps = 1e-4;
   const double zpl_lam = 5.0, zpl_turb = 30.0;
@@ -865,9 +906,9 @@ static double wm_flat_ustar(double ustar, double vel, double dist, double nu) {
     if (z_pl <= zpl_lam)
       ustar = sqrt(nu * vel / dist);
     else if (z_pl < zpl_turb)
-      ustar = vel / (5 * log(z_pl) - (double)3.05f);
+      ustar = vel / (5 * oracle_det_log(z_pl) - (double)3.05f);
     else
-      ustar = vel / (log(z_pl) / 0.4 + 5.5);
+      ustar = vel / (oracle_det_log(z_pl) / 0.4 + 5.5);
     if (fabs(ustar - ustar2) / fabs(ustar) < eps) break;
     if (i >= 50) {
       ustar = 0;
@@ -1734,8 +1775,8 @@ static void scalar_rk3(oracle *o, int stage, double dt) {
    src/poisson.f90:23-60) directly: DFT in the periodic x and y directions,
    a tridiagonal system per mode in z.  The transform is an iterative
    radix-2 decimation-in-time FFT with twiddles exp(-2 pi i q/n) taken from a
-   long-double table, and the tridiagonal systems are reduced by parallel
-   cyclic reduction (executed serially here); the singular horizontal-mean
+   long-double table, and the tridiagonal systems are solved by Thomas
+   elimination in the operation order of the GPU kernel; the singular horizontal-mean
    mode is integrated exactly with compensated sums.  The operation order is
    stated so that results can be compared bit for bit. */
 typedef struct {
@@ -1786,40 +1827,28 @@ static void fft_line(c2 *x, int n, long stride, const c2 *tw, int inverse, c2 *w
   for (int m = 0; m < n; m++) x[m * stride] = work[m];
 }
 
-/* Parallel cyclic reduction of one tridiagonal system (serial execution of
-   the parallel steps): a_k x_{k-h} + b_k x_k + c_k x_{k+h} = d_k. */
-static void pcr_solve(int nz, double *a, double *b, double *c, c2 *d, double *na, double *nb, double *nc, c2 *nd) {
-  for (int h = 1; h < nz; h <<= 1) {
-    for (int k = 0; k < nz; k++) {
-      double a2 = 0, c2v = 0, b2 = b[k];
-      c2 d2 = d[k];
-      if (k - h >= 0) {
-        double k1 = a[k] / b[k - h];
-        a2 = -a[k - h] * k1;
-        b2 = b2 - c[k - h] * k1;
-        d2.re = d2.re - d[k - h].re * k1;
-        d2.im = d2.im - d[k - h].im * k1;
-      }
-      if (k + h < nz) {
-        double k2 = c[k] / b[k + h];
-        c2v = -c[k + h] * k2;
-        b2 = b2 - a[k + h] * k2;
-        d2.re = d2.re - d[k + h].re * k2;
-        d2.im = d2.im - d[k + h].im * k2;
-      }
-      na[k] = a2;
-      nb[k] = b2;
-      nc[k] = c2v;
-      nd[k] = d2;
-    }
-    memcpy(a, na, nz * sizeof(double));
-    memcpy(b, nb, nz * sizeof(double));
-    memcpy(c, nc, nz * sizeof(double));
-    memcpy(d, nd, nz * sizeof(c2));
-  }
+/* Thomas elimination of one tridiagonal system a_k x_{k-1} + b_k x_k +
+   c_k x_{k+1} = d_k (a_0 = c_{nz-1} = 0; strictly diagonally dominant for
+   every mode but the horizontal mean): den_k = b_k - a_k c'_{k-1},
+   c'_k = c_k / den_k, d'_k = (d_k - a_k d'_{k-1}) / den_k with each division
+   taken as a multiplication by 1/den_k, then x_k = d'_k - c'_k x_{k+1}. */
+static void thomas_solve(int nz, const double *a, const double *b, const double *c, c2 *d, double *cp) {
+  double pc = 0, dr = 0, di = 0;
   for (int k = 0; k < nz; k++) {
-    d[k].re = d[k].re / b[k];
-    d[k].im = d[k].im / b[k];
+    double inv = 1.0 / (b[k] - a[k] * pc);
+    pc = c[k] * inv;
+    dr = (d[k].re - a[k] * dr) * inv;
+    di = (d[k].im - a[k] * di) * inv;
+    cp[k] = pc;
+    d[k].re = dr;
+    d[k].im = di;
+  }
+  double xr = 0, xi = 0;
+  for (int k = nz - 1; k >= 0; k--) {
+    xr = d[k].re - cp[k] * xr;
+    xi = d[k].im - cp[k] * xi;
+    d[k].re = xr;
+    d[k].im = xi;
   }
 }
 
@@ -1882,8 +1911,8 @@ static void poisson_direct(oracle *o, const double *rhs, double *phi) {
 #pragma omp parallel
   {
     double *a = xcalloc(nz, sizeof(double)), *b = xcalloc(nz, sizeof(double)), *c = xcalloc(nz, sizeof(double));
-    double *na = xcalloc(nz, sizeof(double)), *nb = xcalloc(nz, sizeof(double)), *nc = xcalloc(nz, sizeof(double));
-    c2 *d = xcalloc(nz, sizeof(c2)), *nd = xcalloc(nz, sizeof(c2));
+    double *na = xcalloc(nz, sizeof(double));
+    c2 *d = xcalloc(nz, sizeof(c2));
 #pragma omp for collapse(2)
     for (int j = 0; j < ny; j++)
       for (int m = 0; m < M; m++) {
@@ -1917,11 +1946,11 @@ static void poisson_direct(oracle *o, const double *rhs, double *phi) {
             c[k] = cz[k];
             b[k] = lam - a[k] - c[k];
           }
-          pcr_solve(nz, a, b, c, d, na, nb, nc, nd);
+          thomas_solve(nz, a, b, c, d, na);
         }
         for (int k = 0; k < nz; k++) X[(long)k * plane + (long)j * M + m] = d[k];
       }
-    free(a); free(b); free(c); free(na); free(nb); free(nc); free(d); free(nd);
+    free(a); free(b); free(c); free(na); free(d);
   }
   free(az); free(cz); free(lamx); free(lamy);
   /* inverse y, then inverse x from the half spectrum (Hermitian symmetry) */

@@ -58,6 +58,9 @@ void oracle_op_kappa(oracle *o, int32_t sc);   /* KappaScalar -> Sadv(sc)       
 void oracle_op_sgs(oracle *o);                 /* SubgridStresses              */
 void oracle_op_poisson(oracle *o, const double *rhs_interior, double *phi_interior);
 double oracle_op_residual(oracle *o, const double *phi_ghosted, const double *rhs_ghosted);
+/* the logarithm both the oracle and the GPU library use in the smooth-wall
+   law (positive normal x) */
+double oracle_det_log(double x);
 void oracle_op_wm_fluxes(oracle *o);            /* ComputeUVWFluxesWM           */
 /* fluxp/fluxm of momentum wall-model list `list` (comp*6+dir); returns the
    number of points (buffers may be NULL to query it) */

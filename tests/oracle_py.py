@@ -40,6 +40,8 @@ def _load():
         lib.oracle_op_poisson.argtypes = [C.c_void_p, pd, pd]
         lib.oracle_op_residual.argtypes = [C.c_void_p, pd, pd]
         lib.oracle_op_residual.restype = C.c_double
+        lib.oracle_det_log.argtypes = [C.c_double]
+        lib.oracle_det_log.restype = C.c_double
         lib.oracle_op_wm_fluxes.argtypes = [C.c_void_p]
         lib.oracle_wm_list.argtypes = [C.c_void_p, C.c_int32, pd, pd]
         lib.oracle_wm_list.restype = C.c_int32
@@ -48,6 +50,11 @@ def _load():
 
 
 F_USTAR, F_VSTAR, F_WSTAR, F_RHS, F_SADV = 8, 9, 10, 11, 32
+
+
+def det_log(x):
+    """The logarithm of the oracle's smooth-wall law."""
+    return _load().oracle_det_log(float(x))
 
 
 class Oracle:

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from elmm_amd import capi, cases
-from oracle_py import F_SADV, F_USTAR, F_VSTAR, Oracle
+from oracle_py import F_SADV, F_USTAR, F_VSTAR, Oracle, det_log
 
 RNG = np.random.default_rng(12345)
 
@@ -458,3 +458,20 @@ def test_pressure_reference_top_mean_zero():
     o, _ = run_steps(cfg, 2)
     pr = o.download(capi.F_PR)[3:19, 3:19, 3 + 11]
     assert abs(pr.mean()) < 1e-15 * max(1.0, np.abs(pr).max())
+
+
+# ------------------------------------------------------- smooth-wall log
+def test_det_log_within_one_ulp_of_libm():
+    """The logarithm of WMFlatUstar's log-law branches (src/wallmodels.f90:
+    1312-1391) is restated without libm so CPU and GPU agree bit for bit; it
+    stays within one ulp of the C library's log, is exact at 1, and gives
+    k ln 2 for powers of two."""
+    import math
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([np.exp(rng.uniform(-700, 700, 20000)), rng.uniform(0.5, 2.0, 20000),
+                         rng.uniform(5.0, 1e4, 20000)])
+    worst = max(abs(det_log(x) - math.log(x)) / np.spacing(abs(math.log(x))) for x in xs if x != 1.0)
+    assert worst <= 1.0
+    assert det_log(1.0) == 0.0
+    for k in (-1000, -3, 1, 7, 1000):
+        assert abs(det_log(2.0 ** k) - k * math.log(2.0)) <= np.spacing(abs(k * math.log(2.0)))
