@@ -189,8 +189,8 @@ struct Engine : EngineIface {
   DevBuf coords;
   // wall model
   DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
-  DevBuf width_k;
-  bool have_width_k = false;
+  DevBuf width_tab, width_idx;
+  SgsWidth sgs_width{nullptr, nullptr, 0};
   int wm_n = 0;
   DevBuf wmp_ijk, wmp_div;
   int wmp_n = 0;
@@ -308,17 +308,39 @@ void Engine<T>::setup_geo() {
   g.dzW = base + off[3] + 3;
   g.zPr = base + off[4] + 3;
   g.zW = base + off[5] + 3;
-  // SGS filter width per level (src/subgrid.f90:62): filter_ratio *
-  // (dxPr*dyPr*dzPr)**(1/3), tabulated when dxPr(i)*dyPr(j) is uniform
-  const double dxy = Grid1D::at(grid.dxPr, 1) * Grid1D::at(grid.dyPr, 1);
-  bool uni = true;
-  for (int i = 1; i <= nx && uni; i++)
-    for (int j = 1; j <= ny && uni; j++) uni = Grid1D::at(grid.dxPr, i) * Grid1D::at(grid.dyPr, j) == dxy;
-  if (uni) {
-    std::vector<double> wk(nz + 2, 0.0);
-    for (int k = 1; k <= nz; k++) wk[k] = cfg.filter_ratio * std::pow(dxy * Grid1D::at(grid.dzPr, k), 1.0 / 3.0);
-    upload_vec(width_k, wk);
-    have_width_k = true;
+  // SGS filter width (src/subgrid.f90:62): filter_ratio *
+  // (dxPr*dyPr*dzPr)**(1/3), from the host libm like the reference's.  The
+  // products dxPr(i)*dyPr(j) take few distinct values (one on exactly
+  // representable spacings), so the width is tabulated per distinct product
+  // and level, with a column -> product index (uint8).
+  {
+    std::vector<double> prods;
+    std::vector<uint8_t> idx((size_t)nx * ny, 0);
+    bool fits = true;
+    for (int j = 1; j <= ny && fits; j++)
+      for (int i = 1; i <= nx && fits; i++) {
+        const double d = Grid1D::at(grid.dxPr, i) * Grid1D::at(grid.dyPr, j);
+        size_t q = 0;
+        while (q < prods.size() && prods[q] != d) q++;
+        if (q == prods.size()) {
+          if (prods.size() == 255) {
+            fits = false;
+            break;
+          }
+          prods.push_back(d);
+        }
+        idx[(size_t)(i - 1) + (size_t)nx * (j - 1)] = (uint8_t)q;
+      }
+    if (fits) {
+      const int ld = nz + 2;
+      std::vector<double> wk(prods.size() * ld, 0.0);
+      for (size_t q = 0; q < prods.size(); q++)
+        for (int k = 1; k <= nz; k++)
+          wk[q * ld + k] = cfg.filter_ratio * std::pow(prods[q] * Grid1D::at(grid.dzPr, k), 1.0 / 3.0);
+      upload_vec(width_tab, wk);
+      if (prods.size() > 1) upload_vec(width_idx, idx);
+      sgs_width = SgsWidth{width_tab.as<double>(), prods.size() > 1 ? width_idx.as<uint8_t>() : nullptr, ld};
+    }
   }
 }
 
@@ -819,7 +841,7 @@ template <typename T>
 void Engine<T>::subgrid_stresses(bool cfl) {
   timed(ELMM_T_SGS, [&] {
     launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
-               have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model,
+               sgs_width, cfg.sgs_model,
                cfl ? cfl_partial.as<double>() : nullptr, cfl_blocks, stream);
   });
   if (cfg.wall_model && wm_n) timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream); });
@@ -932,7 +954,7 @@ void Engine<T>::init() {
   launch_bound_pr<T>(g, Pr.as<T>(), stream);
   pressure_correction(u, v, w, false, 1.0);
   launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
-             have_width_k ? width_k.as<double>() : nullptr, cfg.sgs_model, nullptr, 0, stream);
+             sgs_width, cfg.sgs_model, nullptr, 0, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
   if (nsc > 0) {
     launch_tdiff_init(g, Visc.as<T>(), TDiff.as<T>(), cfg.nu, cfg.nu_scalar, stream);

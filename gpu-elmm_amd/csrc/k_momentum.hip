@@ -24,7 +24,7 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 template <typename T>
 __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, const T *__restrict__ V,
                                              const T *__restrict__ W, T *__restrict__ Visc, double nu_,
-                                             double c_smag_, double filter_ratio, const double *__restrict__ width_k,
+                                             double c_smag_, double filter_ratio, SgsWidth width_t,
                                              int sgs_model, double *__restrict__ cfl_partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -57,9 +57,15 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
           s = s + Sij * Sij;
         }
       T Sbar = sqrt((T)2.0 * s);
-      // filter width: per-level table from the host libm when dxPr*dyPr is
-      // uniform (the cube root of a static geometric factor)
-      T width = width_k ? (T)width_k[k] : (T)(filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0));
+      // filter width: table from the host libm (the cube root of a static
+      // geometric factor per column class and level)
+      T width;
+      if (width_t.tab) {
+        const int cls = width_t.idx ? width_t.idx[(i - 1) + (long)g.nx * (j - 1)] : 0;
+        width = (T)width_t.tab[cls * width_t.ld + k];
+      } else {
+        width = (T)(filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0));
+      }
       T wc = width * c_smag;
       v = Sbar * (wc * wc);
       if (nu > 0) v = v + nu;
@@ -97,9 +103,9 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
 
 template <typename T>
 void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
-                double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial, int, hipStream_t st) {
+                double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, int, hipStream_t st) {
   dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
-  hipLaunchKernelGGL(k_sgs<T>, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width_k, sgs_model,
+  hipLaunchKernelGGL(k_sgs<T>, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width, sgs_model,
                      cfl_partial);
 }
 
@@ -696,7 +702,7 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
 // explicit instantiations: FP64 (reference arithmetic) and FP32 (precision form)
 #define ELMM_INST_MOMENTUM(T)                                                                                       \
   template void launch_sgs<T>(const Geo &, const T *, const T *, const T *, T *, double, double, double,           \
-                              const double *, int, double *, int, hipStream_t);                                    \
+                              SgsWidth, int, double *, int, hipStream_t);                                          \
   template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, WMPoints, double, int,           \
                                     hipStream_t);                                                                  \
   template void launch_momentum<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *, T *,   \

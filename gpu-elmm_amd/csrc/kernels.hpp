@@ -43,10 +43,19 @@ struct IBDev {
   const double *coef;
 };
 
+// Smagorinsky filter width filter_ratio (dxPr dyPr dzPr)^(1/3) tabulated on
+// the host: tab[idx[(i-1) + nx (j-1)] * ld + k] (idx null: one column class;
+// tab null: evaluated on the device).
+struct SgsWidth {
+  const double *tab;
+  const uint8_t *idx;
+  int ld;
+};
+
 // momentum (k_momentum.hip); T = field type (double or float)
 template <typename T>
 void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
-                double filter_ratio, const double *width_k, int sgs_model, double *cfl_partial, int cfl_blocks,
+                double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, int cfl_blocks,
                 hipStream_t st);
 template <typename T>
 void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,

@@ -63,7 +63,8 @@ def build(verbose=False, jobs=None):
                 sys.stdout.write(o)
     objs = [c[-1] for c in jobs_list]
     tmp = lib_path() + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    # --no-undefined: a missing definition fails the build instead of the load
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,--no-undefined", "-o", tmp] + objs)
     os.replace(tmp, lib_path())
     build_fortran_host()
     shutil.rmtree(OBJDIR, ignore_errors=True)

@@ -1,17 +1,19 @@
 """HBM traffic per launch from rocprofv3 counter runs.
 
-    python3 -m elmm_amd.traffic GROUP KERNEL_MATCH WORKLOAD FETCH.csv WRITE.csv \\
-        COPY_FETCH.csv COPY_WRITE.csv COPY_BYTES OUT_DIR
+    python3 -m elmm_amd.traffic GROUP KERNEL_MATCH WORKLOAD FETCH WRITE \\
+        COPY_FETCH COPY_WRITE COPY_BYTES OUT_DIR [COMMANDS]
 
-FETCH.csv / WRITE.csv are the counter_collection files of two separate
-`rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of the workload;
-COPY_*.csv the same counters for `bench.py --copy-probe COPY_BYTES` (a plain
-copy kernel that reads and writes COPY_BYTES per dispatch).  The counters are
+FETCH / WRITE are the outputs of two separate `rocprofv3 --pmc FETCH_SIZE`
+and `--pmc WRITE_SIZE` runs of the workload (the `*_results.db` database of
+rocprofv3's default output format, or a `*_counter_collection.csv`); COPY_*
+the same counters for `bench.py --copy-probe COPY_BYTES` (a plain copy kernel
+that reads and writes COPY_BYTES per dispatch).  The counters are
 in KiB per dispatch; each is scaled by the ratio the copy kernel shows
 between its known bytes and its counter.  Writes
 OUT_DIR/traffic_<group>_<workload>.json (read by bench.py for the kernel
 group of that name) and a .md summary beside it; KERNEL_MATCH selects the
-dispatches by kernel-name substring.
+dispatches by kernel-name substring; COMMANDS (text) is recorded in the
+summary as the commands that produced the counter outputs.
 """
 import csv
 import json
@@ -19,24 +21,39 @@ import os
 import sys
 
 
+def _rows(path):
+    """(kernel name, counter name, dispatch id, value) of a rocprofv3 counter
+    output: the rocpd SQLite database or the CSV format."""
+    if path.endswith(".db"):
+        import sqlite3
+        db = sqlite3.connect(path)
+        try:
+            yield from db.execute("select kernel_name, counter_name, dispatch_id, value from counters_collection")
+        finally:
+            db.close()
+        return
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            yield (row.get("Kernel_Name", ""), row.get("Counter_Name"),
+                   row.get("Dispatch_Id") or row.get("Correlation_Id"), float(row["Counter_Value"]))
+
+
 def per_dispatch(path, kernel_substr, counter):
     """Mean counter value (summed over the counter's instances) per dispatch
     of the kernels whose name contains kernel_substr."""
     by_dispatch = {}
-    with open(path) as f:
-        for row in csv.DictReader(f):
-            name = row.get("Kernel_Name", "")
-            if kernel_substr not in name or row.get("Counter_Name") != counter:
-                continue
-            d = row.get("Dispatch_Id") or row.get("Correlation_Id")
-            by_dispatch[d] = by_dispatch.get(d, 0.0) + float(row["Counter_Value"])
+    for name, cname, d, value in _rows(path):
+        if kernel_substr not in name or cname != counter:
+            continue
+        by_dispatch[d] = by_dispatch.get(d, 0.0) + float(value)
     if not by_dispatch:
         raise SystemExit(f"no {counter} rows for kernels matching {kernel_substr!r} in {path}")
     return sum(by_dispatch.values()) / len(by_dispatch), len(by_dispatch)
 
 
 def main(argv):
-    group, kernel, workload, fetch, write, cfetch, cwrite, copy_bytes, out = argv
+    group, kernel, workload, fetch, write, cfetch, cwrite, copy_bytes, out = argv[:9]
+    commands = argv[9] if len(argv) > 9 else ""
     copy_bytes = int(copy_bytes)
     f, nf = per_dispatch(fetch, kernel, "FETCH_SIZE")
     w, nw = per_dispatch(write, kernel, "WRITE_SIZE")
@@ -59,6 +76,8 @@ def main(argv):
         fh.write(f"# HBM traffic of {kernel}, {workload}\n\n")
         fh.write("rocprofv3 `--pmc FETCH_SIZE` and `--pmc WRITE_SIZE` in separate runs (KiB per dispatch), scaled by "
                  f"a plain copy kernel of {copy_bytes} bytes read + written per dispatch.\n\n")
+        if commands:
+            fh.write(f"Commands (from the repository root on an MI355X):\n\n```\n{commands}\n```\n\n")
         fh.write("| quantity | value |\n|---|---:|\n")
         for k in ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "copy_fetch_kib", "copy_write_kib",
                   "scale_fetch", "scale_write", "fetch_bytes", "write_bytes", "hbm_bytes_per_launch"):
