@@ -172,7 +172,10 @@ def main():
         out["ensemble"] = serving.run_ensemble(n_cases=args.ensemble, steps=args.ensemble_steps, precision=pr)
     if args.serve > 0:
         sv = serving.run_serving(n_requests=args.serve, steps_scale=args.serve_steps_scale, precision=pr)
+        sv["capacity"] = serving.run_capacity(n_requests=args.serve, steps_scale=args.serve_steps_scale,
+                                              precision=pr)
         sv["serial_fp64"] = serving.run_serial(n_requests=args.serve, steps_scale=args.serve_steps_scale)
+        sv["capacity_vs_serial_fp64"] = round(sv["capacity"]["value"] / sv["serial_fp64"]["value"], 3)
         out["serving"] = sv
     print(json.dumps(out))
 

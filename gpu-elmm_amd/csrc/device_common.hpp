@@ -22,6 +22,10 @@ struct Geo {
   long base;          // element offset of (0,0,0)
   long ntot;          // elements per field allocation
   double dxmin, dymin, dzmin;
+  // RN(1/dxmin) etc. in both field types, or 0 where the reciprocal is not
+  // close enough for div_rn (then the kernels divide)
+  double rdx, rdy, rdz;
+  float rdxf, rdyf, rdzf;
   int variable_z;
   int bt[6];          // velocity boundary types
   int sbt[6];         // scalar boundary types
@@ -34,6 +38,42 @@ struct Geo {
 __host__ __device__ inline long gix(const Geo &g, int i, int j, int k) {
   return g.base + (long)i + (long)j * g.sy + (long)k * g.sz;
 }
+
+// x / d correctly rounded for a divisor d whose reciprocal r = RN(1/d) is
+// known: q = RN(x r) is within one ulp of x/d when |r d - 1| <= 2^-(p+1)
+// (checked on the host, see div_recip), the remainder x - q d is then exact
+// in one fma, and RN(q + rem r) is the correctly rounded quotient
+// (P. Markstein, IBM J. Res. Dev. 34, 1990).  An exact q (rem = 0) is kept as
+// it is, which also preserves the sign of a zero quotient.  r = 0 divides.
+template <typename T>
+__device__ inline T div_rn(T x, T d, T r) {
+  if (r == (T)0) return x / d;
+  const T q = x * r;
+  const T rem = fma(-q, d, x);
+  return rem == (T)0 ? q : fma(rem, r, q);
+}
+
+// RN(1/d) if it satisfies div_rn's condition, else 0 (host).
+inline double div_recip(double d) {
+  const double r = 1.0 / d;
+  const double e = __builtin_fma(r, d, -1.0);  // exact
+  return __builtin_fabs(e) <= 0x1p-54 ? r : 0.0;
+}
+inline float div_recipf(float d) {
+  const float r = 1.0f / d;
+  const double e = (double)r * (double)d - 1.0;  // exact in FP64
+  return __builtin_fabs(e) <= 0x1p-25 ? r : 0.0f;
+}
+
+template <typename T>
+__device__ inline T recip3() {
+  return sizeof(T) == 4 ? (T)(1.0f / 3.0f) : (T)(1.0 / 3.0);
+}
+
+// Raises a kernel's dynamic-LDS limit above 64 KiB once per kernel and size
+// (host; later launches, including those recorded into a graph, make no
+// attribute call).
+bool lds_limit_raise(const void *fn, size_t bytes);
 
 // Per-cell code bytes used by the bulk kernels.
 enum : uint8_t {

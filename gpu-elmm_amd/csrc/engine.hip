@@ -121,6 +121,9 @@ struct EngineIface {
   std::vector<hipEvent_t> event_pool;
   double tms[ELMM_T_NGROUPS] = {};
   int64_t tlaunch[ELMM_T_NGROUPS] = {};
+  // two-step graphs per field-pointer parity (captured on first use)
+  bool use_graphs = true;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
 
   virtual ~EngineIface() = default;
   virtual void create(const elmm_config &c, const elmm_lists &L) = 0;
@@ -129,6 +132,13 @@ struct EngineIface {
   virtual void step(int nsteps) = 0;
   virtual void enqueue_steps(int nsteps) = 0;
   virtual double divergence() = 0;
+  void release_graphs() {
+    for (auto &e : graph_exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  }
 
   hipEvent_t get_event() {
     if (!event_pool.empty()) {
@@ -252,6 +262,8 @@ struct Engine : EngineIface {
   void init() override;
   void step(int nsteps) override;
   void enqueue_steps(int nsteps) override;
+  void first_call();
+  int replay_steps(int nsteps);
   double divergence() override;
   double max_div(T *a, T *b, T *c, bool use_q);
   double reduce_max(DevBuf &buf, int n);
@@ -273,12 +285,20 @@ void Engine<T>::setup_geo() {
   g.dxmin = grid.dxmin;
   g.dymin = grid.dymin;
   g.dzmin = grid.dzmin;
+  g.rdx = div_recip(g.dxmin);
+  g.rdy = div_recip(g.dymin);
+  g.rdz = div_recip(g.dzmin);
+  g.rdxf = div_recipf((float)g.dxmin);
+  g.rdyf = div_recipf((float)g.dymin);
+  g.rdzf = div_recipf((float)g.dzmin);
   g.variable_z = cfg.stretched_z != 0;
   {
     const char *m = std::getenv("ELMM_MOMENTUM");
     fused_momentum = momentum_zm_supported(g) && !(m && std::string(m) == "percomp");
     const char *sk = std::getenv("ELMM_SCALAR");
     zmarch_scalar = !g.variable_z && !(sk && std::string(sk) == "cellwise");
+    const char *gv = std::getenv("ELMM_GRAPH");
+    use_graphs = !(gv && std::string(gv) == "0");
     const char *zc = std::getenv("ELMM_ZCHUNK");
     if (zc && std::atoi(zc) > 0) zchunk = std::atoi(zc);
   }
@@ -968,18 +988,52 @@ void Engine<T>::init() {
   initialized = true;
 }
 
+// first call of TMarchRK3 (src/tsteps.f90:54-77)
+template <typename T>
+void Engine<T>::first_call() {
+  if (first_step_done) return;
+  bound_uvw(u, v, w);
+  if (has_ib()) ib_momentum(u, v, w);
+  first_step_done = true;
+}
+
+// Enqueues the largest even number of plain steps (<= nsteps) as replays of a
+// captured graph of two steps and returns that number.  Every stage swaps
+// the U/U2 (and scalar) pointers, so two steps restore them: one graph per
+// pointer parity covers every call.  The captured work is kernels only (no
+// memset or copy nodes); timed runs and ELMM_GRAPH=0 enqueue directly.
+template <typename T>
+int Engine<T>::replay_steps(int nsteps) {
+  if (!use_graphs || timers_on || nsteps < 2 || !first_step_done) return 0;
+  const int par = u == U.as<T>() ? 0 : 1;
+  if (!graph_exec[par]) {
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < 2; r++)
+      for (int st = 0; st < 3; st++) stage(st, false);
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamEndCapture(stream, &gr));
+    const hipError_t e = hipGraphInstantiate(&graph_exec[par], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPCHK(e);
+  }
+  const int n2 = nsteps / 2;
+  for (int r = 0; r < n2; r++) HIPCHK(hipGraphLaunch(graph_exec[par], stream));
+  return 2 * n2;
+}
+
 template <typename T>
 void Engine<T>::step(int nsteps) {
-  for (int s = 0; s < nsteps; s++) {
-    if (!first_step_done) {
-      // first call of TMarchRK3 (src/tsteps.f90:54-77)
-      bound_uvw(u, v, w);
-      if (has_ib()) ib_momentum(u, v, w);
-      first_step_done = true;
-    }
-    for (int st = 0; st < 3; st++) stage(st, s == nsteps - 1 && st == 2);
-    if (cfg.cfl_check && s == nsteps - 1) last_cfl = reduce_max(cfl_partial, cfl_blocks) * cfg.dt;
+  int s = 0;
+  if (nsteps > 0 && !first_step_done) {
+    first_call();
+    for (int st = 0; st < 3; st++) stage(st, nsteps == 1 && st == 2);
+    s = 1;
   }
+  // every step but the last (which carries the diagnostics) may replay
+  if (nsteps - s > 1) s += replay_steps(nsteps - s - 1);
+  for (; s < nsteps; s++)
+    for (int st = 0; st < 3; st++) stage(st, s == nsteps - 1 && st == 2);
+  if (cfg.cfl_check && nsteps > 0) last_cfl = reduce_max(cfl_partial, cfl_blocks) * cfg.dt;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
   collect_timers();
@@ -989,14 +1043,15 @@ void Engine<T>::step(int nsteps) {
 template <typename T>
 void Engine<T>::enqueue_steps(int nsteps) {
   want_diag = false;
-  for (int s = 0; s < nsteps; s++) {
-    if (!first_step_done) {
-      bound_uvw(u, v, w);
-      if (has_ib()) ib_momentum(u, v, w);
-      first_step_done = true;
-    }
+  int s = 0;
+  if (nsteps > 0 && !first_step_done) {
+    first_call();
     for (int st = 0; st < 3; st++) stage(st, false);
+    s = 1;
   }
+  s += replay_steps(nsteps - s);
+  for (; s < nsteps; s++)
+    for (int st = 0; st < 3; st++) stage(st, false);
   HIPCHK(hipGetLastError());
 }
 
@@ -1036,6 +1091,7 @@ static int guarded(F &&f) {
 static void release_engine(elmm::EngineIface *e) {
   if (!e || !e->stream) return;
   (void)hipStreamSynchronize(e->stream);
+  e->release_graphs();
   for (auto &t : e->pending) {
     (void)hipEventDestroy(std::get<1>(t));
     (void)hipEventDestroy(std::get<2>(t));

@@ -277,6 +277,8 @@ __device__ inline double cd4(const Geo &g, const T *__restrict__ U, const T *__r
   const T *F = C == 0 ? U : C == 1 ? V : W;
   const long st[3] = {1, sy, sz};
   const T inv[3] = {(T)g.dxmin, (T)g.dymin, (T)g.dzmin};
+  const T rinv[3] = {sizeof(T) == 4 ? (T)g.rdxf : (T)g.rdx, sizeof(T) == 4 ? (T)g.rdyf : (T)g.rdy,
+                     sizeof(T) == 4 ? (T)g.rdzf : (T)g.rdz};
 #pragma unroll
   for (int dd = 0; dd < 3; dd++) {
     const long s = st[dd];
@@ -306,8 +308,8 @@ __device__ inline double cd4(const Geo &g, const T *__restrict__ U, const T *__r
 #undef AI
       }
       d = 9 * (uv1b - uv1a);
-      d = d - (uv3b - uv3a) / 3;
-      acc = acc + d / inv[dd];
+      d = d - div_rn<T>(uv3b - uv3a, (T)3, recip3<T>());
+      acc = acc + div_rn<T>(d, inv[dd], rinv[dd]);
     } else {
       // 2nd order near boundaries (coef2ord = 64)
       if (dd == C) {
@@ -319,7 +321,7 @@ __device__ inline double cd4(const Geo &g, const T *__restrict__ U, const T *__r
         // W/y term lists the last pair as (V(j-1)+V(j-1,k+1)): a+b = b+a.
         d = 64 * ((F[p + s] + F[p]) * (A[p + c] + A[p]) - (F[p] + F[p - s]) * (A[p + c - s] + A[p - s]));
       }
-      acc = acc + d / inv[dd];
+      acc = acc + div_rn<T>(d, inv[dd], rinv[dd]);
     }
   }
   return acc * (T)(-1.0 / 256);

@@ -56,7 +56,7 @@ struct MomZArgs {
 template <typename T>
 __device__ inline T d4(T uv1b, T uv1a, T uv3b, T uv3a) {
   T d = 9 * (uv1b - uv1a);
-  return d - (uv3b - uv3a) / 3;
+  return d - div_rn<T>(uv3b - uv3a, (T)3, recip3<T>());
 }
 
 template <typename T, typename FF>
@@ -186,6 +186,8 @@ __global__ void __launch_bounds__(TX *TY)
   const T rx = (T)1.0 / ((T)g.dxmin * (T)g.dxmin), ry = (T)1.0 / ((T)g.dymin * (T)g.dymin),
           rz = (T)1.0 / ((T)g.dzmin * (T)g.dzmin);
   const T ix = (T)g.dxmin, iy = (T)g.dymin, iz = (T)g.dzmin;
+  const T rix = sizeof(T) == 4 ? (T)g.rdxf : (T)g.rdx, riy = sizeof(T) == 4 ? (T)g.rdyf : (T)g.rdy,
+          riz = sizeof(T) == 4 ? (T)g.rdzf : (T)g.rdz;
   const T q4 = (T)0.25, m256 = (T)(-1.0 / 256);
   const bool in_xy = i <= g.nx && j <= g.ny;
 
@@ -229,18 +231,18 @@ __global__ void __launch_bounds__(TX *TY)
             T d;
             if (cls == 0) {
               d = self4<T>(Fx);
-              acc = acc + d / ix;
+              acc = acc + div_rn<T>(d, ix, rix);
               d = cross4<T>(Fy, [&](int o) { return m4<T>(at(v0, -1, o), at(v0, 0, o), at(v0, 1, o), at(v0, 2, o)); });
-              acc = acc + d / iy;
+              acc = acc + div_rn<T>(d, iy, riy);
               d = cross4<T>(Fz, [&](int o) { return wx[2 + o]; });
-              acc = acc + d / iz;
+              acc = acc + div_rn<T>(d, iz, riz);
             } else {
               d = self2<T>(Fx);
-              acc = acc + d / ix;
+              acc = acc + div_rn<T>(d, ix, rix);
               d = cross2<T>(Fy, at(v0, 1, 0) + at(v0, 0, 0), at(v0, 1, -1) + at(v0, 0, -1));
-              acc = acc + d / iy;
+              acc = acc + div_rn<T>(d, iy, riy);
               d = cross2<T>(Fz, at(w0, 1, 0) + at(w0, 0, 0), at(wm, 1, 0) + at(wm, 0, 0));
-              acc = acc + d / iz;
+              acc = acc + div_rn<T>(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -267,18 +269,18 @@ __global__ void __launch_bounds__(TX *TY)
             T d;
             if (cls == 0) {
               d = cross4<T>(Fx, [&](int o) { return m4<T>(at(u0, o, -1), at(u0, o, 0), at(u0, o, 1), at(u0, o, 2)); });
-              acc = acc + d / ix;
+              acc = acc + div_rn<T>(d, ix, rix);
               d = self4<T>(Fy);
-              acc = acc + d / iy;
+              acc = acc + div_rn<T>(d, iy, riy);
               d = cross4<T>(Fz, [&](int o) { return wy[2 + o]; });
-              acc = acc + d / iz;
+              acc = acc + div_rn<T>(d, iz, riz);
             } else {
               d = cross2<T>(Fx, at(u0, 0, 1) + at(u0, 0, 0), at(u0, -1, 1) + at(u0, -1, 0));
-              acc = acc + d / ix;
+              acc = acc + div_rn<T>(d, ix, rix);
               d = self2<T>(Fy);
-              acc = acc + d / iy;
+              acc = acc + div_rn<T>(d, iy, riy);
               d = cross2<T>(Fz, at(w0, 0, 1) + at(w0, 0, 0), at(wm, 0, 1) + at(wm, 0, 0));
-              acc = acc + d / iz;
+              acc = acc + div_rn<T>(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -305,18 +307,18 @@ __global__ void __launch_bounds__(TX *TY)
             T d;
             if (cls == 0) {
               d = cross4<T>(Fx, [&](int o) { return m4<T>(at(um, o, 0), at(u0, o, 0), at(u1, o, 0), at(u2s, o, 0)); });
-              acc = acc + d / ix;
+              acc = acc + div_rn<T>(d, ix, rix);
               d = cross4<T>(Fy, [&](int o) { return m4<T>(at(vm, 0, o), at(v0, 0, o), at(v1, 0, o), at(v2s, 0, o)); });
-              acc = acc + d / iy;
+              acc = acc + div_rn<T>(d, iy, riy);
               d = self4<T>(Fz);
-              acc = acc + d / iz;
+              acc = acc + div_rn<T>(d, iz, riz);
             } else {
               d = cross2<T>(Fx, at(u1, 0, 0) + at(u0, 0, 0), at(u1, -1, 0) + at(u0, -1, 0));
-              acc = acc + d / ix;
+              acc = acc + div_rn<T>(d, ix, rix);
               d = cross2<T>(Fy, at(v1, 0, 0) + at(v0, 0, 0), at(v1, 0, -1) + at(v0, 0, -1));
-              acc = acc + d / iy;
+              acc = acc + div_rn<T>(d, iy, riy);
               d = self2<T>(Fz);
-              acc = acc + d / iz;
+              acc = acc + div_rn<T>(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -376,23 +378,21 @@ namespace {
 template <typename T, int TX, int TY>
 void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
   const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
-  if (lds > 65536)
-    (void)hipFuncSetAttribute((const void *)k_momentum_zm<T, TX, TY>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+  if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY>, lds);
   dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
   hipLaunchKernelGGL((k_momentum_zm<T, TX, TY>), gr, b, lds, st, g, a, sc, kchunk);
 }
 }  // namespace
 
-// Tile shape: 32x8 by default; ELMM_MOM_TILE=64x4, 32x4, 64x8 or 32x16 selects
-// another instantiation (the last two only for 4-byte fields: their FP64 LDS
-// rings exceed the 160 KiB of a workgroup).
+// Tile shape: 32x8 by default; ELMM_MOM_TILE=64x4, 32x4, 64x8, 32x16 or 64x16
+// selects another instantiation (the last three only for 4-byte fields: their
+// FP64 LDS rings exceed the 160 KiB of a workgroup).
 static int mom_tile_choice() {
   static int choice = -1;
   if (choice < 0) {
     const char *e = std::getenv("ELMM_MOM_TILE");
     const std::string v = e ? e : "";
-    choice = v == "64x4" ? 1 : v == "32x4" ? 2 : v == "64x8" ? 3 : v == "32x16" ? 4 : 0;
+    choice = v == "64x4" ? 1 : v == "32x4" ? 2 : v == "64x8" ? 3 : v == "32x16" ? 4 : v == "64x16" ? 5 : 0;
   }
   return choice;
 }
@@ -425,6 +425,13 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
     case 4:
       if (sizeof(T) == 4) {
         launch_tile<T, 32, 16>(g, a, sc, kchunk, st);
+        break;
+      }
+      launch_tile<T, 32, 8>(g, a, sc, kchunk, st);
+      break;
+    case 5:
+      if (sizeof(T) == 4) {
+        launch_tile<T, 64, 16>(g, a, sc, kchunk, st);
         break;
       }
       launch_tile<T, 32, 8>(g, a, sc, kchunk, st);

@@ -1,10 +1,14 @@
 // k_probe.hip - plain device copy of a known size: the reference point of
 // the HBM traffic counters (FETCH_SIZE/WRITE_SIZE per dispatch) and of the
-// achievable copy bandwidth reported beside the roofline.
+// achievable copy bandwidth reported beside the roofline; and the launch
+// attribute helper shared by the kernels with more than 64 KiB of LDS.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <string>
 
+#include "device_common.hpp"
 #include "elmm_gpu.h"
 #include "errors.hpp"
 
@@ -17,6 +21,19 @@ __global__ void __launch_bounds__(256) k_copy_probe(const double4 *__restrict__ 
 }
 
 }  // namespace
+
+namespace elmm {
+bool lds_limit_raise(const void *fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void *, size_t> done;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = done.find(fn);
+  if (it != done.end() && it->second >= bytes) return true;
+  const bool ok = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
+  if (ok) done[fn] = bytes;
+  return ok;
+}
+}  // namespace elmm
 
 extern "C" int elmm_copy_probe(int64_t nbytes, int32_t reps, double *gbps) {
   if (nbytes < 32 || reps < 1 || !gbps) return elmm::fail(ELMM_E_ARG, "elmm_copy_probe: bad arguments");

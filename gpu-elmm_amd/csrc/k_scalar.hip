@@ -22,7 +22,7 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 // Real = field and arithmetic type (double: FP64 form, float: precision form)
 template <typename Real>
 __device__ inline Real flux_limiter(Real r) {
-  Real a = (1 + 2 * r) / 3;
+  Real a = div_rn<Real>(1 + 2 * r, (Real)3, recip3<Real>());
   Real b = (Real)2.0 < a ? (Real)2.0 : a;
   Real c = 2 * r < b ? 2 * r : b;
   return (Real)0.0 > c ? (Real)0.0 : c;
@@ -139,8 +139,11 @@ void launch_scalar(const Geo &g, const Real *U, const Real *V, const Real *W, co
 // tile of columns; S at level k (halo 2) and TDiff at level k (halo 1) are
 // LDS planes, double-buffered with the next level fetched into registers
 // while the current one is computed; the column values S(k-2..k+2),
-// TDiff(k-1..k+1), W(k-1..k) and the z masks live in registers.  Per cell the
-// operations are those of scalar_cell + scalar_finish in the same order.
+// TDiff(k-1..k+1), W(k-1..k) and the z masks live in registers.  Every face
+// flux is evaluated once: each thread computes the upper x and y faces of its
+// cell (edge threads also the tile's lower faces) into LDS, and the upper z
+// face of level k is the lower z face of level k+1.  Per cell the operations
+// are those of scalar_cell + scalar_finish in the same order.
 constexpr int TXS = 64, TYS = 4, NTS = TXS * TYS;
 constexpr int SPW = TXS + 4, SPH = TYS + 4, SPS = SPW * SPH;   // S plane, halo 2
 constexpr int TPW = TXS + 2, TPH = TYS + 2, TPS = TPW * TPH;   // TDiff plane, halo 1
@@ -154,6 +157,8 @@ __global__ void __launch_bounds__(NTS)
                 const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc, int kchunk) {
   __shared__ Real sS[2][SPS];
   __shared__ Real sT[2][TPS];
+  __shared__ Real fX[TYS][TXS + 1];  // x faces i0-1 .. i0+TXS-1 of each row
+  __shared__ Real fY[TYS + 1][TXS];  // y faces j0-1 .. j0+TYS-1 of each column
   const int tid = threadIdx.x, tx = tid % TXS, ty = tid / TXS;
   const int i0 = 1 + blockIdx.x * TXS, j0 = 1 + blockIdx.y * TYS;
   const int i = i0 + tx, j = j0 + ty;
@@ -199,6 +204,7 @@ __global__ void __launch_bounds__(NTS)
   const Real Ax = 1 / (Real)g.dxmin, Ay = 1 / (Real)g.dymin, Az = 1 / (Real)g.dzmin;
   const Real Dx = 1 / (2 * ((Real)g.dxmin * (Real)g.dxmin)), Dy = 1 / (2 * ((Real)g.dymin * (Real)g.dymin)),
              Dz = 1 / (2 * ((Real)g.dzmin * (Real)g.dzmin));
+  Real fz_lo = 0;
 
   for (int k = kbeg; k <= kend; k++) {
     const bool more = k < kend;
@@ -217,25 +223,39 @@ __global__ void __launch_bounds__(NTS)
       nw = W[pc + (long)(k + 1) * sz];
       nm = mz[pc + (long)(k + 1) * sz];
     }
+    const long p = gix(g, min(i, g.nx), min(j, g.ny), k);
+    const Real *P = sS[b];
+    auto Sx = [&](int o) { return P[ls + o]; };
+    auto Sy = [&](int o) { return P[ls + o * SPW]; };
+    // face fluxes of this level: upper x and y faces of the cell, the tile's
+    // lower faces by its first column / row (values of out-of-domain threads
+    // are never read)
+    const uint8_t mxu = mx[p], myu = my[p];
     if (in_xy) {
-      const long p = gix(g, i, j, k);
+      fX[ty][tx + 1] = mxu ? kappa_flux_v<Real>(U[p], Sx(-1), Sx(0), Sx(1), Sx(2)) : (Real)0;
+      fY[ty + 1][tx] = myu ? kappa_flux_v<Real>(V[p], Sy(-1), Sy(0), Sy(1), Sy(2)) : (Real)0;
+      if (tx == 0 && mx[p - 1]) fX[ty][0] = kappa_flux_v<Real>(U[p - 1], Sx(-2), Sx(-1), Sx(0), Sx(1));
+      if (ty == 0 && my[p - sy]) fY[0][tx] = kappa_flux_v<Real>(V[p - sy], Sy(-2), Sy(-1), Sy(0), Sy(1));
+    }
+    // z faces: the lower one is the previous level's upper one
+    const bool mlo = cm[0] != 0, mhi = cm[1] != 0;
+    if (k == kbeg && mlo) fz_lo = kappa_flux_v<Real>(cw[0], cs[0], cs[1], cs[2], cs[3]);
+    const Real fz_hi = mhi ? kappa_flux_v<Real>(cw[1], cs[1], cs[2], cs[3], cs[4]) : (Real)0;
+    __syncthreads();
+    if (in_xy) {
       const uint8_t cd = code[p];
       if (!(cd & CODE_CORRECTED)) {
-        const Real *P = sS[b];
-        auto Sx = [&](int o) { return P[ls + o]; };
-        auto Sy = [&](int o) { return P[ls + o * SPW]; };
         Real a = 0;
-        if (mx[p - 1]) a = a + Ax * kappa_flux_v<Real>(U[p - 1], Sx(-2), Sx(-1), Sx(0), Sx(1));
-        if (mx[p]) a = a - Ax * kappa_flux_v<Real>(U[p], Sx(-1), Sx(0), Sx(1), Sx(2));
-        if (my[p - sy]) a = a + Ay * kappa_flux_v<Real>(V[p - sy], Sy(-2), Sy(-1), Sy(0), Sy(1));
-        if (my[p]) a = a - Ay * kappa_flux_v<Real>(V[p], Sy(-1), Sy(0), Sy(1), Sy(2));
-        const bool mlo = cm[0] != 0, mhi = cm[1] != 0;
+        if (mx[p - 1]) a = a + Ax * fX[ty][tx];
+        if (mxu) a = a - Ax * fX[ty][tx + 1];
+        if (my[p - sy]) a = a + Ay * fY[ty][tx];
+        if (myu) a = a - Ay * fY[ty + 1][tx];
         if ((k & 1) == 0) {
-          if (mhi) a = a - Az * kappa_flux_v<Real>(cw[1], cs[1], cs[2], cs[3], cs[4]);
-          if (mlo) a = a + Az * kappa_flux_v<Real>(cw[0], cs[0], cs[1], cs[2], cs[3]);
+          if (mhi) a = a - Az * fz_hi;
+          if (mlo) a = a + Az * fz_lo;
         } else {
-          if (mlo) a = a + Az * kappa_flux_v<Real>(cw[0], cs[0], cs[1], cs[2], cs[3]);
-          if (mhi) a = a - Az * kappa_flux_v<Real>(cw[1], cs[1], cs[2], cs[3], cs[4]);
+          if (mlo) a = a + Az * fz_lo;
+          if (mhi) a = a - Az * fz_hi;
         }
         if (cd & CODE_SOLID) a = 0;
         const Real *Q = sT[b];
@@ -270,6 +290,7 @@ __global__ void __launch_bounds__(NTS)
       cm[0] = cm[1];
       cm[1] = nm;
     }
+    fz_lo = fz_hi;
     __syncthreads();
   }
 }
