@@ -8,8 +8,12 @@
 !   elmm_host case.nml
 ! Namelist groups (a subset of the reference's grid.conf, main.conf,
 ! boundconds.conf, les.conf and time_stepping.conf inputs):
-!   &grid        nx, ny, nz, dx, dy, dz, x0, y0, z0
-!   &physics     nu, nu_scalar, c_smag, z0_bottom, top_bc, nscalars, precision
+!   &grid        nx, ny, nz, dx, dy, dz, x0, y0, z0, zgrid_file
+!   &physics     nu, nu_scalar, c_smag, z0_bottom, z0_top, top_bc, nscalars, precision,
+!                advection_method, pr_gradient_x
+! zgrid_file (optional): the z levels zW(0..nz) one per line, as the
+! reference's zgrid.txt (src/initial.f90:254-279); it sets nz and a stretched
+! z grid (which needs advection_method = 2).
 !   &time_stepping dt, nsteps, report_every
 !   &obstacles   n_cuboids, cuboids(7, n) = xmin xmax ymin ymax zmin zmax z0
 !   &point_sources n_sources, sources(5, n) = x y z flux scalar(1-based)
@@ -31,9 +35,13 @@ program elmm_host
   integer :: nsteps = 10, report_every = 5, n_cuboids = 0, n_sources = 0
   integer, parameter :: maxobj = 256
   real(c_double) :: cuboids(7, maxobj) = 0, sources(5, maxobj) = 0
-  character(len=512) :: init_file = "", out_file = ""
-  namelist /grid/ nx, ny, nz, dx, dy, dz, x0, y0, z0
-  namelist /physics/ nu, nu_scalar, c_smag, z0_bottom, top_bc, nscalars, precision
+  character(len=512) :: init_file = "", out_file = "", zgrid_file = ""
+  integer :: advection_method = 4
+  real(c_double) :: z0_top = 0, pr_gradient_x = 0, rtmp
+  real(c_double), allocatable, target :: zwl(:)
+  namelist /grid/ nx, ny, nz, dx, dy, dz, x0, y0, z0, zgrid_file
+  namelist /physics/ nu, nu_scalar, c_smag, z0_bottom, z0_top, top_bc, nscalars, precision, advection_method, &
+                     pr_gradient_x
   namelist /time_stepping/ dt, nsteps, report_every
   namelist /obstacles/ n_cuboids, cuboids
   namelist /point_sources/ n_sources, sources
@@ -68,20 +76,41 @@ program elmm_host
   read(unit, nml=io, iostat=ios)
   close(unit)
 
+  ! z levels from a file (zgridfromfile): nz = number of values - 1
+  cfg%stretched_z = 0
+  if (len_trim(zgrid_file) > 0) then
+    open(newunit=unit, file=trim(zgrid_file), status="old", action="read")
+    i = -1
+    do
+      read(unit, *, iostat=ios) rtmp
+      if (ios /= 0) exit
+      i = i + 1
+    end do
+    rewind(unit)
+    nz = i
+    allocate(zwl(0:nz))
+    do i = 0, nz
+      read(unit, *) zwl(i)
+    end do
+    close(unit)
+    z0 = zwl(0)
+    cfg%stretched_z = 1
+    cfg%zw = c_loc(zwl)
+  end if
+
   ! periodic x/y, log-law no-slip bottom, free-slip (or no-slip) top
   cfg%nx = nx; cfg%ny = ny; cfg%nz = nz
   cfg%dx = dx; cfg%dy = dy; cfg%dz = dz
   cfg%x0 = x0; cfg%y0 = y0; cfg%z0 = z0
-  cfg%stretched_z = 0
   cfg%btype = [ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_NOSLIP, &
                int(top_bc, c_int32_t)]
-  cfg%z0_wall = [0.0_c_double, 0.0_c_double, 0.0_c_double, 0.0_c_double, z0_bottom, 0.0_c_double]
+  cfg%z0_wall = [0.0_c_double, 0.0_c_double, 0.0_c_double, 0.0_c_double, z0_bottom, z0_top]
   cfg%scal_btype = [ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_NEUMANN, &
                     ELMM_BC_NEUMANN]
   cfg%side_scal = 0
   cfg%nu = nu; cfg%nu_scalar = nu_scalar; cfg%c_smag = c_smag; cfg%filter_ratio = 1
-  cfg%dt = dt; cfg%pr_gradient_x = 0; cfg%pr_gradient_y = 0
-  cfg%advection_method = 4; cfg%sgs_model = 1; cfg%wall_model = 1
+  cfg%dt = dt; cfg%pr_gradient_x = pr_gradient_x; cfg%pr_gradient_y = 0
+  cfg%advection_method = int(advection_method, c_int32_t); cfg%sgs_model = 1; cfg%wall_model = 1
   cfg%nscalars = nscalars; cfg%poisson_solver = ELMM_POISSON_FFT; cfg%poisson_tol = 1e-10_c_double
   cfg%cfl_check = 1; cfg%cfl_max = 0
   cfg%precision = precision; cfg%ib_mass_sources = 1; cfg%refine_iters = 0

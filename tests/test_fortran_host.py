@@ -85,3 +85,42 @@ def test_fortran_host_matches_c_abi(require_gpu, tmp_path):
         assert np.array_equal(parts[c].reshape(g, order="F"), s.download(c))
     assert np.array_equal(parts[3].reshape(gp, order="F"), s.download(capi.F_PR)[1:-1, 1:-1, 1:-1])
     assert np.array_equal(parts[4].reshape(g, order="F"), s.download(capi.F_SCALAR))
+
+
+@pytest.mark.gpu
+@needs_host
+def test_fortran_host_stretched_channel(require_gpu, tmp_path):
+    """zgrid file (the reference's zgrid.txt form) -> stretched channel
+    through the host, identical to the C ABI driven from Python."""
+    nx, ny, nz, steps = 32, 16, 16, 3
+    cfg = cases.channel_config(nx, ny, nz)
+    U, V, W, Pr, _ = cases.channel_fields(cfg)
+    zg = tmp_path / "zgrid.txt"
+    zg.write_text("\n".join(f"{z:.17g}" for z in cfg._zw_keep) + "\n")
+    init = tmp_path / "init.bin"
+    with open(init, "wb") as f:
+        for a in (U, V, W):
+            f.write(np.asfortranarray(a).tobytes(order="F"))
+        f.write(np.asfortranarray(Pr[1:-1, 1:-1, 1:-1]).tobytes(order="F"))
+    out = tmp_path / "out.bin"
+    nml = tmp_path / "chan.nml"
+    nml.write_text("\n".join([
+        f"&grid nx={nx}, ny={ny}, dx={cfg.dx:.17g}, dy={cfg.dy:.17g}, zgrid_file='{zg}' /",
+        f"&physics nu={cfg.nu:.17g}, nu_scalar={cfg.nu_scalar:.17g}, nscalars=0, z0_bottom=0, z0_top=0, "
+        f"top_bc={capi.NOSLIP}, advection_method=2, pr_gradient_x=-1.0 /",
+        f"&time_stepping dt={cfg.dt:.17g}, nsteps={steps}, report_every={steps} /",
+        f"&io init_file='{init}', out_file='{out}' /"]) + "\n")
+    r = subprocess.run([HOST, str(nml)], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s = capi.Solver(cfg, capi.Case(cfg))
+    cases.upload_initial(s, cfg, (U, V, W, Pr, []))
+    s.init()
+    s.step(steps)
+    g = (nx + 6, ny + 6, nz + 6)
+    gp = (nx + 4, ny + 4, nz + 4)
+    raw = np.fromfile(out, dtype=np.float64)
+    parts = np.split(raw, np.cumsum([np.prod(g)] * 3)[:3])
+    for c in range(3):
+        assert np.array_equal(parts[c].reshape(g, order="F"), s.download(c))
+    assert np.array_equal(parts[3].reshape(gp, order="F"), s.download(capi.F_PR)[1:-1, 1:-1, 1:-1])
