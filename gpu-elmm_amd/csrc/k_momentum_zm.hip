@@ -93,9 +93,9 @@ __device__ inline T cross2(FF F, T ac, T acm) {
 }
 
 template <typename T>
-__device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr, T dprz_var, bool varz,
+__device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr, T dprz_var, bool varz, T us_prev,
                                    T *__restrict__ Us, T *__restrict__ U2, long p) {
-  T u2 = sc.stage > 0 ? Us[p] * (T)sc.rho * (T)sc.dt : (T)0;
+  T u2 = sc.stage > 0 ? us_prev * (T)sc.rho * (T)sc.dt : (T)0;
   u2 = u2 + w * (T)sc.beta * (T)sc.dt;
   if (C == 0 && sc.prgx != 0) u2 = u2 + (T)sc.A * (T)sc.prgx;
   if (C == 1 && sc.prgy != 0) u2 = u2 + (T)sc.A * (T)sc.prgy;
@@ -113,7 +113,7 @@ __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr
 }
 
 template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY)
+__global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 2) : 1)
     k_momentum_zm(Geo g, MomZArgs<T> a, StageConst sc, int kchunk) {
   constexpr int PW = Tile<TX, TY>::PW, PS = Tile<TX, TY>::PS, NTH = Tile<TX, TY>::NTH;
   constexpr int PLOAD = Tile<TX, TY>::PLOAD;
@@ -190,6 +190,18 @@ __global__ void __launch_bounds__(TX *TY)
           riz = sizeof(T) == 4 ? (T)g.rdzf : (T)g.rdz;
   const T q4 = (T)0.25, m256 = (T)(-1.0 / 256);
   const bool in_xy = i <= g.nx && j <= g.ny;
+  // per-cell operands of the level being computed (Pr and its +x, +y, +z
+  // neighbours, the three codes, the three previous Ustar values), fetched
+  // one level ahead like the planes
+  T cpr = a.Pr[pc + (long)kbeg * sz], cprx = a.Pr[pc + (long)kbeg * sz + 1],
+    cpry = a.Pr[pc + (long)kbeg * sz + g.sy], cprz = a.Pr[pc + (long)(kbeg + 1) * sz];
+  uint8_t ccd[3];
+  T cus[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    ccd[c] = a.code[c][pc + (long)kbeg * sz];
+    cus[c] = sc.stage > 0 ? a.Us[c][pc + (long)kbeg * sz] : (T)0;
+  }
 
   for (int k = kbeg; k <= kend; k++) {
     const bool more = k < kend;
@@ -209,6 +221,20 @@ __global__ void __launch_bounds__(TX *TY)
       ov = a.V[o];
       ow = a.W[o];
     }
+    T npr = 0, nprx = 0, npry = 0, nprz = 0, nus[3] = {0, 0, 0};
+    uint8_t ncd[3] = {0, 0, 0};
+    if (more) {
+      const long q = pc + (long)(k + 1) * sz;
+      nprx = a.Pr[q + 1];
+      npry = a.Pr[q + g.sy];
+      nprz = a.Pr[q + sz];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        ncd[c] = a.code[c][q];
+        if (sc.stage > 0) nus[c] = a.Us[c][q];
+      }
+      npr = cprz;
+    }
 
     if (in_xy) {
       const long p = gix(g, i, j, k);
@@ -216,11 +242,11 @@ __global__ void __launch_bounds__(TX *TY)
       const int v0 = slotV(k), vm = slotV(k - 1), v1 = slotV(k + 1), v2s = slotV(k + 2);
       const int w0 = slotW(k), wm = slotW(k - 1);
       const int n0 = slotN(k), nm = slotN(k - 1), n1 = slotN(k + 1);
-      const T pr0 = a.Pr[p];
+      const T pr0 = cpr;
 
       // ---------------- U
       if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) {
-        const uint8_t cd = a.code[0][p];
+        const uint8_t cd = ccd[0];
         if (!(cd & CODE_CORRECTED)) {
           const int cls = cd & CODE_ADV_MASK;
           T acc = 0;
@@ -253,12 +279,12 @@ __global__ void __launch_bounds__(TX *TY)
           w = w - q4 * (at(n0, 1, 0) + at(n0, 1, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(u0, 0, -1)) * ry;
           w = w + q4 * (at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (cu[4] - cu[3]) * rz;
           w = w - q4 * (at(n0, 1, 0) + at(nm, 1, 0) + at(n0, 0, 0) + at(nm, 0, 0)) * (cu[3] - cu[2]) * rz;
-          finish_cell<T>(sc, 0, w, f0, a.Pr[p + 1] - pr0, (T)0, false, a.Us[0], a.U2[0], p);
+          finish_cell<T>(sc, 0, w, f0, cprx - pr0, (T)0, false, cus[0], a.Us[0], a.U2[0], p);
         }
       }
       // ---------------- V
       if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) {
-        const uint8_t cd = a.code[1][p];
+        const uint8_t cd = ccd[1];
         if (!(cd & CODE_CORRECTED)) {
           const int cls = cd & CODE_ADV_MASK;
           T acc = 0;
@@ -291,12 +317,12 @@ __global__ void __launch_bounds__(TX *TY)
           w = w - at(n0, 0, 0) * (f0 - at(v0, 0, -1)) * ry;
           w = w + q4 * (at(n1, 0, 1) + at(n0, 0, 1) + at(n1, 0, 0) + at(n0, 0, 0)) * (cv[4] - cv[3]) * rz;
           w = w - q4 * (at(n0, 0, 1) + at(nm, 0, 1) + at(n0, 0, 0) + at(nm, 0, 0)) * (cv[3] - cv[2]) * rz;
-          finish_cell<T>(sc, 1, w, f0, a.Pr[p + g.sy] - pr0, (T)0, false, a.Us[1], a.U2[1], p);
+          finish_cell<T>(sc, 1, w, f0, cpry - pr0, (T)0, false, cus[1], a.Us[1], a.U2[1], p);
         }
       }
       // ---------------- W
       if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
-        const uint8_t cd = a.code[2][p];
+        const uint8_t cd = ccd[2];
         if (!(cd & CODE_CORRECTED)) {
           const int cls = cd & CODE_ADV_MASK;
           T acc = 0;
@@ -329,7 +355,7 @@ __global__ void __launch_bounds__(TX *TY)
           w = w - q4 * (at(n1, 0, 0) + at(n1, 0, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(w0, 0, -1)) * ry;
           w = w + at(n1, 0, 0) * (cw[4] - f0) * rz;
           w = w - at(n0, 0, 0) * (f0 - cw[2]) * rz;
-          finish_cell<T>(sc, 2, w, f0, a.Pr[p + sz] - pr0, (T)0, false, a.Us[2], a.U2[2], p);
+          finish_cell<T>(sc, 2, w, f0, cprz - pr0, (T)0, false, cus[2], a.Us[2], a.U2[2], p);
         }
       }
     }
@@ -356,6 +382,15 @@ __global__ void __launch_bounds__(TX *TY)
       cu[6] = ou;
       cv[6] = ov;
       cw[6] = ow;
+      cpr = npr;
+      cprx = nprx;
+      cpry = npry;
+      cprz = nprz;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        ccd[c] = ncd[c];
+        cus[c] = nus[c];
+      }
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         wx[q] = wx[q + 1];

@@ -205,6 +205,28 @@ __global__ void __launch_bounds__(NTS)
   const Real Dx = 1 / (2 * ((Real)g.dxmin * (Real)g.dxmin)), Dy = 1 / (2 * ((Real)g.dymin * (Real)g.dymin)),
              Dz = 1 / (2 * ((Real)g.dzmin * (Real)g.dzmin));
   Real fz_lo = 0;
+  // per-cell operands of the level being computed (face velocities and
+  // masks of the lower and upper x/y faces, the code, the previous RK term),
+  // fetched one level ahead like the planes
+  struct Ops {
+    Real um, u0, vm, v0, sadv;
+    uint8_t mxm, mx0, mym, my0, cd;
+  };
+  auto fetch = [&](long q, Ops &o) {
+    o.um = U[q - 1];
+    o.u0 = U[q];
+    o.vm = V[q - sy];
+    o.v0 = V[q];
+    o.mxm = mx[q - 1];
+    o.mx0 = mx[q];
+    o.mym = my[q - sy];
+    o.my0 = my[q];
+    o.cd = code[q];
+    o.sadv = sc.stage > 0 ? Sadv[q] : (Real)0;
+  };
+  Ops cur, nxt;
+  fetch(pc + (long)kbeg * sz, cur);
+  nxt = cur;
 
   for (int k = kbeg; k <= kend; k++) {
     const bool more = k < kend;
@@ -222,6 +244,7 @@ __global__ void __launch_bounds__(NTS)
       nt = T[pc + (long)(k + 2) * sz];
       nw = W[pc + (long)(k + 1) * sz];
       nm = mz[pc + (long)(k + 1) * sz];
+      fetch(pc + (long)(k + 1) * sz, nxt);
     }
     const long p = gix(g, min(i, g.nx), min(j, g.ny), k);
     const Real *P = sS[b];
@@ -230,12 +253,12 @@ __global__ void __launch_bounds__(NTS)
     // face fluxes of this level: upper x and y faces of the cell, the tile's
     // lower faces by its first column / row (values of out-of-domain threads
     // are never read)
-    const uint8_t mxu = mx[p], myu = my[p];
+    const uint8_t mxu = cur.mx0, myu = cur.my0;
     if (in_xy) {
-      fX[ty][tx + 1] = mxu ? kappa_flux_v<Real>(U[p], Sx(-1), Sx(0), Sx(1), Sx(2)) : (Real)0;
-      fY[ty + 1][tx] = myu ? kappa_flux_v<Real>(V[p], Sy(-1), Sy(0), Sy(1), Sy(2)) : (Real)0;
-      if (tx == 0 && mx[p - 1]) fX[ty][0] = kappa_flux_v<Real>(U[p - 1], Sx(-2), Sx(-1), Sx(0), Sx(1));
-      if (ty == 0 && my[p - sy]) fY[0][tx] = kappa_flux_v<Real>(V[p - sy], Sy(-2), Sy(-1), Sy(0), Sy(1));
+      fX[ty][tx + 1] = mxu ? kappa_flux_v<Real>(cur.u0, Sx(-1), Sx(0), Sx(1), Sx(2)) : (Real)0;
+      fY[ty + 1][tx] = myu ? kappa_flux_v<Real>(cur.v0, Sy(-1), Sy(0), Sy(1), Sy(2)) : (Real)0;
+      if (tx == 0 && cur.mxm) fX[ty][0] = kappa_flux_v<Real>(cur.um, Sx(-2), Sx(-1), Sx(0), Sx(1));
+      if (ty == 0 && cur.mym) fY[0][tx] = kappa_flux_v<Real>(cur.vm, Sy(-2), Sy(-1), Sy(0), Sy(1));
     }
     // z faces: the lower one is the previous level's upper one
     const bool mlo = cm[0] != 0, mhi = cm[1] != 0;
@@ -243,12 +266,12 @@ __global__ void __launch_bounds__(NTS)
     const Real fz_hi = mhi ? kappa_flux_v<Real>(cw[1], cs[1], cs[2], cs[3], cs[4]) : (Real)0;
     __syncthreads();
     if (in_xy) {
-      const uint8_t cd = code[p];
+      const uint8_t cd = cur.cd;
       if (!(cd & CODE_CORRECTED)) {
         Real a = 0;
-        if (mx[p - 1]) a = a + Ax * fX[ty][tx];
+        if (cur.mxm) a = a + Ax * fX[ty][tx];
         if (mxu) a = a - Ax * fX[ty][tx + 1];
-        if (my[p - sy]) a = a + Ay * fY[ty][tx];
+        if (cur.mym) a = a + Ay * fY[ty][tx];
         if (myu) a = a - Ay * fY[ty + 1][tx];
         if ((k & 1) == 0) {
           if (mhi) a = a - Az * fz_hi;
@@ -266,12 +289,13 @@ __global__ void __launch_bounds__(NTS)
         a = a - (t0 + Q[lt - TPW]) * (s0 - Sy(-1)) * Dy;
         a = a + (ct[2] + t0) * (cs[3] - s0) * Dz;
         a = a - (t0 + ct[0]) * (s0 - cs[1]) * Dz;
-        Real s2 = sc.stage > 0 ? Sadv[p] * (Real)sc.rho_dt : (Real)0.0;
+        Real s2 = sc.stage > 0 ? cur.sadv * (Real)sc.rho_dt : (Real)0.0;
         s2 = s2 + a * (Real)sc.beta_dt;
         Sadv[p] = a;
         Snew[p] = s0 + s2;
       }
     }
+    cur = nxt;
     if (more) {
 #pragma unroll
       for (int r = 0; r < SLD; r++)
