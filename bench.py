@@ -148,7 +148,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 stencils, f64 refined pressure" if prim == "fp32r" else "f64",
+        "dtype": "f32 stencils, f64 pressure solve" if prim == "fp32r" else "f64",
         "data": "synthetic (seeded perturbations, float32-representable)",
         "config": {"workload": wl.name, "grid": [cfg.nx, cfg.ny, cfg.nz], "cells": cells, "dt": cfg.dt,
                    "precision": prim, "obstacles": len(cuboids), "scalars": cfg.nscalars},

@@ -217,6 +217,11 @@ struct Engine : EngineIface {
   // pressure
   DevBuf X, twx, twy, lamx, lamy, az64, cz64;
   PoissonPlan<T> pp{};
+  // precision form without refinement sweeps: the spectral solve runs in
+  // FP64 (complex double array) on the FP32 fields' FP64 divergence
+  bool solve64 = false;
+  DevBuf X64, twx64, twy64;
+  PoissonPlan<double> pp64{};
   DevBuf partial, cfl_partial, div_partial;
   int top_blocks = 0, red_blocks = 0, cfl_blocks = 0;
   int refine_iters = 0;
@@ -385,20 +390,27 @@ void Engine<T>::setup_poisson() {
   if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
   if (nz > 256) throw ArgError("z solver supports nz <= 256");
   const int M = nx / 2 + 1;
-  using V2 = C2<T>;
-  X.alloc((size_t)M * ny * nz * sizeof(V2));
-  auto tw = [](int n) {
-    std::vector<V2> t(std::max(1, n / 2));
+  // exp(-2 pi i q/n), evaluated in long double and rounded once to R
+  auto tw = [](int n, auto one) {
+    using R = decltype(one);
+    std::vector<C2<R>> t(std::max(1, n / 2));
     for (int q = 0; q < n / 2; q++) {
-      // exp(-2 pi i q/n), evaluated in long double and rounded once
       long double a = 2.0L * 3.14159265358979323846264338327950288L * q / n;
-      t[q].x = (T)(double)cosl(a);
-      t[q].y = (T)(double)-sinl(a);
+      t[q].x = (R)(double)cosl(a);
+      t[q].y = (R)(double)-sinl(a);
     }
     return t;
   };
-  upload_vec(twx, tw(nx));
-  upload_vec(twy, tw(ny));
+  using V2 = C2<T>;
+  if (!solve64) {
+    X.alloc((size_t)M * ny * nz * sizeof(V2));
+    upload_vec(twx, tw(nx, (T)1));
+    upload_vec(twy, tw(ny, (T)1));
+  } else {
+    X64.alloc((size_t)M * ny * nz * sizeof(C2<double>));
+    upload_vec(twx64, tw(nx, 1.0));
+    upload_vec(twy64, tw(ny, 1.0));
+  }
   std::vector<double> lx(M), ly(ny), a(nz), c(nz);
   for (int m = 0; m < M; m++) {
     double s = std::sin(M_PI * m / nx);
@@ -416,8 +428,13 @@ void Engine<T>::setup_poisson() {
   upload_vec(lamy, ly);
   upload_vec(az64, a);
   upload_vec(cz64, c);
-  pp = PoissonPlan<T>{nx, ny, nz, M, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<double>(), lamy.as<double>(),
-                      az64.as<double>(), cz64.as<double>(), g.dzPr, g.dzW};
+  if (!solve64)
+    pp = PoissonPlan<T>{nx, ny, nz, M, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<double>(),
+                        lamy.as<double>(), az64.as<double>(), cz64.as<double>(), g.dzPr, g.dzW};
+  else
+    pp64 = PoissonPlan<double>{nx, ny, nz, M, X64.as<C2<double>>(), twx64.as<C2<double>>(), twy64.as<C2<double>>(),
+                               lamx.as<double>(), lamy.as<double>(), az64.as<double>(), cz64.as<double>(), g.dzPr,
+                               g.dzW};
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
   cfl_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
@@ -696,7 +713,10 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   if (!err.empty()) throw ArgError(err);
   const int want = std::is_same<T, double>::value ? ELMM_FP64 : ELMM_FP32_REFINED;
   if (cfg.precision != want) throw ArgError("engine/precision mismatch");
-  refine_iters = want == ELMM_FP32_REFINED ? (cfg.refine_iters > 0 ? cfg.refine_iters : 1) : 0;
+  // FP32 form: refine_iters > 0 -> FP32 spectral solves + that many FP64
+  // refinement sweeps; 0 -> one FP64 spectral solve
+  refine_iters = want == ELMM_FP32_REFINED ? (cfg.refine_iters > 0 ? cfg.refine_iters : 0) : 0;
+  solve64 = want == ELMM_FP32_REFINED && refine_iters == 0;
   if (cfg.poisson_solver != ELMM_POISSON_FFT) throw ArgError("poisson_solver must be ELMM_POISSON_FFT");
   for (int s = 4; s < 6; s++)
     if (cfg.nscalars > 0 && cfg.scal_btype[s] == ELMM_BC_PERIODIC) throw ArgError("periodic scalar z boundaries");
@@ -825,20 +845,31 @@ double Engine<T>::max_div(T *a, T *b, T *c, bool use_q) {
 // X: y FFT, z tridiagonal, inverse y, inverse x into Phi (or added to Phi).
 template <typename T>
 void Engine<T>::poisson_solve(int accumulate) {
-  timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(pp, 0, stream); });
-  timed(ELMM_T_POISSON_Z, [&] { launch_poisson_solve_z(g, pp, stream); });
-  timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(pp, 1, stream); });
-  timed(ELMM_T_POISSON_X_INV, [&] { launch_fft_x_inv(g, pp, Phi.as<double>(), accumulate, stream); });
+  auto passes = [&](const auto &plan) {
+    timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(plan, 0, stream); });
+    timed(ELMM_T_POISSON_Z, [&] { launch_poisson_solve_z(g, plan, stream); });
+    timed(ELMM_T_POISSON_Y, [&] { launch_fft_y(plan, 1, stream); });
+    timed(ELMM_T_POISSON_X_INV, [&] { launch_fft_x_inv(g, plan, Phi.as<double>(), accumulate, stream); });
+  };
+  if (solve64)
+    passes(pp64);
+  else
+    passes(pp);
 }
 
 template <typename T>
 void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, bool diag) {
   const double dt2 = coef, dt2_rec = 1.0 / dt2;
   double *q = use_q ? Q.as<double>() : nullptr;
-  // first solve; the precision form keeps the FP64 right-hand side
-  double *rhs_keep = refine_iters > 0 ? RHS.as<double>() : nullptr;
-  timed(ELMM_T_POISSON_X_FWD,
-        [&] { launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, nullptr, dt2_rec, rhs_keep, stream); });
+  // first solve; the precision form keeps the FP64 right-hand side for the
+  // refinement sweeps or for the residual diagnostic
+  double *rhs_keep = refine_iters > 0 || (solve64 && diag) ? RHS.as<double>() : nullptr;
+  timed(ELMM_T_POISSON_X_FWD, [&] {
+    if (solve64)
+      launch_fft_x_fwd(g, pp64, a, b, c, q, nullptr, nullptr, dt2_rec, rhs_keep, stream);
+    else
+      launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, nullptr, dt2_rec, rhs_keep, stream);
+  });
   poisson_solve(0);
   // iterative refinement: the x pass evaluates r = RHS - A Phi in FP64 and
   // transforms it; FP32 solve of A e = r; Phi += e (FP64)
@@ -848,7 +879,7 @@ void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, b
     });
     poisson_solve(1);
   }
-  if (diag && refine_iters > 0) {
+  if (diag && (refine_iters > 0 || solve64)) {
     // FP64 residual of the final Phi; the divergence of the FP64-corrected
     // velocity equals coef * r before the correction is rounded to T
     launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(), nullptr,
@@ -964,7 +995,7 @@ void Engine<T>::stage(int st, bool last) {
   if (last && want_diag) {
     bound_uvw(u2, v2, w2);
     last_stored_div = max_div(u2, v2, w2, use_q);
-    if (refine_iters == 0) last_div = last_stored_div;
+    if (refine_iters == 0 && !solve64) last_div = last_stored_div;
   }
   std::swap(u, u2);
   std::swap(v, v2);

@@ -626,6 +626,9 @@ ELMM_INST_PRESSURE_T(float)
 template void launch_fft_x_fwd<double, double>(const Geo &, const PoissonPlan<double> &, const double *,
                                                const double *, const double *, const double *, const double *,
                                                const double *, double, double *, hipStream_t);
+template void launch_fft_x_fwd<float, double>(const Geo &, const PoissonPlan<double> &, const float *, const float *,
+                                              const float *, const double *, const double *, const double *, double,
+                                              double *, hipStream_t);
 template void launch_fft_x_fwd<float, float>(const Geo &, const PoissonPlan<float> &, const float *, const float *,
                                              const float *, const double *, const double *, const double *, double,
                                              double *, hipStream_t);

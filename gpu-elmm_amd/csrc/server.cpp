@@ -127,7 +127,9 @@ int elmm_footprint(const elmm_config *cfg, int64_t *bytes) {
   // Q, Phi and RHS in FP64, the spectral array, masks and codes
   const long M = cfg->nx / 2 + 1;
   int64_t b = (int64_t)ntot * (12 * e + 3 * e * cfg->nscalars + 3 * 8 + 8);
-  b += (int64_t)M * cfg->ny * cfg->nz * 2 * e;
+  // spectral array: complex FP32 with refinement sweeps, complex FP64 otherwise
+  const int es = cfg->precision == ELMM_FP32_REFINED && cfg->refine_iters > 0 ? 4 : 8;
+  b += (int64_t)M * cfg->ny * cfg->nz * 2 * es;
   *bytes = b;
   return 0;
 }

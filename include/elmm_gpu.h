@@ -45,8 +45,10 @@ enum {
 enum {
   ELMM_FP64 = 0,          /* every field and every operation in FP64          */
   ELMM_FP32_REFINED = 1   /* FP32 fields and stencils; FP64 divergence/RHS,
-                             Phi, residual, corrections and global sums; FP32
-                             spectral solves inside FP64 iterative refinement */
+                             Phi, residual, corrections and global sums; the
+                             pressure solve either one FP64 spectral solve
+                             (refine_iters = 0) or FP32 spectral solves inside
+                             FP64 iterative refinement (refine_iters > 0)     */
 };
 
 /* Pressure solvers (src/pressure.f90:17-20 numbering). */
@@ -101,7 +103,8 @@ typedef struct elmm_config {
   int32_t precision;         /* ELMM_FP64 / ELMM_FP32_REFINED                 */
   int32_t ib_mass_sources;   /* enable_ibm_mass_sources                       */
   int32_t refine_iters;      /* ELMM_FP32_REFINED: FP32 solves of the FP64
-                                residual after the first solve (0 = default 1) */
+                                residual after a first FP32 solve; 0 = one
+                                FP64 spectral solve instead (default)        */
   int32_t reserved[7];
 } elmm_config;
 

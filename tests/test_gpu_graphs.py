@@ -21,9 +21,9 @@ def solver(cfg, case, fields, monkeypatch, graphs):
     return s
 
 
-@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32_REFINED])
-def test_graph_replays_equal_eager(require_gpu, monkeypatch, precision):
-    cfg = capi.with_precision(cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1), precision)
+@pytest.mark.parametrize("precision,refine", [(capi.FP64, 0), (capi.FP32_REFINED, 0), (capi.FP32_REFINED, 1)])
+def test_graph_replays_equal_eager(require_gpu, monkeypatch, precision, refine):
+    cfg = capi.with_precision(cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1), precision, refine)
     case = capi.Case(cfg, cuboids=OBSTACLES, sources=[(3.5, 3.5, 0.5, 1.0, 0)])
     f = cases.initial_fields(cfg, u0=3.0, amp=0.3, seed=9)
     eager = solver(cfg, case, f, monkeypatch, graphs=False)
