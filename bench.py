@@ -40,6 +40,10 @@ CANONICAL = {
         "sgs": (16, 16),
         "momentum": (44, 56),
         "scalar": (28, 32),
+        "poisson_x_fwd": (20, 20),       # FP32 U2,V2,W2 -> FP64 spectral write (default FP64 solve)
+        "poisson_y": (16, 16),
+        "poisson_z": (16, 16),
+        "poisson_x_inv": (16, 16),
         "projection": (36, 36),
     },
 }

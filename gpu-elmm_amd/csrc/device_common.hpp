@@ -33,10 +33,6 @@ struct Geo {
   // 1-D coordinate tables (device pointers already offset so index i works
   // directly for the reference's index i)
   const double *dxPr, *dyPr, *dzPr, *dzW, *zPr, *zW;
-  // per level k (4 entries): div_rn reciprocals of dzPr(k) and of
-  // 2 (zPr(k+1) - zPr(k-1)) in FP64, then the same divisors formed in FP32
-  // (stored as doubles); 0 where div_rn's condition fails
-  const double *rz;
 };
 
 __host__ __device__ inline long gix(const Geo &g, int i, int j, int k) {

@@ -196,7 +196,7 @@ struct Engine : EngineIface {
   // masks and codes
   DevBuf code[3], code_s, mask[3], prtype;
   // coordinates
-  DevBuf coords, rz_tab;
+  DevBuf coords;
   // wall model
   DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
   DevBuf width_tab, width_idx;
@@ -333,18 +333,6 @@ void Engine<T>::setup_geo() {
   g.dzW = base + off[3] + 3;
   g.zPr = base + off[4] + 3;
   g.zW = base + off[5] + 3;
-  {
-    std::vector<double> rz(4 * (nz + 2), 0.0);
-    for (int k = 1; k <= nz; k++) {
-      const double zp = Grid1D::at(grid.zPr, k + 1), zm = Grid1D::at(grid.zPr, k - 1);
-      rz[4 * k + 0] = div_recip(Grid1D::at(grid.dzPr, k));
-      rz[4 * k + 1] = div_recip(2 * (zp - zm));
-      rz[4 * k + 2] = div_recipf((float)Grid1D::at(grid.dzPr, k));
-      rz[4 * k + 3] = div_recipf(2.0f * ((float)zp - (float)zm));
-    }
-    upload_vec(rz_tab, rz);
-    g.rz = rz_tab.as<double>();
-  }
   // SGS filter width (src/subgrid.f90:62): filter_ratio *
   // (dxPr*dyPr*dzPr)**(1/3), from the host libm like the reference's.  The
   // products dxPr(i)*dyPr(j) take few distinct values (one on exactly

@@ -38,21 +38,15 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
     if (sgs_model == 1) {
       const T dx = (T)g.dxmin, dy = (T)g.dymin;
       const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
-      // every quotient correctly rounded (div_rn), so equal to the plain
-      // division; 4 dx, 4 dy have the reciprocals r/4
-      const bool f32 = sizeof(T) == 4;
-      const T rdx = f32 ? (T)g.rdxf : (T)g.rdx, rdy = f32 ? (T)g.rdyf : (T)g.rdy;
-      const T rz1 = (T)g.rz[4 * k + (f32 ? 2 : 0)], rz2 = (T)g.rz[4 * k + (f32 ? 3 : 1)];
-      const T c2 = 2 * (zp1 - zm1);
-      T D00 = div_rn<T>(U[p] - U[p - 1], dx, rdx);
-      T D11 = div_rn<T>(V[p] - V[p - sy], dy, rdy);
-      T D22 = div_rn<T>(W[p] - W[p - sz], (T)g.dzPr[k], rz1);
-      T D01 = div_rn<T>(U[p + sy] + U[p - 1 + sy] - U[p - sy] - U[p - 1 - sy], 4 * dy, rdy * (T)0.25);
-      T D02 = div_rn<T>(U[p + sz] + U[p - 1 + sz] - U[p - sz] - U[p - 1 - sz], c2, rz2);
-      T D10 = div_rn<T>(V[p + 1] + V[p + 1 - sy] - V[p - 1] - V[p - 1 - sy], 4 * dx, rdx * (T)0.25);
-      T D12 = div_rn<T>(V[p + sz] + V[p - sy + sz] - V[p - sz] - V[p - sy - sz], c2, rz2);
-      T D20 = div_rn<T>(W[p + 1] + W[p + 1 - sz] - W[p - 1] - W[p - 1 - sz], 4 * dx, rdx * (T)0.25);
-      T D21 = div_rn<T>(W[p + sy] + W[p + sy - sz] - W[p - sy] - W[p - sy - sz], 4 * dy, rdy * (T)0.25);
+      T D00 = (U[p] - U[p - 1]) / dx;
+      T D11 = (V[p] - V[p - sy]) / dy;
+      T D22 = (W[p] - W[p - sz]) / (T)g.dzPr[k];
+      T D01 = (U[p + sy] + U[p - 1 + sy] - U[p - sy] - U[p - 1 - sy]) / (4 * dy);
+      T D02 = (U[p + sz] + U[p - 1 + sz] - U[p - sz] - U[p - 1 - sz]) / (2 * (zp1 - zm1));
+      T D10 = (V[p + 1] + V[p + 1 - sy] - V[p - 1] - V[p - 1 - sy]) / (4 * dx);
+      T D12 = (V[p + sz] + V[p - sy + sz] - V[p - sz] - V[p - sy - sz]) / (2 * (zp1 - zm1));
+      T D20 = (W[p + 1] + W[p + 1 - sz] - W[p - 1] - W[p - 1 - sz]) / (4 * dx);
+      T D21 = (W[p + sy] + W[p + sy - sz] - W[p - sy] - W[p - sy - sz]) / (4 * dy);
       const T D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
       T s = 0;
 #pragma unroll
@@ -84,16 +78,12 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
       double a1 = fabs((double)U[p]), a0 = fabs((double)U[p - 1]);
       double b1 = fabs((double)V[p]), b0 = fabs((double)V[p - sy]);
       double c1 = fabs((double)W[p]), c0 = fabs((double)W[p - sz]);
-      double q;
-      if (g.variable_z) {
-        q = (a1 > a0 ? a1 : a0) / g.dxPr[i];
-        q = q + (b1 > b0 ? b1 : b0) / g.dyPr[j];
-        q = q + div_rn<double>(c1 > c0 ? c1 : c0, g.dzPr[k], g.rz[4 * k]);
-      } else {
-        q = div_rn<double>(a1 > a0 ? a1 : a0, g.dxmin, g.rdx);
-        q = q + div_rn<double>(b1 > b0 ? b1 : b0, g.dymin, g.rdy);
-        q = q + div_rn<double>(c1 > c0 ? c1 : c0, g.dzmin, g.rdz);
-      }
+      double dxk = g.variable_z ? g.dxPr[i] : g.dxmin;
+      double dyk = g.variable_z ? g.dyPr[j] : g.dymin;
+      double dzk = g.variable_z ? g.dzPr[k] : g.dzmin;
+      double q = (a1 > a0 ? a1 : a0) / dxk;
+      q = q + (b1 > b0 ? b1 : b0) / dyk;
+      q = q + (c1 > c0 ? c1 : c0) / dzk;
       cfl = (q != q) ? INFINITY : q;
     }
   }

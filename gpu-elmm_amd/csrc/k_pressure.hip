@@ -52,6 +52,11 @@ __device__ inline int bitrev(int v, int logn) { return (int)(__brev((unsigned)v)
 // every butterfly is the same operation on the same operands as a
 // stage-by-stage radix-2 sweep.
 __device__ inline int fpad(int m) { return m + (m >> 3); }
+// Twiddle table in LDS with one pad slot per 16 entries: a stage reads
+// tw[pos * tstep] with power-of-two tstep, which unpadded puts all lanes of a
+// wavefront in one bank.
+__host__ __device__ inline int twpad(int t) { return t + (t >> 4); }
+__host__ __device__ inline int twlen(int n) { return twpad(n / 2) + 1; }
 
 template <typename V>
 __device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *tw, bool inverse) {
@@ -77,7 +82,7 @@ __device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *t
           if (e >= ne || (e >> b) & 1) continue;
           const int e1 = e | (1 << b);
           const int pos = (base + (e << s0)) & (half - 1);
-          V w = tw[pos * tstep];
+          V w = tw[twpad(pos * tstep)];
           if (inverse) w.y = -w.y;
           const V a = x[e], c = cmul(x[e1], w);
           x[e] = cadd(a, c);
@@ -95,7 +100,7 @@ __device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *t
 // twiddle table of length n/2 from global memory into LDS
 template <typename V>
 __device__ inline void load_twiddles(V *dst, const V *__restrict__ src, int n) {
-  for (int t = threadIdx.x; t < n / 2; t += blockDim.x) dst[t] = src[t];
+  for (int t = threadIdx.x; t < n / 2; t += blockDim.x) dst[twpad(t)] = src[t];
 }
 
 // Pass 1: RHS = ((dU/dx + dV/dy + dW/dz) - Q) / coef for the rows of this
@@ -130,7 +135,7 @@ __global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, c
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
   const int nx = pp.nx, ls = fpad(nx);
-  C2<R> *sh = tw + nx / 2;
+  C2<R> *sh = tw + twlen(nx);
   load_twiddles(tw, pp.twx, nx);
   const int row0 = blockIdx.x * rows_per_block;
   const int nrows = (long)pp.ny * pp.nz;
@@ -177,7 +182,7 @@ __global__ void k_fft_y(PoissonPlan<R> pp, int logn, int inverse) {
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
   const int ny = pp.ny, M = pp.M, ls = ystride(ny);
-  C2<R> *sh = tw + ny / 2;
+  C2<R> *sh = tw + twlen(ny);
   load_twiddles(tw, pp.twy, ny);
   const int m0 = blockIdx.x * TX, k = blockIdx.y;
   C2<R> *X = pp.X + (long)k * ny * M;
@@ -320,7 +325,7 @@ __global__ void k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, 
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
   const int nx = pp.nx, M = pp.M, ls = fpad(nx);
-  C2<R> *sh = tw + nx / 2;
+  C2<R> *sh = tw + twlen(nx);
   load_twiddles(tw, pp.twx, nx);
   const int row0 = blockIdx.x * rows_per_block;
   const int nrows = pp.ny * pp.nz;
@@ -373,7 +378,7 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
                       const double *rhs, const double *phi, double dt2_rec, double *rhs_out, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
-  size_t sh = ((size_t)rows * padded(pp.nx) + pp.nx / 2) * sizeof(C2<R>);
+  size_t sh = ((size_t)rows * padded(pp.nx) + twlen(pp.nx)) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_x_fwd<T, R>, sh);
   hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, phi, dt2_rec,
                      rhs_out, rows, ilog2(pp.nx));
@@ -381,7 +386,7 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
 
 template <int TX, typename R>
 static void launch_fft_y_tx(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
-  const size_t sh = ((size_t)TX * ystride(pp.ny) + pp.ny / 2) * sizeof(C2<R>);
+  const size_t sh = ((size_t)TX * ystride(pp.ny) + twlen(pp.ny)) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_y<TX, R>, sh);
   hipLaunchKernelGGL((k_fft_y<TX, R>), dim3(nblk(pp.M, TX), pp.nz), dim3(256), sh, st, pp, ilog2(pp.ny), inverse);
 }
@@ -421,7 +426,7 @@ template <typename R>
 void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st) {
   const int rows = rows_for(pp.nx);
   const int nrows = pp.ny * pp.nz;
-  size_t sh = ((size_t)rows * padded(pp.nx) + pp.nx / 2) * sizeof(C2<R>);
+  size_t sh = ((size_t)rows * padded(pp.nx) + twlen(pp.nx)) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_x_inv<R>, sh);
   hipLaunchKernelGGL(k_fft_x_inv<R>, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, Phi, accumulate, rows,
                      ilog2(pp.nx));

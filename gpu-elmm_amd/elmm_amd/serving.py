@@ -108,7 +108,9 @@ def run_ensemble(n_cases=48, steps=20, seed=4, precision=capi.FP32_REFINED, warm
         sv.wait()
     wall = time.perf_counter() - t2
     cells = [c.nx * c.ny * c.nz for c, _, _ in prepared]
-    divs = [sv.divergence() for sv in solvers]
+    # after the timed run: one more step per case with the projection's FP64
+    # divergence diagnostic (check_divergence, src/pressure.f90:884-989)
+    divs = [sv.step(1).max_div for sv in solvers]
     for sv in solvers:
         sv.close()
     return {"cases": n_cases, "steps": steps, "cells_min": int(min(cells)), "cells_max": int(max(cells)),

@@ -8,6 +8,10 @@ urban512 : 512x512x128, dx=1 m, periodic x/y, rough log-law bottom
            point source at street level, dt=0.05 s.
 flat512  : the same grid and flow without obstacles.
 uniform64: 64^3 uniform case (dx=1 m, u=1 m/s +- 0.1, dt=0.05 s).
+channel256: turbulent channel 256x128x128, 2 pi x pi x 2, tanh-stretched z
+           (gamma 1.5), smooth-wall wall model at both walls, nu = 1e-3,
+           pr_gradient_x = -1 (Re_tau = 1000), law-of-the-wall start with 5 %
+           noise, dt = 2e-4 s, variable-z CD2 advection.
 """
 import json
 import os
@@ -59,7 +63,12 @@ def _uniform64():
     return cases.config1(64), [], []
 
 
+def _channel256():
+    return cases.channel_config(256, 128, 128), [], []
+
+
 _WL = {
+    "channel256": Workload("channel256", _channel256, lambda cfg: cases.channel_fields(cfg)),
     "flat512": Workload("flat512", _flat512, _fields_u3),
     "urban512": Workload("urban512", _urban512, _fields_u3),
     "uniform64": Workload("uniform64", _uniform64, lambda cfg: cases.initial_fields(cfg)),
