@@ -52,11 +52,7 @@ __device__ inline int bitrev(int v, int logn) { return (int)(__brev((unsigned)v)
 // every butterfly is the same operation on the same operands as a
 // stage-by-stage radix-2 sweep.
 __device__ inline int fpad(int m) { return m + (m >> 3); }
-// Twiddle table in LDS with one pad slot per 16 entries: a stage reads
-// tw[pos * tstep] with power-of-two tstep, which unpadded puts all lanes of a
-// wavefront in one bank.
-__host__ __device__ inline int twpad(int t) { return t + (t >> 4); }
-__host__ __device__ inline int twlen(int n) { return twpad(n / 2) + 1; }
+__host__ __device__ inline int twlen(int n) { return n / 2; }
 
 template <typename V>
 __device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *tw, bool inverse) {
@@ -82,7 +78,7 @@ __device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *t
           if (e >= ne || (e >> b) & 1) continue;
           const int e1 = e | (1 << b);
           const int pos = (base + (e << s0)) & (half - 1);
-          V w = tw[twpad(pos * tstep)];
+          V w = tw[pos * tstep];
           if (inverse) w.y = -w.y;
           const V a = x[e], c = cmul(x[e1], w);
           x[e] = cadd(a, c);
@@ -100,7 +96,7 @@ __device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *t
 // twiddle table of length n/2 from global memory into LDS
 template <typename V>
 __device__ inline void load_twiddles(V *dst, const V *__restrict__ src, int n) {
-  for (int t = threadIdx.x; t < n / 2; t += blockDim.x) dst[twpad(t)] = src[t];
+  for (int t = threadIdx.x; t < n / 2; t += blockDim.x) dst[t] = src[t];
 }
 
 // Pass 1: RHS = ((dU/dx + dV/dy + dW/dz) - Q) / coef for the rows of this
