@@ -360,7 +360,18 @@ static int ilog2(int n) {
   return l;
 }
 
-static int rows_for(int nx) { return std::max(1, 2048 / nx); }
+// Rows per block of the x passes (ELMM_FFTX_ROWS overrides; default 2048/nx)
+// and the block size: one thread per 8-element butterfly set of the rows,
+// at most 256.
+static int rows_for(int nx) {
+  static int env = -1;
+  if (env < 0) {
+    const char *e = std::getenv("ELMM_FFTX_ROWS");
+    env = e ? std::max(0, std::atoi(e)) : 0;
+  }
+  return env > 0 ? env : std::max(1, 2048 / nx);
+}
+static int xthreads(int nx, int rows) { return std::max(64, std::min(256, rows * std::max(1, nx / 8))); }
 static int padded(int n) { return n + n / 8; }
 
 // Dynamic LDS above 64 KiB must be opted into per kernel (gfx950 has 160 KiB
@@ -376,7 +387,8 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
   const int nrows = pp.ny * pp.nz;
   size_t sh = ((size_t)rows * padded(pp.nx) + twlen(pp.nx)) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_x_fwd<T, R>, sh);
-  hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, U, V, W, Q, rhs, phi, dt2_rec,
+  hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(xthreads(pp.nx, rows)), sh, st, g, pp, U, V, W,
+                     Q, rhs, phi, dt2_rec,
                      rhs_out, rows, ilog2(pp.nx));
 }
 
@@ -424,7 +436,8 @@ void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int a
   const int nrows = pp.ny * pp.nz;
   size_t sh = ((size_t)rows * padded(pp.nx) + twlen(pp.nx)) * sizeof(C2<R>);
   allow_lds((const void *)k_fft_x_inv<R>, sh);
-  hipLaunchKernelGGL(k_fft_x_inv<R>, dim3(nblk(nrows, rows)), dim3(256), sh, st, g, pp, Phi, accumulate, rows,
+  hipLaunchKernelGGL(k_fft_x_inv<R>, dim3(nblk(nrows, rows)), dim3(xthreads(pp.nx, rows)), sh, st, g, pp, Phi,
+                     accumulate, rows,
                      ilog2(pp.nx));
 }
 
