@@ -52,11 +52,27 @@ struct MomZArgs {
   const uint8_t *code[3];
 };
 
+// Quotient by a grid constant: correctly rounded (div_rn, the FP64
+// instantiation, equal to x / d) or, with ELMM_ZM_RECIP (the precision form),
+// the product with a reciprocal.
+#ifndef ELMM_ZM_RECIP
+#define ELMM_ZM_RECIP 0
+#endif
+template <typename T>
+__device__ inline T zm_div(T x, T d, T r) {
+#if ELMM_ZM_RECIP
+  (void)d;
+  return x * r;
+#else
+  return div_rn<T>(x, d, r);
+#endif
+}
+
 // d of one direction, 4th order, from the UV1/UV3 products
 template <typename T>
 __device__ inline T d4(T uv1b, T uv1a, T uv3b, T uv3a) {
   T d = 9 * (uv1b - uv1a);
-  return d - div_rn<T>(uv3b - uv3a, (T)3, recip3<T>());
+  return d - zm_div<T>(uv3b - uv3a, (T)3, recip3<T>());
 }
 
 template <typename T, typename FF>
@@ -186,8 +202,12 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   const T rx = (T)1.0 / ((T)g.dxmin * (T)g.dxmin), ry = (T)1.0 / ((T)g.dymin * (T)g.dymin),
           rz = (T)1.0 / ((T)g.dzmin * (T)g.dzmin);
   const T ix = (T)g.dxmin, iy = (T)g.dymin, iz = (T)g.dzmin;
+#if ELMM_ZM_RECIP
+  const T rix = (T)g.invdxf, riy = (T)g.invdyf, riz = (T)g.invdzf;
+#else
   const T rix = sizeof(T) == 4 ? (T)g.rdxf : (T)g.rdx, riy = sizeof(T) == 4 ? (T)g.rdyf : (T)g.rdy,
           riz = sizeof(T) == 4 ? (T)g.rdzf : (T)g.rdz;
+#endif
   const T q4 = (T)0.25, m256 = (T)(-1.0 / 256);
   const bool in_xy = i <= g.nx && j <= g.ny;
   // per-cell operands of the level being computed (Pr and its +x, +y, +z
@@ -257,18 +277,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             T d;
             if (cls == 0) {
               d = self4<T>(Fx);
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + zm_div<T>(d, ix, rix);
               d = cross4<T>(Fy, [&](int o) { return m4<T>(at(v0, -1, o), at(v0, 0, o), at(v0, 1, o), at(v0, 2, o)); });
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + zm_div<T>(d, iy, riy);
               d = cross4<T>(Fz, [&](int o) { return wx[2 + o]; });
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + zm_div<T>(d, iz, riz);
             } else {
               d = self2<T>(Fx);
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + zm_div<T>(d, ix, rix);
               d = cross2<T>(Fy, at(v0, 1, 0) + at(v0, 0, 0), at(v0, 1, -1) + at(v0, 0, -1));
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + zm_div<T>(d, iy, riy);
               d = cross2<T>(Fz, at(w0, 1, 0) + at(w0, 0, 0), at(wm, 1, 0) + at(wm, 0, 0));
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + zm_div<T>(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -295,18 +315,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             T d;
             if (cls == 0) {
               d = cross4<T>(Fx, [&](int o) { return m4<T>(at(u0, o, -1), at(u0, o, 0), at(u0, o, 1), at(u0, o, 2)); });
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + zm_div<T>(d, ix, rix);
               d = self4<T>(Fy);
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + zm_div<T>(d, iy, riy);
               d = cross4<T>(Fz, [&](int o) { return wy[2 + o]; });
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + zm_div<T>(d, iz, riz);
             } else {
               d = cross2<T>(Fx, at(u0, 0, 1) + at(u0, 0, 0), at(u0, -1, 1) + at(u0, -1, 0));
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + zm_div<T>(d, ix, rix);
               d = self2<T>(Fy);
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + zm_div<T>(d, iy, riy);
               d = cross2<T>(Fz, at(w0, 0, 1) + at(w0, 0, 0), at(wm, 0, 1) + at(wm, 0, 0));
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + zm_div<T>(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -333,18 +353,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             T d;
             if (cls == 0) {
               d = cross4<T>(Fx, [&](int o) { return m4<T>(at(um, o, 0), at(u0, o, 0), at(u1, o, 0), at(u2s, o, 0)); });
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + zm_div<T>(d, ix, rix);
               d = cross4<T>(Fy, [&](int o) { return m4<T>(at(vm, 0, o), at(v0, 0, o), at(v1, 0, o), at(v2s, 0, o)); });
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + zm_div<T>(d, iy, riy);
               d = self4<T>(Fz);
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + zm_div<T>(d, iz, riz);
             } else {
               d = cross2<T>(Fx, at(u1, 0, 0) + at(u0, 0, 0), at(u1, -1, 0) + at(u0, -1, 0));
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + zm_div<T>(d, ix, rix);
               d = cross2<T>(Fy, at(v1, 0, 0) + at(v0, 0, 0), at(v1, 0, -1) + at(v0, 0, -1));
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + zm_div<T>(d, iy, riy);
               d = self2<T>(Fz);
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + zm_div<T>(d, iz, riz);
             }
           }
           T w = acc * m256;

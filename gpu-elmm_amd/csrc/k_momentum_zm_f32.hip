@@ -1,9 +1,11 @@
 // k_momentum_zm_f32.hip - FP32 instantiation of the fused z-marching
 // momentum kernel (k_momentum_zm.inl) for the precision form.  Products
-// feeding a sum inside one expression may fuse into FMAs here (one rounding
-// instead of two): the precision form is checked against the FP64 oracle by
-// its error bars, not bit for bit, and the fused forms save VALU issue slots.
+// feeding a sum inside one expression may fuse into FMAs, and quotients by
+// the grid spacings and by 3 are products with reciprocals (one rounding
+// each): the precision form is checked against the FP64 oracle by its error
+// bars, not bit for bit, and these forms save VALU issue slots.
 #pragma clang fp contract(on)
+#define ELMM_ZM_RECIP 1
 
 #include "k_momentum_zm.inl"
 
