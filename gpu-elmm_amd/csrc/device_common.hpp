@@ -26,7 +26,6 @@ struct Geo {
   // close enough for div_rn (then the kernels divide)
   double rdx, rdy, rdz;
   float rdxf, rdyf, rdzf;
-  float invdxf, invdyf, invdzf;  // 1/dx, 1/dy, 1/dz in FP32 (precision-form kernels)
   int variable_z;
   int bt[6];          // velocity boundary types
   int sbt[6];         // scalar boundary types

@@ -296,9 +296,6 @@ void Engine<T>::setup_geo() {
   g.rdxf = div_recipf((float)g.dxmin);
   g.rdyf = div_recipf((float)g.dymin);
   g.rdzf = div_recipf((float)g.dzmin);
-  g.invdxf = 1.0f / (float)g.dxmin;
-  g.invdyf = 1.0f / (float)g.dymin;
-  g.invdzf = 1.0f / (float)g.dzmin;
   g.variable_z = cfg.stretched_z != 0;
   {
     const char *m = std::getenv("ELMM_MOMENTUM");
