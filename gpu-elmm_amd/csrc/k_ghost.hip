@@ -39,31 +39,38 @@ __host__ inline Shell make_shell(int n0, int n1, int n2, int gl, int gh, int zi0
 
 __device__ inline int ghost_index(int t, int gl, int n) { return t < gl ? t - gl + 1 : n + 1 + (t - gl); }
 
-__device__ inline bool decode(const Shell &s, long t, int &i, int &j, int &k) {
-  int g = s.gl + s.gh;
-  if (t < s.cz) {
-    int w = s.zi1 - s.zi0 + 1, h = s.zj1 - s.zj0 + 1;
-    i = s.zi0 + (int)(t % w);
-    long r = t / w;
-    j = s.zj0 + (int)(r % h);
-    k = ghost_index((int)(r / h), s.gl, s.n2);
+// 32-bit unsigned index arithmetic (a 64-bit division costs several times a
+// 32-bit one).  The engine admits nx, ny <= 1024 and nz <= 256, so a shell
+// has fewer than 2^25 cells.
+__device__ inline bool decode(const Shell &s, long t_, int &i, int &j, int &k) {
+  const unsigned g = (unsigned)(s.gl + s.gh);
+  unsigned t = (unsigned)t_;
+  if (t < (unsigned)s.cz) {
+    const unsigned w = (unsigned)(s.zi1 - s.zi0 + 1), h = (unsigned)(s.zj1 - s.zj0 + 1);
+    const unsigned r = t / w;
+    i = s.zi0 + (int)(t - r * w);
+    const unsigned kk = r / h;
+    j = s.zj0 + (int)(r - kk * h);
+    k = ghost_index((int)kk, s.gl, s.n2);
     return true;
   }
-  t -= s.cz;
-  if (t < s.cy) {
-    int w = s.yi1 - s.yi0 + 1;
-    i = s.yi0 + (int)(t % w);
-    long r = t / w;
-    j = ghost_index((int)(r % g), s.gl, s.n1);
-    k = 1 + (int)(r / g);
+  t -= (unsigned)s.cz;
+  if (t < (unsigned)s.cy) {
+    const unsigned w = (unsigned)(s.yi1 - s.yi0 + 1);
+    const unsigned r = t / w;
+    i = s.yi0 + (int)(t - r * w);
+    const unsigned kk = r / g;
+    j = ghost_index((int)(r - kk * g), s.gl, s.n1);
+    k = 1 + (int)kk;
     return true;
   }
-  t -= s.cy;
-  if (t < s.cx) {
-    i = ghost_index((int)(t % g), s.gl, s.n0);
-    long r = t / g;
-    j = 1 + (int)(r % s.n1);
-    k = 1 + (int)(r / s.n1);
+  t -= (unsigned)s.cy;
+  if (t < (unsigned)s.cx) {
+    const unsigned r = t / g;
+    i = ghost_index((int)(t - r * g), s.gl, s.n0);
+    const unsigned kk = r / (unsigned)s.n1;
+    j = 1 + (int)(r - kk * (unsigned)s.n1);
+    k = 1 + (int)kk;
     return true;
   }
   return false;

@@ -7,15 +7,16 @@
 // FD2 operator of Residue_variable_z (src/poisson.f90:23-60):
 //   periodic x, y : discrete Fourier modes, eigenvalue -4 sin^2(pi m/n)/d^2
 //   z (walls)     : tridiagonal per mode with the staggered-Neumann rows,
-//                   solved by parallel cyclic reduction in LDS.
+//                   solved by Thomas elimination, one thread per mode.
 // Passes (each one read + one write of the spectral array in HBM):
 //   1 divergence + x FFT of each (j,k) row  (RHS never goes to HBM)
 //   2 y FFT of tiles of 8 x-modes
-//   3 z tridiagonal solve of tiles of 8 x-modes
+//   3 z tridiagonal solve of every (x-mode, y-mode) pair
 //   4 inverse y FFT
 //   5 inverse x FFT, real part -> Phi
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -41,57 +42,110 @@ __device__ inline V csub(V a, V b) {
   return mk2(a.x - b.x, a.y - b.y);
 }
 
-__device__ inline int bitrev(int v, int logn) { return (int)(__brev((unsigned)v) >> (32 - logn)); }
+__device__ inline int bitrev(int v, int logn) { return logn ? (int)(__brev((unsigned)v) >> (32 - logn)) : 0; }
 
-// In-place radix-2 decimation-in-time FFT of `cnt` independent lines stored
-// in LDS, line l at buf[l*stride + pad(m)], pad(m) = m + m/8 (one pad slot per
-// 8 elements keeps the 8-element register sets free of bank conflicts).  Input
-// must already be in bit-reversed order.  tw[t] = exp(-2 pi i t / n) (in LDS);
-// inverse uses conj.  Stages are executed three at a time on 8-element sets
-// held in registers (the sets are closed under three consecutive stages), so
-// every butterfly is the same operation on the same operands as a
-// stage-by-stage radix-2 sweep.
-__device__ inline int fpad(int m) { return m + (m >> 3); }
+// LDS slot of element m of a line: one pad slot per 8 elements, plus one per
+// 64 with padmode 1.  fspan: slots one line occupies.
+__device__ inline int fpad(int m, int padmode) { return m + (m >> 3) + (padmode ? (m >> 6) : 0); }
+__host__ __device__ inline int fspan(int n, int padmode) {
+  const int m = n - 1;
+  return m + (m >> 3) + (padmode ? (m >> 6) : 0) + 1;
+}
 __host__ __device__ inline int twlen(int n) { return n / 2; }
 
-template <typename V>
-__device__ void lds_fft(V *buf, int n, int logn, int cnt, int stride, const V *tw, bool inverse) {
-  for (int s0 = 0; s0 < logn; s0 += 3) {
-    const int nb = min(3, logn - s0);
-    const int ne = 1 << nb;
-    const int per_line = n >> nb;
-    const int nsets = per_line * cnt;
-    for (int q = threadIdx.x; q < nsets; q += blockDim.x) {
-      const int l = q / per_line, r = q % per_line;
-      const int lo = r & ((1 << s0) - 1), hi = r >> s0;
-      const int base = (hi << (s0 + nb)) | lo;
-      V x[8];
+// Radix-2 decimation-in-time FFT of CNT independent lines of length 2^LOGN
+// by NT threads (the block size).  The stages run three at a time on
+// 8-element sets held in registers (the sets {base + (e << s0)} are closed
+// under stages s0..s0+2), so every butterfly is the same operation on the
+// same operands as a stage-by-stage radix-2 sweep over bit-reversed input.
+//   first round  : each set is loaded straight from global memory
+//                  (load(line, natural index)), so the input never takes a
+//                  bit-reversed trip through LDS.  With PermFirst the sets
+//                  are dealt to threads in bit-reversed order, which makes
+//                  the natural indices of one element slot consecutive
+//                  across consecutive threads (coalesced loads along a line);
+//   middle rounds: in place in LDS, line l at buf[l*stride + fpad(pos)];
+//   last round   : the outputs (natural order) go from registers straight to
+//                  global memory (store(line, index, value)).
+// LineMajor: consecutive threads take consecutive sets of one line (rows
+// contiguous along the transform, the x passes); otherwise consecutive
+// threads take the same set of consecutive lines (lines side by side in
+// memory, the y pass).  tw_g (global) and tw_s (LDS, written before the
+// first round) hold the same table tw[t] = exp(-2 pi i t / n); inverse uses
+// the conjugate.
+template <int LOGN, int CNT, int NT, bool LineMajor, bool PermFirst, typename V, typename Load, typename Store>
+__device__ __forceinline__ void fft_lines(V *buf, int stride, int padmode, const V *__restrict__ tw_g, const V *tw_s,
+                                          bool inverse, Load load, Store store) {
+  constexpr int N = 1 << LOGN;
+  if constexpr (LOGN == 0) {
+    for (int l = threadIdx.x; l < CNT; l += NT) store(l, 0, load(l, 0));  // length 1: identity
+  } else {
 #pragma unroll
-      for (int e = 0; e < 8; e++)
-        if (e < ne) x[e] = buf[l * stride + fpad(base + (e << s0))];
+    for (int s0 = 0; s0 < LOGN; s0 += 3) {
+      const int nb = LOGN - s0 < 3 ? LOGN - s0 : 3;
+      const int ne = 1 << nb;
+      const int per_line = N >> nb;
+      const int nsets = per_line * CNT;
+      const bool first = s0 == 0, last = s0 + nb >= LOGN;
+      const V *tw = first ? tw_g : tw_s;
 #pragma unroll
-      for (int b = 0; b < 3; b++) {
-        if (b >= nb) break;
-        const int half = 1 << (s0 + b), tstep = n / (2 * half);
+      for (int it = 0; it < (nsets + NT - 1) / NT; it++) {
+        const int q = (int)threadIdx.x + it * NT;
+        if (nsets % NT == 0 || q < nsets) {
+          const int l = LineMajor ? q / per_line : q % CNT;
+          int r = LineMajor ? q % per_line : q / CNT;
+          if (PermFirst && first) r = bitrev(r, LOGN - nb);
+          const int lo = r & ((1 << s0) - 1), hi = r >> s0;
+          const int base = (hi << (s0 + nb)) | lo;
+          V x[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-          if (e >= ne || (e >> b) & 1) continue;
-          const int e1 = e | (1 << b);
-          const int pos = (base + (e << s0)) & (half - 1);
-          V w = tw[pos * tstep];
-          if (inverse) w.y = -w.y;
-          const V a = x[e], c = cmul(x[e1], w);
-          x[e] = cadd(a, c);
-          x[e1] = csub(a, c);
+          for (int e = 0; e < 8; e++) {
+            if (e >= ne) break;
+            const int pos = base + (e << s0);
+            x[e] = first ? load(l, bitrev(pos, LOGN)) : buf[l * stride + fpad(pos, padmode)];
+          }
+#pragma unroll
+          for (int b = 0; b < 3; b++) {
+            if (b >= nb) break;
+            const int half = 1 << (s0 + b), tstep = N / (2 * half);
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+              if (e >= ne || (e >> b) & 1) continue;
+              const int e1 = e | (1 << b);
+              const int pos = (base + (e << s0)) & (half - 1);
+              V w = tw[pos * tstep];
+              if (inverse) w.y = -w.y;
+              const V a = x[e], c = cmul(x[e1], w);
+              x[e] = cadd(a, c);
+              x[e1] = csub(a, c);
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            if (e >= ne) break;
+            const int pos = base + (e << s0);
+            if (last)
+              store(l, pos, x[e]);
+            else
+              buf[l * stride + fpad(pos, padmode)] = x[e];
+          }
         }
       }
-#pragma unroll
-      for (int e = 0; e < 8; e++)
-        if (e < ne) buf[l * stride + fpad(base + (e << s0))] = x[e];
+      if (!last) __syncthreads();
     }
-    __syncthreads();
   }
 }
+
+// Lines per block of the x passes: 2048 elements per block (4 rows of 512;
+// 2 rows measured 5 % slower, 8 rows 45 % slower); threads: one per
+// 8-element set of the rows, 64..256.
+__host__ __device__ constexpr int xrows(int logn) { return logn >= 11 ? 1 : (2048 >> logn); }
+__host__ __device__ constexpr int xthreads(int logn, int rows) {
+  return rows * (logn >= 3 ? 1 << (logn - 3) : 1) < 64    ? 64
+         : rows * (logn >= 3 ? 1 << (logn - 3) : 1) > 256 ? 256
+                                                          : rows * (logn >= 3 ? 1 << (logn - 3) : 1);
+}
+constexpr int kFftThreads = 256;
 
 // twiddle table of length n/2 from global memory into LDS
 template <typename V>
@@ -120,81 +174,83 @@ __device__ inline double fd2_residual(const Geo &g, const double *__restrict__ P
   return RHS[p] - a;
 }
 
-// Right-hand side of the row: (a) the divergence of (U, V, W) minus Q over
-// coef (first solve); (b) rhs_in (a stored right-hand side); (c) with phi,
-// the FP64 residual rhs_in - A phi (refinement sweeps).
-template <typename T, typename R>
-__global__ void k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, const T *__restrict__ V,
-                            const T *__restrict__ W, const double *__restrict__ Q,
-                            const double *__restrict__ rhs_in, const double *__restrict__ phi, double dt2_rec,
-                            double *__restrict__ rhs_out, int rows_per_block, int logn) {
+// Right-hand side of the row, by MODE: kRhsDiv (+ kRhsQ, + kRhsOut) the
+// divergence of (U, V, W) (minus Q) over coef, stored to rhs_out with
+// kRhsOut (first solve); kRhsIn a stored right-hand side; kRhsResid the FP64
+// residual rhs_in - A phi (refinement sweeps).  The variants are separate
+// instantiations so that the element loads carry no branches and issue
+// together.  Rows past the end load row nrows-1 and are not stored.  ny is a
+// power of two: row = (j-1) + ny (k-1).
+enum { kRhsDiv = 0, kRhsQ = 1, kRhsOut = 2, kRhsIn = 4, kRhsResid = 8 };
+
+template <typename T, typename R, int LOGN, int MODE, int ROWS>
+__global__ void __launch_bounds__(kFftThreads)
+    k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+                const double *__restrict__ Q, const double *__restrict__ rhs_in, const double *__restrict__ phi,
+                double dt2_rec, double *__restrict__ rhs_out, int logny, int padmode) {
+  constexpr int N = 1 << LOGN, NT = xthreads(LOGN, ROWS);
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
-  const int nx = pp.nx, ls = fpad(nx);
-  C2<R> *sh = tw + twlen(nx);
-  load_twiddles(tw, pp.twx, nx);
-  const int row0 = blockIdx.x * rows_per_block;
-  const int nrows = (long)pp.ny * pp.nz;
-  for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
-    const int l = t / nx, m = t % nx;
-    const int row = row0 + l;
-    double r = 0;
-    if (row < nrows) {
-      const int j = row % pp.ny + 1, k = row / pp.ny + 1, i = m + 1;
-      const long p = gix(g, i, j, k);
-      if (phi) {
-        r = fd2_residual(g, phi, rhs_in, pp.az, pp.cz, i, j, k, p);
-      } else if (rhs_in) {
-        r = rhs_in[p];
-      } else {
-        const double dz = g.variable_z ? g.dzPr[k] : g.dzmin;
-        r = ((double)U[p] - (double)U[p - 1]) / g.dxmin + ((double)V[p] - (double)V[p - g.sy]) / g.dymin +
-            ((double)W[p] - (double)W[p - g.sz]) / dz;
-        if (Q) r = r - Q[p];
-        r = r * dt2_rec;
-        if (rhs_out) rhs_out[p] = r;
+  C2<R> *sh = tw + twlen(N);
+  load_twiddles(tw, pp.twx, N);
+  const int ls = fspan(N, padmode), M = pp.M;
+  const int row0 = blockIdx.x * ROWS;
+  const int nrows = pp.ny * pp.nz;
+  auto load = [&](int l, int m) -> C2<R> {
+    const int row = min(row0 + l, nrows - 1);
+    const int j = (row & (pp.ny - 1)) + 1, k = (row >> logny) + 1, i = m + 1;
+    const long p = gix(g, i, j, k);
+    double r;
+    if constexpr (MODE == kRhsResid) {
+      r = fd2_residual(g, phi, rhs_in, pp.az, pp.cz, i, j, k, p);
+    } else if constexpr (MODE == kRhsIn) {
+      r = rhs_in[p];
+    } else {
+      const double dw = (double)W[p] - (double)W[p - g.sz];
+      r = div_rn((double)U[p] - (double)U[p - 1], g.dxmin, g.rdx) +
+          div_rn((double)V[p] - (double)V[p - g.sy], g.dymin, g.rdy) +
+          (g.variable_z ? dw / g.dzPr[k] : div_rn(dw, g.dzmin, g.rdz));
+      if constexpr ((MODE & kRhsQ) != 0) r = r - Q[p];
+      r = r * dt2_rec;
+      if constexpr ((MODE & kRhsOut) != 0) {
+        if (row0 + l < nrows) rhs_out[p] = r;
       }
     }
-    sh[l * ls + fpad(bitrev(m, logn))] = mk2((R)r, (R)0);
-  }
-  __syncthreads();
-  lds_fft(sh, nx, logn, rows_per_block, ls, tw, false);
-  const int M = pp.M;
-  for (int t = threadIdx.x; t < rows_per_block * M; t += blockDim.x) {
-    const int l = t / M, m = t % M;
+    return mk2((R)r, (R)0);
+  };
+  auto store = [&](int l, int m, C2<R> v) {
     const int row = row0 + l;
-    if (row < nrows) pp.X[(long)row * M + m] = sh[l * ls + fpad(m)];
-  }
+    if (row < nrows && m < M) pp.X[(long)row * M + m] = v;
+  };
+  fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, false, load, store);
 }
 
 // LDS line stride of the y pass: odd, so that the TX lines touched by one
 // global row segment (consecutive lanes, one element each) fall in distinct
 // banks.
-__host__ __device__ inline int ystride(int ny) { return (ny + ny / 8) | 1; }
+__host__ __device__ inline int ystride(int ny, int padmode) { return fspan(ny, padmode) | 1; }
 
 // Passes 2 and 4: FFT along y of TX consecutive x-modes for one z-plane.
-template <int TX, typename R>
-__global__ void k_fft_y(PoissonPlan<R> pp, int logn, int inverse) {
+template <int TX, typename R, int LOGN>
+__global__ void __launch_bounds__(kFftThreads) k_fft_y(PoissonPlan<R> pp, int inverse, int padmode) {
+  constexpr int N = 1 << LOGN;
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
-  const int ny = pp.ny, M = pp.M, ls = ystride(ny);
-  C2<R> *sh = tw + twlen(ny);
-  load_twiddles(tw, pp.twy, ny);
+  C2<R> *sh = tw + twlen(N);
+  load_twiddles(tw, pp.twy, N);
+  const int M = pp.M, ls = ystride(N, padmode);
   const int m0 = blockIdx.x * TX, k = blockIdx.y;
-  C2<R> *X = pp.X + (long)k * ny * M;
-  for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
-    const int c = t % TX, j = t / TX;
+  C2<R> *X = pp.X + (long)k * N * M;
+  auto load = [&](int c, int j) -> C2<R> {  // branch-free: modes past M load M-1
     const int m = m0 + c;
-    C2<R> v = m < M ? X[(long)j * M + m] : mk2((R)0, (R)0);
-    sh[c * ls + fpad(bitrev(j, logn))] = v;
-  }
-  __syncthreads();
-  lds_fft(sh, ny, logn, TX, ls, tw, inverse != 0);
-  for (int t = threadIdx.x; t < TX * ny; t += blockDim.x) {
-    const int c = t % TX, j = t / TX;
+    const C2<R> v = X[(long)j * M + min(m, M - 1)];
+    return m < M ? v : mk2((R)0, (R)0);
+  };
+  auto store = [&](int c, int j, C2<R> v) {
     const int m = m0 + c;
-    if (m < M) X[(long)j * M + m] = sh[c * ls + fpad(j)];
-  }
+    if (m < M) X[(long)j * M + m] = v;
+  };
+  fft_lines<LOGN, TX, kFftThreads, false, false>(sh, ls, padmode, pp.twy, tw, inverse != 0, load, store);
 }
 
 // Pass 3: per (m, j) mode, a_k x_{k-1} + (lam - a_k - c_k) x_k + c_k x_{k+1}
@@ -315,43 +371,34 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
 // Pass 5: inverse x FFT from the half spectrum (Hermitian symmetry), real
 // part scaled by 1/(nx*ny) -> Phi interior (FP64); with accumulate the
 // result is added to Phi (iterative-refinement correction).
-template <typename R>
-__global__ void k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, int accumulate, int rows_per_block,
-                            int logn) {
+template <typename R, int LOGN, int ROWS>
+__global__ void __launch_bounds__(kFftThreads)
+    k_fft_x_inv(Geo g, PoissonPlan<R> pp, double *__restrict__ Phi, int accumulate, int logny, int padmode) {
+  constexpr int N = 1 << LOGN, NT = xthreads(LOGN, ROWS);
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
-  const int nx = pp.nx, M = pp.M, ls = fpad(nx);
-  C2<R> *sh = tw + twlen(nx);
-  load_twiddles(tw, pp.twx, nx);
-  const int row0 = blockIdx.x * rows_per_block;
+  C2<R> *sh = tw + twlen(N);
+  load_twiddles(tw, pp.twx, N);
+  const int ls = fspan(N, padmode), M = pp.M;
+  const int row0 = blockIdx.x * ROWS;
   const int nrows = pp.ny * pp.nz;
-  for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
-    const int l = t / nx, m = t % nx;
-    const int row = row0 + l;
-    C2<R> v = mk2((R)0, (R)0);
-    if (row < nrows) {
-      if (m < M) {
-        v = pp.X[(long)row * M + m];
-      } else {
-        v = pp.X[(long)row * M + (nx - m)];
-        v.y = -v.y;
-      }
-    }
-    sh[l * ls + fpad(bitrev(m, logn))] = v;
-  }
-  __syncthreads();
-  lds_fft(sh, nx, logn, rows_per_block, ls, tw, true);
-  const R scale = (R)1.0 / ((R)pp.nx * pp.ny);
-  for (int t = threadIdx.x; t < rows_per_block * nx; t += blockDim.x) {
-    const int l = t / nx, m = t % nx;
+  const R scale = (R)1.0 / ((R)N * pp.ny);
+  auto load = [&](int l, int m) -> C2<R> {  // branch-free; rows past the end load row nrows-1
+    const int row = min(row0 + l, nrows - 1);
+    C2<R> v = pp.X[(long)row * M + (m < M ? m : N - m)];
+    if (m >= M) v.y = -v.y;
+    return v;
+  };
+  auto store = [&](int l, int m, C2<R> v) {
     const int row = row0 + l;
     if (row < nrows) {
-      const int j = row % pp.ny + 1, k = row / pp.ny + 1;
+      const int j = (row & (pp.ny - 1)) + 1, k = (row >> logny) + 1;
       const long q = gix(g, m + 1, j, k);
-      const double val = (double)(sh[l * ls + fpad(m)].x * scale);
+      const double val = (double)(v.x * scale);
       Phi[q] = accumulate ? Phi[q] + val : val;
     }
-  }
+  };
+  fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, true, load, store);
 }
 
 static int ilog2(int n) {
@@ -360,19 +407,11 @@ static int ilog2(int n) {
   return l;
 }
 
-// Rows per block of the x passes (ELMM_FFTX_ROWS overrides; default 2048/nx)
-// and the block size: one thread per 8-element butterfly set of the rows,
-// at most 256.
-static int rows_for(int nx) {
-  static int env = -1;
-  if (env < 0) {
-    const char *e = std::getenv("ELMM_FFTX_ROWS");
-    env = e ? std::max(0, std::atoi(e)) : 0;
-  }
-  return env > 0 ? env : std::max(1, 2048 / nx);
-}
-static int xthreads(int nx, int rows) { return std::max(64, std::min(256, rows * std::max(1, nx / 8))); }
-static int padded(int n) { return n + n / 8; }
+// LDS padding of the lines (fpad's padmode), measured per pass on the
+// 512-point lines: the forward x pass is fastest with one pad slot per 8
+// elements, the inverse x pass with one more per 64 (-9 %), the y pass is the
+// same with either.
+constexpr int kPadXFwd = 0, kPadXInv = 1, kPadY = 0;
 
 // Dynamic LDS above 64 KiB must be opted into per kernel (gfx950 has 160 KiB
 // per workgroup).
@@ -380,47 +419,65 @@ static void allow_lds(const void *fn, size_t bytes) {
   if (bytes > 65536) (void)lds_limit_raise(fn, bytes);
 }
 
+// Runs f(std::integral_constant<int, L>{}) for L = logn (0..10: the engine
+// admits power-of-two nx, ny <= 1024).
+template <typename F>
+static void with_logn(int logn, F f) {
+  switch (logn) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 9: f(std::integral_constant<int, 9>{}); break;
+    default: f(std::integral_constant<int, 10>{}); break;
+  }
+}
+
 template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
                       const double *rhs, const double *phi, double dt2_rec, double *rhs_out, hipStream_t st) {
-  const int rows = rows_for(pp.nx);
-  const int nrows = pp.ny * pp.nz;
-  size_t sh = ((size_t)rows * padded(pp.nx) + twlen(pp.nx)) * sizeof(C2<R>);
-  allow_lds((const void *)k_fft_x_fwd<T, R>, sh);
-  hipLaunchKernelGGL((k_fft_x_fwd<T, R>), dim3(nblk(nrows, rows)), dim3(xthreads(pp.nx, rows)), sh, st, g, pp, U, V, W,
-                     Q, rhs, phi, dt2_rec,
-                     rhs_out, rows, ilog2(pp.nx));
+  const int pm = kPadXFwd, nrows = pp.ny * pp.nz;
+  const int mode = phi ? kRhsResid : rhs ? kRhsIn : (Q ? kRhsQ : 0) | (rhs_out ? kRhsOut : 0);
+  with_logn(ilog2(pp.nx), [&](auto L) {
+    constexpr int LOGN = decltype(L)::value, ROWS = xrows(LOGN);
+    const size_t sh = ((size_t)ROWS * fspan(1 << LOGN, pm) + twlen(1 << LOGN)) * sizeof(C2<R>);
+    auto go = [&](auto MD) {
+      constexpr int MODE = decltype(MD)::value;
+      allow_lds((const void *)k_fft_x_fwd<T, R, LOGN, MODE, ROWS>, sh);
+      hipLaunchKernelGGL((k_fft_x_fwd<T, R, LOGN, MODE, ROWS>), dim3(nblk(nrows, ROWS)), dim3(xthreads(LOGN, ROWS)),
+                         sh, st, g, pp, U, V, W, Q, rhs, phi, dt2_rec, rhs_out, ilog2(pp.ny), pm);
+    };
+    switch (mode) {
+      case kRhsResid: go(std::integral_constant<int, kRhsResid>{}); break;
+      case kRhsIn: go(std::integral_constant<int, kRhsIn>{}); break;
+      case kRhsQ: go(std::integral_constant<int, kRhsQ>{}); break;
+      case kRhsOut: go(std::integral_constant<int, kRhsOut>{}); break;
+      case kRhsQ | kRhsOut: go(std::integral_constant<int, kRhsQ | kRhsOut>{}); break;
+      default: go(std::integral_constant<int, kRhsDiv>{}); break;
+    }
+  });
 }
 
-template <int TX, typename R>
-static void launch_fft_y_tx(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
-  const size_t sh = ((size_t)TX * ystride(pp.ny) + twlen(pp.ny)) * sizeof(C2<R>);
-  allow_lds((const void *)k_fft_y<TX, R>, sh);
-  hipLaunchKernelGGL((k_fft_y<TX, R>), dim3(nblk(pp.M, TX), pp.nz), dim3(256), sh, st, pp, ilog2(pp.ny), inverse);
-}
-
-// modes per block: 8 by default (128-byte FP64 / 64-byte FP32 row segments;
-// the smaller FP32 tile doubles the blocks resident per CU, measured 25 %
-// faster than 16); ELMM_FFTY_TX=4, 8 or 16 selects another width
-static int ffty_tx_choice() {
-  static int tx = -1;
-  if (tx < 0) {
-    const char *e = std::getenv("ELMM_FFTY_TX");
-    tx = e ? std::atoi(e) : 0;
-    if (tx != 4 && tx != 8 && tx != 16) tx = 0;
-  }
-  return tx;
-}
+// Modes per block of the y pass: 8 (128-byte FP64 row segments; 4 measured
+// 30 % slower, 16 slower still: its LDS tile halves the blocks resident per
+// CU).
+constexpr int kFftYModes = 8;
 
 template <typename R>
 void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
-  const int tx = ffty_tx_choice() ? ffty_tx_choice() : 8;
-  if (tx == 4)
-    launch_fft_y_tx<4, R>(pp, inverse, st);
-  else if (tx == 8)
-    launch_fft_y_tx<8, R>(pp, inverse, st);
-  else
-    launch_fft_y_tx<16, R>(pp, inverse, st);
+  const int pm = kPadY;
+  with_logn(ilog2(pp.ny), [&](auto L) {
+    constexpr int LOGN = decltype(L)::value, TX = kFftYModes;
+    const size_t sh = ((size_t)TX * ystride(1 << LOGN, pm) + twlen(1 << LOGN)) * sizeof(C2<R>);
+    allow_lds((const void *)k_fft_y<TX, R, LOGN>, sh);
+    hipLaunchKernelGGL((k_fft_y<TX, R, LOGN>), dim3(nblk(pp.M, TX), pp.nz), dim3(kFftThreads), sh, st, pp, inverse,
+                       pm);
+  });
 }
 
 template <typename R>
@@ -432,13 +489,14 @@ void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t s
 
 template <typename R>
 void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st) {
-  const int rows = rows_for(pp.nx);
-  const int nrows = pp.ny * pp.nz;
-  size_t sh = ((size_t)rows * padded(pp.nx) + twlen(pp.nx)) * sizeof(C2<R>);
-  allow_lds((const void *)k_fft_x_inv<R>, sh);
-  hipLaunchKernelGGL(k_fft_x_inv<R>, dim3(nblk(nrows, rows)), dim3(xthreads(pp.nx, rows)), sh, st, g, pp, Phi,
-                     accumulate, rows,
-                     ilog2(pp.nx));
+  const int pm = kPadXInv, nrows = pp.ny * pp.nz;
+  with_logn(ilog2(pp.nx), [&](auto L) {
+    constexpr int LOGN = decltype(L)::value, ROWS = xrows(LOGN);
+    const size_t sh = ((size_t)ROWS * fspan(1 << LOGN, pm) + twlen(1 << LOGN)) * sizeof(C2<R>);
+    allow_lds((const void *)k_fft_x_inv<R, LOGN, ROWS>, sh);
+    hipLaunchKernelGGL((k_fft_x_inv<R, LOGN, ROWS>), dim3(nblk(nrows, ROWS)), dim3(xthreads(LOGN, ROWS)), sh, st, g,
+                       pp, Phi, accumulate, ilog2(pp.ny), pm);
+  });
 }
 
 // ------------------------------------------------------------ reductions
