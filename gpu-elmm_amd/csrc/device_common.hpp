@@ -27,6 +27,7 @@ struct Geo {
   double rdx, rdy, rdz;
   float rdxf, rdyf, rdzf;
   int variable_z;
+  int sgs_cellwise;   // one thread per cell SGS kernel instead of the z-march
   int bt[6];          // velocity boundary types
   int sbt[6];         // scalar boundary types
   double side_scal[6];

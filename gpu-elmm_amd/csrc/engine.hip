@@ -300,6 +300,8 @@ void Engine<T>::setup_geo() {
   {
     const char *m = std::getenv("ELMM_MOMENTUM");
     fused_momentum = momentum_zm_supported(g) && !(m && std::string(m) == "percomp");
+    const char *sg = std::getenv("ELMM_SGS");
+    g.sgs_cellwise = sg && std::string(sg) == "cellwise";
     const char *sk = std::getenv("ELMM_SCALAR");
     zmarch_scalar = !g.variable_z && !(sk && std::string(sk) == "cellwise");
     const char *gv = std::getenv("ELMM_GRAPH");
@@ -425,7 +427,7 @@ void Engine<T>::setup_poisson() {
                                g.dzW};
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
-  cfl_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
+  cfl_blocks = sgs_partials(g, sgs_width);
   partial.alloc((std::max(top_blocks, 1) + 1) * sizeof(double));
   div_partial.alloc(std::max(red_blocks, 1) * sizeof(double));
   cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));

@@ -21,6 +21,104 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 // ------------------------------------------------------------- Smagorinsky
 // NuSmag/StrainIJ/Strainu, src/subgrid.f90:55-67, 170-209.  T is the field
 // and arithmetic type (double: the reference's FP64; float: precision form).
+// Uf(di, dj, dk) etc. return the field at (i+di, j+dj, k+dk); both SGS
+// kernels evaluate the same expressions in the same order.
+template <typename T, typename FU, typename FV, typename FW>
+__device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width, T nu, T c_smag) {
+  const T dx = (T)g.dxmin, dy = (T)g.dymin;
+  const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
+  if constexpr (sizeof(T) == 4) {
+    // precision form: products with reciprocals (two divisions per cell
+    // instead of nine) and the hardware square root; within a few ulp of
+    // the quotient form
+    const float rdx = g.rdxf != 0.f ? g.rdxf : 1.f / dx, rdy = g.rdyf != 0.f ? g.rdyf : 1.f / dy;
+    const float rdz = 1.f / (float)g.dzPr[k], rz2 = 1.f / (2 * (zp1 - zm1));
+    const float r4x = 0.25f * rdx, r4y = 0.25f * rdy;
+    float D00 = (Uf(0, 0, 0) - Uf(-1, 0, 0)) * rdx;
+    float D11 = (Vf(0, 0, 0) - Vf(0, -1, 0)) * rdy;
+    float D22 = (Wf(0, 0, 0) - Wf(0, 0, -1)) * rdz;
+    float D01 = (Uf(0, 1, 0) + Uf(-1, 1, 0) - Uf(0, -1, 0) - Uf(-1, -1, 0)) * r4y;
+    float D02 = (Uf(0, 0, 1) + Uf(-1, 0, 1) - Uf(0, 0, -1) - Uf(-1, 0, -1)) * rz2;
+    float D10 = (Vf(1, 0, 0) + Vf(1, -1, 0) - Vf(-1, 0, 0) - Vf(-1, -1, 0)) * r4x;
+    float D12 = (Vf(0, 0, 1) + Vf(0, -1, 1) - Vf(0, 0, -1) - Vf(0, -1, -1)) * rz2;
+    float D20 = (Wf(1, 0, 0) + Wf(1, 0, -1) - Wf(-1, 0, 0) - Wf(-1, 0, -1)) * r4x;
+    float D21 = (Wf(0, 1, 0) + Wf(0, 1, -1) - Wf(0, -1, 0) - Wf(0, -1, -1)) * r4y;
+    const float S01 = (D01 + D10) * 0.5f, S02 = (D02 + D20) * 0.5f, S12 = (D12 + D21) * 0.5f;
+    const float s = D00 * D00 + D11 * D11 + D22 * D22 + 2.f * (S01 * S01 + S02 * S02 + S12 * S12);
+    const float Sbar = __builtin_amdgcn_sqrtf(2.f * s);
+    const float wc = width * c_smag;
+    float v = Sbar * (wc * wc);
+    if (nu > 0) v = v + nu;
+    return v;
+  }
+  T D00 = (Uf(0, 0, 0) - Uf(-1, 0, 0)) / dx;
+  T D11 = (Vf(0, 0, 0) - Vf(0, -1, 0)) / dy;
+  T D22 = (Wf(0, 0, 0) - Wf(0, 0, -1)) / (T)g.dzPr[k];
+  T D01 = (Uf(0, 1, 0) + Uf(-1, 1, 0) - Uf(0, -1, 0) - Uf(-1, -1, 0)) / (4 * dy);
+  T D02 = (Uf(0, 0, 1) + Uf(-1, 0, 1) - Uf(0, 0, -1) - Uf(-1, 0, -1)) / (2 * (zp1 - zm1));
+  T D10 = (Vf(1, 0, 0) + Vf(1, -1, 0) - Vf(-1, 0, 0) - Vf(-1, -1, 0)) / (4 * dx);
+  T D12 = (Vf(0, 0, 1) + Vf(0, -1, 1) - Vf(0, 0, -1) - Vf(0, -1, -1)) / (2 * (zp1 - zm1));
+  T D20 = (Wf(1, 0, 0) + Wf(1, 0, -1) - Wf(-1, 0, 0) - Wf(-1, 0, -1)) / (4 * dx);
+  T D21 = (Wf(0, 1, 0) + Wf(0, 1, -1) - Wf(0, -1, 0) - Wf(0, -1, -1)) / (4 * dy);
+  const T D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
+  T s = 0;
+#pragma unroll
+  for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+      T Sij = (D[ii][jj] + D[jj][ii]) / 2;
+      s = s + Sij * Sij;
+    }
+  T Sbar = sqrt((T)2.0 * s);
+  T wc = width * c_smag;
+  T v = Sbar * (wc * wc);
+  if (nu > 0) v = v + nu;
+  return v;
+}
+
+// Filter width of column (i, j) at level k: table from the host libm (the
+// cube root of a static geometric factor per column class and level).
+// TabOnly: the table is known to be present (no pow code in the kernel).
+template <typename T, bool TabOnly = false>
+__device__ inline T smag_width(const Geo &g, const SgsWidth &w, double filter_ratio, int i, int j, int k) {
+  if (TabOnly || w.tab) {
+    const int cls = w.idx ? w.idx[(i - 1) + (long)g.nx * (j - 1)] : 0;
+    return (T)w.tab[cls * w.ld + k];
+  }
+  return (T)(filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0));
+}
+
+// TimeStepLength's U_dx_max term of one cell, src/dynamics.f90:556-594
+// (NaN -> +inf); u1 = U(i), u0 = U(i-1), v1/v0 over j, w1/w0 over k.
+__device__ inline double cfl_cell(const Geo &g, double u1, double u0, double v1, double v0, double w1, double w0,
+                                  int i, int j, int k) {
+  double a1 = fabs(u1), a0 = fabs(u0);
+  double b1 = fabs(v1), b0 = fabs(v0);
+  double c1 = fabs(w1), c0 = fabs(w0);
+  double dxk = g.variable_z ? g.dxPr[i] : g.dxmin;
+  double dyk = g.variable_z ? g.dyPr[j] : g.dymin;
+  double dzk = g.variable_z ? g.dzPr[k] : g.dzmin;
+  double q = (a1 > a0 ? a1 : a0) / dxk;
+  q = q + (b1 > b0 ? b1 : b0) / dyk;
+  q = q + (c1 > c0 ? c1 : c0) / dzk;
+  return (q != q) ? INFINITY : q;
+}
+
+// Block maximum of v (blockDim.x a multiple of 64, at most 1024 threads);
+// thread 0 holds the result.
+__device__ inline double block_max_cfl(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  __shared__ double red[16];
+  const int lane = threadIdx.x + blockDim.x * threadIdx.y;
+  if ((lane & 63) == 0) red[lane >> 6] = v;
+  __syncthreads();
+  double m = 0;
+  if (lane == 0)
+    for (int w = 0; w < (int)(blockDim.x * blockDim.y) / 64; w++) m = fmax(m, red[w]);
+  return m;
+}
+
+// One thread per cell (the stretched-z path and ELMM_SGS=cellwise).
 template <typename T>
 __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, const T *__restrict__ V,
                                              const T *__restrict__ W, T *__restrict__ Visc, double nu_,
@@ -34,79 +132,142 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
   if (i <= g.nx && j <= g.ny) {
     const long p = gix(g, i, j, k);
     const long sy = g.sy, sz = g.sz;
-    T v;
-    if (sgs_model == 1) {
-      const T dx = (T)g.dxmin, dy = (T)g.dymin;
-      const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
-      T D00 = (U[p] - U[p - 1]) / dx;
-      T D11 = (V[p] - V[p - sy]) / dy;
-      T D22 = (W[p] - W[p - sz]) / (T)g.dzPr[k];
-      T D01 = (U[p + sy] + U[p - 1 + sy] - U[p - sy] - U[p - 1 - sy]) / (4 * dy);
-      T D02 = (U[p + sz] + U[p - 1 + sz] - U[p - sz] - U[p - 1 - sz]) / (2 * (zp1 - zm1));
-      T D10 = (V[p + 1] + V[p + 1 - sy] - V[p - 1] - V[p - 1 - sy]) / (4 * dx);
-      T D12 = (V[p + sz] + V[p - sy + sz] - V[p - sz] - V[p - sy - sz]) / (2 * (zp1 - zm1));
-      T D20 = (W[p + 1] + W[p + 1 - sz] - W[p - 1] - W[p - 1 - sz]) / (4 * dx);
-      T D21 = (W[p + sy] + W[p + sy - sz] - W[p - sy] - W[p - sy - sz]) / (4 * dy);
-      const T D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
-      T s = 0;
+    auto Uf = [&](int di, int dj, int dk) { return U[p + di + dj * sy + dk * sz]; };
+    auto Vf = [&](int di, int dj, int dk) { return V[p + di + dj * sy + dk * sz]; };
+    auto Wf = [&](int di, int dj, int dk) { return W[p + di + dj * sy + dk * sz]; };
+    Visc[p] = sgs_model == 1 ? smag_visc<T>(g, Uf, Vf, Wf, k, smag_width<T>(g, width_t, filter_ratio, i, j, k), nu,
+                                            c_smag)
+                             : nu;
+    if (cfl_partial) cfl = cfl_cell(g, U[p], U[p - 1], V[p], V[p - sy], W[p], W[p - sz], i, j, k);
+  }
+  if (cfl_partial) {
+    const double m = block_max_cfl(cfl);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      cfl_partial[(long)blockIdx.z * gridDim.y * gridDim.x + blockIdx.y * gridDim.x + blockIdx.x] = m;
+  }
+}
+
+// z-marching form: a block owns a TX x TY tile of columns and walks k up a
+// chunk of levels.  LDS holds rings of (TX+2) x (TY+2) planes (halo 1): U
+// and V at k-1..k+1, W at k-1..k; the planes of level k+1 are loaded into
+// registers while level k is evaluated, then written into the slots level k
+// no longer needs.  Global loads per cell: one plane of U, V and W plus the
+// halo ring, instead of 21 stencil values.
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY)
+    k_sgs_zm(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W, T *__restrict__ Visc,
+             double nu_, double c_smag_, double filter_ratio, SgsWidth width_t, int sgs_model,
+             double *__restrict__ cfl_partial, int zchunk) {
+  constexpr int NT = TX * TY, RX = TX + 2, RC = RX * (TY + 2), PER = (RC + NT - 1) / NT;
+  __shared__ T sU[3][RC], sV[3][RC], sW[2][RC];
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+  const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
+  const int kbeg = blockIdx.z * zchunk + 1, kend = min(g.nz, kbeg + zchunk - 1);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool inside = i <= g.nx && j <= g.ny;
+  const T nu = (T)nu_, c_smag = (T)c_smag_;
+  // region cells of this thread (halo 1 around the tile), clamped into the
+  // allocation; values of clamped cells only reach cells outside the domain
+  long roff[PER];
 #pragma unroll
-      for (int jj = 0; jj < 3; jj++)
+  for (int q = 0; q < PER; q++) {
+    const int r = min(t + q * NT, RC - 1);
+    const int ri = min(i0 - 1 + r % RX, g.nx + 3), rj = min(j0 - 1 + r / RX, g.ny + 3);
+    roff[q] = gix(g, ri, rj, 0);
+  }
+  auto put = [&](T *dst, const T *src, int k) {
 #pragma unroll
-        for (int ii = 0; ii < 3; ii++) {
-          T Sij = (D[ii][jj] + D[jj][ii]) / 2;
-          s = s + Sij * Sij;
-        }
-      T Sbar = sqrt((T)2.0 * s);
-      // filter width: table from the host libm (the cube root of a static
-      // geometric factor per column class and level)
-      T width;
-      if (width_t.tab) {
-        const int cls = width_t.idx ? width_t.idx[(i - 1) + (long)g.nx * (j - 1)] : 0;
-        width = (T)width_t.tab[cls * width_t.ld + k];
-      } else {
-        width = (T)(filter_ratio * pow(g.dxPr[i] * g.dyPr[j] * g.dzPr[k], 1.0 / 3.0));
+    for (int q = 0; q < PER; q++)
+      if (t + q * NT < RC) dst[t + q * NT] = src[roff[q] + (long)k * g.sz];
+  };
+  put(sU[0], U, kbeg - 1);
+  put(sU[1], U, kbeg);
+  put(sU[2], U, kbeg + 1);
+  put(sV[0], V, kbeg - 1);
+  put(sV[1], V, kbeg);
+  put(sV[2], V, kbeg + 1);
+  put(sW[0], W, kbeg - 1);
+  put(sW[1], W, kbeg);
+  __syncthreads();
+  int u0 = 0, u1 = 1, u2 = 2, w0 = 0, w1 = 1;
+  const int c = (tx + 1) + RX * (ty + 1);
+  double cfl = 0;
+  for (int k = kbeg; k <= kend; k++) {
+    const bool more = k < kend;
+    T pu[PER], pv[PER], pw[PER];
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const long o = roff[q] + (long)(k + 2) * g.sz;
+        pu[q] = U[o];
+        pv[q] = V[o];
+        pw[q] = W[o - g.sz];
       }
-      T wc = width * c_smag;
-      v = Sbar * (wc * wc);
-      if (nu > 0) v = v + nu;
-    } else {
-      v = nu;
     }
-    Visc[p] = v;
-    if (cfl_partial) {
-      // TimeStepLength's U_dx_max, src/dynamics.f90:556-594 (NaN -> +inf)
-      double a1 = fabs((double)U[p]), a0 = fabs((double)U[p - 1]);
-      double b1 = fabs((double)V[p]), b0 = fabs((double)V[p - sy]);
-      double c1 = fabs((double)W[p]), c0 = fabs((double)W[p - sz]);
-      double dxk = g.variable_z ? g.dxPr[i] : g.dxmin;
-      double dyk = g.variable_z ? g.dyPr[j] : g.dymin;
-      double dzk = g.variable_z ? g.dzPr[k] : g.dzmin;
-      double q = (a1 > a0 ? a1 : a0) / dxk;
-      q = q + (b1 > b0 ? b1 : b0) / dyk;
-      q = q + (c1 > c0 ? c1 : c0) / dzk;
-      cfl = (q != q) ? INFINITY : q;
+    if (inside) {
+      const T *Ul = sU[u0], *Uc = sU[u1], *Uh = sU[u2];
+      const T *Vl = sV[u0], *Vc = sV[u1], *Vh = sV[u2];
+      const T *Wl = sW[w0], *Wc = sW[w1];
+      auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
+      auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
+      auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
+      Visc[gix(g, i, j, k)] = sgs_model == 1 ? smag_visc<T>(g, Uf, Vf, Wf, k,
+                                                            smag_width<T, true>(g, width_t, filter_ratio, i, j, k),
+                                                            nu, c_smag)
+                                             : nu;
+      if (cfl_partial)
+        cfl = fmax(cfl, cfl_cell(g, Uc[c], Uc[c - 1], Vc[c], Vc[c - RX], Wc[c], Wl[c], i, j, k));
+    }
+    if (more) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PER; q++)
+        if (t + q * NT < RC) {
+          sU[u0][t + q * NT] = pu[q];
+          sV[u0][t + q * NT] = pv[q];
+          sW[w0][t + q * NT] = pw[q];
+        }
+      const int tu = u0;
+      u0 = u1;
+      u1 = u2;
+      u2 = tu;
+      const int tw = w0;
+      w0 = w1;
+      w1 = tw;
+      __syncthreads();
     }
   }
   if (cfl_partial) {
-    // wavefront max via cross-lane shuffles, then LDS across the 4 waves
-    for (int off = 32; off > 0; off >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, off));
-    __shared__ double red[4];
-    const int lane = (threadIdx.y * blockDim.x + threadIdx.x);
-    if ((lane & 63) == 0) red[lane >> 6] = cfl;
-    __syncthreads();
-    if (lane == 0) {
-      double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      cfl_partial[(long)blockIdx.z * gridDim.y * gridDim.x + blockIdx.y * gridDim.x + blockIdx.x] = m;
-    }
+    const double m = block_max_cfl(cfl);
+    if (t == 0) cfl_partial[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
   }
+}
+
+// 64 x 4 columns, 32 levels per block (16 measured equal, 64 12 % slower in
+// FP32: fewer blocks to hide the per-level load latency)
+constexpr int kSgsTX = 64, kSgsTY = 4, kSgsZChunk = 32;
+
+// The z-march needs the filter-width table (present whenever the products
+// dxPr(i) dyPr(j) take at most 255 values, always on uniform x-y spacing).
+static bool sgs_cellwise(const Geo &g, const SgsWidth &w) { return g.sgs_cellwise || g.variable_z || !w.tab; }
+
+int sgs_partials(const Geo &g, const SgsWidth &w) {
+  if (sgs_cellwise(g, w)) return (int)(nblk(g.nx, 64) * nblk(g.ny, 4) * g.nz);
+  return (int)(nblk(g.nx, kSgsTX) * nblk(g.ny, kSgsTY) * nblk(g.nz, kSgsZChunk));
 }
 
 template <typename T>
 void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
                 double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, int, hipStream_t st) {
-  dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
-  hipLaunchKernelGGL(k_sgs<T>, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width, sgs_model,
-                     cfl_partial);
+  if (sgs_cellwise(g, width)) {
+    dim3 b(64, 4, 1), gr(nblk(g.nx, 64), nblk(g.ny, 4), g.nz);
+    hipLaunchKernelGGL(k_sgs<T>, gr, b, 0, st, g, U, V, W, Visc, nu, c_smag, filter_ratio, width, sgs_model,
+                       cfl_partial);
+    return;
+  }
+  dim3 gr(nblk(g.nx, kSgsTX), nblk(g.ny, kSgsTY), nblk(g.nz, kSgsZChunk));
+  hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc, nu, c_smag,
+                     filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
 }
 
 // ------------------------------------------------------------- wall model

@@ -53,6 +53,8 @@ struct SgsWidth {
 };
 
 // momentum (k_momentum.hip); T = field type (double or float)
+// Number of CFL partial maxima launch_sgs writes (one per block).
+int sgs_partials(const Geo &g, const SgsWidth &w);
 template <typename T>
 void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
                 double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, int cfl_blocks,

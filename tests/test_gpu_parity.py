@@ -125,12 +125,13 @@ def test_immersed_boundary_case_parity(require_gpu):
     assert max(err.values()) <= 1e-11, err
 
 
-@pytest.mark.parametrize("env", [{"ELMM_MOMENTUM": "percomp"}, {"ELMM_ZCHUNK": "5"}, {"ELMM_SCALAR": "cellwise"}])
+@pytest.mark.parametrize("env", [{"ELMM_MOMENTUM": "percomp"}, {"ELMM_ZCHUNK": "5"}, {"ELMM_SCALAR": "cellwise"},
+                                 {"ELMM_SGS": "cellwise"}])
 def test_momentum_kernel_variants_bit_identical(require_gpu, monkeypatch, env):
-    """The per-component momentum kernels, the one-thread-per-cell scalar
-    kernel and the z-marching kernels with short z chunks (chunk boundaries
-    inside the domain) give the same bits as the oracle on the obstacle
-    case."""
+    """The per-component momentum kernels, the one-thread-per-cell scalar and
+    SGS kernels and the z-marching kernels with short z chunks (chunk
+    boundaries inside the domain) give the same bits as the oracle on the
+    obstacle case."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     cfg = cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1)
