@@ -231,8 +231,11 @@ struct Engine : EngineIface {
   bool fused_momentum = false;
   int zchunk = 64;
   // scalar kernel: z-marching on uniform grids (ELMM_SCALAR=cellwise selects
-  // the one-thread-per-cell kernel)
+  // the one-thread-per-cell kernel), 16 levels per block (measured 7 % faster
+  // than 64 in FP32: more blocks to hide the per-level load latency;
+  // ELMM_ZCHUNK sets both chunk lengths)
   bool zmarch_scalar = false;
+  int zchunk_scalar = 16;
 
   IBDev ibdev(int c) const {
     return IBDev{ib_n[c], ib_tgt[c].as<int32_t>(), ib_ptr[c].as<int32_t>(), ib_src[c].as<int32_t>(),
@@ -307,7 +310,7 @@ void Engine<T>::setup_geo() {
     const char *gv = std::getenv("ELMM_GRAPH");
     use_graphs = !(gv && std::string(gv) == "0");
     const char *zc = std::getenv("ELMM_ZCHUNK");
-    if (zc && std::atoi(zc) > 0) zchunk = std::atoi(zc);
+    if (zc && std::atoi(zc) > 0) zchunk = zchunk_scalar = std::atoi(zc);
   }
   for (int s = 0; s < 6; s++) {
     g.bt[s] = cfg.btype[s];
@@ -958,7 +961,8 @@ void Engine<T>::stage(int st, bool last) {
       timed(ELMM_T_SCALAR, [&] {
         if (zmarch_scalar)
           launch_scalar_zm(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(), code_s.as<uint8_t>(),
-                           mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, zchunk, stream);
+                           mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, zchunk_scalar,
+                           stream);
         else
           launch_scalar(g, u, v, w, TDiff.as<T>(), s_cur[s], s_nxt[s], Sadv[s]->as<T>(), code_s.as<uint8_t>(),
                         mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc, stream);

@@ -613,8 +613,8 @@ __device__ inline T zterm_vz(const Geo &g, const T *__restrict__ nu, const T *__
 template <typename T, int C>
 __device__ inline void finish(const Geo &g, const StageConst &sc, const T *__restrict__ F,
                               const T *__restrict__ Pr, T *__restrict__ Ustar, T *__restrict__ U2,
-                              long p, int k, T w) {
-  T u2 = sc.stage > 0 ? Ustar[p] * (T)sc.rho * (T)sc.dt : (T)0.0;
+                              long p, int k, T w, T us_prev) {
+  T u2 = sc.stage > 0 ? us_prev * (T)sc.rho * (T)sc.dt : (T)0.0;
   u2 = u2 + w * (T)sc.beta * (T)sc.dt;
   if (C == 0 && sc.prgx != 0) u2 = u2 + (T)sc.A * (T)sc.prgx;
   if (C == 1 && sc.prgy != 0) u2 = u2 + (T)sc.A * (T)sc.prgy;
@@ -656,10 +656,17 @@ __global__ void __launch_bounds__(256)
   if (i > g.n[C][0] || j > g.n[C][1]) return;
   const long p = gix(g, i, j, k);
   const uint8_t cd = code[p];
-  if (cd & CODE_CORRECTED) return;
   const T *F = C == 0 ? U : C == 1 ? V : W;
   const T w = momentum_rhs<T, C, VZ>(g, U, V, W, Visc, p, k, cd);
-  finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+  const T us_prev = Ustar[p];
+  if (cd & CODE_CORRECTED) {
+    // hand-over to the ordered-correction kernel: the bulk value in Ustar,
+    // the previous stage's Ustar in U2
+    Ustar[p] = w;
+    U2[p] = us_prev;
+  } else {
+    finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w, us_prev);
+  }
 }
 
 template <typename T, int VZ>
@@ -686,7 +693,9 @@ void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W,
 
 // Cells with wall-model face-flux replacements (wmfluxes-nobranch-inc.f90:
 // 49-208; wmfluxes-variable_z-nobranch-{U,V,W}-inc.f90 on stretched z): the
-// bulk value, then the terms in the reference's scatter order.
+// bulk value (evaluated by the bulk momentum kernel and passed through
+// Ustar, the previous Ustar through U2), then the terms in the reference's
+// scatter order.
 // term = {axis | neg<<2, WM point index, face-cell linear index, face level}
 // A negative term (the cell before the face) carries fluxm, a positive one
 // fluxp.
@@ -701,7 +710,10 @@ __global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restr
   const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
   const long p = gix(g, i, j, k);
   const T *F = C == 0 ? U : C == 1 ? V : W;
-  T w = momentum_rhs<T, C, VZ>(g, U, V, W, Visc, p, k, code[p]);
+  // the bulk kernel left momentum_rhs of this cell in Ustar and the previous
+  // stage's Ustar in U2
+  T w = Ustar[p];
+  const T us_prev = U2[p];
   for (int t = tl.ptr[c]; t < tl.ptr[c + 1]; t++) {
     const int32_t *tt = &tl.term[4 * t];
     const int ax = tt[0] & 3;
@@ -717,7 +729,7 @@ __global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restr
       w = w + fp;
     }
   }
-  finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w);
+  finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w, us_prev);
 }
 
 template <typename T, int VZ>

@@ -94,7 +94,7 @@ __device__ inline T cross2(FF F, T ac, T acm) {
 
 template <typename T>
 __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr, T dprz_var, bool varz, T us_prev,
-                                   T *__restrict__ Us, T *__restrict__ U2, long p) {
+                                   T *__restrict__ Us, T *__restrict__ U2, long p, bool corr) {
   T u2 = sc.stage > 0 ? us_prev * (T)sc.rho * (T)sc.dt : (T)0;
   u2 = u2 + w * (T)sc.beta * (T)sc.dt;
   if (C == 0 && sc.prgx != 0) u2 = u2 + (T)sc.A * (T)sc.prgx;
@@ -108,8 +108,10 @@ __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr
       u2 = u2 + (T)sc.Az * dpr;
   }
   u2 = u2 + f0;
+  // a cell with ordered corrections hands over to the correction kernel: the
+  // bulk value in Ustar, the previous Ustar in U2
   Us[p] = w;
-  U2[p] = u2;
+  U2[p] = corr ? us_prev : u2;
 }
 
 template <typename T, int TX, int TY>
@@ -247,7 +249,8 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
       // ---------------- U
       if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) {
         const uint8_t cd = ccd[0];
-        if (!(cd & CODE_CORRECTED)) {
+        const bool corr = (cd & CODE_CORRECTED) != 0;
+        {
           const int cls = cd & CODE_ADV_MASK;
           T acc = 0;
           if (cls != 2) {
@@ -279,13 +282,14 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
           w = w - q4 * (at(n0, 1, 0) + at(n0, 1, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(u0, 0, -1)) * ry;
           w = w + q4 * (at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (cu[4] - cu[3]) * rz;
           w = w - q4 * (at(n0, 1, 0) + at(nm, 1, 0) + at(n0, 0, 0) + at(nm, 0, 0)) * (cu[3] - cu[2]) * rz;
-          finish_cell<T>(sc, 0, w, f0, cprx - pr0, (T)0, false, cus[0], a.Us[0], a.U2[0], p);
+          finish_cell<T>(sc, 0, w, f0, cprx - pr0, (T)0, false, cus[0], a.Us[0], a.U2[0], p, corr);
         }
       }
       // ---------------- V
       if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) {
         const uint8_t cd = ccd[1];
-        if (!(cd & CODE_CORRECTED)) {
+        const bool corr = (cd & CODE_CORRECTED) != 0;
+        {
           const int cls = cd & CODE_ADV_MASK;
           T acc = 0;
           if (cls != 2) {
@@ -317,13 +321,14 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
           w = w - at(n0, 0, 0) * (f0 - at(v0, 0, -1)) * ry;
           w = w + q4 * (at(n1, 0, 1) + at(n0, 0, 1) + at(n1, 0, 0) + at(n0, 0, 0)) * (cv[4] - cv[3]) * rz;
           w = w - q4 * (at(n0, 0, 1) + at(nm, 0, 1) + at(n0, 0, 0) + at(nm, 0, 0)) * (cv[3] - cv[2]) * rz;
-          finish_cell<T>(sc, 1, w, f0, cpry - pr0, (T)0, false, cus[1], a.Us[1], a.U2[1], p);
+          finish_cell<T>(sc, 1, w, f0, cpry - pr0, (T)0, false, cus[1], a.Us[1], a.U2[1], p, corr);
         }
       }
       // ---------------- W
       if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
         const uint8_t cd = ccd[2];
-        if (!(cd & CODE_CORRECTED)) {
+        const bool corr = (cd & CODE_CORRECTED) != 0;
+        {
           const int cls = cd & CODE_ADV_MASK;
           T acc = 0;
           if (cls != 2) {
@@ -355,7 +360,7 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
           w = w - q4 * (at(n1, 0, 0) + at(n1, 0, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(w0, 0, -1)) * ry;
           w = w + at(n1, 0, 0) * (cw[4] - f0) * rz;
           w = w - at(n0, 0, 0) * (f0 - cw[2]) * rz;
-          finish_cell<T>(sc, 2, w, f0, cprz - pr0, (T)0, false, cus[2], a.Us[2], a.U2[2], p);
+          finish_cell<T>(sc, 2, w, f0, cprz - pr0, (T)0, false, cus[2], a.Us[2], a.U2[2], p, corr);
         }
       }
     }

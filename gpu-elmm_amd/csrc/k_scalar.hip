@@ -121,9 +121,11 @@ __global__ void __launch_bounds__(256)
   if (i > g.nx || j > g.ny) return;
   const long p = gix(g, i, j, k);
   const uint8_t cd = code[p];
-  if (cd & CODE_CORRECTED) return;
   Real a = scalar_cell<Real>(g, U, V, W, T, S, mx, my, mz, p, k, cd);
-  scalar_finish<Real>(sc, S, Snew, Sadv, p, a);
+  if (cd & CODE_CORRECTED)
+    Snew[p] = a;  // bulk value for the ordered-correction kernel
+  else
+    scalar_finish<Real>(sc, S, Snew, Sadv, p, a);
 }
 
 template <typename Real>
@@ -267,7 +269,7 @@ __global__ void __launch_bounds__(NTS)
     __syncthreads();
     if (in_xy) {
       const uint8_t cd = cur.cd;
-      if (!(cd & CODE_CORRECTED)) {
+      {
         Real a = 0;
         if (cur.mxm) a = a + Ax * fX[ty][tx];
         if (mxu) a = a - Ax * fX[ty][tx + 1];
@@ -289,10 +291,14 @@ __global__ void __launch_bounds__(NTS)
         a = a - (t0 + Q[lt - TPW]) * (s0 - Sy(-1)) * Dy;
         a = a + (ct[2] + t0) * (cs[3] - s0) * Dz;
         a = a - (t0 + ct[0]) * (s0 - cs[1]) * Dz;
-        Real s2 = sc.stage > 0 ? cur.sadv * (Real)sc.rho_dt : (Real)0.0;
-        s2 = s2 + a * (Real)sc.beta_dt;
-        Sadv[p] = a;
-        Snew[p] = s0 + s2;
+        if (cd & CODE_CORRECTED) {
+          Snew[p] = a;  // bulk value for the ordered-correction kernel
+        } else {
+          Real s2 = sc.stage > 0 ? cur.sadv * (Real)sc.rho_dt : (Real)0.0;
+          s2 = s2 + a * (Real)sc.beta_dt;
+          Sadv[p] = a;
+          Snew[p] = s0 + s2;
+        }
       }
     }
     cur = nxt;
@@ -349,7 +355,9 @@ __global__ void k_scalar_corr(Geo g, const Real *__restrict__ U, const Real *__r
   if (c >= tl.n) return;
   const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
   const long p = gix(g, i, j, k);
-  Real a = scalar_cell<Real>(g, U, V, W, T, S, mx, my, mz, p, k, code[p]);
+  // the bulk kernel left scalar_cell of this cell in Snew (which receives the
+  // cell's result only here)
+  Real a = Snew[p];
   const Real Dd[3] = {1 / (2 * ((Real)g.dxmin * (Real)g.dxmin)), 1 / (2 * ((Real)g.dymin * (Real)g.dymin)),
                       1 / (2 * ((Real)g.dzmin * (Real)g.dzmin))};
   const long st[3] = {1, g.sy, g.sz};
