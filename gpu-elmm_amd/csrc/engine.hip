@@ -383,6 +383,9 @@ void Engine<T>::setup_poisson() {
   if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
   if (nz > 256) throw ArgError("z solver supports nz <= 256");
   const int M = nx / 2 + 1;
+  // spectral rows padded to 128 bytes, so that the y pass's 8-mode segments
+  // are whole cache lines
+  const int Mp = solve64 || sizeof(T) == 8 ? (M + 7) / 8 * 8 : (M + 15) / 16 * 16;
   // exp(-2 pi i q/n), evaluated in long double and rounded once to R
   auto tw = [](int n, auto one) {
     using R = decltype(one);
@@ -396,11 +399,11 @@ void Engine<T>::setup_poisson() {
   };
   using V2 = C2<T>;
   if (!solve64) {
-    X.alloc((size_t)M * ny * nz * sizeof(V2));
+    X.alloc((size_t)Mp * ny * nz * sizeof(V2));
     upload_vec(twx, tw(nx, (T)1));
     upload_vec(twy, tw(ny, (T)1));
   } else {
-    X64.alloc((size_t)M * ny * nz * sizeof(C2<double>));
+    X64.alloc((size_t)Mp * ny * nz * sizeof(C2<double>));
     upload_vec(twx64, tw(nx, 1.0));
     upload_vec(twy64, tw(ny, 1.0));
   }
@@ -422,10 +425,10 @@ void Engine<T>::setup_poisson() {
   upload_vec(az64, a);
   upload_vec(cz64, c);
   if (!solve64)
-    pp = PoissonPlan<T>{nx, ny, nz, M, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<double>(),
+    pp = PoissonPlan<T>{nx, ny, nz, M, Mp, X.as<V2>(), twx.as<V2>(), twy.as<V2>(), lamx.as<double>(),
                         lamy.as<double>(), az64.as<double>(), cz64.as<double>(), g.dzPr, g.dzW};
   else
-    pp64 = PoissonPlan<double>{nx, ny, nz, M, X64.as<C2<double>>(), twx64.as<C2<double>>(), twy64.as<C2<double>>(),
+    pp64 = PoissonPlan<double>{nx, ny, nz, M, Mp, X64.as<C2<double>>(), twx64.as<C2<double>>(), twy64.as<C2<double>>(),
                                lamx.as<double>(), lamy.as<double>(), az64.as<double>(), cz64.as<double>(), g.dzPr,
                                g.dzW};
   top_blocks = (int)((nx * ny + 255) / 256);

@@ -245,10 +245,11 @@ __global__ void __launch_bounds__(TX *TY)
 
 // 64 x 4 columns, 32 levels per block (16 measured equal, 64 12 % slower in
 // FP32: fewer blocks to hide the per-level load latency)
+// 64 x 4 columns (32 x 8 and 64 x 8 measured 3 % and 6 % slower), 32 levels
+// per block (16 measured equal, 64 12 % slower in FP32: fewer blocks to hide
+// the per-level load latency)
 constexpr int kSgsTX = 64, kSgsTY = 4, kSgsZChunk = 32;
 
-// The z-march needs the filter-width table (present whenever the products
-// dxPr(i) dyPr(j) take at most 255 values, always on uniform x-y spacing).
 static bool sgs_cellwise(const Geo &g, const SgsWidth &w) { return g.sgs_cellwise || g.variable_z || !w.tab; }
 
 int sgs_partials(const Geo &g, const SgsWidth &w) {

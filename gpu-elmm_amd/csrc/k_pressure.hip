@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(kFftThreads)
   };
   auto store = [&](int l, int m, C2<R> v) {
     const int row = row0 + l;
-    if (row < nrows && m < M) pp.X[(long)row * M + m] = v;
+    if (row < nrows && m < M) pp.X[(long)row * pp.Mp + m] = v;
   };
   fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, false, load, store);
 }
@@ -238,17 +238,17 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(PoissonPlan<R> pp, int in
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
   C2<R> *sh = tw + twlen(N);
   load_twiddles(tw, pp.twy, N);
-  const int M = pp.M, ls = ystride(N, padmode);
+  const int M = pp.M, Mp = pp.Mp, ls = ystride(N, padmode);
   const int m0 = blockIdx.x * TX, k = blockIdx.y;
-  C2<R> *X = pp.X + (long)k * N * M;
+  C2<R> *X = pp.X + (long)k * N * Mp;
   auto load = [&](int c, int j) -> C2<R> {  // branch-free: modes past M load M-1
     const int m = m0 + c;
-    const C2<R> v = X[(long)j * M + min(m, M - 1)];
+    const C2<R> v = X[(long)j * Mp + min(m, M - 1)];
     return m < M ? v : mk2((R)0, (R)0);
   };
   auto store = [&](int c, int j, C2<R> v) {
     const int m = m0 + c;
-    if (m < M) X[(long)j * M + m] = v;
+    if (m < M) X[(long)j * Mp + m] = v;
   };
   fft_lines<LOGN, TX, kFftThreads, false, false>(sh, ls, padmode, pp.twy, tw, inverse != 0, load, store);
 }
@@ -270,9 +270,14 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
   __shared__ double ck[3][ZSEG_MAX][64];
   __shared__ double gm[2][256];
   const int t = threadIdx.x;
-  const long plane = (long)pp.ny * pp.M;
+  // one thread per slot of a padded level (the padding threads return), so
+  // that a wavefront's loads are one contiguous segment (measured 5 %
+  // faster than a compact mode numbering)
+  const long plane = (long)pp.ny * pp.Mp;
   const long q = (long)blockIdx.x * 64 + t;
   if (q >= plane) return;
+  const int m = (int)(q % pp.Mp), j = (int)(q / pp.Mp);
+  if (m >= pp.M) return;
   const int nz = pp.nz;
   C2<R> *X = pp.X + q;
   if (q == 0) {
@@ -313,7 +318,6 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
     }
     return;
   }
-  const int m = (int)(q % pp.M), j = (int)(q / pp.M);
   const double lam = pp.lamx[m] + pp.lamy[j];
   // forward elimination; den = b - a c', every division a multiplication by
   // 1/den
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(kFftThreads)
   const R scale = (R)1.0 / ((R)N * pp.ny);
   auto load = [&](int l, int m) -> C2<R> {  // branch-free; rows past the end load row nrows-1
     const int row = min(row0 + l, nrows - 1);
-    C2<R> v = pp.X[(long)row * M + (m < M ? m : N - m)];
+    C2<R> v = pp.X[(long)row * pp.Mp + (m < M ? m : N - m)];
     if (m >= M) v.y = -v.y;
     return v;
   };
@@ -483,7 +487,7 @@ void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
 template <typename R>
 void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t st) {
   // nz <= ZSEG * ZSEG_MAX (256) is checked when the plan is set up
-  const long modes = (long)pp.M * pp.ny;
+  const long modes = (long)pp.Mp * pp.ny;
   hipLaunchKernelGGL(k_solve_z<R>, dim3((unsigned)((modes + 63) / 64)), dim3(64), 0, st, pp);
 }
 

@@ -115,8 +115,9 @@ template <typename R> using C2 = typename cplx_of<R>::type;
 
 template <typename R>
 struct PoissonPlan {
-  int nx, ny, nz, M;            // M = nx/2+1
-  C2<R> *X;                     // spectral work array M*ny*nz
+  int nx, ny, nz, M;            // M = nx/2+1 x-modes per row
+  int Mp;                       // row pitch: M rounded up to 128 bytes
+  C2<R> *X;                     // spectral work array Mp*ny*nz (m fastest)
   const C2<R> *twx, *twy;       // exp(-2 pi i t/N), t < N/2
   const double *lamx, *lamy;    // eigenvalues of the periodic directions (FP64 in both forms)
   const double *az, *cz;        // z coupling coefficients per k (0-based)

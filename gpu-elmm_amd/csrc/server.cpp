@@ -126,10 +126,11 @@ int elmm_footprint(const elmm_config *cfg, int64_t *bytes) {
   // 12 velocity/pressure/viscosity fields of the field type, 3 per scalar,
   // Q, Phi and RHS in FP64, the spectral array, masks and codes
   const long M = cfg->nx / 2 + 1;
+  const int es = cfg->precision == ELMM_FP32_REFINED && cfg->refine_iters > 0 ? 4 : 8;
+  const long Mp = es == 8 ? (M + 7) / 8 * 8 : (M + 15) / 16 * 16;  // 128-byte rows
   int64_t b = (int64_t)ntot * (12 * e + 3 * e * cfg->nscalars + 3 * 8 + 8);
   // spectral array: complex FP32 with refinement sweeps, complex FP64 otherwise
-  const int es = cfg->precision == ELMM_FP32_REFINED && cfg->refine_iters > 0 ? 4 : 8;
-  b += (int64_t)M * cfg->ny * cfg->nz * 2 * es;
+  b += (int64_t)Mp * cfg->ny * cfg->nz * 2 * es;
   *bytes = b;
   return 0;
 }
