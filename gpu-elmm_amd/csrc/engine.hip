@@ -262,6 +262,19 @@ struct Engine : EngineIface {
   void bound_uvw(T *a, T *b, T *c) {
     timed(ELMM_T_GHOST, [&] { launch_bound_uvw<T>(g, a, b, c, stream); });
   }
+  // BoundUVW after IBMomentum: only the ghost cells whose source is an IB
+  // point change, so only those are rewritten (lists built at set-up;
+  // ELMM_GHOST_DELTA=0 runs the full fill)
+  DevBuf gdelta;
+  int gdelta_n[3] = {0, 0, 0};
+  bool use_gdelta = false;
+  void bound_uvw_after_ib(T *a, T *b, T *c) {
+    if (use_gdelta)
+      timed(ELMM_T_GHOST, [&] { launch_ghost_delta<T>(a, b, c, gdelta.as<long long>(), gdelta_n, stream); });
+    else
+      bound_uvw(a, b, c);
+  }
+  void setup_ghost_delta();
   void ib_momentum(T *a, T *b, T *c);
   void pressure_correction(T *a, T *b, T *c, bool use_q, double coef, bool diag = false);
   void poisson_solve(int accumulate);
@@ -437,6 +450,32 @@ void Engine<T>::setup_poisson() {
   partial.alloc((std::max(top_blocks, 1) + 1) * sizeof(double));
   div_partial.alloc(std::max(red_blocks, 1) * sizeof(double));
   cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));
+}
+
+template <typename T>
+void Engine<T>::setup_ghost_delta() {
+  use_gdelta = false;
+  const char *e = std::getenv("ELMM_GHOST_DELTA");
+  if (!has_ib() || (e && std::string(e) == "0")) return;
+  long cap = 1;
+  for (int c = 0; c < 3; c++) cap = std::max(cap, ghost_shell_cells(g, c));
+  DevBuf mark, tmp, cnt;
+  mark.alloc((size_t)g.ntot);
+  tmp.alloc((size_t)cap * 2 * sizeof(long long));
+  cnt.alloc(sizeof(int));
+  std::vector<long long> all;
+  for (int c = 0; c < 3; c++) {
+    const int n = ghost_delta_build(g, c, ib_tgt[c].as<int32_t>(), ib_n[c], mark.as<uint8_t>(), tmp.as<long long>(),
+                                    (int)cap, cnt.as<int>(), stream);
+    HIPCHK(hipGetLastError());
+    if (n < 0 || n > cap) throw HipError("ghost delta list construction failed");
+    const size_t off = all.size();
+    all.resize(off + 2 * (size_t)n);
+    if (n) HIPCHK(hipMemcpy(all.data() + off, tmp.p, 2 * (size_t)n * sizeof(long long), hipMemcpyDeviceToHost));
+    gdelta_n[c] = n;
+  }
+  upload_vec(gdelta, all);
+  use_gdelta = true;
 }
 
 template <typename T>
@@ -661,6 +700,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
     upload_vec(ib_coef[c], cf);
   }
   ib_tmp.alloc((size_t)std::max({ib_n[0], ib_n[1], ib_n[2], 1}) * sizeof(double));
+  setup_ghost_delta();
   // IBMassSources gather terms (src/tsteps.f90:490-531) + Bound_Q fold targets
   if (cfg.ib_mass_sources && has_ib()) {
     TermBuilder tb;
@@ -985,7 +1025,7 @@ void Engine<T>::stage(int st, bool last) {
   const bool use_q = cfg.ib_mass_sources && has_ib();
   if (has_ib()) {
     ib_momentum(u2, v2, w2);
-    bound_uvw(u2, v2, w2);
+    bound_uvw_after_ib(u2, v2, w2);
   }
   if (use_q) timed(ELMM_T_IB, [&] { launch_mass_sources(g, q_terms.view(), u2, v2, w2, Q.as<double>(), stream); });
   pressure_correction(u2, v2, w2, use_q, sc.coef, last && want_diag);
@@ -1003,7 +1043,7 @@ void Engine<T>::stage(int st, bool last) {
   bound_uvw(u, v, w);
   if (has_ib()) {
     ib_momentum(u, v, w);
-    bound_uvw(u, v, w);
+    bound_uvw_after_ib(u, v, w);
   }
 }
 

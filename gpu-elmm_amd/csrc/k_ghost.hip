@@ -98,18 +98,18 @@ __device__ inline bool vel_rule(int m, int n, int blo, int bhi, bool normal, int
   src = n; neg = 0; return false;
 }
 
-template <typename T>
-__global__ void k_bound_uvw(Geo g, T *U, T *V, T *W, Shell su, Shell sv, Shell sw) {
-  const int c = blockIdx.y;
-  const Shell &s = c == 0 ? su : c == 1 ? sv : sw;
-  T *F = c == 0 ? U : c == 1 ? V : W;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!decode(s, t, i, j, k)) return;
+// Composite rule of velocity ghost cell t of component c's shell: the cell
+// (i, j, k) receives zero or (neg ? -1 : 1) times the interior value at
+// (si, sj, sk).  Returns false past the end of the shell.
+__device__ inline bool uvw_rule(const Geo &g, const Shell &s, int c, long t, int &i, int &j, int &k, int &si,
+                                int &sj, int &sk, int &neg, bool &zero) {
+  if (!decode(s, t, i, j, k)) return false;
   const int n[3] = {s.n0, s.n1, s.n2};
-  int neg = 0;
-  bool zero = false;
-  int si = i, sj = j, sk = k;
+  neg = 0;
+  zero = false;
+  si = i;
+  sj = j;
+  sk = k;
   // outermost (last) pass first: z, then y, then x
   if (k < 1 || k > n[2]) {
     int nn;
@@ -126,8 +126,56 @@ __global__ void k_bound_uvw(Geo g, T *U, T *V, T *W, Shell su, Shell sv, Shell s
     zero = vel_rule(i, n[0], g.bt[0], g.bt[1], c == 0, si, nn);
     neg ^= zero ? 0 : nn;
   }
+  return true;
+}
+
+template <typename T>
+__global__ void k_bound_uvw(Geo g, T *U, T *V, T *W, Shell su, Shell sv, Shell sw) {
+  const int c = blockIdx.y;
+  const Shell &s = c == 0 ? su : c == 1 ? sv : sw;
+  T *F = c == 0 ? U : c == 1 ? V : W;
+  int i, j, k, si, sj, sk, neg;
+  bool zero;
+  if (!uvw_rule(g, s, c, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k, si, sj, sk, neg, zero)) return;
   T v = zero ? T(0) : F[gix(g, si, sj, sk)];
   F[gix(g, i, j, k)] = neg ? -v : v;
+}
+
+// Ghost cells of component c whose source is a marked interior cell, as
+// (ghost offset, source offset | neg << 62) pairs appended at list[2*count].
+// Built once at set-up (the order of the entries is immaterial: every ghost
+// cell appears at most once).
+__global__ void k_ghost_map(Geo g, Shell s, int c, const uint8_t *__restrict__ mark, long long *__restrict__ list,
+                            int cap, int *__restrict__ count) {
+  int i, j, k, si, sj, sk, neg;
+  bool zero;
+  if (!uvw_rule(g, s, c, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k, si, sj, sk, neg, zero)) return;
+  if (zero) return;
+  const long src = gix(g, si, sj, sk);
+  if (!mark[src]) return;
+  const int q = atomicAdd(count, 1);
+  if (q < cap) {
+    list[2 * q] = gix(g, i, j, k);
+    list[2 * q + 1] = src | (neg ? (1LL << 62) : 0);
+  }
+}
+
+__global__ void k_mark_cells(Geo g, const int32_t *__restrict__ ijk, int n, uint8_t *__restrict__ mark, uint8_t v) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) mark[gix(g, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2])] = v;
+}
+
+// Re-applies the listed ghost rules (n[c] entries per component, U's
+// first): after a change of the listed source cells only, this equals a full
+// BoundUVW.
+template <typename T>
+__global__ void k_ghost_delta(T *U, T *V, T *W, const long long *__restrict__ list, int n0, int n1, int n2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n0 + n1 + n2) return;
+  T *F = q < n0 ? U : q < n0 + n1 ? V : W;
+  const long long dst = list[2 * q], e = list[2 * q + 1];
+  const T v = F[e & ((1LL << 62) - 1)];
+  F[dst] = (e >> 62) ? -v : v;
 }
 
 // ------------------------------------------------------ BoundViscosity
@@ -257,16 +305,43 @@ __global__ void k_bound_pr(Geo g, T *P, Shell s) {
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+static Shell uvw_shell(const Geo &g, int c) {
+  const int *n = g.n[c];
+  return make_shell(n[0], n[1], n[2], 3, 3, -2, n[0] + 3, -2, n[1] + 3, -2, n[0] + 3);
+}
+
 template <typename T>
 void launch_bound_uvw(const Geo &g, T *U, T *V, T *W, hipStream_t st) {
   Shell s[3];
-  for (int c = 0; c < 3; c++) {
-    const int *n = g.n[c];
-    s[c] = make_shell(n[0], n[1], n[2], 3, 3, -2, n[0] + 3, -2, n[1] + 3, -2, n[0] + 3);
-  }
+  for (int c = 0; c < 3; c++) s[c] = uvw_shell(g, c);
   long mx = 0;
   for (int c = 0; c < 3; c++) mx = std::max(mx, s[c].cz + s[c].cy + s[c].cx);
   hipLaunchKernelGGL(k_bound_uvw<T>, dim3(nblk(mx, 256), 3), dim3(256), 0, st, g, U, V, W, s[0], s[1], s[2]);
+}
+
+long ghost_shell_cells(const Geo &g, int c) {
+  const Shell s = uvw_shell(g, c);
+  return s.cz + s.cy + s.cx;
+}
+
+int ghost_delta_build(const Geo &g, int c, const int32_t *ijk, int n, uint8_t *mark, long long *list, int cap,
+                      int *count, hipStream_t st) {
+  const Shell s = uvw_shell(g, c);
+  int h = 0;
+  if (hipMemcpyAsync(count, &h, sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+  if (n) hipLaunchKernelGGL(k_mark_cells, dim3(nblk(n, 256)), dim3(256), 0, st, g, ijk, n, mark, (uint8_t)1);
+  hipLaunchKernelGGL(k_ghost_map, dim3(nblk(s.cz + s.cy + s.cx, 256)), dim3(256), 0, st, g, s, c, mark, list, cap,
+                     count);
+  if (n) hipLaunchKernelGGL(k_mark_cells, dim3(nblk(n, 256)), dim3(256), 0, st, g, ijk, n, mark, (uint8_t)0);
+  if (hipMemcpyAsync(&h, count, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+  if (hipStreamSynchronize(st) != hipSuccess) return -1;
+  return h;
+}
+
+template <typename T>
+void launch_ghost_delta(T *U, T *V, T *W, const long long *list, const int n[3], hipStream_t st) {
+  const int tot = n[0] + n[1] + n[2];
+  if (tot) hipLaunchKernelGGL(k_ghost_delta<T>, dim3(nblk(tot, 256)), dim3(256), 0, st, U, V, W, list, n[0], n[1], n[2]);
 }
 
 template <typename T>
@@ -287,6 +362,8 @@ void launch_bound_pr(const Geo &g, T *P, hipStream_t st) {
   hipLaunchKernelGGL(k_bound_pr<T>, dim3(nblk(s.cz + s.cy + s.cx, 256)), dim3(256), 0, st, g, P, s);
 }
 
+template void launch_ghost_delta<double>(double *, double *, double *, const long long *, const int[3], hipStream_t);
+template void launch_ghost_delta<float>(float *, float *, float *, const long long *, const int[3], hipStream_t);
 template void launch_bound_uvw<double>(const Geo &, double *, double *, double *, hipStream_t);
 template void launch_bound_uvw<float>(const Geo &, float *, float *, float *, hipStream_t);
 template void launch_bound_visc<double>(const Geo &, double *, hipStream_t);

@@ -10,6 +10,18 @@ namespace elmm {
 
 // ghost fills (k_ghost.hip)
 template <typename T> void launch_bound_uvw(const Geo &g, T *U, T *V, T *W, hipStream_t st);
+// Ghost cells of component c's BoundUVW shell (count), and the ghost cells
+// whose source is one of the n interior cells ijk (device, 3 ints each):
+// written to list as (ghost offset, source offset | neg << 62) pairs, at most
+// cap; mark is a zeroed byte per cell (left zeroed).  Returns the number of
+// entries (synchronises st), -1 on a runtime error.
+long ghost_shell_cells(const Geo &g, int c);
+int ghost_delta_build(const Geo &g, int c, const int32_t *ijk, int n, uint8_t *mark, long long *list, int cap,
+                      int *count, hipStream_t st);
+// Re-applies the ghost rules of the concatenated lists (n[0] for U, n[1] for
+// V, n[2] for W).
+template <typename T>
+void launch_ghost_delta(T *U, T *V, T *W, const long long *list, const int n[3], hipStream_t st);
 template <typename T> void launch_bound_visc(const Geo &g, T *F, hipStream_t st);
 template <typename T> void launch_bound_scalar(const Geo &g, T *F, hipStream_t st);
 template <typename T> void launch_bound_pr(const Geo &g, T *P, hipStream_t st);
