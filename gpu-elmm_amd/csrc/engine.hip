@@ -699,7 +699,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
     upload_vec(ib_src[c], sr);
     upload_vec(ib_coef[c], cf);
   }
-  ib_tmp.alloc((size_t)std::max({ib_n[0], ib_n[1], ib_n[2], 1}) * sizeof(double));
+  ib_tmp.alloc((size_t)std::max(ib_n[0] + ib_n[1] + ib_n[2], 1) * sizeof(double));
   setup_ghost_delta();
   // IBMassSources gather terms (src/tsteps.f90:490-531) + Bound_Q fold targets
   if (cfg.ib_mass_sources && has_ib()) {
@@ -848,12 +848,8 @@ void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool
 
 template <typename T>
 void Engine<T>::ib_momentum(T *a, T *b, T *c) {
-  T *F[3] = {a, b, c};
-  for (int comp = 0; comp < 3; comp++) {
-    if (!ib_n[comp]) continue;
-    timed(ELMM_T_IB, [&] { launch_ib_interp(g, ibdev(comp), F[comp], ib_tmp.as<double>(), stream); });
-    timed(ELMM_T_IB, [&] { launch_ib_scatter(g, ibdev(comp), ib_tmp.as<double>(), F[comp], stream); });
-  }
+  const IBDev ib[3] = {ibdev(0), ibdev(1), ibdev(2)};
+  timed(ELMM_T_IB, [&] { launch_ib_momentum(g, ib, a, b, c, ib_tmp.as<double>(), stream); });
 }
 
 template <typename T>
@@ -1037,9 +1033,10 @@ void Engine<T>::stage(int st, bool last) {
   std::swap(u, u2);
   std::swap(v, v2);
   std::swap(w, w2);
-  for (int c = 0; c < 3; c++)
-    if (null_n[c])
-      timed(ELMM_T_IB, [&] { launch_null(g, null_n[c], null_ijk[c].as<int32_t>(), c == 0 ? u : c == 1 ? v : w, stream); });
+  if (null_n[0] + null_n[1] + null_n[2]) {
+    const int32_t *const nl[3] = {null_ijk[0].as<int32_t>(), null_ijk[1].as<int32_t>(), null_ijk[2].as<int32_t>()};
+    timed(ELMM_T_IB, [&] { launch_null(g, null_n, nl, u, v, w, stream); });
+  }
   bound_uvw(u, v, w);
   if (has_ib()) {
     ib_momentum(u, v, w);

@@ -762,31 +762,58 @@ void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const 
 
 // ------------------------------------------------------------- IB gathers
 // TIBPoint_Interpolate, src/immersedboundary.f90:637-651; IBMomentum is
-// gather-all-then-scatter (src/tsteps.f90:368-418).
+// gather-all-then-scatter (src/tsteps.f90:368-418).  The components are
+// independent (each interpolates its own field), so one launch gathers the
+// points of all three and a second scatters them.
 // The interpolation is evaluated in FP64 (coefficients are FP64) and the
 // result rounded to the field type.
+struct IB3 {
+  IBDev c0, c1, c2;
+  int n0, n01, n012;  // prefix counts
+};
+
+// Point p of the concatenated lists: its component's list (selected field by
+// field, so the kernel arguments are not copied to private memory) and its
+// index q in that list.
+struct IBSel {
+  const int32_t *tgt, *ptr, *src;
+  const double *coef;
+  int q;
+};
+__device__ inline IBSel ib_pick(const IB3 &ib, int p) {
+  const bool a0 = p < ib.n0, a1 = p < ib.n01;
+  return IBSel{a0 ? ib.c0.tgt : a1 ? ib.c1.tgt : ib.c2.tgt, a0 ? ib.c0.ptr : a1 ? ib.c1.ptr : ib.c2.ptr,
+               a0 ? ib.c0.src : a1 ? ib.c1.src : ib.c2.src, a0 ? ib.c0.coef : a1 ? ib.c1.coef : ib.c2.coef,
+               a0 ? p : a1 ? p - ib.n0 : p - ib.n01};
+}
+
 template <typename T>
-__global__ void k_ib_interp(Geo g, IBDev ib, const T *__restrict__ F, double *__restrict__ tmp) {
+__global__ void k_ib_interp(Geo g, IB3 ib, const T *__restrict__ U, const T *__restrict__ V,
+                            const T *__restrict__ W, double *__restrict__ tmp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ib.n) return;
+  if (p >= ib.n012) return;
+  const IBSel d = ib_pick(ib, p);
+  const T *F = p < ib.n0 ? U : p < ib.n01 ? V : W;
   double s = 0;
-  for (int t = ib.ptr[p]; t < ib.ptr[p + 1]; t++)
-    s = s + ib.coef[t] * (double)F[gix(g, ib.src[3 * t], ib.src[3 * t + 1], ib.src[3 * t + 2])];
+  for (int t = d.ptr[d.q]; t < d.ptr[d.q + 1]; t++)
+    s = s + d.coef[t] * (double)F[gix(g, d.src[3 * t], d.src[3 * t + 1], d.src[3 * t + 2])];
   tmp[p] = s;
 }
 template <typename T>
-__global__ void k_ib_scatter(Geo g, IBDev ib, const double *__restrict__ tmp, T *__restrict__ F) {
+__global__ void k_ib_scatter(Geo g, IB3 ib, const double *__restrict__ tmp, T *__restrict__ U, T *__restrict__ V,
+                             T *__restrict__ W) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ib.n) return;
-  F[gix(g, ib.tgt[3 * p], ib.tgt[3 * p + 1], ib.tgt[3 * p + 2])] = (T)tmp[p];
+  if (p >= ib.n012) return;
+  const IBSel d = ib_pick(ib, p);
+  T *F = p < ib.n0 ? U : p < ib.n01 ? V : W;
+  F[gix(g, d.tgt[3 * d.q], d.tgt[3 * d.q + 1], d.tgt[3 * d.q + 2])] = (T)tmp[p];
 }
 template <typename T>
-void launch_ib_interp(const Geo &g, IBDev ib, const T *F, double *tmp, hipStream_t st) {
-  if (ib.n) hipLaunchKernelGGL(k_ib_interp<T>, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, F, tmp);
-}
-template <typename T>
-void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, T *F, hipStream_t st) {
-  if (ib.n) hipLaunchKernelGGL(k_ib_scatter<T>, dim3(nblk(ib.n, 256)), dim3(256), 0, st, g, ib, tmp, F);
+void launch_ib_momentum(const Geo &g, const IBDev ib[3], T *U, T *V, T *W, double *tmp, hipStream_t st) {
+  IB3 b{ib[0], ib[1], ib[2], ib[0].n, ib[0].n + ib[1].n, ib[0].n + ib[1].n + ib[2].n};
+  if (!b.n012) return;
+  hipLaunchKernelGGL(k_ib_interp<T>, dim3(nblk(b.n012, 256)), dim3(256), 0, st, g, b, U, V, W, tmp);
+  hipLaunchKernelGGL(k_ib_scatter<T>, dim3(nblk(b.n012, 256)), dim3(256), 0, st, g, b, tmp, U, V, W);
 }
 
 // ScalFlIBPoints viscosity loop (src/dynamics.f90:899-902) is sequential:
@@ -810,22 +837,34 @@ void launch_ib_visc_serial(const Geo &g, IBDev ib, T *Visc, const int32_t *level
   }
 }
 
-// NullInterior, src/tsteps.f90:543-567
+// NullInterior, src/tsteps.f90:543-567 (the three components in one launch)
 template <typename T>
-__global__ void k_null(Geo g, int n, const int32_t *__restrict__ ijk, T *__restrict__ F) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) F[gix(g, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2])] = 0;
+__global__ void k_null(Geo g, int n0, int n1, int n2, const int32_t *__restrict__ a, const int32_t *__restrict__ b,
+                       const int32_t *__restrict__ c, T *__restrict__ U, T *__restrict__ V, T *__restrict__ W) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t *ijk = a;
+  T *F = U;
+  if (p >= n0) {
+    p -= n0;
+    ijk = b;
+    F = V;
+    if (p >= n1) {
+      p -= n1;
+      ijk = c;
+      F = W;
+      if (p >= n2) return;
+    }
+  }
+  F[gix(g, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2])] = 0;
 }
 template <typename T>
-void launch_null(const Geo &g, int n, const int32_t *ijk, T *F, hipStream_t st) {
-  if (n) hipLaunchKernelGGL(k_null<T>, dim3(nblk(n, 256)), dim3(256), 0, st, g, n, ijk, F);
+void launch_null(const Geo &g, const int n[3], const int32_t *const ijk[3], T *U, T *V, T *W, hipStream_t st) {
+  const int tot = n[0] + n[1] + n[2];
+  if (tot)
+    hipLaunchKernelGGL(k_null<T>, dim3(nblk(tot, 256)), dim3(256), 0, st, g, n[0], n[1], n[2], ijk[0], ijk[1], ijk[2],
+                       U, V, W);
 }
 
-// IBMassSources (2nd order), src/tsteps.f90:490-531, as a gather: every
-// written Q cell sums its contributions in the reference's single-thread
-// order (the reference scatters with omp atomic).  Cells that only receive a
-// Bound_Q fold are listed with no terms, so their pre-fold value is 0.
-// term = {component | neg<<2, source linear index, spacing index, 0}
 template <typename T>
 __global__ void k_mass_sources(Geo g, TermList tl, const T *__restrict__ U, const T *__restrict__ V,
                                const T *__restrict__ W, double *__restrict__ Q) {
@@ -886,10 +925,9 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
   template void launch_momentum_corr<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *,   \
                                         T *, T *, const uint8_t *, StageConst, TermList, const double *,           \
                                         const double *, hipStream_t);                                              \
-  template void launch_ib_interp<T>(const Geo &, IBDev, const T *, double *, hipStream_t);                         \
-  template void launch_ib_scatter<T>(const Geo &, IBDev, const double *, T *, hipStream_t);                        \
+  template void launch_ib_momentum<T>(const Geo &, const IBDev[3], T *, T *, T *, double *, hipStream_t);         \
   template void launch_ib_visc_serial<T>(const Geo &, IBDev, T *, const int32_t *, int, hipStream_t);              \
-  template void launch_null<T>(const Geo &, int, const int32_t *, T *, hipStream_t);                               \
+  template void launch_null<T>(const Geo &, const int[3], const int32_t *const[3], T *, T *, T *, hipStream_t);    \
   template void launch_mass_sources<T>(const Geo &, TermList, const T *, const T *, const T *, double *,           \
                                        hipStream_t);
 ELMM_INST_MOMENTUM(double)

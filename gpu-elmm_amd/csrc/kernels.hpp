@@ -87,14 +87,14 @@ template <typename T>
 void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
                           T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
                           const double *fluxm, hipStream_t st);
+// IBMomentum of the three components: interpolation of every point (tmp
+// holds ib[0].n + ib[1].n + ib[2].n values), then the scatter.
 template <typename T>
-void launch_ib_interp(const Geo &g, IBDev ib, const T *F, double *tmp, hipStream_t st);
-template <typename T>
-void launch_ib_scatter(const Geo &g, IBDev ib, const double *tmp, T *F, hipStream_t st);
+void launch_ib_momentum(const Geo &g, const IBDev ib[3], T *U, T *V, T *W, double *tmp, hipStream_t st);
 template <typename T>
 void launch_ib_visc_serial(const Geo &g, IBDev ib, T *Visc, const int32_t *level_ptr, int nlevels, hipStream_t st);
 template <typename T>
-void launch_null(const Geo &g, int n, const int32_t *ijk, T *F, hipStream_t st);
+void launch_null(const Geo &g, const int n[3], const int32_t *const ijk[3], T *U, T *V, T *W, hipStream_t st);
 template <typename T>
 void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, const T *W, double *Q, hipStream_t st);
 
