@@ -212,6 +212,7 @@ struct Engine : EngineIface {
   int ib_n[4] = {0, 0, 0, 0};
   std::vector<int32_t> visc_levels;
   DevTerms q_terms;
+  DevBuf qmask;  // per (j, k) row: bit s set when cells 64s+1..64s+64 may hold a non-zero Q
   DevBuf null_ijk[3];
   int null_n[3] = {0, 0, 0};
   // pressure
@@ -734,6 +735,13 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       tb.cell_of(lin(d[0], d[1], d[2]), d[0], d[1], d[2]);
     }
     tb.upload(q_terms);
+    // rows (j, k) and 64-cell segments of x where Q may be non-zero: the
+    // listed interior cells (everything else stays zero)
+    std::vector<uint32_t> qm((size_t)cfg.ny * cfg.nz, 0);
+    for (auto &q : tb.cells)
+      if (q[0] >= 1 && q[0] <= cfg.nx && q[1] >= 1 && q[1] <= cfg.ny && q[2] >= 1 && q[2] <= cfg.nz)
+        qm[(size_t)(q[1] - 1) + (size_t)cfg.ny * (q[2] - 1)] |= 1u << ((q[0] - 1) >> 6);
+    upload_vec(qmask, qm);
   }
   for (int c = 0; c < 3; c++) {
     null_n[c] = L.null_n[c];
@@ -893,21 +901,22 @@ template <typename T>
 void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, bool diag) {
   const double dt2 = coef, dt2_rec = 1.0 / dt2;
   double *q = use_q ? Q.as<double>() : nullptr;
+  const uint32_t *qm = use_q ? qmask.as<uint32_t>() : nullptr;
   // first solve; the precision form keeps the FP64 right-hand side for the
   // refinement sweeps or for the residual diagnostic
   double *rhs_keep = refine_iters > 0 || (solve64 && diag) ? RHS.as<double>() : nullptr;
   timed(ELMM_T_POISSON_X_FWD, [&] {
     if (solve64)
-      launch_fft_x_fwd(g, pp64, a, b, c, q, nullptr, nullptr, dt2_rec, rhs_keep, stream);
+      launch_fft_x_fwd(g, pp64, a, b, c, q, qm, nullptr, nullptr, dt2_rec, rhs_keep, stream);
     else
-      launch_fft_x_fwd(g, pp, a, b, c, q, nullptr, nullptr, dt2_rec, rhs_keep, stream);
+      launch_fft_x_fwd(g, pp, a, b, c, q, qm, nullptr, nullptr, dt2_rec, rhs_keep, stream);
   });
   poisson_solve(0);
   // iterative refinement: the x pass evaluates r = RHS - A Phi in FP64 and
   // transforms it; FP32 solve of A e = r; Phi += e (FP64)
   for (int it = 0; it < refine_iters; it++) {
     timed(ELMM_T_REFINE, [&] {
-      launch_fft_x_fwd(g, pp, a, b, c, nullptr, RHS.as<double>(), Phi.as<double>(), 1.0, nullptr, stream);
+      launch_fft_x_fwd(g, pp, a, b, c, nullptr, nullptr, RHS.as<double>(), Phi.as<double>(), 1.0, nullptr, stream);
     });
     poisson_solve(1);
   }

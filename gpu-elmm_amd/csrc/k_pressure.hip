@@ -59,7 +59,9 @@ __host__ __device__ inline int twlen(int n) { return n / 2; }
 // under stages s0..s0+2), so every butterfly is the same operation on the
 // same operands as a stage-by-stage radix-2 sweep over bit-reversed input.
 //   first round  : each set is loaded straight from global memory
-//                  (load(line, natural index)), so the input never takes a
+//                  (load(line, natural index), all eight elements before
+//                  fin(line, index, loaded value) turns each into the
+//                  transform's input), so the input never takes a
 //                  bit-reversed trip through LDS.  With PermFirst the sets
 //                  are dealt to threads in bit-reversed order, which makes
 //                  the natural indices of one element slot consecutive
@@ -73,12 +75,13 @@ __host__ __device__ inline int twlen(int n) { return n / 2; }
 // memory, the y pass).  tw_g (global) and tw_s (LDS, written before the
 // first round) hold the same table tw[t] = exp(-2 pi i t / n); inverse uses
 // the conjugate.
-template <int LOGN, int CNT, int NT, bool LineMajor, bool PermFirst, typename V, typename Load, typename Store>
+template <int LOGN, int CNT, int NT, bool LineMajor, bool PermFirst, typename V, typename Load, typename Fin,
+          typename Store>
 __device__ __forceinline__ void fft_lines(V *buf, int stride, int padmode, const V *__restrict__ tw_g, const V *tw_s,
-                                          bool inverse, Load load, Store store) {
+                                          bool inverse, Load load, Fin fin, Store store) {
   constexpr int N = 1 << LOGN;
   if constexpr (LOGN == 0) {
-    for (int l = threadIdx.x; l < CNT; l += NT) store(l, 0, load(l, 0));  // length 1: identity
+    for (int l = threadIdx.x; l < CNT; l += NT) store(l, 0, fin(l, 0, load(l, 0)));  // length 1: identity
   } else {
 #pragma unroll
     for (int s0 = 0; s0 < LOGN; s0 += 3) {
@@ -98,11 +101,18 @@ __device__ __forceinline__ void fft_lines(V *buf, int stride, int padmode, const
           const int lo = r & ((1 << s0) - 1), hi = r >> s0;
           const int base = (hi << (s0 + nb)) | lo;
           V x[8];
+          if (first) {
+            decltype(load(0, 0)) raw[8];
 #pragma unroll
-          for (int e = 0; e < 8; e++) {
-            if (e >= ne) break;
-            const int pos = base + (e << s0);
-            x[e] = first ? load(l, bitrev(pos, LOGN)) : buf[l * stride + fpad(pos, padmode)];
+            for (int e = 0; e < 8; e++)
+              if (e < ne) raw[e] = load(l, bitrev(base + (e << s0), LOGN));
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+              if (e < ne) x[e] = fin(l, bitrev(base + (e << s0), LOGN), raw[e]);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+              if (e < ne) x[e] = buf[l * stride + fpad(base + (e << s0), padmode)];
           }
 #pragma unroll
           for (int b = 0; b < 3; b++) {
@@ -179,15 +189,18 @@ __device__ inline double fd2_residual(const Geo &g, const double *__restrict__ P
 // kRhsOut (first solve); kRhsIn a stored right-hand side; kRhsResid the FP64
 // residual rhs_in - A phi (refinement sweeps).  The variants are separate
 // instantiations so that the element loads carry no branches and issue
-// together.  Rows past the end load row nrows-1 and are not stored.  ny is a
-// power of two: row = (j-1) + ny (k-1).
+// together.  Q is non-zero on few cells: qmask[row] has bit s set when
+// elements 64s..64s+63 of the row may hold a non-zero Q, and only those are
+// read (elsewhere r - 0 = r).  Rows past the end load row nrows-1 and are not
+// stored.  ny is a power of two: row = (j-1) + ny (k-1).
 enum { kRhsDiv = 0, kRhsQ = 1, kRhsOut = 2, kRhsIn = 4, kRhsResid = 8 };
 
 template <typename T, typename R, int LOGN, int MODE, int ROWS>
 __global__ void __launch_bounds__(kFftThreads)
     k_fft_x_fwd(Geo g, PoissonPlan<R> pp, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
-                const double *__restrict__ Q, const double *__restrict__ rhs_in, const double *__restrict__ phi,
-                double dt2_rec, double *__restrict__ rhs_out, int logny, int padmode) {
+                const double *__restrict__ Q, const uint32_t *__restrict__ qmask, const double *__restrict__ rhs_in,
+                const double *__restrict__ phi, double dt2_rec, double *__restrict__ rhs_out, int logny,
+                int padmode) {
   constexpr int N = 1 << LOGN, NT = xthreads(LOGN, ROWS);
   extern __shared__ __align__(16) unsigned char sh_raw[];
   C2<R> *tw = reinterpret_cast<C2<R> *>(sh_raw);
@@ -196,21 +209,32 @@ __global__ void __launch_bounds__(kFftThreads)
   const int ls = fspan(N, padmode), M = pp.M;
   const int row0 = blockIdx.x * ROWS;
   const int nrows = pp.ny * pp.nz;
-  auto load = [&](int l, int m) -> C2<R> {
-    const int row = min(row0 + l, nrows - 1);
-    const int j = (row & (pp.ny - 1)) + 1, k = (row >> logny) + 1, i = m + 1;
-    const long p = gix(g, i, j, k);
-    double r;
+  auto cell = [&](int l, int m, int &row) {
+    row = min(row0 + l, nrows - 1);
+    return gix(g, m + 1, (row & (pp.ny - 1)) + 1, (row >> logny) + 1);
+  };
+  auto load = [&](int l, int m) -> double {
+    int row;
+    const long p = cell(l, m, row);
+    const int k = (row >> logny) + 1;
     if constexpr (MODE == kRhsResid) {
-      r = fd2_residual(g, phi, rhs_in, pp.az, pp.cz, i, j, k, p);
+      return fd2_residual(g, phi, rhs_in, pp.az, pp.cz, m + 1, (row & (pp.ny - 1)) + 1, k, p);
     } else if constexpr (MODE == kRhsIn) {
-      r = rhs_in[p];
+      return rhs_in[p];
     } else {
       const double dw = (double)W[p] - (double)W[p - g.sz];
-      r = div_rn((double)U[p] - (double)U[p - 1], g.dxmin, g.rdx) +
-          div_rn((double)V[p] - (double)V[p - g.sy], g.dymin, g.rdy) +
-          (g.variable_z ? dw / g.dzPr[k] : div_rn(dw, g.dzmin, g.rdz));
-      if constexpr ((MODE & kRhsQ) != 0) r = r - Q[p];
+      return div_rn((double)U[p] - (double)U[p - 1], g.dxmin, g.rdx) +
+             div_rn((double)V[p] - (double)V[p - g.sy], g.dymin, g.rdy) +
+             (g.variable_z ? dw / g.dzPr[k] : div_rn(dw, g.dzmin, g.rdz));
+    }
+  };
+  auto fin = [&](int l, int m, double r) -> C2<R> {
+    if constexpr (MODE != kRhsResid && MODE != kRhsIn) {
+      int row;
+      const long p = cell(l, m, row);
+      if constexpr ((MODE & kRhsQ) != 0) {
+        if ((qmask[row] >> (m >> 6)) & 1u) r = r - Q[p];
+      }
       r = r * dt2_rec;
       if constexpr ((MODE & kRhsOut) != 0) {
         if (row0 + l < nrows) rhs_out[p] = r;
@@ -222,7 +246,7 @@ __global__ void __launch_bounds__(kFftThreads)
     const int row = row0 + l;
     if (row < nrows && m < M) pp.X[(long)row * pp.Mp + m] = v;
   };
-  fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, false, load, store);
+  fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, false, load, fin, store);
 }
 
 // LDS line stride of the y pass: odd, so that the TX lines touched by one
@@ -250,7 +274,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(PoissonPlan<R> pp, int in
     const int m = m0 + c;
     if (m < M) X[(long)j * Mp + m] = v;
   };
-  fft_lines<LOGN, TX, kFftThreads, false, false>(sh, ls, padmode, pp.twy, tw, inverse != 0, load, store);
+  fft_lines<LOGN, TX, kFftThreads, false, false>(sh, ls, padmode, pp.twy, tw, inverse != 0, load,
+                                                 [](int, int, C2<R> v) { return v; }, store);
 }
 
 // Pass 3: per (m, j) mode, a_k x_{k-1} + (lam - a_k - c_k) x_k + c_k x_{k+1}
@@ -402,7 +427,8 @@ __global__ void __launch_bounds__(kFftThreads)
       Phi[q] = accumulate ? Phi[q] + val : val;
     }
   };
-  fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, true, load, store);
+  fft_lines<LOGN, ROWS, NT, true, true>(sh, ls, padmode, pp.twx, tw, true, load, [](int, int, C2<R> v) { return v; },
+                                        store);
 }
 
 static int ilog2(int n) {
@@ -444,7 +470,8 @@ static void with_logn(int logn, F f) {
 
 template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
-                      const double *rhs, const double *phi, double dt2_rec, double *rhs_out, hipStream_t st) {
+                      const uint32_t *qmask, const double *rhs, const double *phi, double dt2_rec, double *rhs_out,
+                      hipStream_t st) {
   const int pm = kPadXFwd, nrows = pp.ny * pp.nz;
   const int mode = phi ? kRhsResid : rhs ? kRhsIn : (Q ? kRhsQ : 0) | (rhs_out ? kRhsOut : 0);
   with_logn(ilog2(pp.nx), [&](auto L) {
@@ -454,7 +481,7 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
       constexpr int MODE = decltype(MD)::value;
       allow_lds((const void *)k_fft_x_fwd<T, R, LOGN, MODE, ROWS>, sh);
       hipLaunchKernelGGL((k_fft_x_fwd<T, R, LOGN, MODE, ROWS>), dim3(nblk(nrows, ROWS)), dim3(xthreads(LOGN, ROWS)),
-                         sh, st, g, pp, U, V, W, Q, rhs, phi, dt2_rec, rhs_out, ilog2(pp.ny), pm);
+                         sh, st, g, pp, U, V, W, Q, qmask, rhs, phi, dt2_rec, rhs_out, ilog2(pp.ny), pm);
     };
     switch (mode) {
       case kRhsResid: go(std::integral_constant<int, kRhsResid>{}); break;
@@ -699,14 +726,12 @@ ELMM_INST_PRESSURE_R(double)
 ELMM_INST_PRESSURE_R(float)
 ELMM_INST_PRESSURE_T(double)
 ELMM_INST_PRESSURE_T(float)
-template void launch_fft_x_fwd<double, double>(const Geo &, const PoissonPlan<double> &, const double *,
-                                               const double *, const double *, const double *, const double *,
-                                               const double *, double, double *, hipStream_t);
-template void launch_fft_x_fwd<float, double>(const Geo &, const PoissonPlan<double> &, const float *, const float *,
-                                              const float *, const double *, const double *, const double *, double,
-                                              double *, hipStream_t);
-template void launch_fft_x_fwd<float, float>(const Geo &, const PoissonPlan<float> &, const float *, const float *,
-                                             const float *, const double *, const double *, const double *, double,
-                                             double *, hipStream_t);
+#define ELMM_INST_XFWD(T, R)                                                                                       \
+  template void launch_fft_x_fwd<T, R>(const Geo &, const PoissonPlan<R> &, const T *, const T *, const T *,        \
+                                       const double *, const uint32_t *, const double *, const double *, double,     \
+                                       double *, hipStream_t);
+ELMM_INST_XFWD(double, double)
+ELMM_INST_XFWD(float, double)
+ELMM_INST_XFWD(float, float)
 
 }  // namespace elmm

@@ -137,7 +137,8 @@ struct PoissonPlan {
 };
 template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
-                      const double *rhs, const double *phi, double dt2_rec, double *rhs_out, hipStream_t st);
+                      const uint32_t *qmask, const double *rhs, const double *phi, double dt2_rec, double *rhs_out,
+                      hipStream_t st);
 template <typename R> void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st);
 template <typename R> void launch_poisson_solve_z(const Geo &g, const PoissonPlan<R> &pp, hipStream_t st);
 template <typename R>
