@@ -44,14 +44,20 @@ __host__ __device__ inline long gix(const Geo &g, int i, int j, int k) {
 // known: q = RN(x r) is within one ulp of x/d when |r d - 1| <= 2^-(p+1)
 // (checked on the host, see div_recip), the remainder x - q d is then exact
 // in one fma, and RN(q + rem r) is the correctly rounded quotient
-// (P. Markstein, IBM J. Res. Dev. 34, 1990).  An exact q (rem = 0) is kept as
-// it is, which also preserves the sign of a zero quotient.  r = 0 divides.
+// (P. Markstein, IBM J. Res. Dev. 34, 1990).  For rem = 0 that sum is q
+// itself except for the sign of a zero quotient, which q carries (sign(q) =
+// sign(x/d), and a non-zero result has the sign of q): copysign restores it
+// without a branch.  div_rn_nz needs r != 0; div_rn divides when r = 0.
+template <typename T>
+__device__ inline T div_rn_nz(T x, T d, T r) {
+  const T q = x * r;
+  const T rem = fma(-q, d, x);
+  return copysign(fma(rem, r, q), q);
+}
 template <typename T>
 __device__ inline T div_rn(T x, T d, T r) {
   if (r == (T)0) return x / d;
-  const T q = x * r;
-  const T rem = fma(-q, d, x);
-  return rem == (T)0 ? q : fma(rem, r, q);
+  return div_rn_nz(x, d, r);
 }
 
 // RN(1/d) if it satisfies div_rn's condition, else 0 (host).

@@ -470,7 +470,7 @@ __device__ inline double cd4(const Geo &g, const T *__restrict__ U, const T *__r
 #undef AI
       }
       d = 9 * (uv1b - uv1a);
-      d = d - div_rn<T>(uv3b - uv3a, (T)3, recip3<T>());
+      d = d - div_rn_nz<T>(uv3b - uv3a, (T)3, recip3<T>());
       acc = acc + div_rn<T>(d, inv[dd], rinv[dd]);
     } else {
       // 2nd order near boundaries (coef2ord = 64)

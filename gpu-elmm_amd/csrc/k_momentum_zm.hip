@@ -56,7 +56,7 @@ struct MomZArgs {
 template <typename T>
 __device__ inline T d4(T uv1b, T uv1a, T uv3b, T uv3a) {
   T d = 9 * (uv1b - uv1a);
-  return d - div_rn<T>(uv3b - uv3a, (T)3, recip3<T>());
+  return d - div_rn_nz<T>(uv3b - uv3a, (T)3, recip3<T>());
 }
 
 template <typename T, typename FF>
@@ -114,7 +114,9 @@ __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr
   U2[p] = corr ? us_prev : u2;
 }
 
-template <typename T, int TX, int TY>
+// FAST: the reciprocals of dx, dy, dz satisfy div_rn's condition (checked on
+// the host), so the quotients need no fallback branch.
+template <typename T, int TX, int TY, bool FAST>
 __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 2) : 1)
     k_momentum_zm(Geo g, MomZArgs<T> a, StageConst sc, int kchunk) {
   constexpr int PW = Tile<TX, TY>::PW, PS = Tile<TX, TY>::PS, NTH = Tile<TX, TY>::NTH;
@@ -191,6 +193,12 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   const T rix = sizeof(T) == 4 ? (T)g.rdxf : (T)g.rdx, riy = sizeof(T) == 4 ? (T)g.rdyf : (T)g.rdy,
           riz = sizeof(T) == 4 ? (T)g.rdzf : (T)g.rdz;
   const T q4 = (T)0.25, m256 = (T)(-1.0 / 256);
+  auto dv = [](T x, T d, T r) {
+    if constexpr (FAST)
+      return div_rn_nz<T>(x, d, r);
+    else
+      return div_rn<T>(x, d, r);
+  };
   const bool in_xy = i <= g.nx && j <= g.ny;
   // per-cell operands of the level being computed (Pr and its +x, +y, +z
   // neighbours, the three codes, the three previous Ustar values), fetched
@@ -260,18 +268,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             T d;
             if (cls == 0) {
               d = self4<T>(Fx);
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + dv(d, ix, rix);
               d = cross4<T>(Fy, [&](int o) { return m4<T>(at(v0, -1, o), at(v0, 0, o), at(v0, 1, o), at(v0, 2, o)); });
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + dv(d, iy, riy);
               d = cross4<T>(Fz, [&](int o) { return wx[2 + o]; });
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + dv(d, iz, riz);
             } else {
               d = self2<T>(Fx);
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + dv(d, ix, rix);
               d = cross2<T>(Fy, at(v0, 1, 0) + at(v0, 0, 0), at(v0, 1, -1) + at(v0, 0, -1));
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + dv(d, iy, riy);
               d = cross2<T>(Fz, at(w0, 1, 0) + at(w0, 0, 0), at(wm, 1, 0) + at(wm, 0, 0));
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + dv(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -299,18 +307,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             T d;
             if (cls == 0) {
               d = cross4<T>(Fx, [&](int o) { return m4<T>(at(u0, o, -1), at(u0, o, 0), at(u0, o, 1), at(u0, o, 2)); });
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + dv(d, ix, rix);
               d = self4<T>(Fy);
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + dv(d, iy, riy);
               d = cross4<T>(Fz, [&](int o) { return wy[2 + o]; });
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + dv(d, iz, riz);
             } else {
               d = cross2<T>(Fx, at(u0, 0, 1) + at(u0, 0, 0), at(u0, -1, 1) + at(u0, -1, 0));
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + dv(d, ix, rix);
               d = self2<T>(Fy);
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + dv(d, iy, riy);
               d = cross2<T>(Fz, at(w0, 0, 1) + at(w0, 0, 0), at(wm, 0, 1) + at(wm, 0, 0));
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + dv(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -338,18 +346,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             T d;
             if (cls == 0) {
               d = cross4<T>(Fx, [&](int o) { return m4<T>(at(um, o, 0), at(u0, o, 0), at(u1, o, 0), at(u2s, o, 0)); });
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + dv(d, ix, rix);
               d = cross4<T>(Fy, [&](int o) { return m4<T>(at(vm, 0, o), at(v0, 0, o), at(v1, 0, o), at(v2s, 0, o)); });
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + dv(d, iy, riy);
               d = self4<T>(Fz);
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + dv(d, iz, riz);
             } else {
               d = cross2<T>(Fx, at(u1, 0, 0) + at(u0, 0, 0), at(u1, -1, 0) + at(u0, -1, 0));
-              acc = acc + div_rn<T>(d, ix, rix);
+              acc = acc + dv(d, ix, rix);
               d = cross2<T>(Fy, at(v1, 0, 0) + at(v0, 0, 0), at(v1, 0, -1) + at(v0, 0, -1));
-              acc = acc + div_rn<T>(d, iy, riy);
+              acc = acc + dv(d, iy, riy);
               d = self2<T>(Fz);
-              acc = acc + div_rn<T>(d, iz, riz);
+              acc = acc + dv(d, iz, riz);
             }
           }
           T w = acc * m256;
@@ -415,25 +423,33 @@ size_t momentum_zm_lds_bytes(int elem) { return (size_t)NSLOT * Tile<32, 8>::PS 
 bool momentum_zm_supported(const Geo &g) { return !g.variable_z; }
 
 namespace {
+template <typename T, int TX, int TY, bool FAST>
+void launch_tile_f(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
+  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
+  if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY, FAST>, lds);
+  dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
+  hipLaunchKernelGGL((k_momentum_zm<T, TX, TY, FAST>), gr, b, lds, st, g, a, sc, kchunk);
+}
 template <typename T, int TX, int TY>
 void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
-  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
-  if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY>, lds);
-  dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL((k_momentum_zm<T, TX, TY>), gr, b, lds, st, g, a, sc, kchunk);
+  const bool fast = sizeof(T) == 4 ? (g.rdxf != 0 && g.rdyf != 0 && g.rdzf != 0) : (g.rdx != 0 && g.rdy != 0 && g.rdz != 0);
+  if (fast)
+    launch_tile_f<T, TX, TY, true>(g, a, sc, kchunk, st);
+  else
+    launch_tile_f<T, TX, TY, false>(g, a, sc, kchunk, st);
 }
 }  // namespace
 
 // Tile shape: 64x4 by default (measured 5 % faster than 32x8 in FP32, equal
-// in FP64); ELMM_MOM_TILE=32x8, 32x4, 64x8, 32x16 or 64x16 selects another
-// instantiation (the last three only for 4-byte fields: their FP64 LDS rings
-// exceed the 160 KiB of a workgroup).
+// in FP64; 32x4, 32x16 and 64x16 measured slower); ELMM_MOM_TILE=32x8 or
+// 64x8 selects another instantiation (64x8 only for 4-byte fields: its FP64
+// LDS rings exceed the 160 KiB of a workgroup).
 static int mom_tile_choice() {
   static int choice = -1;
   if (choice < 0) {
     const char *e = std::getenv("ELMM_MOM_TILE");
     const std::string v = e ? e : "";
-    choice = v == "32x8" ? 0 : v == "32x4" ? 2 : v == "64x8" ? 3 : v == "32x16" ? 4 : v == "64x16" ? 5 : 1;
+    choice = v == "32x8" ? 0 : v == "64x8" ? 3 : 1;
   }
   return choice;
 }
@@ -455,24 +471,9 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
   if (kchunk <= 0) kchunk = g.nz;
   switch (mom_tile_choice()) {
     case 1: launch_tile<T, 64, 4>(g, a, sc, kchunk, st); break;
-    case 2: launch_tile<T, 32, 4>(g, a, sc, kchunk, st); break;
     case 3:
-      if (sizeof(T) == 4) {
+      if constexpr (sizeof(T) == 4) {
         launch_tile<T, 64, 8>(g, a, sc, kchunk, st);
-        break;
-      }
-      launch_tile<T, 32, 8>(g, a, sc, kchunk, st);
-      break;
-    case 4:
-      if (sizeof(T) == 4) {
-        launch_tile<T, 32, 16>(g, a, sc, kchunk, st);
-        break;
-      }
-      launch_tile<T, 32, 8>(g, a, sc, kchunk, st);
-      break;
-    case 5:
-      if (sizeof(T) == 4) {
-        launch_tile<T, 64, 16>(g, a, sc, kchunk, st);
         break;
       }
       launch_tile<T, 32, 8>(g, a, sc, kchunk, st);

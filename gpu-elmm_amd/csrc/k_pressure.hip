@@ -193,7 +193,9 @@ __device__ inline double fd2_residual(const Geo &g, const double *__restrict__ P
 // elements 64s..64s+63 of the row may hold a non-zero Q, and only those are
 // read (elsewhere r - 0 = r).  Rows past the end load row nrows-1 and are not
 // stored.  ny is a power of two: row = (j-1) + ny (k-1).
-enum { kRhsDiv = 0, kRhsQ = 1, kRhsOut = 2, kRhsIn = 4, kRhsResid = 8 };
+// kRhsFast: uniform z and reciprocals of dx, dy, dz that satisfy div_rn's
+// condition (the quotients need no fallback branch).
+enum { kRhsDiv = 0, kRhsQ = 1, kRhsOut = 2, kRhsIn = 4, kRhsResid = 8, kRhsFast = 16 };
 
 template <typename T, typename R, int LOGN, int MODE, int ROWS>
 __global__ void __launch_bounds__(kFftThreads)
@@ -222,14 +224,17 @@ __global__ void __launch_bounds__(kFftThreads)
     } else if constexpr (MODE == kRhsIn) {
       return rhs_in[p];
     } else {
+      const double du = (double)U[p] - (double)U[p - 1], dv = (double)V[p] - (double)V[p - g.sy];
       const double dw = (double)W[p] - (double)W[p - g.sz];
-      return div_rn((double)U[p] - (double)U[p - 1], g.dxmin, g.rdx) +
-             div_rn((double)V[p] - (double)V[p - g.sy], g.dymin, g.rdy) +
-             (g.variable_z ? dw / g.dzPr[k] : div_rn(dw, g.dzmin, g.rdz));
+      if constexpr ((MODE & kRhsFast) != 0)
+        return div_rn_nz(du, g.dxmin, g.rdx) + div_rn_nz(dv, g.dymin, g.rdy) + div_rn_nz(dw, g.dzmin, g.rdz);
+      else
+        return div_rn(du, g.dxmin, g.rdx) + div_rn(dv, g.dymin, g.rdy) +
+               (g.variable_z ? dw / g.dzPr[k] : div_rn(dw, g.dzmin, g.rdz));
     }
   };
   auto fin = [&](int l, int m, double r) -> C2<R> {
-    if constexpr (MODE != kRhsResid && MODE != kRhsIn) {
+    if constexpr ((MODE & (kRhsResid | kRhsIn)) == 0) {
       int row;
       const long p = cell(l, m, row);
       if constexpr ((MODE & kRhsQ) != 0) {
@@ -473,7 +478,9 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
                       const uint32_t *qmask, const double *rhs, const double *phi, double dt2_rec, double *rhs_out,
                       hipStream_t st) {
   const int pm = kPadXFwd, nrows = pp.ny * pp.nz;
-  const int mode = phi ? kRhsResid : rhs ? kRhsIn : (Q ? kRhsQ : 0) | (rhs_out ? kRhsOut : 0);
+  const bool fast = !g.variable_z && g.rdx != 0 && g.rdy != 0 && g.rdz != 0;
+  const int mode =
+      phi ? kRhsResid : rhs ? kRhsIn : (Q ? kRhsQ : 0) | (rhs_out ? kRhsOut : 0) | (fast ? kRhsFast : 0);
   with_logn(ilog2(pp.nx), [&](auto L) {
     constexpr int LOGN = decltype(L)::value, ROWS = xrows(LOGN);
     const size_t sh = ((size_t)ROWS * fspan(1 << LOGN, pm) + twlen(1 << LOGN)) * sizeof(C2<R>);
@@ -489,6 +496,10 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
       case kRhsQ: go(std::integral_constant<int, kRhsQ>{}); break;
       case kRhsOut: go(std::integral_constant<int, kRhsOut>{}); break;
       case kRhsQ | kRhsOut: go(std::integral_constant<int, kRhsQ | kRhsOut>{}); break;
+      case kRhsFast: go(std::integral_constant<int, kRhsFast>{}); break;
+      case kRhsQ | kRhsFast: go(std::integral_constant<int, kRhsQ | kRhsFast>{}); break;
+      case kRhsOut | kRhsFast: go(std::integral_constant<int, kRhsOut | kRhsFast>{}); break;
+      case kRhsQ | kRhsOut | kRhsFast: go(std::integral_constant<int, kRhsQ | kRhsOut | kRhsFast>{}); break;
       default: go(std::integral_constant<int, kRhsDiv>{}); break;
     }
   });

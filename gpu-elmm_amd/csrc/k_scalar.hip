@@ -22,7 +22,7 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 // Real = field and arithmetic type (double: FP64 form, float: precision form)
 template <typename Real>
 __device__ inline Real flux_limiter(Real r) {
-  Real a = div_rn<Real>(1 + 2 * r, (Real)3, recip3<Real>());
+  Real a = div_rn_nz<Real>(1 + 2 * r, (Real)3, recip3<Real>());
   Real b = (Real)2.0 < a ? (Real)2.0 : a;
   Real c = 2 * r < b ? 2 * r : b;
   return (Real)0.0 > c ? (Real)0.0 : c;
