@@ -199,7 +199,7 @@ struct Engine : EngineIface {
   DevBuf coords;
   // wall model
   DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
-  DevBuf width_tab, width_idx;
+  DevBuf width_tab, width_idx, sgs_rdz, sgs_rz2;
   SgsWidth sgs_width{nullptr, nullptr, 0};
   int wm_n = 0;
   DevBuf wmp_ijk, wmp_div;
@@ -384,6 +384,26 @@ void Engine<T>::setup_geo() {
       upload_vec(width_tab, wk);
       if (prods.size() > 1) upload_vec(width_idx, idx);
       sgs_width = SgsWidth{width_tab.as<double>(), prods.size() > 1 ? width_idx.as<uint8_t>() : nullptr, ld};
+    }
+  }
+  // reciprocals of the strain divisors (FP64 form): used when every one
+  // satisfies div_rn's condition, so that the quotients equal the divisions
+  if (sgs_width.tab) {
+    const int ld = nz + 2;
+    std::vector<double> rdz(ld, 0.0), rz2(ld, 0.0);
+    bool ok = g.rdx != 0 && g.rdy != 0;
+    for (int k = 1; k <= nz && ok; k++) {
+      rdz[k] = div_recip(Grid1D::at(grid.dzPr, k));
+      rz2[k] = div_recip(2 * (Grid1D::at(grid.zPr, k + 1) - Grid1D::at(grid.zPr, k - 1)));
+      ok = rdz[k] != 0 && rz2[k] != 0;
+    }
+    if (ok) {
+      upload_vec(sgs_rdz, rdz);
+      upload_vec(sgs_rz2, rz2);
+      sgs_width.rdx = g.rdx;
+      sgs_width.rdy = g.rdy;
+      sgs_width.rdz = sgs_rdz.as<double>();
+      sgs_width.rz2 = sgs_rz2.as<double>();
     }
   }
 }

@@ -23,8 +23,11 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 // and arithmetic type (double: the reference's FP64; float: precision form).
 // Uf(di, dj, dk) etc. return the field at (i+di, j+dj, k+dk); both SGS
 // kernels evaluate the same expressions in the same order.
-template <typename T, typename FU, typename FV, typename FW>
-__device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width, T nu, T c_smag) {
+// RCP (FP64): the quotients as correctly rounded products with the host's
+// reciprocals (div_rn_nz), equal to the divisions.
+template <typename T, bool RCP = false, typename FU, typename FV, typename FW>
+__device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width, T nu, T c_smag,
+                              const SgsWidth &wr) {
   const T dx = (T)g.dxmin, dy = (T)g.dymin;
   const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
   if constexpr (sizeof(T) == 4) {
@@ -51,15 +54,22 @@ __device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width,
     if (nu > 0) v = v + nu;
     return v;
   }
-  T D00 = (Uf(0, 0, 0) - Uf(-1, 0, 0)) / dx;
-  T D11 = (Vf(0, 0, 0) - Vf(0, -1, 0)) / dy;
-  T D22 = (Wf(0, 0, 0) - Wf(0, 0, -1)) / (T)g.dzPr[k];
-  T D01 = (Uf(0, 1, 0) + Uf(-1, 1, 0) - Uf(0, -1, 0) - Uf(-1, -1, 0)) / (4 * dy);
-  T D02 = (Uf(0, 0, 1) + Uf(-1, 0, 1) - Uf(0, 0, -1) - Uf(-1, 0, -1)) / (2 * (zp1 - zm1));
-  T D10 = (Vf(1, 0, 0) + Vf(1, -1, 0) - Vf(-1, 0, 0) - Vf(-1, -1, 0)) / (4 * dx);
-  T D12 = (Vf(0, 0, 1) + Vf(0, -1, 1) - Vf(0, 0, -1) - Vf(0, -1, -1)) / (2 * (zp1 - zm1));
-  T D20 = (Wf(1, 0, 0) + Wf(1, 0, -1) - Wf(-1, 0, 0) - Wf(-1, 0, -1)) / (4 * dx);
-  T D21 = (Wf(0, 1, 0) + Wf(0, 1, -1) - Wf(0, -1, 0) - Wf(0, -1, -1)) / (4 * dy);
+  // (4 d)'s reciprocal is RN(1/d) / 4 exactly
+  auto q = [&](T x, T d, double r) {
+    if constexpr (RCP)
+      return div_rn_nz<T>(x, d, (T)r);
+    else
+      return x / d;
+  };
+  T D00 = q(Uf(0, 0, 0) - Uf(-1, 0, 0), dx, wr.rdx);
+  T D11 = q(Vf(0, 0, 0) - Vf(0, -1, 0), dy, wr.rdy);
+  T D22 = q(Wf(0, 0, 0) - Wf(0, 0, -1), (T)g.dzPr[k], RCP ? wr.rdz[k] : 0.0);
+  T D01 = q(Uf(0, 1, 0) + Uf(-1, 1, 0) - Uf(0, -1, 0) - Uf(-1, -1, 0), 4 * dy, 0.25 * wr.rdy);
+  T D02 = q(Uf(0, 0, 1) + Uf(-1, 0, 1) - Uf(0, 0, -1) - Uf(-1, 0, -1), 2 * (zp1 - zm1), RCP ? wr.rz2[k] : 0.0);
+  T D10 = q(Vf(1, 0, 0) + Vf(1, -1, 0) - Vf(-1, 0, 0) - Vf(-1, -1, 0), 4 * dx, 0.25 * wr.rdx);
+  T D12 = q(Vf(0, 0, 1) + Vf(0, -1, 1) - Vf(0, 0, -1) - Vf(0, -1, -1), 2 * (zp1 - zm1), RCP ? wr.rz2[k] : 0.0);
+  T D20 = q(Wf(1, 0, 0) + Wf(1, 0, -1) - Wf(-1, 0, 0) - Wf(-1, 0, -1), 4 * dx, 0.25 * wr.rdx);
+  T D21 = q(Wf(0, 1, 0) + Wf(0, 1, -1) - Wf(0, -1, 0) - Wf(0, -1, -1), 4 * dy, 0.25 * wr.rdy);
   const T D[3][3] = {{D00, D01, D02}, {D10, D11, D12}, {D20, D21, D22}};
   T s = 0;
 #pragma unroll
@@ -136,7 +146,7 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
     auto Vf = [&](int di, int dj, int dk) { return V[p + di + dj * sy + dk * sz]; };
     auto Wf = [&](int di, int dj, int dk) { return W[p + di + dj * sy + dk * sz]; };
     Visc[p] = sgs_model == 1 ? smag_visc<T>(g, Uf, Vf, Wf, k, smag_width<T>(g, width_t, filter_ratio, i, j, k), nu,
-                                            c_smag)
+                                            c_smag, width_t)
                              : nu;
     if (cfl_partial) cfl = cfl_cell(g, U[p], U[p - 1], V[p], V[p - sy], W[p], W[p - sz], i, j, k);
   }
@@ -153,7 +163,7 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
 // registers while level k is evaluated, then written into the slots level k
 // no longer needs.  Global loads per cell: one plane of U, V and W plus the
 // halo ring, instead of 21 stencil values.
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, bool RCP>
 __global__ void __launch_bounds__(TX *TY)
     k_sgs_zm(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W, T *__restrict__ Visc,
              double nu_, double c_smag_, double filter_ratio, SgsWidth width_t, int sgs_model,
@@ -211,9 +221,9 @@ __global__ void __launch_bounds__(TX *TY)
       auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
       auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
       auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
-      Visc[gix(g, i, j, k)] = sgs_model == 1 ? smag_visc<T>(g, Uf, Vf, Wf, k,
-                                                            smag_width<T, true>(g, width_t, filter_ratio, i, j, k),
-                                                            nu, c_smag)
+      Visc[gix(g, i, j, k)] = sgs_model == 1 ? smag_visc<T, RCP>(g, Uf, Vf, Wf, k,
+                                                                 smag_width<T, true>(g, width_t, filter_ratio, i, j, k),
+                                                                 nu, c_smag, width_t)
                                              : nu;
       if (cfl_partial)
         cfl = fmax(cfl, cfl_cell(g, Uc[c], Uc[c - 1], Vc[c], Vc[c - RX], Wc[c], Wl[c], i, j, k));
@@ -267,8 +277,12 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
     return;
   }
   dim3 gr(nblk(g.nx, kSgsTX), nblk(g.ny, kSgsTY), nblk(g.nz, kSgsZChunk));
-  hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc, nu, c_smag,
-                     filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
+  if (sizeof(T) == 8 && width.rcp())
+    hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, true>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc, nu,
+                       c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
+  else
+    hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, false>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc, nu,
+                       c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
 }
 
 // ------------------------------------------------------------- wall model

@@ -58,10 +58,16 @@ struct IBDev {
 // Smagorinsky filter width filter_ratio (dxPr dyPr dzPr)^(1/3) tabulated on
 // the host: tab[idx[(i-1) + nx (j-1)] * ld + k] (idx null: one column class;
 // tab null: evaluated on the device).
+// rcp: reciprocals for the FP64 strain quotients, all satisfying div_rn's
+// condition: rdx = RN(1/dxmin), rdy = RN(1/dymin), rdz[k] = RN(1/dzPr(k)),
+// rz2[k] = RN(1/(2 (zPr(k+1) - zPr(k-1)))) (null when some does not qualify).
 struct SgsWidth {
   const double *tab;
   const uint8_t *idx;
   int ld;
+  double rdx = 0, rdy = 0;
+  const double *rdz = nullptr, *rz2 = nullptr;
+  bool rcp() const { return rdz != nullptr; }
 };
 
 // momentum (k_momentum.hip); T = field type (double or float)
