@@ -316,7 +316,10 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
     // x_{k+1} = x_k + G_k dzW(k).  Both sums are compensated (Kahan), so the
     // mode is exact to rounding instead of carrying the nz^2 growth of an
     // elimination on the singular operator.
+    // (loops unrolled so that the independent loads of several levels are in
+    // flight together: this one thread bounds the kernel on small grids)
     double Gr = 0, Gi = 0, cr = 0, ci = 0;
+#pragma unroll 8
     for (int k = nz; k >= 2; k--) {
       const double dz = pp.dzPr[k];
       const C2<R> d = X[(long)(k - 1) * plane];
@@ -334,6 +337,7 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
     double xr = 0, xi = 0;
     cr = ci = 0;
     X[0] = mk2((R)0, (R)0);
+#pragma unroll 8
     for (int k = 1; k <= nz - 1; k++) {
       const double dz = pp.dzW[k];
       double tt = gm[0][k - 1] * dz, y = tt - cr, s2 = xr + y;
@@ -352,6 +356,7 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
   // forward elimination; den = b - a c', every division a multiplication by
   // 1/den
   double cp = 0, dr = 0, di = 0;
+#pragma unroll 8
   for (int k = 0; k < nz; k++) {
     const double a = pp.az[k], c = pp.cz[k];
     const double b = lam - a - c;
