@@ -132,14 +132,18 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   const long sz = g.sz;
   const int lx = tx + HALO, ly = ty + HALO;
 
-  // plane element offsets at level 0 (clamped into the allocation)
+  // plane element offsets at level 0 (clamped into the allocation).  A
+  // thread past the end of the plane loads the plane's last element and
+  // writes it to a spare LDS cell of its own behind the rings, so plane
+  // loads and writes carry no branches.
   long poff[PLOAD];
   int pidx[PLOAD];
 #pragma unroll
   for (int r = 0; r < PLOAD; r++) {
     const int e = tid + r * NTH;
     pidx[r] = e < PS ? e : -1;
-    const int ex = e % PW, ey = e / PW;
+    const int ec = min(e, PS - 1);
+    const int ex = ec % PW, ey = ec / PW;
     const int gi = min(i0 - HALO + ex, g.nx + 3), gj = min(j0 - HALO + ey, g.ny + 3);
     poff[r] = gix(g, gi, gj, 0);
   }
@@ -151,10 +155,10 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   auto slotW = [](int L) { return SW + (L + 3) % 3; };
   auto slotN = [](int L) { return SN + (L + 3) % 3; };
   auto at = [&](int slot, int dx, int dy) -> T & { return lds[slot * PS + (ly + dy) * PW + lx + dx]; };
+  auto wr = [&](int slot, int r) -> T & { return lds[pidx[r] >= 0 ? slot * PS + pidx[r] : NSLOT * PS + tid]; };
   auto load_plane = [&](const T *__restrict__ F, int L, int slot) {
 #pragma unroll
-    for (int r = 0; r < PLOAD; r++)
-      if (pidx[r] >= 0) lds[slot * PS + pidx[r]] = F[poff[r] + (long)L * sz];
+    for (int r = 0; r < PLOAD; r++) wr(slot, r) = F[poff[r] + (long)L * sz];
   };
 
   // ---- prologue: rings for k = kbeg
@@ -219,13 +223,12 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
     T pu[PLOAD], pv[PLOAD], pw[PLOAD], pn[PLOAD], ou = 0, ov = 0, ow = 0;
     if (more) {
 #pragma unroll
-      for (int r = 0; r < PLOAD; r++)
-        if (pidx[r] >= 0) {
-          pu[r] = a.U[poff[r] + (long)(k + 3) * sz];
-          pv[r] = a.V[poff[r] + (long)(k + 3) * sz];
-          pw[r] = a.W[poff[r] + (long)(k + 2) * sz];
-          pn[r] = a.Visc[poff[r] + (long)(k + 2) * sz];
-        }
+      for (int r = 0; r < PLOAD; r++) {
+        pu[r] = a.U[poff[r] + (long)(k + 3) * sz];
+        pv[r] = a.V[poff[r] + (long)(k + 3) * sz];
+        pw[r] = a.W[poff[r] + (long)(k + 2) * sz];
+        pn[r] = a.Visc[poff[r] + (long)(k + 2) * sz];
+      }
       const long o = pc + (long)(k + 4) * sz;
       ou = a.U[o];
       ov = a.V[o];
@@ -376,13 +379,12 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
     __syncthreads();
     if (more) {
 #pragma unroll
-      for (int r = 0; r < PLOAD; r++)
-        if (pidx[r] >= 0) {
-          lds[slotU(k + 3) * PS + pidx[r]] = pu[r];
-          lds[slotV(k + 3) * PS + pidx[r]] = pv[r];
-          lds[slotW(k + 2) * PS + pidx[r]] = pw[r];
-          lds[slotN(k + 2) * PS + pidx[r]] = pn[r];
-        }
+      for (int r = 0; r < PLOAD; r++) {
+        wr(slotU(k + 3), r) = pu[r];
+        wr(slotV(k + 3), r) = pv[r];
+        wr(slotW(k + 2), r) = pw[r];
+        wr(slotN(k + 2), r) = pn[r];
+      }
     }
     __syncthreads();
     if (more) {
@@ -418,14 +420,14 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
 
 }  // namespace
 
-size_t momentum_zm_lds_bytes(int elem) { return (size_t)NSLOT * Tile<32, 8>::PS * elem; }
+size_t momentum_zm_lds_bytes(int elem) { return ((size_t)NSLOT * Tile<32, 8>::PS + 256) * elem; }
 
 bool momentum_zm_supported(const Geo &g) { return !g.variable_z; }
 
 namespace {
 template <typename T, int TX, int TY, bool FAST>
 void launch_tile_f(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
-  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
+  const size_t lds = ((size_t)NSLOT * Tile<TX, TY>::PS + TX * TY) * sizeof(T);  // rings + spare cells
   if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY, FAST>, lds);
   dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
   hipLaunchKernelGGL((k_momentum_zm<T, TX, TY, FAST>), gr, b, lds, st, g, a, sc, kchunk);
