@@ -169,7 +169,10 @@ __global__ void __launch_bounds__(TX *TY)
              double nu_, double c_smag_, double filter_ratio, SgsWidth width_t, int sgs_model,
              double *__restrict__ cfl_partial, int zchunk) {
   constexpr int NT = TX * TY, RX = TX + 2, RC = RX * (TY + 2), PER = (RC + NT - 1) / NT;
-  __shared__ T sU[3][RC], sV[3][RC], sW[2][RC];
+  // ring slots (U 0-2, V 3-5, W 6-7) and one spare cell per thread, written
+  // by threads past the region, so the ring writes carry no branches
+  __shared__ T ring[8 * RC + NT];
+  constexpr int SP = 8 * RC;
   const int t = threadIdx.x, tx = t % TX, ty = t / TX;
   const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
   const int kbeg = blockIdx.z * zchunk + 1, kend = min(g.nz, kbeg + zchunk - 1);
@@ -179,25 +182,27 @@ __global__ void __launch_bounds__(TX *TY)
   // region cells of this thread (halo 1 around the tile), clamped into the
   // allocation; values of clamped cells only reach cells outside the domain
   long roff[PER];
+  int ridx[PER];  // region cell of the thread, -1 past the region
 #pragma unroll
   for (int q = 0; q < PER; q++) {
+    ridx[q] = t + q * NT < RC ? t + q * NT : -1;
     const int r = min(t + q * NT, RC - 1);
     const int ri = min(i0 - 1 + r % RX, g.nx + 3), rj = min(j0 - 1 + r / RX, g.ny + 3);
     roff[q] = gix(g, ri, rj, 0);
   }
-  auto put = [&](T *dst, const T *src, int k) {
+  auto put = [&](int dst, const T *src, int k) {  // dst: slot offset in ring
 #pragma unroll
     for (int q = 0; q < PER; q++)
-      if (t + q * NT < RC) dst[t + q * NT] = src[roff[q] + (long)k * g.sz];
+      ring[ridx[q] >= 0 ? dst + ridx[q] : SP + t] = src[roff[q] + (long)k * g.sz];
   };
-  put(sU[0], U, kbeg - 1);
-  put(sU[1], U, kbeg);
-  put(sU[2], U, kbeg + 1);
-  put(sV[0], V, kbeg - 1);
-  put(sV[1], V, kbeg);
-  put(sV[2], V, kbeg + 1);
-  put(sW[0], W, kbeg - 1);
-  put(sW[1], W, kbeg);
+  put(0 * RC, U, kbeg - 1);
+  put(1 * RC, U, kbeg);
+  put(2 * RC, U, kbeg + 1);
+  put(3 * RC, V, kbeg - 1);
+  put(4 * RC, V, kbeg);
+  put(5 * RC, V, kbeg + 1);
+  put(6 * RC, W, kbeg - 1);
+  put(7 * RC, W, kbeg);
   __syncthreads();
   int u0 = 0, u1 = 1, u2 = 2, w0 = 0, w1 = 1;
   const int c = (tx + 1) + RX * (ty + 1);
@@ -215,9 +220,9 @@ __global__ void __launch_bounds__(TX *TY)
       }
     }
     if (inside) {
-      const T *Ul = sU[u0], *Uc = sU[u1], *Uh = sU[u2];
-      const T *Vl = sV[u0], *Vc = sV[u1], *Vh = sV[u2];
-      const T *Wl = sW[w0], *Wc = sW[w1];
+      const T *Ul = ring + u0 * RC, *Uc = ring + u1 * RC, *Uh = ring + u2 * RC;
+      const T *Vl = ring + (3 + u0) * RC, *Vc = ring + (3 + u1) * RC, *Vh = ring + (3 + u2) * RC;
+      const T *Wl = ring + (6 + w0) * RC, *Wc = ring + (6 + w1) * RC;
       auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
       auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
       auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
@@ -231,12 +236,12 @@ __global__ void __launch_bounds__(TX *TY)
     if (more) {
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < PER; q++)
-        if (t + q * NT < RC) {
-          sU[u0][t + q * NT] = pu[q];
-          sV[u0][t + q * NT] = pv[q];
-          sW[w0][t + q * NT] = pw[q];
-        }
+      for (int q = 0; q < PER; q++) {
+        const bool ok = ridx[q] >= 0;
+        ring[ok ? u0 * RC + ridx[q] : SP + t] = pu[q];
+        ring[ok ? (3 + u0) * RC + ridx[q] : SP + t] = pv[q];
+        ring[ok ? (6 + w0) * RC + ridx[q] : SP + t] = pw[q];
+      }
       const int tu = u0;
       u0 = u1;
       u1 = u2;

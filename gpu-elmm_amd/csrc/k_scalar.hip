@@ -157,8 +157,11 @@ __global__ void __launch_bounds__(NTS)
                 const Real *__restrict__ T, const Real *__restrict__ S, Real *__restrict__ Snew,
                 Real *__restrict__ Sadv, const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
                 const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc, int kchunk) {
-  __shared__ Real sS[2][SPS];
-  __shared__ Real sT[2][TPS];
+  // planes plus one spare cell per thread: a thread past the end of a plane
+  // loads a clamped (in-bounds) element and writes its spare cell, so the
+  // plane loads and writes carry no branches
+  __shared__ Real sS[2][SPS + NTS];
+  __shared__ Real sT[2][TPS + NTS];
   __shared__ Real fX[TYS][TXS + 1];  // x faces i0-1 .. i0+TXS-1 of each row
   __shared__ Real fY[TYS + 1][TXS];  // y faces j0-1 .. j0+TYS-1 of each column
   const int tid = threadIdx.x, tx = tid % TXS, ty = tid / TXS;
@@ -171,13 +174,13 @@ __global__ void __launch_bounds__(NTS)
 #pragma unroll
   for (int r = 0; r < SLD; r++) {
     const int e = tid + r * NTS;
-    sidx[r] = e < SPS ? e : -1;
+    sidx[r] = e < SPS ? e : SPS + tid;
     soff[r] = gix(g, min(i0 - 2 + e % SPW, g.nx + 3), min(j0 - 2 + e / SPW, g.ny + 3), 0);
   }
 #pragma unroll
   for (int r = 0; r < TLD; r++) {
     const int e = tid + r * NTS;
-    tidx[r] = e < TPS ? e : -1;
+    tidx[r] = e < TPS ? e : TPS + tid;
     toff[r] = gix(g, min(i0 - 1 + e % TPW, g.nx + 3), min(j0 - 1 + e / TPW, g.ny + 3), 0);
   }
   const long pc = gix(g, min(i, g.nx), min(j, g.ny), 0);
@@ -187,10 +190,10 @@ __global__ void __launch_bounds__(NTS)
   // prologue: planes of level kbeg, column rings
 #pragma unroll
   for (int r = 0; r < SLD; r++)
-    if (sidx[r] >= 0) sS[kbeg & 1][sidx[r]] = S[soff[r] + (long)kbeg * sz];
+    sS[kbeg & 1][sidx[r]] = S[soff[r] + (long)kbeg * sz];
 #pragma unroll
   for (int r = 0; r < TLD; r++)
-    if (tidx[r] >= 0) sT[kbeg & 1][tidx[r]] = T[toff[r] + (long)kbeg * sz];
+    sT[kbeg & 1][tidx[r]] = T[toff[r] + (long)kbeg * sz];
   Real cs[5], ct[3], cw[2];
   uint8_t cm[2];
 #pragma unroll
@@ -238,10 +241,10 @@ __global__ void __launch_bounds__(NTS)
     if (more) {
 #pragma unroll
       for (int r = 0; r < SLD; r++)
-        if (sidx[r] >= 0) ps[r] = S[soff[r] + (long)(k + 1) * sz];
+        ps[r] = S[soff[r] + (long)(k + 1) * sz];
 #pragma unroll
       for (int r = 0; r < TLD; r++)
-        if (tidx[r] >= 0) pt[r] = T[toff[r] + (long)(k + 1) * sz];
+        pt[r] = T[toff[r] + (long)(k + 1) * sz];
       ns = S[pc + (long)(k + 3) * sz];
       nt = T[pc + (long)(k + 2) * sz];
       nw = W[pc + (long)(k + 1) * sz];
@@ -305,10 +308,10 @@ __global__ void __launch_bounds__(NTS)
     if (more) {
 #pragma unroll
       for (int r = 0; r < SLD; r++)
-        if (sidx[r] >= 0) sS[b ^ 1][sidx[r]] = ps[r];
+        sS[b ^ 1][sidx[r]] = ps[r];
 #pragma unroll
       for (int r = 0; r < TLD; r++)
-        if (tidx[r] >= 0) sT[b ^ 1][tidx[r]] = pt[r];
+        sT[b ^ 1][tidx[r]] = pt[r];
 #pragma unroll
       for (int q = 0; q < 4; q++) cs[q] = cs[q + 1];
       cs[4] = ns;
