@@ -619,20 +619,24 @@ __global__ void __launch_bounds__(256)
   const int k = blockIdx.z + 1;
   if (i > g.nx || j > g.ny) return;
   const long p = gix(g, i, j, k);
+  // every operand is loaded unconditionally (all in the allocation: Phi(k+1)
+  // at k = nz is its ghost level), only the stores depend on the component
+  // extents, so the loads of a thread issue together
   const double ph = Phi[p];
   const double phx = Phi[i < g.nx ? p + 1 : p + 1 - g.nx];
   const double phy = Phi[j < g.ny ? p + g.sy : p + g.sy - (long)g.ny * g.sy];
+  const double phz = Phi[p + g.sz];
+  const T u = U[p], v = V[p], w = W[p], pr0 = Pr[p];
   const double Au = dt2 / g.dxmin, Av = dt2 / g.dymin, Aw = dt2 / g.dzmin;
-  if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) U[p] = (T)((double)U[p] - Au * (phx - ph));
-  if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) V[p] = (T)((double)V[p] - Av * (phy - ph));
+  if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) U[p] = (T)((double)u - Au * (phx - ph));
+  if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) V[p] = (T)((double)v - Av * (phy - ph));
   if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
-    const double phz = Phi[p + g.sz];
     if (g.variable_z)
-      W[p] = (T)((double)W[p] - dt2 * (phz - ph) / g.dzW[k]);
+      W[p] = (T)((double)w - dt2 * (phz - ph) / g.dzW[k]);
     else
-      W[p] = (T)((double)W[p] - Aw * (phz - ph));
+      W[p] = (T)((double)w - Aw * (phz - ph));
   }
-  double pr = (double)Pr[p] + ph;
+  double pr = (double)pr0 + ph;
   if (ref != 0) pr = pr - ref;
   Pr[p] = (T)pr;
 }
