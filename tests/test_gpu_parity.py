@@ -125,6 +125,25 @@ def test_immersed_boundary_case_parity(require_gpu):
     assert max(err.values()) <= 1e-11, err
 
 
+EDGE_OBSTACLES = [(0.0, 3.0, 10.0, 14.0, 0.0, 5.0), (60.0, 70.0, 20.0, 26.0, 0.0, 8.0),
+                  (100.0, 110.0, 28.0, 32.0, 0.0, 6.0, 0.01)]
+
+
+def test_obstacles_at_domain_edges_bit_identical(require_gpu):
+    """Cuboids against the periodic x and y edges (IB points at i = 1..3 and
+    j = 30..32, sources of the wrapped ghost cells) and one across the
+    64-cell boundary of the x pass's Q row segments (nx = 128): the ghost
+    cells rewritten after IBMomentum from the set-up lists and the segment
+    masks of Q must give the oracle's bits, 1 and 3 steps."""
+    cfg = cases.uniform_config(128, 32, 16, dx=1.0, dt=0.05, nscalars=1)
+    o, s = pair(cfg, seed=5, cuboids=EDGE_OBSTACLES, sources=[(64.5, 4.5, 0.5, 1.0, 0)], u0=3.0)
+    for n in (1, 2):
+        o.step(n)
+        s.step(n)
+        err = compare_state(s, o, cfg)
+        assert max(err.values()) == 0.0, (n, err)
+
+
 @pytest.mark.parametrize("env", [{"ELMM_MOMENTUM": "percomp"}, {"ELMM_ZCHUNK": "5"}, {"ELMM_SCALAR": "cellwise"},
                                  {"ELMM_SGS": "cellwise"}, {"ELMM_GHOST_DELTA": "0"}])
 def test_momentum_kernel_variants_bit_identical(require_gpu, monkeypatch, env):
