@@ -163,21 +163,21 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
 // registers while level k is evaluated, then written into the slots level k
 // no longer needs.  Global loads per cell: one plane of U, V and W plus the
 // halo ring, instead of 21 stencil values.
-template <typename T, int TX, int TY, bool RCP>
+// RPT: rows per thread (the tile is TX x TY*RPT columns).
+template <typename T, int TX, int TY, int RPT, bool RCP>
 __global__ void __launch_bounds__(TX *TY)
     k_sgs_zm(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W, T *__restrict__ Visc,
              double nu_, double c_smag_, double filter_ratio, SgsWidth width_t, int sgs_model,
              double *__restrict__ cfl_partial, int zchunk) {
-  constexpr int NT = TX * TY, RX = TX + 2, RC = RX * (TY + 2), PER = (RC + NT - 1) / NT;
+  constexpr int NT = TX * TY, RX = TX + 2, RC = RX * (TY * RPT + 2), PER = (RC + NT - 1) / NT;
   // ring slots (U 0-2, V 3-5, W 6-7) and one spare cell per thread, written
   // by threads past the region, so the ring writes carry no branches
   __shared__ T ring[8 * RC + NT];
   constexpr int SP = 8 * RC;
   const int t = threadIdx.x, tx = t % TX, ty = t / TX;
-  const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
+  const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY * RPT + 1;
   const int kbeg = blockIdx.z * zchunk + 1, kend = min(g.nz, kbeg + zchunk - 1);
-  const int i = i0 + tx, j = j0 + ty;
-  const bool inside = i <= g.nx && j <= g.ny;
+  const int i = i0 + tx;
   const T nu = (T)nu_, c_smag = (T)c_smag_;
   // region cells of this thread (halo 1 around the tile), clamped into the
   // allocation; values of clamped cells only reach cells outside the domain
@@ -205,7 +205,6 @@ __global__ void __launch_bounds__(TX *TY)
   put(7 * RC, W, kbeg);
   __syncthreads();
   int u0 = 0, u1 = 1, u2 = 2, w0 = 0, w1 = 1;
-  const int c = (tx + 1) + RX * (ty + 1);
   double cfl = 0;
   for (int k = kbeg; k <= kend; k++) {
     const bool more = k < kend;
@@ -219,19 +218,24 @@ __global__ void __launch_bounds__(TX *TY)
         pw[q] = W[o - g.sz];
       }
     }
-    if (inside) {
-      const T *Ul = ring + u0 * RC, *Uc = ring + u1 * RC, *Uh = ring + u2 * RC;
-      const T *Vl = ring + (3 + u0) * RC, *Vc = ring + (3 + u1) * RC, *Vh = ring + (3 + u2) * RC;
-      const T *Wl = ring + (6 + w0) * RC, *Wc = ring + (6 + w1) * RC;
-      auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
-      auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
-      auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
-      Visc[gix(g, i, j, k)] = sgs_model == 1 ? smag_visc<T, RCP>(g, Uf, Vf, Wf, k,
-                                                                 smag_width<T, true>(g, width_t, filter_ratio, i, j, k),
-                                                                 nu, c_smag, width_t)
-                                             : nu;
-      if (cfl_partial)
-        cfl = fmax(cfl, cfl_cell(g, Uc[c], Uc[c - 1], Vc[c], Vc[c - RX], Wc[c], Wl[c], i, j, k));
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) {
+      const int j = j0 + ty + rr * TY;
+      if (i <= g.nx && j <= g.ny) {
+        const int c = (tx + 1) + RX * (ty + rr * TY + 1);
+        const T *Ul = ring + u0 * RC, *Uc = ring + u1 * RC, *Uh = ring + u2 * RC;
+        const T *Vl = ring + (3 + u0) * RC, *Vc = ring + (3 + u1) * RC, *Vh = ring + (3 + u2) * RC;
+        const T *Wl = ring + (6 + w0) * RC, *Wc = ring + (6 + w1) * RC;
+        auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
+        auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
+        auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
+        Visc[gix(g, i, j, k)] =
+            sgs_model == 1 ? smag_visc<T, RCP>(g, Uf, Vf, Wf, k, smag_width<T, true>(g, width_t, filter_ratio, i, j, k),
+                                               nu, c_smag, width_t)
+                           : nu;
+        if (cfl_partial)
+          cfl = fmax(cfl, cfl_cell(g, Uc[c], Uc[c - 1], Vc[c], Vc[c - RX], Wc[c], Wl[c], i, j, k));
+      }
     }
     if (more) {
       __syncthreads();
@@ -258,18 +262,29 @@ __global__ void __launch_bounds__(TX *TY)
   }
 }
 
-// 64 x 4 columns, 32 levels per block (16 measured equal, 64 12 % slower in
-// FP32: fewer blocks to hide the per-level load latency)
-// 64 x 4 columns (32 x 8 and 64 x 8 measured 3 % and 6 % slower), 32 levels
-// per block (16 measured equal, 64 12 % slower in FP32: fewer blocks to hide
-// the per-level load latency)
-constexpr int kSgsTX = 64, kSgsTY = 4, kSgsZChunk = 32;
+// 64 x 4 threads, two rows each (a 64 x 8 tile: 2 % faster than one row per
+// thread; 32 x 8 and 64 x 8 threads measured slower), 32 levels per block (16
+// measured equal, 64 12 % slower in FP32: fewer blocks to hide the per-level
+// load latency)
+constexpr int kSgsTX = 64, kSgsTY = 4, kSgsRPT = 2, kSgsZChunk = 32;
 
 static bool sgs_cellwise(const Geo &g, const SgsWidth &w) { return g.sgs_cellwise || g.variable_z || !w.tab; }
 
 int sgs_partials(const Geo &g, const SgsWidth &w) {
   if (sgs_cellwise(g, w)) return (int)(nblk(g.nx, 64) * nblk(g.ny, 4) * g.nz);
-  return (int)(nblk(g.nx, kSgsTX) * nblk(g.ny, kSgsTY) * nblk(g.nz, kSgsZChunk));
+  return (int)(nblk(g.nx, kSgsTX) * nblk(g.ny, kSgsTY * kSgsRPT) * nblk(g.nz, kSgsZChunk));
+}
+
+template <typename T, int RPT>
+static void launch_sgs_zm(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
+                          double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, hipStream_t st) {
+  dim3 gr(nblk(g.nx, kSgsTX), nblk(g.ny, kSgsTY * RPT), nblk(g.nz, kSgsZChunk));
+  if (sizeof(T) == 8 && width.rcp())
+    hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, RPT, true>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc,
+                       nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
+  else
+    hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, RPT, false>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc,
+                       nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
 }
 
 template <typename T>
@@ -281,13 +296,7 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
                        cfl_partial);
     return;
   }
-  dim3 gr(nblk(g.nx, kSgsTX), nblk(g.ny, kSgsTY), nblk(g.nz, kSgsZChunk));
-  if (sizeof(T) == 8 && width.rcp())
-    hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, true>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc, nu,
-                       c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
-  else
-    hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, false>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc, nu,
-                       c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
+  launch_sgs_zm<T, kSgsRPT>(g, U, V, W, Visc, nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, st);
 }
 
 // ------------------------------------------------------------- wall model
