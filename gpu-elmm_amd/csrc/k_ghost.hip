@@ -137,7 +137,8 @@ __global__ void k_bound_uvw(Geo g, T *U, T *V, T *W, Shell su, Shell sv, Shell s
   int i, j, k, si, sj, sk, neg;
   bool zero;
   if (!uvw_rule(g, s, c, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k, si, sj, sk, neg, zero)) return;
-  T v = zero ? T(0) : F[gix(g, si, sj, sk)];
+  const T src = F[gix(g, si, sj, sk)];  // in the allocation also for a zero rule (load without a branch)
+  const T v = zero ? T(0) : src;
   F[gix(g, i, j, k)] = neg ? -v : v;
 }
 
