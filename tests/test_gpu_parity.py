@@ -76,6 +76,21 @@ def test_bit_identical_to_oracle(require_gpu):
     assert max(err.values()) == 0.0, err
 
 
+@pytest.mark.parametrize("dx", [0.09, 0.11])
+def test_bit_identical_without_qualifying_reciprocals(require_gpu, dx):
+    """Grid spacings whose rounded reciprocal fails the correction step's
+    condition (|RN(1/d) d - 1| > 2^-54 for d = 0.09 and 0.11) select the
+    kernel instantiations that divide: still the oracle's bits, with an
+    obstacle (IB mass sources in the x pass)."""
+    cfg = cases.uniform_config(32, 32, 16, dx=dx, dt=0.004, nscalars=1)
+    cub = [(dx * 8, dx * 16, dx * 8, dx * 16, 0.0, dx * 6)]
+    o, s = pair(cfg, seed=6, cuboids=cub, u0=3.0 * dx)
+    o.step(3)
+    s.step(3)
+    err = compare_state(s, o, cfg)
+    assert max(err.values()) == 0.0, err
+
+
 def test_deterministic_across_runs(require_gpu):
     cfg = cases.config1(32)
     _, s1 = pair(cfg, scalar_amp=0.5)
