@@ -1004,18 +1004,20 @@ void Engine<T>::stage(int st, bool last) {
                                  stream);
     });
   }
-  for (int c = 0; c < 3; c++) {
-    if (!fused_momentum)
+  if (!fused_momentum)
+    for (int c = 0; c < 3; c++)
       timed(ELMM_T_MOMENTUM, [&] {
         launch_momentum(g, c, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar[c], U2p[c], code[c].as<uint8_t>(),
                         sc, stream);
       });
-    if (mom_terms[c].n)
-      timed(ELMM_T_MOMENTUM_CORR, [&] {
-        launch_momentum_corr(g, c, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar[c], U2p[c],
-                             code[c].as<uint8_t>(), sc, mom_terms[c].view(), wm_fluxp.as<double>(),
-                             wm_fluxm.as<double>(), stream);
-      });
+  if (mom_terms[0].n + mom_terms[1].n + mom_terms[2].n) {
+    // the three components' ordered corrections read only the stage-start
+    // fields and write disjoint arrays: one launch
+    const TermList tl3[3] = {mom_terms[0].view(), mom_terms[1].view(), mom_terms[2].view()};
+    timed(ELMM_T_MOMENTUM_CORR, [&] {
+      launch_momentum_corr3(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, sc, tl3, wm_fluxp.as<double>(),
+                            wm_fluxm.as<double>(), stream);
+    });
   }
   if (nsc > 0) {
     // ScalarRK3 (src/scalars.f90:88-299) with the stage-start velocity;

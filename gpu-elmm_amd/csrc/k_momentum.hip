@@ -729,13 +729,13 @@ void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W,
 // A negative term (the cell before the face) carries fluxm, a positive one
 // fluxp.
 template <typename T, int C, int VZ>
-__global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restrict__ V,
-                                const T *__restrict__ W, const T *__restrict__ Visc,
-                                const T *__restrict__ Pr, T *__restrict__ Ustar, T *__restrict__ U2,
-                                const uint8_t *__restrict__ code, StageConst sc, TermList tl,
-                                const double *__restrict__ fluxp, const double *__restrict__ fluxm) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= tl.n) return;
+__device__ inline void corr_cell(const Geo &g, const T *__restrict__ U, const T *__restrict__ V,
+                                 const T *__restrict__ W, const T *__restrict__ Visc, const T *__restrict__ Pr,
+                                 T *__restrict__ Ustar, T *__restrict__ U2, const StageConst &sc,
+                                 const int32_t *__restrict__ cell, const int32_t *__restrict__ ptr,
+                                 const int32_t *__restrict__ term, const double *__restrict__ fluxp,
+                                 const double *__restrict__ fluxm, int c) {
+  const TermList tl{0, cell, ptr, term};
   const int i = tl.cell[3 * c], j = tl.cell[3 * c + 1], k = tl.cell[3 * c + 2];
   const long p = gix(g, i, j, k);
   const T *F = C == 0 ? U : C == 1 ? V : W;
@@ -759,6 +759,56 @@ __global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restr
     }
   }
   finish<T, C>(g, sc, F, Pr, Ustar, U2, p, k, w, us_prev);
+}
+
+template <typename T, int C, int VZ>
+__global__ void k_momentum_corr(Geo g, const T *__restrict__ U, const T *__restrict__ V,
+                                const T *__restrict__ W, const T *__restrict__ Visc,
+                                const T *__restrict__ Pr, T *__restrict__ Ustar, T *__restrict__ U2,
+                                const uint8_t *__restrict__ code, StageConst sc, TermList tl,
+                                const double *__restrict__ fluxp, const double *__restrict__ fluxm) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= tl.n) return;
+  corr_cell<T, C, VZ>(g, U, V, W, Visc, Pr, Ustar, U2, sc, tl.cell, tl.ptr, tl.term, fluxp, fluxm, c);
+}
+
+// The three components' corrections in one launch (they touch disjoint
+// arrays: component c reads the stage-start field and writes its own Ustar
+// and U2), so their short dependent load chains overlap.
+template <typename T>
+struct Corr3 {
+  TermList t0, t1, t2;
+  T *us0, *us1, *us2, *u20, *u21, *u22;
+};
+
+template <typename T, int VZ>
+__global__ void k_momentum_corr3(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+                                 const T *__restrict__ Visc, const T *__restrict__ Pr, Corr3<T> a, StageConst sc,
+                                 const double *__restrict__ fluxp, const double *__restrict__ fluxm) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n0 = a.t0.n, n1 = a.t1.n;
+  if (c < n0) {
+    corr_cell<T, 0, VZ>(g, U, V, W, Visc, Pr, a.us0, a.u20, sc, a.t0.cell, a.t0.ptr, a.t0.term, fluxp, fluxm, c);
+  } else if ((c -= n0) < n1) {
+    corr_cell<T, 1, VZ>(g, U, V, W, Visc, Pr, a.us1, a.u21, sc, a.t1.cell, a.t1.ptr, a.t1.term, fluxp, fluxm, c);
+  } else if ((c -= n1) < a.t2.n) {
+    corr_cell<T, 2, VZ>(g, U, V, W, Visc, Pr, a.us2, a.u22, sc, a.t2.cell, a.t2.ptr, a.t2.term, fluxp, fluxm, c);
+  }
+}
+
+template <typename T>
+void launch_momentum_corr3(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
+                           T *const Ustar[3], T *const U2[3], StageConst sc, const TermList tl[3],
+                           const double *fluxp, const double *fluxm, hipStream_t st) {
+  const int tot = tl[0].n + tl[1].n + tl[2].n;
+  if (tot == 0) return;
+  const Corr3<T> a{tl[0], tl[1], tl[2], Ustar[0], Ustar[1], Ustar[2], U2[0], U2[1], U2[2]};
+  if (g.variable_z)
+    hipLaunchKernelGGL((k_momentum_corr3<T, 1>), dim3(nblk(tot, 128)), dim3(128), 0, st, g, U, V, W, Visc, Pr, a, sc,
+                       fluxp, fluxm);
+  else
+    hipLaunchKernelGGL((k_momentum_corr3<T, 0>), dim3(nblk(tot, 128)), dim3(128), 0, st, g, U, V, W, Visc, Pr, a, sc,
+                       fluxp, fluxm);
 }
 
 template <typename T, int VZ>
@@ -950,6 +1000,9 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
                                     hipStream_t);                                                                  \
   template void launch_momentum<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *, T *,   \
                                    T *, const uint8_t *, StageConst, hipStream_t);                                 \
+  template void launch_momentum_corr3<T>(const Geo &, const T *, const T *, const T *, const T *, const T *,      \
+                                         T *const[3], T *const[3], StageConst, const TermList[3], const double *,  \
+                                         const double *, hipStream_t);                                             \
   template void launch_momentum_corr<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *,   \
                                         T *, T *, const uint8_t *, StageConst, TermList, const double *,           \
                                         const double *, hipStream_t);                                              \

@@ -93,6 +93,12 @@ template <typename T>
 void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
                           T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
                           const double *fluxm, hipStream_t st);
+// The three components' corrections in one launch (same arithmetic as
+// launch_momentum_corr per component).
+template <typename T>
+void launch_momentum_corr3(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
+                           T *const Ustar[3], T *const U2[3], StageConst sc, const TermList tl[3],
+                           const double *fluxp, const double *fluxm, hipStream_t st);
 // IBMomentum of the three components: interpolation of every point (tmp
 // holds ib[0].n + ib[1].n + ib[2].n values), then the scatter.
 template <typename T>
