@@ -9,7 +9,9 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <type_traits>
 #include <cstring>
@@ -77,28 +79,64 @@ struct DevTerms {
   TermList view() const { return TermList{n, cell.as<int32_t>(), ptr.as<int32_t>(), term.as<int32_t>()}; }
 };
 
-// Host-side accumulation of ordered per-cell terms, keyed by target cell.
-struct TermBuilder {
-  std::vector<std::array<int, 3>> cells;
-  std::vector<std::vector<std::array<int32_t, 4>>> terms;
-  std::unordered_map<long, int> index;
-  int cell_of(long key, int i, int j, int k) {
-    auto it = index.find(key);
-    if (it != index.end()) return it->second;
-    int c = (int)cells.size();
-    index.emplace(key, c);
-    cells.push_back({i, j, k});
-    terms.emplace_back();
-    return c;
+// Set-up phase times on stderr when ELMM_SETUP_TRACE is set (each phase ends
+// with a device synchronisation, so its time includes its device work).
+struct SetupTrace {
+  bool on = std::getenv("ELMM_SETUP_TRACE") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void mark(const char *what) {
+    if (!on) return;
+    (void)hipDeviceSynchronize();
+    const auto n = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[elmm setup] %-14s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(n - t).count());
+    t = n;
   }
-  void add(long key, int i, int j, int k, std::array<int32_t, 4> t) { terms[cell_of(key, i, j, k)].push_back(t); }
-  void upload(DevTerms &d) const {
-    std::vector<int32_t> c, p{0}, t;
-    for (size_t q = 0; q < cells.size(); q++) {
-      c.insert(c.end(), cells[q].begin(), cells[q].end());
-      for (auto &x : terms[q]) t.insert(t.end(), x.begin(), x.end());
-      p.push_back((int32_t)(t.size() / 4));
+};
+
+// Host-side accumulation of ordered per-cell terms, keyed by target cell
+// (the element index of the cell in the field layout, 0 <= key < nkeys):
+// cells in order of first appearance, each cell's terms in insertion order.
+// The key -> cell map is a dense array from calloc, whose untouched pages
+// cost nothing; terms are kept flat and grouped per cell by a stable
+// counting sort.
+struct TermBuilder {
+  using Term = std::array<int32_t, 4>;
+  std::vector<std::array<int, 3>> cells;
+  std::vector<long> keys;                    // per cell
+  std::vector<std::pair<int, Term>> items;   // (cell, term), insertion order
+  long nkeys;
+  std::unique_ptr<int, decltype(&std::free)> slot{nullptr, &std::free};  // key -> cell + 1
+  explicit TermBuilder(long n) : nkeys(n) {}
+  int find(long key) const { return slot ? slot.get()[key] - 1 : -1; }
+  int cell_of(long key, int i, int j, int k) {
+    if (key < 0 || key >= nkeys) throw ArgError("correction term outside the field allocation");
+    if (!slot) {
+      slot.reset(static_cast<int *>(std::calloc((size_t)nkeys, sizeof(int))));
+      if (!slot) throw std::bad_alloc();
     }
+    int &s = slot.get()[key];
+    if (s == 0) {
+      cells.push_back({i, j, k});
+      keys.push_back(key);
+      s = (int)cells.size();
+    }
+    return s - 1;
+  }
+  void add(long key, int i, int j, int k, const Term &t) { items.emplace_back(cell_of(key, i, j, k), t); }
+  // CSR form: ptr (cells + 1) and the terms grouped per cell
+  void csr(std::vector<int32_t> &ptr, std::vector<int32_t> &flat) const {
+    ptr.assign(cells.size() + 1, 0);
+    for (auto &it : items) ptr[it.first + 1]++;
+    for (size_t q = 0; q < cells.size(); q++) ptr[q + 1] += ptr[q];
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    flat.assign(4 * items.size(), 0);
+    for (auto &it : items) std::memcpy(&flat[4 * (size_t)pos[it.first]++], it.second.data(), sizeof(Term));
+  }
+  void upload(DevTerms &d) const {
+    std::vector<int32_t> c(3 * cells.size()), p, t;
+    for (size_t q = 0; q < cells.size(); q++)
+      for (int e = 0; e < 3; e++) c[3 * q + e] = cells[q][e];
+    csr(p, t);
     d.n = (int)cells.size();
     upload_vec(d.cell, c);
     upload_vec(d.ptr, p);
@@ -509,39 +547,39 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
     for (int l = 0; l < 18; l++) extra = extra || (l % 6 < 4 && L.wm_n[l]);
     if (extra) throw ArgError("solid bodies on stretched z grids are not built in this version");
   }
+  SetupTrace tr;
   auto lin = [&](int i, int j, int k) { return gix(g, i, j, k); };
-  // host copies of masks in the device layout
-  auto remap_i32 = [&](const int32_t *src) {
-    std::vector<int32_t> out(ntot, 0);
+  // host copies of masks in the device layout: the input arrays span
+  // -2..n+3 in each direction, x fastest, so every (j, k) row is one
+  // contiguous run in both layouts
+  auto remap = [&](auto *out, const auto *src, auto f) {
     const long hy = cfg.nx + 6, hz = (long)(cfg.nx + 6) * (cfg.ny + 6);
     for (int k = -2; k <= cfg.nz + 3; k++)
-      for (int j = -2; j <= cfg.ny + 3; j++)
-        for (int i = -2; i <= cfg.nx + 3; i++) out[lin(i, j, k)] = src[(i + 2) + (j + 2) * hy + (k + 2) * hz];
-    return out;
+      for (int j = -2; j <= cfg.ny + 3; j++) {
+        const auto *s = src + (j + 2) * hy + (k + 2) * hz;
+        auto *o = out + lin(-2, j, k);
+        for (int i = 0; i < cfg.nx + 6; i++) o[i] = f(s[i]);
+      }
   };
-  auto remap_u8 = [&](const uint8_t *src) {
-    std::vector<uint8_t> out(ntot, 0);
-    const long hy = cfg.nx + 6, hz = (long)(cfg.nx + 6) * (cfg.ny + 6);
-    for (int k = -2; k <= cfg.nz + 3; k++)
-      for (int j = -2; j <= cfg.ny + 3; j++)
-        for (int i = -2; i <= cfg.nx + 3; i++) out[lin(i, j, k)] = src[(i + 2) + (j + 2) * hy + (k + 2) * hz];
-    return out;
-  };
-  std::vector<int32_t> types[3] = {remap_i32(L.utype), remap_i32(L.vtype), remap_i32(L.wtype)};
-  std::vector<int32_t> pt = remap_i32(L.prtype);
   std::vector<uint8_t> codes[3];
+  const int32_t *types[3] = {L.utype, L.vtype, L.wtype};
   for (int c = 0; c < 3; c++) {
     codes[c].assign(ntot, 0);
-    for (long p = 0; p < ntot; p++) {
-      int t = types[c][p];
-      codes[c][p] = t == 0 ? 0 : t < 0 ? 1 : 2;
-    }
+    remap(codes[c].data(), types[c], [](int32_t t) -> uint8_t { return t == 0 ? 0 : t < 0 ? 1 : 2; });
+  }
+  {
+    std::vector<int32_t> pt(ntot, 0);
+    remap(pt.data(), L.prtype, [](int32_t t) { return t; });
+    upload_vec(prtype, pt);
   }
   std::vector<uint8_t> scode(ntot, 0);
-  for (long p = 0; p < ntot; p++)
-    if (pt[p] > 0) scode[p] = CODE_SOLID;
-  upload_vec(prtype, pt);
-  for (int d = 0; d < 3; d++) upload_vec(mask[d], remap_u8(L.scfl_mask[d]));
+  remap(scode.data(), L.prtype, [](int32_t t) -> uint8_t { return t > 0 ? CODE_SOLID : 0; });
+  for (int d = 0; d < 3; d++) {
+    std::vector<uint8_t> m(ntot, 0);
+    remap(m.data(), L.scfl_mask[d], [](uint8_t t) { return t; });
+    upload_vec(mask[d], m);
+  }
+  tr.mark(" masks");
 
   // wall-model points of the 18 momentum lists, concatenated in list order
   std::vector<int32_t> wijk, wlist;
@@ -577,7 +615,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   // ordered momentum corrections (wmfluxes-nobranch-inc.f90:49-208)
   if (cfg.wall_model) {
     for (int c = 0; c < 3; c++) {
-      TermBuilder tb;
+      TermBuilder tb(ntot);
       const int *n = g.n[c];
       for (int dir = 0; dir < 6; dir++) {
         const int l = c * 6 + dir, ax = dir / 2, plus = dir % 2;
@@ -604,6 +642,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
     }
   }
   for (int c = 0; c < 3; c++) upload_vec(code[c], codes[c]);
+  tr.mark(" wall model");
 
   // scalar wall-model points and ordered scalar terms
   wmp_n = cfg.wall_model ? L.wmp_n : 0;
@@ -615,7 +654,8 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   upload_vec(src_flux, sflux);
   for (int s = 0; s < nsc; s++) sc_terms.emplace_back(new DevTerms);
   if (nsc > 0) {
-    std::vector<TermBuilder> tbs(nsc);
+    std::vector<TermBuilder> tbs;
+    for (int s = 0; s < nsc; s++) tbs.emplace_back(ntot);
     auto interior = [&](int i, int j, int k) {
       return i >= 1 && i <= cfg.nx && j >= 1 && j <= cfg.ny && k >= 1 && k <= cfg.nz;
     };
@@ -642,25 +682,31 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       }
     }
     // every scalar lists the union of corrected cells (same cell order)
-    TermBuilder uni;
+    TermBuilder uni(ntot);
     for (int s = 0; s < nsc; s++)
       for (size_t q = 0; q < tbs[s].cells.size(); q++) {
         auto &cc = tbs[s].cells[q];
-        uni.cell_of(lin(cc[0], cc[1], cc[2]), cc[0], cc[1], cc[2]);
+        uni.cell_of(tbs[s].keys[q], cc[0], cc[1], cc[2]);
       }
     for (int s = 0; s < nsc; s++) {
-      TermBuilder out;
-      for (auto &cc : uni.cells) {
-        long key = lin(cc[0], cc[1], cc[2]);
-        out.cell_of(key, cc[0], cc[1], cc[2]);
-        auto it = tbs[s].index.find(key);
-        if (it != tbs[s].index.end()) out.terms[out.index[key]] = tbs[s].terms[it->second];
+      std::vector<int32_t> sp, st;
+      tbs[s].csr(sp, st);
+      TermBuilder out(ntot);
+      for (size_t q = 0; q < uni.cells.size(); q++) {
+        const auto &cc = uni.cells[q];
+        const long key = uni.keys[q];
+        const int oc = out.cell_of(key, cc[0], cc[1], cc[2]);
+        const int ic = tbs[s].find(key);
+        if (ic < 0) continue;
+        for (int t = sp[ic]; t < sp[ic + 1]; t++)
+          out.items.emplace_back(oc, TermBuilder::Term{st[4 * t], st[4 * t + 1], st[4 * t + 2], st[4 * t + 3]});
       }
       out.upload(*sc_terms[s]);
     }
-    for (auto &cc : uni.cells) scode[lin(cc[0], cc[1], cc[2])] |= CODE_CORRECTED;
+    for (long key : uni.keys) scode[key] |= CODE_CORRECTED;
   }
   upload_vec(code_s, scode);
+  tr.mark(" scalar terms");
 
   // immersed-boundary lists
   for (int c = 0; c < 4; c++) {
@@ -721,10 +767,12 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
     upload_vec(ib_coef[c], cf);
   }
   ib_tmp.alloc((size_t)std::max(ib_n[0] + ib_n[1] + ib_n[2], 1) * sizeof(double));
+  tr.mark(" ib lists");
   setup_ghost_delta();
+  tr.mark(" ghost delta");
   // IBMassSources gather terms (src/tsteps.f90:490-531) + Bound_Q fold targets
   if (cfg.ib_mass_sources && has_ib()) {
-    TermBuilder tb;
+    TermBuilder tb(ntot);
     for (int c = 0; c < 3; c++)
       for (int p = 0; p < ib_n[c]; p++) {
         int q[3] = {L.ib_ijk[c][3 * p], L.ib_ijk[c][3 * p + 1], L.ib_ijk[c][3 * p + 2]};
@@ -768,6 +816,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
     std::vector<int32_t> v(L.null_ijk[c], L.null_ijk[c] + 3 * (size_t)null_n[c]);
     upload_vec(null_ijk[c], v);
   }
+  tr.mark(" mass sources");
 }
 
 template <typename T>
@@ -789,8 +838,11 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     cfg.zw = zw_copy.data();
   }
   nsc = cfg.nscalars;
+  SetupTrace tr;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  tr.mark("stream");
   setup_geo();
+  tr.mark("geometry");
   const size_t fb = g.ntot * sizeof(T), fb64 = g.ntot * sizeof(double);
   for (DevBuf *b : {&U, &V, &W, &U2, &V2, &W2, &Us, &Vs, &Ws, &Pr, &Visc, &TDiff}) b->alloc(fb);
   for (DevBuf *b : {&Q, &Phi, &RHS}) b->alloc(fb64);
@@ -810,9 +862,12 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   u2 = U2.as<T>();
   v2 = V2.as<T>();
   w2 = W2.as<T>();
+  tr.mark("fields");
   setup_lists(L);
+  tr.mark("lists");
   setup_poisson();
   HIPCHK(hipDeviceSynchronize());
+  tr.mark("poisson");
 }
 
 template <typename T>

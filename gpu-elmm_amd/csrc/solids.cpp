@@ -336,19 +336,31 @@ void set_type(const Geometry &G, const Body &B, CaseData &cd, std::vector<int32_
   }
 }
 
-// AuxNeighbours: free cells touching a solid one (27-point) become -1
+// AuxNeighbours: free cells (i, j, k in -1..ext+2) with a solid cell among
+// their 27 neighbours become -1.  Solid cells are never changed, so the test
+// reads the solid flags of the input only, and "a neighbour is solid" is an
+// OR over the 3x3x3 box, computed as three one-axis ORs (x, then y, then z).
 void aux_neighbours(CaseData &cd, std::vector<int32_t> &T, const int ext[3]) {
+  const long n = cd.ntot;
+  std::vector<uint8_t> a(n, 0), b(n, 0);
+  for (long p = 0; p < n; p++) a[p] = T[p] > 0;
+  for (int k = -2; k <= ext[2] + 3; k++)
+    for (int j = -2; j <= ext[1] + 3; j++) {
+      const long r = cd.lin(0, j, k);
+      for (int i = -1; i <= ext[0] + 2; i++) b[r + i] = a[r + i - 1] | a[r + i] | a[r + i + 1];
+    }
+  const long sy = cd.sy, sz = cd.sz;
+  for (int k = -2; k <= ext[2] + 3; k++)
+    for (int j = -1; j <= ext[1] + 2; j++) {
+      const long r = cd.lin(0, j, k);
+      for (int i = -1; i <= ext[0] + 2; i++) a[r + i] = b[r + i - sy] | b[r + i] | b[r + i + sy];
+    }
   for (int k = -1; k <= ext[2] + 2; k++)
-    for (int j = -1; j <= ext[1] + 2; j++)
-      for (int i = -1; i <= ext[0] + 2; i++) {
-        if (T[cd.lin(i, j, k)] != 0) continue;
-        bool any = false;
-        for (int c = -1; c <= 1 && !any; c++)
-          for (int b = -1; b <= 1 && !any; b++)
-            for (int a = -1; a <= 1 && !any; a++)
-              if (T[cd.lin(i + a, j + b, k + c)] > 0) any = true;
-        if (any) T[cd.lin(i, j, k)] = -1;
-      }
+    for (int j = -1; j <= ext[1] + 2; j++) {
+      const long r = cd.lin(0, j, k);
+      for (int i = -1; i <= ext[0] + 2; i++)
+        if (T[r + i] == 0 && (a[r + i - sz] | a[r + i] | a[r + i + sz])) T[r + i] = -1;
+    }
 }
 
 struct Tap {
