@@ -265,31 +265,24 @@ static void init_scalar_masks(const elmm_config &c, CaseData &cd) {
   for (int d = 0; d < 3; d++) cd.scfl[d].clear();
   if (c.wall_model == 0) return;
   const int nx = c.nx, ny = c.ny, nz = c.nz;
-  auto solid = [&](int i, int j, int k) { return cd.prtype[cd.lin(i, j, k)] > 0; };
-  for (int k = 1; k <= nz; k++)
-    for (int j = 1; j <= ny; j++)
-      for (int i = 0; i <= nx; i++)
-        if (solid(i, j, k) || solid(i + 1, j, k)) cd.scfl_mask[0][cd.lin(i, j, k)] = 0;
-  for (int k = 1; k <= nz; k++)
-    for (int j = 0; j <= ny; j++)
-      for (int i = 1; i <= nx; i++)
-        if (solid(i, j, k) || solid(i, j + 1, k)) cd.scfl_mask[1][cd.lin(i, j, k)] = 0;
-  for (int k = 0; k <= nz; k++)
-    for (int j = 1; j <= ny; j++)
-      for (int i = 1; i <= nx; i++)
-        if (solid(i, j, k) || solid(i, j, k + 1)) cd.scfl_mask[2][cd.lin(i, j, k)] = 0;
-  for (int k = 1; k <= nz; k++)
-    for (int j = 1; j <= ny; j++)
-      for (int i = 0; i <= nx; i++)
-        if (solid(i, j, k) != solid(i + 1, j, k)) cd.scfl[0].insert(cd.scfl[0].end(), {i, j, k});
-  for (int k = 1; k <= nz; k++)
-    for (int j = 0; j <= ny; j++)
-      for (int i = 1; i <= nx; i++)
-        if (solid(i, j, k) != solid(i, j + 1, k)) cd.scfl[1].insert(cd.scfl[1].end(), {i, j, k});
-  for (int k = 0; k <= nz; k++)
-    for (int j = 1; j <= ny; j++)
-      for (int i = 1; i <= nx; i++)
-        if (solid(i, j, k) != solid(i, j, k + 1)) cd.scfl[2].insert(cd.scfl[2].end(), {i, j, k});
+  // face d of cell (i, j, k) lies between it and its +d neighbour: masked
+  // when either side is solid, listed when exactly one is (one task per
+  // direction, each in k, j, i order)
+  run_tasks(3, [&](int d) {
+    const long off = d == 0 ? 1 : d == 1 ? cd.sy : cd.sz;
+    const int lo[3] = {d == 0 ? 0 : 1, d == 1 ? 0 : 1, d == 2 ? 0 : 1};
+    const int32_t *pt = cd.prtype.data();
+    uint8_t *mask = cd.scfl_mask[d].data();
+    std::vector<int32_t> &list = cd.scfl[d];
+    for (int k = lo[2]; k <= nz; k++)
+      for (int j = lo[1]; j <= ny; j++)
+        for (int i = lo[0]; i <= nx; i++) {
+          const long p = cd.lin(i, j, k);
+          const bool a = pt[p] > 0, b = pt[p + off] > 0;
+          if (a || b) mask[p] = 0;
+          if (a != b) list.insert(list.end(), {i, j, k});
+        }
+  });
 }
 
 // point_source + GridCoords, src/pointsources.f90:20-41, src/boundaries.f90:54-92

@@ -3,6 +3,7 @@
 // with lower bound -2 and extents n+6, Fortran order).
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -60,5 +61,8 @@ void view_case(const CaseData &cd, elmm_lists &v);
 // Solid bodies (axis-aligned cuboids): masks, IB lists, solid-body WM points,
 // nullified points.  Defined in solids.cpp.
 int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cuboid *cub, CaseData &cd);
+// Runs f(0) .. f(n-1) on n threads (f(0) on the caller's) and joins them; an
+// exception in a task is rethrown after every task has finished.
+void run_tasks(int n, const std::function<void(int)> &f);
 
 }  // namespace elmm

@@ -30,10 +30,18 @@
 // same as testing every point); a singular 3x3 system in Closest, where the
 // reference would continue with unset values, is reported as an error;
 // norm2 is sqrt(x*x+y*y+z*z) (used only in comparisons and thresholds).
+// Builders that fill different arrays or lists (the four type arrays, the
+// four IB lists, the WM lists of Pr and of each velocity component, the
+// nullified points of each component) run on their own threads; each list
+// is still filled in the reference's order, so the result does not depend
+// on the threads.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <exception>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "case_setup.hpp"
@@ -633,6 +641,109 @@ bool has_free_face_neighbour(const CaseData &cd, const std::vector<int32_t> &T, 
 
 }  // namespace
 
+void run_tasks(int n, const std::function<void(int)> &f) {
+  std::vector<std::exception_ptr> ex(n);
+  auto one = [&](int a) {
+    try {
+      f(a);
+    } catch (...) {
+      ex[a] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int a = 1; a < n; a++) th.emplace_back(one, a);
+  one(0);
+  for (auto &t : th) t.join();
+  for (auto &e : ex)
+    if (e) std::rethrow_exception(e);
+}
+
+// GetSolidBodiesWM: free Pr points with a solid face neighbour (only cells
+// marked -1 by FindNeighbouringCells can have one).  Returns an error
+// message, empty on success.
+std::string solid_wm_pr(Geometry &G, const std::vector<Body> &bodies, CaseData &cd, double min_wall_distance) {
+  const elmm_config &c = G.c;
+  static const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int k = 1; k <= c.nz; k++)
+    for (int j = 1; j <= c.ny; j++)
+      for (int i = 1; i <= c.nx; i++) {
+        if (cd.prtype[cd.lin(i, j, k)] >= 0) continue;
+        double dist = DBL_MAX, nx_ = 0, ny_ = 0, nz_ = 0;
+        int body = 0;
+        for (int r = 0; r < 6; r++) {
+          const int t = cd.prtype[cd.lin(i + nb[r][0], j + nb[r][1], k + nb[r][2])];
+          if (t > 0 && t != body) {
+            if (!G.closest(bodies[t - 1], nx_, ny_, nz_, G.xPr(i), G.yPr(j), G.zPr(k))) return G.err;
+            const double dd = norm3(nx_ - G.xPr(i), ny_ - G.yPr(j), nz_ - G.zPr(k));
+            if (dd < dist) {
+              dist = dd;
+              body = t;
+            }
+          }
+        }
+        if (body == 0) continue;
+        const double dx = nx_ - G.xPr(i), dy = ny_ - G.yPr(j), dz = nz_ - G.zPr(k);
+        const double d = norm3(dx, dy, dz);
+        if (d == 0) return "zero distance wall-model point next to a solid body";
+        if (d < min_wall_distance) return "wall-model point too close to a solid body";
+        const double z0 = bodies[body - 1].z0 > 0 ? bodies[body - 1].z0 : 0.0;
+        cd.wmp.ijk.insert(cd.wmp.ijk.end(), {i, j, k});
+        cd.wmp.dist.insert(cd.wmp.dist.end(), {dx, dy, dz});
+        cd.wmp.z0.push_back(z0);
+      }
+  return std::string();
+}
+
+// GetSolidBodiesWM_UVW for one velocity component: -1 points, per direction.
+std::string solid_wm_uvw(Geometry &G, const std::vector<Body> &bodies, CaseData &cd, int comp,
+                         const std::vector<int32_t> &T, const std::vector<double> &cx, const std::vector<double> &cy,
+                         const std::vector<double> &cz, double min_wall_distance) {
+  static const int dvec[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  auto X = [&](int i) { return Grid1D::at(cx, i); };
+  auto Y = [&](int j) { return Grid1D::at(cy, j); };
+  auto Z = [&](int k) { return Grid1D::at(cz, k); };
+  for (int k = 1; k <= G.n[comp][2]; k++)
+    for (int j = 1; j <= G.n[comp][1]; j++)
+      for (int i = 1; i <= G.n[comp][0]; i++) {
+        if (T[cd.lin(i, j, k)] >= 0) continue;
+        for (int dir = 0; dir < 6; dir++) {
+          const int m = dvec[dir][0], n = dvec[dir][1], o = dvec[dir][2];
+          const int t = T[cd.lin(i + m, j + n, k + o)];
+          if (t <= 0) continue;
+          const Body &SB = bodies[t - 1];
+          double nx_, ny_, nz_;
+          if (!G.closest(SB, nx_, ny_, nz_, X(i), Y(j), Z(k))) return G.err;
+          double dv[3] = {nx_ - X(i), ny_ - Y(j), nz_ - Z(k)};
+          const double dist = norm3(dv[0], dv[1], dv[2]);
+          const double cosang =
+              (dv[0] * m + dv[1] * n + dv[2] * o) / (dist * norm3((double)m, (double)n, (double)o));
+          if (!(cosang > 0.4)) {
+            const double tl = G.closest_on_line_out(SB, X(i + m), Y(j + n), Z(k + o), X(i), Y(j), Z(k));
+            nx_ = X(i + m) + tl * (X(i) - X(i + m));
+            ny_ = Y(j + n) + tl * (Y(j) - Y(j + n));
+            nz_ = Z(k + o) + tl * (Z(k) - Z(k + o));
+            dv[0] = nx_ - X(i);
+            dv[1] = ny_ - Y(j);
+            dv[2] = nz_ - Z(k);
+            const double dl = norm3(dv[0], dv[1], dv[2]);
+            if (dl < dist && dl < min_wall_distance) {
+              if (!G.closest(SB, nx_, ny_, nz_, X(i), Y(j), Z(k))) return G.err;
+              dv[0] = nx_ - X(i);
+              dv[1] = ny_ - Y(j);
+              dv[2] = nz_ - Z(k);
+            }
+          }
+          if (norm3(dv[0], dv[1], dv[2]) < min_wall_distance)
+            return "velocity wall-model point too close to a solid body";
+          WMList &L = cd.wm[comp * 6 + dir];
+          L.ijk.insert(L.ijk.end(), {i, j, k});
+          L.dist.insert(L.dist.end(), {dv[0], dv[1], dv[2]});
+          L.z0.push_back(SB.z0 > 0 ? SB.z0 : 0.0);
+        }
+      }
+  return std::string();
+}
+
 int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cuboid *cub, CaseData &cd) {
   Geometry G(c, g);
   std::vector<Body> bodies;
@@ -648,130 +759,64 @@ int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cub
     ext[0][d] = d == 0 ? c.nx : d == 1 ? c.ny : c.nz;
     for (int comp = 0; comp < 3; comp++) ext[comp + 1][d] = G.n[comp][d];
   }
-  // FindInsideCells: body-major, Pr, U, V, W
-  for (const Body &B : bodies) {
-    set_type(G, B, cd, cd.prtype, g.xPr, g.yPr, g.zPr, ext[0], delta);
-    set_type(G, B, cd, cd.utype, g.xU, g.yPr, g.zPr, ext[1], delta);
-    set_type(G, B, cd, cd.vtype, g.xPr, g.yV, g.zPr, ext[2], delta);
-    set_type(G, B, cd, cd.wtype, g.xPr, g.yPr, g.zW, ext[3], delta);
-  }
-  // FindNeighbouringCells
-  aux_neighbours(cd, cd.prtype, ext[0]);
-  aux_neighbours(cd, cd.utype, ext[1]);
-  aux_neighbours(cd, cd.vtype, ext[2]);
-  aux_neighbours(cd, cd.wtype, ext[3]);
-  // InitImBoundaries (serial k, j, i order)
-  const std::vector<int32_t> *vt[3] = {&cd.utype, &cd.vtype, &cd.wtype};
-  const std::vector<double> *cx[3] = {&g.xU, &g.xPr, &g.xPr}, *cy[3] = {&g.yPr, &g.yV, &g.yPr},
-                            *cz[3] = {&g.zPr, &g.zPr, &g.zW};
-  for (int comp = 0; comp < 3; comp++) {
-    const std::vector<int32_t> &T = *vt[comp];
-    cd.ib[comp] = IBList();
-    for (int k = 1; k <= G.n[comp][2]; k++)
-      for (int j = 1; j <= G.n[comp][1]; j++)
-        for (int i = 1; i <= G.n[comp][0]; i++)
-          if (T[cd.lin(i, j, k)] > 0 && has_free_face_neighbour(cd, T, i, j, k))
-            push_ib(cd.ib[comp], i, j, k, vel_ib_point(G, bodies, cd, T, i, j, k, *cx[comp], *cy[comp], *cz[comp]));
-  }
-  cd.ib[3] = IBList();
-  for (int k = 1; k <= c.nz; k++)
-    for (int j = 1; j <= c.ny; j++)
-      for (int i = 1; i <= c.nx; i++)
-        if (cd.prtype[cd.lin(i, j, k)] > 0 && has_free_face_neighbour(cd, cd.prtype, i, j, k))
-          push_ib(cd.ib[3], i, j, k, scalfl_ib_point(G, bodies, cd, i, j, k));
-
-  if (c.wall_model) {
-    const double min_wall_distance = delta;
-    // GetSolidBodiesWM: Pr points next to a solid face neighbour
-    static const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-    for (int k = 1; k <= c.nz; k++)
-      for (int j = 1; j <= c.ny; j++)
-        for (int i = 1; i <= c.nx; i++) {
-          if (cd.prtype[cd.lin(i, j, k)] > 0) continue;
-          double dist = DBL_MAX, nx_ = 0, ny_ = 0, nz_ = 0;
-          int body = 0;
-          for (int r = 0; r < 6; r++) {
-            const int t = cd.prtype[cd.lin(i + nb[r][0], j + nb[r][1], k + nb[r][2])];
-            if (t > 0 && t != body) {
-              if (!G.closest(bodies[t - 1], nx_, ny_, nz_, G.xPr(i), G.yPr(j), G.zPr(k)))
-                return fail(ELMM_E_ARG, G.err);
-              const double dd = norm3(nx_ - G.xPr(i), ny_ - G.yPr(j), nz_ - G.zPr(k));
-              if (dd < dist) {
-                dist = dd;
-                body = t;
-              }
-            }
-          }
-          if (body == 0) continue;
-          const double dx = nx_ - G.xPr(i), dy = ny_ - G.yPr(j), dz = nz_ - G.zPr(k);
-          const double d = norm3(dx, dy, dz);
-          if (d == 0) return fail(ELMM_E_ARG, "zero distance wall-model point next to a solid body");
-          if (d < min_wall_distance) return fail(ELMM_E_ARG, "wall-model point too close to a solid body");
-          const double z0 = bodies[body - 1].z0 > 0 ? bodies[body - 1].z0 : 0.0;
-          cd.wmp.ijk.insert(cd.wmp.ijk.end(), {i, j, k});
-          cd.wmp.dist.insert(cd.wmp.dist.end(), {dx, dy, dz});
-          cd.wmp.z0.push_back(z0);
-        }
-    // GetSolidBodiesWM_UVW: -1 points of each component, per direction
-    static const int dvec[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    for (int comp = 0; comp < 3; comp++) {
-      const std::vector<int32_t> &T = *vt[comp];
-      auto X = [&](int i) { return Grid1D::at(*cx[comp], i); };
-      auto Y = [&](int j) { return Grid1D::at(*cy[comp], j); };
-      auto Z = [&](int k) { return Grid1D::at(*cz[comp], k); };
-      for (int k = 1; k <= G.n[comp][2]; k++)
-        for (int j = 1; j <= G.n[comp][1]; j++)
-          for (int i = 1; i <= G.n[comp][0]; i++) {
-            if (T[cd.lin(i, j, k)] >= 0) continue;
-            for (int dir = 0; dir < 6; dir++) {
-              const int m = dvec[dir][0], n = dvec[dir][1], o = dvec[dir][2];
-              const int t = T[cd.lin(i + m, j + n, k + o)];
-              if (t <= 0) continue;
-              const Body &SB = bodies[t - 1];
-              double nx_, ny_, nz_;
-              if (!G.closest(SB, nx_, ny_, nz_, X(i), Y(j), Z(k))) return fail(ELMM_E_ARG, G.err);
-              double dv[3] = {nx_ - X(i), ny_ - Y(j), nz_ - Z(k)};
-              const double dist = norm3(dv[0], dv[1], dv[2]);
-              const double cosang = (dv[0] * m + dv[1] * n + dv[2] * o) /
-                                    (dist * norm3((double)m, (double)n, (double)o));
-              if (!(cosang > 0.4)) {
-                const double tl = G.closest_on_line_out(SB, X(i + m), Y(j + n), Z(k + o), X(i), Y(j), Z(k));
-                nx_ = X(i + m) + tl * (X(i) - X(i + m));
-                ny_ = Y(j + n) + tl * (Y(j) - Y(j + n));
-                nz_ = Z(k + o) + tl * (Z(k) - Z(k + o));
-                dv[0] = nx_ - X(i);
-                dv[1] = ny_ - Y(j);
-                dv[2] = nz_ - Z(k);
-                const double dl = norm3(dv[0], dv[1], dv[2]);
-                if (dl < dist && dl < min_wall_distance) {
-                  if (!G.closest(SB, nx_, ny_, nz_, X(i), Y(j), Z(k))) return fail(ELMM_E_ARG, G.err);
-                  dv[0] = nx_ - X(i);
-                  dv[1] = ny_ - Y(j);
-                  dv[2] = nz_ - Z(k);
-                }
-              }
-              if (norm3(dv[0], dv[1], dv[2]) < min_wall_distance)
-                return fail(ELMM_E_ARG, "velocity wall-model point too close to a solid body");
-              WMList &L = cd.wm[comp * 6 + dir];
-              L.ijk.insert(L.ijk.end(), {i, j, k});
-              L.dist.insert(L.dist.end(), {dv[0], dv[1], dv[2]});
-              L.z0.push_back(SB.z0 > 0 ? SB.z0 : 0.0);
-            }
-          }
+  // per-array and per-component tasks: T[0] Pr, T[1..3] U, V, W
+  std::vector<int32_t> *T[4] = {&cd.prtype, &cd.utype, &cd.vtype, &cd.wtype};
+  const std::vector<double> *cx[4] = {&g.xPr, &g.xU, &g.xPr, &g.xPr}, *cy[4] = {&g.yPr, &g.yPr, &g.yV, &g.yPr},
+                            *cz[4] = {&g.zPr, &g.zPr, &g.zPr, &g.zW};
+  // FindInsideCells (body-major per array) and FindNeighbouringCells
+  run_tasks(4, [&](int a) {
+    for (const Body &B : bodies) set_type(G, B, cd, *T[a], *cx[a], *cy[a], *cz[a], ext[a], delta);
+    aux_neighbours(cd, *T[a], ext[a]);
+  });
+  // InitImBoundaries (serial k, j, i order per list); lists 0-2 velocity,
+  // 3 scalar flux
+  run_tasks(4, [&](int a) {
+    Geometry Gt(c, g);
+    IBList &L = cd.ib[a];
+    L = IBList();
+    if (a < 3) {
+      const std::vector<int32_t> &Tc = *T[a + 1];
+      for (int k = 1; k <= G.n[a][2]; k++)
+        for (int j = 1; j <= G.n[a][1]; j++)
+          for (int i = 1; i <= G.n[a][0]; i++)
+            if (Tc[cd.lin(i, j, k)] > 0 && has_free_face_neighbour(cd, Tc, i, j, k))
+              push_ib(L, i, j, k, vel_ib_point(Gt, bodies, cd, Tc, i, j, k, *cx[a + 1], *cy[a + 1], *cz[a + 1]));
+    } else {
+      for (int k = 1; k <= c.nz; k++)
+        for (int j = 1; j <= c.ny; j++)
+          for (int i = 1; i <= c.nx; i++)
+            if (cd.prtype[cd.lin(i, j, k)] > 0 && has_free_face_neighbour(cd, cd.prtype, i, j, k))
+              push_ib(L, i, j, k, scalfl_ib_point(Gt, bodies, cd, i, j, k));
     }
-  }
-  // SetNullifiedPoints: velocity points whose 7-point stencil is solid
-  for (int comp = 0; comp < 3; comp++) {
-    const std::vector<int32_t> &T = *vt[comp];
+  });
+
+  // wall-model points (task 0: Pr, tasks 1-3: U, V, W) and nullified points
+  // (tasks 1-3); a task that finds an error stops and keeps its message, and
+  // the first one in task order is reported (the one the serial order meets
+  // first)
+  std::string errs[4];
+  const double min_wall_distance = delta;
+  run_tasks(4, [&](int a) {
+    Geometry Gt(c, g);
+    if (c.wall_model && a == 0) errs[0] = solid_wm_pr(Gt, bodies, cd, min_wall_distance);
+    if (a == 0) return;
+    const int comp = a - 1;
+    if (c.wall_model) errs[a] = solid_wm_uvw(Gt, bodies, cd, comp, *T[a], *cx[a], *cy[a], *cz[a], min_wall_distance);
+    if (!errs[a].empty()) return;
+    // SetNullifiedPoints: velocity points whose 7-point stencil is solid
+    const std::vector<int32_t> &Tc = *T[a];
     cd.null_ijk[comp].clear();
     for (int k = 1; k <= G.n[comp][2]; k++)
       for (int j = 1; j <= G.n[comp][1]; j++)
         for (int i = 1; i <= G.n[comp][0]; i++) {
-          auto s = [&](int a, int b, int cc) { return T[cd.lin(i + a, j + b, k + cc)] > 0; };
-          if (s(0, 0, 0) && s(0, 0, 1) && s(0, 0, -1) && s(0, -1, 0) && s(0, 1, 0) && s(-1, 0, 0) && s(1, 0, 0))
+          auto sol = [&](int p, int q, int r) { return Tc[cd.lin(i + p, j + q, k + r)] > 0; };
+          if (sol(0, 0, 0) && sol(0, 0, 1) && sol(0, 0, -1) && sol(0, -1, 0) && sol(0, 1, 0) && sol(-1, 0, 0) &&
+              sol(1, 0, 0))
             cd.null_ijk[comp].insert(cd.null_ijk[comp].end(), {i, j, k});
         }
-  }
+  });
+  for (const std::string &e : errs)
+    if (!e.empty()) return fail(ELMM_E_ARG, e);
   return 0;
 }
 
