@@ -1133,9 +1133,12 @@ void Engine<T>::stage(int st, bool last) {
 template <typename T>
 void Engine<T>::init() {
   // src/initial.f90:4284-4342
+  SetupTrace tr;
   bound_uvw(u, v, w);
   launch_bound_pr<T>(g, Pr.as<T>(), stream);
+  tr.mark("init bounds");
   pressure_correction(u, v, w, false, 1.0);
+  tr.mark("init projection");
   launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
              sgs_width, cfg.sgs_model, nullptr, 0, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
@@ -1148,6 +1151,7 @@ void Engine<T>::init() {
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
+  tr.mark("init rest");
   initialized = true;
 }
 
