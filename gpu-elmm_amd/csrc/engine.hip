@@ -503,6 +503,9 @@ void Engine<T>::setup_poisson() {
     pp64 = PoissonPlan<double>{nx, ny, nz, M, Mp, X64.as<C2<double>>(), twx64.as<C2<double>>(), twy64.as<C2<double>>(),
                                lamx.as<double>(), lamy.as<double>(), az64.as<double>(), cz64.as<double>(), g.dzPr,
                                g.dzW};
+  // ELMM_ZPIPE=0/1 forces the z solve's load pipelining off/on (default: on
+  // for grids with more than 1024 wavefronts of modes)
+  if (const char *zp = std::getenv("ELMM_ZPIPE")) pp.zpipe = pp64.zpipe = std::atoi(zp) ? 1 : 0;
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
   cfl_blocks = sgs_partials(g, sgs_width);

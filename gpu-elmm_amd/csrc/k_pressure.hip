@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(PoissonPlan<R> pp, int in
 // The singular mode (m = j = 0) pins x_1 = 0 and is integrated exactly.
 constexpr int ZSEG = 16, ZSEG_MAX = 16;  // nz <= 256
 
-template <typename R>
+template <typename R, bool PIPE>
 __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
   __shared__ double ck[3][ZSEG_MAX][64];
   __shared__ double gm[2][256];
@@ -354,28 +354,73 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
   }
   const double lam = pp.lamx[m] + pp.lamy[j];
   // forward elimination; den = b - a c', every division a multiplication by
-  // 1/den
+  // 1/den.  PIPE (grids with several wavefronts per CU): the levels are
+  // loaded one segment ahead of the elimination (the load index clamped into
+  // the column), so that a wavefront keeps a segment of loads in flight while
+  // it works through the dependent chain.  Without PIPE (few modes, one
+  // wavefront per CU, measured faster there) each level is loaded where it
+  // is used, the loop unrolled by 8.  Same operations in the same order.
   double cp = 0, dr = 0, di = 0;
+  C2<R> cur[ZSEG], nxt[ZSEG];
+  if constexpr (PIPE) {
+#pragma unroll
+    for (int r = 0; r < ZSEG; r++) cur[r] = X[(long)min(r, nz - 1) * plane];
+    for (int k0 = 0; k0 < nz; k0 += ZSEG) {
+#pragma unroll
+      for (int r = 0; r < ZSEG; r++) nxt[r] = X[(long)min(k0 + ZSEG + r, nz - 1) * plane];
+#pragma unroll
+      for (int r = 0; r < ZSEG; r++) {
+        const int k = k0 + r;
+        if (k < nz) {
+          const double a = pp.az[k], c = pp.cz[k];
+          const double b = lam - a - c;
+          const C2<R> x = cur[r];
+          const double inv = 1.0 / (b - a * cp);
+          cp = c * inv;
+          dr = ((double)x.x - a * dr) * inv;
+          di = ((double)x.y - a * di) * inv;
+        }
+      }
+      // state at the end of the segment (levels k0 .. k0+ZSEG-1)
+      ck[0][k0 / ZSEG][t] = cp;
+      ck[1][k0 / ZSEG][t] = dr;
+      ck[2][k0 / ZSEG][t] = di;
+#pragma unroll
+      for (int r = 0; r < ZSEG; r++) cur[r] = nxt[r];
+    }
+  } else {
 #pragma unroll 8
-  for (int k = 0; k < nz; k++) {
-    const double a = pp.az[k], c = pp.cz[k];
-    const double b = lam - a - c;
-    const C2<R> x = X[(long)k * plane];
-    const double inv = 1.0 / (b - a * cp);
-    cp = c * inv;
-    dr = ((double)x.x - a * dr) * inv;
-    di = ((double)x.y - a * di) * inv;
-    if (k % ZSEG == ZSEG - 1) {
-      ck[0][k / ZSEG][t] = cp;
-      ck[1][k / ZSEG][t] = dr;
-      ck[2][k / ZSEG][t] = di;
+    for (int k = 0; k < nz; k++) {
+      const double a = pp.az[k], c = pp.cz[k];
+      const double b = lam - a - c;
+      const C2<R> x = X[(long)k * plane];
+      const double inv = 1.0 / (b - a * cp);
+      cp = c * inv;
+      dr = ((double)x.x - a * dr) * inv;
+      di = ((double)x.y - a * di) * inv;
+      if (k % ZSEG == ZSEG - 1) {
+        ck[0][k / ZSEG][t] = cp;
+        ck[1][k / ZSEG][t] = dr;
+        ck[2][k / ZSEG][t] = di;
+      }
     }
   }
-  // back substitution x_k = d'_k - c'_k x_{k+1}, segment by segment
+  // back substitution x_k = d'_k - c'_k x_{k+1}, segment by segment (PIPE:
+  // the next, lower segment's levels loaded while this one is worked
+  // through)
   double xr = 0, xi = 0;
-  for (int s = (nz - 1) / ZSEG; s >= 0; s--) {
+  const int stop = (nz - 1) / ZSEG;
+  if constexpr (PIPE) {
+#pragma unroll
+    for (int r = 0; r < ZSEG; r++) cur[r] = X[(long)min(stop * ZSEG + r, nz - 1) * plane];
+  }
+  for (int s = stop; s >= 0; s--) {
     const int k0 = s * ZSEG;
     double scp[ZSEG], sdr[ZSEG], sdi[ZSEG];
+    if constexpr (PIPE) {
+#pragma unroll
+      for (int r = 0; r < ZSEG; r++) nxt[r] = X[(long)max(k0 - ZSEG + r, 0) * plane];
+    }
     cp = s ? ck[0][s - 1][t] : 0.0;
     dr = s ? ck[1][s - 1][t] : 0.0;
     di = s ? ck[2][s - 1][t] : 0.0;
@@ -385,7 +430,7 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
       if (k < nz) {
         const double a = pp.az[k], c = pp.cz[k];
         const double b = lam - a - c;
-        const C2<R> x = X[(long)k * plane];
+        const C2<R> x = PIPE ? cur[r] : X[(long)k * plane];
         const double inv = 1.0 / (b - a * cp);
         cp = c * inv;
         dr = ((double)x.x - a * dr) * inv;
@@ -403,6 +448,10 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
         xi = sdi[r] - scp[r] * xi;
         X[(long)k * plane] = mk2((R)xr, (R)xi);
       }
+    }
+    if constexpr (PIPE) {
+#pragma unroll
+      for (int r = 0; r < ZSEG; r++) cur[r] = nxt[r];
     }
   }
 }
@@ -531,7 +580,12 @@ template <typename R>
 void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t st) {
   // nz <= ZSEG * ZSEG_MAX (256) is checked when the plan is set up
   const long modes = (long)pp.Mp * pp.ny;
-  hipLaunchKernelGGL(k_solve_z<R>, dim3((unsigned)((modes + 63) / 64)), dim3(64), 0, st, pp);
+  const unsigned blocks = (unsigned)((modes + 63) / 64);
+  // one wavefront per block: more than 1024 fill several per CU
+  if (pp.zpipe == 1 || (pp.zpipe < 0 && blocks > 1024))
+    hipLaunchKernelGGL((k_solve_z<R, true>), dim3(blocks), dim3(64), 0, st, pp);
+  else
+    hipLaunchKernelGGL((k_solve_z<R, false>), dim3(blocks), dim3(64), 0, st, pp);
 }
 
 template <typename R>

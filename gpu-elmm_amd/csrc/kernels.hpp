@@ -146,6 +146,7 @@ struct PoissonPlan {
   const double *lamx, *lamy;    // eigenvalues of the periodic directions (FP64 in both forms)
   const double *az, *cz;        // z coupling coefficients per k (0-based)
   const double *dzPr, *dzW;     // grid spacings indexed like the reference (k = 1..nz)
+  int zpipe = -1;               // z solve load pipelining: -1 by grid size, 0 off, 1 on
 };
 template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,

@@ -160,13 +160,13 @@ def test_obstacles_at_domain_edges_bit_identical(require_gpu):
 
 
 @pytest.mark.parametrize("env", [{"ELMM_MOMENTUM": "percomp"}, {"ELMM_ZCHUNK": "5"}, {"ELMM_SCALAR": "cellwise"},
-                                 {"ELMM_SGS": "cellwise"}, {"ELMM_GHOST_DELTA": "0"}])
+                                 {"ELMM_SGS": "cellwise"}, {"ELMM_GHOST_DELTA": "0"}, {"ELMM_ZPIPE": "1"}])
 def test_momentum_kernel_variants_bit_identical(require_gpu, monkeypatch, env):
     """The per-component momentum kernels, the one-thread-per-cell scalar and
     SGS kernels, the z-marching kernels with short z chunks (chunk boundaries
-    inside the domain) and the full ghost fill after IBMomentum (instead of
-    the listed ghost cells) give the same bits as the oracle on the obstacle
-    case."""
+    inside the domain), the full ghost fill after IBMomentum (instead of the
+    listed ghost cells) and the z solve with pipelined loads (the variant
+    large grids use) give the same bits as the oracle on the obstacle case."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     cfg = cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1)
