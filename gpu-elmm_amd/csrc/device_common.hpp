@@ -28,6 +28,7 @@ struct Geo {
   float rdxf, rdyf, rdzf;
   int variable_z;
   int sgs_cellwise;   // one thread per cell SGS kernel instead of the z-march
+  int sgs_zchunk;     // levels per block of the SGS z-march (0: default)
   int bt[6];          // velocity boundary types
   int sbt[6];         // scalar boundary types
   double side_scal[6];

@@ -357,12 +357,16 @@ void Engine<T>::setup_geo() {
     fused_momentum = momentum_zm_supported(g) && !(m && std::string(m) == "percomp");
     const char *sg = std::getenv("ELMM_SGS");
     g.sgs_cellwise = sg && std::string(sg) == "cellwise";
+    const char *sz = std::getenv("ELMM_SGS_ZCHUNK");
+    g.sgs_zchunk = sz ? std::max(0, std::atoi(sz)) : 0;
     const char *sk = std::getenv("ELMM_SCALAR");
     zmarch_scalar = !g.variable_z && !(sk && std::string(sk) == "cellwise");
     const char *gv = std::getenv("ELMM_GRAPH");
     use_graphs = !(gv && std::string(gv) == "0");
     const char *zc = std::getenv("ELMM_ZCHUNK");
     if (zc && std::atoi(zc) > 0) zchunk = zchunk_scalar = std::atoi(zc);
+    const char *zs = std::getenv("ELMM_SCALAR_ZCHUNK");
+    if (zs && std::atoi(zs) > 0) zchunk_scalar = std::atoi(zs);
   }
   for (int s = 0; s < 6; s++) {
     g.bt[s] = cfg.btype[s];

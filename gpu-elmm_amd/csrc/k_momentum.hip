@@ -266,25 +266,29 @@ __global__ void __launch_bounds__(TX *TY)
 // thread; 32 x 8 and 64 x 8 threads measured slower), 32 levels per block (16
 // measured equal, 64 12 % slower in FP32: fewer blocks to hide the per-level
 // load latency)
-constexpr int kSgsTX = 64, kSgsTY = 4, kSgsRPT = 2, kSgsZChunk = 32;
+// z chunk of 26 levels: on 512x512x128 its 2,560 blocks are two full rounds
+// of the 1,280 resident FP32 blocks (5 per CU); measured 237 -> 222 us
+// against 32 levels (ELMM_SGS_ZCHUNK overrides)
+constexpr int kSgsTX = 64, kSgsTY = 4, kSgsRPT = 2, kSgsZChunk = 26;
+static int sgs_zchunk(const Geo &g) { return g.sgs_zchunk > 0 ? g.sgs_zchunk : kSgsZChunk; }
 
 static bool sgs_cellwise(const Geo &g, const SgsWidth &w) { return g.sgs_cellwise || g.variable_z || !w.tab; }
 
 int sgs_partials(const Geo &g, const SgsWidth &w) {
   if (sgs_cellwise(g, w)) return (int)(nblk(g.nx, 64) * nblk(g.ny, 4) * g.nz);
-  return (int)(nblk(g.nx, kSgsTX) * nblk(g.ny, kSgsTY * kSgsRPT) * nblk(g.nz, kSgsZChunk));
+  return (int)(nblk(g.nx, kSgsTX) * nblk(g.ny, kSgsTY * kSgsRPT) * nblk(g.nz, sgs_zchunk(g)));
 }
 
 template <typename T, int RPT>
 static void launch_sgs_zm(const Geo &g, const T *U, const T *V, const T *W, T *Visc, double nu, double c_smag,
                           double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, hipStream_t st) {
-  dim3 gr(nblk(g.nx, kSgsTX), nblk(g.ny, kSgsTY * RPT), nblk(g.nz, kSgsZChunk));
+  dim3 gr(nblk(g.nx, kSgsTX), nblk(g.ny, kSgsTY * RPT), nblk(g.nz, sgs_zchunk(g)));
   if (sizeof(T) == 8 && width.rcp())
     hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, RPT, true>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc,
-                       nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
+                       nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, sgs_zchunk(g));
   else
     hipLaunchKernelGGL((k_sgs_zm<T, kSgsTX, kSgsTY, RPT, false>), gr, dim3(kSgsTX * kSgsTY), 0, st, g, U, V, W, Visc,
-                       nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, kSgsZChunk);
+                       nu, c_smag, filter_ratio, width, sgs_model, cfl_partial, sgs_zchunk(g));
 }
 
 template <typename T>

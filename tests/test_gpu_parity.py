@@ -160,7 +160,8 @@ def test_obstacles_at_domain_edges_bit_identical(require_gpu):
 
 
 @pytest.mark.parametrize("env", [{"ELMM_MOMENTUM": "percomp"}, {"ELMM_ZCHUNK": "5"}, {"ELMM_SCALAR": "cellwise"},
-                                 {"ELMM_SGS": "cellwise"}, {"ELMM_GHOST_DELTA": "0"}, {"ELMM_ZPIPE": "1"}])
+                                 {"ELMM_SGS": "cellwise"}, {"ELMM_GHOST_DELTA": "0"}, {"ELMM_ZPIPE": "1"},
+                                 {"ELMM_SGS_ZCHUNK": "5"}])
 def test_momentum_kernel_variants_bit_identical(require_gpu, monkeypatch, env):
     """The per-component momentum kernels, the one-thread-per-cell scalar and
     SGS kernels, the z-marching kernels with short z chunks (chunk boundaries
