@@ -168,3 +168,37 @@ def test_bad_cuboid_is_rejected():
     cfg = cases.uniform_config(16, 16, 8, dx=1.0, dt=0.05, nscalars=1)
     with pytest.raises(RuntimeError, match="cuboid"):
         capi.Case(cfg, cuboids=[(4.0, 2.0, 1.0, 5.0, 0.0, 3.0)])
+
+
+def _case_arrays(case):
+    out = list(case.masks())
+    for l in range(18):
+        out.extend(case.wm_list(l))
+    out.extend(case.wmp_list())
+    for c in range(4):
+        out.extend(case.ib_list(c))
+    for c in range(3):
+        out.append(case.null_list(c))
+    v = case.view
+    n = int(np.prod(case.shape()))
+    for d in range(3):
+        out.append(np.ctypeslib.as_array(v.scfl_mask[d], shape=(n,)).copy())
+        out.append(np.ctypeslib.as_array(v.scfl_ijk[d], shape=(3 * v.scfl_n[d],)).copy() if v.scfl_n[d] else
+                   np.zeros(0, np.int32))
+    return out
+
+
+def test_builder_threads_do_not_change_the_lists():
+    """The builders of the four type arrays, the four IB lists, the Pr and
+    per-component wall-model lists, the nullified points and the three scalar
+    face directions run on their own threads; every list must come out the
+    same, in the same order, on every build of a case with many bodies."""
+    from elmm_amd import serving
+    cfg, cub, src, _ = serving.scenario(4007, 400000, capi.FP32_REFINED)
+    assert len(cub) > 10
+    ref = _case_arrays(capi.Case(cfg, cuboids=cub, sources=src))
+    for _ in range(3):
+        got = _case_arrays(capi.Case(cfg, cuboids=cub, sources=src))
+        assert len(got) == len(ref)
+        for a, b in zip(ref, got):
+            assert a.shape == b.shape and a.tobytes() == b.tobytes()
