@@ -132,15 +132,27 @@ struct TermBuilder {
     flat.assign(4 * items.size(), 0);
     for (auto &it : items) std::memcpy(&flat[4 * (size_t)pos[it.first]++], it.second.data(), sizeof(Term));
   }
+  // Uploads the cells in increasing field offset (consecutive threads of the
+  // kernels that apply the terms then touch neighbouring cells; every cell
+  // is applied independently of the others), each cell's terms in order.
   void upload(DevTerms &d) const {
-    std::vector<int32_t> c(3 * cells.size()), p, t;
-    for (size_t q = 0; q < cells.size(); q++)
-      for (int e = 0; e < 3; e++) c[3 * q + e] = cells[q][e];
+    std::vector<int32_t> p, t;
     csr(p, t);
+    std::vector<int> ord(cells.size());
+    for (size_t q = 0; q < ord.size(); q++) ord[q] = (int)q;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+    std::vector<int32_t> c(3 * cells.size()), p2{0}, t2;
+    t2.reserve(t.size());
+    for (size_t r = 0; r < ord.size(); r++) {
+      const int q = ord[r];
+      for (int e = 0; e < 3; e++) c[3 * r + e] = cells[q][e];
+      t2.insert(t2.end(), t.begin() + 4 * (size_t)p[q], t.begin() + 4 * (size_t)p[q + 1]);
+      p2.push_back((int32_t)(t2.size() / 4));
+    }
     d.n = (int)cells.size();
     upload_vec(d.cell, c);
-    upload_vec(d.ptr, p);
-    upload_vec(d.term, t);
+    upload_vec(d.ptr, p2);
+    upload_vec(d.term, t2);
   }
 };
 
