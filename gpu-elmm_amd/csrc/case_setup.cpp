@@ -126,7 +126,11 @@ std::string validate_config(const elmm_config &c) {
   if (c.sgs_model != 0 && c.sgs_model != 1) return "sgs_model must be 0 or 1";
   if (c.nscalars < 0 || c.nscalars > 8) return "nscalars must be in 0..8";
   if (c.precision != ELMM_FP64 && c.precision != ELMM_FP32_REFINED) return "unknown precision";
-  if (!(c.dt > 0)) return "dt must be positive";
+  if (c.variable_dt) {
+    if (!(c.cfl_target > 0) || !(c.dt_max > 0)) return "variable time steps need cfl_target > 0 and dt_max > 0";
+  } else if (!(c.dt > 0)) {
+    return "dt must be positive";
+  }
   return "";
 }
 
@@ -395,6 +399,30 @@ void view_case(const CaseData &cd, elmm_lists &v) {
 struct elmm_case {
   elmm::CaseData data;
 };
+
+namespace elmm {
+// Device bytes of a case's surface lists once elmm_create has copied them:
+// wall-model points (ijk, list id, dist, z0, ustar, fluxp, fluxm, log term:
+// 64 B), their ordered correction terms (about two 16-byte terms and a
+// 16-byte cell entry per point and pass), scalar WM and face points, the IB
+// lists (target 12 B, pointer 4 B, taps 20 B each) with their mass-source
+// terms and the ghost-delta list, and the nullified points.
+int64_t case_list_bytes(const elmm_case *c) {
+  if (!c) return 0;
+  const CaseData &d = c->data;
+  int64_t b = 0;
+  for (int l = 0; l < 18; l++) b += (int64_t)(d.wm[l].ijk.size() / 3) * (64 + 2 * (32 + 16));
+  b += (int64_t)(d.wmp.ijk.size() / 3) * (12 + 8 + 2 * 32);
+  for (int a = 0; a < 3; a++) b += (int64_t)(d.scfl[a].size() / 3) * 2 * 32;
+  for (int q = 0; q < 4; q++) {
+    const int64_t n = (int64_t)(d.ib[q].ijk.size() / 3), taps = (int64_t)d.ib[q].coef.size();
+    b += n * (12 + 4 + 8) + taps * 20 + (q < 3 ? n * (2 * 32 + 16 * 6) : 0);
+  }
+  for (int a = 0; a < 3; a++) b += (int64_t)d.null_ijk[a].size() * 4;
+  b += (int64_t)d.src_flux.size() * 48;
+  return b;
+}
+}  // namespace elmm
 
 extern "C" int elmm_case_build(const elmm_config *cfg, int32_t n_cuboids, const elmm_cuboid *cuboids,
                                int32_t n_sources, const elmm_point_source *sources, elmm_case **out) {

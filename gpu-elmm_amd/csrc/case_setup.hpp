@@ -58,6 +58,8 @@ void remove_duplicate_wm_points(const elmm_config &c, WMList &L);
 int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc, const elmm_point_source *src,
                CaseData &cd);
 void view_case(const CaseData &cd, elmm_lists &v);
+// Estimated device bytes of a case's surface lists (elmm_create's copies).
+int64_t case_list_bytes(const elmm_case *c);
 // Solid bodies (axis-aligned cuboids): masks, IB lists, solid-body WM points,
 // nullified points.  Defined in solids.cpp.
 int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cuboid *cub, CaseData &cd);

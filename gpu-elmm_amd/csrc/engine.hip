@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <type_traits>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -161,10 +162,21 @@ struct TermBuilder {
 struct EngineIface {
   elmm_config cfg{};
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at creation (owns the stream and buffers)
   bool initialized = false, first_step_done = false;
-  double last_div = 0, last_cfl = 0, last_res = 0, last_stored_div = 0;
+  double last_div = 0, last_cfl = 0, last_res = 0, last_stored_div = 0, last_delta = 0;
   int last_refinements = 0;
   bool want_diag = false;
+  // time stepping (src/main.f90:81-128): the current dt, the simulated time,
+  // steps since creation, steps enqueued since the last synchronisation and
+  // the first step diagnosed as diverging (-1 none, 0 none yet)
+  double dt_cur = 0, time = 0;
+  long long steps_total = 0;
+  int steps_pending = 0;
+  long long diverged_step = -1;
+  std::string diverged_msg;
+  // device record of the CFL monitor and the delta (k_monitor.hip)
+  StepMon *mon = nullptr;
   // kernel timers (HIP events on the stream, read after each sync)
   bool timers_on = false;
   std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
@@ -182,6 +194,33 @@ struct EngineIface {
   virtual void step(int nsteps) = 0;
   virtual void enqueue_steps(int nsteps) = 0;
   virtual double divergence() = 0;
+  // Reads the device step record (synchronises the stream): CFL and delta
+  // of the last step, divergence by CFL_max (src/main.f90:161-168).
+  void read_monitor() {
+    if (!mon) return;
+    StepMon m{};
+    HIPCHK(hipMemcpyAsync(&m, mon, sizeof m, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (cfg.cfl_check && !cfg.variable_dt) last_cfl = m.cfl_last;
+    if (cfg.delta_diag) last_delta = m.delta_last;
+    if (m.first_bad >= 0 && diverged_step < 0) {
+      diverged_step = m.first_bad;
+      diverged_msg = "Solution diverged. 2 (CFL " + std::to_string(m.cfl_last) + " above CFL_max at step " +
+                     std::to_string(m.first_bad) + ")";
+    }
+  }
+  // main.f90's time update after a step of length dt (src/main.f90:116-127)
+  void advance_time(double dt) {
+    steps_total++;
+    if (cfg.variable_dt) {
+      time = time + dt;
+      if (cfg.end_time > 0 && std::fabs(time - cfg.end_time) < 5 * spacing(cfg.end_time)) time = cfg.end_time;
+    } else {
+      time = cfg.start_time + (double)steps_total * dt;
+      if (cfg.end_time > 0 && std::fabs(time - cfg.end_time) < dt / 10) time = cfg.end_time;
+    }
+  }
+  static double spacing(double x) { return std::nextafter(std::fabs(x), INFINITY) - std::fabs(x); }
   void release_graphs() {
     for (auto &e : graph_exec)
       if (e) {
@@ -330,8 +369,15 @@ struct Engine : EngineIface {
   void pressure_correction(T *a, T *b, T *c, bool use_q, double coef, bool diag = false);
   void poisson_solve(int accumulate);
   void subgrid_stresses(bool cfl);
+  // variable time steps: the first stage's SGS kernel (which also writes the
+  // CFL block maxima) is launched before the step to find dt
+  bool sgs_prelaunched = false;
+  void choose_dt();
+  DevBuf mon_buf, delta_partial;
+  int dblocks = 0;
   void stage(int st, bool last);
   void init() override;
+  void run_steps(int nsteps, bool sync_diag);
   void step(int nsteps) override;
   void enqueue_steps(int nsteps) override;
   void first_call();
@@ -528,6 +574,15 @@ void Engine<T>::setup_poisson() {
   partial.alloc((std::max(top_blocks, 1) + 1) * sizeof(double));
   div_partial.alloc(std::max(red_blocks, 1) * sizeof(double));
   cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));
+  mon_buf.alloc(sizeof(StepMon));
+  mon = mon_buf.as<StepMon>();
+  StepMon m0{};
+  m0.first_bad = -1;
+  HIPCHK(hipMemcpy(mon, &m0, sizeof m0, hipMemcpyHostToDevice));
+  if (cfg.delta_diag) {
+    dblocks = delta_blocks(g);
+    delta_partial.alloc((size_t)3 * dblocks * sizeof(double));
+  }
 }
 
 template <typename T>
@@ -857,7 +912,10 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     cfg.zw = zw_copy.data();
   }
   nsc = cfg.nscalars;
+  dt_cur = cfg.variable_dt ? cfg.dt_max : cfg.dt;
+  time = cfg.start_time;
   SetupTrace tr;
+  HIPCHK(hipGetDevice(&device));
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   tr.mark("stream");
   setup_geo();
@@ -959,8 +1017,9 @@ double Engine<T>::reduce_max(DevBuf &buf, int n) {
   std::vector<double> h(n);
   HIPCHK(hipMemcpyAsync(h.data(), buf.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  // a NaN partial makes the maximum +inf (std::max would drop it)
   double m = 0;
-  for (double x : h) m = std::max(m, x);
+  for (double x : h) m = x != x ? INFINITY : std::max(m, x);
   return m;
 }
 
@@ -1037,11 +1096,18 @@ void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, b
 // BoundViscosity(TDiff) is idempotent on the frozen TDiff and done at init).
 template <typename T>
 void Engine<T>::subgrid_stresses(bool cfl) {
-  timed(ELMM_T_SGS, [&] {
-    launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio,
-               sgs_width, cfg.sgs_model,
-               cfl ? cfl_partial.as<double>() : nullptr, cfl_blocks, stream);
-  });
+  if (!sgs_prelaunched)
+    timed(ELMM_T_SGS, [&] {
+      launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio, sgs_width, cfg.sgs_model,
+                 cfl ? cfl_partial.as<double>() : nullptr, cfl_blocks, stream);
+    });
+  sgs_prelaunched = false;
+  // constant dt with the CFL check: the step's CFL number and the divergence
+  // test, on the device (graph-safe)
+  if (cfl && !cfg.variable_dt)
+    timed(ELMM_T_REDUCE, [&] {
+      launch_cfl_monitor(cfl_partial.as<double>(), cfl_blocks, dt_cur, cfg.cfl_max, mon, stream);
+    });
   if (cfg.wall_model && wm_n) timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream); });
   timed(ELMM_T_GHOST, [&] { launch_bound_visc<T>(g, Visc.as<T>(), stream); });
   if (ib_n[3])
@@ -1052,7 +1118,7 @@ void Engine<T>::subgrid_stresses(bool cfl) {
 
 template <typename T>
 void Engine<T>::stage(int st, bool last) {
-  const double dt = cfg.dt;
+  const double dt = dt_cur;
   StageConst sc{};
   sc.stage = st;
   sc.dt = dt;
@@ -1068,7 +1134,7 @@ void Engine<T>::stage(int st, bool last) {
   sc.prgx = cfg.pr_gradient_x;
   sc.prgy = cfg.pr_gradient_y;
 
-  subgrid_stresses(st == 0 && cfg.cfl_check);
+  subgrid_stresses(st == 0 && (cfg.cfl_check || cfg.variable_dt));
   T *ustar[3] = {Us.as<T>(), Vs.as<T>(), Ws.as<T>()};
   T *U2p[3] = {u2, v2, w2};
   if (fused_momentum) {
@@ -1130,6 +1196,10 @@ void Engine<T>::stage(int st, bool last) {
   }
   if (use_q) timed(ELMM_T_IB, [&] { launch_mass_sources(g, q_terms.view(), u2, v2, w2, Q.as<double>(), stream); });
   pressure_correction(u2, v2, w2, use_q, sc.coef, last && want_diag);
+  if (cfg.delta_diag)
+    timed(ELMM_T_REDUCE, [&] {
+      launch_delta<T>(g, u, u2, v, v2, w, w2, delta_partial.as<double>(), dblocks, st, dt, mon, stream);
+    });
   if (last && want_diag) {
     bound_uvw(u2, v2, w2);
     last_stored_div = max_div(u2, v2, w2, use_q);
@@ -1190,7 +1260,8 @@ void Engine<T>::first_call() {
 // memset or copy nodes); timed runs and ELMM_GRAPH=0 enqueue directly.
 template <typename T>
 int Engine<T>::replay_steps(int nsteps) {
-  if (!use_graphs || timers_on || nsteps < 2 || !first_step_done) return 0;
+  // a variable dt changes the stage constants every step: no replay
+  if (!use_graphs || timers_on || nsteps < 2 || !first_step_done || cfg.variable_dt) return 0;
   const int par = u == U.as<T>() ? 0 : 1;
   if (!graph_exec[par]) {
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -1207,38 +1278,78 @@ int Engine<T>::replay_steps(int nsteps) {
   return 2 * n2;
 }
 
+// TimeStepLength with variable time steps (src/dynamics.f90:517-549): the
+// first stage's SGS kernel writes the CFL block maxima of the step-start
+// velocity; their maximum gives dt = min(CFL / max, dt_max) (dt_max for a
+// field at rest, the smallest normal number for a NaN velocity), cut to reach
+// end_time.  Synchronises the stream.
 template <typename T>
-void Engine<T>::step(int nsteps) {
+void Engine<T>::choose_dt() {
+  timed(ELMM_T_SGS, [&] {
+    launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio, sgs_width, cfg.sgs_model,
+               cfl_partial.as<double>(), cfl_blocks, stream);
+  });
+  sgs_prelaunched = true;
+  const double m = reduce_max(cfl_partial, cfl_blocks);  // +inf for a NaN velocity
+  double dt;
+  if (!(m < INFINITY))
+    dt = std::numeric_limits<double>::min();
+  else if (m > 0)
+    dt = std::min(cfg.cfl_target / m, cfg.dt_max);
+  else
+    dt = cfg.dt_max;
+  if (cfg.end_time > 0 && dt + time > cfg.end_time) dt = cfg.end_time - time;
+  dt_cur = dt;
+  last_cfl = m * dt;
+}
+
+// Steps of TMarchRK3 and the main loop's time update (src/main.f90:112-169).
+// Constant dt: every step but the last (which carries the diagnostics) may
+// replay a step graph; the CFL monitor runs on the device and is read once
+// at the end.  Variable dt: one synchronisation per step to find dt; a dt
+// below dt_min from step 3 on ends the call as divergence (main.f90:153-159).
+template <typename T>
+void Engine<T>::run_steps(int nsteps, bool sync_diag) {
   int s = 0;
+  auto one = [&](bool last) {
+    if (cfg.variable_dt) choose_dt();
+    for (int st = 0; st < 3; st++) stage(st, last && st == 2);
+    advance_time(dt_cur);
+    if (cfg.variable_dt && steps_total >= 3 && dt_cur < cfg.dt_min && diverged_step < 0) {
+      diverged_step = steps_total;
+      diverged_msg = "Solution diverged. 1 (dt " + std::to_string(dt_cur) + " below dt_min at step " +
+                     std::to_string(steps_total) + ")";
+    }
+  };
   if (nsteps > 0 && !first_step_done) {
     first_call();
-    for (int st = 0; st < 3; st++) stage(st, nsteps == 1 && st == 2);
+    one(sync_diag && nsteps == 1);
     s = 1;
   }
-  // every step but the last (which carries the diagnostics) may replay
-  if (nsteps - s > 1) s += replay_steps(nsteps - s - 1);
-  for (; s < nsteps; s++)
-    for (int st = 0; st < 3; st++) stage(st, s == nsteps - 1 && st == 2);
-  if (cfg.cfl_check && nsteps > 0) last_cfl = reduce_max(cfl_partial, cfl_blocks) * cfg.dt;
+  if (nsteps - s > 1 && diverged_step < 0) {
+    const int r = replay_steps(sync_diag ? nsteps - s - 1 : nsteps - s);
+    for (int q = 0; q < r; q++) advance_time(dt_cur);
+    s += r;
+  }
+  for (; s < nsteps && !(cfg.variable_dt && diverged_step >= 0); s++) one(sync_diag && s == nsteps - 1);
+  steps_pending += s;
   HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+void Engine<T>::step(int nsteps) {
+  run_steps(nsteps, true);
   HIPCHK(hipStreamSynchronize(stream));
+  read_monitor();
   collect_timers();
 }
 
-// Steps enqueued on the engine's stream without a host synchronisation.
+// Steps enqueued on the engine's stream without a host synchronisation
+// (a variable dt synchronises once per step).
 template <typename T>
 void Engine<T>::enqueue_steps(int nsteps) {
   want_diag = false;
-  int s = 0;
-  if (nsteps > 0 && !first_step_done) {
-    first_call();
-    for (int st = 0; st < 3; st++) stage(st, false);
-    s = 1;
-  }
-  s += replay_steps(nsteps - s);
-  for (; s < nsteps; s++)
-    for (int st = 0; st < 3; st++) stage(st, false);
-  HIPCHK(hipGetLastError());
+  run_steps(nsteps, false);
 }
 
 template <typename T>
@@ -1330,11 +1441,26 @@ extern "C" int elmm_init(elmm_handle *h) {
   return guarded([&] { h->e->init(); });
 }
 
+// The main loop's divergence tests (src/main.f90:153-168): a variable dt
+// below dt_min, or a CFL number above CFL_max (a NaN velocity counts as
+// above), from step 3 on.  The call completes its steps and reports the
+// first step that failed.
 static int check_diverged(elmm_handle *h) {
-  const elmm_config &c = h->e->cfg;
-  if (c.cfl_check && c.cfl_max > 0 && !(h->e->last_cfl <= c.cfl_max))
-    return elmm::fail(ELMM_E_DIVERGED, "Solution diverged. 2 (CFL above CFL_max or NaN)");
+  if (h->e->diverged_step >= 0) return elmm::fail(ELMM_E_DIVERGED, h->e->diverged_msg);
   return 0;
+}
+
+static void fill_diag(elmm_handle *h, elmm_diag *diag) {
+  std::memset(diag, 0, sizeof *diag);
+  diag->cfl = h->e->last_cfl;
+  diag->delta = h->e->last_delta;
+  diag->max_div = h->e->last_div;
+  diag->stored_div = h->e->last_stored_div;
+  diag->poisson_res = h->e->last_res;
+  diag->refinements = h->e->last_refinements;
+  diag->steps_done = h->e->steps_pending;
+  diag->dt = h->e->dt_cur;
+  diag->time = h->e->time;
 }
 
 extern "C" int elmm_step(elmm_handle *h, int32_t nsteps, elmm_diag *diag) {
@@ -1342,17 +1468,11 @@ extern "C" int elmm_step(elmm_handle *h, int32_t nsteps, elmm_diag *diag) {
   if (nsteps < 0) return elmm::fail(ELMM_E_ARG, "nsteps must be >= 0");
   if (!h->e->initialized) return elmm::fail(ELMM_E_STATE, "elmm_step before elmm_init");
   h->e->want_diag = diag != nullptr;
+  h->e->steps_pending = 0;
   int rc = guarded([&] { h->e->step(nsteps); });
   if (rc) return rc;
-  if (diag) {
-    std::memset(diag, 0, sizeof *diag);
-    diag->cfl = h->e->last_cfl;
-    diag->max_div = h->e->last_div;
-    diag->stored_div = h->e->last_stored_div;
-    diag->poisson_res = h->e->last_res;
-    diag->refinements = h->e->last_refinements;
-    diag->steps_done = nsteps;
-  }
+  if (diag) fill_diag(h, diag);
+  h->e->steps_pending = 0;
   return check_diverged(h);
 }
 
@@ -1364,12 +1484,19 @@ extern "C" int elmm_step_async(elmm_handle *h, int32_t nsteps) {
   return guarded([&] { h->e->enqueue_steps(nsteps); });
 }
 
+// Joins the enqueued steps; the diagnostics are those of a call without
+// diag (CFL, delta, dt, time, steps since the last join) and the divergence
+// tests cover every joined step.
 extern "C" int elmm_wait(elmm_handle *h, elmm_diag *diag) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
-  int rc = guarded([&] { HIPCHK(hipStreamSynchronize(h->e->stream)); });
+  int rc = guarded([&] {
+    HIPCHK(hipStreamSynchronize(h->e->stream));
+    h->e->read_monitor();
+  });
   if (rc) return rc;
-  if (diag) std::memset(diag, 0, sizeof *diag);
-  return 0;
+  if (diag) fill_diag(h, diag);
+  h->e->steps_pending = 0;
+  return check_diverged(h);
 }
 
 extern "C" int elmm_query(elmm_handle *h, int32_t *idle) {

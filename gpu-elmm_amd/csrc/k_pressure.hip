@@ -614,7 +614,10 @@ __device__ inline double block_sum(double v) {
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
   return s;
 }
+// NaN inputs count as +inf, so a maximum over a field holding a NaN is not
+// finite (fmax alone would drop the NaN).
 __device__ inline double block_max(double v) {
+  if (v != v) v = INFINITY;
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   __shared__ double red[16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;

@@ -164,6 +164,22 @@ void launch_projection(const Geo &g, T *U, T *V, T *W, T *Pr, const double *Phi,
 template <typename T>
 void launch_max_div(const Geo &g, const T *U, const T *V, const T *W, const double *Q, double *partial, int nblocks,
                     hipStream_t st);
+// Step diagnostics kept in device memory (k_monitor.hip).
+struct StepMon {
+  double cfl_last;        // CFL number of the last step (max * dt)
+  double umax_last;       // TimeStepLength's maximum of the last step
+  double delta;           // delta of the step in progress
+  double delta_last;      // delta of the last completed step
+  long long step;         // steps counted by the CFL monitor
+  long long first_bad;    // first step (1-based) with CFL > CFL_max, -1 none
+};
+void launch_cfl_monitor(const double *partial, int nblocks, double dt, double cfl_max, StepMon *mon,
+                        hipStream_t st);
+int delta_blocks(const Geo &g);
+template <typename T>
+void launch_delta(const Geo &g, const T *U, const T *U2, const T *V, const T *V2, const T *W, const T *W2,
+                  double *partial, int nblocks, int stage, double dt, StepMon *mon, hipStream_t st);
+
 void launch_residual(const Geo &g, const double *Phi, const double *RHS, double *partial, int nblocks,
                      hipStream_t st);
 void launch_refine_residual(const Geo &g, const double *Phi, const double *RHS, const double *az, const double *cz,

@@ -11,6 +11,7 @@
 // No counterpart in the reference, which runs one case per process
 // (src/main.f90); this is the serving form of the step.
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -103,6 +104,8 @@ int retire(elmm_server *s, Request &r) {
   for (size_t q = 0; !rc && q < r.oS.size(); q++)
     if (r.oS[q]) rc = elmm_download(r.h, ELMM_F_SCALAR + (int)q, -2, e, r.oS[q]);
   if (!rc) rc = elmm_divergence(r.h, &r.max_div);
+  // a NaN or infinite divergence is a diverged run even without the CFL test
+  if (!rc && !std::isfinite(r.max_div)) rc = elmm::fail(ELMM_E_DIVERGED, "Solution diverged (non-finite divergence)");
   r.latency = std::chrono::duration<double>(Clock::now() - r.submitted).count();
   s->cell_steps += (int64_t)r.cfg.nx * r.cfg.ny * r.cfg.nz * r.nsteps;
   elmm_destroy(r.h);
@@ -124,18 +127,33 @@ int elmm_footprint(const elmm_config *cfg, int64_t *bytes) {
   const long ntot = sy * (cfg->ny + 6) * (cfg->nz + 6) + 16;
   const int e = cfg->precision == ELMM_FP32_REFINED ? 4 : 8;
   // 12 velocity/pressure/viscosity fields of the field type, 3 per scalar,
-  // Q, Phi and RHS in FP64, the spectral array, masks and codes
+  // Q, Phi and RHS in FP64; per cell the advection codes of U, V, W and the
+  // scalar code, the three scalar flux masks (7 bytes) and Prtype (int32);
+  // the one-byte mark array of the set-up (freed after elmm_create, counted
+  // because it is live at the peak)
   const long M = cfg->nx / 2 + 1;
   const int es = cfg->precision == ELMM_FP32_REFINED && cfg->refine_iters > 0 ? 4 : 8;
   const long Mp = es == 8 ? (M + 7) / 8 * 8 : (M + 15) / 16 * 16;  // 128-byte rows
-  int64_t b = (int64_t)ntot * (12 * e + 3 * e * cfg->nscalars + 3 * 8 + 8);
+  int64_t b = (int64_t)ntot * (12 * e + 3 * e * cfg->nscalars + 3 * 8 + 7 + 4 + 1);
   // spectral array: complex FP32 with refinement sweeps, complex FP64 otherwise
   b += (int64_t)Mp * cfg->ny * cfg->nz * 2 * es;
+  // block partials of the reductions (divergence, CFL, delta: at most four
+  // doubles per 256 cells) and the allocation granularity of the ~60
+  // buffers of a handle (2 MiB each at most)
+  b += (int64_t)ntot / 256 * 4 * 8 + (int64_t)60 * (2 << 20);
   *bytes = b;
   return 0;
 }
 
 
+
+int elmm_case_footprint(const elmm_config *cfg, const elmm_case *c, int64_t *bytes) {
+  if (!cfg || !c || !bytes) return elmm::fail(ELMM_E_ARG, "null argument to elmm_case_footprint");
+  int rc = elmm_footprint(cfg, bytes);
+  if (rc) return rc;
+  *bytes += elmm::case_list_bytes(c);
+  return 0;
+}
 
 int elmm_server_create(int64_t hbm_budget, int32_t max_active, int32_t chunk_steps, elmm_server **out) {
   if (!out || hbm_budget <= 0 || max_active <= 0 || chunk_steps <= 0)
@@ -178,7 +196,7 @@ int elmm_server_submit(elmm_server *s, const elmm_request *q, int64_t *id) {
   r->oPr = q->out_Pr;
   for (int k = 0; k < r->cfg.nscalars; k++) r->oS.push_back(q->out_scalars ? q->out_scalars[k] : nullptr);
   int64_t fb = 0;
-  elmm_footprint(&r->cfg, &fb);
+  elmm_case_footprint(&r->cfg, r->c, &fb);
   r->footprint = (size_t)fb;
   if (r->footprint > s->budget) return elmm::fail(ELMM_E_NOMEM, "request larger than the server's HBM budget");
   r->submitted = Clock::now();

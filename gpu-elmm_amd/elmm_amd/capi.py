@@ -43,7 +43,11 @@ class Config(C.Structure):
         ("cfl_check", C.c_int32),
         ("cfl_max", C.c_double),
         ("precision", C.c_int32), ("ib_mass_sources", C.c_int32),
-        ("refine_iters", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("refine_iters", C.c_int32),
+        ("variable_dt", C.c_int32), ("delta_diag", C.c_int32),
+        ("cfl_target", C.c_double), ("dt_max", C.c_double), ("dt_min", C.c_double),
+        ("start_time", C.c_double), ("end_time", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -69,7 +73,8 @@ class Lists(C.Structure):
 class Diag(C.Structure):
     _fields_ = [("delta", C.c_double), ("cfl", C.c_double), ("max_div", C.c_double),
                 ("poisson_res", C.c_double), ("refinements", C.c_int32), ("steps_done", C.c_int32),
-                ("stored_div", C.c_double), ("reserved", C.c_int32 * 4)]
+                ("stored_div", C.c_double), ("dt", C.c_double), ("time", C.c_double),
+                ("reserved", C.c_int32 * 4)]
 
 
 T_NGROUPS = 16
@@ -113,6 +118,7 @@ EXPORTS = {
     "elmm_copy_probe": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "elmm_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "elmm_footprint": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_int64)]),
+    "elmm_case_footprint": (C.c_int, [C.POINTER(Config), C.c_void_p, C.POINTER(C.c_int64)]),
     "elmm_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "elmm_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
     "elmm_server_progress": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -320,6 +326,12 @@ class Solver:
     def wait(self):
         _check(_lib.elmm_wait(self._h, None))
 
+    def wait_diag(self):
+        """elmm_wait with diagnostics (CFL, delta, dt, time, joined steps)."""
+        d = Diag()
+        _check(_lib.elmm_wait(self._h, C.byref(d)))
+        return d
+
     def timers(self, enable=None, reset=False):
         """Per kernel group: (name, milliseconds, launches)."""
         if enable is not None:
@@ -336,10 +348,14 @@ class Solver:
         return x.value
 
 
-def footprint(cfg):
-    """Device bytes of one case (elmm_footprint)."""
+def footprint(cfg, case=None):
+    """Device bytes of one case: elmm_footprint, or elmm_case_footprint
+    (with the estimate of the case's surface lists) when a Case is given."""
     b = C.c_int64()
-    _check(load().elmm_footprint(C.byref(cfg), C.byref(b)))
+    if case is None:
+        _check(load().elmm_footprint(C.byref(cfg), C.byref(b)))
+    else:
+        _check(load().elmm_case_footprint(C.byref(cfg), case._h, C.byref(b)))
     return b.value
 
 

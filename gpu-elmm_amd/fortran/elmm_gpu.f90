@@ -14,7 +14,8 @@ module elmm_gpu
   private
   public :: elmm_config, elmm_cuboid, elmm_point_source, elmm_diag
   public :: elmm_case_build, elmm_case_free, elmm_create_case, elmm_destroy, elmm_init, elmm_step
-  public :: elmm_upload, elmm_download, elmm_divergence, elmm_put, elmm_get, elmm_check, elmm_error_message
+  public :: elmm_upload, elmm_download, elmm_divergence
+  public :: elmm_put, elmm_get, elmm_check, elmm_error_message
   public :: ELMM_WE, ELMM_EA, ELMM_SO, ELMM_NO, ELMM_BO, ELMM_TO
   public :: ELMM_BC_NOSLIP, ELMM_BC_FREESLIP, ELMM_BC_PERIODIC, ELMM_BC_DIRICHLET, ELMM_BC_NEUMANN
   public :: ELMM_FP64, ELMM_FP32_REFINED, ELMM_POISSON_FFT
@@ -46,7 +47,9 @@ module elmm_gpu
     integer(c_int32_t) :: cfl_check
     real(c_double) :: cfl_max
     integer(c_int32_t) :: precision, ib_mass_sources, refine_iters
-    integer(c_int32_t) :: reserved(7)
+    integer(c_int32_t) :: variable_dt, delta_diag
+    real(c_double) :: cfl_target, dt_max, dt_min, start_time, end_time
+    integer(c_int32_t) :: reserved(4)
   end type
 
   type, bind(C) :: elmm_cuboid
@@ -61,7 +64,7 @@ module elmm_gpu
   type, bind(C) :: elmm_diag
     real(c_double) :: delta, cfl, max_div, poisson_res
     integer(c_int32_t) :: refinements, steps_done
-    real(c_double) :: stored_div
+    real(c_double) :: stored_div, dt, time
     integer(c_int32_t) :: reserved(4)
   end type
 

@@ -105,7 +105,19 @@ typedef struct elmm_config {
   int32_t refine_iters;      /* ELMM_FP32_REFINED: FP32 solves of the FP64
                                 residual after a first FP32 solve; 0 = one
                                 FP64 spectral solve instead (default)        */
-  int32_t reserved[7];
+  int32_t variable_dt;       /* time_stepping%variable_time_steps: dt =
+                                min(cfl_target / max, dt_max) every step
+                                (src/dynamics.f90:519-530)                    */
+  int32_t delta_diag;        /* debuglevel > 0 .or. steady == 1: compute the
+                                step's delta (src/tsteps.f90:188-207)         */
+  double  cfl_target;        /* time_stepping%CFL (variable dt)               */
+  double  dt_max;            /* time_stepping%dt_max (variable dt)            */
+  double  dt_min;            /* time_stepping%dt_min: a variable dt below it
+                                from step 3 on is divergence (main.f90:153-159) */
+  double  start_time;        /* time_stepping%start_time                      */
+  double  end_time;          /* time_stepping%end_time: a variable dt is cut to
+                                reach it (src/dynamics.f90:548-549); 0 = none */
+  int32_t reserved[4];
 } elmm_config;
 
 /*
@@ -160,6 +172,8 @@ typedef struct elmm_diag {
   int32_t steps_done;
   double  stored_div;     /* max |div U - Q| of the stored field (equals
                              max_div in the FP64 form)                      */
+  double  dt;             /* time step of the last step                    */
+  double  time;           /* simulated time after the call (main.f90:116-127) */
   int32_t reserved[4];
 } elmm_diag;
 
@@ -239,10 +253,16 @@ int elmm_destroy(elmm_handle *h);
  * (non-blocking).  No counterpart in the reference (synchronous loop). */
 int elmm_query(elmm_handle *h, int32_t *idle);
 
-/* Device bytes elmm_create allocates for a configuration (fields, spectral
- * work array; surface lists excluded).  Used to admit cases against an HBM
- * budget. */
+/* Device bytes elmm_create allocates for a configuration: fields, masks,
+ * spectral work array, reduction partials, the set-up's transient mark array
+ * and the allocation granularity of its buffers (the surface lists, which
+ * depend on the case, are excluded; the server adds an estimate of them).
+ * Used to admit cases against an HBM budget. */
 int elmm_footprint(const elmm_config *cfg, int64_t *bytes);
+/* elmm_footprint plus an estimate of the case's surface lists on the device:
+ * the bytes the serving form admits a request by.  No counterpart in the
+ * reference. */
+int elmm_case_footprint(const elmm_config *cfg, const elmm_case *c, int64_t *bytes);
 
 /* ---------------------------------------------------------------------------
  * Serving form: continuous batching of independent cases on one GPU.  No

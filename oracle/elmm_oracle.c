@@ -11,6 +11,7 @@
  */
 #include "elmm_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -69,6 +70,9 @@ struct oracle {
   double *src_flux;
   int initialized, first_step_done;
   double last_res, last_div;
+  /* time stepping (src/main.f90:81-169) */
+  double time, dt_last, delta_last, cfl_last;
+  long long steps_total;
 };
 
 #define C1D(a, i) ((a)[(i) + 3])
@@ -194,6 +198,7 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
   }
   oracle *o = xcalloc(1, sizeof(oracle));
   o->c = *cfg;
+  o->time = cfg->start_time;
   o->nx = cfg->nx;
   o->ny = cfg->ny;
   o->nz = cfg->nz;
@@ -1521,6 +1526,7 @@ static void convection(oracle *o, int stage, double dt) {
   const int *N[3] = {o->Un, o->Vn, o->Wn};
   for (int c = 0; c < 3; c++) {
     const int *n = N[c];
+#pragma omp parallel for collapse(2)
     for (int k = 1; k <= n[2]; k++)
       for (int j = 1; j <= n[1]; j++)
         for (int i = 1; i <= n[0]; i++) {
@@ -1562,6 +1568,7 @@ static void convection(oracle *o, int stage, double dt) {
   }
   for (int c = 0; c < 3; c++) {
     const int *n = N[c];
+#pragma omp parallel for collapse(2)
     for (int k = 1; k <= n[2]; k++)
       for (int j = 1; j <= n[1]; j++)
         for (int i = 1; i <= n[0]; i++)
@@ -1575,23 +1582,28 @@ static void other_terms(oracle *o, double coef) {
   double A = -coef, Ax = -coef / o->dxmin, Ay = -coef / o->dymin, Az = -coef / o->dzmin;
   const double *Pr = o->Pr;
   if (o->c.pr_gradient_x != 0)
+#pragma omp parallel for collapse(2)
     for (int k = 1; k <= o->Un[2]; k++)
       for (int j = 1; j <= o->Un[1]; j++)
         for (int i = 1; i <= o->Un[0]; i++)
           A3(o, o->U2, i, j, k) = A3(o, o->U2, i, j, k) + A * o->c.pr_gradient_x;
   if (o->c.pr_gradient_y != 0)
+#pragma omp parallel for collapse(2)
     for (int k = 1; k <= o->Vn[2]; k++)
       for (int j = 1; j <= o->Vn[1]; j++)
         for (int i = 1; i <= o->Vn[0]; i++)
           A3(o, o->V2, i, j, k) = A3(o, o->V2, i, j, k) + A * o->c.pr_gradient_y;
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Un[2]; k++)
     for (int j = 1; j <= o->Un[1]; j++)
       for (int i = 1; i <= o->Un[0]; i++)
         A3(o, o->U2, i, j, k) = A3(o, o->U2, i, j, k) + Ax * (A3(o, Pr, i + 1, j, k) - A3(o, Pr, i, j, k));
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Vn[2]; k++)
     for (int j = 1; j <= o->Vn[1]; j++)
       for (int i = 1; i <= o->Vn[0]; i++)
         A3(o, o->V2, i, j, k) = A3(o, o->V2, i, j, k) + Ay * (A3(o, Pr, i, j + 1, k) - A3(o, Pr, i, j, k));
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Wn[2]; k++)
     for (int j = 1; j <= o->Wn[1]; j++)
       for (int i = 1; i <= o->Wn[0]; i++) {
@@ -1601,6 +1613,7 @@ static void other_terms(oracle *o, double coef) {
         else
           A3(o, o->W2, i, j, k) = A3(o, o->W2, i, j, k) + Az * (A3(o, Pr, i, j, k + 1) - A3(o, Pr, i, j, k));
       }
+#pragma omp parallel for
   for (long p = 0; p < o->ntot; p++) {
     o->U2[p] = o->U2[p] + o->U[p];
     o->V2[p] = o->V2[p] + o->V[p];
@@ -1627,7 +1640,8 @@ static void kappa_scalar(oracle *o, double *Sadv, const double *S) {
   const double Ax = 1 / o->dxmin, Ay = 1 / o->dymin, Az = 1 / o->dzmin;
   for (long p = 0; p < o->ntot; p++) Sadv[p] = 0;
   const double *U = o->U, *V = o->V, *W = o->W;
-  /* x pass (faces i = 0..nx) */
+  /* x pass (faces i = 0..nx); a face writes only its own row */
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 0; i <= nx; i++) {
@@ -1650,6 +1664,8 @@ static void kappa_scalar(oracle *o, double *Sadv, const double *S) {
           A3(o, Sadv, i + 1, j, k) = A3(o, Sadv, i + 1, j, k) + Ax * flux;
         }
       }
+  /* y pass: a face writes only its own k plane */
+#pragma omp parallel for
   for (int k = 1; k <= nz; k++)
     for (int j = 0; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -1672,7 +1688,9 @@ static void kappa_scalar(oracle *o, double *Sadv, const double *S) {
           A3(o, Sadv, i, j + 1, k) = A3(o, Sadv, i, j + 1, k) + Ay * flux;
         }
       }
+  /* z pass: faces of one parity write disjoint level pairs (k, k+1) */
   for (int l = 0; l <= 1; l++)
+#pragma omp parallel for
     for (int k = l; k <= nz; k += 2)
       for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
@@ -1741,11 +1759,13 @@ static void scalar_rk3(oracle *o, int stage, double dt) {
     bound_scalar(o, S);
     if (stage > 0) {
       double a = RK_rho[stage] * dt;
+#pragma omp parallel for
       for (long p = 0; p < o->ntot; p++) S2[p] = Sadv[p] * a;
     } else {
       for (long p = 0; p < o->ntot; p++) S2[p] = 0;
     }
     kappa_scalar(o, Sadv, S);
+#pragma omp parallel for collapse(2)
     for (int k = 0; k <= o->nz + 1; k++)
       for (int j = 0; j <= o->ny + 1; j++)
         for (int i = 0; i <= o->nx + 1; i++)
@@ -1762,7 +1782,9 @@ static void scalar_rk3(oracle *o, int stage, double dt) {
       A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
     }
     double b = RK_beta[stage] * dt;
+#pragma omp parallel for
     for (long p = 0; p < o->ntot; p++) S2[p] = S2[p] + Sadv[p] * b;
+#pragma omp parallel for
     for (long p = 0; p < o->ntot; p++) S[p] = S[p] + S2[p];
     bound_scalar(o, S);
   }
@@ -2050,6 +2072,7 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
     for (int j = 1; j <= o->Wn[1]; j++)
       for (int i = 1; i <= o->Wn[0]; i++)
         if (A3(o, o->Wtype, i, j, 0) > 0) A3(o, W, i, j, 0) = 0;
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -2062,10 +2085,12 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
       }
   long plane = (long)nx * ny;
   double *rhs = xcalloc(plane * nz, sizeof(double)), *phi = xcalloc(plane * nz, sizeof(double));
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) rhs[(i - 1) + (j - 1) * nx + (k - 1) * plane] = A3(o, o->RHS, i, j, k);
   poisson_direct(o, rhs, phi);
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) A3(o, o->Phi, i, j, k) = phi[(i - 1) + (j - 1) * nx + (k - 1) * plane];
@@ -2076,14 +2101,17 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
   /* PostPoisson (2nd order) */
   double Au = dt2 / o->dxmin, Av = dt2 / o->dymin, Aw = dt2 / o->dzmin;
   const double *Phi = o->Phi;
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Un[2]; k++)
     for (int j = 1; j <= o->Un[1]; j++)
       for (int i = 1; i <= o->Un[0]; i++)
         A3(o, U, i, j, k) = A3(o, U, i, j, k) - Au * (A3(o, Phi, i + 1, j, k) - A3(o, Phi, i, j, k));
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Vn[2]; k++)
     for (int j = 1; j <= o->Vn[1]; j++)
       for (int i = 1; i <= o->Vn[0]; i++)
         A3(o, V, i, j, k) = A3(o, V, i, j, k) - Av * (A3(o, Phi, i, j + 1, k) - A3(o, Phi, i, j, k));
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Wn[2]; k++)
     for (int j = 1; j <= o->Wn[1]; j++)
       for (int i = 1; i <= o->Wn[0]; i++) {
@@ -2093,6 +2121,7 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
         else
           A3(o, W, i, j, k) = A3(o, W, i, j, k) - Aw * (A3(o, Phi, i, j, k + 1) - A3(o, Phi, i, j, k));
       }
+#pragma omp parallel for collapse(2)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) A3(o, o->Pr, i, j, k) = A3(o, o->Pr, i, j, k) + A3(o, Phi, i, j, k);
@@ -2108,6 +2137,7 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
   free(top);
   ref = ref / (double)(nx * ny);
   if (ref != 0)
+#pragma omp parallel for collapse(2)
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) A3(o, o->Pr, i, j, k) = A3(o, o->Pr, i, j, k) - ref;
@@ -2122,6 +2152,7 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
   memcpy(Wc, W, o->ntot * sizeof(double));
   bound_uvw(o, Uc, Vc, Wc);
   double S = 0;
+#pragma omp parallel for collapse(2) reduction(max : S)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -2142,6 +2173,8 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
    src/dynamics.f90:511-600 */
 static double cfl_max(oracle *o) {
   double m = 0;
+  int nan = 0;
+#pragma omp parallel for collapse(2) reduction(max : m) reduction(| : nan)
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++) {
@@ -2154,10 +2187,32 @@ static double cfl_max(oracle *o) {
         double p = (a1 > a0 ? a1 : a0) / dxk;
         p = p + (b1 > b0 ? b1 : b0) / dyk;
         p = p + (c1 > c0 ? c1 : c0) / dzk;
-        if (p != p) return INFINITY;
+        if (p != p) nan = 1;
         if (p > m) m = p;
       }
-  return m;
+  return nan ? INFINITY : m;
+}
+
+/* TMarchRK3's delta term of one stage (src/tsteps.f90:188-201): the mean of
+   |U - U2| over each component's interior, added in the order U, V, W.  The
+   reference sums sequentially; the sums here are tree_sum's (the device's
+   order), over the interior in x-fastest order. */
+static double delta_stage(oracle *o, double d) {
+  const double *A[3] = {o->U, o->V, o->W}, *B[3] = {o->U2, o->V2, o->W2};
+  const int *n[3] = {o->Un, o->Vn, o->Wn};
+  for (int c = 0; c < 3; c++) {
+    long N = (long)n[c][0] * n[c][1] * n[c][2];
+    if (N <= 0) continue;
+    double *v = xcalloc(N, sizeof(double));
+    long t = 0;
+    for (int k = 1; k <= n[c][2]; k++)
+      for (int j = 1; j <= n[c][1]; j++)
+        for (int i = 1; i <= n[c][0]; i++) v[t++] = fabs(A3(o, A[c], i, j, k) - A3(o, B[c], i, j, k));
+    double sum = tree_sum(v, N);
+    free(v);
+    d = d + sum / (double)N;
+  }
+  return d;
 }
 
 /* ----------------------------------------------------- public driver --- */
@@ -2193,8 +2248,8 @@ int oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag) {
     set_err("oracle_step before oracle_init");
     return ELMM_E_STATE;
   }
-  double dt = o->c.dt;
-  double cfl = 0;
+  double cfl = o->cfl_last;
+  int done = 0;
   for (int s = 0; s < nsteps; s++) {
     if (!o->first_step_done) {
       /* first call of TMarchRK3, src/tsteps.f90:54-77 */
@@ -2208,14 +2263,25 @@ int oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag) {
       ib_momentum(o, o->U, o->V, o->W);
       o->first_step_done = 1;
     }
-    if (o->c.cfl_check) {
+    /* TimeStepLength (src/dynamics.f90:511-549) */
+    double dt = o->c.dt;
+    int check_cfl = 0;
+    if (o->c.variable_dt) {
+      double m = cfl_max(o); /* +inf for a NaN velocity */
+      if (!(m < INFINITY))
+        dt = DBL_MIN;
+      else if (m > 0)
+        dt = fmin(o->c.cfl_target / m, o->c.dt_max);
+      else
+        dt = o->c.dt_max;
+      if (o->c.end_time > 0 && dt + o->time > o->c.end_time) dt = o->c.end_time - o->time;
+      cfl = m * dt;
+    } else if (o->c.cfl_check) {
       double m = cfl_max(o);
       cfl = m * dt;
-      if (o->c.cfl_max > 0 && !(cfl <= o->c.cfl_max)) {
-        set_err("Solution diverged. 2");
-        return ELMM_E_DIVERGED;
-      }
+      check_cfl = 1;
     }
+    double delta = 0;
     for (int st = 0; st < 3; st++) {
       double coef = 2 * RK_alpha[st] * dt;
       subgrid_stresses(o);
@@ -2228,6 +2294,10 @@ int oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag) {
       int use_q = o->c.ib_mass_sources != 0;
       if (use_q) ib_mass_sources(o, o->U2, o->V2, o->W2);
       pressure_correction(o, o->U2, o->V2, o->W2, use_q, coef);
+      if (o->c.delta_diag) {
+        delta = delta_stage(o, delta);
+        if (st == 2) delta = delta / dt;
+      }
       swap(&o->U, &o->U2);
       swap(&o->V, &o->V2);
       swap(&o->W, &o->W2);
@@ -2236,15 +2306,41 @@ int oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag) {
       ib_momentum(o, o->U, o->V, o->W);
       bound_uvw(o, o->U, o->V, o->W);
     }
+    /* the main loop's time update and divergence tests (src/main.f90:116-169) */
+    o->steps_total++;
+    done++;
+    o->dt_last = dt;
+    o->cfl_last = cfl;
+    if (o->c.delta_diag) o->delta_last = delta;
+    if (o->c.variable_dt) {
+      o->time = o->time + dt;
+      if (o->c.end_time > 0 && fabs(o->time - o->c.end_time) < 5 * (nextafter(fabs(o->c.end_time), INFINITY) -
+                                                                    fabs(o->c.end_time)))
+        o->time = o->c.end_time;
+    } else {
+      o->time = o->c.start_time + (double)o->steps_total * dt;
+      if (o->c.end_time > 0 && fabs(o->time - o->c.end_time) < dt / 10) o->time = o->c.end_time;
+    }
+    if (o->steps_total >= 3 && o->c.variable_dt && dt < o->c.dt_min) {
+      set_err("Solution diverged. 1");
+      break;
+    }
+    if (o->steps_total >= 3 && check_cfl && o->c.cfl_max > 0 && !(cfl <= o->c.cfl_max)) {
+      set_err("Solution diverged. 2");
+      break;
+    }
   }
   if (diag) {
     memset(diag, 0, sizeof *diag);
     diag->cfl = cfl;
     diag->poisson_res = o->last_res;
     diag->max_div = o->last_div;
-    diag->steps_done = nsteps;
+    diag->steps_done = done;
+    diag->delta = o->delta_last;
+    diag->dt = o->dt_last;
+    diag->time = o->time;
   }
-  return 0;
+  return done < nsteps ? ELMM_E_DIVERGED : 0;
 }
 
 /* max |div U - Q| as in PostPoisson check_divergence (2nd order),
@@ -2302,5 +2398,34 @@ int32_t oracle_wm_list(oracle *o, int32_t list, double *fluxp, double *fluxm) {
 }
 void oracle_op_kappa(oracle *o, int32_t sc) { kappa_scalar(o, o->Sadv[sc], o->S[sc]); }
 void oracle_op_sgs(oracle *o) { subgrid_stresses(o); }
+void oracle_op_ib_momentum(oracle *o) { ib_momentum(o, o->U, o->V, o->W); }
+void oracle_op_mass_sources(oracle *o) { ib_mass_sources(o, o->U, o->V, o->W); }
+void oracle_op_null(oracle *o) { null_interior(o); }
+void oracle_op_ib_visc(oracle *o) {
+  for (int p = 0; p < o->ib_n[3]; p++) {
+    const int32_t *q = &o->ib_ijk[3][3 * p];
+    A3(o, o->Visc, q[0], q[1], q[2]) = ib_interp(o, 3, p, o->Visc);
+  }
+}
+int32_t oracle_op_wm_div(oracle *o, double *out) {
+  wm_divergence(o);
+  if (out) memcpy(out, o->wmp_div, (size_t)o->wmp.n * sizeof(double));
+  return o->wmp.n;
+}
+void oracle_op_scalar_extra(oracle *o, int32_t sc) {
+  double *S = o->S[sc], *Sadv = o->Sadv[sc];
+  for (int p = 0; p < o->src_n; p++) {
+    if (o->src_sc[p] != sc) continue;
+    const int32_t *q = &o->src_ijk[3 * p];
+    A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->src_flux[p];
+  }
+  for (int p = 0; p < o->wmp.n; p++) {
+    const int32_t *q = &o->wmp.ijk[3 * p];
+    A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
+  }
+}
+void oracle_op_bound_pr(oracle *o) { bound_pr(o, o->Pr); }
+void oracle_op_bound_scalar(oracle *o, int32_t sc) { bound_scalar(o, o->S[sc]); }
+void oracle_op_bound_visc(oracle *o) { bound_visc(o, o->Visc); }
 void oracle_op_poisson(oracle *o, const double *rhs, double *phi) { poisson_direct(o, rhs, phi); }
 double oracle_op_residual(oracle *o, const double *phi, const double *rhs) { return residue(o, phi, rhs); }

@@ -62,6 +62,15 @@ double oracle_op_residual(oracle *o, const double *phi_ghosted, const double *rh
    law (positive normal x) */
 double oracle_det_log(double x);
 void oracle_op_wm_fluxes(oracle *o);            /* ComputeUVWFluxesWM           */
+void oracle_op_ib_momentum(oracle *o);          /* IBMomentum(U,V,W)            */
+void oracle_op_mass_sources(oracle *o);         /* IBMassSources + Bound_Q -> Q */
+void oracle_op_null(oracle *o);                 /* NullInterior(U,V,W)          */
+void oracle_op_ib_visc(oracle *o);              /* serial ScalFl gather on Visc */
+int32_t oracle_op_wm_div(oracle *o, double *out); /* DivergenceWM -> out (n)    */
+void oracle_op_scalar_extra(oracle *o, int32_t sc); /* ScalarExtra on Sadv(sc)  */
+void oracle_op_bound_pr(oracle *o);             /* Bound_Pr(Pr)                 */
+void oracle_op_bound_scalar(oracle *o, int32_t sc); /* BoundScalar(S(sc))       */
+void oracle_op_bound_visc(oracle *o);           /* BoundViscosity(Visc)         */
 /* fluxp/fluxm of momentum wall-model list `list` (comp*6+dir); returns the
    number of points (buffers may be NULL to query it) */
 int32_t oracle_wm_list(oracle *o, int32_t list, double *fluxp, double *fluxm);

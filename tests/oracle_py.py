@@ -45,6 +45,13 @@ def _load():
         lib.oracle_op_wm_fluxes.argtypes = [C.c_void_p]
         lib.oracle_wm_list.argtypes = [C.c_void_p, C.c_int32, pd, pd]
         lib.oracle_wm_list.restype = C.c_int32
+        for name in ("oracle_op_ib_momentum", "oracle_op_mass_sources", "oracle_op_null", "oracle_op_ib_visc",
+                     "oracle_op_bound_pr", "oracle_op_bound_visc"):
+            getattr(lib, name).argtypes = [C.c_void_p]
+        lib.oracle_op_wm_div.argtypes = [C.c_void_p, pd]
+        lib.oracle_op_wm_div.restype = C.c_int32
+        lib.oracle_op_scalar_extra.argtypes = [C.c_void_p, C.c_int32]
+        lib.oracle_op_bound_scalar.argtypes = [C.c_void_p, C.c_int32]
         _lib = lib
     return _lib
 
@@ -128,6 +135,36 @@ class Oracle:
         fp, fm = np.zeros(n), np.zeros(n)
         _lib.oracle_wm_list(self._h, comp * 6 + direction, fp.ctypes.data_as(pd), fm.ctypes.data_as(pd))
         return fp, fm
+
+    def op_ib_momentum(self):
+        _lib.oracle_op_ib_momentum(self._h)
+
+    def op_mass_sources(self):
+        _lib.oracle_op_mass_sources(self._h)
+
+    def op_null(self):
+        _lib.oracle_op_null(self._h)
+
+    def op_ib_visc(self):
+        _lib.oracle_op_ib_visc(self._h)
+
+    def op_wm_div(self):
+        n = _lib.oracle_op_wm_div(self._h, None)
+        out = np.zeros(n)
+        _lib.oracle_op_wm_div(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def op_scalar_extra(self, sc=0):
+        _lib.oracle_op_scalar_extra(self._h, sc)
+
+    def op_bound_pr(self):
+        _lib.oracle_op_bound_pr(self._h)
+
+    def op_bound_scalar(self, sc=0):
+        _lib.oracle_op_bound_scalar(self._h, sc)
+
+    def op_bound_visc(self):
+        _lib.oracle_op_bound_visc(self._h)
 
     def op_kappa(self, sc=0):
         _lib.oracle_op_kappa(self._h, sc)

@@ -66,3 +66,32 @@ def test_mix_equals_alone_and_budget_is_respected(require_gpu):
         for got, want in zip([U, V, W, Pr, S[0]], ref):
             assert np.array_equal(got, want)
     srv.close()
+
+
+def _hip_free_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32_REFINED])
+def test_case_footprint_covers_device_allocation(require_gpu, precision):
+    """The admission footprint is at least what elmm_create takes from the
+    device (hipMemGetInfo before and after) on an obstacle case."""
+    cfg = capi.with_precision(cases.uniform_config(128, 128, 64, dx=1.0, dt=0.05, nscalars=1), precision)
+    cub = [(float(x), float(x + 12), float(y), float(y + 12), 0.0, 20.0) for x in range(8, 120, 32)
+           for y in range(8, 120, 32)]
+    case = capi.Case(cfg, cuboids=cub, sources=[(3.5, 3.5, 0.5, 1.0, 0)])
+    fp = capi.footprint(cfg, case)
+    s0 = capi.Solver(capi.with_precision(cases.uniform_config(32, 32, 16), precision), capi.Case(
+        cases.uniform_config(32, 32, 16)))   # runtime and code objects loaded before measuring
+    before = _hip_free_bytes()
+    s = capi.Solver(cfg, case)
+    after = _hip_free_bytes()
+    drop = before - after
+    print(f"precision {precision}: footprint {fp} B, device memory taken by elmm_create {drop} B", flush=True)
+    assert fp >= drop > 0
+    s.close()
+    s0.close()

@@ -1,0 +1,500 @@
+"""Independent checks of the oracle's list-driven and ghost-cell operators
+(CPU only).  Each operator of the oracle is run on seeded fields of a case
+with solid bodies and compared with a numpy restatement written from the
+reference's definition, or with a closed form; where the definition fixes an
+order of operations (two-phase gather, serial gather, odd/even z faces) the
+test also shows that the other order gives different bits, so the check is
+sensitive to it.
+
+  IBMomentum            src/tsteps.f90:368-418, immersedboundary.f90:637-651
+  IBMassSources, Bound_Q src/tsteps.f90:490-531, boundaries.f90:2094-2192
+  NullInterior          src/tsteps.f90:543-567
+  ScalFl viscosity      src/dynamics.f90:899-902
+  DivergenceWM          src/wallmodels.f90:1153-1166
+  ScalarExtra           src/scalars.f90:364-379
+  KappaScalar z order   src/scalar_advection.f90:233-255
+  CD4 fallback Xtype<0  src/advection.f90:386-390, 430-434, 474-478 (U), 555-643 (V),
+                        729-817 (W)
+  Bound_Pr              src/boundaries.f90:1951-2090
+  BoundScalar           src/scalarboundaries.f90:18-279
+  BoundViscosity        src/scalarboundaries.f90:326-431
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from elmm_amd import capi, cases
+from oracle_py import F_SADV, F_USTAR, F_VSTAR, F_WSTAR, Oracle
+
+OBSTACLES = [(8.0, 16.0, 8.0, 16.0, 0.0, 6.0), (20.0, 26.0, 18.0, 30.0, 0.0, 10.0, 0.01),
+             (0.0, 3.0, 24.0, 28.0, 0.0, 4.0)]
+
+
+def G(a, i, j, k):
+    """Element (i, j, k) of a ghosted (lower bound -2) array."""
+    return a[i + 2, j + 2, k + 2]
+
+
+def rand_field(shape, seed, lo=-1.0, hi=1.0):
+    g = np.random.Generator(np.random.Philox(key=[seed, 99]))
+    return np.asfortranarray(g.uniform(lo, hi, shape))
+
+
+@pytest.fixture(scope="module")
+def obstacle_case():
+    cfg = cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1)
+    case = capi.Case(cfg, cuboids=OBSTACLES, sources=[(4.5, 4.5, 0.5, 1.0, 0)])
+    return cfg, case
+
+
+def fresh(cfg, case, seed=11):
+    o = Oracle(cfg, case)
+    shape = o.shape()
+    for f in (capi.F_U, capi.F_V, capi.F_W, capi.F_PR, capi.F_VISC, capi.F_SCALAR):
+        o.upload(f, rand_field(shape, seed + f))
+    return o
+
+
+def two_phase(F, tgt, ptr, src, coef):
+    """IBMomentum's semantics: every interpolation from the unmodified
+    field, then every assignment."""
+    vals = []
+    for p in range(len(tgt)):
+        s = 0.0
+        for t in range(ptr[p], ptr[p + 1]):
+            s = s + coef[t] * G(F, *src[t])
+        vals.append(s)
+    out = F.copy()
+    for p in range(len(tgt)):
+        out[tgt[p][0] + 2, tgt[p][1] + 2, tgt[p][2] + 2] = vals[p]
+    return out
+
+
+def one_phase(F, tgt, ptr, src, coef):
+    """In-place assignment in list order (what the two phases avoid)."""
+    out = F.copy()
+    for p in range(len(tgt)):
+        s = 0.0
+        for t in range(ptr[p], ptr[p + 1]):
+            s = s + coef[t] * G(out, *src[t])
+        out[tgt[p][0] + 2, tgt[p][1] + 2, tgt[p][2] + 2] = s
+    return out
+
+
+class ListsCase:
+    """A copy of a case's list view with the IB list of one component
+    replaced (the arrays are kept alive here)."""
+
+    def __init__(self, case, comp, tgt, ptr, src, coef):
+        self.view = capi.Lists()
+        C.memmove(C.byref(self.view), C.byref(case.view), C.sizeof(capi.Lists))
+        self._keep = [np.ascontiguousarray(tgt, dtype=np.int32), np.ascontiguousarray(ptr, dtype=np.int32),
+                      np.ascontiguousarray(src, dtype=np.int32), np.ascontiguousarray(coef, dtype=np.float64)]
+        t, pt, sr, cf = self._keep
+        self.view.ib_n[comp] = len(t)
+        self.view.ib_ijk[comp] = t.ctypes.data_as(C.POINTER(C.c_int32))
+        self.view.ib_ptr[comp] = pt.ctypes.data_as(C.POINTER(C.c_int32))
+        self.view.ib_src_ijk[comp] = sr.ctypes.data_as(C.POINTER(C.c_int32))
+        self.view.ib_coef[comp] = cf.ctypes.data_as(C.POINTER(C.c_double))
+        self._case = case
+
+
+def test_ib_momentum_two_phase(obstacle_case):
+    cfg, case = obstacle_case
+    o = fresh(cfg, case)
+    before = [o.download(c) for c in range(3)]
+    o.op_ib_momentum()
+    for c in range(3):
+        tgt, ptr, src, coef = case.ib_list(c)
+        assert len(tgt) > 0
+        assert np.array_equal(o.download(c), two_phase(before[c], tgt, ptr, src, coef)), c
+
+
+def test_ib_momentum_target_that_is_also_a_source():
+    """A chain of points whose targets are sources of other points (as at
+    thin or touching solids): the oracle reads every source before it writes
+    any target; in-place list order would give other values."""
+    cfg = cases.uniform_config(16, 16, 8, dx=1.0, dt=0.05, nscalars=0)
+    case = capi.Case(cfg)
+    tgt = np.array([[4, 4, 2], [5, 4, 2], [6, 4, 2]])
+    src = np.array([[5, 4, 2], [4, 5, 2], [6, 4, 2], [4, 4, 3], [4, 4, 2], [7, 4, 2]])
+    ptr = np.array([0, 2, 4, 6])
+    coef = np.array([0.7, 0.3, 0.6, 0.4, 1.5, -0.5])
+    lc = ListsCase(case, 0, tgt, ptr, src, coef)
+    o = Oracle(cfg, lc)
+    U = rand_field(o.shape(), 5)
+    o.upload(capi.F_U, U)
+    o.op_ib_momentum()
+    got = o.download(capi.F_U)
+    assert np.array_equal(got, two_phase(U, tgt, ptr, src, coef))
+    assert not np.array_equal(got, one_phase(U, tgt, ptr, src, coef))
+
+
+def test_mass_sources_scatter_and_fold(obstacle_case):
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=21)
+    F = [o.download(c) for c in range(3)]
+    o.op_mass_sources()
+    got = o.download(capi.F_Q)
+    Q = np.zeros_like(got)
+    dl = (cfg.dx, cfg.dy, cfg.dz)
+    for c in range(3):
+        tgt, _, _, _ = case.ib_list(c)
+        for q in tgt:
+            val = G(F[c], *q)
+            Q[q[0] + 2, q[1] + 2, q[2] + 2] = Q[q[0] + 2, q[1] + 2, q[2] + 2] + val / dl[c]
+            r = list(q)
+            r[c] += 1
+            Q[r[0] + 2, r[1] + 2, r[2] + 2] = Q[r[0] + 2, r[1] + 2, r[2] + 2] - val / dl[c]
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    # Bound_Q, periodic x and y, walls in z: ghost layers folded into the
+    # interior in the order x-, x+, y-, y+, z-, z+
+    Q[nx + 2, 3:ny + 3, 3:nz + 3] = Q[2, 3:ny + 3, 3:nz + 3] + Q[nx + 2, 3:ny + 3, 3:nz + 3]
+    Q[3, 3:ny + 3, 3:nz + 3] = Q[3, 3:ny + 3, 3:nz + 3] + Q[nx + 3, 3:ny + 3, 3:nz + 3]
+    Q[3:nx + 3, ny + 2, 3:nz + 3] = Q[3:nx + 3, ny + 2, 3:nz + 3] + Q[3:nx + 3, 2, 3:nz + 3]
+    Q[3:nx + 3, 3, 3:nz + 3] = Q[3:nx + 3, 3, 3:nz + 3] + Q[3:nx + 3, ny + 3, 3:nz + 3]
+    Q[3:nx + 3, 3:ny + 3, 3] = Q[3:nx + 3, 3:ny + 3, 3] + Q[3:nx + 3, 3:ny + 3, 2]
+    Q[3:nx + 3, 3:ny + 3, nz + 2] = Q[3:nx + 3, 3:ny + 3, nz + 2] + Q[3:nx + 3, 3:ny + 3, nz + 3]
+    interior = np.s_[3:nx + 3, 3:ny + 3, 3:nz + 3]
+    assert np.array_equal(got[interior], Q[interior])
+    # every listed point adds a flux and removes it one cell further on: the
+    # folded interior sums to zero (dx = dy = dz), the ghost layers aside
+    # (non-periodic z faces of the top layer are outside the fold)
+    tot = got[interior].sum()
+    assert abs(tot) <= 1e-12 * np.abs(got[interior]).sum(), tot
+    assert np.abs(got[interior]).max() > 0
+
+
+def test_null_interior_zeroes_only_listed_points(obstacle_case):
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=31)
+    before = [o.download(c) for c in range(3)]
+    o.op_null()
+    for c in range(3):
+        pts = case.null_list(c)
+        assert len(pts) > 0
+        want = before[c].copy()
+        for q in pts:
+            want[q[0] + 2, q[1] + 2, q[2] + 2] = 0.0
+        assert np.array_equal(o.download(c), want)
+
+
+def test_scalfl_viscosity_gather_is_serial(obstacle_case):
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=41)
+    nu = rand_field(o.shape(), 5, 1.0, 2.0)
+    o.upload(capi.F_VISC, nu)
+    o.op_ib_visc()
+    tgt, ptr, src, coef = case.ib_list(3)
+    assert len(tgt) > 0
+    assert np.array_equal(o.download(capi.F_VISC), one_phase(nu, tgt, ptr, src, coef))
+
+
+# A ScalFl chain: point 1 reads point 0's target, point 2 reads point 1's.
+SCALFL_CHAIN = (np.array([[4, 4, 2], [5, 4, 2], [6, 4, 2]]),
+                np.array([0, 2, 4, 6]),
+                np.array([[4, 5, 2], [3, 4, 2], [4, 4, 2], [5, 5, 2], [5, 4, 2], [6, 5, 2]]),
+                np.array([0.5, 0.5, 0.75, 0.25, 0.5, 0.5]))
+
+
+def test_scalfl_viscosity_gather_chain_reads_earlier_targets():
+    """The ScalFl loop runs in list order on the live array
+    (dynamics.f90:899-902): a later point reads the value an earlier point
+    just wrote; a gather-then-scatter would not."""
+    cfg = cases.uniform_config(16, 16, 8, dx=1.0, dt=0.05, nscalars=1)
+    case = capi.Case(cfg)
+    lc = ListsCase(case, 3, *SCALFL_CHAIN)
+    o = Oracle(cfg, lc)
+    nu = rand_field(o.shape(), 6, 1.0, 2.0)
+    o.upload(capi.F_VISC, nu)
+    o.op_ib_visc()
+    got = o.download(capi.F_VISC)
+    assert np.array_equal(got, one_phase(nu, *SCALFL_CHAIN))
+    assert not np.array_equal(got, two_phase(nu, *SCALFL_CHAIN))
+
+
+def wm_div_numpy(cfg, case, U, V, W):
+    prtype = case.masks()[0]
+    ijk, _, _ = case.wmp_list()
+    out = np.zeros(len(ijk))
+    for p, (i, j, k) in enumerate(ijk):
+        d = 0.0
+        if G(prtype, i + 1, j, k) <= 0:
+            d = d + G(U, i, j, k) / cfg.dx
+        if G(prtype, i - 1, j, k) <= 0:
+            d = d - G(U, i - 1, j, k) / cfg.dx
+        if G(prtype, i, j + 1, k) <= 0:
+            d = d + G(V, i, j, k) / cfg.dy
+        if G(prtype, i, j - 1, k) <= 0:
+            d = d - G(V, i, j - 1, k) / cfg.dy
+        if G(prtype, i, j, k + 1) <= 0:
+            d = d + G(W, i, j, k) / cfg.dz
+        if G(prtype, i, j, k - 1) <= 0:
+            d = d - G(W, i, j, k - 1) / cfg.dz
+        out[p] = d
+    return out
+
+
+def test_divergence_wm_and_scalar_extra(obstacle_case):
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=51)
+    U, V, W = (o.download(c) for c in range(3))
+    div = o.op_wm_div()
+    assert len(div) > 0
+    assert np.array_equal(div, wm_div_numpy(cfg, case, U, V, W))
+    # a point with six fluid neighbours gets the full discrete divergence
+    prtype = case.masks()[0]
+    ijk, _, _ = case.wmp_list()
+    full = 0
+    for p, (i, j, k) in enumerate(ijk):
+        if all(G(prtype, *q) <= 0 for q in ((i + 1, j, k), (i - 1, j, k), (i, j + 1, k), (i, j - 1, k),
+                                            (i, j, k + 1), (i, j, k - 1))):
+            d = (G(U, i, j, k) - G(U, i - 1, j, k)) + (G(V, i, j, k) - G(V, i, j - 1, k)) + \
+                (G(W, i, j, k) - G(W, i, j, k - 1))
+            assert abs(div[p] - d) <= 1e-13
+            full += 1
+    assert full > 0
+    # ScalarExtra: the point-source flux, then div * S at every WM point (in
+    # list order, several points may share a cell)
+    sadv = rand_field(o.shape(), 61)
+    o.upload(F_SADV, sadv)
+    S = o.download(capi.F_SCALAR)
+    o.op_scalar_extra(0)
+    want = sadv.copy()
+    v = case.view
+    for p in range(v.src_n):
+        q = [v.src_ijk[3 * p + e] for e in range(3)]
+        want[q[0] + 2, q[1] + 2, q[2] + 2] = want[q[0] + 2, q[1] + 2, q[2] + 2] + v.src_flux[p]
+    for p, (i, j, k) in enumerate(ijk):
+        want[i + 2, j + 2, k + 2] = want[i + 2, j + 2, k + 2] + div[p] * G(S, i, j, k)
+    assert np.array_equal(o.download(F_SADV), want)
+
+
+def test_kappa_z_faces_odd_even_order():
+    """The z pass adds the face fluxes with the even faces first, then the
+    odd ones (scalar_advection.f90:233-255).  The x and y passes are
+    replayed in numpy, then the z faces in that order: equal bits; in plain
+    face order the bits differ on this grid."""
+    cfg = cases.uniform_config(8, 8, 8, dx=1.0, dt=0.05, nscalars=1, top=capi.FREESLIP)
+    case = capi.Case(cfg)
+    o = Oracle(cfg, case)
+    shape = o.shape()
+    for f in (capi.F_U, capi.F_V, capi.F_W):
+        o.upload(f, rand_field(shape, 70 + f, -2.0, 2.0))
+    S = rand_field(shape, 77, 0.0, 3.0)
+    o.upload(capi.F_SCALAR, S)
+    o.op_kappa(0)
+    got = o.download(F_SADV)
+    U, V, W = (o.download(c) for c in range(3))
+    mx, my, mz = (np.ctypeslib.as_array(case.view.scfl_mask[d], shape=(int(np.prod(shape)),)).reshape(
+        shape, order="F") for d in range(3))
+    eps = float(np.float32(1e-8))
+
+    def lim(r):
+        a = (1 + 2 * r) / 3
+        b = min(2.0, a)
+        c = min(2 * r, b)
+        return max(0.0, c)
+
+    def sg(x):
+        return np.copysign(1.0, x)
+
+    nx, ny, nz = 8, 8, 8
+
+    def flux(vel, s_m1, s0, s1, s2):
+        if vel > 0:
+            sr, sl = s1 - s0, s0 - s_m1
+        else:
+            sr, sl = s0 - s1, s1 - s2
+        slope = lim((sr + eps * sg(sl)) / (sl + eps * sg(sl)))
+        if vel > 0:
+            return vel * (s0 + (s0 - s_m1) * slope / 2.0)
+        return vel * (s1 + (s1 - s2) * slope / 2.0)
+
+    def xy(A):
+        for k in range(1, nz + 1):
+            for j in range(1, ny + 1):
+                for i in range(0, nx + 1):
+                    if G(mx, i, j, k):
+                        f = flux(G(U, i, j, k), G(S, i - 1, j, k), G(S, i, j, k), G(S, i + 1, j, k), G(S, i + 2, j, k))
+                        A[i + 2, j + 2, k + 2] -= f
+                        A[i + 3, j + 2, k + 2] += f
+        for k in range(1, nz + 1):
+            for j in range(0, ny + 1):
+                for i in range(1, nx + 1):
+                    if G(my, i, j, k):
+                        f = flux(G(V, i, j, k), G(S, i, j - 1, k), G(S, i, j, k), G(S, i, j + 1, k), G(S, i, j + 2, k))
+                        A[i + 2, j + 2, k + 2] -= f
+                        A[i + 2, j + 3, k + 2] += f
+
+    def zface(A, k):
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 1):
+                if G(mz, i, j, k):
+                    f = flux(G(W, i, j, k), G(S, i, j, k - 1), G(S, i, j, k), G(S, i, j, k + 1), G(S, i, j, k + 2))
+                    A[i + 2, j + 2, k + 2] -= f
+                    A[i + 2, j + 2, k + 3] += f
+
+    eo = np.zeros(shape, order="F")
+    xy(eo)
+    for l in (0, 1):
+        for k in range(l, nz + 1, 2):
+            zface(eo, k)
+    seq = np.zeros(shape, order="F")
+    xy(seq)
+    for k in range(0, nz + 1):
+        zface(seq, k)
+    assert np.array_equal(got, eo)
+    assert not np.array_equal(got, seq)
+
+
+def test_cd4_fallback_at_negative_utype_is_second_order_divergence_form():
+    """Where Xtype < 0 CD4div evaluates the second-order divergence form
+    64 ((u_{i+1}+u_i)^2 - (u_i+u_{i-1})^2) / dx (self) and
+    64 ((u_{j+1}+u_j)(v_{i+1}+v_i) - (u_j+u_{j-1})(v_{i+1,j-1}+v_{i,j-1})) / dy
+    (cross), times -1/256: the textbook staggered CD2 advection
+    -d(uu)/dx - d(uv)/dy - d(uw)/dz with midpoint averages.  Every interior
+    cell is marked Xtype < 0 on a periodic 8^3 grid."""
+    n = 8
+    cfg = cases.uniform_config(n, n, n, dx=1.0, dt=0.05, nscalars=0, top=capi.FREESLIP,
+                               btype=[capi.PERIODIC] * 4 + [capi.FREESLIP] * 2, wall_model=0)
+    view = capi.Lists()
+    shape = (n + 6, n + 6, n + 6)
+    types = [np.full(shape, -1, dtype=np.int32, order="F") for _ in range(3)]
+    prt = np.zeros(shape, dtype=np.int32, order="F")
+    keep = [prt] + types
+    view.prtype = prt.ctypes.data_as(C.POINTER(C.c_int32))
+    view.utype, view.vtype, view.wtype = (t.ctypes.data_as(C.POINTER(C.c_int32)) for t in types)
+    masks = [np.ones(shape, dtype=np.uint8, order="F") for _ in range(3)]
+    keep += masks
+    for d in range(3):
+        view.scfl_mask[d] = masks[d].ctypes.data_as(C.POINTER(C.c_uint8))
+    ptr0 = np.zeros(1, dtype=np.int32)
+    keep.append(ptr0)
+    for c in range(4):
+        view.ib_ptr[c] = ptr0.ctypes.data_as(C.POINTER(C.c_int32))
+
+    class FakeCase:
+        pass
+
+    fc = FakeCase()
+    fc.view = view
+    fc._keep = keep
+    o = Oracle(cfg, fc)
+    g = np.random.Generator(np.random.Philox(key=[8, 8]))
+    per = lambda a: np.pad(a, 3, mode="wrap")  # noqa: E731
+    u, v, w = (g.uniform(-1, 1, (n, n, n)) for _ in range(3))
+    for f, a in ((capi.F_U, u), (capi.F_V, v), (capi.F_W, w)):
+        o.upload(f, np.asfortranarray(per(a)))
+    o.op_advection()
+    U, V, W = per(u), per(v), per(w)
+    s = np.s_[3:3 + n, 3:3 + n, 3:3 + n]
+
+    def sh(a, dx=0, dy=0, dz=0):
+        return a[3 + dx:3 + dx + n, 3 + dy:3 + dy + n, 3 + dz:3 + dz + n]
+
+    # u component at (i+1/2, j, k)
+    au = ((sh(U, 1) + sh(U)) ** 2 - (sh(U) + sh(U, -1)) ** 2) \
+        + ((sh(U, dy=1) + sh(U)) * (sh(V, 1) + sh(V)) - (sh(U) + sh(U, dy=-1)) * (sh(V, 1, -1) + sh(V, 0, -1))) \
+        + ((sh(U, dz=1) + sh(U)) * (sh(W, 1) + sh(W)) - (sh(U) + sh(U, dz=-1)) * (sh(W, 1, 0, -1) + sh(W, 0, 0, -1)))
+    av = ((sh(V, 1) + sh(V)) * (sh(U, 0, 1) + sh(U)) - (sh(V) + sh(V, -1)) * (sh(U, -1, 1) + sh(U, -1))) \
+        + ((sh(V, dy=1) + sh(V)) ** 2 - (sh(V) + sh(V, dy=-1)) ** 2) \
+        + ((sh(V, dz=1) + sh(V)) * (sh(W, 0, 1) + sh(W)) - (sh(V) + sh(V, dz=-1)) * (sh(W, 0, 1, -1) + sh(W, 0, 0, -1)))
+    aw = ((sh(W, 1) + sh(W)) * (sh(U, 0, 0, 1) + sh(U)) - (sh(W) + sh(W, -1)) * (sh(U, -1, 0, 1) + sh(U, -1))) \
+        + ((sh(W, dy=1) + sh(W)) * (sh(V, 0, 0, 1) + sh(V)) - (sh(W) + sh(W, dy=-1)) * (sh(V, 0, -1, 1) + sh(V, 0, -1))) \
+        + ((sh(W, dz=1) + sh(W)) ** 2 - (sh(W) + sh(W, dz=-1)) ** 2)
+    # (the ghosts hold the periodic images; W's interior ends at k = n - 1
+    # between the z walls)
+    for fid, a, nk in ((F_USTAR, au, n), (F_VSTAR, av, n), (F_WSTAR, aw, n - 1)):
+        got = o.download(fid)[s][:, :, :nk]
+        want = -a[:, :, :nk] / 4.0
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+        assert np.abs(want).max() > 0.1
+
+
+def test_bound_pr_extrapolation_is_exact_for_quadratics(obstacle_case):
+    """Bound_Pr at a wall: Pr(0) = Pr(3) - 3 (Pr(2) - Pr(1)) is the value of
+    the quadratic through levels 1..3, Pr(-1) = Pr(2); periodic sides copy."""
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=81)
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    g = np.random.Generator(np.random.Philox(key=[81, 1]))
+    a, b, c = g.uniform(-1, 1, (3, nx, ny))
+    P = np.zeros(o.shape(), order="F")
+    kk = np.arange(1, nz + 1, dtype=float)
+    P[3:3 + nx, 3:3 + ny, 3:3 + nz] = a[..., None] + b[..., None] * kk + c[..., None] * kk ** 2
+    o.upload(capi.F_PR, P)
+    o.op_bound_pr()
+    R = o.download(capi.F_PR)
+    I = np.s_[3:3 + nx, 3:3 + ny]
+    np.testing.assert_allclose(R[I][..., 2], a, rtol=0, atol=1e-12)                       # k = 0
+    np.testing.assert_allclose(R[I][..., nz + 3], a + b * (nz + 1) + c * (nz + 1) ** 2, rtol=0, atol=1e-10)
+    assert np.array_equal(R[I][..., 1], R[I][..., 4])                                      # Pr(-1) = Pr(2)
+    assert np.array_equal(R[I][..., nz + 4], R[I][..., nz + 1])                            # Pr(nz+2) = Pr(nz-1)
+    # periodic x and y
+    assert np.array_equal(R[2, 3:3 + ny, 3:3 + nz], R[nx + 2, 3:3 + ny, 3:3 + nz])
+    assert np.array_equal(R[nx + 4, 3:3 + ny, 3:3 + nz], R[4, 3:3 + ny, 3:3 + nz])
+    assert np.array_equal(R[3:3 + nx, 1, 3:3 + nz], R[3:3 + nx, ny + 1, 3:3 + nz])
+
+
+@pytest.mark.parametrize("kind", ["neumann", "dirichlet"])
+def test_bound_scalar_reproduces_linear_profiles(kind):
+    """BoundScalar at the z walls: a Neumann flux alpha gives ghost values on
+    the linear profile alpha z (S(0) = S(1) - alpha dzW(0), S(-1) = S(1) -
+    alpha (zPr(1) - zPr(-1)), ...); a Dirichlet value S_w mirrors a profile
+    linear through the wall value.  x and y are periodic copies."""
+    alpha, sw = 0.37, 1.25
+    side = [0, 0, 0, 0, alpha, alpha] if kind == "neumann" else [0, 0, 0, 0, sw, sw]
+    bt = capi.NEUMANN if kind == "neumann" else capi.DIRICHLET
+    cfg = cases.uniform_config(8, 8, 8, dx=0.5, dt=0.05, nscalars=1,
+                               scal_btype=[capi.PERIODIC] * 4 + [bt, bt], side_scal=side)
+    o = Oracle(cfg, capi.Case(cfg))
+    nz = 8
+    zpr = (np.arange(-2, nz + 4) - 0.5) * 0.5          # zPr(k) = (k - 1/2) dz, k = -2..nz+3
+    if kind == "neumann":
+        prof = alpha * zpr
+    else:
+        prof = sw + 0.8 * (zpr - 0.0)                   # linear through S_w at z = 0
+    S = np.zeros(o.shape(), order="F")
+    S[3:11, 3:11, 3:3 + nz] = prof[3:3 + nz]
+    o.upload(capi.F_SCALAR, S)
+    o.op_bound_scalar(0)
+    R = o.download(capi.F_SCALAR)
+    col = R[5, 6, :]
+    if kind == "neumann":
+        np.testing.assert_allclose(col[:3], prof[:3], rtol=0, atol=1e-14)
+        # the top flux is +alpha outward: S(nz+m) = S(nz) + alpha (zPr(nz+m) - zPr(nz))
+        np.testing.assert_allclose(col[3 + nz:], prof[3 + nz:], rtol=0, atol=1e-14)
+    else:
+        np.testing.assert_allclose(col[:3], prof[:3], rtol=0, atol=1e-14)
+        top = sw - (prof[3 + nz - 1::-1][:3] - sw)
+        np.testing.assert_allclose(col[3 + nz:], top, rtol=0, atol=1e-14)
+    assert np.array_equal(R[0:3, 3:11, 3:11], R[8:11, 3:11, 3:11])
+    assert np.array_equal(R[3:11, 11:14, 3:11], R[3:11, 3:6, 3:11])
+
+
+def test_bound_viscosity_rules_brute_force(obstacle_case):
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=91)
+    nu = rand_field(o.shape(), 92, 1.0, 2.0)
+    o.upload(capi.F_VISC, nu)
+    o.op_bound_visc()
+    R = o.download(capi.F_VISC)
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    want = nu.copy()
+    # x (j, k interior): periodic
+    for k in range(1, nz + 1):
+        for j in range(1, ny + 1):
+            want[0 + 2, j + 2, k + 2] = want[nx + 2, j + 2, k + 2]
+            want[nx + 3, j + 2, k + 2] = want[1 + 2, j + 2, k + 2]
+    for k in range(1, nz + 1):
+        for i in range(0, nx + 2):
+            want[i + 2, 2, k + 2] = want[i + 2, ny + 2, k + 2]
+            want[i + 2, ny + 3, k + 2] = want[i + 2, 3, k + 2]
+    # z walls: zero gradient over i, j = 0..n+1
+    for j in range(0, ny + 2):
+        for i in range(0, nx + 2):
+            want[i + 2, j + 2, 2] = want[i + 2, j + 2, 3]
+            want[i + 2, j + 2, nz + 3] = want[i + 2, j + 2, nz + 2]
+    assert np.array_equal(R, want)
