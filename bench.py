@@ -125,7 +125,8 @@ def main():
     ap.add_argument("--serve", type=int, default=8, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-steps-scale", type=float, default=0.1,
                     help="run lengths U{200..2000} scaled by this factor to bound the run")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--cpu-threads", type=int, default=cpu_threads_default())
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps of the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES and print its bandwidth")
     args = ap.parse_args()
@@ -150,7 +151,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, cfg, case, fields, args.cpu_threads)
+        cpu = cpu_baseline(wl, cfg, case, fields, args.cpu_threads, steps=args.cpu_steps)
 
     cells = cfg.nx * cfg.ny * cfg.nz
     out = {
@@ -198,25 +199,54 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline(wl, cfg, case, fields, threads):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads_default():
+    """Host cores this process may use: OMP_NUM_THREADS when set (the GPU
+    box sets its CPU share there), else the CPUs of the process's affinity
+    mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(wl, cfg, case, fields, threads, warmup=1, steps=5):
     """The oracle (C restatement of the reference step, OpenMP FP64) timed on
-    a bounded sample: one step of the same case after its initial
-    projection."""
+    the host cores: after the initial projection, `warmup` untimed steps,
+    then `steps` timed steps (reduced-step sample of the workload), with the
+    OpenMP threads bound to cores (OMP_PROC_BIND=close, OMP_PLACES=cores; set
+    before the oracle library and its OpenMP runtime are loaded)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import Oracle
     o = Oracle(cfg, case)
     cases.upload_initial(o, cfg, fields)
     o.init()
-    nsteps = 1
+    if warmup:
+        o.step(warmup)
     t0 = time.perf_counter()
-    o.step(nsteps)
+    o.step(steps)
     dt = time.perf_counter() - t0
     cells = cfg.nx * cfg.ny * cfg.nz
     o.close()
-    return {"value": round(cells * nsteps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps} step of {wl.name} ({cfg.nx}x{cfg.ny}x{cfg.nz}) after the initial projection, "
-                      f"oracle C/OpenMP FP64, {dt:.1f} s"}
+    return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"reduced-step: {steps} timed steps after {warmup} warm-up step(s) of {wl.name} "
+                      f"({cfg.nx}x{cfg.ny}x{cfg.nz}) following the initial projection; oracle C/OpenMP FP64, "
+                      f"{dt:.1f} s, {dt / steps:.2f} s per step",
+            "nproc": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)), "cpu_model": _cpu_model(),
+            "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}}
 
 
 if __name__ == "__main__":

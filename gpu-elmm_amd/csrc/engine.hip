@@ -18,7 +18,9 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 #include <vector>
@@ -194,6 +196,12 @@ struct EngineIface {
   virtual void step(int nsteps) = 0;
   virtual void enqueue_steps(int nsteps) = 0;
   virtual double divergence() = 0;
+  // asynchronous field frame (elmm_frame_async / elmm_frame_wait)
+  virtual void frame_async(const std::string &dir) = 0;
+  virtual int frame_wait() = 0;
+  virtual std::string frame_error_text() const = 0;
+  // tile shapes and z chunks in use (elmm_get_tiles)
+  virtual void tiles(elmm_tiles *out) const = 0;
   // Reads the device step record (synchronises the stream): CFL and delta
   // of the last step, divergence by CFL_max (src/main.f90:161-168).
   void read_monitor() {
@@ -313,7 +321,7 @@ struct Engine : EngineIface {
   DevBuf X64, twx64, twy64;
   PoissonPlan<double> pp64{};
   DevBuf partial, cfl_partial, div_partial;
-  int top_blocks = 0, red_blocks = 0, cfl_blocks = 0;
+  int top_blocks = 0, red_blocks = 0, cfl_blocks = 0, cfl_cap = 0;
   int refine_iters = 0;
   // momentum kernel selection: the fused z-marching kernel on uniform grids
   // (ELMM_MOMENTUM=percomp selects the per-component kernels), its z chunk
@@ -326,6 +334,13 @@ struct Engine : EngineIface {
   // ELMM_ZCHUNK sets both chunk lengths)
   bool zmarch_scalar = false;
   int zchunk_scalar = 16;
+  // fused momentum tile (ELMM_MOM_TILE=32x8 / 64x8 / 64x4 fixes it)
+  int mom_tile = kMomTile64x4;
+  // settings fixed by the environment are not tuned
+  bool env_mom_tile = false, env_zchunk = false, env_scalar_zchunk = false, env_sgs_zchunk = false;
+  bool tuned = false;
+  void autotune();
+  StageConst stage_const(int st) const;
 
   IBDev ibdev(int c) const {
     return IBDev{ib_n[c], ib_tgt[c].as<int32_t>(), ib_ptr[c].as<int32_t>(), ib_src[c].as<int32_t>(),
@@ -383,6 +398,32 @@ struct Engine : EngineIface {
   void first_call();
   int replay_steps(int nsteps);
   double divergence() override;
+  // frames: pinned host staging of U, V, W, Pr and the scalars in the device
+  // layout, filled by copies on the engine's stream; a writer thread waits
+  // for them and writes the reference's VTK files
+  std::vector<void *> frame_host;
+  hipEvent_t frame_done = nullptr;
+  std::thread frame_thread;
+  int frame_status = 0;
+  std::string frame_error;
+  void frame_async(const std::string &dir) override;
+  int frame_wait() override;
+  std::string frame_error_text() const override { return frame_error; }
+  ~Engine() override {
+    if (frame_thread.joinable()) frame_thread.join();
+    for (void *p : frame_host) (void)hipHostFree(p);
+    if (frame_done) (void)hipEventDestroy(frame_done);
+  }
+  void tiles(elmm_tiles *out) const override {
+    std::memset(out, 0, sizeof *out);
+    out->mom_tx = !fused_momentum ? 0 : mom_tile == kMomTile32x8 ? 32 : 64;
+    out->mom_ty = !fused_momentum ? 0 : mom_tile == kMomTile64x4 ? 4 : 8;
+    if (sizeof(T) == 8 && mom_tile == kMomTile64x8) out->mom_tx = 32;
+    out->mom_zchunk = fused_momentum ? zchunk : 0;
+    out->sgs_zchunk = g.sgs_zchunk > 0 ? g.sgs_zchunk : 26;
+    out->scalar_zchunk = zmarch_scalar ? zchunk_scalar : 0;
+    out->tuned = tuned ? 1 : 0;
+  }
   double max_div(T *a, T *b, T *c, bool use_q);
   double reduce_max(DevBuf &buf, int n);
 };
@@ -417,14 +458,26 @@ void Engine<T>::setup_geo() {
     g.sgs_cellwise = sg && std::string(sg) == "cellwise";
     const char *sz = std::getenv("ELMM_SGS_ZCHUNK");
     g.sgs_zchunk = sz ? std::max(0, std::atoi(sz)) : 0;
+    env_sgs_zchunk = g.sgs_zchunk > 0;
+    if (const char *mt = std::getenv("ELMM_MOM_TILE")) {
+      const std::string v = mt;
+      mom_tile = v == "32x8" ? kMomTile32x8 : v == "64x8" ? kMomTile64x8 : kMomTile64x4;
+      env_mom_tile = true;
+    }
     const char *sk = std::getenv("ELMM_SCALAR");
     zmarch_scalar = !g.variable_z && !(sk && std::string(sk) == "cellwise");
     const char *gv = std::getenv("ELMM_GRAPH");
     use_graphs = !(gv && std::string(gv) == "0");
     const char *zc = std::getenv("ELMM_ZCHUNK");
-    if (zc && std::atoi(zc) > 0) zchunk = zchunk_scalar = std::atoi(zc);
+    if (zc && std::atoi(zc) > 0) {
+      zchunk = zchunk_scalar = std::atoi(zc);
+      env_zchunk = env_scalar_zchunk = true;
+    }
     const char *zs = std::getenv("ELMM_SCALAR_ZCHUNK");
-    if (zs && std::atoi(zs) > 0) zchunk_scalar = std::atoi(zs);
+    if (zs && std::atoi(zs) > 0) {
+      zchunk_scalar = std::atoi(zs);
+      env_scalar_zchunk = true;
+    }
   }
   for (int s = 0; s < 6; s++) {
     g.bt[s] = cfg.btype[s];
@@ -508,23 +561,25 @@ void Engine<T>::setup_geo() {
   }
 }
 
-static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-
 template <typename T>
 void Engine<T>::setup_poisson() {
   const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
-  if (!is_pow2(nx) || !is_pow2(ny)) throw ArgError("FFT pressure solver needs power-of-two nx and ny");
+  if (!fft_len_ok(nx) || !fft_len_ok(ny)) throw ArgError("FFT pressure solver needs nx and ny of the form 2^a 3^b");
   if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
   if (nz > 256) throw ArgError("z solver supports nz <= 256");
   const int M = nx / 2 + 1;
   // spectral rows padded to 128 bytes, so that the y pass's 8-mode segments
   // are whole cache lines
   const int Mp = solve64 || sizeof(T) == 8 ? (M + 7) / 8 * 8 : (M + 15) / 16 * 16;
-  // exp(-2 pi i q/n), evaluated in long double and rounded once to R
-  auto tw = [](int n, auto one) {
+  // exp(-2 pi i q/n), evaluated in long double and rounded once to R; q <
+  // n/2 for the radix-2 passes, q < n for the mixed-radix ones (x passes:
+  // when nx or ny is not a power of two; y passes: when ny is not)
+  const bool mr_x = !fft_pow2(nx) || !fft_pow2(ny), mr_y = !fft_pow2(ny);
+  auto tw = [](int n, bool full, auto one) {
     using R = decltype(one);
-    std::vector<C2<R>> t(std::max(1, n / 2));
-    for (int q = 0; q < n / 2; q++) {
+    const int len = full ? n : n / 2;
+    std::vector<C2<R>> t(std::max(1, len));
+    for (int q = 0; q < len; q++) {
       long double a = 2.0L * 3.14159265358979323846264338327950288L * q / n;
       t[q].x = (R)(double)cosl(a);
       t[q].y = (R)(double)-sinl(a);
@@ -534,12 +589,12 @@ void Engine<T>::setup_poisson() {
   using V2 = C2<T>;
   if (!solve64) {
     X.alloc((size_t)Mp * ny * nz * sizeof(V2));
-    upload_vec(twx, tw(nx, (T)1));
-    upload_vec(twy, tw(ny, (T)1));
+    upload_vec(twx, tw(nx, mr_x, (T)1));
+    upload_vec(twy, tw(ny, mr_y, (T)1));
   } else {
     X64.alloc((size_t)Mp * ny * nz * sizeof(C2<double>));
-    upload_vec(twx64, tw(nx, 1.0));
-    upload_vec(twy64, tw(ny, 1.0));
+    upload_vec(twx64, tw(nx, mr_x, 1.0));
+    upload_vec(twy64, tw(ny, mr_y, 1.0));
   }
   std::vector<double> lx(M), ly(ny), a(nz), c(nz);
   for (int m = 0; m < M; m++) {
@@ -570,10 +625,18 @@ void Engine<T>::setup_poisson() {
   if (const char *zp = std::getenv("ELMM_ZPIPE")) pp.zpipe = pp64.zpipe = std::atoi(zp) ? 1 : 0;
   top_blocks = (int)((nx * ny + 255) / 256);
   red_blocks = (int)(((long)nx * ny * nz + 255) / 256);
+  {
+    // room for the most partials any SGS z chunk the tuner tries gives
+    const int saved = g.sgs_zchunk;
+    if (cfg.autotune && !env_sgs_zchunk) g.sgs_zchunk = 13;
+    cfl_cap = sgs_partials(g, sgs_width);
+    g.sgs_zchunk = saved;
+  }
   cfl_blocks = sgs_partials(g, sgs_width);
+  cfl_cap = std::max(cfl_cap, cfl_blocks);
   partial.alloc((std::max(top_blocks, 1) + 1) * sizeof(double));
   div_partial.alloc(std::max(red_blocks, 1) * sizeof(double));
-  cfl_partial.alloc(std::max(cfl_blocks, 1) * sizeof(double));
+  cfl_partial.alloc(std::max(cfl_cap, 1) * sizeof(double));
   mon_buf.alloc(sizeof(StepMon));
   mon = mon_buf.as<StepMon>();
   StepMon m0{};
@@ -1117,7 +1180,7 @@ void Engine<T>::subgrid_stresses(bool cfl) {
 }
 
 template <typename T>
-void Engine<T>::stage(int st, bool last) {
+StageConst Engine<T>::stage_const(int st) const {
   const double dt = dt_cur;
   StageConst sc{};
   sc.stage = st;
@@ -1133,6 +1196,13 @@ void Engine<T>::stage(int st, bool last) {
   sc.Az = -sc.coef / g.dzmin;
   sc.prgx = cfg.pr_gradient_x;
   sc.prgy = cfg.pr_gradient_y;
+  return sc;
+}
+
+template <typename T>
+void Engine<T>::stage(int st, bool last) {
+  const double dt = dt_cur;
+  const StageConst sc = stage_const(st);
 
   subgrid_stresses(st == 0 && (cfg.cfl_check || cfg.variable_dt));
   T *ustar[3] = {Us.as<T>(), Vs.as<T>(), Ws.as<T>()};
@@ -1140,7 +1210,7 @@ void Engine<T>::stage(int st, bool last) {
   if (fused_momentum) {
     const uint8_t *codes[3] = {code[0].as<uint8_t>(), code[1].as<uint8_t>(), code[2].as<uint8_t>()};
     timed(ELMM_T_MOMENTUM, [&] {
-      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, codes, sc, zchunk,
+      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, codes, sc, zchunk, mom_tile,
                                  stream);
     });
   }
@@ -1303,6 +1373,123 @@ void Engine<T>::choose_dt() {
   last_cfl = m * dt;
 }
 
+// ---------------------------------------------------------------- autotuner
+// The reference sizes its CPU loop tiles from the L1 size and the number of
+// arrays a loop touches (InitTiles, src/tiling.f90:16-111).  Here the tile
+// shapes and z-chunk lengths of the z-marching kernels are measured: on the
+// first step of a handle (before any step graph is captured) every candidate
+// runs on the case's own fields with the first stage's constants, timed with
+// HIP events (one warm-up launch, then three timed), and the fastest is
+// kept.  Every candidate computes the same bits (only the work split
+// differs).  The launches write only what the first stage then rewrites
+// (Ustar and U2 interiors, the viscosity, the new scalar and its tendency),
+// so tuning leaves the step unchanged.  Choices are cached per process by
+// field type and grid shape; settings fixed by environment variables are
+// kept.  ELMM_TUNE_LOG=1 prints the choices.
+namespace {
+struct TileChoice {
+  int mom_tile, zchunk, sgs_zchunk, scalar_zchunk;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, TileChoice> g_tune_cache;
+}  // namespace
+
+template <typename T>
+void Engine<T>::autotune() {
+  tuned = true;
+  const auto key = std::make_tuple((int)sizeof(T), cfg.nx, cfg.ny, cfg.nz, nsc);
+  TileChoice c{mom_tile, zchunk, g.sgs_zchunk, zchunk_scalar};
+  bool hit = false;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      c = it->second;
+      hit = true;
+    }
+  }
+  if (!hit) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto time_of = [&](auto launch) {
+      launch();
+      HIPCHK(hipEventRecord(e0, stream));
+      for (int r = 0; r < 3; r++) launch();
+      HIPCHK(hipEventRecord(e1, stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    const StageConst sc0 = stage_const(0);
+    if (fused_momentum) {
+      T *ustar[3] = {Us.as<T>(), Vs.as<T>(), Ws.as<T>()};
+      T *U2p[3] = {u2, v2, w2};
+      const uint8_t *codes[3] = {code[0].as<uint8_t>(), code[1].as<uint8_t>(), code[2].as<uint8_t>()};
+      std::vector<int> tiles = env_mom_tile ? std::vector<int>{mom_tile}
+                                            : std::vector<int>{kMomTile64x4, kMomTile32x8};
+      if (!env_mom_tile && sizeof(T) == 4) tiles.push_back(kMomTile64x8);
+      const std::vector<int> chunks = env_zchunk ? std::vector<int>{zchunk} : std::vector<int>{64, 32};
+      float best = 1e30f;
+      for (int t : tiles)
+        for (int ch : chunks) {
+          const float ms = time_of([&] {
+            launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, codes, sc0, ch, t, stream);
+          });
+          if (ms < best) {
+            best = ms;
+            c.mom_tile = t;
+            c.zchunk = ch;
+          }
+        }
+    }
+    if (!env_sgs_zchunk && !g.sgs_cellwise && !g.variable_z && sgs_width.tab) {
+      const int saved = g.sgs_zchunk;
+      float best = 1e30f;
+      for (int ch : {26, 32, 16, 13}) {
+        g.sgs_zchunk = ch;
+        const float ms = time_of([&] {
+          launch_sgs(g, u, v, w, Visc.as<T>(), cfg.nu, cfg.c_smag, cfg.filter_ratio, sgs_width, cfg.sgs_model,
+                     (double *)nullptr, 0, stream);
+        });
+        if (ms < best) {
+          best = ms;
+          c.sgs_zchunk = ch;
+        }
+      }
+      g.sgs_zchunk = saved;
+    }
+    if (zmarch_scalar && nsc > 0 && !env_scalar_zchunk) {
+      float best = 1e30f;
+      for (int ch : {16, 13, 26, 32}) {
+        const float ms = time_of([&] {
+          launch_scalar_zm(g, u, v, w, TDiff.as<T>(), s_cur[0], s_nxt[0], Sadv[0]->as<T>(), code_s.as<uint8_t>(),
+                           mask[0].as<uint8_t>(), mask[1].as<uint8_t>(), mask[2].as<uint8_t>(), sc0, ch, stream);
+        });
+        if (ms < best) {
+          best = ms;
+          c.scalar_zchunk = ch;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[key] = c;
+  }
+  mom_tile = c.mom_tile;
+  zchunk = c.zchunk;
+  g.sgs_zchunk = c.sgs_zchunk;
+  zchunk_scalar = c.scalar_zchunk;
+  cfl_blocks = sgs_partials(g, sgs_width);
+  if (std::getenv("ELMM_TUNE_LOG"))
+    std::fprintf(stderr, "[elmm tune] %dx%dx%d %s%s: momentum tile %s, z chunk %d; sgs z chunk %d; scalar z chunk %d\n",
+                 cfg.nx, cfg.ny, cfg.nz, sizeof(T) == 4 ? "fp32r" : "fp64", hit ? " (cached)" : "",
+                 c.mom_tile == kMomTile64x4 ? "64x4" : c.mom_tile == kMomTile32x8 ? "32x8" : "64x8", c.zchunk,
+                 c.sgs_zchunk > 0 ? c.sgs_zchunk : 26, c.scalar_zchunk);
+}
+
 // Steps of TMarchRK3 and the main loop's time update (src/main.f90:112-169).
 // Constant dt: every step but the last (which carries the diagnostics) may
 // replay a step graph; the CFL monitor runs on the device and is read once
@@ -1310,6 +1497,7 @@ void Engine<T>::choose_dt() {
 // below dt_min from step 3 on ends the call as divergence (main.f90:153-159).
 template <typename T>
 void Engine<T>::run_steps(int nsteps, bool sync_diag) {
+  if (cfg.autotune && !tuned && nsteps > 0) autotune();
   int s = 0;
   auto one = [&](bool last) {
     if (cfg.variable_dt) choose_dt();
@@ -1357,6 +1545,66 @@ double Engine<T>::divergence() {
   bound_uvw(u, v, w);
   const bool use_q = cfg.ib_mass_sources && has_ib() && first_step_done;
   return max_div(u, v, w, use_q);
+}
+
+// Field frame without a host synchronisation (the reference writes frames
+// from pthreads, src/frames.f90:369-420): device-to-host copies of the
+// current U, V, W, Pr and scalars into pinned staging on the engine's stream
+// (so the frame is the state after every step enqueued before it), then a
+// writer thread waits for the copies, converts them to ghosted FP64 arrays
+// and writes out.vtk, scalars.vtk and U/V/W.vtk (elmm_vtk_write).  Steps
+// enqueued afterwards run while the files are written.  A frame started
+// while another is being written first joins it.
+template <typename T>
+void Engine<T>::frame_async(const std::string &dir) {
+  if (frame_thread.joinable()) frame_wait();
+  const int nf = 4 + nsc;
+  if (frame_host.empty()) {
+    for (int f = 0; f < nf; f++) {
+      void *p = nullptr;
+      HIPCHK(hipHostMalloc(&p, (size_t)g.ntot * sizeof(T), hipHostMallocDefault));
+      frame_host.push_back(p);
+    }
+    HIPCHK(hipEventCreateWithFlags(&frame_done, hipEventDisableTiming));
+  }
+  const void *src[12] = {u, v, w, Pr.p};
+  for (int s = 0; s < nsc; s++) src[4 + s] = s_cur[s];
+  for (int f = 0; f < nf; f++)
+    HIPCHK(hipMemcpyAsync(frame_host[f], src[f], (size_t)g.ntot * sizeof(T), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipEventRecord(frame_done, stream));
+  frame_status = 0;
+  frame_error.clear();
+  frame_thread = std::thread([this, dir, nf] {
+    if (hipEventSynchronize(frame_done) != hipSuccess) {
+      frame_status = ELMM_E_HIP;
+      frame_error = "frame copies failed";
+      return;
+    }
+    const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+    const long hy = nx + 6, hz = (long)(nx + 6) * (ny + 6), hn = hz * (nz + 6);
+    std::vector<std::vector<double>> host(nf, std::vector<double>(hn, 0.0));
+    for (int f = 0; f < nf; f++) {
+      const T *d = static_cast<const T *>(frame_host[f]);
+      for (int k = -2; k <= nz + 3; k++)
+        for (int j = -2; j <= ny + 3; j++)
+          for (int i = -2; i <= nx + 3; i++)
+            host[f][(i + 2) + (j + 2) * hy + (k + 2) * hz] = (double)d[gix(g, i, j, k)];
+    }
+    std::vector<const double *> sp;
+    for (int s = 0; s < nsc; s++) sp.push_back(host[4 + s].data());
+    const int rc = elmm_vtk_write(dir.c_str(), &cfg, host[0].data(), host[1].data(), host[2].data(), host[3].data(),
+                                  nullptr, nsc, sp.empty() ? nullptr : sp.data());
+    if (rc) {
+      frame_status = rc;
+      frame_error = elmm_last_error();
+    }
+  });
+}
+
+template <typename T>
+int Engine<T>::frame_wait() {
+  if (frame_thread.joinable()) frame_thread.join();
+  return frame_status;
 }
 
 template struct Engine<double>;
@@ -1548,6 +1796,25 @@ extern "C" const char *elmm_timer_name(int32_t g) {
       "sgs", "wall_model", "momentum", "momentum_corr", "scalar", "scalar_corr", "ghost", "immersed_boundary",
       "poisson_x_fwd", "poisson_y", "poisson_z", "poisson_x_inv", "projection", "reduce", "other", "refine"};
   return (g >= 0 && g < ELMM_T_NGROUPS) ? names[g] : "invalid";
+}
+
+extern "C" int elmm_frame_async(elmm_handle *h, const char *dir) {
+  if (!h || !dir) return elmm::fail(ELMM_E_ARG, "null argument to elmm_frame_async");
+  if (h->e->cfg.nscalars > 8) return elmm::fail(ELMM_E_ARG, "frames hold at most 8 scalars");
+  return guarded([&] { h->e->frame_async(dir); });
+}
+
+extern "C" int elmm_frame_wait(elmm_handle *h) {
+  if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  const int rc = h->e->frame_wait();
+  if (rc) return elmm::fail(rc, static_cast<elmm::EngineIface *>(h->e.get())->frame_error_text());
+  return 0;
+}
+
+extern "C" int elmm_get_tiles(elmm_handle *h, elmm_tiles *out) {
+  if (!h || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_get_tiles");
+  h->e->tiles(out);
+  return 0;
 }
 
 extern "C" int elmm_destroy(elmm_handle *h) {

@@ -442,23 +442,14 @@ void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, 
 }
 }  // namespace
 
-// Tile shape: 64x4 by default (measured 5 % faster than 32x8 in FP32, equal
-// in FP64; 32x4, 32x16 and 64x16 measured slower); ELMM_MOM_TILE=32x8 or
-// 64x8 selects another instantiation (64x8 only for 4-byte fields: its FP64
-// LDS rings exceed the 160 KiB of a workgroup).
-static int mom_tile_choice() {
-  static int choice = -1;
-  if (choice < 0) {
-    const char *e = std::getenv("ELMM_MOM_TILE");
-    const std::string v = e ? e : "";
-    choice = v == "32x8" ? 0 : v == "64x8" ? 3 : 1;
-  }
-  return choice;
-}
-
+// Tile ids: kMomTile64x4 (default; measured 5 % faster than 32x8 in FP32 on
+// 512x512x128, equal in FP64), kMomTile32x8, kMomTile64x8 (4-byte fields
+// only: its FP64 LDS rings exceed the 160 KiB of a workgroup; an FP64 launch
+// asking for it runs 32x8).  The engine's autotuner times them per grid.
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *const Us[3],
-                        T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, hipStream_t st) {
+                        T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, int tile,
+                        hipStream_t st) {
   MomZArgs<T> a;
   a.U = U;
   a.V = V;
@@ -471,9 +462,9 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
     a.code[c] = code[c];
   }
   if (kchunk <= 0) kchunk = g.nz;
-  switch (mom_tile_choice()) {
-    case 1: launch_tile<T, 64, 4>(g, a, sc, kchunk, st); break;
-    case 3:
+  switch (tile) {
+    case kMomTile64x4: launch_tile<T, 64, 4>(g, a, sc, kchunk, st); break;
+    case kMomTile64x8:
       if constexpr (sizeof(T) == 4) {
         launch_tile<T, 64, 8>(g, a, sc, kchunk, st);
         break;
@@ -486,9 +477,9 @@ void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const 
 
 template void launch_momentum_zm<double>(const Geo &, const double *, const double *, const double *, const double *,
                                          const double *, double *const[3], double *const[3],
-                                         const uint8_t *const[3], StageConst, int, hipStream_t);
+                                         const uint8_t *const[3], StageConst, int, int, hipStream_t);
 template void launch_momentum_zm<float>(const Geo &, const float *, const float *, const float *, const float *,
                                         const float *, float *const[3], float *const[3], const uint8_t *const[3],
-                                        StageConst, int, hipStream_t);
+                                        StageConst, int, int, hipStream_t);
 
 }  // namespace elmm

@@ -531,6 +531,10 @@ template <typename T, typename R>
 void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
                       const uint32_t *qmask, const double *rhs, const double *phi, double dt2_rec, double *rhs_out,
                       hipStream_t st) {
+  if (!fft_pow2(pp.nx) || !fft_pow2(pp.ny)) {
+    launch_fft_x_fwd_mr<T, R>(g, pp, U, V, W, Q, qmask, rhs, phi, dt2_rec, rhs_out, st);
+    return;
+  }
   const int pm = kPadXFwd, nrows = pp.ny * pp.nz;
   const bool fast = !g.variable_z && g.rdx != 0 && g.rdy != 0 && g.rdz != 0;
   const int mode =
@@ -566,6 +570,10 @@ constexpr int kFftYModes = 8;
 
 template <typename R>
 void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
+  if (!fft_pow2(pp.ny)) {
+    launch_fft_y_mr<R>(pp, inverse, st);
+    return;
+  }
   const int pm = kPadY;
   with_logn(ilog2(pp.ny), [&](auto L) {
     constexpr int LOGN = decltype(L)::value, TX = kFftYModes;
@@ -590,6 +598,10 @@ void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t s
 
 template <typename R>
 void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st) {
+  if (!fft_pow2(pp.nx) || !fft_pow2(pp.ny)) {
+    launch_fft_x_inv_mr<R>(g, pp, Phi, accumulate, st);
+    return;
+  }
   const int pm = kPadXInv, nrows = pp.ny * pp.nz;
   with_logn(ilog2(pp.nx), [&](auto L) {
     constexpr int LOGN = decltype(L)::value, ROWS = xrows(LOGN);

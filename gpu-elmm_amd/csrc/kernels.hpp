@@ -86,9 +86,11 @@ void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W,
 // fused three-component momentum kernel, z-marching (k_momentum_zm.hip)
 bool momentum_zm_supported(const Geo &g);
 size_t momentum_zm_lds_bytes(int elem);
+enum { kMomTile32x8 = 0, kMomTile64x4 = 1, kMomTile64x8 = 3 };
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *const Us[3],
-                        T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, hipStream_t st);
+                        T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, int tile,
+                        hipStream_t st);
 template <typename T>
 void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
                           T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,
@@ -153,6 +155,22 @@ void launch_fft_x_fwd(const Geo &g, const PoissonPlan<R> &pp, const T *U, const 
                       const uint32_t *qmask, const double *rhs, const double *phi, double dt2_rec, double *rhs_out,
                       hipStream_t st);
 template <typename R> void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st);
+// Mixed-radix (2^a 3^b) forms of the transform passes (k_fft_mr.hip); the
+// launchers above hand over to them when nx or ny is not a power of two.
+template <typename T, typename R>
+void launch_fft_x_fwd_mr(const Geo &g, const PoissonPlan<R> &pp, const T *U, const T *V, const T *W, const double *Q,
+                         const uint32_t *qmask, const double *rhs, const double *phi, double dt2_rec,
+                         double *rhs_out, hipStream_t st);
+template <typename R> void launch_fft_y_mr(const PoissonPlan<R> &pp, int inverse, hipStream_t st);
+template <typename R>
+void launch_fft_x_inv_mr(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st);
+// n = 2^a 3^b; a power of two
+inline bool fft_len_ok(int n) {
+  if (n < 1) return false;
+  while (n % 3 == 0) n /= 3;
+  return (n & (n - 1)) == 0;
+}
+inline bool fft_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 template <typename R> void launch_poisson_solve_z(const Geo &g, const PoissonPlan<R> &pp, hipStream_t st);
 template <typename R>
 void launch_fft_x_inv(const Geo &g, const PoissonPlan<R> &pp, double *Phi, int accumulate, hipStream_t st);

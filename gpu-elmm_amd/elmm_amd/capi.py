@@ -47,7 +47,7 @@ class Config(C.Structure):
         ("variable_dt", C.c_int32), ("delta_diag", C.c_int32),
         ("cfl_target", C.c_double), ("dt_max", C.c_double), ("dt_min", C.c_double),
         ("start_time", C.c_double), ("end_time", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("autotune", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -97,6 +97,12 @@ class Timers(C.Structure):
     _fields_ = [("ms", C.c_double * T_NGROUPS), ("launches", C.c_int64 * T_NGROUPS)]
 
 
+class Tiles(C.Structure):
+    _fields_ = [("mom_tx", C.c_int32), ("mom_ty", C.c_int32), ("mom_zchunk", C.c_int32),
+                ("sgs_zchunk", C.c_int32), ("scalar_zchunk", C.c_int32), ("tuned", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
 class Cuboid(C.Structure):
     _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
                 ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double), ("z0", C.c_double)]
@@ -133,7 +139,13 @@ EXPORTS = {
     "elmm_wait": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
     "elmm_divergence": (C.c_int, [C.c_void_p, _pd]),
     "elmm_destroy": (C.c_int, [C.c_void_p]),
+    "elmm_vtk_write": (C.c_int, [C.c_char_p, C.POINTER(Config), _pd, _pd, _pd, _pd, _pd, C.c_int32,
+                                 C.POINTER(_pd)]),
+    "elmm_vtk_read": (C.c_int, [C.c_char_p, C.POINTER(Config), _pd, _pd, _pd, _pd, C.c_int32, C.POINTER(_pd)]),
+    "elmm_frame_async": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "elmm_frame_wait": (C.c_int, [C.c_void_p]),
     "elmm_timers_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "elmm_get_tiles": (C.c_int, [C.c_void_p, C.POINTER(Tiles)]),
     "elmm_timers_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "elmm_timer_name": (C.c_char_p, [C.c_int32]),
     "elmm_version": (C.c_char_p, []),
@@ -342,6 +354,20 @@ class Solver:
         return {_lib.elmm_timer_name(g).decode(): (t.ms[g], t.launches[g]) for g in range(T_NGROUPS)
                 if t.launches[g] > 0}
 
+    def frame_async(self, directory):
+        """elmm_frame_async: VTK files of the current state, written while
+        later steps run."""
+        _check(_lib.elmm_frame_async(self._h, os.fsencode(directory)))
+
+    def frame_wait(self):
+        _check(_lib.elmm_frame_wait(self._h))
+
+    def tiles(self):
+        """Tile shape and z chunks in use (elmm_get_tiles) as a dict."""
+        t = Tiles()
+        _check(_lib.elmm_get_tiles(self._h, C.byref(t)))
+        return {f[0]: getattr(t, f[0]) for f in Tiles._fields_ if f[0] != "reserved"}
+
     def divergence(self):
         x = C.c_double()
         _check(_lib.elmm_divergence(self._h, C.byref(x)))
@@ -426,3 +452,27 @@ def copy_probe(nbytes=1 << 30, reps=5):
     g = C.c_double()
     _check(load().elmm_copy_probe(int(nbytes), int(reps), C.byref(g)))
     return g.value
+
+
+def vtk_write(directory, cfg, U, V, W, Pr, scalars=(), visc=None):
+    """elmm_vtk_write: the reference's out.vtk, scalars.vtk and U/V/W.vtk
+    from ghosted host arrays (lower bound -2)."""
+    lib = load()
+    arrs = [np.asfortranarray(a, dtype=np.float64) for a in (U, V, W, Pr)]
+    sc = [np.asfortranarray(a, dtype=np.float64) for a in scalars]
+    vs = np.asfortranarray(visc, dtype=np.float64) if visc is not None else None
+    sp = (_pd * max(1, len(sc)))(*[a.ctypes.data_as(_pd) for a in sc])
+    _check(lib.elmm_vtk_write(os.fsencode(directory), C.byref(cfg), *[a.ctypes.data_as(_pd) for a in arrs],
+                              vs.ctypes.data_as(_pd) if vs is not None else None, len(sc), sp))
+
+
+def vtk_read(directory, cfg, nscalars=0):
+    """elmm_vtk_read: (U, V, W, Pr, [scalars]) as ghosted arrays."""
+    lib = load()
+    shape = ghosted_shape(cfg)
+    out = [np.zeros(shape, order="F") for _ in range(4)]
+    sc = [np.zeros(shape, order="F") for _ in range(nscalars)]
+    sp = (_pd * max(1, len(sc)))(*[a.ctypes.data_as(_pd) for a in sc])
+    _check(lib.elmm_vtk_read(os.fsencode(directory), C.byref(cfg), *[a.ctypes.data_as(_pd) for a in out],
+                             len(sc), sp))
+    return out[0], out[1], out[2], out[3], sc

@@ -2,8 +2,8 @@
 forms (both on top of the library's continuous-batching server,
 elmm_server_*).
 
-A scenario is a case of N cells (x/y rounded to powers of two for the
-spectral solver, z to a multiple of 8, Nx = Ny ~ 2 Nz), random cuboid
+A scenario is a case of N cells (x/y rounded to multiples of 16 of the form
+2^a 3^b for the spectral solver, z to a multiple of 8, Nx = Ny ~ 2 Nz), random cuboid
 obstacles up to a plan-area density, a wind of random direction and speed,
 and a point source at a free street-level cell; dt = 0.05 s.
 """
@@ -22,20 +22,25 @@ def _rng(seed, salt):
     return np.random.Generator(np.random.Philox(key=[int(seed), int(salt)]))
 
 
+# x/y extents the spectral solver takes (2^a 3^b) that are multiples of 16
+XY_SIZES = sorted({16 * (2 ** a) * (3 ** b) for a in range(7) for b in range(3) if 16 * (2 ** a) * (3 ** b) <= 1024})
+
+
 def grid_for_cells(n):
-    """(nx, ny, nz) with nx = ny a power of two near (4n)^(1/3) and nz a
-    multiple of 8 in [16, 256] so that nx*ny*nz ~ n."""
-    nx = 2 ** int(round(math.log2((4 * n) ** (1.0 / 3.0))))
-    nx = max(32, min(1024, nx))
+    """(nx, ny, nz) with nx = ny ~ 2 nz: nx the multiple of 16 of the form
+    2^a 3^b nearest (in ratio) to (2n)^(1/3), nz a multiple of 8 in
+    [8, 256] so that nx*ny*nz ~ n (64^3 -> 96x96x32, 192x192x96 -> itself)."""
+    target = (2.0 * n) ** (1.0 / 3.0)
+    nx = min((s for s in XY_SIZES if s >= 32), key=lambda s: abs(math.log(s / target)))
     nz = int(round(n / (nx * nx) / 8.0)) * 8
-    return nx, nx, max(16, min(256, nz))
+    return nx, nx, max(8, min(256, nz))
 
 
 def scenario(seed, ncells, precision=capi.FP32_REFINED, density=None, speed=None, theta=None):
     """Config, cuboids, point sources and initial fields of one scenario."""
     rng = _rng(seed, 17)
     nx, ny, nz = grid_for_cells(ncells)
-    cfg = capi.with_precision(cases.uniform_config(nx, ny, nz, dx=1.0, dt=0.05, nscalars=1), precision)
+    cfg = capi.with_precision(cases.uniform_config(nx, ny, nz, dx=1.0, dt=0.05, nscalars=1, autotune=1), precision)
     density = rng.uniform(0.10, 0.30) if density is None else density
     # one cuboid per 16x16 tile, in random tile order, until the plan-area
     # density is reached; footprints 6-12 cells with a free margin

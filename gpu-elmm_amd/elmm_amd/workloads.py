@@ -32,7 +32,7 @@ class Workload:
 
 
 def _flat512():
-    cfg = cases.uniform_config(512, 512, 128, dx=1.0, dt=0.05, nscalars=1)
+    cfg = cases.uniform_config(512, 512, 128, dx=1.0, dt=0.05, nscalars=1, autotune=1)
     return cfg, [], [(101.3, 257.7, 0.5, 1.0, 0)]
 
 
@@ -64,7 +64,7 @@ def _uniform64():
 
 
 def _channel256():
-    return cases.channel_config(256, 128, 128), [], []
+    return cases.channel_config(256, 128, 128, autotune=1), [], []
 
 
 _WL = {

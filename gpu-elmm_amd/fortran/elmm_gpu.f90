@@ -49,7 +49,8 @@ module elmm_gpu
     integer(c_int32_t) :: precision, ib_mass_sources, refine_iters
     integer(c_int32_t) :: variable_dt, delta_diag
     real(c_double) :: cfl_target, dt_max, dt_min, start_time, end_time
-    integer(c_int32_t) :: reserved(4)
+    integer(c_int32_t) :: autotune
+    integer(c_int32_t) :: reserved(3)
   end type
 
   type, bind(C) :: elmm_cuboid

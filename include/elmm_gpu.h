@@ -117,7 +117,11 @@ typedef struct elmm_config {
   double  start_time;        /* time_stepping%start_time                      */
   double  end_time;          /* time_stepping%end_time: a variable dt is cut to
                                 reach it (src/dynamics.f90:548-549); 0 = none */
-  int32_t reserved[4];
+  int32_t autotune;          /* 1: time the candidate tile shapes and z chunks
+                                of the z-marching kernels on the first step
+                                and keep the fastest (cached per grid shape);
+                                replaces InitTiles (src/tiling.f90:16-111)    */
+  int32_t reserved[3];
 } elmm_config;
 
 /*
@@ -265,6 +269,32 @@ int elmm_footprint(const elmm_config *cfg, int64_t *bytes);
 int elmm_case_footprint(const elmm_config *cfg, const elmm_case *c, int64_t *bytes);
 
 /* ---------------------------------------------------------------------------
+ * Field files (host only): the reference's legacy VTK files, RECTILINEAR_GRID,
+ * BINARY, big-endian float32.  Arrays are ghosted host arrays (lower bound
+ * -2, extents n+6, Fortran order), as elmm_download returns them.
+ *   elmm_vtk_write: out.vtk (p, visc when given, cell-centred VECTORS u),
+ *     scalars.vtk (scalar01..), U.vtk, V.vtk, W.vtk in directory dir.
+ *     replaces: OutputOut, OutputScalars, OutputUVW
+ *               (src/outputs.f90:1560-1854, 2260-2490)
+ *   elmm_vtk_read: the restart reader; p (0 when absent), the scalars, and
+ *     U, V, W from U/V/W.vtk, else from out.vtk's vectors.
+ *     replaces: ReadInitialConditions (src/initial.f90:3575-3811)
+ * ------------------------------------------------------------------------- */
+int elmm_vtk_write(const char *dir, const elmm_config *cfg, const double *U, const double *V, const double *W,
+                   const double *Pr, const double *visc, int32_t nscal, const double *const *scalars);
+int elmm_vtk_read(const char *dir, const elmm_config *cfg, double *U, double *V, double *W, double *Pr,
+                  int32_t nscal, double *const *scalars);
+
+/* Asynchronous field frame: copies the current U, V, W, Pr and scalars to
+ * pinned host memory in stream order (after every step enqueued before the
+ * call) and writes the VTK files of elmm_vtk_write into dir from a writer
+ * thread, while later steps run.  elmm_frame_wait joins the writer and
+ * returns its status.  replaces: the frame writers (src/frames.f90:369-420,
+ * pthreads.c). */
+int elmm_frame_async(elmm_handle *h, const char *dir);
+int elmm_frame_wait(elmm_handle *h);
+
+/* ---------------------------------------------------------------------------
  * Serving form: continuous batching of independent cases on one GPU.  No
  * counterpart in the reference (one case per process, src/main.f90).
  * Requests are admitted in arrival order while their footprint fits the HBM
@@ -320,6 +350,17 @@ typedef struct elmm_timers {
   int64_t launches[ELMM_T_NGROUPS];  /* kernel launches per group            */
 } elmm_timers;
 int elmm_timers_enable(elmm_handle *h, int32_t enable);
+
+/* Tile shape (x by y columns per block) and z-chunk lengths of the
+ * z-marching kernels a handle uses; tuned = 1 once the autotuner has chosen
+ * them.  replaces: the tile sizes of InitTiles (src/tiling.f90:16-111). */
+typedef struct elmm_tiles {
+  int32_t mom_tx, mom_ty, mom_zchunk;   /* fused momentum kernel (0: per-cell kernels) */
+  int32_t sgs_zchunk, scalar_zchunk;    /* SGS and scalar z-marches (0: per-cell)      */
+  int32_t tuned;
+  int32_t reserved[2];
+} elmm_tiles;
+int elmm_get_tiles(elmm_handle *h, elmm_tiles *out);
 int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset);
 const char *elmm_timer_name(int32_t group);
 

@@ -1849,6 +1849,81 @@ static void fft_line(c2 *x, int n, long stride, const c2 *tw, int inverse, c2 *w
   for (int m = 0; m < n; m++) x[m * stride] = work[m];
 }
 
+/* Mixed-radix FFT for n = 2^a 3^b that is not a power of two: self-sorting
+   (Stockham) stages, radix 3 first, then radix 2.  A stage with radix r and
+   span ns (product of the earlier radices) takes element j + q n/r (q < r),
+   multiplies it by tw[k q n/(ns r)] (k = j mod ns; conjugate for the
+   inverse; no product when k q = 0), forms the r-point DFT and writes
+   element (j / ns) ns r + k + q ns.  3-point DFT: s = b + c, d = b - c,
+   y0 = a + s, m = a - s/2, t = (s3 d.im, -s3 d.re) (forward) or
+   (-s3 d.im, s3 d.re) (inverse), y1 = m + t, y2 = m - t, s3 = sin(pi/3).
+   tw holds exp(-2 pi i t/n) for t < n; work holds 2n values. */
+static void fft_line_mr(c2 *x, int n, long stride, const c2 *tw, int inverse, c2 *work) {
+  c2 *src = work, *dst = work + n;
+  for (int m = 0; m < n; m++) src[m] = x[m * stride];
+  const double s3 = 0.86602540378443864676;
+  int rest = n, ns = 1;
+  while (rest > 1) {
+    const int r = rest % 3 == 0 ? 3 : 2, nb = n / r, tstride = n / (ns * r);
+    for (int j = 0; j < nb; j++) {
+      const int k = j % ns;
+      c2 v[3];
+      for (int q = 0; q < r; q++) {
+        v[q] = src[j + q * nb];
+        if (q > 0 && k > 0) {
+          c2 w = tw[k * q * tstride], a = v[q];
+          if (inverse) w.im = -w.im;
+          v[q].re = a.re * w.re - a.im * w.im;
+          v[q].im = a.re * w.im + a.im * w.re;
+        }
+      }
+      const int base = (j / ns) * ns * r + k;
+      if (r == 2) {
+        dst[base].re = v[0].re + v[1].re;
+        dst[base].im = v[0].im + v[1].im;
+        dst[base + ns].re = v[0].re - v[1].re;
+        dst[base + ns].im = v[0].im - v[1].im;
+      } else {
+        c2 sm = {v[1].re + v[2].re, v[1].im + v[2].im}, df = {v[1].re - v[2].re, v[1].im - v[2].im};
+        c2 m = {v[0].re - 0.5 * sm.re, v[0].im - 0.5 * sm.im};
+        c2 t = inverse ? (c2){-(s3 * df.im), s3 * df.re} : (c2){s3 * df.im, -(s3 * df.re)};
+        dst[base].re = v[0].re + sm.re;
+        dst[base].im = v[0].im + sm.im;
+        dst[base + ns].re = m.re + t.re;
+        dst[base + ns].im = m.im + t.im;
+        dst[base + 2 * ns].re = m.re - t.re;
+        dst[base + 2 * ns].im = m.im - t.im;
+      }
+    }
+    c2 *tmp = src;
+    src = dst;
+    dst = tmp;
+    ns *= r;
+    rest /= r;
+  }
+  for (int m = 0; m < n; m++) x[m * stride] = src[m];
+}
+
+static c2 *twiddles_full(int n) {
+  c2 *t = xcalloc(n, sizeof(c2));
+  for (int q = 0; q < n; q++) {
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * q / n;
+    t[q].re = (double)cosl(a);
+    t[q].im = (double)-sinl(a);
+  }
+  return t;
+}
+
+static int is_pow2i(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* One transform of a line: radix 2 for powers of two, else mixed radix. */
+static void fft_any(c2 *x, int n, long stride, const c2 *tw, int inverse, c2 *work) {
+  if (is_pow2i(n))
+    fft_line(x, n, stride, tw, inverse, work);
+  else
+    fft_line_mr(x, n, stride, tw, inverse, work);
+}
+
 /* Thomas elimination of one tridiagonal system a_k x_{k-1} + b_k x_k +
    c_k x_{k+1} = d_k (a_0 = c_{nz-1} = 0; strictly diagonally dominant for
    every mode but the horizontal mean): den_k = b_k - a_k c'_{k-1},
@@ -1885,20 +1960,25 @@ static void kahan(double *s, double *cmp, double t) {
    The singular mode pins Phi_hat(k=1) = 0. */
 static void poisson_direct(oracle *o, const double *rhs, double *phi) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  if ((nx & (nx - 1)) || (ny & (ny - 1))) {
-    fprintf(stderr, "oracle: power-of-two nx, ny required by the FFT solver\n");
-    abort();
+  {
+    int a = nx, b = ny;
+    while (a % 3 == 0) a /= 3;
+    while (b % 3 == 0) b /= 3;
+    if (!is_pow2i(a) || !is_pow2i(b)) {
+      fprintf(stderr, "oracle: nx, ny of the form 2^a 3^b required by the FFT solver\n");
+      abort();
+    }
   }
   const int M = nx / 2 + 1;
   long plane = (long)M * ny;
   c2 *X = xcalloc(plane * nz, sizeof(c2));
-  c2 *twx = twiddles(nx), *twy = twiddles(ny);
+  c2 *twx = is_pow2i(nx) ? twiddles(nx) : twiddles_full(nx), *twy = is_pow2i(ny) ? twiddles(ny) : twiddles_full(ny);
   int nmax = nx > ny ? nx : ny;
   nmax = nmax > nz ? nmax : nz;
   /* x transform of each real row, keep modes 0..nx/2 */
 #pragma omp parallel
   {
-    c2 *work = xcalloc(nmax, sizeof(c2)), *line = xcalloc(nx, sizeof(c2));
+    c2 *work = xcalloc(2 * (size_t)nmax, sizeof(c2)), *line = xcalloc(nx, sizeof(c2));
 #pragma omp for collapse(2)
     for (int k = 0; k < nz; k++)
       for (int j = 0; j < ny; j++) {
@@ -1906,12 +1986,12 @@ static void poisson_direct(oracle *o, const double *rhs, double *phi) {
           line[i].re = rhs[i + (long)j * nx + (long)k * nx * ny];
           line[i].im = 0;
         }
-        fft_line(line, nx, 1, twx, 0, work);
+        fft_any(line, nx, 1, twx, 0, work);
         for (int m = 0; m < M; m++) X[(long)k * plane + (long)j * M + m] = line[m];
       }
 #pragma omp for collapse(2)
     for (int k = 0; k < nz; k++)
-      for (int m = 0; m < M; m++) fft_line(X + (long)k * plane + m, ny, M, twy, 0, work);
+      for (int m = 0; m < M; m++) fft_any(X + (long)k * plane + m, ny, M, twy, 0, work);
     free(work);
     free(line);
   }
@@ -1979,10 +2059,10 @@ static void poisson_direct(oracle *o, const double *rhs, double *phi) {
   double scale = 1.0 / ((double)nx * ny);
 #pragma omp parallel
   {
-    c2 *work = xcalloc(nmax, sizeof(c2)), *line = xcalloc(nx, sizeof(c2));
+    c2 *work = xcalloc(2 * (size_t)nmax, sizeof(c2)), *line = xcalloc(nx, sizeof(c2));
 #pragma omp for collapse(2)
     for (int k = 0; k < nz; k++)
-      for (int m = 0; m < M; m++) fft_line(X + (long)k * plane + m, ny, M, twy, 1, work);
+      for (int m = 0; m < M; m++) fft_any(X + (long)k * plane + m, ny, M, twy, 1, work);
 #pragma omp for collapse(2)
     for (int k = 0; k < nz; k++)
       for (int j = 0; j < ny; j++) {
@@ -1995,7 +2075,7 @@ static void poisson_direct(oracle *o, const double *rhs, double *phi) {
             line[i].im = -row[nx - i].im;
           }
         }
-        fft_line(line, nx, 1, twx, 1, work);
+        fft_any(line, nx, 1, twx, 1, work);
         for (int i = 0; i < nx; i++) phi[i + (long)j * nx + (long)k * nx * ny] = line[i].re * scale;
       }
     free(work);

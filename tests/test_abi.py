@@ -49,7 +49,7 @@ def _c_layout(struct_name, fields):
 @pytest.mark.parametrize("cls,cname", [(capi.Config, "elmm_config"), (capi.Lists, "elmm_lists"),
                                       (capi.Diag, "elmm_diag"), (capi.Cuboid, "elmm_cuboid"),
                                       (capi.PointSource, "elmm_point_source"), (capi.Timers, "elmm_timers"),
-                                      (capi.Request, "elmm_request"),
+                                      (capi.Request, "elmm_request"), (capi.Tiles, "elmm_tiles"),
                                       (capi.RequestStatus, "elmm_request_status")])
 def test_struct_layout(cls, cname):
     names = [f[0] for f in cls._fields_]

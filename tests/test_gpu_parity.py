@@ -176,3 +176,61 @@ def test_momentum_kernel_variants_bit_identical(require_gpu, monkeypatch, env):
     s.step(2)
     err = compare_state(s, o, cfg)
     assert max(err.values()) == 0.0, err
+
+
+@pytest.mark.parametrize("env", [{"ELMM_ZPIPE": "1"}, {"ELMM_MOM_TILE": "32x8"}, {"ELMM_MOM_TILE": "64x8"},
+                                 {"ELMM_MOM_TILE": "64x4", "ELMM_ZCHUNK": "17"}])
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32_REFINED])
+def test_variants_across_z_segments(require_gpu, monkeypatch, env, precision):
+    """nz = 40: two full 16-level segments of the pipelined z solve and a
+    partial one (its cross-segment prefetch and register rotation), and the
+    other momentum tile shapes: the same bits as the default launch (FP64:
+    and as the oracle) after 2 steps."""
+    cfg = capi.with_precision(cases.uniform_config(64, 32, 40, dx=1.0, dt=0.05, nscalars=1), precision)
+    cub = [(8.0, 16.0, 8.0, 16.0, 0.0, 22.0), (40.0, 52.0, 18.0, 30.0, 0.0, 33.0, 0.01)]
+    case = capi.Case(cfg, cuboids=cub, sources=[(4.5, 4.5, 0.5, 1.0, 0)])
+    f = cases.initial_fields(cfg, u0=3.0, amp=0.3, seed=6)
+    ref = capi.Solver(cfg, case)
+    cases.upload_initial(ref, cfg, f)
+    ref.init()
+    ref.step(2)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    s = capi.Solver(cfg, case)
+    cases.upload_initial(s, cfg, f)
+    s.init()
+    s.step(2)
+    err = compare_state(s, ref, cfg)
+    assert max(err.values()) == 0.0, (env, err)
+    if precision == capi.FP64:
+        o = Oracle(cfg, case)
+        cases.upload_initial(o, cfg, f)
+        o.init()
+        o.step(2)
+        err = compare_state(s, o, cfg)
+        assert max(err.values()) == 0.0, (env, err)
+
+
+def test_autotuned_tiles_give_the_same_bits(require_gpu):
+    """The autotuner times the candidate tiles on the first step and keeps the
+    fastest; the results equal the default launch bit for bit, and the chosen
+    shapes are reported (elmm_get_tiles)."""
+    for precision in (capi.FP64, capi.FP32_REFINED):
+        base = capi.with_precision(cases.uniform_config(128, 64, 48, dx=1.0, dt=0.05, nscalars=1), precision)
+        tuned = capi.with_precision(cases.uniform_config(128, 64, 48, dx=1.0, dt=0.05, nscalars=1, autotune=1),
+                                    precision)
+        case = capi.Case(base, cuboids=[(20.0, 36.0, 20.0, 36.0, 0.0, 14.0)])
+        f = cases.initial_fields(base, u0=2.0, amp=0.2, seed=8)
+        out = []
+        for cfg in (base, tuned):
+            s = capi.Solver(cfg, case)
+            cases.upload_initial(s, cfg, f)
+            s.init()
+            s.step(3)
+            out.append(s)
+        t = out[1].tiles()
+        print(f"precision {precision}: tuned tiles {t}; default {out[0].tiles()}", flush=True)
+        assert t["tuned"] == 1 and out[0].tiles()["tuned"] == 0
+        assert t["mom_tx"] * t["mom_ty"] == 256 and t["mom_zchunk"] in (32, 64)
+        err = compare_state(out[1], out[0], base)
+        assert max(err.values()) == 0.0, err

@@ -4,11 +4,21 @@ import numpy as np
 from elmm_amd import capi, serving
 
 
+def _two_three(n):
+    while n % 3 == 0:
+        n //= 3
+    return n & (n - 1) == 0
+
+
 def test_grid_for_cells_shapes():
-    for n in (262144, 1.4e6, 3538944, 8e6):
+    """Nx = Ny a multiple of 16 of the form 2^a 3^b near (2N)^(1/3), Nz a
+    multiple of 8; both ends of the ensemble's range as stated."""
+    for n in (262144, 1.4e6, 3538944, 8e6, 0.25e6):
         nx, ny, nz = serving.grid_for_cells(n)
-        assert nx == ny and nx & (nx - 1) == 0 and nz % 8 == 0 and 16 <= nz <= 256
+        assert nx == ny and nx % 16 == 0 and _two_three(nx) and nz % 8 == 0 and 8 <= nz <= 256
         assert 0.5 < nx * ny * nz / n < 2.0
+    assert serving.grid_for_cells(3538944) == (192, 192, 96)
+    assert serving.grid_for_cells(64 ** 3) == (96, 96, 32)
 
 
 def test_scenario_obstacles_and_source():
