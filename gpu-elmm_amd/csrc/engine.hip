@@ -196,6 +196,10 @@ struct EngineIface {
   virtual void step(int nsteps) = 0;
   virtual void enqueue_steps(int nsteps) = 0;
   virtual double divergence() = 0;
+  // divergence() with caller-provided device memory for the block maxima
+  // (at least partial_slots doubles) and the final maximum (result)
+  virtual double divergence_into(double *partials, long partial_slots, double *result) = 0;
+  virtual long divergence_blocks() const = 0;
   // asynchronous field frame (elmm_frame_async / elmm_frame_wait)
   virtual void frame_async(const std::string &dir) = 0;
   virtual int frame_wait() = 0;
@@ -398,6 +402,8 @@ struct Engine : EngineIface {
   void first_call();
   int replay_steps(int nsteps);
   double divergence() override;
+  double divergence_into(double *partials, long partial_slots, double *result) override;
+  long divergence_blocks() const override { return red_blocks; }
   // frames: pinned host staging of U, V, W, Pr and the scalars in the device
   // layout, filled by copies on the engine's stream; a writer thread waits
   // for them and writes the reference's VTK files
@@ -1547,6 +1553,22 @@ double Engine<T>::divergence() {
   return max_div(u, v, w, use_q);
 }
 
+// divergence() over caller-provided device memory: the block maxima into
+// `partials`, their maximum (a NaN counts as +inf) into `result` on the
+// device, then 8 bytes copied to the host.  Launches kernels only.
+template <typename T>
+double Engine<T>::divergence_into(double *partials, long partial_slots, double *result) {
+  if (red_blocks > partial_slots) throw ArgError("divergence scratch too small for the grid");
+  bound_uvw(u, v, w);
+  const bool use_q = cfg.ib_mass_sources && has_ib() && first_step_done;
+  launch_max_div(g, u, v, w, use_q ? Q.as<double>() : nullptr, partials, red_blocks, stream);
+  launch_max_final(partials, red_blocks, result, stream);
+  double m = 0;
+  HIPCHK(hipMemcpyAsync(&m, result, sizeof m, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return m;
+}
+
 // Field frame without a host synchronisation (the reference writes frames
 // from pthreads, src/frames.f90:369-420): device-to-host copies of the
 // current U, V, W, Pr and scalars into pinned staging on the engine's stream
@@ -1648,6 +1670,44 @@ static void release_engine(elmm::EngineIface *e) {
   e->stream = nullptr;
 }
 
+// ------------------------------------------- allocation-free divergence
+// elmm_divergence_noalloc keeps its block maxima and final maximum in this
+// file-scope device array instead of handle memory: one block maximum per
+// 256 cells, so grids of at most kNoallocMaxCells cells.  Every device the
+// library runs on holds its own copy until the process ends; the host uses
+// the per-device address from hipGetSymbolAddress (the array's name is a
+// host-side stand-in, not a device address).  Every block maximum and the
+// result are rewritten by each call, so nothing needs clearing; calls that
+// overlap on one device race on the array.
+constexpr long kNoallocSlots = 1L << 18;
+constexpr long kNoallocMaxCells = kNoallocSlots * 256;
+static __device__ __align__(256) double scratch[kNoallocSlots + 32];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (called from elmm_create).
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return elmm::fail(ELMM_E_ARG, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return elmm::fail(ELMM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  return 0;
+}
+
+double *scratch_of(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 extern "C" int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm_handle **out) {
   if (!cfg || !lists || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_create");
   *out = nullptr;
@@ -1659,6 +1719,7 @@ extern "C" int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm
   else
     h->e.reset(new elmm::Engine<float>());
   int rc = guarded([&] { h->e->create(*cfg, *lists); });
+  if (rc == 0) rc = scratch_setup(h->e->device);
   if (rc == 0)
     *out = h.release();
   else
@@ -1764,6 +1825,20 @@ extern "C" int elmm_query(elmm_handle *h, int32_t *idle) {
 extern "C" int elmm_divergence(elmm_handle *h, double *max_div) {
   if (!h || !max_div) return elmm::fail(ELMM_E_ARG, "null argument to elmm_divergence");
   return guarded([&] { *max_div = h->e->divergence(); });
+}
+
+extern "C" int elmm_divergence_noalloc(elmm_handle *h, double *max_div) {
+  if (!h || !max_div) return elmm::fail(ELMM_E_ARG, "null argument to elmm_divergence_noalloc");
+  const elmm_config &c = h->e->cfg;
+  const long cells = (long)c.nx * c.ny * c.nz;
+  if (cells > kNoallocMaxCells)
+    return elmm::fail(ELMM_E_ARG, "elmm_divergence_noalloc: grid of " + std::to_string(cells) +
+                                      " cells exceeds its scratch limit of " + std::to_string(kNoallocMaxCells) +
+                                      " cells (use elmm_divergence)");
+  double *ptr = scratch_of(h->e->device);
+  if (!ptr) return elmm::fail(ELMM_E_STATE, "elmm_divergence_noalloc: no scratch address for the handle's device");
+  // slots [0, kNoallocSlots): block maxima; slot kNoallocSlots: the result
+  return guarded([&] { *max_div = h->e->divergence_into(ptr, kNoallocSlots, ptr + kNoallocSlots); });
 }
 
 extern "C" int elmm_timers_enable(elmm_handle *h, int32_t enable) {

@@ -742,6 +742,21 @@ void launch_max_div(const Geo &g, const T *U, const T *V, const T *W, const doub
   hipLaunchKernelGGL(k_max_div<T>, dim3(nblocks), dim3(256), 0, st, g, U, V, W, Q, partial);
 }
 
+// Maximum of n block maxima (a NaN counts as +inf) into out[0]; one block.
+__global__ void __launch_bounds__(256) k_max_final(const double *__restrict__ partial, int n, double *__restrict__ out) {
+  double v = 0;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) {
+    const double x = partial[b];
+    v = x != x ? INFINITY : fmax(v, x);
+  }
+  const double m = block_max(v);
+  if (threadIdx.x == 0) out[0] = m;
+}
+
+void launch_max_final(const double *partial, int n, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, partial, n, out);
+}
+
 // Residue_variable_z, src/poisson.f90:23-60 (Phi ghosts: periodic x/y by
 // index wrap, mirror in z)
 __global__ void k_residual(Geo g, const double *__restrict__ Phi, const double *__restrict__ RHS,

@@ -182,6 +182,7 @@ void launch_projection(const Geo &g, T *U, T *V, T *W, T *Pr, const double *Phi,
 template <typename T>
 void launch_max_div(const Geo &g, const T *U, const T *V, const T *W, const double *Q, double *partial, int nblocks,
                     hipStream_t st);
+void launch_max_final(const double *partial, int n, double *out, hipStream_t st);
 // Step diagnostics kept in device memory (k_monitor.hip).
 struct StepMon {
   double cfl_last;        // CFL number of the last step (max * dt)

@@ -138,6 +138,7 @@ EXPORTS = {
     "elmm_step_async": (C.c_int, [C.c_void_p, C.c_int32]),
     "elmm_wait": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
     "elmm_divergence": (C.c_int, [C.c_void_p, _pd]),
+    "elmm_divergence_noalloc": (C.c_int, [C.c_void_p, _pd]),
     "elmm_destroy": (C.c_int, [C.c_void_p]),
     "elmm_vtk_write": (C.c_int, [C.c_char_p, C.POINTER(Config), _pd, _pd, _pd, _pd, _pd, C.c_int32,
                                  C.POINTER(_pd)]),
@@ -371,6 +372,13 @@ class Solver:
     def divergence(self):
         x = C.c_double()
         _check(_lib.elmm_divergence(self._h, C.byref(x)))
+        return x.value
+
+    def divergence_noalloc(self):
+        """elmm_divergence_noalloc: the same maximum without device
+        allocation (its partials in the library's static device array)."""
+        x = C.c_double()
+        _check(_lib.elmm_divergence_noalloc(self._h, C.byref(x)))
         return x.value
 
 

@@ -251,6 +251,14 @@ int elmm_wait(elmm_handle *h, elmm_diag *diag);
  * src/pressure.f90:884-989), computed in FP64 on the device. */
 int elmm_divergence(elmm_handle *h, double *max_div);
 
+/* elmm_divergence without device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops).  Its block maxima live in a
+ * file-scope device array of 2 MiB per device (held until the process
+ * ends), so grids of at most 67,108,864 cells (larger: ELMM_E_ARG).  Calls
+ * that overlap on one device race on that array. */
+int elmm_divergence_noalloc(elmm_handle *h, double *max_div);
+
 int elmm_destroy(elmm_handle *h);
 
 /* 1 in *idle when every step enqueued on the handle has completed, else 0
