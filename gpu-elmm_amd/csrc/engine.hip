@@ -200,6 +200,9 @@ struct EngineIface {
   // (at least partial_slots doubles) and the final maximum (result)
   virtual double divergence_into(double *partials, long partial_slots, double *result) = 0;
   virtual long divergence_blocks() const = 0;
+  // WMPoints%ustar (count; copied to out when out is not null and cap
+  // suffices; synchronises the stream)
+  virtual int point_ustar(double *out, long cap) = 0;
   // asynchronous field frame (elmm_frame_async / elmm_frame_wait)
   virtual void frame_async(const std::string &dir) = 0;
   virtual int frame_wait() = 0;
@@ -303,8 +306,16 @@ struct Engine : EngineIface {
   DevBuf width_tab, width_idx, sgs_rdz, sgs_rz2;
   SgsWidth sgs_width{nullptr, nullptr, 0};
   int wm_n = 0;
-  DevBuf wmp_ijk, wmp_div;
+  DevBuf wmp_ijk, wmp_div, wmp_dist, wmp_z0, wmp_logdz0, wmp_ustar;
   int wmp_n = 0;
+  WMUstarPoints wmpdev() const {
+    return WMUstarPoints{wmp_n,
+                         wmp_ijk.as<int32_t>(),
+                         wmp_dist.as<double>(),
+                         wmp_z0.as<double>(),
+                         wmp_logdz0.as<double>(),
+                         wmp_ustar.as<double>()};
+  }
   DevTerms mom_terms[3];
   std::vector<std::unique_ptr<DevTerms>> sc_terms;
   DevBuf src_flux;
@@ -404,6 +415,13 @@ struct Engine : EngineIface {
   double divergence() override;
   double divergence_into(double *partials, long partial_slots, double *result) override;
   long divergence_blocks() const override { return red_blocks; }
+  int point_ustar(double *out, long cap) override {
+    if (out && cap >= wmp_n && wmp_n > 0) {
+      HIPCHK(hipMemcpyAsync(out, wmp_ustar.p, (size_t)wmp_n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    return wmp_n;
+  }
   // frames: pinned host staging of U, V, W, Pr and the scalars in the device
   // layout, filled by copies on the engine's stream; a writer thread waits
   // for them and writes the reference's VTK files
@@ -793,6 +811,21 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   for (int p = 0; p < wmp_n; p++) pijk.insert(pijk.end(), {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]});
   upload_vec(wmp_ijk, pijk);
   wmp_div.alloc(std::max(wmp_n, 1) * sizeof(double));
+  {
+    // WMPoints' distance vectors, roughness, log(|dist|/z0) and u* (initially
+    // 1, the WMpoint default, src/wallmodels.f90:40)
+    std::vector<double> pd(L.wmp_dist, L.wmp_dist + 3 * (size_t)wmp_n), pz(L.wmp_z0, L.wmp_z0 + wmp_n),
+        plog(wmp_n, 0.0), pu(wmp_n, 1.0);
+    for (int p = 0; p < wmp_n; p++) {
+      const double *d = &pd[3 * p];
+      const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      if (pz[p] > 0) plog[p] = std::log(dist / pz[p]);
+    }
+    upload_vec(wmp_dist, pd);
+    upload_vec(wmp_z0, pz);
+    upload_vec(wmp_logdz0, plog);
+    upload_vec(wmp_ustar, pu);
+  }
   std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
   upload_vec(src_flux, sflux);
   for (int s = 0; s < nsc; s++) sc_terms.emplace_back(new DevTerms);
@@ -1171,6 +1204,10 @@ void Engine<T>::subgrid_stresses(bool cfl) {
                  cfl ? cfl_partial.as<double>() : nullptr, cfl_blocks, stream);
     });
   sgs_prelaunched = false;
+  // ComputeViscsWM: WMPoints%ustar (the viscosity it writes is overwritten
+  // by SubgridModel)
+  if (cfg.wall_model && wmp_n)
+    timed(ELMM_T_WALL, [&] { launch_wm_ustar<T>(g, u, v, w, wmpdev(), cfg.nu, (T *)nullptr, stream); });
   // constant dt with the CFL check: the step's CFL number and the divergence
   // test, on the device (graph-safe)
   if (cfl && !cfg.variable_dt)
@@ -1312,6 +1349,7 @@ void Engine<T>::init() {
     launch_bound_visc<T>(g, TDiff.as<T>(), stream);
   }
   for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(g, s_cur[s], stream);
+  if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, wmpdev(), cfg.nu, Visc.as<T>(), stream);
   if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
   HIPCHK(hipGetLastError());
@@ -1839,6 +1877,19 @@ extern "C" int elmm_divergence_noalloc(elmm_handle *h, double *max_div) {
   if (!ptr) return elmm::fail(ELMM_E_STATE, "elmm_divergence_noalloc: no scratch address for the handle's device");
   // slots [0, kNoallocSlots): block maxima; slot kNoallocSlots: the result
   return guarded([&] { *max_div = h->e->divergence_into(ptr, kNoallocSlots, ptr + kNoallocSlots); });
+}
+
+extern "C" int elmm_wm_ustar(elmm_handle *h, int32_t cap, double *ustar, int32_t *n) {
+  if (!h || !n) return elmm::fail(ELMM_E_ARG, "null argument to elmm_wm_ustar");
+  return guarded([&] {
+    const int m = h->e->point_ustar(nullptr, 0);
+    *n = m;
+    if (ustar) {
+      if (cap < m) throw elmm::ArgError("elmm_wm_ustar: cap " + std::to_string(cap) + " below the " + std::to_string(m) +
+                                        " wall-model points");
+      h->e->point_ustar(ustar, cap);
+    }
+  });
 }
 
 extern "C" int elmm_timers_enable(elmm_handle *h, int32_t enable) {

@@ -455,6 +455,65 @@ void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints
   hipLaunchKernelGGL(k_wm_fluxes<T>, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, wm, nu, variable_z);
 }
 
+// ComputeViscsWM (src/wallmodels.f90:2050-2152; neutral, wall velocity 0,
+// wallmodeltype 1): the persistent WMPoints%ustar of every scalar wall-model
+// point from the cell-centred two-point velocity (2069-2071), by
+// WMRoughVisc (1362-1379) where z0 > 0, else WMFlatVisc (1268-1304).  With
+// Visc (the initial-condition tail) the point's wall-model viscosity is
+// stored too; in SubgridStresses SubgridModel overwrites it, so the step
+// keeps only u*.  FP64 whatever the field type.
+template <typename T>
+__global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict__ V_, const T *__restrict__ W_,
+                           WMUstarPoints wp, double nu, T *__restrict__ Visc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= wp.n) return;
+  const int xi = wp.ijk[3 * p], yj = wp.ijk[3 * p + 1], zk = wp.ijk[3 * p + 2];
+  const long q = gix(g, xi, yj, zk);
+  const double v0 = ((double)U_[q] + (double)U_[q - 1]) / 2.0;
+  const double v1 = ((double)V_[q] + (double)V_[q - g.sy]) / 2.0;
+  const double v2 = ((double)W_[q] + (double)W_[q - g.sz]) / 2.0;
+  const double d0 = wp.dist[3 * p], d1 = wp.dist[3 * p + 1], d2 = wp.dist[3 * p + 2];
+  const double z0 = wp.z0[p];
+  double ustar = wp.ustar[p], visc;
+  if (z0 <= 0 && nu <= 0) {
+    ustar = 0;
+    visc = 0;
+  } else {
+    const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    double t0 = v0 - 0.0, t1 = v1 - 0.0, t2 = v2 - 0.0;
+    const double dot = t0 * d0 + t1 * d1 + t2 * d2;
+    const double dd = dist * dist;
+    t0 = t0 - dot * d0 / dd;
+    t1 = t1 - dot * d1 / dd;
+    t2 = t2 - dot * d2 / dd;
+    const double vel = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    if (z0 > 0) {
+      if (vel != 0) ustar = wm_rough_ustar(ustar, vel, dist, z0, nu, wp.logdz0[p]);
+      if (ustar < 0) ustar = 0;
+      if (vel > 0 && ustar * ustar * dist / vel > nu)
+        visc = ustar * ustar * dist / vel;
+      else
+        visc = nu > 0 ? nu : 0.0;
+    } else {
+      if (vel != 0) ustar = wm_flat_ustar(ustar, vel, dist, nu);
+      if (ustar < 0) ustar = 0;
+      if (vel > 0)
+        visc = dist * ustar / nu > 1 ? ustar * ustar * dist / vel : nu;
+      else
+        visc = nu;
+    }
+  }
+  wp.ustar[p] = ustar;
+  if (Visc) Visc[q] = (T)visc;
+}
+
+template <typename T>
+void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, WMUstarPoints wp, double nu, T *Visc,
+                     hipStream_t st) {
+  if (wp.n == 0) return;
+  hipLaunchKernelGGL(k_wm_ustar<T>, dim3(nblk(wp.n, 256)), dim3(256), 0, st, g, U, V, W, wp, nu, Visc);
+}
+
 // ------------------------------------------------------ CD4 advection
 template <typename T>
 __device__ inline T mask4(T a, T b, T c, T d) { return ((-a + 9 * b) + 9 * c) + -d; }
@@ -1002,6 +1061,8 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
                               SgsWidth, int, double *, int, hipStream_t);                                          \
   template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, WMPoints, double, int,           \
                                     hipStream_t);                                                                  \
+  template void launch_wm_ustar<T>(const Geo &, const T *, const T *, const T *, WMUstarPoints, double, T *,      \
+                                   hipStream_t);                                                                   \
   template void launch_momentum<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *, T *,   \
                                    T *, const uint8_t *, StageConst, hipStream_t);                                 \
   template void launch_momentum_corr3<T>(const Geo &, const T *, const T *, const T *, const T *, const T *,      \

@@ -46,6 +46,16 @@ struct WMScalarPoints {
   double *div;
 };
 
+// WMPoints with their persistent friction velocity (ComputeViscsWM).
+struct WMUstarPoints {
+  int n;
+  const int32_t *ijk;     // 3 per point
+  const double *dist;     // 3 per point
+  const double *z0;
+  const double *logdz0;   // log(|dist| / z0) where z0 > 0 (host libm)
+  double *ustar;
+};
+
 // IB list on the device (CSR).
 struct IBDev {
   int n;
@@ -80,6 +90,9 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
 template <typename T>
 void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,
                       hipStream_t st);
+template <typename T>
+void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, WMUstarPoints wp, double nu, T *Visc,
+                     hipStream_t st);
 template <typename T>
 void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *Ustar,
                      T *U2, const uint8_t *code, StageConst sc, hipStream_t st);

@@ -47,7 +47,7 @@ class Config(C.Structure):
         ("variable_dt", C.c_int32), ("delta_diag", C.c_int32),
         ("cfl_target", C.c_double), ("dt_max", C.c_double), ("dt_min", C.c_double),
         ("start_time", C.c_double), ("end_time", C.c_double),
-        ("autotune", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("autotune", C.c_int32), ("mg_cycles_max", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -139,6 +139,7 @@ EXPORTS = {
     "elmm_wait": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
     "elmm_divergence": (C.c_int, [C.c_void_p, _pd]),
     "elmm_divergence_noalloc": (C.c_int, [C.c_void_p, _pd]),
+    "elmm_wm_ustar": (C.c_int, [C.c_void_p, C.c_int32, _pd, C.POINTER(C.c_int32)]),
     "elmm_destroy": (C.c_int, [C.c_void_p]),
     "elmm_vtk_write": (C.c_int, [C.c_char_p, C.POINTER(Config), _pd, _pd, _pd, _pd, _pd, C.c_int32,
                                  C.POINTER(_pd)]),
@@ -373,6 +374,14 @@ class Solver:
         x = C.c_double()
         _check(_lib.elmm_divergence(self._h, C.byref(x)))
         return x.value
+
+    def wm_ustar(self):
+        """WMPoints%ustar of every scalar wall-model point (elmm_wm_ustar)."""
+        n = C.c_int32()
+        _check(_lib.elmm_wm_ustar(self._h, 0, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        _check(_lib.elmm_wm_ustar(self._h, n.value, out.ctypes.data_as(_pd), C.byref(n)))
+        return out[:n.value]
 
     def divergence_noalloc(self):
         """elmm_divergence_noalloc: the same maximum without device

@@ -50,7 +50,8 @@ module elmm_gpu
     integer(c_int32_t) :: variable_dt, delta_diag
     real(c_double) :: cfl_target, dt_max, dt_min, start_time, end_time
     integer(c_int32_t) :: autotune
-    integer(c_int32_t) :: reserved(3)
+    integer(c_int32_t) :: mg_cycles_max
+    integer(c_int32_t) :: reserved(2)
   end type
 
   type, bind(C) :: elmm_cuboid

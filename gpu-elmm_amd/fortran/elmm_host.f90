@@ -117,6 +117,7 @@ program elmm_host
   cfg%variable_dt = 0; cfg%delta_diag = 0
   cfg%cfl_target = 0.3_c_double; cfg%dt_max = dt; cfg%dt_min = 0; cfg%start_time = 0; cfg%end_time = 0
   cfg%autotune = 1
+  cfg%mg_cycles_max = 0
   cfg%reserved = 0
 
   allocate(cub(max(1, n_cuboids)), src(max(1, n_sources)))

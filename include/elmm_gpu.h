@@ -121,7 +121,9 @@ typedef struct elmm_config {
                                 of the z-marching kernels on the first step
                                 and keep the fastest (cached per grid shape);
                                 replaces InitTiles (src/tiling.f90:16-111)    */
-  int32_t reserved[3];
+  int32_t mg_cycles_max;     /* ELMM_POISSON_MG: V-cycles at most per solve
+                                (maxPoissoniter); 0 = 50                      */
+  int32_t reserved[2];
 } elmm_config;
 
 /*
@@ -250,6 +252,12 @@ int elmm_wait(elmm_handle *h, elmm_diag *diag);
 /* Maximum |div U| of the current state (PostPoisson check_divergence,
  * src/pressure.f90:884-989), computed in FP64 on the device. */
 int elmm_divergence(elmm_handle *h, double *max_div);
+
+/* Friction velocity u* of every scalar wall-model point (WMPoints%ustar,
+ * the state ComputeViscsWM keeps, src/wallmodels.f90:2050-2152), in the
+ * order of elmm_lists.wmp_ijk: *n receives the count; with ustar not NULL,
+ * cap >= *n values are copied (synchronises the handle's stream). */
+int elmm_wm_ustar(elmm_handle *h, int32_t cap, double *ustar, int32_t *n);
 
 /* elmm_divergence without device allocation: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under

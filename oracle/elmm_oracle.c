@@ -70,6 +70,7 @@ struct oracle {
   double *src_flux;
   int initialized, first_step_done;
   double last_res, last_div;
+  int last_mg_cycles;
   /* time stepping (src/main.f90:81-169) */
   double time, dt_last, delta_last, cfl_last;
   long long steps_total;
@@ -1082,10 +1083,59 @@ static void null_interior(oracle *o) {
     }
 }
 
-/* SubgridStresses, src/dynamics.f90:869-908 (neutral; ComputeViscsWM
-   only writes Viscosity that SubgridModel overwrites and the diagnostic
-   WMPoints%ustar, so it is not restated). */
+/* ComputeViscsWM, src/wallmodels.f90:2050-2152 (neutral: no buoyancy, no
+   prescribed u*, wall velocity 0, wallmodeltype 1): per scalar wall-model
+   point the cell-centred velocity from two-point averages (2069-2071), then
+   WMRoughVisc (1362-1379, via WMRoughStress 1336-1359) where z0 > 0, else
+   WMFlatVisc (1268-1304, via WMFlatStress 1243-1265).  WMPoints%ustar is the
+   persistent state (the flat law iterates from it); the viscosity written
+   at the point is returned in visc when write_visc is set (the caller in
+   SubgridStresses lets SubgridModel overwrite it, as the reference does). */
+static void wm_visc_points(oracle *o, int write_visc) {
+  if (!o->c.wall_model) return;
+  const double nu = o->c.nu;
+  wm_list *L = &o->wmp;
+  for (int p = 0; p < L->n; p++) {
+    const int xi = L->ijk[3 * p], yj = L->ijk[3 * p + 1], zk = L->ijk[3 * p + 2];
+    const double d[3] = {L->dist[3 * p], L->dist[3 * p + 1], L->dist[3 * p + 2]};
+    double vel[3], tan[3], dist;
+    const double wallu[3] = {0, 0, 0};
+    vel[0] = (A3(o, o->U, xi, yj, zk) + A3(o, o->U, xi - 1, yj, zk)) / 2.0;
+    vel[1] = (A3(o, o->V, xi, yj, zk) + A3(o, o->V, xi, yj - 1, zk)) / 2.0;
+    vel[2] = (A3(o, o->W, xi, yj, zk) + A3(o, o->W, xi, yj, zk - 1)) / 2.0;
+    double ustar = L->ustar[p], visc;
+    if (L->z0[p] > 0) {
+      vel_and_dist(tan, &dist, vel, wallu, d);
+      const double v = norm3(tan);
+      if (v != 0) ustar = wm_rough_ustar(ustar, v, dist, L->z0[p], nu);
+      if (ustar < 0) ustar = 0;
+      if (v > 0 && ustar * ustar * dist / v > nu)
+        visc = ustar * ustar * dist / v;
+      else if (nu > 0)
+        visc = nu;
+      else
+        visc = 0;
+    } else if (nu <= 0) {
+      ustar = 0;
+      visc = 0;
+    } else {
+      vel_and_dist(tan, &dist, vel, wallu, d);
+      const double v = norm3(tan);
+      if (v != 0) ustar = wm_flat_ustar(ustar, v, dist, nu);
+      if (ustar < 0) ustar = 0;
+      if (v > 0)
+        visc = dist * ustar / nu > 1 ? ustar * ustar * dist / v : nu;
+      else
+        visc = nu;
+    }
+    L->ustar[p] = ustar;
+    if (write_visc) A3(o, o->Visc, xi, yj, zk) = visc;
+  }
+}
+
+/* SubgridStresses, src/dynamics.f90:869-908 (neutral). */
 static void subgrid_stresses(oracle *o) {
+  wm_visc_points(o, 1);
   sgs_smag(o);
   wm_uvw_fluxes(o);
   bound_visc(o, o->Visc);
@@ -1956,6 +2006,220 @@ static void kahan(double *s, double *cmp, double t) {
   *s = s2;
 }
 
+/* ------------------------------------------------ geometric multigrid --- */
+/* The iterative solver of the pressure equation (poisson_solver = 3; the
+   role of PoissMG, src/multigrid.f90:1047-1172, with MG_CGC's V-cycle
+   994-1044, MG_GS 786-910, MG_res 912-977, Restrict 114-165, Prolongate
+   58-111).  The reference's multigrid is vertex-centred with uniform
+   coefficients; this one solves the cell-centred operator of
+   Residue_variable_z (src/poisson.f90:23-60: periodic x/y, staggered
+   Neumann walls in z, stretched z) that the direct solve inverts:
+     level l: nx/2^l x ny/2^l x nz cells (coarsened in x and y only), ax =
+       1/(dx 2^l)^2, ay = 1/(dy 2^l)^2, the fine z coefficients az(k), cz(k);
+     smoother: z-line Gauss-Seidel, columns with (i + j) even, then odd; a
+       column solves D_k P_k - az_k P_{k-1} - cz_k P_{k+1} = h_k - rhs_k with
+       h_k = ax (P_w + P_e) + ay (P_s + P_n) and D_k = (((ax + ax) + (ay +
+       ay)) + az_k) + cz_k by Thomas elimination (den_k = D_k - az_k e_{k-1},
+       e_k = cz_k / den_k from tables; g_k = ((h_k - rhs_k) + az_k g_{k-1}) /
+       den_k; P_k = g_k + e_k P_{k+1});
+     residual r = rhs - s, s = ((P_w - P) + (P_e - P)) ax + ((P_s - P) + (P_n
+       - P)) ay [+ az (P_b - P)] [+ cz (P_t - P)];
+     restriction: coarse rhs = ((r1 + r2) + (r3 + r4)) * 0.25 over the 2x2
+       children (i-pairs first), coarse P = 0 (MG_clear);
+     prolongation (bilinear in x, y): fine P += 0.75 (0.75 c + 0.25 c_x) +
+       0.25 (0.75 c_y + 0.25 c_xy), c_x / c_y the coarse neighbour on the
+       child's side;
+     cycle: 2 pre-smoothing sweeps, coarse correction, 2 post-smoothing
+       sweeps; the coarsest level (x or y below 4 cells after halving) takes
+       40 sweeps; cycles from P = 0 (PoissMG zeroes Phi) until the max-norm
+       residual is below poisson_tol (the signed-max test of MG_res,
+       src/multigrid.f90:975, taken on |r|) or mg_cycles_max cycles. */
+typedef struct {
+  int nx, ny;
+  double ax, ay;
+  double *phi, *rhs, *den, *e;
+} mg_level;
+
+#ifndef MG_COARSE_SWEEPS
+#define MG_COARSE_SWEEPS 40
+#endif
+static int mg_levels(int nx, int ny) {
+  int L = 1;
+  while (nx % 4 == 0 && ny % 4 == 0 && nx / 2 >= 4 && ny / 2 >= 4) {
+    nx /= 2;
+    ny /= 2;
+    L++;
+  }
+  return L;
+}
+
+static void mg_sweep(const mg_level *v, int nz, const double *az, const double *cz, int color) {
+  const int nx = v->nx, ny = v->ny;
+  const long plane = (long)nx * ny;
+  double *P = v->phi;
+#pragma omp parallel for collapse(2)
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      if (((i + j) & 1) != color) continue;
+      const int iw = i > 1 ? i - 1 : nx, ie = i < nx ? i + 1 : 1, js = j > 1 ? j - 1 : ny, jn = j < ny ? j + 1 : 1;
+      const long c = (i - 1) + (long)(j - 1) * nx;
+      const long w = (iw - 1) + (long)(j - 1) * nx, e = (ie - 1) + (long)(j - 1) * nx;
+      const long so = (i - 1) + (long)(js - 1) * nx, no = (i - 1) + (long)(jn - 1) * nx;
+      double g = 0;
+      for (int k = 1; k <= nz; k++) {
+        const long o = (long)(k - 1) * plane;
+        const double h = v->ax * (P[w + o] + P[e + o]) + v->ay * (P[so + o] + P[no + o]);
+        g = ((h - v->rhs[c + o]) + az[k - 1] * g) / v->den[k - 1];
+        P[c + o] = g;
+      }
+      for (int k = nz - 1; k >= 1; k--) {
+        const long o = (long)(k - 1) * plane;
+        P[c + o] = P[c + o] + v->e[k - 1] * P[c + o + plane];
+      }
+    }
+}
+
+static double mg_res_cell(const mg_level *v, int nz, const double *az, const double *cz, int i, int j, int k) {
+  const int nx = v->nx, ny = v->ny;
+  const long plane = (long)nx * ny, o = (long)(k - 1) * plane;
+  const int iw = i > 1 ? i - 1 : nx, ie = i < nx ? i + 1 : 1, js = j > 1 ? j - 1 : ny, jn = j < ny ? j + 1 : 1;
+  const double *P = v->phi;
+  const double p = P[(i - 1) + (long)(j - 1) * nx + o];
+  double s = ((P[(iw - 1) + (long)(j - 1) * nx + o] - p) + (P[(ie - 1) + (long)(j - 1) * nx + o] - p)) * v->ax +
+             ((P[(i - 1) + (long)(js - 1) * nx + o] - p) + (P[(i - 1) + (long)(jn - 1) * nx + o] - p)) * v->ay;
+  if (k > 1) s = s + az[k - 1] * (P[(i - 1) + (long)(j - 1) * nx + o - plane] - p);
+  if (k < nz) s = s + cz[k - 1] * (P[(i - 1) + (long)(j - 1) * nx + o + plane] - p);
+  return v->rhs[(i - 1) + (long)(j - 1) * nx + o] - s;
+}
+
+static double mg_residual_max(const mg_level *v, int nz, const double *az, const double *cz) {
+  double m = 0;
+#pragma omp parallel for collapse(2) reduction(max : m)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= v->ny; j++)
+      for (int i = 1; i <= v->nx; i++) {
+        const double r = fabs(mg_res_cell(v, nz, az, cz, i, j, k));
+        m = r != r ? INFINITY : (r > m ? r : m);
+      }
+  return m;
+}
+
+static void mg_restrict(const mg_level *f, mg_level *c, int nz, const double *az, const double *cz) {
+  const long cplane = (long)c->nx * c->ny;
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= nz; k++)
+    for (int J = 1; J <= c->ny; J++)
+      for (int I = 1; I <= c->nx; I++) {
+        const double r1 = mg_res_cell(f, nz, az, cz, 2 * I - 1, 2 * J - 1, k);
+        const double r2 = mg_res_cell(f, nz, az, cz, 2 * I, 2 * J - 1, k);
+        const double r3 = mg_res_cell(f, nz, az, cz, 2 * I - 1, 2 * J, k);
+        const double r4 = mg_res_cell(f, nz, az, cz, 2 * I, 2 * J, k);
+        const long q = (I - 1) + (long)(J - 1) * c->nx + (long)(k - 1) * cplane;
+        c->rhs[q] = ((r1 + r2) + (r3 + r4)) * 0.25;
+        c->phi[q] = 0;
+      }
+}
+
+static void mg_prolong(const mg_level *c, mg_level *f, int nz) {
+  const long cplane = (long)c->nx * c->ny, fplane = (long)f->nx * f->ny;
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= f->ny; j++)
+      for (int i = 1; i <= f->nx; i++) {
+        const int I = (i + 1) / 2, J = (j + 1) / 2;
+        const int Ix = (i & 1) ? (I > 1 ? I - 1 : c->nx) : (I < c->nx ? I + 1 : 1);
+        const int Jy = (j & 1) ? (J > 1 ? J - 1 : c->ny) : (J < c->ny ? J + 1 : 1);
+        const double *E = c->phi + (long)(k - 1) * cplane;
+        const double c00 = E[(I - 1) + (long)(J - 1) * c->nx], c10 = E[(Ix - 1) + (long)(J - 1) * c->nx];
+        const double c01 = E[(I - 1) + (long)(Jy - 1) * c->nx], c11 = E[(Ix - 1) + (long)(Jy - 1) * c->nx];
+        const double ex0 = 0.75 * c00 + 0.25 * c10, ex1 = 0.75 * c01 + 0.25 * c11;
+        const long q = (i - 1) + (long)(j - 1) * f->nx + (long)(k - 1) * fplane;
+        f->phi[q] = f->phi[q] + (0.75 * ex0 + 0.25 * ex1);
+      }
+}
+
+static void mg_vcycle(mg_level *lv, int l, int L, int nz, const double *az, const double *cz) {
+  if (l == L - 1) {
+    for (int s = 0; s < MG_COARSE_SWEEPS; s++) {
+      mg_sweep(&lv[l], nz, az, cz, 0);
+      mg_sweep(&lv[l], nz, az, cz, 1);
+    }
+    return;
+  }
+  for (int s = 0; s < 2; s++) {
+    mg_sweep(&lv[l], nz, az, cz, 0);
+    mg_sweep(&lv[l], nz, az, cz, 1);
+  }
+  mg_restrict(&lv[l], &lv[l + 1], nz, az, cz);
+  mg_vcycle(lv, l + 1, L, nz, az, cz);
+  mg_prolong(&lv[l + 1], &lv[l], nz);
+  for (int s = 0; s < 2; s++) {
+    mg_sweep(&lv[l], nz, az, cz, 0);
+    mg_sweep(&lv[l], nz, az, cz, 1);
+  }
+}
+
+/* Line-solve tables of a level: den_k and e_k (k = 1..nz). */
+void oracle_mg_tables(int nz, double ax, double ay, const double *az, const double *cz, double *den, double *e) {
+  double eprev = 0;
+  for (int k = 1; k <= nz; k++) {
+    const double D = (((ax + ax) + (ay + ay)) + az[k - 1]) + cz[k - 1];
+    den[k - 1] = D - az[k - 1] * eprev;
+    e[k - 1] = cz[k - 1] / den[k - 1];
+    eprev = e[k - 1];
+  }
+}
+
+/* Multigrid solve; phi and rhs as poisson_direct.  Returns the cycles run
+   (cycles_out) and the final max-norm residual (res_out). */
+static void poisson_mg(oracle *o, const double *rhs, double *phi, int *cycles_out, double *res_out) {
+  const int nx = o->nx, ny = o->ny, nz = o->nz;
+  const int L = mg_levels(nx, ny);
+  double *az = xcalloc(nz, sizeof(double)), *cz = xcalloc(nz, sizeof(double));
+  for (int k = 1; k <= nz; k++) {
+    az[k - 1] = k > 1 ? 1 / (C1D(o->dzW, k - 1) * C1D(o->dzPr, k)) : 0;
+    cz[k - 1] = k < nz ? 1 / (C1D(o->dzW, k) * C1D(o->dzPr, k)) : 0;
+  }
+  mg_level *lv = xcalloc(L, sizeof(mg_level));
+  for (int l = 0; l < L; l++) {
+    lv[l].nx = nx >> l;
+    lv[l].ny = ny >> l;
+    const double dxl = o->dxmin * (double)(1 << l), dyl = o->dymin * (double)(1 << l);
+    lv[l].ax = 1.0 / (dxl * dxl);
+    lv[l].ay = 1.0 / (dyl * dyl);
+    const long n = (long)lv[l].nx * lv[l].ny * nz;
+    lv[l].phi = l == 0 ? phi : xcalloc(n, sizeof(double));
+    lv[l].rhs = l == 0 ? (double *)rhs : xcalloc(n, sizeof(double));
+    lv[l].den = xcalloc(nz, sizeof(double));
+    lv[l].e = xcalloc(nz, sizeof(double));
+    oracle_mg_tables(nz, lv[l].ax, lv[l].ay, az, cz, lv[l].den, lv[l].e);
+  }
+  const long n0 = (long)nx * ny * nz;
+  for (long q = 0; q < n0; q++) phi[q] = 0;
+  const int maxc = o->c.mg_cycles_max > 0 ? o->c.mg_cycles_max : 50;
+  const double tol = o->c.poisson_tol > 0 ? o->c.poisson_tol : 1e-10;
+  int cyc = 0;
+  double R = mg_residual_max(&lv[0], nz, az, cz);
+  while (cyc < maxc && !(R < tol)) {
+    mg_vcycle(lv, 0, L, nz, az, cz);
+    cyc++;
+    R = mg_residual_max(&lv[0], nz, az, cz);
+  }
+  if (cycles_out) *cycles_out = cyc;
+  if (res_out) *res_out = R;
+  for (int l = 0; l < L; l++) {
+    if (l > 0) {
+      free(lv[l].phi);
+      free(lv[l].rhs);
+    }
+    free(lv[l].den);
+    free(lv[l].e);
+  }
+  free(lv);
+  free(az);
+  free(cz);
+}
+
 /* Direct solve; phi and rhs are interior arrays nx*ny*nz, x fastest.
    The singular mode pins Phi_hat(k=1) = 0. */
 static void poisson_direct(oracle *o, const double *rhs, double *phi) {
@@ -2169,7 +2433,10 @@ static void pressure_correction(oracle *o, double *U, double *V, double *W, int 
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) rhs[(i - 1) + (j - 1) * nx + (k - 1) * plane] = A3(o, o->RHS, i, j, k);
-  poisson_direct(o, rhs, phi);
+  if (o->c.poisson_solver == 3)
+    poisson_mg(o, rhs, phi, &o->last_mg_cycles, NULL);
+  else
+    poisson_direct(o, rhs, phi);
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
@@ -2311,6 +2578,7 @@ int oracle_init(oracle *o) {
     bound_visc(o, o->TDiff);
   }
   for (int s = 0; s < o->c.nscalars; s++) bound_scalar(o, o->S[s]);
+  wm_visc_points(o, 1);
   wm_uvw_fluxes(o);
   bound_visc(o, o->Visc);
   o->initialized = 1;
@@ -2487,6 +2755,11 @@ void oracle_op_ib_visc(oracle *o) {
     A3(o, o->Visc, q[0], q[1], q[2]) = ib_interp(o, 3, p, o->Visc);
   }
 }
+int32_t oracle_wm_ustar(oracle *o, double *out) {
+  if (out) memcpy(out, o->wmp.ustar, (size_t)o->wmp.n * sizeof(double));
+  return o->wmp.n;
+}
+void oracle_op_wm_visc(oracle *o) { wm_visc_points(o, 1); }
 int32_t oracle_op_wm_div(oracle *o, double *out) {
   wm_divergence(o);
   if (out) memcpy(out, o->wmp_div, (size_t)o->wmp.n * sizeof(double));
@@ -2508,4 +2781,9 @@ void oracle_op_bound_pr(oracle *o) { bound_pr(o, o->Pr); }
 void oracle_op_bound_scalar(oracle *o, int32_t sc) { bound_scalar(o, o->S[sc]); }
 void oracle_op_bound_visc(oracle *o) { bound_visc(o, o->Visc); }
 void oracle_op_poisson(oracle *o, const double *rhs, double *phi) { poisson_direct(o, rhs, phi); }
+int32_t oracle_op_poisson_mg(oracle *o, const double *rhs, double *phi, double *res) {
+  int cyc = 0;
+  poisson_mg(o, rhs, phi, &cyc, res);
+  return cyc;
+}
 double oracle_op_residual(oracle *o, const double *phi, const double *rhs) { return residue(o, phi, rhs); }

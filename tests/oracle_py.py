@@ -50,6 +50,11 @@ def _load():
             getattr(lib, name).argtypes = [C.c_void_p]
         lib.oracle_op_wm_div.argtypes = [C.c_void_p, pd]
         lib.oracle_op_wm_div.restype = C.c_int32
+        lib.oracle_op_poisson_mg.argtypes = [C.c_void_p, pd, pd, pd]
+        lib.oracle_op_poisson_mg.restype = C.c_int32
+        lib.oracle_op_wm_visc.argtypes = [C.c_void_p]
+        lib.oracle_wm_ustar.argtypes = [C.c_void_p, pd]
+        lib.oracle_wm_ustar.restype = C.c_int32
         lib.oracle_op_scalar_extra.argtypes = [C.c_void_p, C.c_int32]
         lib.oracle_op_bound_scalar.argtypes = [C.c_void_p, C.c_int32]
         _lib = lib
@@ -154,6 +159,16 @@ class Oracle:
         _lib.oracle_op_wm_div(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
+    def op_wm_visc(self):
+        _lib.oracle_op_wm_visc(self._h)
+
+    def wm_ustar(self):
+        """WMPoints%ustar in list order."""
+        n = _lib.oracle_wm_ustar(self._h, None)
+        out = np.zeros(n)
+        _lib.oracle_wm_ustar(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
     def op_scalar_extra(self, sc=0):
         _lib.oracle_op_scalar_extra(self._h, sc)
 
@@ -176,6 +191,16 @@ class Oracle:
         pd = C.POINTER(C.c_double)
         _lib.oracle_op_poisson(self._h, r.ctypes.data_as(pd), out.ctypes.data_as(pd))
         return out
+
+    def op_poisson_mg(self, rhs):
+        """Multigrid solve of the same problem: (phi, V-cycles, final max
+        residual)."""
+        r = np.asfortranarray(rhs, dtype=np.float64)
+        out = np.zeros_like(r, order="F")
+        res = C.c_double()
+        pd = C.POINTER(C.c_double)
+        cyc = _lib.oracle_op_poisson_mg(self._h, r.ctypes.data_as(pd), out.ctypes.data_as(pd), C.byref(res))
+        return out, cyc, res.value
 
     def op_residual(self, phi_ghosted, rhs_ghosted):
         pd = C.POINTER(C.c_double)

@@ -498,3 +498,73 @@ def test_bound_viscosity_rules_brute_force(obstacle_case):
             want[i + 2, j + 2, 2] = want[i + 2, j + 2, 3]
             want[i + 2, j + 2, nz + 3] = want[i + 2, j + 2, nz + 2]
     assert np.array_equal(R, want)
+
+
+# ------------------------------------------------------------ ComputeViscsWM
+def _cell_velocity_and_tangent(o, ijk, dist):
+    """Two-point cell-centred velocity (src/wallmodels.f90:2069-2071) and its
+    part tangential to the wall-distance vector (vel_and_dist, 1382-1391)."""
+    U, V, W = (o.download(c) for c in range(3))
+    out = []
+    for (i, j, k), d in zip(ijk, dist):
+        vel = np.array([(G(U, i, j, k) + G(U, i - 1, j, k)) / 2, (G(V, i, j, k) + G(V, i, j - 1, k)) / 2,
+                        (G(W, i, j, k) + G(W, i, j, k - 1)) / 2])
+        t = vel - np.dot(vel, d) * d / np.dot(d, d)
+        out.append((np.linalg.norm(t), np.linalg.norm(d)))
+    return out
+
+
+def test_wm_visc_rough_points_follow_the_log_law(obstacle_case):
+    """Rough points (z0 > 0) outside the laminar check: u* = 0.41 |u_t| /
+    ln(d/z0) (WMRoughUstar, src/wallmodels.f90:1309-1333); the viscosity
+    ComputeViscsWM stores is u*^2 d / |u_t| when above nu."""
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=31)
+    ijk, dist, z0 = case.wmp_list()
+    assert len(z0) > 0 and (z0 > 0).any() and (z0 == 0).any()
+    o.op_wm_visc()
+    ust = o.wm_ustar()
+    visc = o.download(capi.F_VISC)
+    nu = cfg.nu
+    checked = 0
+    for p, (v, d) in enumerate(_cell_velocity_and_tangent(o, ijk, dist)):
+        if z0[p] <= 0 or v == 0:
+            continue
+        lam = np.sqrt(nu * v / d)
+        if d * lam / nu < 11.225 * 1.1 or d <= 2 * z0[p]:
+            continue
+        want = 0.41 * v / np.log(d / z0[p])
+        assert abs(ust[p] - want) <= 4e-16 * want, (p, ust[p], want)
+        w = want * want * d / v
+        assert abs(G(visc, *ijk[p]) - (w if w > nu else nu)) <= 1e-15 * max(w, nu)
+        checked += 1
+    assert checked > 20
+
+
+def test_wm_visc_flat_points_are_fixed_points_and_keep_state(obstacle_case):
+    """Smooth points (z0 = 0): u* is a fixed point of the wall law to the
+    iteration's tolerance (WMFlatUstar, src/wallmodels.f90:1190-1240), and the
+    state persists: starting from the stored u* the next call moves it by less
+    than the tolerance, where the first call started from 1."""
+    cfg, case = obstacle_case
+    o = fresh(cfg, case, seed=32)
+    ijk, dist, z0 = case.wmp_list()
+    nu = cfg.nu
+    o.op_wm_visc()
+    u1 = o.wm_ustar()
+    o.op_wm_visc()
+    u2 = o.wm_ustar()
+    turb = 0
+    for p, (v, d) in enumerate(_cell_velocity_and_tangent(o, ijk, dist)):
+        if z0[p] > 0 or v == 0:
+            continue
+        lam = np.sqrt(nu * v / d)
+        if d * lam / nu < 5:
+            assert u1[p] == lam and u2[p] == lam
+            continue
+        zp = d * u1[p] / nu
+        law = v / (np.log(zp) / 0.4 + 5.5) if zp >= 30 else (v / (5 * np.log(zp) - 3.05) if zp > 5 else lam)
+        assert abs(law - u1[p]) <= 2e-4 * u1[p], (p, law, u1[p])
+        assert abs(u2[p] - u1[p]) <= 1e-4 * u1[p]
+        turb += 1
+    assert turb > 20
