@@ -335,6 +335,19 @@ struct Engine : EngineIface {
   bool solve64 = false;
   DevBuf X64, twx64, twy64;
   PoissonPlan<double> pp64{};
+  // geometric multigrid (poisson_solver = ELMM_POISSON_MG, k_multigrid.hip):
+  // dense level arrays, line-solve tables, the coarsest level's work arrays
+  bool use_mg = false;
+  std::vector<MgLevel> mg;
+  std::vector<std::unique_ptr<DevBuf>> mg_bufs;
+  DevBuf mg_tab, mg_dz, mg_cs, mg_lam, mg_Y, mg_X, mg_cp, mg_partial;
+  MgCoarse mgc{};
+  int mg_cycles = 0;
+  double mg_res_last = 0;
+  void setup_mg();
+  double mg_resmax0();
+  void mg_vcycle(int l);
+  void mg_solve();
   DevBuf partial, cfl_partial, div_partial;
   int top_blocks = 0, red_blocks = 0, cfl_blocks = 0, cfl_cap = 0;
   int refine_iters = 0;
@@ -588,9 +601,11 @@ void Engine<T>::setup_geo() {
 template <typename T>
 void Engine<T>::setup_poisson() {
   const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
-  if (!fft_len_ok(nx) || !fft_len_ok(ny)) throw ArgError("FFT pressure solver needs nx and ny of the form 2^a 3^b");
-  if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
-  if (nz > 256) throw ArgError("z solver supports nz <= 256");
+  if (!use_mg) {
+    if (!fft_len_ok(nx) || !fft_len_ok(ny)) throw ArgError("FFT pressure solver needs nx and ny of the form 2^a 3^b");
+    if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
+    if (nz > 256) throw ArgError("z solver supports nz <= 256");
+  }
   const int M = nx / 2 + 1;
   // spectral rows padded to 128 bytes, so that the y pass's 8-mode segments
   // are whole cache lines
@@ -611,7 +626,9 @@ void Engine<T>::setup_poisson() {
     return t;
   };
   using V2 = C2<T>;
-  if (!solve64) {
+  if (use_mg) {
+    // no spectral plan: the FFT passes are not used
+  } else if (!solve64) {
     X.alloc((size_t)Mp * ny * nz * sizeof(V2));
     upload_vec(twx, tw(nx, mr_x, (T)1));
     upload_vec(twy, tw(ny, mr_y, (T)1));
@@ -670,6 +687,152 @@ void Engine<T>::setup_poisson() {
     dblocks = delta_blocks(g);
     delta_partial.alloc((size_t)3 * dblocks * sizeof(double));
   }
+  if (use_mg) setup_mg();
+}
+
+// Multigrid levels: x and y halved while both stay multiples of 2 and at
+// least 4 cells, every z level kept; per level the line-solve tables (den_k
+// = D_k - az_k e_{k-1}, e_k = cz_k / den_k with D_k = (((ax + ax) + (ay +
+// ay)) + az_k) + cz_k) and, on the coarsest, the Fourier tables and the
+// mode eigenvalues.  The tables are computed on the host exactly as the
+// oracle computes them.
+template <typename T>
+void Engine<T>::setup_mg() {
+  const int nz = cfg.nz;
+  int nx = cfg.nx, ny = cfg.ny, L = 1;
+  while (nx % 4 == 0 && ny % 4 == 0 && nx / 2 >= 4 && ny / 2 >= 4) {
+    nx /= 2;
+    ny /= 2;
+    L++;
+  }
+  std::vector<double> az(nz), cz(nz);
+  for (int k = 1; k <= nz; k++) {
+    az[k - 1] = k > 1 ? 1 / (Grid1D::at(grid.dzW, k - 1) * Grid1D::at(grid.dzPr, k)) : 0;
+    cz[k - 1] = k < nz ? 1 / (Grid1D::at(grid.dzW, k) * Grid1D::at(grid.dzPr, k)) : 0;
+  }
+  std::vector<double> tab(2 * (size_t)L * nz);
+  mg.assign(L, MgLevel{});
+  mg_bufs.clear();
+  for (int l = 0; l < L; l++) {
+    MgLevel &v = mg[l];
+    v.nx = cfg.nx >> l;
+    v.ny = cfg.ny >> l;
+    v.nz = nz;
+    const double dxl = grid.dxmin * (double)(1 << l), dyl = grid.dymin * (double)(1 << l);
+    v.ax = 1.0 / (dxl * dxl);
+    v.ay = 1.0 / (dyl * dyl);
+    const size_t n = (size_t)v.nx * v.ny * nz;
+    mg_bufs.emplace_back(new DevBuf);
+    mg_bufs.back()->alloc(n * sizeof(double));
+    v.phi = mg_bufs.back()->as<double>();
+    mg_bufs.emplace_back(new DevBuf);
+    mg_bufs.back()->alloc(n * sizeof(double));
+    v.rhs = mg_bufs.back()->as<double>();
+    double *den = &tab[(size_t)(2 * l) * nz], *e = &tab[(size_t)(2 * l + 1) * nz];
+    double eprev = 0;
+    for (int k = 1; k <= nz; k++) {
+      const double D = (((v.ax + v.ax) + (v.ay + v.ay)) + az[k - 1]) + cz[k - 1];
+      den[k - 1] = D - az[k - 1] * eprev;
+      e[k - 1] = cz[k - 1] / den[k - 1];
+      eprev = e[k - 1];
+    }
+  }
+  upload_vec(mg_tab, tab);
+  std::vector<double> dz(2 * (size_t)(nz + 1), 0.0);
+  for (int k = 1; k <= nz; k++) {
+    dz[k] = Grid1D::at(grid.dzPr, k);
+    dz[nz + 1 + k] = Grid1D::at(grid.dzW, k);
+  }
+  upload_vec(mg_dz, dz);
+  const MgLevel &c = mg[L - 1];
+  const long plane = (long)c.nx * c.ny;
+  std::vector<double> cs(2 * (size_t)(c.nx + c.ny));
+  auto twid = [](int n, double *cv, double *sv) {
+    for (int q = 0; q < n; q++) {
+      long double t = 2.0L * 3.14159265358979323846264338327950288L * q / n;
+      cv[q] = (double)cosl(t);
+      sv[q] = -(double)sinl(t);
+    }
+  };
+  twid(c.nx, &cs[0], &cs[c.nx]);
+  twid(c.ny, &cs[2 * c.nx], &cs[2 * c.nx + c.ny]);
+  upload_vec(mg_cs, cs);
+  std::vector<double> lam(plane);
+  for (int n = 0; n < c.ny; n++)
+    for (int m = 0; m < c.nx; m++) {
+      const double sxm = std::sin(M_PI * m / c.nx), syn = std::sin(M_PI * n / c.ny);
+      lam[m + (long)n * c.nx] = -4 * sxm * sxm * c.ax - 4 * syn * syn * c.ay;
+    }
+  upload_vec(mg_lam, lam);
+  mg_Y.alloc((size_t)plane * nz * 2 * sizeof(double));
+  mg_X.alloc((size_t)plane * nz * 2 * sizeof(double));
+  mg_cp.alloc((size_t)plane * nz * sizeof(double));
+  mgc.Y = mg_Y.as<double2>();
+  mgc.X = mg_X.as<double2>();
+  mgc.cp = mg_cp.as<double>();
+  mgc.cx = mg_cs.as<double>();
+  mgc.sx = mg_cs.as<double>() + c.nx;
+  mgc.cy = mg_cs.as<double>() + 2 * c.nx;
+  mgc.sy = mg_cs.as<double>() + 2 * c.nx + c.ny;
+  mgc.lam = mg_lam.as<double>();
+  mg_partial.alloc(((size_t)mg_resmax_blocks(mg[0]) + 1) * sizeof(double));
+}
+
+// Max-norm residual of level 0 (synchronises the stream).
+template <typename T>
+double Engine<T>::mg_resmax0() {
+  const int nb = mg_resmax_blocks(mg[0]);
+  double *part = mg_partial.as<double>();
+  launch_mg_resmax(mg[0], az64.as<double>(), cz64.as<double>(), part, stream);
+  launch_max_final(part, nb, part + nb, stream);
+  double r = 0;
+  HIPCHK(hipMemcpyAsync(&r, part + nb, sizeof r, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return r;
+}
+
+// V-cycle from level l: 2 pre-smoothing sweeps (both colours), residual
+// restricted to level l + 1, its cycle, prolongation, 2 post-smoothing
+// sweeps; the coarsest level is solved exactly.
+template <typename T>
+void Engine<T>::mg_vcycle(int l) {
+  const int L = (int)mg.size(), nz = cfg.nz;
+  const double *az = az64.as<double>(), *cz = cz64.as<double>();
+  if (l == L - 1) {
+    const double *dz = mg_dz.as<double>();
+    launch_mg_coarse(mg[l], mgc, az, cz, dz, dz + nz + 1, stream);
+    return;
+  }
+  const double *den = mg_tab.as<double>() + (size_t)(2 * l) * nz, *e = den + nz;
+  auto smooth = [&] {
+    for (int s = 0; s < 2; s++) {
+      launch_mg_sweep(mg[l], az, den, e, 0, stream);
+      launch_mg_sweep(mg[l], az, den, e, 1, stream);
+    }
+  };
+  smooth();
+  launch_mg_restrict(mg[l], mg[l + 1], az, cz, stream);
+  mg_vcycle(l + 1);
+  launch_mg_prolong(mg[l + 1], mg[l], stream);
+  smooth();
+}
+
+// PoissMG's loop (src/multigrid.f90:1161-1168): from P = 0, V-cycles until
+// the max-norm residual is below poisson_tol or mg_cycles_max cycles ran.
+template <typename T>
+void Engine<T>::mg_solve() {
+  launch_mg_zero(mg[0].phi, (long)cfg.nx * cfg.ny * cfg.nz, stream);
+  const int maxc = cfg.mg_cycles_max > 0 ? cfg.mg_cycles_max : 50;
+  const double tol = cfg.poisson_tol > 0 ? cfg.poisson_tol : 1e-10;
+  int cyc = 0;
+  double R = mg_resmax0();
+  while (cyc < maxc && !(R < tol)) {
+    mg_vcycle(0);
+    cyc++;
+    R = mg_resmax0();
+  }
+  mg_cycles = cyc;
+  mg_res_last = R;
 }
 
 template <typename T>
@@ -1006,7 +1169,13 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   // refinement sweeps; 0 -> one FP64 spectral solve
   refine_iters = want == ELMM_FP32_REFINED ? (cfg.refine_iters > 0 ? cfg.refine_iters : 0) : 0;
   solve64 = want == ELMM_FP32_REFINED && refine_iters == 0;
-  if (cfg.poisson_solver != ELMM_POISSON_FFT) throw ArgError("poisson_solver must be ELMM_POISSON_FFT");
+  if (cfg.poisson_solver != ELMM_POISSON_FFT && cfg.poisson_solver != ELMM_POISSON_MG)
+    throw ArgError("poisson_solver must be ELMM_POISSON_FFT or ELMM_POISSON_MG");
+  use_mg = cfg.poisson_solver == ELMM_POISSON_MG;
+  if (use_mg) {
+    if (cfg.nx % 2 || cfg.ny % 2) throw ArgError("the multigrid solver needs even nx and ny");
+    if (refine_iters > 0) throw ArgError("the multigrid solver runs in FP64: refine_iters must be 0");
+  }
   for (int s = 4; s < 6; s++)
     if (cfg.nscalars > 0 && cfg.scal_btype[s] == ELMM_BC_PERIODIC) throw ArgError("periodic scalar z boundaries");
   if (cfg.stretched_z) {
@@ -1160,6 +1329,28 @@ void Engine<T>::pressure_correction(T *a, T *b, T *c, bool use_q, double coef, b
   // first solve; the precision form keeps the FP64 right-hand side for the
   // refinement sweeps or for the residual diagnostic
   double *rhs_keep = refine_iters > 0 || (solve64 && diag) ? RHS.as<double>() : nullptr;
+  if (use_mg) {
+    timed(ELMM_T_POISSON_X_FWD, [&] {
+      launch_mg_rhs<T>(g, a, b, c, q, dt2_rec, mg[0].rhs, RHS.as<double>(), stream);
+    });
+    timed(ELMM_T_POISSON_Z, [&] { mg_solve(); });
+    timed(ELMM_T_POISSON_X_INV, [&] { launch_mg_to_phi(g, mg[0].phi, Phi.as<double>(), stream); });
+    if (diag) {
+      launch_refine_residual(g, Phi.as<double>(), RHS.as<double>(), az64.as<double>(), cz64.as<double>(), nullptr,
+                             div_partial.as<double>(), red_blocks, stream);
+      last_res = reduce_max(div_partial, red_blocks);
+      last_div = dt2 * last_res;
+      last_refinements = mg_cycles;
+    }
+    timed(ELMM_T_REDUCE, [&] {
+      launch_top_sum(g, Pr.as<T>(), Phi.as<double>(), partial.as<double>(), top_blocks, stream);
+    });
+    timed(ELMM_T_PROJECTION, [&] {
+      launch_projection(g, a, b, c, Pr.as<T>(), Phi.as<double>(), partial.as<double>(), top_blocks, dt2, stream);
+    });
+    timed(ELMM_T_GHOST, [&] { launch_bound_pr<T>(g, Pr.as<T>(), stream); });
+    return;
+  }
   timed(ELMM_T_POISSON_X_FWD, [&] {
     if (solve64)
       launch_fft_x_fwd(g, pp64, a, b, c, q, qm, nullptr, nullptr, dt2_rec, rhs_keep, stream);
@@ -1375,7 +1566,8 @@ void Engine<T>::first_call() {
 template <typename T>
 int Engine<T>::replay_steps(int nsteps) {
   // a variable dt changes the stage constants every step: no replay
-  if (!use_graphs || timers_on || nsteps < 2 || !first_step_done || cfg.variable_dt) return 0;
+  // nor for the multigrid solve, whose cycle count depends on the data
+  if (!use_graphs || timers_on || nsteps < 2 || !first_step_done || cfg.variable_dt || use_mg) return 0;
   const int par = u == U.as<T>() ? 0 : 1;
   if (!graph_exec[par]) {
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));

@@ -90,6 +90,33 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
 template <typename T>
 void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,
                       hipStream_t st);
+// Multigrid level (k_multigrid.hip): dense nx x ny x nz arrays, x fastest.
+struct MgLevel {
+  int nx, ny, nz;
+  double ax, ay;  // 1/dx^2, 1/dy^2 of the level
+  double *phi, *rhs;
+};
+// Work arrays and host tables of the exact coarsest-level solve.
+struct MgCoarse {
+  double2 *Y, *X;      // nx ny nz complex each
+  double *cp;          // nx ny nz
+  const double *cx, *sx, *cy, *sy;  // cos, -sin of 2 pi q / N
+  const double *lam;   // -4 sin^2(pi m/nx) ax - 4 sin^2(pi n/ny) ay per mode
+};
+template <typename T>
+void launch_mg_rhs(const Geo &g, const T *U, const T *V, const T *W, const double *Q, double dt2_rec, double *rhs0,
+                   double *RHS, hipStream_t st);
+void launch_mg_to_phi(const Geo &g, const double *phi0, double *Phi, hipStream_t st);
+void launch_mg_zero(double *a, long n, hipStream_t st);
+void launch_mg_sweep(const MgLevel &L, const double *az, const double *den, const double *e, int color,
+                     hipStream_t st);
+int mg_resmax_blocks(const MgLevel &L);
+void launch_mg_resmax(const MgLevel &L, const double *az, const double *cz, double *partial, hipStream_t st);
+void launch_mg_restrict(const MgLevel &F, const MgLevel &Cs, const double *az, const double *cz, hipStream_t st);
+void launch_mg_prolong(const MgLevel &Cs, const MgLevel &F, hipStream_t st);
+void launch_mg_coarse(const MgLevel &L, const MgCoarse &C, const double *az, const double *cz, const double *dzPr,
+                      const double *dzW, hipStream_t st);
+
 template <typename T>
 void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, WMUstarPoints wp, double nu, T *Visc,
                      hipStream_t st);

@@ -22,7 +22,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # -ffp-contract=off keeps every multiply and add separately rounded (the
-# reference's arithmetic); packed FP32 instructions are disabled.
+# reference's arithmetic; *_f32.hip files, FP32 kernels of the precision form,
+# use -ffp-contract=fast); packed FP32 instructions are disabled.
 HIP_FLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -51,7 +52,12 @@ def build(verbose=False, jobs=None):
         src = os.path.join(CSRC, f)
         obj = os.path.join(OBJDIR, f + ".o")
         if f.endswith(".hip"):
-            jobs_list.append([HIPCC] + HIP_FLAGS + inc + ["-c", src, "-o", obj])
+            flags = HIP_FLAGS
+            if f.endswith("_f32.hip"):
+                # FP32 instantiations of the precision form: multiply-add
+                # pairs may fuse (the FP64 form keeps every operation rounded)
+                flags = [x if x != "-ffp-contract=off" else "-ffp-contract=fast" for x in HIP_FLAGS]
+            jobs_list.append([HIPCC] + flags + inc + ["-c", src, "-o", obj])
         elif f.endswith(".cpp"):
             jobs_list.append(["g++"] + CXX_FLAGS + inc + ["-c", src, "-o", obj])
     n = jobs or min(8, len(jobs_list))

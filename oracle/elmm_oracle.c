@@ -2030,8 +2030,9 @@ static void kahan(double *s, double *cmp, double t) {
        0.25 (0.75 c_y + 0.25 c_xy), c_x / c_y the coarse neighbour on the
        child's side;
      cycle: 2 pre-smoothing sweeps, coarse correction, 2 post-smoothing
-       sweeps; the coarsest level (x or y below 4 cells after halving) takes
-       40 sweeps; cycles from P = 0 (PoissMG zeroes Phi) until the max-norm
+       sweeps; the coarsest level (halving stops where x or y would drop
+       below 4 cells or become odd) is solved exactly (mg_coarse); cycles
+       from P = 0 (PoissMG zeroes Phi) until the max-norm
        residual is below poisson_tol (the signed-max test of MG_res,
        src/multigrid.f90:975, taken on |r|) or mg_cycles_max cycles. */
 typedef struct {
@@ -2040,9 +2041,6 @@ typedef struct {
   double *phi, *rhs, *den, *e;
 } mg_level;
 
-#ifndef MG_COARSE_SWEEPS
-#define MG_COARSE_SWEEPS 40
-#endif
 static int mg_levels(int nx, int ny) {
   int L = 1;
   while (nx % 4 == 0 && ny % 4 == 0 && nx / 2 >= 4 && ny / 2 >= 4) {
@@ -2138,12 +2136,143 @@ static void mg_prolong(const mg_level *c, mg_level *f, int nz) {
       }
 }
 
-static void mg_vcycle(mg_level *lv, int l, int L, int nz, const double *az, const double *cz) {
-  if (l == L - 1) {
-    for (int s = 0; s < MG_COARSE_SWEEPS; s++) {
-      mg_sweep(&lv[l], nz, az, cz, 0);
-      mg_sweep(&lv[l], nz, az, cz, 1);
+/* Coarsest level: exact solve of L P = rhs (P = 0 on entry) by discrete
+   Fourier transforms in x and y, evaluated by direct summation (a few
+   columns), and a tridiagonal solve in z per mode (the direct solver's
+   operator on the coarse grid):
+     Y(m, j, k) = sum_i rhs(i, j, k) w_x(m (i-1) mod nx), i increasing,
+     X(m, n, k) = sum_j Y(m, j, k) w_y(n (j-1) mod ny), j increasing,
+   with w(q) = (cos 2pi q/N, -sin 2pi q/N) and complex products (a.re b.re -
+   a.im b.im, a.re b.im + a.im b.re) accumulated as s = s + p;
+     mode (m, n) != 0: Thomas elimination of a_k x_{k-1} + (lam - a_k - c_k)
+       x_k + c_k x_{k+1} = X_k, lam = -4 sin^2(pi m/nx)/dx^2 - 4 sin^2(pi
+       n/ny)/dy^2 (den_k = b_k - a_k cp_{k-1}, cp_k = c_k / den_k, dp_k = (X_k
+       - a_k dp_{k-1}) / den_k, x_k = dp_k - cp_k x_{k+1});
+     mode (0, 0): G_k = G_{k-1} + X_k dzPr(k) from G_0 = 0, x_{k+1} = x_k +
+       G_k dzW(k) from x_1 = 0 (the singular Neumann problem);
+   then the conjugate sums over n, then m, real part times 1/(nx ny). */
+typedef struct {
+  double re, im;
+} mgc;
+static mgc mgc_mul(mgc a, mgc b) {
+  mgc r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return r;
+}
+void oracle_mg_twiddles(int n, double *cs, double *sn) {
+  for (int q = 0; q < n; q++) {
+    long double t = 2.0L * 3.14159265358979323846264338327950288L * q / n;
+    cs[q] = (double)cosl(t);
+    sn[q] = -(double)sinl(t);
+  }
+}
+static void mg_coarse(mg_level *v, int nz, const double *az, const double *cz, const double *dzPr,
+                      const double *dzW) {
+  const int nx = v->nx, ny = v->ny;
+  const long plane = (long)nx * ny;
+  double *cx = xcalloc(nx, sizeof(double)), *sx = xcalloc(nx, sizeof(double));
+  double *cy = xcalloc(ny, sizeof(double)), *sy = xcalloc(ny, sizeof(double));
+  oracle_mg_twiddles(nx, cx, sx);
+  oracle_mg_twiddles(ny, cy, sy);
+  mgc *Y = xcalloc(plane * nz, sizeof(mgc)), *X = xcalloc(plane * nz, sizeof(mgc));
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int m = 0; m < nx; m++) {
+        mgc acc = {0, 0};
+        for (int i = 0; i < nx; i++) {
+          const mgc w = {cx[(m * i) % nx], sx[(m * i) % nx]};
+          const mgc r = {v->rhs[i + (long)j * nx + (long)k * plane], 0};
+          const mgc p = mgc_mul(r, w);
+          acc.re = acc.re + p.re;
+          acc.im = acc.im + p.im;
+        }
+        Y[m + (long)j * nx + (long)k * plane] = acc;
+      }
+  for (int k = 0; k < nz; k++)
+    for (int n = 0; n < ny; n++)
+      for (int m = 0; m < nx; m++) {
+        mgc acc = {0, 0};
+        for (int j = 0; j < ny; j++) {
+          const mgc w = {cy[(n * j) % ny], sy[(n * j) % ny]};
+          const mgc p = mgc_mul(Y[m + (long)j * nx + (long)k * plane], w);
+          acc.re = acc.re + p.re;
+          acc.im = acc.im + p.im;
+        }
+        X[m + (long)n * nx + (long)k * plane] = acc;
+      }
+  double *cp = xcalloc(nz, sizeof(double));
+  mgc *dp = xcalloc(nz, sizeof(mgc));
+  for (int n = 0; n < ny; n++)
+    for (int m = 0; m < nx; m++) {
+      mgc *col = X + m + (long)n * nx;
+      if (m == 0 && n == 0) {
+        double Gr = 0, Gi = 0, xr = 0, xi = 0;
+        for (int k = 1; k <= nz; k++) {
+          const mgc d = col[(long)(k - 1) * plane];
+          col[(long)(k - 1) * plane].re = xr;
+          col[(long)(k - 1) * plane].im = xi;
+          if (k < nz) {
+            Gr = Gr + d.re * dzPr[k];
+            Gi = Gi + d.im * dzPr[k];
+            xr = xr + Gr * dzW[k];
+            xi = xi + Gi * dzW[k];
+          }
+        }
+        continue;
+      }
+      const double sxm = sin(M_PI * m / nx), syn = sin(M_PI * n / ny);
+      const double lam = -4 * sxm * sxm * v->ax - 4 * syn * syn * v->ay;
+      double cprev = 0;
+      mgc dprev = {0, 0};
+      for (int k = 0; k < nz; k++) {
+        const double b = (lam - az[k]) - cz[k];
+        const double den = b - az[k] * cprev;
+        cp[k] = cz[k] / den;
+        const mgc d = col[(long)k * plane];
+        dp[k].re = (d.re - az[k] * dprev.re) / den;
+        dp[k].im = (d.im - az[k] * dprev.im) / den;
+        cprev = cp[k];
+        dprev = dp[k];
+      }
+      mgc x = dp[nz - 1];
+      col[(long)(nz - 1) * plane] = x;
+      for (int k = nz - 2; k >= 0; k--) {
+        x.re = dp[k].re - cp[k] * x.re;
+        x.im = dp[k].im - cp[k] * x.im;
+        col[(long)k * plane] = x;
+      }
     }
+  /* inverse: conjugate twiddles over n, then over m */
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int m = 0; m < nx; m++) {
+        mgc acc = {0, 0};
+        for (int n = 0; n < ny; n++) {
+          const mgc w = {cy[(n * j) % ny], -sy[(n * j) % ny]};
+          const mgc p = mgc_mul(X[m + (long)n * nx + (long)k * plane], w);
+          acc.re = acc.re + p.re;
+          acc.im = acc.im + p.im;
+        }
+        Y[m + (long)j * nx + (long)k * plane] = acc;
+      }
+  const double scale = 1.0 / (double)plane;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double acc = 0;
+        for (int m = 0; m < nx; m++) {
+          const mgc w = {cx[(m * i) % nx], -sx[(m * i) % nx]};
+          const mgc y = Y[m + (long)j * nx + (long)k * plane];
+          acc = acc + (y.re * w.re - y.im * w.im);
+        }
+        v->phi[i + (long)j * nx + (long)k * plane] = acc * scale;
+      }
+  free(cx); free(sx); free(cy); free(sy); free(Y); free(X); free(cp); free(dp);
+}
+
+static void mg_vcycle(mg_level *lv, int l, int L, int nz, const double *az, const double *cz, const double *dzPr,
+                      const double *dzW) {
+  if (l == L - 1) {
+    mg_coarse(&lv[l], nz, az, cz, dzPr, dzW);
     return;
   }
   for (int s = 0; s < 2; s++) {
@@ -2151,7 +2280,7 @@ static void mg_vcycle(mg_level *lv, int l, int L, int nz, const double *az, cons
     mg_sweep(&lv[l], nz, az, cz, 1);
   }
   mg_restrict(&lv[l], &lv[l + 1], nz, az, cz);
-  mg_vcycle(lv, l + 1, L, nz, az, cz);
+  mg_vcycle(lv, l + 1, L, nz, az, cz, dzPr, dzW);
   mg_prolong(&lv[l + 1], &lv[l], nz);
   for (int s = 0; s < 2; s++) {
     mg_sweep(&lv[l], nz, az, cz, 0);
@@ -2200,8 +2329,13 @@ static void poisson_mg(oracle *o, const double *rhs, double *phi, int *cycles_ou
   const double tol = o->c.poisson_tol > 0 ? o->c.poisson_tol : 1e-10;
   int cyc = 0;
   double R = mg_residual_max(&lv[0], nz, az, cz);
+  double *dzPr = xcalloc(nz + 1, sizeof(double)), *dzW = xcalloc(nz + 1, sizeof(double));
+  for (int k = 1; k <= nz; k++) {
+    dzPr[k] = C1D(o->dzPr, k);
+    dzW[k] = C1D(o->dzW, k);
+  }
   while (cyc < maxc && !(R < tol)) {
-    mg_vcycle(lv, 0, L, nz, az, cz);
+    mg_vcycle(lv, 0, L, nz, az, cz, dzPr, dzW);
     cyc++;
     R = mg_residual_max(&lv[0], nz, az, cz);
   }
@@ -2218,6 +2352,8 @@ static void poisson_mg(oracle *o, const double *rhs, double *phi, int *cycles_ou
   free(lv);
   free(az);
   free(cz);
+  free(dzPr);
+  free(dzW);
 }
 
 /* Direct solve; phi and rhs are interior arrays nx*ny*nz, x fastest.
@@ -2683,6 +2819,7 @@ int oracle_step(oracle *o, int32_t nsteps, elmm_diag *diag) {
     diag->cfl = cfl;
     diag->poisson_res = o->last_res;
     diag->max_div = o->last_div;
+    diag->refinements = o->c.poisson_solver == 3 ? o->last_mg_cycles : 0;
     diag->steps_done = done;
     diag->delta = o->delta_last;
     diag->dt = o->dt_last;
