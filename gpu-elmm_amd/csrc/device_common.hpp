@@ -100,6 +100,9 @@ struct StageConst {
   double coef;        // 2*RK_alpha(stage)*dt
   double A, Ax, Ay, Az;     // -coef, -coef/dxmin, ...
   double prgx, prgy;  // uniform pressure gradient (0 = off)
+  // buoyancy (BuoyancyForce, 2nd order): W += bA2 (T(k+1) + T(k)) - bATref
+  int buoy;
+  double bA2, bATref;
 };
 
 // Ordered per-cell corrections (built on the host in the reference's scatter

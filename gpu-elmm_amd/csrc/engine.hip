@@ -291,7 +291,11 @@ struct Engine : EngineIface {
   std::vector<double> zw_copy;
   Geo g{};
   Grid1D grid;
-  int nsc = 0;
+  int nsc = 0;  // scalar slots: the passive scalars, then the temperature (buoyancy)
+  int tslot = -1;  // slot of the temperature, -1 without buoyancy
+  Geo gtemp{};     // g with the temperature's boundary conditions
+  DevBuf tfix_ijk;  // solid cells whose temperature is the mean of fluid neighbours
+  int tfix_n = 0;
   // fields
   DevBuf U, V, W, U2, V2, W2, Us, Vs, Ws, Pr, Visc, TDiff, Q, Phi, RHS;
   std::vector<std::unique_ptr<DevBuf>> S, Snew, Sadv;
@@ -991,6 +995,25 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   }
   std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
   upload_vec(src_flux, sflux);
+  if (tslot >= 0) {
+    // solid cells of -1..n+2 with a fluid neighbour there: their temperature
+    // is the mean of those neighbours after each stage
+    const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+    const long hy = nx + 6, hz = (long)(nx + 6) * (ny + 6);
+    auto pt = [&](int i, int j, int k) { return L.prtype[(i + 2) + (j + 2) * hy + (k + 2) * hz]; };
+    std::vector<int32_t> fix;
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int i = -1; i <= nx + 2; i++) {
+          if (pt(i, j, k) <= 0) continue;
+          const bool any = (i > -1 && pt(i - 1, j, k) <= 0) || (i < nx + 2 && pt(i + 1, j, k) <= 0) ||
+                           (j > -1 && pt(i, j - 1, k) <= 0) || (j < ny + 2 && pt(i, j + 1, k) <= 0) ||
+                           (k > -1 && pt(i, j, k - 1) <= 0) || (k < nz + 2 && pt(i, j, k + 1) <= 0);
+          if (any) fix.insert(fix.end(), {i, j, k});
+        }
+    tfix_n = (int)(fix.size() / 3);
+    upload_vec(tfix_ijk, fix);
+  }
   for (int s = 0; s < nsc; s++) sc_terms.emplace_back(new DevTerms);
   if (nsc > 0) {
     std::vector<TermBuilder> tbs;
@@ -1182,7 +1205,8 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     zw_copy.assign(cfg.zw, cfg.zw + cfg.nz + 1);
     cfg.zw = zw_copy.data();
   }
-  nsc = cfg.nscalars;
+  nsc = cfg.nscalars + (cfg.enable_buoyancy ? 1 : 0);
+  tslot = cfg.enable_buoyancy ? cfg.nscalars : -1;
   dt_cur = cfg.variable_dt ? cfg.dt_max : cfg.dt;
   time = cfg.start_time;
   SetupTrace tr;
@@ -1190,6 +1214,24 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   tr.mark("stream");
   setup_geo();
+  if (cfg.enable_buoyancy) {
+    // temperature: periodic x/y, Dirichlet or Neumann walls in z; the fused
+    // momentum kernel carries the buoyancy force
+    if (g.variable_z) throw ArgError("buoyancy needs a uniform grid (scalars on stretched z are not built)");
+    if (!fused_momentum) throw ArgError("buoyancy needs the fused momentum kernel (not ELMM_MOMENTUM=percomp)");
+    if (!(cfg.temperature_ref > 0)) throw ArgError("temperature_ref must be positive");
+    if (!(cfg.nu > 0)) throw ArgError("buoyancy needs a positive molecular viscosity");
+    for (int s = 0; s < 4; s++)
+      if (cfg.temp_btype[s] != ELMM_BC_PERIODIC) throw ArgError("the temperature must be periodic in x and y");
+    for (int s = 4; s < 6; s++)
+      if (cfg.temp_btype[s] != ELMM_BC_DIRICHLET && cfg.temp_btype[s] != ELMM_BC_NEUMANN)
+        throw ArgError("temperature z boundaries must be Dirichlet or Neumann");
+    gtemp = g;
+    for (int s = 0; s < 6; s++) {
+      gtemp.sbt[s] = cfg.temp_btype[s];
+      gtemp.side_scal[s] = cfg.side_temp[s];
+    }
+  }
   tr.mark("geometry");
   const size_t fb = g.ntot * sizeof(T), fb64 = g.ntot * sizeof(double);
   for (DevBuf *b : {&U, &V, &W, &U2, &V2, &W2, &Us, &Vs, &Ws, &Pr, &Visc, &TDiff}) b->alloc(fb);
@@ -1232,7 +1274,8 @@ void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool
     case ELMM_F_PHI: d = Phi.p; is64 = true; break;
     case ELMM_F_Q: d = Q.p; is64 = true; break;
     default:
-      if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + nsc) d = s_cur[field - ELMM_F_SCALAR];
+      if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + cfg.nscalars) d = s_cur[field - ELMM_F_SCALAR];
+      if (field == ELMM_F_TEMPERATURE && tslot >= 0) d = s_cur[tslot];
   }
   if (!d) throw ArgError("unknown field id " + std::to_string(field));
   if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) throw ArgError("extents must be positive");
@@ -1272,8 +1315,10 @@ void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool
     HIPCHK(hipMemcpy(d, src, g.ntot * es, hipMemcpyHostToDevice));
     // scalar buffers rotate: both copies start from the uploaded state so
     // ghost cells that no fill touches keep the uploaded value
-    if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + nsc)
+    if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + cfg.nscalars)
       HIPCHK(hipMemcpy(s_nxt[field - ELMM_F_SCALAR], src, g.ntot * es, hipMemcpyHostToDevice));
+    if (field == ELMM_F_TEMPERATURE && tslot >= 0)
+      HIPCHK(hipMemcpy(s_nxt[tslot], src, g.ntot * es, hipMemcpyHostToDevice));
   }
 }
 
@@ -1411,6 +1456,13 @@ void Engine<T>::subgrid_stresses(bool cfl) {
     timed(ELMM_T_IB, [&] {
       launch_ib_visc_serial(g, ibdev(3), Visc.as<T>(), visc_levels.data(), (int)visc_levels.size() - 1, stream);
     });
+  // ComputeTDiff (src/scalar_diffusion.f90:983-1009) with buoyancy, then
+  // BoundViscosity(TDiff); without it TDiff keeps its initial values
+  if (cfg.enable_buoyancy)
+    timed(ELMM_T_OTHER, [&] {
+      launch_tdiff_init(g, Visc.as<T>(), TDiff.as<T>(), cfg.nu, cfg.nu_scalar, stream);
+      launch_bound_visc<T>(g, TDiff.as<T>(), stream);
+    });
 }
 
 template <typename T>
@@ -1430,6 +1482,13 @@ StageConst Engine<T>::stage_const(int st) const {
   sc.Az = -sc.coef / g.dzmin;
   sc.prgx = cfg.pr_gradient_x;
   sc.prgy = cfg.pr_gradient_y;
+  if (cfg.enable_buoyancy) {
+    // A = grav_acc / temperature_ref, A2 = A / 2, and A temperature_ref
+    const double A = cfg.grav_acc / cfg.temperature_ref;
+    sc.buoy = 1;
+    sc.bA2 = A / 2.0;
+    sc.bATref = A * cfg.temperature_ref;
+  }
   return sc;
 }
 
@@ -1444,7 +1503,8 @@ void Engine<T>::stage(int st, bool last) {
   if (fused_momentum) {
     const uint8_t *codes[3] = {code[0].as<uint8_t>(), code[1].as<uint8_t>(), code[2].as<uint8_t>()};
     timed(ELMM_T_MOMENTUM, [&] {
-      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, codes, sc, zchunk, mom_tile,
+      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), tslot >= 0 ? s_cur[tslot] : nullptr, ustar, U2p,
+                            codes, sc, zchunk, mom_tile,
                                  stream);
     });
   }
@@ -1488,7 +1548,11 @@ void Engine<T>::stage(int st, bool last) {
                              mask[2].as<uint8_t>(), sc, sc_terms[s]->view(), src_flux.as<double>(),
                              wmp_div.as<double>(), stream);
         });
-      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<T>(g, s_nxt[s], stream); });
+      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<T>(s == tslot ? gtemp : g, s_nxt[s], stream); });
+      if (s == tslot && tfix_n)
+        timed(ELMM_T_SCALAR_CORR, [&] {
+          launch_temp_solid_fix<T>(g, tfix_ijk.as<int32_t>(), tfix_n, prtype.as<int32_t>(), s_nxt[s], stream);
+        });
       std::swap(s_cur[s], s_nxt[s]);
     }
   }
@@ -1539,7 +1603,7 @@ void Engine<T>::init() {
     launch_tdiff_init(g, Visc.as<T>(), TDiff.as<T>(), cfg.nu, cfg.nu_scalar, stream);
     launch_bound_visc<T>(g, TDiff.as<T>(), stream);
   }
-  for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(g, s_cur[s], stream);
+  for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(s == tslot ? gtemp : g, s_cur[s], stream);
   if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, wmpdev(), cfg.nu, Visc.as<T>(), stream);
   if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
@@ -1671,7 +1735,8 @@ void Engine<T>::autotune() {
       for (int t : tiles)
         for (int ch : chunks) {
           const float ms = time_of([&] {
-            launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), ustar, U2p, codes, sc0, ch, t, stream);
+            launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), tslot >= 0 ? s_cur[tslot] : nullptr, ustar,
+                                  U2p, codes, sc0, ch, t, stream);
           });
           if (ms < best) {
             best = ms;
@@ -1810,7 +1875,7 @@ double Engine<T>::divergence_into(double *partials, long partial_slots, double *
 template <typename T>
 void Engine<T>::frame_async(const std::string &dir) {
   if (frame_thread.joinable()) frame_wait();
-  const int nf = 4 + nsc;
+  const int nf = 4 + cfg.nscalars;  // U, V, W, Pr and the passive scalars
   if (frame_host.empty()) {
     for (int f = 0; f < nf; f++) {
       void *p = nullptr;
@@ -1820,7 +1885,7 @@ void Engine<T>::frame_async(const std::string &dir) {
     HIPCHK(hipEventCreateWithFlags(&frame_done, hipEventDisableTiming));
   }
   const void *src[12] = {u, v, w, Pr.p};
-  for (int s = 0; s < nsc; s++) src[4 + s] = s_cur[s];
+  for (int s = 0; s < cfg.nscalars; s++) src[4 + s] = s_cur[s];
   for (int f = 0; f < nf; f++)
     HIPCHK(hipMemcpyAsync(frame_host[f], src[f], (size_t)g.ntot * sizeof(T), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipEventRecord(frame_done, stream));
@@ -1843,9 +1908,9 @@ void Engine<T>::frame_async(const std::string &dir) {
             host[f][(i + 2) + (j + 2) * hy + (k + 2) * hz] = (double)d[gix(g, i, j, k)];
     }
     std::vector<const double *> sp;
-    for (int s = 0; s < nsc; s++) sp.push_back(host[4 + s].data());
+    for (int s = 0; s < cfg.nscalars; s++) sp.push_back(host[4 + s].data());
     const int rc = elmm_vtk_write(dir.c_str(), &cfg, host[0].data(), host[1].data(), host[2].data(), host[3].data(),
-                                  nullptr, nsc, sp.empty() ? nullptr : sp.data());
+                                  nullptr, cfg.nscalars, sp.empty() ? nullptr : sp.data());
     if (rc) {
       frame_status = rc;
       frame_error = elmm_last_error();

@@ -105,12 +105,18 @@ __device__ inline double cfl_cell(const Geo &g, double u1, double u0, double v1,
   double a1 = fabs(u1), a0 = fabs(u0);
   double b1 = fabs(v1), b0 = fabs(v0);
   double c1 = fabs(w1), c0 = fabs(w0);
-  double dxk = g.variable_z ? g.dxPr[i] : g.dxmin;
-  double dyk = g.variable_z ? g.dyPr[j] : g.dymin;
-  double dzk = g.variable_z ? g.dzPr[k] : g.dzmin;
-  double q = (a1 > a0 ? a1 : a0) / dxk;
-  q = q + (b1 > b0 ? b1 : b0) / dyk;
-  q = q + (c1 > c0 ? c1 : c0) / dzk;
+  double q;
+  if (g.variable_z) {
+    q = (a1 > a0 ? a1 : a0) / g.dxPr[i];
+    q = q + (b1 > b0 ? b1 : b0) / g.dyPr[j];
+    q = q + (c1 > c0 ? c1 : c0) / g.dzPr[k];
+  } else {
+    // uniform spacing: correctly rounded reciprocal products (the same
+    // quotients as the divisions) where the host's reciprocals qualify
+    q = div_rn(a1 > a0 ? a1 : a0, g.dxmin, g.rdx);
+    q = q + div_rn(b1 > b0 ? b1 : b0, g.dymin, g.rdy);
+    q = q + div_rn(c1 > c0 ? c1 : c0, g.dzmin, g.rdz);
+  }
   return (q != q) ? INFINITY : q;
 }
 

@@ -52,6 +52,7 @@ __device__ inline T m4(T a, T b, T c, T d) {
 template <typename T>
 struct MomZArgs {
   const T *U, *V, *W, *Visc, *Pr;
+  const T *Temp;  // temperature (buoyancy), stage start
   T *Us[3];
   T *U2[3];
   const uint8_t *code[3];
@@ -214,6 +215,12 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   // one level ahead like the planes
   T cpr = a.Pr[pc + (long)kbeg * sz], cprx = a.Pr[pc + (long)kbeg * sz + 1],
     cpry = a.Pr[pc + (long)kbeg * sz + g.sy], cprz = a.Pr[pc + (long)(kbeg + 1) * sz];
+  // temperature at k and k+1 of the column (buoyancy only)
+  T ct0 = 0, ct1 = 0;
+  if (sc.buoy) {
+    ct0 = a.Temp[pc + (long)kbeg * sz];
+    ct1 = a.Temp[pc + (long)(kbeg + 1) * sz];
+  }
   uint8_t ccd[3];
   T cus[3];
 #pragma unroll
@@ -239,7 +246,8 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
       ov = a.V[o];
       ow = a.W[o];
     }
-    T npr = 0, nprx = 0, npry = 0, nprz = 0, nus[3] = {0, 0, 0};
+    T npr = 0, nprx = 0, npry = 0, nprz = 0, nus[3] = {0, 0, 0}, nt = 0;
+    if (more && sc.buoy) nt = a.Temp[pc + (long)(k + 2) * sz];
     uint8_t ncd[3] = {0, 0, 0};
     if (more) {
       const long q = pc + (long)(k + 1) * sz;
@@ -369,6 +377,7 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
             }
           }
           T w = acc * m256;
+          if (sc.buoy) w = w + (T)sc.bA2 * (ct1 + ct0) - (T)sc.bATref;
           const T f0 = cw[3];
           w = w + q4 * (at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (at(w0, 1, 0) - f0) * rx;
           w = w - q4 * (at(n1, 0, 0) + at(n0, 0, 0) + at(n1, -1, 0) + at(n0, -1, 0)) * (f0 - at(w0, -1, 0)) * rx;
@@ -402,6 +411,8 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
       cu[6] = ou;
       cv[6] = ov;
       cw[6] = ow;
+      ct0 = ct1;
+      ct1 = nt;
       cpr = npr;
       cprx = nprx;
       cpry = npry;
@@ -450,10 +461,11 @@ void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, 
 // only: its FP64 LDS rings exceed the 160 KiB of a workgroup; an FP64 launch
 // asking for it runs 32x8).  The engine's autotuner times them per grid.
 template <typename T>
-void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *const Us[3],
-                        T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk, int tile,
-                        hipStream_t st) {
+void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, const T *Temp,
+                        T *const Us[3], T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk,
+                        int tile, hipStream_t st) {
   MomZArgs<T> a;
+  a.Temp = Temp;
   a.U = U;
   a.V = V;
   a.W = W;

@@ -437,6 +437,33 @@ void launch_tdiff_init(const Geo &g, const Real *Visc, Real *TDiff, double nu, d
   hipLaunchKernelGGL(k_tdiff_init<Real>, gr, b, 0, st, g, Visc, TDiff, nu, nu_scalar);
 }
 
+// The temperature in solid cells after its stage (src/scalars.f90:131-207):
+// each listed solid cell (within -1..n+2, with fluid neighbours there) takes
+// the mean of its fluid neighbours, summed in the order i-1, i+1, j-1, j+1,
+// k-1, k+1.  Fluid cells are never written, so the cells are independent.
+template <typename Real>
+__global__ void k_temp_solid_fix(Geo g, const int32_t *__restrict__ ijk, int n, const int32_t *__restrict__ pt,
+                                 Real *__restrict__ T) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+  const long p = gix(g, i, j, k), sy = g.sy, sz = g.sz;
+  Real t = 0;
+  int m = 0;
+  if (i > -1 && pt[p - 1] <= 0) { t = t + T[p - 1]; m++; }
+  if (i < g.nx + 2 && pt[p + 1] <= 0) { t = t + T[p + 1]; m++; }
+  if (j > -1 && pt[p - sy] <= 0) { t = t + T[p - sy]; m++; }
+  if (j < g.ny + 2 && pt[p + sy] <= 0) { t = t + T[p + sy]; m++; }
+  if (k > -1 && pt[p - sz] <= 0) { t = t + T[p - sz]; m++; }
+  if (k < g.nz + 2 && pt[p + sz] <= 0) { t = t + T[p + sz]; m++; }
+  if (m > 0) T[p] = t / (Real)m;
+}
+
+template <typename Real>
+void launch_temp_solid_fix(const Geo &g, const int32_t *ijk, int n, const int32_t *prtype, Real *T, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_temp_solid_fix<Real>, dim3(nblk(n, 256)), dim3(256), 0, st, g, ijk, n, prtype, T);
+}
+
 #define ELMM_INST_SCALAR(R)                                                                                        \
   template void launch_scalar<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *,     \
                                  const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *, StageConst,   \
@@ -447,6 +474,7 @@ void launch_tdiff_init(const Geo &g, const Real *Visc, Real *TDiff, double nu, d
   template void launch_wm_div<R>(const Geo &, WMScalarPoints, const int32_t *, const R *, const R *, const R *,    \
                                  hipStream_t);                                                                    \
   template void launch_tdiff_init<R>(const Geo &, const R *, R *, double, double, hipStream_t);                 \
+  template void launch_temp_solid_fix<R>(const Geo &, const int32_t *, int, const int32_t *, R *, hipStream_t);  \
   template void launch_scalar_zm<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *,  \
                                     const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *, StageConst,\
                                     int, hipStream_t);

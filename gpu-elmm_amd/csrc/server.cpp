@@ -134,9 +134,16 @@ int elmm_footprint(const elmm_config *cfg, int64_t *bytes) {
   const long M = cfg->nx / 2 + 1;
   const int es = cfg->precision == ELMM_FP32_REFINED && cfg->refine_iters > 0 ? 4 : 8;
   const long Mp = es == 8 ? (M + 7) / 8 * 8 : (M + 15) / 16 * 16;  // 128-byte rows
-  int64_t b = (int64_t)ntot * (12 * e + 3 * e * cfg->nscalars + 3 * 8 + 7 + 4 + 1);
-  // spectral array: complex FP32 with refinement sweeps, complex FP64 otherwise
-  b += (int64_t)Mp * cfg->ny * cfg->nz * 2 * es;
+  const int nslots = cfg->nscalars + (cfg->enable_buoyancy ? 1 : 0);  // the temperature is one more slot
+  int64_t b = (int64_t)ntot * (12 * e + 3 * e * nslots + 3 * 8 + 7 + 4 + 1);
+  if (cfg->poisson_solver == ELMM_POISSON_MG) {
+    // multigrid: phi and rhs per level (x and y halved: at most 4/3 of the
+    // cells) and the coarsest level's work arrays (below one level 0 array)
+    b += (int64_t)cfg->nx * cfg->ny * cfg->nz * 2 * 8 * 4 / 3 + (int64_t)cfg->nx * cfg->ny * cfg->nz * 8;
+  } else {
+    // spectral array: complex FP32 with refinement sweeps, complex FP64 otherwise
+    b += (int64_t)Mp * cfg->ny * cfg->nz * 2 * es;
+  }
   // block partials of the reductions (divergence, CFL, delta: at most four
   // doubles per 256 cells) and the allocation granularity of the ~60
   // buffers of a handle (2 MiB each at most)

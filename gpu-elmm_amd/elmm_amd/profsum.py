@@ -3,6 +3,9 @@
     python3 -m elmm_amd.profsum RESULTS OUT.md TITLE COMMAND
     python3 -m elmm_amd.profsum --counters OUT.md TITLE COMMAND LABEL=DB[,DB...] ...
 
+With --counters each LABEL=DB,... group merges the counters of several
+`rocprofv3 --pmc` passes over the same program.
+
 RESULTS is the `*_results.db` database that `rocprofv3 --kernel-trace
 --stats -d DIR -o run -- ...` writes (or its `*_kernel_stats.csv`); with
 --counters, each LABEL names a set of `rocprofv3 --pmc ...` databases whose
@@ -33,35 +36,70 @@ def short(name):
     return name.replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0]
 
 
-COUNTER_KERNELS = ("k_momentum_zm", "k_scalar_zm", "k_sgs<", "k_projection", "k_fft_x_fwd", "k_fft_x_inv",
-                   "k_fft_y", "k_solve_z")
+COUNTER_KERNELS = ("k_momentum_zm", "k_scalar_zm", "k_sgs_zm", "k_sgs<", "k_projection", "k_fft_x_fwd",
+                   "k_fft_x_inv", "k_fft_y", "k_solve_z")
+
+# Per dispatch a counter has one value per hardware instance (shader engine,
+# XCD...): SQ and TCC counters are summed over instances, GRBM ones (cycles
+# of each XCD's graphics engine) averaged; dispatches are then averaged.
+SIMDS = 1024  # 256 CUs x 4 SIMDs on MI355X
+
+
+def _dispatch_values(path):
+    import sqlite3
+    db = sqlite3.connect(path)
+    try:
+        per = {}
+        for did, name, cn, cv in db.execute("select dispatch_id, name, counter_name, counter_value from pmc_events"):
+            d = per.setdefault((name, did), {})
+            d.setdefault(cn, []).append(cv)
+    finally:
+        db.close()
+    out = {}
+    for (name, _), d in per.items():
+        agg = out.setdefault(name, {})
+        for cn, vals in d.items():
+            v = sum(vals) / len(vals) if cn.startswith("GRBM") else sum(vals)
+            agg.setdefault(cn, []).append(v)
+    return {n: {c: sum(v) / len(v) for c, v in d.items()} for n, d in out.items()}
 
 
 def counters(dst, title, cmd, groups):
-    import sqlite3
     rows, cols = [], []
     for g in groups:
         label, dbs = g.split("=", 1)
         vals = {}
         for path in dbs.split(","):
-            db = sqlite3.connect(path)
-            try:
-                for kn, cn, v in db.execute("select kernel_name, counter_name, avg(value) from counters_collection "
-                                            "group by kernel_name, counter_name"):
-                    vals.setdefault(kn, {})[cn] = v
+            for kn, d in _dispatch_values(path).items():
+                vals.setdefault(kn, {}).update(d)
+                for cn in d:
                     if cn not in cols:
                         cols.append(cn)
-            finally:
-                db.close()
         for key in COUNTER_KERNELS:
             for kn, d in vals.items():
                 if key in kn:
                     rows.append((label, short(kn), d))
+    derived = []
+    if "SQ_ACTIVE_INST_VALU" in cols and "GRBM_GUI_ACTIVE" in cols:
+        derived.append(("VALU busy %", lambda d: 100 * d["SQ_ACTIVE_INST_VALU"] * 4 / SIMDS / d["GRBM_GUI_ACTIVE"]))
+    if "SQ_INSTS_VALU" in cols and "GRBM_GUI_ACTIVE" in cols:
+        derived.append(("VALU instr / SIMD-cycle", lambda d: d["SQ_INSTS_VALU"] / SIMDS / d["GRBM_GUI_ACTIVE"]))
+    if "SQ_INSTS_VALU" in cols and "SQ_WAVES" in cols:
+        derived.append(("VALU instr / wave", lambda d: d["SQ_INSTS_VALU"] / d["SQ_WAVES"]))
     with open(dst, "w") as f:
         f.write(f"# {title}\n\nCommands (from the repository root on an MI355X):\n\n```\n{cmd}\n```\n\n")
-        f.write("| form | kernel | " + " | ".join(cols) + " |\n|---|---|" + "---:|" * len(cols) + "\n")
+        f.write("Per dispatch: SQ counters summed over instances, GRBM_GUI_ACTIVE averaged over XCDs; "
+                "dispatches averaged.\n\n")
+        names = [c for c in cols] + [n for n, _ in derived]
+        f.write("| form | kernel | " + " | ".join(names) + " |\n|---|---|" + "---:|" * len(names) + "\n")
         for label, k, d in rows:
-            f.write(f"| {label} | `{k}` | " + " | ".join(f"{d.get(c, 0):.3g}" for c in cols) + " |\n")
+            vals = [f"{d.get(c, 0):.4g}" for c in cols]
+            for _, fn in derived:
+                try:
+                    vals.append(f"{fn(d):.3g}")
+                except (KeyError, ZeroDivisionError):
+                    vals.append("")
+            f.write(f"| {label} | `{k}` | " + " | ".join(vals) + " |\n")
 
 
 def main(argv):

@@ -51,6 +51,10 @@ module elmm_gpu
     real(c_double) :: cfl_target, dt_max, dt_min, start_time, end_time
     integer(c_int32_t) :: autotune
     integer(c_int32_t) :: mg_cycles_max
+    integer(c_int32_t) :: enable_buoyancy
+    real(c_double) :: grav_acc, temperature_ref
+    integer(c_int32_t) :: temp_btype(6)
+    real(c_double) :: side_temp(6)
     integer(c_int32_t) :: reserved(2)
   end type
 

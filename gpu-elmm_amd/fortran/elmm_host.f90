@@ -118,6 +118,10 @@ program elmm_host
   cfg%cfl_target = 0.3_c_double; cfg%dt_max = dt; cfg%dt_min = 0; cfg%start_time = 0; cfg%end_time = 0
   cfg%autotune = 1
   cfg%mg_cycles_max = 0
+  cfg%enable_buoyancy = 0; cfg%grav_acc = 9.81_c_double; cfg%temperature_ref = 295._c_double
+  cfg%temp_btype = [ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_NEUMANN, &
+                    ELMM_BC_NEUMANN]
+  cfg%side_temp = 0
   cfg%reserved = 0
 
   allocate(cub(max(1, n_cuboids)), src(max(1, n_sources)))

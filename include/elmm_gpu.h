@@ -58,6 +58,7 @@ enum { ELMM_POISSON_FFT = 2, ELMM_POISSON_MG = 3 };
 enum {
   ELMM_F_U = 0, ELMM_F_V = 1, ELMM_F_W = 2, ELMM_F_PR = 3,
   ELMM_F_VISC = 4, ELMM_F_TDIFF = 5, ELMM_F_PHI = 6, ELMM_F_Q = 7,
+  ELMM_F_TEMPERATURE = 15, /* with enable_buoyancy */
   ELMM_F_SCALAR = 16      /* + scalar index (0-based) */
 };
 
@@ -123,6 +124,18 @@ typedef struct elmm_config {
                                 replaces InitTiles (src/tiling.f90:16-111)    */
   int32_t mg_cycles_max;     /* ELMM_POISSON_MG: V-cycles at most per solve
                                 (maxPoissoniter); 0 = 50                      */
+  int32_t enable_buoyancy;   /* 1: temperature as an active scalar: its RK3
+                                stage (src/scalars.f90:124-211), the buoyancy
+                                force on W (BuoyancyForce, 2nd order,
+                                src/dynamics.f90:608-700) and TDiff recomputed
+                                every stage (ComputeTDiff,
+                                src/scalar_diffusion.f90:983-1009); uniform
+                                grids, adiabatic wall model                   */
+  double  grav_acc;          /* grav_acc                                      */
+  double  temperature_ref;   /* temperature_ref                               */
+  int32_t temp_btype[6];     /* TempBtype (periodic x/y; Dirichlet or Neumann
+                                in z)                                        */
+  double  side_temp[6];      /* sideTemp (value or gradient)                  */
   int32_t reserved[2];
 } elmm_config;
 

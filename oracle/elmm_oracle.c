@@ -239,7 +239,8 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
   o->Q = field(o);
   o->Phi = field(o);
   o->RHS = field(o);
-  int ns = cfg->nscalars;
+  /* scalar slots: the passive scalars, then the temperature (buoyancy) */
+  int ns = cfg->nscalars + (cfg->enable_buoyancy ? 1 : 0);
   o->S = xcalloc(ns + 1, sizeof(double *));
   o->S2 = xcalloc(ns + 1, sizeof(double *));
   o->Sadv = xcalloc(ns + 1, sizeof(double *));
@@ -315,7 +316,7 @@ void oracle_destroy(oracle *o) {
                     o->dzW, o->U, o->V, o->W, o->U2, o->V2, o->W2, o->Us, o->Vs, o->Ws, o->Pr, o->Visc,
                     o->TDiff, o->Q, o->Phi, o->RHS};
   for (size_t i = 0; i < sizeof arrs / sizeof arrs[0]; i++) free(arrs[i]);
-  for (int s = 0; s < o->c.nscalars; s++) {
+  for (int s = 0; s < o->c.nscalars + (o->c.enable_buoyancy ? 1 : 0); s++) {
     free(o->S[s]);
     free(o->S2[s]);
     free(o->Sadv[s]);
@@ -364,6 +365,7 @@ static double *field_ptr(oracle *o, int32_t f) {
     case ORACLE_F_RHS: return o->RHS;
     default:
       if (f >= ELMM_F_SCALAR && f < ELMM_F_SCALAR + o->c.nscalars) return o->S[f - ELMM_F_SCALAR];
+      if (f == ELMM_F_TEMPERATURE && o->c.enable_buoyancy) return o->S[o->c.nscalars];
       if (f >= ORACLE_F_SADV && f < ORACLE_F_SADV + o->c.nscalars) return o->Sadv[f - ORACLE_F_SADV];
   }
   return NULL;
@@ -532,10 +534,8 @@ static void bound_uvw(oracle *o, double *U, double *V, double *W) {
 }
 
 /* CommonBase, src/scalarboundaries.f90:18-279 (Dirichlet, periodic, Neumann). */
-static void bound_scalar(oracle *o, double *a) {
+static void bound_scalar_bc(oracle *o, double *a, const int *bt, const double *side) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  const int *bt = o->c.scal_btype;
-  const double *side = o->c.side_scal;
 #define S_(i, j, k) A3(o, a, i, j, k)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++) {
@@ -625,6 +625,11 @@ static void bound_scalar(oracle *o, double *a) {
     }
 #undef S_
 }
+
+/* BoundScalar (ScalBtype, sideScal) and BoundTemperature (TempBtype,
+   sideTemp; src/scalarboundaries.f90:296-308): the same rules. */
+static void bound_scalar(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.scal_btype, o->c.side_scal); }
+static void bound_temperature(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.temp_btype, o->c.side_temp); }
 
 /* BoundViscosity, src/scalarboundaries.f90:326-431 */
 static void bound_visc(oracle *o, double *a) {
@@ -1133,7 +1138,18 @@ static void wm_visc_points(oracle *o, int write_visc) {
   }
 }
 
-/* SubgridStresses, src/dynamics.f90:869-908 (neutral). */
+/* ComputeTDiff, src/scalar_diffusion.f90:983-1009 (constPr_sgs = 0.6, a
+   default-real literal); with buoyancy, every stage. */
+static void compute_tdiff(oracle *o) {
+  const double prs = (double)0.6f;
+  for (int k = 1; k <= o->nz; k++)
+    for (int j = 1; j <= o->ny; j++)
+      for (int i = 1; i <= o->nx; i++)
+        A3(o, o->TDiff, i, j, k) = o->c.nu > 0 ? (A3(o, o->Visc, i, j, k) - o->c.nu) / prs + o->c.nu_scalar
+                                               : A3(o, o->Visc, i, j, k) / prs;
+}
+
+/* SubgridStresses, src/dynamics.f90:869-908 (ComputeTDiff with buoyancy). */
 static void subgrid_stresses(oracle *o) {
   wm_visc_points(o, 1);
   sgs_smag(o);
@@ -1144,7 +1160,8 @@ static void subgrid_stresses(oracle *o) {
     const int32_t *q = &o->ib_ijk[3][3 * p];
     A3(o, o->Visc, q[0], q[1], q[2]) = ib_interp(o, 3, p, o->Visc);
   }
-  if (o->c.nscalars > 0) bound_visc(o, o->TDiff);
+  if (o->c.enable_buoyancy) compute_tdiff(o);
+  if (o->c.nscalars > 0 || o->c.enable_buoyancy) bound_visc(o, o->TDiff);
 }
 
 /* ------------------------------------------------- momentum advection --- */
@@ -1570,6 +1587,18 @@ static void momentum_diffusion_vz(oracle *o) {
 
 /* Convection (advection_method 4 on uniform grids, 2 on stretched z; explicit
    diffusion), src/dynamics.f90:230-499 */
+/* BuoyancyForce with buoyancy only, 2nd order (src/dynamics.f90:608-700):
+   W += A/2 (T(k+1) + T(k)) - A T_ref, A = grav_acc / temperature_ref. */
+static void buoyancy_force(oracle *o, double *Ws, const double *T) {
+  const double A = o->c.grav_acc / o->c.temperature_ref, A2 = A / 2.0;
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= o->Wn[2]; k++)
+    for (int j = 1; j <= o->Wn[1]; j++)
+      for (int i = 1; i <= o->Wn[0]; i++)
+        A3(o, Ws, i, j, k) =
+            A3(o, Ws, i, j, k) + A2 * (A3(o, T, i, j, k + 1) + A3(o, T, i, j, k)) - A * o->c.temperature_ref;
+}
+
 static void convection(oracle *o, int stage, double dt) {
   double *U2[3] = {o->U2, o->V2, o->W2};
   double *S[3] = {o->Us, o->Vs, o->Ws};
@@ -1614,6 +1643,7 @@ static void convection(oracle *o, int stage, double dt) {
           for (int i = 1; i <= n[0]; i++)
             A3(o, s, i, j, k) = c == 0 ? cd4_u(o, i, j, k) : c == 1 ? cd4_v(o, i, j, k) : cd4_w(o, i, j, k);
     }
+    if (o->c.enable_buoyancy) buoyancy_force(o, o->Ws, o->S[o->c.nscalars]);
     momentum_diffusion(o);
   }
   for (int c = 0; c < 3; c++) {
@@ -1801,43 +1831,85 @@ static void scalar_diffusion(oracle *o, double *Sadv, const double *S) {
 }
 
 /* ScalarRK3 stage for passive scalars, src/scalars.f90:88-299, 364-379 */
-static void scalar_rk3(oracle *o, int stage, double dt) {
-  if (o->c.nscalars == 0) return;
-  wm_divergence(o);
-  for (int sc = 0; sc < o->c.nscalars; sc++) {
-    double *S = o->S[sc], *S2 = o->S2[sc], *Sadv = o->Sadv[sc];
+/* One scalar's RK3 stage (the `stage` procedure of ScalarRK3,
+   src/scalars.f90:233-299): slot sc (the temperature when temp: its
+   boundary conditions, no point sources). */
+static void scalar_stage(oracle *o, int sc, int stage, double dt, int temp) {
+  double *S = o->S[sc], *S2 = o->S2[sc], *Sadv = o->Sadv[sc];
+  if (temp)
+    bound_temperature(o, S);
+  else
     bound_scalar(o, S);
-    if (stage > 0) {
-      double a = RK_rho[stage] * dt;
+  if (stage > 0) {
+    double a = RK_rho[stage] * dt;
 #pragma omp parallel for
-      for (long p = 0; p < o->ntot; p++) S2[p] = Sadv[p] * a;
-    } else {
-      for (long p = 0; p < o->ntot; p++) S2[p] = 0;
-    }
-    kappa_scalar(o, Sadv, S);
-#pragma omp parallel for collapse(2)
-    for (int k = 0; k <= o->nz + 1; k++)
-      for (int j = 0; j <= o->ny + 1; j++)
-        for (int i = 0; i <= o->nx + 1; i++)
-          if (A3(o, o->Prtype, i, j, k) > 0) A3(o, Sadv, i, j, k) = 0;
-    scalar_diffusion(o, Sadv, S);
-    /* ScalarExtra: volume (point) sources, then WM divergence term */
-    for (int p = 0; p < o->src_n; p++) {
-      if (o->src_sc[p] != sc) continue;
-      const int32_t *q = &o->src_ijk[3 * p];
-      A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->src_flux[p];
-    }
-    for (int p = 0; p < o->wmp.n; p++) {
-      const int32_t *q = &o->wmp.ijk[3 * p];
-      A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
-    }
-    double b = RK_beta[stage] * dt;
-#pragma omp parallel for
-    for (long p = 0; p < o->ntot; p++) S2[p] = S2[p] + Sadv[p] * b;
-#pragma omp parallel for
-    for (long p = 0; p < o->ntot; p++) S[p] = S[p] + S2[p];
-    bound_scalar(o, S);
+    for (long p = 0; p < o->ntot; p++) S2[p] = Sadv[p] * a;
+  } else {
+    for (long p = 0; p < o->ntot; p++) S2[p] = 0;
   }
+  kappa_scalar(o, Sadv, S);
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k <= o->nz + 1; k++)
+    for (int j = 0; j <= o->ny + 1; j++)
+      for (int i = 0; i <= o->nx + 1; i++)
+        if (A3(o, o->Prtype, i, j, k) > 0) A3(o, Sadv, i, j, k) = 0;
+  scalar_diffusion(o, Sadv, S);
+  /* ScalarExtra / TemperatureExtra: volume (point) sources, then the WM
+     divergence term (src/scalars.f90:301-331, 364-379) */
+  for (int p = 0; p < o->src_n; p++) {
+    if (temp || o->src_sc[p] != sc) continue;
+    const int32_t *q = &o->src_ijk[3 * p];
+    A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->src_flux[p];
+  }
+  for (int p = 0; p < o->wmp.n; p++) {
+    const int32_t *q = &o->wmp.ijk[3 * p];
+    A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
+  }
+  double b = RK_beta[stage] * dt;
+#pragma omp parallel for
+  for (long p = 0; p < o->ntot; p++) S2[p] = S2[p] + Sadv[p] * b;
+#pragma omp parallel for
+  for (long p = 0; p < o->ntot; p++) S[p] = S[p] + S2[p];
+  if (temp)
+    bound_temperature(o, S);
+  else
+    bound_scalar(o, S);
+}
+
+/* The temperature in solid cells after its stage (src/scalars.f90:131-207):
+   over -1..n+2, a solid cell with fluid neighbours (within -1..n+2) takes
+   their mean, summed in the order i-1, i+1, j-1, j+1, k-1, k+1; fluid cells
+   are never written, so the loop order does not matter. */
+static void temperature_solid_fix(oracle *o, double *T) {
+  const int nx = o->nx, ny = o->ny, nz = o->nz;
+  const int32_t *P = o->Prtype;
+  for (int k = -1; k <= nz + 2; k++)
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) {
+        if (A3(o, P, i, j, k) <= 0) continue;
+        double t = 0;
+        int n = 0;
+        if (i > -1 && A3(o, P, i - 1, j, k) <= 0) { t = t + A3(o, T, i - 1, j, k); n++; }
+        if (i < nx + 2 && A3(o, P, i + 1, j, k) <= 0) { t = t + A3(o, T, i + 1, j, k); n++; }
+        if (j > -1 && A3(o, P, i, j - 1, k) <= 0) { t = t + A3(o, T, i, j - 1, k); n++; }
+        if (j < ny + 2 && A3(o, P, i, j + 1, k) <= 0) { t = t + A3(o, T, i, j + 1, k); n++; }
+        if (k > -1 && A3(o, P, i, j, k - 1) <= 0) { t = t + A3(o, T, i, j, k - 1); n++; }
+        if (k < nz + 2 && A3(o, P, i, j, k + 1) <= 0) { t = t + A3(o, T, i, j, k + 1); n++; }
+        if (n > 0) A3(o, T, i, j, k) = t / n;
+      }
+}
+
+/* ScalarRK3, src/scalars.f90:88-299: DivergenceWM, the temperature stage
+   and its solid-cell values (buoyancy), then the passive scalars. */
+static void scalar_rk3(oracle *o, int stage, double dt) {
+  const int nsc = o->c.nscalars, buoy = o->c.enable_buoyancy != 0;
+  if (nsc == 0 && !buoy) return;
+  wm_divergence(o);
+  if (buoy) {
+    scalar_stage(o, nsc, stage, dt, 1);
+    temperature_solid_fix(o, o->S[nsc]);
+  }
+  for (int sc = 0; sc < nsc; sc++) scalar_stage(o, sc, stage, dt, 0);
 }
 
 /* ------------------------------------------------------------ FFT ------- */
@@ -2706,13 +2778,14 @@ int oracle_init(oracle *o) {
   pressure_correction(o, o->U, o->V, o->W, 0, 1.0);
   sgs_smag(o);
   bound_visc(o, o->Visc);
-  if (o->c.nscalars > 0) {
+  if (o->c.nscalars > 0 || o->c.enable_buoyancy) {
     for (int k = 1; k <= o->nz; k++)
       for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++)
           A3(o, o->TDiff, i, j, k) = (A3(o, o->Visc, i, j, k) - o->c.nu) / (double)0.6f + o->c.nu_scalar;
     bound_visc(o, o->TDiff);
   }
+  if (o->c.enable_buoyancy) bound_temperature(o, o->S[o->c.nscalars]);
   for (int s = 0; s < o->c.nscalars; s++) bound_scalar(o, o->S[s]);
   wm_visc_points(o, 1);
   wm_uvw_fluxes(o);
@@ -2897,6 +2970,15 @@ int32_t oracle_wm_ustar(oracle *o, double *out) {
   return o->wmp.n;
 }
 void oracle_op_wm_visc(oracle *o) { wm_visc_points(o, 1); }
+void oracle_op_buoyancy(oracle *o) {
+  if (o->c.enable_buoyancy) buoyancy_force(o, o->Ws, o->S[o->c.nscalars]);
+}
+void oracle_op_temp_fix(oracle *o) {
+  if (o->c.enable_buoyancy) temperature_solid_fix(o, o->S[o->c.nscalars]);
+}
+void oracle_op_bound_temperature(oracle *o) {
+  if (o->c.enable_buoyancy) bound_temperature(o, o->S[o->c.nscalars]);
+}
 int32_t oracle_op_wm_div(oracle *o, double *out) {
   wm_divergence(o);
   if (out) memcpy(out, o->wmp_div, (size_t)o->wmp.n * sizeof(double));

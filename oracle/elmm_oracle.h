@@ -68,6 +68,9 @@ void oracle_op_null(oracle *o);                 /* NullInterior(U,V,W)          
 void oracle_op_ib_visc(oracle *o);              /* serial ScalFl gather on Visc */
 int32_t oracle_op_wm_div(oracle *o, double *out); /* DivergenceWM -> out (n)    */
 void oracle_op_wm_visc(oracle *o);              /* ComputeViscsWM (u*, Visc)    */
+void oracle_op_buoyancy(oracle *o);             /* BuoyancyForce on Wstar       */
+void oracle_op_temp_fix(oracle *o);             /* temperature in solid cells   */
+void oracle_op_bound_temperature(oracle *o);    /* BoundTemperature             */
 int32_t oracle_op_poisson_mg(oracle *o, const double *rhs_interior, double *phi_interior, double *res);
 void oracle_mg_tables(int nz, double ax, double ay, const double *az, const double *cz, double *den, double *e);
 int32_t oracle_wm_ustar(oracle *o, double *out); /* WMPoints%ustar -> out (n)   */

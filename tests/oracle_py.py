@@ -53,6 +53,8 @@ def _load():
         lib.oracle_op_poisson_mg.argtypes = [C.c_void_p, pd, pd, pd]
         lib.oracle_op_poisson_mg.restype = C.c_int32
         lib.oracle_op_wm_visc.argtypes = [C.c_void_p]
+        for name in ("oracle_op_buoyancy", "oracle_op_temp_fix", "oracle_op_bound_temperature"):
+            getattr(lib, name).argtypes = [C.c_void_p]
         lib.oracle_wm_ustar.argtypes = [C.c_void_p, pd]
         lib.oracle_wm_ustar.restype = C.c_int32
         lib.oracle_op_scalar_extra.argtypes = [C.c_void_p, C.c_int32]
@@ -168,6 +170,15 @@ class Oracle:
         out = np.zeros(n)
         _lib.oracle_wm_ustar(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+    def op_buoyancy(self):
+        _lib.oracle_op_buoyancy(self._h)
+
+    def op_temp_fix(self):
+        _lib.oracle_op_temp_fix(self._h)
+
+    def op_bound_temperature(self):
+        _lib.oracle_op_bound_temperature(self._h)
 
     def op_scalar_extra(self, sc=0):
         _lib.oracle_op_scalar_extra(self._h, sc)

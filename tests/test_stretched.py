@@ -260,12 +260,12 @@ def test_gpu_stretched_channel_fp64(require_gpu):
     o, s = channel_pair(cfg)
     do, ds = o.step(1), s.step(1)
     err = compare_state(s, o, cfg, ["U", "V", "W", "Pr", "Visc"])
-    assert max(err.values()) <= 1e-12, err
+    assert max(err.values()) == 0.0, err
     assert ds.max_div <= max(do.max_div, 1e-15) * 1.5 and ds.max_div < 1e-11
     o.step(9)
     s.step(9)
     err = compare_state(s, o, cfg, ["U", "V", "W", "Pr", "Visc"])
-    assert max(err.values()) <= 1e-11, err
+    assert max(err.values()) == 0.0, err
     o.close()
     s.close()
 
