@@ -5,7 +5,7 @@
 
 namespace elmm {
 
-size_t momentum_zm_lds_bytes(int elem) { return ((size_t)NSLOT * Tile<32, 8>::PS + 256) * elem; }
+size_t momentum_zm_lds_bytes(int elem) { return (size_t)NSLOT * Tile<32, 8>::PS * elem; }
 
 bool momentum_zm_supported(const Geo &g) { return !g.variable_z; }
 

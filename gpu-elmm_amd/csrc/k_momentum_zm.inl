@@ -33,8 +33,8 @@ namespace elmm {
 namespace {
 
 constexpr int HALO = 3;
-// ring slots: U 0-3, V 4-7, W 8-10, viscosity 11-13
-constexpr int SU = 0, SV = 4, SW = 8, SN = 11, NSLOT = 14;
+// ring slots: U 0-3, V 4-7, W 8-11, viscosity 12-15
+constexpr int SU = 0, SV = 4, SW = 8, SN = 12, NSLOT = 16;
 
 // tile geometry of a TX x TY block
 template <int TX, int TY>
@@ -62,7 +62,10 @@ struct MomZArgs {
 template <typename T>
 __device__ inline T d4(T uv1b, T uv1a, T uv3b, T uv3a) {
   T d = 9 * (uv1b - uv1a);
-  return d - div_rn_nz<T>(uv3b - uv3a, (T)3, recip3<T>());
+  if constexpr (sizeof(T) == 4)
+    return d - (uv3b - uv3a) * recip3<T>();
+  else
+    return d - div_rn_nz<T>(uv3b - uv3a, (T)3, recip3<T>());
 }
 
 template <typename T, typename FF>
@@ -120,11 +123,23 @@ __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr
   U2[p] = corr ? us_prev : u2;
 }
 
-// FAST: the reciprocals of dx, dy, dz satisfy div_rn's condition (checked on
-// the host), so the quotients need no fallback branch.
+// Quotients of the advective terms.  FP64 (the reference's arithmetic): each
+// direction's difference divided by the spacing, correctly rounded (with
+// FAST the reciprocals satisfy div_rn's condition, checked on the host, so
+// there is no fallback branch), summed, then scaled by -1/256.  FP32 (the
+// precision form): one product per direction with -1/(256 d) rounded once,
+// and the 1/4 of the viscosity averages folded into 1/d^2.
+//
+// Ring slots hold level L at (L - kbeg) mod 4 for every field and the z loop
+// is unrolled by 4, so every slot, and with it every LDS address, is a
+// constant offset from the thread's own base.
+// (launch bounds, in waves per SIMD: 4-byte fields 4 for 512-thread blocks
+// (128 VGPRs, two blocks per CU) and 3 for 256-thread blocks (three blocks,
+// as many as their LDS allows); 8-byte fields 2)
 template <typename T, int TX, int TY, bool FAST>
-__global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 2) : 1)
+__global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 : 3) : 2)
     k_momentum_zm(Geo g, MomZArgs<T> a, StageConst sc, int kchunk) {
+  constexpr bool F32 = sizeof(T) == 4;
   constexpr int PW = Tile<TX, TY>::PW, PS = Tile<TX, TY>::PS, NTH = Tile<TX, TY>::NTH;
   constexpr int PLOAD = Tile<TX, TY>::PLOAD;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -138,17 +153,16 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   const long sz = g.sz;
   const int lx = tx + HALO, ly = ty + HALO;
 
-  // plane element offsets at level 0 (clamped into the allocation).  A
-  // thread past the end of the plane loads the plane's last element and
-  // writes it to a spare LDS cell of its own behind the rings, so plane
-  // loads and writes carry no branches.
+  // plane element offsets at level 0 (clamped into the allocation) and LDS
+  // positions.  A thread past the end of the plane loads the plane's last
+  // element and stores it where its owner does (the same value), so plane
+  // loads and stores carry no branches.
   long poff[PLOAD];
   int pidx[PLOAD];
 #pragma unroll
   for (int r = 0; r < PLOAD; r++) {
-    const int e = tid + r * NTH;
-    pidx[r] = e < PS ? e : -1;
-    const int ec = min(e, PS - 1);
+    const int ec = min(tid + r * NTH, PS - 1);
+    pidx[r] = ec;
     const int ex = ec % PW, ey = ec / PW;
     const int gi = min(i0 - HALO + ex, g.nx + 3), gj = min(j0 - HALO + ey, g.ny + 3);
     poff[r] = gix(g, gi, gj, 0);
@@ -156,15 +170,12 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   const int ic = min(i, g.nx), jc = min(j, g.ny);
   const long pc = gix(g, ic, jc, 0);
 
-  auto slotU = [](int L) { return SU + ((L + 4) & 3); };
-  auto slotV = [](int L) { return SV + ((L + 4) & 3); };
-  auto slotW = [](int L) { return SW + (L + 3) % 3; };
-  auto slotN = [](int L) { return SN + (L + 3) % 3; };
-  auto at = [&](int slot, int dx, int dy) -> T & { return lds[slot * PS + (ly + dy) * PW + lx + dx]; };
-  auto wr = [&](int slot, int r) -> T & { return lds[pidx[r] >= 0 ? slot * PS + pidx[r] : NSLOT * PS + tid]; };
-  auto load_plane = [&](const T *__restrict__ F, int L, int slot) {
+  // slot of the level kbeg + rel
+  auto slot = [](int base, int rel) { return base + ((rel + 8) & 3); };
+  auto at = [&](int s, int dx, int dy) -> T & { return lds[s * PS + (ly + dy) * PW + lx + dx]; };
+  auto load_plane = [&](const T *__restrict__ F, int L, int s) {
 #pragma unroll
-    for (int r = 0; r < PLOAD; r++) wr(slot, r) = F[poff[r] + (long)L * sz];
+    for (int r = 0; r < PLOAD; r++) lds[s * PS + pidx[r]] = F[poff[r] + (long)L * sz];
   };
 
   // ---- prologue: rings for k = kbeg
@@ -176,13 +187,15 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
     cv[q] = a.V[o];
     cw[q] = a.W[o];
   }
-  for (int L = kbeg - 1; L <= kbeg + 2; L++) {
-    load_plane(a.U, L, slotU(L));
-    load_plane(a.V, L, slotV(L));
+#pragma unroll
+  for (int rel = -1; rel <= 2; rel++) {
+    load_plane(a.U, kbeg + rel, slot(SU, rel));
+    load_plane(a.V, kbeg + rel, slot(SV, rel));
   }
-  for (int L = kbeg - 1; L <= kbeg + 1; L++) {
-    load_plane(a.W, L, slotW(L));
-    load_plane(a.Visc, L, slotN(L));
+#pragma unroll
+  for (int rel = -1; rel <= 1; rel++) {
+    load_plane(a.W, kbeg + rel, slot(SW, rel));
+    load_plane(a.Visc, kbeg + rel, slot(SN, rel));
   }
   {
     const long o = pc + (long)(kbeg - 2) * sz;
@@ -192,23 +205,44 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
   __syncthreads();
 #pragma unroll
   for (int q = 1; q < 4; q++) {
-    const int s = slotW(kbeg - 2 + q);
+    const int s = slot(SW, q - 2);
     wx[q] = m4<T>(at(s, -1, 0), at(s, 0, 0), at(s, 1, 0), at(s, 2, 0));
     wy[q] = m4<T>(at(s, 0, -1), at(s, 0, 0), at(s, 0, 1), at(s, 0, 2));
   }
 
+  const T q4 = (T)0.25, m256 = (T)(-1.0 / 256);
   const T rx = (T)1.0 / ((T)g.dxmin * (T)g.dxmin), ry = (T)1.0 / ((T)g.dymin * (T)g.dymin),
           rz = (T)1.0 / ((T)g.dzmin * (T)g.dzmin);
+  // factors of the 4-term viscosity averages (FP32: the 1/4 folded in)
+  const T rxq = F32 ? rx * q4 : rx, ryq = F32 ? ry * q4 : ry, rzq = F32 ? rz * q4 : rz;
+  auto qa = [&](T s4) -> T {
+    if constexpr (F32)
+      return s4;
+    else
+      return q4 * s4;
+  };
   const T ix = (T)g.dxmin, iy = (T)g.dymin, iz = (T)g.dzmin;
-  const T rix = sizeof(T) == 4 ? (T)g.rdxf : (T)g.rdx, riy = sizeof(T) == 4 ? (T)g.rdyf : (T)g.rdy,
-          riz = sizeof(T) == 4 ? (T)g.rdzf : (T)g.rdz;
-  const T q4 = (T)0.25, m256 = (T)(-1.0 / 256);
+  const T rix = F32 ? (T)g.rdxf : (T)g.rdx, riy = F32 ? (T)g.rdyf : (T)g.rdy, riz = F32 ? (T)g.rdzf : (T)g.rdz;
+  const T cx = (T)(-1.0 / (256.0 * g.dxmin)), cy = (T)(-1.0 / (256.0 * g.dymin)), cz = (T)(-1.0 / (256.0 * g.dzmin));
   auto dv = [](T x, T d, T r) {
     if constexpr (FAST)
       return div_rn_nz<T>(x, d, r);
     else
       return div_rn<T>(x, d, r);
   };
+  // advective term from the three directions' differences
+  auto adv = [&](T dX, T dY, T dZ) -> T {
+    if constexpr (F32) {
+      return (dX * cx + dY * cy) + dZ * cz;
+    } else {
+      T acc = 0;
+      acc = acc + dv(dX, ix, rix);
+      acc = acc + dv(dY, iy, riy);
+      acc = acc + dv(dZ, iz, riz);
+      return acc * m256;
+    }
+  };
+  const T wnone = (T)0 * m256;  // the advective term of a cell without one
   const bool in_xy = i <= g.nx && j <= g.ny;
   // per-cell operands of the level being computed (Pr and its +x, +y, +z
   // neighbours, the three codes, the three previous Ustar values), fetched
@@ -228,12 +262,17 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
     ccd[c] = a.code[c][pc + (long)kbeg * sz];
     cus[c] = sc.stage > 0 ? a.Us[c][pc + (long)kbeg * sz] : (T)0;
   }
+  const int kcol = g.nz + 3;  // top level of the allocation
 
-  for (int k = kbeg; k <= kend; k++) {
-    const bool more = k < kend;
-    // prefetch of the next level's planes and column values
-    T pu[PLOAD], pv[PLOAD], pw[PLOAD], pn[PLOAD], ou = 0, ov = 0, ow = 0;
-    if (more) {
+  for (int k0 = kbeg; k0 <= kend; k0 += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k = k0 + u;
+      if (k > kend) break;
+      // prefetch of the next level's planes and column values (after the
+      // chunk's last level they are loaded and not used; levels stay inside
+      // the allocation)
+      T pu[PLOAD], pv[PLOAD], pw[PLOAD], pn[PLOAD];
 #pragma unroll
       for (int r = 0; r < PLOAD; r++) {
         pu[r] = a.U[poff[r] + (long)(k + 3) * sz];
@@ -241,179 +280,152 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
         pw[r] = a.W[poff[r] + (long)(k + 2) * sz];
         pn[r] = a.Visc[poff[r] + (long)(k + 2) * sz];
       }
-      const long o = pc + (long)(k + 4) * sz;
-      ou = a.U[o];
-      ov = a.V[o];
-      ow = a.W[o];
-    }
-    T npr = 0, nprx = 0, npry = 0, nprz = 0, nus[3] = {0, 0, 0}, nt = 0;
-    if (more && sc.buoy) nt = a.Temp[pc + (long)(k + 2) * sz];
-    uint8_t ncd[3] = {0, 0, 0};
-    if (more) {
+      T ou, ov, ow;
+      {
+        const long o = pc + (long)min(k + 4, kcol) * sz;
+        ou = a.U[o];
+        ov = a.V[o];
+        ow = a.W[o];
+      }
+      T nt = 0;
+      if (sc.buoy) nt = a.Temp[pc + (long)(k + 2) * sz];
       const long q = pc + (long)(k + 1) * sz;
-      nprx = a.Pr[q + 1];
-      npry = a.Pr[q + g.sy];
-      nprz = a.Pr[q + sz];
+      const T nprx = a.Pr[q + 1], npry = a.Pr[q + g.sy], nprz = a.Pr[q + sz];
+      uint8_t ncd[3];
+      T nus[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         ncd[c] = a.code[c][q];
-        if (sc.stage > 0) nus[c] = a.Us[c][q];
+        nus[c] = sc.stage > 0 ? a.Us[c][q] : (T)0;
       }
-      npr = cprz;
-    }
 
-    if (in_xy) {
-      const long p = gix(g, i, j, k);
-      const int u0 = slotU(k), um = slotU(k - 1), u1 = slotU(k + 1), u2s = slotU(k + 2);
-      const int v0 = slotV(k), vm = slotV(k - 1), v1 = slotV(k + 1), v2s = slotV(k + 2);
-      const int w0 = slotW(k), wm = slotW(k - 1);
-      const int n0 = slotN(k), nm = slotN(k - 1), n1 = slotN(k + 1);
-      const T pr0 = cpr;
+      if (in_xy) {
+        const long p = gix(g, i, j, k);
+        const int u0 = slot(SU, u), um = slot(SU, u - 1), u1 = slot(SU, u + 1), u2s = slot(SU, u + 2);
+        const int v0 = slot(SV, u), vm = slot(SV, u - 1), v1 = slot(SV, u + 1), v2s = slot(SV, u + 2);
+        const int w0 = slot(SW, u), wm = slot(SW, u - 1);
+        const int n0 = slot(SN, u), nm = slot(SN, u - 1), n1 = slot(SN, u + 1);
+        const T pr0 = cpr;
 
-      // ---------------- U
-      if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) {
-        const uint8_t cd = ccd[0];
-        const bool corr = (cd & CODE_CORRECTED) != 0;
-        {
+        // ---------------- U
+        if (i <= g.n[0][0] && j <= g.n[0][1] && k <= g.n[0][2]) {
+          const uint8_t cd = ccd[0];
+          const bool corr = (cd & CODE_CORRECTED) != 0;
           const int cls = cd & CODE_ADV_MASK;
-          T acc = 0;
+          T w = wnone;
           if (cls != 2) {
             auto Fx = [&](int o) { return at(u0, o, 0); };
             auto Fy = [&](int o) { return at(u0, 0, o); };
             auto Fz = [&](int o) { return cu[3 + o]; };
-            T d;
+            T dX, dY, dZ;
             if (cls == 0) {
-              d = self4<T>(Fx);
-              acc = acc + dv(d, ix, rix);
-              d = cross4<T>(Fy, [&](int o) { return m4<T>(at(v0, -1, o), at(v0, 0, o), at(v0, 1, o), at(v0, 2, o)); });
-              acc = acc + dv(d, iy, riy);
-              d = cross4<T>(Fz, [&](int o) { return wx[2 + o]; });
-              acc = acc + dv(d, iz, riz);
+              dX = self4<T>(Fx);
+              dY = cross4<T>(Fy, [&](int o) { return m4<T>(at(v0, -1, o), at(v0, 0, o), at(v0, 1, o), at(v0, 2, o)); });
+              dZ = cross4<T>(Fz, [&](int o) { return wx[2 + o]; });
             } else {
-              d = self2<T>(Fx);
-              acc = acc + dv(d, ix, rix);
-              d = cross2<T>(Fy, at(v0, 1, 0) + at(v0, 0, 0), at(v0, 1, -1) + at(v0, 0, -1));
-              acc = acc + dv(d, iy, riy);
-              d = cross2<T>(Fz, at(w0, 1, 0) + at(w0, 0, 0), at(wm, 1, 0) + at(wm, 0, 0));
-              acc = acc + dv(d, iz, riz);
+              dX = self2<T>(Fx);
+              dY = cross2<T>(Fy, at(v0, 1, 0) + at(v0, 0, 0), at(v0, 1, -1) + at(v0, 0, -1));
+              dZ = cross2<T>(Fz, at(w0, 1, 0) + at(w0, 0, 0), at(wm, 1, 0) + at(wm, 0, 0));
             }
+            w = adv(dX, dY, dZ);
           }
-          T w = acc * m256;
           const T f0 = at(u0, 0, 0);
           w = w + at(n0, 1, 0) * (at(u0, 1, 0) - f0) * rx;
           w = w - at(n0, 0, 0) * (f0 - at(u0, -1, 0)) * rx;
-          w = w + q4 * (at(n0, 1, 1) + at(n0, 1, 0) + at(n0, 0, 1) + at(n0, 0, 0)) * (at(u0, 0, 1) - f0) * ry;
-          w = w - q4 * (at(n0, 1, 0) + at(n0, 1, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(u0, 0, -1)) * ry;
-          w = w + q4 * (at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (cu[4] - cu[3]) * rz;
-          w = w - q4 * (at(n0, 1, 0) + at(nm, 1, 0) + at(n0, 0, 0) + at(nm, 0, 0)) * (cu[3] - cu[2]) * rz;
+          w = w + qa(at(n0, 1, 1) + at(n0, 1, 0) + at(n0, 0, 1) + at(n0, 0, 0)) * (at(u0, 0, 1) - f0) * ryq;
+          w = w - qa(at(n0, 1, 0) + at(n0, 1, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(u0, 0, -1)) * ryq;
+          w = w + qa(at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (cu[4] - cu[3]) * rzq;
+          w = w - qa(at(n0, 1, 0) + at(nm, 1, 0) + at(n0, 0, 0) + at(nm, 0, 0)) * (cu[3] - cu[2]) * rzq;
           finish_cell<T>(sc, 0, w, f0, cprx - pr0, (T)0, false, cus[0], a.Us[0], a.U2[0], p, corr);
         }
-      }
-      // ---------------- V
-      if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) {
-        const uint8_t cd = ccd[1];
-        const bool corr = (cd & CODE_CORRECTED) != 0;
-        {
+        // ---------------- V
+        if (i <= g.n[1][0] && j <= g.n[1][1] && k <= g.n[1][2]) {
+          const uint8_t cd = ccd[1];
+          const bool corr = (cd & CODE_CORRECTED) != 0;
           const int cls = cd & CODE_ADV_MASK;
-          T acc = 0;
+          T w = wnone;
           if (cls != 2) {
             auto Fx = [&](int o) { return at(v0, o, 0); };
             auto Fy = [&](int o) { return at(v0, 0, o); };
             auto Fz = [&](int o) { return cv[3 + o]; };
-            T d;
+            T dX, dY, dZ;
             if (cls == 0) {
-              d = cross4<T>(Fx, [&](int o) { return m4<T>(at(u0, o, -1), at(u0, o, 0), at(u0, o, 1), at(u0, o, 2)); });
-              acc = acc + dv(d, ix, rix);
-              d = self4<T>(Fy);
-              acc = acc + dv(d, iy, riy);
-              d = cross4<T>(Fz, [&](int o) { return wy[2 + o]; });
-              acc = acc + dv(d, iz, riz);
+              dX = cross4<T>(Fx, [&](int o) { return m4<T>(at(u0, o, -1), at(u0, o, 0), at(u0, o, 1), at(u0, o, 2)); });
+              dY = self4<T>(Fy);
+              dZ = cross4<T>(Fz, [&](int o) { return wy[2 + o]; });
             } else {
-              d = cross2<T>(Fx, at(u0, 0, 1) + at(u0, 0, 0), at(u0, -1, 1) + at(u0, -1, 0));
-              acc = acc + dv(d, ix, rix);
-              d = self2<T>(Fy);
-              acc = acc + dv(d, iy, riy);
-              d = cross2<T>(Fz, at(w0, 0, 1) + at(w0, 0, 0), at(wm, 0, 1) + at(wm, 0, 0));
-              acc = acc + dv(d, iz, riz);
+              dX = cross2<T>(Fx, at(u0, 0, 1) + at(u0, 0, 0), at(u0, -1, 1) + at(u0, -1, 0));
+              dY = self2<T>(Fy);
+              dZ = cross2<T>(Fz, at(w0, 0, 1) + at(w0, 0, 0), at(wm, 0, 1) + at(wm, 0, 0));
             }
+            w = adv(dX, dY, dZ);
           }
-          T w = acc * m256;
           const T f0 = at(v0, 0, 0);
-          w = w + q4 * (at(n0, 1, 1) + at(n0, 1, 0) + at(n0, 0, 1) + at(n0, 0, 0)) * (at(v0, 1, 0) - f0) * rx;
-          w = w - q4 * (at(n0, 0, 1) + at(n0, 0, 0) + at(n0, -1, 1) + at(n0, -1, 0)) * (f0 - at(v0, -1, 0)) * rx;
+          w = w + qa(at(n0, 1, 1) + at(n0, 1, 0) + at(n0, 0, 1) + at(n0, 0, 0)) * (at(v0, 1, 0) - f0) * rxq;
+          w = w - qa(at(n0, 0, 1) + at(n0, 0, 0) + at(n0, -1, 1) + at(n0, -1, 0)) * (f0 - at(v0, -1, 0)) * rxq;
           w = w + at(n0, 0, 1) * (at(v0, 0, 1) - f0) * ry;
           w = w - at(n0, 0, 0) * (f0 - at(v0, 0, -1)) * ry;
-          w = w + q4 * (at(n1, 0, 1) + at(n0, 0, 1) + at(n1, 0, 0) + at(n0, 0, 0)) * (cv[4] - cv[3]) * rz;
-          w = w - q4 * (at(n0, 0, 1) + at(nm, 0, 1) + at(n0, 0, 0) + at(nm, 0, 0)) * (cv[3] - cv[2]) * rz;
+          w = w + qa(at(n1, 0, 1) + at(n0, 0, 1) + at(n1, 0, 0) + at(n0, 0, 0)) * (cv[4] - cv[3]) * rzq;
+          w = w - qa(at(n0, 0, 1) + at(nm, 0, 1) + at(n0, 0, 0) + at(nm, 0, 0)) * (cv[3] - cv[2]) * rzq;
           finish_cell<T>(sc, 1, w, f0, cpry - pr0, (T)0, false, cus[1], a.Us[1], a.U2[1], p, corr);
         }
-      }
-      // ---------------- W
-      if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
-        const uint8_t cd = ccd[2];
-        const bool corr = (cd & CODE_CORRECTED) != 0;
-        {
+        // ---------------- W
+        if (i <= g.n[2][0] && j <= g.n[2][1] && k <= g.n[2][2]) {
+          const uint8_t cd = ccd[2];
+          const bool corr = (cd & CODE_CORRECTED) != 0;
           const int cls = cd & CODE_ADV_MASK;
-          T acc = 0;
+          T w = wnone;
           if (cls != 2) {
             auto Fx = [&](int o) { return at(w0, o, 0); };
             auto Fy = [&](int o) { return at(w0, 0, o); };
             auto Fz = [&](int o) { return cw[3 + o]; };
-            T d;
+            T dX, dY, dZ;
             if (cls == 0) {
-              d = cross4<T>(Fx, [&](int o) { return m4<T>(at(um, o, 0), at(u0, o, 0), at(u1, o, 0), at(u2s, o, 0)); });
-              acc = acc + dv(d, ix, rix);
-              d = cross4<T>(Fy, [&](int o) { return m4<T>(at(vm, 0, o), at(v0, 0, o), at(v1, 0, o), at(v2s, 0, o)); });
-              acc = acc + dv(d, iy, riy);
-              d = self4<T>(Fz);
-              acc = acc + dv(d, iz, riz);
+              dX = cross4<T>(Fx, [&](int o) { return m4<T>(at(um, o, 0), at(u0, o, 0), at(u1, o, 0), at(u2s, o, 0)); });
+              dY = cross4<T>(Fy, [&](int o) { return m4<T>(at(vm, 0, o), at(v0, 0, o), at(v1, 0, o), at(v2s, 0, o)); });
+              dZ = self4<T>(Fz);
             } else {
-              d = cross2<T>(Fx, at(u1, 0, 0) + at(u0, 0, 0), at(u1, -1, 0) + at(u0, -1, 0));
-              acc = acc + dv(d, ix, rix);
-              d = cross2<T>(Fy, at(v1, 0, 0) + at(v0, 0, 0), at(v1, 0, -1) + at(v0, 0, -1));
-              acc = acc + dv(d, iy, riy);
-              d = self2<T>(Fz);
-              acc = acc + dv(d, iz, riz);
+              dX = cross2<T>(Fx, at(u1, 0, 0) + at(u0, 0, 0), at(u1, -1, 0) + at(u0, -1, 0));
+              dY = cross2<T>(Fy, at(v1, 0, 0) + at(v0, 0, 0), at(v1, 0, -1) + at(v0, 0, -1));
+              dZ = self2<T>(Fz);
             }
+            w = adv(dX, dY, dZ);
           }
-          T w = acc * m256;
           if (sc.buoy) w = w + (T)sc.bA2 * (ct1 + ct0) - (T)sc.bATref;
           const T f0 = cw[3];
-          w = w + q4 * (at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (at(w0, 1, 0) - f0) * rx;
-          w = w - q4 * (at(n1, 0, 0) + at(n0, 0, 0) + at(n1, -1, 0) + at(n0, -1, 0)) * (f0 - at(w0, -1, 0)) * rx;
-          w = w + q4 * (at(n1, 0, 1) + at(n1, 0, 0) + at(n0, 0, 1) + at(n0, 0, 0)) * (at(w0, 0, 1) - f0) * ry;
-          w = w - q4 * (at(n1, 0, 0) + at(n1, 0, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(w0, 0, -1)) * ry;
+          w = w + qa(at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (at(w0, 1, 0) - f0) * rxq;
+          w = w - qa(at(n1, 0, 0) + at(n0, 0, 0) + at(n1, -1, 0) + at(n0, -1, 0)) * (f0 - at(w0, -1, 0)) * rxq;
+          w = w + qa(at(n1, 0, 1) + at(n1, 0, 0) + at(n0, 0, 1) + at(n0, 0, 0)) * (at(w0, 0, 1) - f0) * ryq;
+          w = w - qa(at(n1, 0, 0) + at(n1, 0, -1) + at(n0, 0, 0) + at(n0, 0, -1)) * (f0 - at(w0, 0, -1)) * ryq;
           w = w + at(n1, 0, 0) * (cw[4] - f0) * rz;
           w = w - at(n0, 0, 0) * (f0 - cw[2]) * rz;
           finish_cell<T>(sc, 2, w, f0, cprz - pr0, (T)0, false, cus[2], a.Us[2], a.U2[2], p, corr);
         }
       }
-    }
 
-    __syncthreads();
-    if (more) {
+      // U and V of level k+3 replace level k-1, read above; W and the
+      // viscosity of level k+2 replace level k-2
+      __syncthreads();
 #pragma unroll
       for (int r = 0; r < PLOAD; r++) {
-        wr(slotU(k + 3), r) = pu[r];
-        wr(slotV(k + 3), r) = pv[r];
-        wr(slotW(k + 2), r) = pw[r];
-        wr(slotN(k + 2), r) = pn[r];
+        lds[slot(SU, u + 3) * PS + pidx[r]] = pu[r];
+        lds[slot(SV, u + 3) * PS + pidx[r]] = pv[r];
+        lds[slot(SW, u + 2) * PS + pidx[r]] = pw[r];
+        lds[slot(SN, u + 2) * PS + pidx[r]] = pn[r];
       }
-    }
-    __syncthreads();
-    if (more) {
+      __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 6; q++) {
-        cu[q] = cu[q + 1];
-        cv[q] = cv[q + 1];
-        cw[q] = cw[q + 1];
+      for (int q2 = 0; q2 < 6; q2++) {
+        cu[q2] = cu[q2 + 1];
+        cv[q2] = cv[q2 + 1];
+        cw[q2] = cw[q2 + 1];
       }
       cu[6] = ou;
       cv[6] = ov;
       cw[6] = ow;
       ct0 = ct1;
       ct1 = nt;
-      cpr = npr;
+      cpr = cprz;
       cprx = nprx;
       cpry = npry;
       cprz = nprz;
@@ -423,11 +435,11 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
         cus[c] = nus[c];
       }
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        wx[q] = wx[q + 1];
-        wy[q] = wy[q + 1];
+      for (int q2 = 0; q2 < 3; q2++) {
+        wx[q2] = wx[q2 + 1];
+        wy[q2] = wy[q2 + 1];
       }
-      const int s = slotW(k + 2);
+      const int s = slot(SW, u + 2);
       wx[3] = m4<T>(at(s, -1, 0), at(s, 0, 0), at(s, 1, 0), at(s, 2, 0));
       wy[3] = m4<T>(at(s, 0, -1), at(s, 0, 0), at(s, 0, 1), at(s, 0, 2));
     }
@@ -441,15 +453,17 @@ __global__ void __launch_bounds__(TX *TY, TX *TY == 256 ? (sizeof(T) == 4 ? 4 : 
 namespace {
 template <typename T, int TX, int TY, bool FAST>
 void launch_tile_f(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
-  const size_t lds = ((size_t)NSLOT * Tile<TX, TY>::PS + TX * TY) * sizeof(T);  // rings + spare cells
+  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
   if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY, FAST>, lds);
   dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
   hipLaunchKernelGGL((k_momentum_zm<T, TX, TY, FAST>), gr, b, lds, st, g, a, sc, kchunk);
 }
 template <typename T, int TX, int TY>
 void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
-  const bool fast = sizeof(T) == 4 ? (g.rdxf != 0 && g.rdyf != 0 && g.rdzf != 0) : (g.rdx != 0 && g.rdy != 0 && g.rdz != 0);
-  if (fast)
+  // FP32 multiplies by reciprocals whatever the spacing
+  if constexpr (sizeof(T) == 4)
+    launch_tile_f<T, TX, TY, true>(g, a, sc, kchunk, st);
+  else if (g.rdx != 0 && g.rdy != 0 && g.rdz != 0)
     launch_tile_f<T, TX, TY, true>(g, a, sc, kchunk, st);
   else
     launch_tile_f<T, TX, TY, false>(g, a, sc, kchunk, st);
@@ -458,7 +472,7 @@ void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, 
 
 // Tile ids: kMomTile64x4 (default; measured 5 % faster than 32x8 in FP32 on
 // 512x512x128, equal in FP64), kMomTile32x8, kMomTile64x8 (4-byte fields
-// only: its FP64 LDS rings exceed the 160 KiB of a workgroup; an FP64 launch
+// only: its FP64 rings take 125 KiB of LDS, one block per CU; an FP64 launch
 // asking for it runs 32x8).  The engine's autotuner times them per grid.
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, const T *Temp,

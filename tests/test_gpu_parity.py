@@ -231,6 +231,8 @@ def test_autotuned_tiles_give_the_same_bits(require_gpu):
         t = out[1].tiles()
         print(f"precision {precision}: tuned tiles {t}; default {out[0].tiles()}", flush=True)
         assert t["tuned"] == 1 and out[0].tiles()["tuned"] == 0
-        assert t["mom_tx"] * t["mom_ty"] == 256 and t["mom_zchunk"] in (32, 64)
+        # 64x8 (512 threads) is a candidate for 4-byte fields only
+        sizes = (256, 512) if precision == capi.FP32_REFINED else (256,)
+        assert t["mom_tx"] * t["mom_ty"] in sizes and t["mom_zchunk"] in (32, 64)
         err = compare_state(out[1], out[0], base)
         assert max(err.values()) == 0.0, err
