@@ -307,7 +307,7 @@ struct Engine : EngineIface {
   DevBuf coords;
   // wall model
   DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
-  DevBuf width_tab, width_idx, sgs_rdz, sgs_rz2;
+  DevBuf width_tab, width_idx, sgs_rdz, sgs_rz2, sgs_rdzf, sgs_rz2f;
   SgsWidth sgs_width{nullptr, nullptr, 0};
   int wm_n = 0;
   DevBuf wmp_ijk, wmp_div, wmp_dist, wmp_z0, wmp_logdz0, wmp_ustar;
@@ -599,6 +599,17 @@ void Engine<T>::setup_geo() {
       sgs_width.rdz = sgs_rdz.as<double>();
       sgs_width.rz2 = sgs_rz2.as<double>();
     }
+    // the precision form's level reciprocals (single-precision divisions,
+    // the values the per-cell kernel computes in place)
+    std::vector<float> rdzf(ld, 0.f), rz2f(ld, 0.f);
+    for (int k = 1; k <= nz; k++) {
+      rdzf[k] = 1.f / (float)Grid1D::at(grid.dzPr, k);
+      rz2f[k] = 1.f / (2 * ((float)Grid1D::at(grid.zPr, k + 1) - (float)Grid1D::at(grid.zPr, k - 1)));
+    }
+    upload_vec(sgs_rdzf, rdzf);
+    upload_vec(sgs_rz2f, rz2f);
+    sgs_width.rdzf = sgs_rdzf.as<float>();
+    sgs_width.rz2f = sgs_rz2f.as<float>();
   }
 }
 

@@ -25,17 +25,24 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 // kernels evaluate the same expressions in the same order.
 // RCP (FP64): the quotients as correctly rounded products with the host's
 // reciprocals (div_rn_nz), equal to the divisions.
+// FP32 level reciprocals of the strain divisors: RN(1 / dzPr(k)) and
+// RN(1 / (2 (zPr(k+1) - zPr(k-1)))) in single precision (a table per level
+// in the z-march, the same values computed in place by the per-cell kernel).
+__device__ inline float sgs_rdzf(const Geo &g, int k) { return 1.f / (float)g.dzPr[k]; }
+__device__ inline float sgs_rz2f(const Geo &g, int k) {
+  return 1.f / (2 * ((float)g.zPr[k + 1] - (float)g.zPr[k - 1]));
+}
+
 template <typename T, bool RCP = false, typename FU, typename FV, typename FW>
 __device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width, T nu, T c_smag,
-                              const SgsWidth &wr) {
+                              const SgsWidth &wr, float rdzk = 0.f, float rz2k = 0.f) {
   const T dx = (T)g.dxmin, dy = (T)g.dymin;
-  const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
   if constexpr (sizeof(T) == 4) {
-    // precision form: products with reciprocals (two divisions per cell
-    // instead of nine) and the hardware square root; within a few ulp of
-    // the quotient form
+    // precision form: products with reciprocals (the level's two from
+    // rdzk, rz2k) and the hardware square root; within a few ulp of the
+    // quotient form
     const float rdx = g.rdxf != 0.f ? g.rdxf : 1.f / dx, rdy = g.rdyf != 0.f ? g.rdyf : 1.f / dy;
-    const float rdz = 1.f / (float)g.dzPr[k], rz2 = 1.f / (2 * (zp1 - zm1));
+    const float rdz = rdzk, rz2 = rz2k;
     const float r4x = 0.25f * rdx, r4y = 0.25f * rdy;
     float D00 = (Uf(0, 0, 0) - Uf(-1, 0, 0)) * rdx;
     float D11 = (Vf(0, 0, 0) - Vf(0, -1, 0)) * rdy;
@@ -54,6 +61,7 @@ __device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width,
     if (nu > 0) v = v + nu;
     return v;
   }
+  const T zp1 = (T)g.zPr[k + 1], zm1 = (T)g.zPr[k - 1];
   // (4 d)'s reciprocal is RN(1/d) / 4 exactly
   auto q = [&](T x, T d, double r) {
     if constexpr (RCP)
@@ -89,6 +97,10 @@ __device__ inline T smag_visc(const Geo &g, FU Uf, FV Vf, FW Wf, int k, T width,
 // Filter width of column (i, j) at level k: table from the host libm (the
 // cube root of a static geometric factor per column class and level).
 // TabOnly: the table is known to be present (no pow code in the kernel).
+// column class of the width table
+__device__ inline int smag_class(const Geo &g, const SgsWidth &w, int i, int j) {
+  return w.idx ? w.idx[(i - 1) + (long)g.nx * (j - 1)] : 0;
+}
 template <typename T, bool TabOnly = false>
 __device__ inline T smag_width(const Geo &g, const SgsWidth &w, double filter_ratio, int i, int j, int k) {
   if (TabOnly || w.tab) {
@@ -152,7 +164,8 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
     auto Vf = [&](int di, int dj, int dk) { return V[p + di + dj * sy + dk * sz]; };
     auto Wf = [&](int di, int dj, int dk) { return W[p + di + dj * sy + dk * sz]; };
     Visc[p] = sgs_model == 1 ? smag_visc<T>(g, Uf, Vf, Wf, k, smag_width<T>(g, width_t, filter_ratio, i, j, k), nu,
-                                            c_smag, width_t)
+                                            c_smag, width_t, sizeof(T) == 4 ? sgs_rdzf(g, k) : 0.f,
+                                            sizeof(T) == 4 ? sgs_rz2f(g, k) : 0.f)
                              : nu;
     if (cfl_partial) cfl = cfl_cell(g, U[p], U[p - 1], V[p], V[p - sy], W[p], W[p - sz], i, j, k);
   }
@@ -164,58 +177,74 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, const T *__restrict__ U, con
 }
 
 // z-marching form: a block owns a TX x TY tile of columns and walks k up a
-// chunk of levels.  LDS holds rings of (TX+2) x (TY+2) planes (halo 1): U
-// and V at k-1..k+1, W at k-1..k; the planes of level k+1 are loaded into
-// registers while level k is evaluated, then written into the slots level k
-// no longer needs.  Global loads per cell: one plane of U, V and W plus the
-// halo ring, instead of 21 stencil values.
-// RPT: rows per thread (the tile is TX x TY*RPT columns).
+// chunk of levels.  LDS holds rings of (TX+2) x (TY+2) planes (halo 1) of U,
+// V and W, R levels each, level L in slot (L - kbeg + 1) mod R; the z loop is
+// unrolled by R so that every slot is a constant.  The planes of level k+2
+// (U, V) and k+1 (W) are loaded into registers while level k is evaluated
+// and then stored into their slots: with R = 4 (4-byte fields) no thread
+// reads those slots at level k and one barrier per level suffices; with
+// R = 3 (8-byte fields, whose LDS then keeps three blocks per CU) the stores
+// wait for a second barrier.  Global loads per cell: one plane of U, V and W
+// plus the halo ring, instead of 21 stencil values.
+// RPT: rows per thread (the tile is TX x TY*RPT columns).  Launch bounds: 4
+// waves per SIMD for 4-byte fields (128 VGPRs: at 5 waves it spills and runs
+// 181 instead of 166 us on 512x512x128), 3 for 8-byte fields.
 template <typename T, int TX, int TY, int RPT, bool RCP>
-__global__ void __launch_bounds__(TX *TY)
+__global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? 4 : 3)
     k_sgs_zm(Geo g, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W, T *__restrict__ Visc,
              double nu_, double c_smag_, double filter_ratio, SgsWidth width_t, int sgs_model,
              double *__restrict__ cfl_partial, int zchunk) {
   constexpr int NT = TX * TY, RX = TX + 2, RC = RX * (TY * RPT + 2), PER = (RC + NT - 1) / NT;
-  // ring slots (U 0-2, V 3-5, W 6-7) and one spare cell per thread, written
-  // by threads past the region, so the ring writes carry no branches
-  __shared__ T ring[8 * RC + NT];
-  constexpr int SP = 8 * RC;
+  constexpr int R = sizeof(T) == 4 ? 4 : 3;
+  constexpr int SU = 0, SV = R, SW = 2 * R;
+  __shared__ T ring[3 * R * RC];
   const int t = threadIdx.x, tx = t % TX, ty = t / TX;
   const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY * RPT + 1;
   const int kbeg = blockIdx.z * zchunk + 1, kend = min(g.nz, kbeg + zchunk - 1);
   const int i = i0 + tx;
   const T nu = (T)nu_, c_smag = (T)c_smag_;
   // region cells of this thread (halo 1 around the tile), clamped into the
-  // allocation; values of clamped cells only reach cells outside the domain
+  // allocation (values of clamped cells only reach cells outside the
+  // domain); a thread past the region loads and stores the region's last
+  // cell again, the value its owner stores
   long roff[PER];
-  int ridx[PER];  // region cell of the thread, -1 past the region
+  int ridx[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    ridx[q] = t + q * NT < RC ? t + q * NT : -1;
     const int r = min(t + q * NT, RC - 1);
+    ridx[q] = r;
     const int ri = min(i0 - 1 + r % RX, g.nx + 3), rj = min(j0 - 1 + r / RX, g.ny + 3);
     roff[q] = gix(g, ri, rj, 0);
   }
-  auto put = [&](int dst, const T *src, int k) {  // dst: slot offset in ring
+  auto slot = [](int base, int rel) { return base + (rel + 2 * R) % R; };
+  auto put = [&](int s, const T *src, int k) {
 #pragma unroll
-    for (int q = 0; q < PER; q++)
-      ring[ridx[q] >= 0 ? dst + ridx[q] : SP + t] = src[roff[q] + (long)k * g.sz];
+    for (int q = 0; q < PER; q++) ring[s * RC + ridx[q]] = src[roff[q] + (long)k * g.sz];
   };
-  put(0 * RC, U, kbeg - 1);
-  put(1 * RC, U, kbeg);
-  put(2 * RC, U, kbeg + 1);
-  put(3 * RC, V, kbeg - 1);
-  put(4 * RC, V, kbeg);
-  put(5 * RC, V, kbeg + 1);
-  put(6 * RC, W, kbeg - 1);
-  put(7 * RC, W, kbeg);
+  put(slot(SU, -1), U, kbeg - 1);
+  put(slot(SU, 0), U, kbeg);
+  put(slot(SU, 1), U, kbeg + 1);
+  put(slot(SV, -1), V, kbeg - 1);
+  put(slot(SV, 0), V, kbeg);
+  put(slot(SV, 1), V, kbeg + 1);
+  put(slot(SW, -1), W, kbeg - 1);
+  put(slot(SW, 0), W, kbeg);
+  // filter-width class of the thread's columns (constant over k)
+  int cls[RPT];
+#pragma unroll
+  for (int rr = 0; rr < RPT; rr++) {
+    const int j = min(j0 + ty + rr * TY, g.ny);
+    cls[rr] = smag_class(g, width_t, min(i, g.nx), j);
+  }
   __syncthreads();
-  int u0 = 0, u1 = 1, u2 = 2, w0 = 0, w1 = 1;
   double cfl = 0;
-  for (int k = kbeg; k <= kend; k++) {
-    const bool more = k < kend;
-    T pu[PER], pv[PER], pw[PER];
-    if (more) {
+  for (int k0 = kbeg; k0 <= kend; k0 += R) {
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+      const int k = k0 + u;
+      if (k > kend) break;
+      // next planes (levels inside the allocation; unused after the chunk)
+      T pu[PER], pv[PER], pw[PER];
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const long o = roff[q] + (long)(k + 2) * g.sz;
@@ -223,42 +252,33 @@ __global__ void __launch_bounds__(TX *TY)
         pv[q] = V[o];
         pw[q] = W[o - g.sz];
       }
-    }
+      const float rdzk = sizeof(T) == 4 ? width_t.rdzf[k] : 0.f, rz2k = sizeof(T) == 4 ? width_t.rz2f[k] : 0.f;
+      const T *Ul = ring + slot(SU, u - 1) * RC, *Uc = ring + slot(SU, u) * RC, *Uh = ring + slot(SU, u + 1) * RC;
+      const T *Vl = ring + slot(SV, u - 1) * RC, *Vc = ring + slot(SV, u) * RC, *Vh = ring + slot(SV, u + 1) * RC;
+      const T *Wl = ring + slot(SW, u - 1) * RC, *Wc = ring + slot(SW, u) * RC;
 #pragma unroll
-    for (int rr = 0; rr < RPT; rr++) {
-      const int j = j0 + ty + rr * TY;
-      if (i <= g.nx && j <= g.ny) {
-        const int c = (tx + 1) + RX * (ty + rr * TY + 1);
-        const T *Ul = ring + u0 * RC, *Uc = ring + u1 * RC, *Uh = ring + u2 * RC;
-        const T *Vl = ring + (3 + u0) * RC, *Vc = ring + (3 + u1) * RC, *Vh = ring + (3 + u2) * RC;
-        const T *Wl = ring + (6 + w0) * RC, *Wc = ring + (6 + w1) * RC;
-        auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
-        auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
-        auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
-        Visc[gix(g, i, j, k)] =
-            sgs_model == 1 ? smag_visc<T, RCP>(g, Uf, Vf, Wf, k, smag_width<T, true>(g, width_t, filter_ratio, i, j, k),
-                                               nu, c_smag, width_t)
-                           : nu;
-        if (cfl_partial)
-          cfl = fmax(cfl, cfl_cell(g, Uc[c], Uc[c - 1], Vc[c], Vc[c - RX], Wc[c], Wl[c], i, j, k));
+      for (int rr = 0; rr < RPT; rr++) {
+        const int j = j0 + ty + rr * TY;
+        if (i <= g.nx && j <= g.ny) {
+          const int c = (tx + 1) + RX * (ty + rr * TY + 1);
+          auto Uf = [&](int di, int dj, int dk) { return (dk < 0 ? Ul : dk > 0 ? Uh : Uc)[c + di + RX * dj]; };
+          auto Vf = [&](int di, int dj, int dk) { return (dk < 0 ? Vl : dk > 0 ? Vh : Vc)[c + di + RX * dj]; };
+          auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? Wl : Wc)[c + di + RX * dj]; };
+          Visc[gix(g, i, j, k)] =
+              sgs_model == 1 ? smag_visc<T, RCP>(g, Uf, Vf, Wf, k, (T)width_t.tab[cls[rr] * width_t.ld + k], nu,
+                                                 c_smag, width_t, rdzk, rz2k)
+                             : nu;
+          if (cfl_partial)
+            cfl = fmax(cfl, cfl_cell(g, Uc[c], Uc[c - 1], Vc[c], Vc[c - RX], Wc[c], Wl[c], i, j, k));
+        }
       }
-    }
-    if (more) {
-      __syncthreads();
+      if constexpr (R == 3) __syncthreads();
 #pragma unroll
       for (int q = 0; q < PER; q++) {
-        const bool ok = ridx[q] >= 0;
-        ring[ok ? u0 * RC + ridx[q] : SP + t] = pu[q];
-        ring[ok ? (3 + u0) * RC + ridx[q] : SP + t] = pv[q];
-        ring[ok ? (6 + w0) * RC + ridx[q] : SP + t] = pw[q];
+        ring[slot(SU, u + 2) * RC + ridx[q]] = pu[q];
+        ring[slot(SV, u + 2) * RC + ridx[q]] = pv[q];
+        ring[slot(SW, u + 1) * RC + ridx[q]] = pw[q];
       }
-      const int tu = u0;
-      u0 = u1;
-      u1 = u2;
-      u2 = tu;
-      const int tw = w0;
-      w0 = w1;
-      w1 = tw;
       __syncthreads();
     }
   }
@@ -272,9 +292,8 @@ __global__ void __launch_bounds__(TX *TY)
 // thread; 32 x 8 and 64 x 8 threads measured slower), 32 levels per block (16
 // measured equal, 64 12 % slower in FP32: fewer blocks to hide the per-level
 // load latency)
-// z chunk of 26 levels: on 512x512x128 its 2,560 blocks are two full rounds
-// of the 1,280 resident FP32 blocks (5 per CU); measured 237 -> 222 us
-// against 32 levels (ELMM_SGS_ZCHUNK overrides)
+// z chunk of 26 levels (measured 237 -> 222 us against 32 levels with the
+// earlier 5-blocks-per-CU form; ELMM_SGS_ZCHUNK overrides)
 constexpr int kSgsTX = 64, kSgsTY = 4, kSgsRPT = 2, kSgsZChunk = 26;
 static int sgs_zchunk(const Geo &g) { return g.sgs_zchunk > 0 ? g.sgs_zchunk : kSgsZChunk; }
 

@@ -20,12 +20,27 @@ namespace elmm {
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Real = field and arithmetic type (double: FP64 form, float: precision form)
+// FP64 (the reference's arithmetic): the quotient correctly rounded and the
+// reference's comparisons.  FP32 (the precision form): the ratio as a product
+// with the hardware reciprocal (within 2 ulp) and min/max instructions (equal
+// to the comparisons for every non-NaN ratio).
 template <typename Real>
 __device__ inline Real flux_limiter(Real r) {
-  Real a = div_rn_nz<Real>(1 + 2 * r, (Real)3, recip3<Real>());
-  Real b = (Real)2.0 < a ? (Real)2.0 : a;
-  Real c = 2 * r < b ? 2 * r : b;
-  return (Real)0.0 > c ? (Real)0.0 : c;
+  if constexpr (sizeof(Real) == 4) {
+    return fmaxf(0.0f, fminf(2 * r, fminf(2.0f, (1 + 2 * r) * (1.0f / 3.0f))));
+  } else {
+    Real a = div_rn_nz<Real>(1 + 2 * r, (Real)3, recip3<Real>());
+    Real b = (Real)2.0 < a ? (Real)2.0 : a;
+    Real c = 2 * r < b ? 2 * r : b;
+    return (Real)0.0 > c ? (Real)0.0 : c;
+  }
+}
+template <typename Real>
+__device__ inline Real ratio(Real a, Real b) {
+  if constexpr (sizeof(Real) == 4)
+    return a * __builtin_amdgcn_rcpf(b);
+  else
+    return a / b;
 }
 
 // Flux through the face between s0 and s1 with face velocity u, from the four
@@ -42,7 +57,7 @@ __device__ inline Real kappa_flux_v(Real u, Real s_m1, Real s0, Real s1, Real s2
     sl = s1 - s2;
   }
   const Real sg = copysign((Real)1.0, sl);
-  const Real slope = flux_limiter<Real>((sr + eps * sg) / (sl + eps * sg));
+  const Real slope = flux_limiter<Real>(ratio<Real>(sr + eps * sg, sl + eps * sg));
   if (u > 0) return u * (s0 + (s0 - s_m1) * slope / (Real)2.0);
   return u * (s1 + (s1 - s2) * slope / (Real)2.0);
 }
@@ -60,7 +75,7 @@ __device__ inline Real kappa_flux(const Real *__restrict__ S, long p, long s, Re
     sl = S[p + s] - S[p + 2 * s];
   }
   const Real sg = copysign((Real)1.0, sl);
-  const Real slope = flux_limiter<Real>((sr + eps * sg) / (sl + eps * sg));
+  const Real slope = flux_limiter<Real>(ratio<Real>(sr + eps * sg, sl + eps * sg));
   if (u > 0) return u * (S[p] + (S[p] - S[p - s]) * slope / (Real)2.0);
   return u * (S[p + s] + (S[p + s] - S[p + 2 * s]) * slope / (Real)2.0);
 }

@@ -77,6 +77,9 @@ struct SgsWidth {
   int ld;
   double rdx = 0, rdy = 0;
   const double *rdz = nullptr, *rz2 = nullptr;
+  // FP32 level reciprocals RN(1 / dzPr(k)), RN(1 / (2 (zPr(k+1) - zPr(k-1))))
+  // in single precision (the precision form's z-march)
+  const float *rdzf = nullptr, *rz2f = nullptr;
   bool rcp() const { return rdz != nullptr; }
 };
 
