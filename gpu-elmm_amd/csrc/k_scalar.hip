@@ -172,11 +172,12 @@ __global__ void __launch_bounds__(NTS)
                 const Real *__restrict__ T, const Real *__restrict__ S, Real *__restrict__ Snew,
                 Real *__restrict__ Sadv, const uint8_t *__restrict__ code, const uint8_t *__restrict__ mx,
                 const uint8_t *__restrict__ my, const uint8_t *__restrict__ mz, StageConst sc, int kchunk) {
-  // planes plus one spare cell per thread: a thread past the end of a plane
-  // loads a clamped (in-bounds) element and writes its spare cell, so the
-  // plane loads and writes carry no branches
-  __shared__ Real sS[2][SPS + NTS];
-  __shared__ Real sT[2][TPS + NTS];
+  // double-buffered planes, level L in buffer (L - kbeg) & 1 (the level loop
+  // is unrolled by 2, so the buffers are constants); a thread past the end
+  // of a plane loads and stores its last element again (its owner's value),
+  // so the plane loads and stores carry no branches
+  __shared__ Real sS[2][SPS];
+  __shared__ Real sT[2][TPS];
   __shared__ Real fX[TYS][TXS + 1];  // x faces i0-1 .. i0+TXS-1 of each row
   __shared__ Real fY[TYS + 1][TXS];  // y faces j0-1 .. j0+TYS-1 of each column
   const int tid = threadIdx.x, tx = tid % TXS, ty = tid / TXS;
@@ -188,14 +189,14 @@ __global__ void __launch_bounds__(NTS)
   int sidx[SLD], tidx[TLD];
 #pragma unroll
   for (int r = 0; r < SLD; r++) {
-    const int e = tid + r * NTS;
-    sidx[r] = e < SPS ? e : SPS + tid;
+    const int e = min(tid + r * NTS, SPS - 1);
+    sidx[r] = e;
     soff[r] = gix(g, min(i0 - 2 + e % SPW, g.nx + 3), min(j0 - 2 + e / SPW, g.ny + 3), 0);
   }
 #pragma unroll
   for (int r = 0; r < TLD; r++) {
-    const int e = tid + r * NTS;
-    tidx[r] = e < TPS ? e : TPS + tid;
+    const int e = min(tid + r * NTS, TPS - 1);
+    tidx[r] = e;
     toff[r] = gix(g, min(i0 - 1 + e % TPW, g.nx + 3), min(j0 - 1 + e / TPW, g.ny + 3), 0);
   }
   const long pc = gix(g, min(i, g.nx), min(j, g.ny), 0);
@@ -205,10 +206,10 @@ __global__ void __launch_bounds__(NTS)
   // prologue: planes of level kbeg, column rings
 #pragma unroll
   for (int r = 0; r < SLD; r++)
-    sS[kbeg & 1][sidx[r]] = S[soff[r] + (long)kbeg * sz];
+    sS[0][sidx[r]] = S[soff[r] + (long)kbeg * sz];
 #pragma unroll
   for (int r = 0; r < TLD; r++)
-    sT[kbeg & 1][tidx[r]] = T[toff[r] + (long)kbeg * sz];
+    sT[0][tidx[r]] = T[toff[r] + (long)kbeg * sz];
   Real cs[5], ct[3], cw[2];
   uint8_t cm[2];
 #pragma unroll
@@ -248,24 +249,26 @@ __global__ void __launch_bounds__(NTS)
   fetch(pc + (long)kbeg * sz, cur);
   nxt = cur;
 
-  for (int k = kbeg; k <= kend; k++) {
-    const bool more = k < kend;
-    const int b = k & 1;
-    Real ps[SLD], pt[TLD], ns = 0, nt = 0, nw = 0;
-    uint8_t nm = 0;
-    if (more) {
+  for (int k0 = kbeg; k0 <= kend; k0 += 2) {
 #pragma unroll
-      for (int r = 0; r < SLD; r++)
-        ps[r] = S[soff[r] + (long)(k + 1) * sz];
+  for (int b = 0; b < 2; b++) {
+    const int k = k0 + b;
+    if (k > kend) break;
+    // next level's planes and column values (levels inside the allocation;
+    // after the chunk's last level they are loaded and not used)
+    Real ps[SLD], pt[TLD], ns, nt, nw;
+    uint8_t nm;
 #pragma unroll
-      for (int r = 0; r < TLD; r++)
-        pt[r] = T[toff[r] + (long)(k + 1) * sz];
-      ns = S[pc + (long)(k + 3) * sz];
-      nt = T[pc + (long)(k + 2) * sz];
-      nw = W[pc + (long)(k + 1) * sz];
-      nm = mz[pc + (long)(k + 1) * sz];
-      fetch(pc + (long)(k + 1) * sz, nxt);
-    }
+    for (int r = 0; r < SLD; r++)
+      ps[r] = S[soff[r] + (long)(k + 1) * sz];
+#pragma unroll
+    for (int r = 0; r < TLD; r++)
+      pt[r] = T[toff[r] + (long)(k + 1) * sz];
+    ns = S[pc + (long)(k + 3) * sz];
+    nt = T[pc + (long)(k + 2) * sz];
+    nw = W[pc + (long)(k + 1) * sz];
+    nm = mz[pc + (long)(k + 1) * sz];
+    fetch(pc + (long)(k + 1) * sz, nxt);
     const long p = gix(g, min(i, g.nx), min(j, g.ny), k);
     const Real *P = sS[b];
     auto Sx = [&](int o) { return P[ls + o]; };
@@ -320,26 +323,25 @@ __global__ void __launch_bounds__(NTS)
       }
     }
     cur = nxt;
-    if (more) {
 #pragma unroll
-      for (int r = 0; r < SLD; r++)
-        sS[b ^ 1][sidx[r]] = ps[r];
+    for (int r = 0; r < SLD; r++)
+      sS[b ^ 1][sidx[r]] = ps[r];
 #pragma unroll
-      for (int r = 0; r < TLD; r++)
-        sT[b ^ 1][tidx[r]] = pt[r];
+    for (int r = 0; r < TLD; r++)
+      sT[b ^ 1][tidx[r]] = pt[r];
 #pragma unroll
-      for (int q = 0; q < 4; q++) cs[q] = cs[q + 1];
-      cs[4] = ns;
-      ct[0] = ct[1];
-      ct[1] = ct[2];
-      ct[2] = nt;
-      cw[0] = cw[1];
-      cw[1] = nw;
-      cm[0] = cm[1];
-      cm[1] = nm;
-    }
+    for (int q = 0; q < 4; q++) cs[q] = cs[q + 1];
+    cs[4] = ns;
+    ct[0] = ct[1];
+    ct[1] = ct[2];
+    ct[2] = nt;
+    cw[0] = cw[1];
+    cw[1] = nw;
+    cm[0] = cm[1];
+    cm[1] = nm;
     fz_lo = fz_hi;
     __syncthreads();
+  }
   }
 }
 
