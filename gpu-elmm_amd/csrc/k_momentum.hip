@@ -9,6 +9,7 @@
 // 266-295, 450-478), PressureGrad (src/dynamics.f90:58-199) and
 // add(U2,U) (src/tsteps.f90:350-355).  The arithmetic of each value is the
 // reference's, operation by operation (built with -ffp-contract=off).
+#include <algorithm>
 #include <cmath>
 
 #include "device_common.hpp"
@@ -1049,35 +1050,69 @@ __global__ void k_mass_sources(Geo g, TermList tl, const T *__restrict__ U, cons
   Q[gix(g, tl.cell[3 * c], tl.cell[3 * c + 1], tl.cell[3 * c + 2])] = q;
 }
 
-// Bound_Q, src/boundaries.f90:2094-2192: fold ghost layers into the interior,
-// one axis per launch (y reads the x-folded values, z the y-folded ones).
-__global__ void k_fold_q(Geo g, double *__restrict__ Q, int axis, int periodic) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int b = blockIdx.y + 1;
-  const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
-  const int na = axis == 0 ? g.ny : g.nx;
-  if (a > na) return;
-  auto at = [&](int m) -> double & {
-    if (axis == 0) return Q[gix(g, m, a, b)];
-    if (axis == 1) return Q[gix(g, a, m, b)];
-    return Q[gix(g, a, b, m)];
-  };
+// Bound_Q, src/boundaries.f90:2094-2192: the ghost layers of Q fold into
+// the interior in three passes, x (rows j, k in 1..n), then y (columns i, k),
+// then z (columns i, j).  No pass writes a ghost cell, and a pass reads
+// interior cells only where it writes them, so the final value of an
+// interior boundary cell is its own value with the three axes' ghost terms
+// added in pass order; one thread per boundary cell applies them in that
+// order (the same additions, the same results).
+__device__ inline double fold_axis(double v, int m, int n, int periodic, double g0, double g1) {
+  // g0 = ghost 0, g1 = ghost n+1 of the axis
   if (periodic) {
-    at(n) = at(0) + at(n);
-    at(1) = at(1) + at(n + 1);
+    if (m == n) v = g0 + v;
+    if (m == 1) v = v + g1;
   } else {
-    at(1) = at(1) + at(0);
-    at(n) = at(n) + at(n + 1);
+    if (m == 1) v = v + g0;
+    if (m == n) v = v + g1;
   }
+  return v;
+}
+
+// (32-bit unsigned index arithmetic: a boundary shell of the admitted grids
+// has far fewer than 2^32 cells)
+__global__ void k_fold_q(Geo g, double *__restrict__ Q, int px, int py, int pz) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const unsigned fx = nx > 1 ? 2 : 1, fy = ny > 1 ? 2 : 1, fz = nz > 1 ? 2 : 1;
+  const unsigned ix = (unsigned)max(nx - 2, 0), iy = (unsigned)max(ny - 2, 0);
+  const unsigned ca = fx * ny * nz, cb = ix * fy * nz, cc = ix * iy * fz;
+  int i, j, k;
+  if (t < ca) {  // x faces, every j, k
+    const unsigned r = t / fx, q = r / (unsigned)ny;
+    i = t - r * fx ? nx : 1;
+    j = 1 + (int)(r - q * ny);
+    k = 1 + (int)q;
+  } else if (t < ca + cb) {  // y faces, i inside
+    const unsigned u = t - ca, r = u / fy, q = r / ix;
+    j = u - r * fy ? ny : 1;
+    i = 2 + (int)(r - q * ix);
+    k = 1 + (int)q;
+  } else if (t < ca + cb + cc) {  // z faces, i and j inside
+    const unsigned u = t - ca - cb, r = u / fz, q = r / ix;
+    k = u - r * fz ? nz : 1;
+    i = 2 + (int)(r - q * ix);
+    j = 2 + (int)q;
+  } else {
+    return;
+  }
+  const long p = gix(g, i, j, k);
+  double v = Q[p];
+  if (i == 1 || i == nx) v = fold_axis(v, i, nx, px, Q[gix(g, 0, j, k)], Q[gix(g, nx + 1, j, k)]);
+  if (j == 1 || j == ny) v = fold_axis(v, j, ny, py, Q[gix(g, i, 0, k)], Q[gix(g, i, ny + 1, k)]);
+  if (k == 1 || k == nz) v = fold_axis(v, k, nz, pz, Q[gix(g, i, j, 0)], Q[gix(g, i, j, nz + 1)]);
+  Q[p] = v;
 }
 
 template <typename T>
 void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, const T *W, double *Q, hipStream_t st) {
   if (!tl.n) return;
   hipLaunchKernelGGL(k_mass_sources<T>, dim3(nblk(tl.n, 256)), dim3(256), 0, st, g, tl, U, V, W, Q);
-  hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.ny, 64), g.nz), dim3(64), 0, st, g, Q, 0, g.bt[0] == 3);
-  hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.nx, 64), g.nz), dim3(64), 0, st, g, Q, 1, g.bt[2] == 3);
-  hipLaunchKernelGGL(k_fold_q, dim3(nblk(g.nx, 64), g.ny), dim3(64), 0, st, g, Q, 2, g.bt[4] == 3);
+  const int fx = g.nx > 1 ? 2 : 1, fy = g.ny > 1 ? 2 : 1, fz = g.nz > 1 ? 2 : 1;
+  const long n = (long)fx * g.ny * g.nz + (long)std::max(g.nx - 2, 0) * fy * g.nz +
+                 (long)std::max(g.nx - 2, 0) * std::max(g.ny - 2, 0) * fz;
+  hipLaunchKernelGGL(k_fold_q, dim3(nblk(n, 256)), dim3(256), 0, st, g, Q, g.bt[0] == 3, g.bt[2] == 3,
+                     g.bt[4] == 3);
 }
 
 // explicit instantiations: FP64 (reference arithmetic) and FP32 (precision form)

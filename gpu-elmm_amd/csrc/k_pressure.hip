@@ -663,8 +663,17 @@ __global__ void k_top_final(Geo g, double *__restrict__ partial, int nblocks) {
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) buf[b] = partial[b];
   __syncthreads();
   if (threadIdx.x != 0) return;
+  // in block order; eight loads issued ahead of their additions
   double s = 0;
-  for (int b = 0; b < nblocks; b++) s += buf[b];
+  int b = 0;
+  for (; b + 8 <= nblocks; b += 8) {
+    double t[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) t[q] = buf[b + q];
+#pragma unroll
+    for (int q = 0; q < 8; q++) s += t[q];
+  }
+  for (; b < nblocks; b++) s += buf[b];
   partial[nblocks] = s / (double)(g.nx * g.ny);
 }
 
