@@ -619,7 +619,7 @@ void Engine<T>::setup_poisson() {
   if (!use_mg) {
     if (!fft_len_ok(nx) || !fft_len_ok(ny)) throw ArgError("FFT pressure solver needs nx and ny of the form 2^a 3^b");
     if (nx > 1024 || ny > 1024) throw ArgError("FFT pressure solver supports nx, ny <= 1024");
-    if (nz > 256) throw ArgError("z solver supports nz <= 256");
+    if (nz > 1024) throw ArgError("FFT pressure solver supports nz <= 1024");
   }
   const int M = nx / 2 + 1;
   // spectral rows padded to 128 bytes, so that the y pass's 8-mode segments

@@ -40,8 +40,8 @@ __host__ inline Shell make_shell(int n0, int n1, int n2, int gl, int gh, int zi0
 __device__ inline int ghost_index(int t, int gl, int n) { return t < gl ? t - gl + 1 : n + 1 + (t - gl); }
 
 // 32-bit unsigned index arithmetic (a 64-bit division costs several times a
-// 32-bit one).  The engine admits nx, ny <= 1024 and nz <= 256, so a shell
-// has fewer than 2^25 cells.
+// 32-bit one).  A shell of a grid of at most 1024 x 1024 x 1024 cells has
+// fewer than 2^26 cells.
 __device__ inline bool decode(const Shell &s, long t_, int &i, int &j, int &k) {
   const unsigned g = (unsigned)(s.gl + s.gh);
   unsigned t = (unsigned)t_;

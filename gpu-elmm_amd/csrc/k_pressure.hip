@@ -293,13 +293,23 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(PoissonPlan<R> pp, int in
 // at a time into registers from its checkpoint, so the spectral array is
 // read twice and written once, coalesced across the modes of a wavefront.
 // The singular mode (m = j = 0) pins x_1 = 0 and is integrated exactly.
-constexpr int ZSEG = 16, ZSEG_MAX = 16;  // nz <= 256
+// LDS (sized at launch): the checkpoints, 3 x ceil(nz/ZSEG) x 64 values, and
+// the mean mode's partial sums, 2 x nz values.
+constexpr int ZSEG = 16;
+constexpr int kSolveZMaxNz = 1024;  // 112 KiB of LDS at the maximum
+
+__host__ __device__ inline size_t solve_z_lds(int nz) {
+  return (size_t)(3 * ((nz + ZSEG - 1) / ZSEG) * 64 + 2 * nz) * sizeof(double);
+}
 
 template <typename R, bool PIPE>
 __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
-  __shared__ double ck[3][ZSEG_MAX][64];
-  __shared__ double gm[2][256];
+  extern __shared__ double zlds[];
+  const int nseg = (pp.nz + ZSEG - 1) / ZSEG;
   const int t = threadIdx.x;
+  // checkpoint a (0: c', 1: d' real, 2: d' imaginary) of segment s
+  auto ck = [&](int a, int s) -> double & { return zlds[(a * nseg + s) * 64 + t]; };
+  double *gm0 = zlds + 3 * nseg * 64, *gm1 = gm0 + pp.nz;
   // one thread per slot of a padded level (the padding threads return), so
   // that a wavefront's loads are one contiguous segment (measured 5 %
   // faster than a compact mode numbering)
@@ -331,8 +341,8 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
       s2 = Gi + y;
       ci = (s2 - Gi) - y;
       Gi = s2;
-      gm[0][k - 2] = Gr;
-      gm[1][k - 2] = Gi;
+      gm0[k - 2] = Gr;
+      gm1[k - 2] = Gi;
     }
     double xr = 0, xi = 0;
     cr = ci = 0;
@@ -340,10 +350,10 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
 #pragma unroll 8
     for (int k = 1; k <= nz - 1; k++) {
       const double dz = pp.dzW[k];
-      double tt = gm[0][k - 1] * dz, y = tt - cr, s2 = xr + y;
+      double tt = gm0[k - 1] * dz, y = tt - cr, s2 = xr + y;
       cr = (s2 - xr) - y;
       xr = s2;
-      tt = gm[1][k - 1] * dz;
+      tt = gm1[k - 1] * dz;
       y = tt - ci;
       s2 = xi + y;
       ci = (s2 - xi) - y;
@@ -382,9 +392,9 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
         }
       }
       // state at the end of the segment (levels k0 .. k0+ZSEG-1)
-      ck[0][k0 / ZSEG][t] = cp;
-      ck[1][k0 / ZSEG][t] = dr;
-      ck[2][k0 / ZSEG][t] = di;
+      ck(0, k0 / ZSEG) = cp;
+      ck(1, k0 / ZSEG) = dr;
+      ck(2, k0 / ZSEG) = di;
 #pragma unroll
       for (int r = 0; r < ZSEG; r++) cur[r] = nxt[r];
     }
@@ -399,9 +409,9 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
       dr = ((double)x.x - a * dr) * inv;
       di = ((double)x.y - a * di) * inv;
       if (k % ZSEG == ZSEG - 1) {
-        ck[0][k / ZSEG][t] = cp;
-        ck[1][k / ZSEG][t] = dr;
-        ck[2][k / ZSEG][t] = di;
+        ck(0, k / ZSEG) = cp;
+        ck(1, k / ZSEG) = dr;
+        ck(2, k / ZSEG) = di;
       }
     }
   }
@@ -421,9 +431,9 @@ __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {
 #pragma unroll
       for (int r = 0; r < ZSEG; r++) nxt[r] = X[(long)max(k0 - ZSEG + r, 0) * plane];
     }
-    cp = s ? ck[0][s - 1][t] : 0.0;
-    dr = s ? ck[1][s - 1][t] : 0.0;
-    di = s ? ck[2][s - 1][t] : 0.0;
+    cp = s ? ck(0, s - 1) : 0.0;
+    dr = s ? ck(1, s - 1) : 0.0;
+    di = s ? ck(2, s - 1) : 0.0;
 #pragma unroll
     for (int r = 0; r < ZSEG; r++) {
       const int k = k0 + r;
@@ -586,14 +596,18 @@ void launch_fft_y(const PoissonPlan<R> &pp, int inverse, hipStream_t st) {
 
 template <typename R>
 void launch_poisson_solve_z(const Geo &, const PoissonPlan<R> &pp, hipStream_t st) {
-  // nz <= ZSEG * ZSEG_MAX (256) is checked when the plan is set up
+  // nz <= kSolveZMaxNz is checked when the plan is set up
   const long modes = (long)pp.Mp * pp.ny;
   const unsigned blocks = (unsigned)((modes + 63) / 64);
+  const size_t sh = solve_z_lds(pp.nz);
   // one wavefront per block: more than 1024 fill several per CU
-  if (pp.zpipe == 1 || (pp.zpipe < 0 && blocks > 1024))
-    hipLaunchKernelGGL((k_solve_z<R, true>), dim3(blocks), dim3(64), 0, st, pp);
-  else
-    hipLaunchKernelGGL((k_solve_z<R, false>), dim3(blocks), dim3(64), 0, st, pp);
+  if (pp.zpipe == 1 || (pp.zpipe < 0 && blocks > 1024)) {
+    allow_lds((const void *)k_solve_z<R, true>, sh);
+    hipLaunchKernelGGL((k_solve_z<R, true>), dim3(blocks), dim3(64), sh, st, pp);
+  } else {
+    allow_lds((const void *)k_solve_z<R, false>, sh);
+    hipLaunchKernelGGL((k_solve_z<R, false>), dim3(blocks), dim3(64), sh, st, pp);
+  }
 }
 
 template <typename R>

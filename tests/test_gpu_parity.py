@@ -211,6 +211,21 @@ def test_variants_across_z_segments(require_gpu, monkeypatch, env, precision):
         assert max(err.values()) == 0.0, (env, err)
 
 
+@pytest.mark.parametrize("zpipe", ["0", "1"])
+def test_deep_z_beyond_256_levels(require_gpu, monkeypatch, zpipe):
+    """nz = 300: 19 segments of the z solve, the last one partial, with its
+    checkpoints and the mean mode's sums in LDS sized at launch (the limit
+    was nz <= 256); FP64 bit-identical to the oracle after 2 steps, with and
+    without the pipelined z solve."""
+    monkeypatch.setenv("ELMM_ZPIPE", zpipe)
+    cfg = cases.uniform_config(32, 32, 300, dx=1.0, dt=0.05, nscalars=1)
+    o, s = pair(cfg, seed=9, cuboids=OBSTACLES, sources=[(4.5, 4.5, 0.5, 1.0, 0)], u0=3.0)
+    o.step(2)
+    s.step(2)
+    err = compare_state(s, o, cfg)
+    assert max(err.values()) == 0.0, err
+
+
 def test_autotuned_tiles_give_the_same_bits(require_gpu):
     """The autotuner times the candidate tiles on the first step and keeps the
     fastest; the results equal the default launch bit for bit, and the chosen
