@@ -78,6 +78,22 @@ __device__ inline T recip3() {
   return sizeof(T) == 4 ? (T)(1.0f / 3.0f) : (T)(1.0 / 3.0);
 }
 
+// Host operations that HIP refuses in every thread while any stream is
+// being captured into a graph (allocation, free, synchronous copies, event
+// and attribute changes) hold this lock shared; a graph capture holds it
+// exclusively.  Re-entrant within a thread, and a no-op in the capturing
+// thread itself (which makes no such call during its capture).
+class HostOpLock {
+ public:
+  HostOpLock();
+  ~HostOpLock();
+  HostOpLock(const HostOpLock &) = delete;
+  HostOpLock &operator=(const HostOpLock &) = delete;
+
+ private:
+  bool own_;
+};
+
 // Raises a kernel's dynamic-LDS limit above 64 KiB once per kernel and size
 // (host; later launches, including those recorded into a graph, make no
 // attribute call).

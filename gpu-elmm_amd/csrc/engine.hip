@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -32,6 +33,37 @@
 #include "kernels.hpp"
 
 namespace elmm {
+
+// ------------------------------------------- host operations vs. capture
+namespace {
+std::shared_mutex g_capture_mu;
+thread_local bool t_capturing = false;
+thread_local int t_host_ops = 0;
+
+// held by a thread around one graph capture
+struct CaptureLock {
+  CaptureLock() {
+    g_capture_mu.lock();
+    t_capturing = true;
+  }
+  ~CaptureLock() {
+    t_capturing = false;
+    g_capture_mu.unlock();
+  }
+  CaptureLock(const CaptureLock &) = delete;
+  CaptureLock &operator=(const CaptureLock &) = delete;
+};
+}  // namespace
+
+HostOpLock::HostOpLock() : own_(!t_capturing && t_host_ops == 0) {
+  if (own_) g_capture_mu.lock_shared();
+  t_host_ops++;
+}
+HostOpLock::~HostOpLock() {
+  t_host_ops--;
+  if (own_) g_capture_mu.unlock_shared();
+}
+
 
 #define HIPCHK(x)                                                                      \
   do {                                                                                 \
@@ -250,6 +282,7 @@ struct EngineIface {
       event_pool.pop_back();
       return e;
     }
+    HostOpLock host;
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
     return e;
@@ -861,6 +894,9 @@ void Engine<T>::setup_ghost_delta() {
   mark.alloc((size_t)g.ntot);
   tmp.alloc((size_t)cap * 2 * sizeof(long long));
   cnt.alloc(sizeof(int));
+  // the IB lists and these buffers were filled on the null stream; the
+  // kernels below run on the handle's stream
+  HIPCHK(hipStreamSynchronize(nullptr));
   std::vector<long long> all;
   for (int c = 0; c < 3; c++) {
     const int n = ghost_delta_build(g, c, ib_tgt[c].as<int32_t>(), ib_n[c], mark.as<uint8_t>(), tmp.as<long long>(),
@@ -1267,7 +1303,9 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
   setup_lists(L);
   tr.mark("lists");
   setup_poisson();
-  HIPCHK(hipDeviceSynchronize());
+  // the set-up's fills and uploads run on the null stream: drain it (not the
+  // device, whose other handles may be stepping)
+  HIPCHK(hipStreamSynchronize(nullptr));
   tr.mark("poisson");
 }
 
@@ -1645,6 +1683,10 @@ int Engine<T>::replay_steps(int nsteps) {
   if (!use_graphs || timers_on || nsteps < 2 || !first_step_done || cfg.variable_dt || use_mg) return 0;
   const int par = u == U.as<T>() ? 0 : 1;
   if (!graph_exec[par]) {
+    // (a host operation of this thread still open would block the capture
+    // lock: enqueue directly then)
+    if (t_host_ops > 0) return 0;
+    CaptureLock capture;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < 2; r++)
       for (int st = 0; st < 3; st++) stage(st, false);
@@ -1707,6 +1749,7 @@ std::map<std::tuple<int, int, int, int, int>, TileChoice> g_tune_cache;
 
 template <typename T>
 void Engine<T>::autotune() {
+  HostOpLock host;  // events created and destroyed
   tuned = true;
   const auto key = std::make_tuple((int)sizeof(T), cfg.nx, cfg.ny, cfg.nz, nsc);
   TileChoice c{mom_tile, zchunk, g.sgs_zchunk, zchunk_scalar};
@@ -2019,6 +2062,7 @@ extern "C" int elmm_create(const elmm_config *cfg, const elmm_lists *lists, elmm
   *out = nullptr;
   if (cfg->precision != ELMM_FP64 && cfg->precision != ELMM_FP32_REFINED)
     return elmm::fail(ELMM_E_ARG, "precision must be ELMM_FP64 or ELMM_FP32_REFINED");
+  elmm::HostOpLock host;
   auto h = std::make_unique<elmm_handle>();
   if (cfg->precision == ELMM_FP64)
     h->e.reset(new elmm::Engine<double>());
@@ -2043,16 +2087,19 @@ extern "C" int elmm_create_case(const elmm_config *cfg, const elmm_case *c, elmm
 
 extern "C" int elmm_upload(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], const double *src) {
   if (!h || !ext || !src) return elmm::fail(ELMM_E_ARG, "null argument to elmm_upload");
+  elmm::HostOpLock host;
   return guarded([&] { h->e->xfer(field, lb, ext, const_cast<double *>(src), true); });
 }
 
 extern "C" int elmm_download(elmm_handle *h, int32_t field, int32_t lb, const int32_t ext[3], double *dst) {
   if (!h || !ext || !dst) return elmm::fail(ELMM_E_ARG, "null argument to elmm_download");
+  elmm::HostOpLock host;
   return guarded([&] { h->e->xfer(field, lb, ext, dst, false); });
 }
 
 extern "C" int elmm_init(elmm_handle *h) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  elmm::HostOpLock host;
   return guarded([&] { h->e->init(); });
 }
 
@@ -2130,6 +2177,7 @@ extern "C" int elmm_query(elmm_handle *h, int32_t *idle) {
 
 extern "C" int elmm_divergence(elmm_handle *h, double *max_div) {
   if (!h || !max_div) return elmm::fail(ELMM_E_ARG, "null argument to elmm_divergence");
+  elmm::HostOpLock host;
   return guarded([&] { *max_div = h->e->divergence(); });
 }
 
@@ -2144,11 +2192,13 @@ extern "C" int elmm_divergence_noalloc(elmm_handle *h, double *max_div) {
   double *ptr = scratch_of(h->e->device);
   if (!ptr) return elmm::fail(ELMM_E_STATE, "elmm_divergence_noalloc: no scratch address for the handle's device");
   // slots [0, kNoallocSlots): block maxima; slot kNoallocSlots: the result
+  elmm::HostOpLock host;
   return guarded([&] { *max_div = h->e->divergence_into(ptr, kNoallocSlots, ptr + kNoallocSlots); });
 }
 
 extern "C" int elmm_wm_ustar(elmm_handle *h, int32_t cap, double *ustar, int32_t *n) {
   if (!h || !n) return elmm::fail(ELMM_E_ARG, "null argument to elmm_wm_ustar");
+  elmm::HostOpLock host;
   return guarded([&] {
     const int m = h->e->point_ustar(nullptr, 0);
     *n = m;
@@ -2162,6 +2212,7 @@ extern "C" int elmm_wm_ustar(elmm_handle *h, int32_t cap, double *ustar, int32_t
 
 extern "C" int elmm_timers_enable(elmm_handle *h, int32_t enable) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  elmm::HostOpLock host;
   return guarded([&] {
     HIPCHK(hipStreamSynchronize(h->e->stream));
     h->e->collect_timers();
@@ -2171,6 +2222,7 @@ extern "C" int elmm_timers_enable(elmm_handle *h, int32_t enable) {
 
 extern "C" int elmm_timers_read(elmm_handle *h, elmm_timers *out, int32_t reset) {
   if (!h || !out) return elmm::fail(ELMM_E_ARG, "null argument to elmm_timers_read");
+  elmm::HostOpLock host;
   return guarded([&] {
     HIPCHK(hipStreamSynchronize(h->e->stream));
     h->e->collect_timers();
@@ -2195,11 +2247,13 @@ extern "C" const char *elmm_timer_name(int32_t g) {
 extern "C" int elmm_frame_async(elmm_handle *h, const char *dir) {
   if (!h || !dir) return elmm::fail(ELMM_E_ARG, "null argument to elmm_frame_async");
   if (h->e->cfg.nscalars > 8) return elmm::fail(ELMM_E_ARG, "frames hold at most 8 scalars");
+  elmm::HostOpLock host;
   return guarded([&] { h->e->frame_async(dir); });
 }
 
 extern "C" int elmm_frame_wait(elmm_handle *h) {
   if (!h) return elmm::fail(ELMM_E_ARG, "null handle");
+  elmm::HostOpLock host;
   const int rc = h->e->frame_wait();
   if (rc) return elmm::fail(rc, static_cast<elmm::EngineIface *>(h->e.get())->frame_error_text());
   return 0;
@@ -2213,6 +2267,7 @@ extern "C" int elmm_get_tiles(elmm_handle *h, elmm_tiles *out) {
 
 extern "C" int elmm_destroy(elmm_handle *h) {
   if (!h) return 0;
+  elmm::HostOpLock host;  // frees
   release_engine(h->e.get());
   delete h;
   return 0;

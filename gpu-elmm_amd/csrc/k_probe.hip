@@ -34,6 +34,7 @@ __global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ s
 
 namespace elmm {
 bool lds_limit_raise(const void *fn, size_t bytes) {
+  HostOpLock host;
   static std::mutex mu;
   static std::map<const void *, size_t> done;
   std::lock_guard<std::mutex> lock(mu);
@@ -47,6 +48,7 @@ bool lds_limit_raise(const void *fn, size_t bytes) {
 
 extern "C" int elmm_copy_probe(int64_t nbytes, int32_t reps, double *gbps) {
   if (nbytes < 32 || reps < 1 || !gbps) return elmm::fail(ELMM_E_ARG, "elmm_copy_probe: bad arguments");
+  elmm::HostOpLock host;
   const long n = (long)(nbytes / 32) * 2;  // float4 elements (16 bytes), a multiple of 32 bytes in total
   void *a = nullptr, *b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;

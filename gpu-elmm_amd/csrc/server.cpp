@@ -8,12 +8,20 @@
 // (results copied to the caller's buffers, memory returned to the budget) and
 // admits queued requests as memory frees up.  Latency = completion - submit.
 //
+// Admission (handle creation with its list set-up, upload, initial
+// projection, first chunk) and retirement (downloads, divergence, release)
+// run on worker threads, at most kWorkers of each at a time, so that the
+// host work of one request overlaps the steps of the others instead of
+// holding up their next chunks.  A handle is used by one thread at a time:
+// its worker until the task ends, then the thread that calls progress().
+//
 // No counterpart in the reference, which runs one case per process
 // (src/main.f90); this is the serving form of the step.
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <future>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -48,7 +56,13 @@ struct Request {
   int status = 0;
   double latency = 0;
   double max_div = 0;
+  std::string error;      // message of a failure (taken on the failing thread)
+  std::future<int> task;  // admission or retirement in flight on a worker
 };
+
+constexpr int kWorkers = 4;
+
+bool task_ready(Request &r) { return r.task.wait_for(std::chrono::seconds(0)) == std::future_status::ready; }
 
 }  // namespace
 
@@ -57,7 +71,7 @@ struct elmm_server {
   int max_active = 0, chunk = 0;
   int64_t next_id = 1;
   std::deque<std::unique_ptr<Request>> queue;
-  std::vector<std::unique_ptr<Request>> active;
+  std::vector<std::unique_ptr<Request>> preparing, active, retiring;
   std::unordered_map<int64_t, std::unique_ptr<Request>> done;
   int64_t cell_steps = 0;
 };
@@ -67,6 +81,7 @@ namespace {
 int fail_req(Request &r, int rc) {
   r.finished = true;
   r.status = rc;
+  r.error = elmm::last_error();
   if (r.h) {
     elmm_destroy(r.h);
     r.h = nullptr;
@@ -74,8 +89,8 @@ int fail_req(Request &r, int rc) {
   return rc;
 }
 
-// create + upload + init, then the first chunk
-int admit(elmm_server *s, Request &r) {
+// create + upload + init, then the first chunk (a worker thread)
+int admit(int chunk, Request &r) {
   int rc = elmm_create_case(&r.cfg, r.c, &r.h);
   if (rc) return fail_req(r, rc);
   const int32_t e[3] = {r.cfg.nx + 6, r.cfg.ny + 6, r.cfg.nz + 6};
@@ -88,13 +103,14 @@ int admit(elmm_server *s, Request &r) {
   r.U = r.V = r.W = r.Pr = std::vector<double>();
   r.S.clear();
   if ((rc = elmm_init(r.h))) return fail_req(r, rc);
-  const int32_t n = std::min(s->chunk, r.nsteps);
+  const int32_t n = std::min(chunk, r.nsteps);
   if (n > 0 && (rc = elmm_step_async(r.h, n))) return fail_req(r, rc);
   r.done = n;
   return 0;
 }
 
-int retire(elmm_server *s, Request &r) {
+// results to the caller's buffers, divergence, release (a worker thread)
+int retire(Request &r) {
   const int32_t e[3] = {r.cfg.nx + 6, r.cfg.ny + 6, r.cfg.nz + 6};
   int rc = elmm_wait(r.h, nullptr);
   if (!rc && r.oU) rc = elmm_download(r.h, ELMM_F_U, -2, e, r.oU);
@@ -107,7 +123,7 @@ int retire(elmm_server *s, Request &r) {
   // a NaN or infinite divergence is a diverged run even without the CFL test
   if (!rc && !std::isfinite(r.max_div)) rc = elmm::fail(ELMM_E_DIVERGED, "Solution diverged (non-finite divergence)");
   r.latency = std::chrono::duration<double>(Clock::now() - r.submitted).count();
-  s->cell_steps += (int64_t)r.cfg.nx * r.cfg.ny * r.cfg.nz * r.nsteps;
+  if (rc) r.error = elmm::last_error();
   elmm_destroy(r.h);
   r.h = nullptr;
   r.finished = true;
@@ -215,7 +231,27 @@ int elmm_server_submit(elmm_server *s, const elmm_request *q, int64_t *id) {
 
 int elmm_server_progress(elmm_server *s, int32_t *n_active, int32_t *n_queued) {
   if (!s) return elmm::fail(ELMM_E_ARG, "null server");
-  // advance or retire the active cases
+  // a failed request's message becomes this thread's last error
+  auto finish = [&](std::unique_ptr<Request> r) {
+    if (r->status) (void)elmm::fail(r->status, "request " + std::to_string(r->id) + ": " + r->error);
+    s->used -= r->footprint;
+    s->done.emplace(r->id, std::move(r));
+  };
+  // admissions that have ended
+  for (size_t a = 0; a < s->preparing.size();) {
+    if (!task_ready(*s->preparing[a])) {
+      a++;
+      continue;
+    }
+    std::unique_ptr<Request> r = std::move(s->preparing[a]);
+    s->preparing.erase(s->preparing.begin() + a);
+    r->task.get();
+    if (r->finished)
+      finish(std::move(r));
+    else
+      s->active.push_back(std::move(r));
+  }
+  // advance the active cases or hand them to a retirement worker
   for (size_t a = 0; a < s->active.size();) {
     Request &r = *s->active[a];
     int32_t idle = 0;
@@ -224,39 +260,54 @@ int elmm_server_progress(elmm_server *s, int32_t *n_active, int32_t *n_queued) {
       fail_req(r, rc);
     } else if (idle) {
       if (r.done >= r.nsteps) {
-        retire(s, r);
-      } else {
-        const int32_t k = std::min(s->chunk, r.nsteps - r.done);
-        rc = elmm_step_async(r.h, k);
-        if (rc)
-          fail_req(r, rc);
-        else
-          r.done += k;
+        if ((int)s->retiring.size() >= kWorkers) {
+          a++;
+          continue;
+        }
+        Request *rp = &r;
+        r.task = std::async(std::launch::async, [rp] { return retire(*rp); });
+        s->retiring.push_back(std::move(s->active[a]));
+        s->active.erase(s->active.begin() + a);
+        continue;
       }
+      const int32_t k = std::min(s->chunk, r.nsteps - r.done);
+      rc = elmm_step_async(r.h, k);
+      if (rc)
+        fail_req(r, rc);
+      else
+        r.done += k;
     }
     if (r.finished) {
-      s->used -= r.footprint;
-      s->done.emplace(r.id, std::move(s->active[a]));
+      finish(std::move(s->active[a]));
       s->active.erase(s->active.begin() + a);
     } else {
       a++;
     }
   }
-  // admit in arrival order while memory and slots allow
-  while (!s->queue.empty() && (int)s->active.size() < s->max_active &&
-         s->used + s->queue.front()->footprint <= s->budget) {
+  // retirements that have ended
+  for (size_t a = 0; a < s->retiring.size();) {
+    if (!task_ready(*s->retiring[a])) {
+      a++;
+      continue;
+    }
+    std::unique_ptr<Request> r = std::move(s->retiring[a]);
+    s->retiring.erase(s->retiring.begin() + a);
+    r->task.get();
+    s->cell_steps += (int64_t)r->cfg.nx * r->cfg.ny * r->cfg.nz * r->nsteps;
+    finish(std::move(r));
+  }
+  // admit in arrival order while memory, slots and workers allow
+  while (!s->queue.empty() && (int)(s->preparing.size() + s->active.size() + s->retiring.size()) < s->max_active &&
+         (int)s->preparing.size() < kWorkers && s->used + s->queue.front()->footprint <= s->budget) {
     std::unique_ptr<Request> r = std::move(s->queue.front());
     s->queue.pop_front();
     s->used += r->footprint;
-    admit(s, *r);
-    if (r->finished) {
-      s->used -= r->footprint;
-      s->done.emplace(r->id, std::move(r));
-    } else {
-      s->active.push_back(std::move(r));
-    }
+    Request *rp = r.get();
+    const int chunk = s->chunk;
+    r->task = std::async(std::launch::async, [rp, chunk] { return admit(chunk, *rp); });
+    s->preparing.push_back(std::move(r));
   }
-  if (n_active) *n_active = (int32_t)s->active.size();
+  if (n_active) *n_active = (int32_t)(s->preparing.size() + s->active.size() + s->retiring.size());
   if (n_queued) *n_queued = (int32_t)s->queue.size();
   return 0;
 }
@@ -267,11 +318,12 @@ int elmm_server_status(elmm_server *s, int64_t id, elmm_request_status *st) {
   auto it = s->done.find(id);
   if (it == s->done.end()) {
     st->state = 0;
-    for (auto &r : s->active)
-      if (r->id == id) {
-        st->state = 1;
-        st->steps_done = r->done;
-      }
+    for (auto *list : {&s->preparing, &s->active, &s->retiring})
+      for (auto &r : *list)
+        if (r->id == id) {
+          st->state = 1;
+          st->steps_done = r->done;
+        }
     return 0;
   }
   const Request &r = *it->second;
@@ -292,11 +344,15 @@ int elmm_server_stats(elmm_server *s, int64_t *cell_steps, int64_t *hbm_in_use) 
 
 int elmm_server_destroy(elmm_server *s) {
   if (!s) return 0;
-  for (auto &r : s->active)
-    if (r->h) {
-      elmm_wait(r->h, nullptr);
-      elmm_destroy(r->h);
-    }
+  // workers first (a retirement releases its own handle)
+  for (auto &r : s->preparing) r->task.wait();
+  for (auto &r : s->retiring) r->task.wait();
+  for (auto *list : {&s->preparing, &s->active})
+    for (auto &r : *list)
+      if (r->h) {
+        elmm_wait(r->h, nullptr);
+        elmm_destroy(r->h);
+      }
   delete s;
   return 0;
 }

@@ -216,7 +216,8 @@ def test_c5_serving_stream_mix_equals_alone(require_gpu):
     print("serving stream: cells " + ", ".join(str(c.nx * c.ny * c.nz) for c, _, _, _ in reqs), flush=True)
     for rid, (cfg, case, fields, nsteps) in zip(ids, reqs):
         st = srv.status(rid)
-        assert st.state == 2 and st.steps_done == nsteps and np.isfinite(st.max_div)
+        assert st.state == 2 and st.steps_done == nsteps and np.isfinite(st.max_div), \
+            (st.state, st.status, capi._lib.elmm_last_error().decode())
         U, V, W, Pr, S = srv.result(rid)
         s = capi.Solver(cfg, case)
         cases.upload_initial(s, cfg, fields)
