@@ -66,5 +66,10 @@ int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cub
 // Runs f(0) .. f(n-1) on n threads (f(0) on the caller's) and joins them; an
 // exception in a task is rethrown after every task has finished.
 void run_tasks(int n, const std::function<void(int)> &f);
+// The calling thread's current HIP device, and making d current on it (the
+// server's worker threads run on the device the server was created on).
+// Defined in engine.hip.
+int current_device();
+void use_device(int d);
 
 }  // namespace elmm

@@ -15,6 +15,8 @@
 #include <cstdlib>
 #include <type_traits>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -65,11 +67,13 @@ HostOpLock::~HostOpLock() {
 }
 
 
+
 #define HIPCHK(x)                                                                      \
   do {                                                                                 \
     hipError_t e_ = (x);                                                               \
     if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
 
 struct HipError {
   std::string msg;
@@ -79,6 +83,42 @@ struct ArgError {
   std::string msg;
   explicit ArgError(std::string m) : msg(std::move(m)) {}
 };
+
+namespace {
+// A thread started by a thread that holds a HostOpLock, and joined before
+// that lock is released, works under it.
+struct HostOpInherit {
+  HostOpInherit() { t_host_ops++; }
+  ~HostOpInherit() { t_host_ops--; }
+};
+
+// Runs the tasks on threads of their own, on device dev and under the
+// calling thread's HostOpLock, and rethrows the first task's exception.
+void run_parallel(std::vector<std::function<void()>> &tasks, int dev) {
+  std::vector<std::exception_ptr> err(tasks.size());
+  std::vector<std::thread> th;
+  th.reserve(tasks.size());
+  for (size_t q = 0; q < tasks.size(); q++)
+    th.emplace_back([&, q] {
+      HostOpInherit inherit;
+      try {
+        HIPCHK(hipSetDevice(dev));
+        tasks[q]();
+      } catch (...) {
+        err[q] = std::current_exception();
+      }
+    });
+  for (auto &t : th) t.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
+}
+}  // namespace
+
+int current_device() {
+  int d = 0;
+  return hipGetDevice(&d) == hipSuccess ? d : 0;
+}
+void use_device(int d) { (void)hipSetDevice(d); }
 
 static const double RK_alpha[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
 static const double RK_beta[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
@@ -936,26 +976,6 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
         for (int i = 0; i < cfg.nx + 6; i++) o[i] = f(s[i]);
       }
   };
-  std::vector<uint8_t> codes[3];
-  const int32_t *types[3] = {L.utype, L.vtype, L.wtype};
-  for (int c = 0; c < 3; c++) {
-    codes[c].assign(ntot, 0);
-    remap(codes[c].data(), types[c], [](int32_t t) -> uint8_t { return t == 0 ? 0 : t < 0 ? 1 : 2; });
-  }
-  {
-    std::vector<int32_t> pt(ntot, 0);
-    remap(pt.data(), L.prtype, [](int32_t t) { return t; });
-    upload_vec(prtype, pt);
-  }
-  std::vector<uint8_t> scode(ntot, 0);
-  remap(scode.data(), L.prtype, [](int32_t t) -> uint8_t { return t > 0 ? CODE_SOLID : 0; });
-  for (int d = 0; d < 3; d++) {
-    std::vector<uint8_t> m(ntot, 0);
-    remap(m.data(), L.scfl_mask[d], [](uint8_t t) { return t; });
-    upload_vec(mask[d], m);
-  }
-  tr.mark(" masks");
-
   // wall-model points of the 18 momentum lists, concatenated in list order
   std::vector<int32_t> wijk, wlist;
   std::vector<double> wdist, wz0, wustar;
@@ -987,245 +1007,273 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   wm_fluxp.alloc(std::max(wm_n, 1) * sizeof(double));
   wm_fluxm.alloc(std::max(wm_n, 1) * sizeof(double));
 
-  // ordered momentum corrections (wmfluxes-nobranch-inc.f90:49-208)
-  if (cfg.wall_model) {
-    for (int c = 0; c < 3; c++) {
-      TermBuilder tb(ntot);
-      const int *n = g.n[c];
-      for (int dir = 0; dir < 6; dir++) {
-        const int l = c * 6 + dir, ax = dir / 2, plus = dir % 2;
-        for (int pass = 0; pass < 2; pass++)
-          for (int p = 0; p < L.wm_n[l]; p++) {
-            int q[3] = {L.wm_ijk[l][3 * p], L.wm_ijk[l][3 * p + 1], L.wm_ijk[l][3 * p + 2]};
-            int f[3] = {q[0], q[1], q[2]}, t[3] = {q[0], q[1], q[2]};
-            if (!plus) {
-              f[ax] -= 1;
-              if (pass == 1) t[ax] -= 1;
-            } else if (pass == 0) {
-              t[ax] += 1;
-            }
-            const int neg = pass == 1;
-            if (t[0] < 1 || t[0] > n[0] || t[1] < 1 || t[1] > n[1] || t[2] < 1 || t[2] > n[2]) continue;
-            long key = lin(t[0], t[1], t[2]);
-            // last slot: level of the face's lower cell (stretched-z terms)
-            tb.add(key, t[0], t[1], t[2],
-                   {ax | (neg << 2), list_off[l] + p, (int32_t)lin(f[0], f[1], f[2]), f[2]});
-            codes[c][key] |= CODE_CORRECTED;
-          }
-      }
-      tb.upload(mom_terms[c]);
-    }
-  }
-  for (int c = 0; c < 3; c++) upload_vec(code[c], codes[c]);
-  tr.mark(" wall model");
+  tr.mark(" wall model points");
 
-  // scalar wall-model points and ordered scalar terms
-  wmp_n = cfg.wall_model ? L.wmp_n : 0;
-  std::vector<int32_t> pijk;
-  for (int p = 0; p < wmp_n; p++) pijk.insert(pijk.end(), {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]});
-  upload_vec(wmp_ijk, pijk);
-  wmp_div.alloc(std::max(wmp_n, 1) * sizeof(double));
-  {
-    // WMPoints' distance vectors, roughness, log(|dist|/z0) and u* (initially
-    // 1, the WMpoint default, src/wallmodels.f90:40)
-    std::vector<double> pd(L.wmp_dist, L.wmp_dist + 3 * (size_t)wmp_n), pz(L.wmp_z0, L.wmp_z0 + wmp_n),
-        plog(wmp_n, 0.0), pu(wmp_n, 1.0);
-    for (int p = 0; p < wmp_n; p++) {
-      const double *d = &pd[3 * p];
-      const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-      if (pz[p] > 0) plog[p] = std::log(dist / pz[p]);
-    }
-    upload_vec(wmp_dist, pd);
-    upload_vec(wmp_z0, pz);
-    upload_vec(wmp_logdz0, plog);
-    upload_vec(wmp_ustar, pu);
-  }
-  std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
-  upload_vec(src_flux, sflux);
-  if (tslot >= 0) {
-    // solid cells of -1..n+2 with a fluid neighbour there: their temperature
-    // is the mean of those neighbours after each stage
-    const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
-    const long hy = nx + 6, hz = (long)(nx + 6) * (ny + 6);
-    auto pt = [&](int i, int j, int k) { return L.prtype[(i + 2) + (j + 2) * hy + (k + 2) * hz]; };
-    std::vector<int32_t> fix;
-    for (int k = -1; k <= nz + 2; k++)
-      for (int j = -1; j <= ny + 2; j++)
-        for (int i = -1; i <= nx + 2; i++) {
-          if (pt(i, j, k) <= 0) continue;
-          const bool any = (i > -1 && pt(i - 1, j, k) <= 0) || (i < nx + 2 && pt(i + 1, j, k) <= 0) ||
-                           (j > -1 && pt(i, j - 1, k) <= 0) || (j < ny + 2 && pt(i, j + 1, k) <= 0) ||
-                           (k > -1 && pt(i, j, k - 1) <= 0) || (k < nz + 2 && pt(i, j, k + 1) <= 0);
-          if (any) fix.insert(fix.end(), {i, j, k});
-        }
-    tfix_n = (int)(fix.size() / 3);
-    upload_vec(tfix_ijk, fix);
-  }
-  for (int s = 0; s < nsc; s++) sc_terms.emplace_back(new DevTerms);
-  if (nsc > 0) {
-    std::vector<TermBuilder> tbs;
-    for (int s = 0; s < nsc; s++) tbs.emplace_back(ntot);
-    auto interior = [&](int i, int j, int k) {
-      return i >= 1 && i <= cfg.nx && j >= 1 && j <= cfg.ny && k >= 1 && k <= cfg.nz;
-    };
-    for (int s = 0; s < nsc; s++) {
-      TermBuilder &tb = tbs[s];
-      for (int d = 0; d < 3; d++)
-        for (int pass = 0; pass < 2; pass++)
-          for (int p = 0; p < L.scfl_n[d]; p++) {
-            int q[3] = {L.scfl_ijk[d][3 * p], L.scfl_ijk[d][3 * p + 1], L.scfl_ijk[d][3 * p + 2]};
-            int t[3] = {q[0], q[1], q[2]};
-            if (pass == 1) t[d] += 1;
-            if (!interior(t[0], t[1], t[2])) continue;
-            tb.add(lin(t[0], t[1], t[2]), t[0], t[1], t[2],
-                   {(pass == 0 ? 0 : 1) | (d << 2), (int32_t)lin(q[0], q[1], q[2]), 0, 0});
+  // the rest on threads of their own: per velocity component its advection
+  // codes and ordered wall-model terms; Prtype and the scalar flux masks; the
+  // scalar code and ordered scalar terms; the IB lists; the mass-source terms
+  // and nullified points (disjoint members, read-only inputs)
+  std::vector<uint8_t> codes[3];
+  const int32_t *types[3] = {L.utype, L.vtype, L.wtype};
+  std::vector<std::function<void()>> tasks;
+  for (int c = 0; c < 3; c++)
+    tasks.emplace_back([&, c] {
+      codes[c].assign(ntot, 0);
+      remap(codes[c].data(), types[c], [](int32_t t) -> uint8_t { return t == 0 ? 0 : t < 0 ? 1 : 2; });
+      // ordered momentum corrections (wmfluxes-nobranch-inc.f90:49-208)
+      if (cfg.wall_model) {
+        {  // (one component's builder)
+          TermBuilder tb(ntot);
+          const int *n = g.n[c];
+          for (int dir = 0; dir < 6; dir++) {
+            const int l = c * 6 + dir, ax = dir / 2, plus = dir % 2;
+            for (int pass = 0; pass < 2; pass++)
+              for (int p = 0; p < L.wm_n[l]; p++) {
+                int q[3] = {L.wm_ijk[l][3 * p], L.wm_ijk[l][3 * p + 1], L.wm_ijk[l][3 * p + 2]};
+                int f[3] = {q[0], q[1], q[2]}, t[3] = {q[0], q[1], q[2]};
+                if (!plus) {
+                  f[ax] -= 1;
+                  if (pass == 1) t[ax] -= 1;
+                } else if (pass == 0) {
+                  t[ax] += 1;
+                }
+                const int neg = pass == 1;
+                if (t[0] < 1 || t[0] > n[0] || t[1] < 1 || t[1] > n[1] || t[2] < 1 || t[2] > n[2]) continue;
+                long key = lin(t[0], t[1], t[2]);
+                // last slot: level of the face's lower cell (stretched-z terms)
+                tb.add(key, t[0], t[1], t[2],
+                       {ax | (neg << 2), list_off[l] + p, (int32_t)lin(f[0], f[1], f[2]), f[2]});
+                codes[c][key] |= CODE_CORRECTED;
+              }
           }
-      for (int p = 0; p < L.src_n; p++) {
-        if (L.src_scalar[p] != s) continue;
-        int q[3] = {L.src_ijk[3 * p], L.src_ijk[3 * p + 1], L.src_ijk[3 * p + 2]};
-        tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {2, p, 0, 0});
+          tb.upload(mom_terms[c]);
+        }
       }
+      upload_vec(code[c], codes[c]);
+    });
+  tasks.emplace_back([&] {
+    std::vector<int32_t> pt(ntot, 0);
+    remap(pt.data(), L.prtype, [](int32_t t) { return t; });
+    upload_vec(prtype, pt);
+    for (int d = 0; d < 3; d++) {
+      std::vector<uint8_t> m(ntot, 0);
+      remap(m.data(), L.scfl_mask[d], [](uint8_t t) { return t; });
+      upload_vec(mask[d], m);
+    }
+  });
+  tasks.emplace_back([&] {
+    std::vector<uint8_t> scode(ntot, 0);
+    remap(scode.data(), L.prtype, [](int32_t t) -> uint8_t { return t > 0 ? CODE_SOLID : 0; });
+    // scalar wall-model points and ordered scalar terms
+    wmp_n = cfg.wall_model ? L.wmp_n : 0;
+    std::vector<int32_t> pijk;
+    for (int p = 0; p < wmp_n; p++) pijk.insert(pijk.end(), {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]});
+    upload_vec(wmp_ijk, pijk);
+    wmp_div.alloc(std::max(wmp_n, 1) * sizeof(double));
+    {
+      // WMPoints' distance vectors, roughness, log(|dist|/z0) and u* (initially
+      // 1, the WMpoint default, src/wallmodels.f90:40)
+      std::vector<double> pd(L.wmp_dist, L.wmp_dist + 3 * (size_t)wmp_n), pz(L.wmp_z0, L.wmp_z0 + wmp_n),
+          plog(wmp_n, 0.0), pu(wmp_n, 1.0);
       for (int p = 0; p < wmp_n; p++) {
-        int q[3] = {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]};
-        tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {3, p, 0, 0});
+        const double *d = &pd[3 * p];
+        const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (pz[p] > 0) plog[p] = std::log(dist / pz[p]);
       }
+      upload_vec(wmp_dist, pd);
+      upload_vec(wmp_z0, pz);
+      upload_vec(wmp_logdz0, plog);
+      upload_vec(wmp_ustar, pu);
     }
-    // every scalar lists the union of corrected cells (same cell order)
-    TermBuilder uni(ntot);
-    for (int s = 0; s < nsc; s++)
-      for (size_t q = 0; q < tbs[s].cells.size(); q++) {
-        auto &cc = tbs[s].cells[q];
-        uni.cell_of(tbs[s].keys[q], cc[0], cc[1], cc[2]);
-      }
-    for (int s = 0; s < nsc; s++) {
-      std::vector<int32_t> sp, st;
-      tbs[s].csr(sp, st);
-      TermBuilder out(ntot);
-      for (size_t q = 0; q < uni.cells.size(); q++) {
-        const auto &cc = uni.cells[q];
-        const long key = uni.keys[q];
-        const int oc = out.cell_of(key, cc[0], cc[1], cc[2]);
-        const int ic = tbs[s].find(key);
-        if (ic < 0) continue;
-        for (int t = sp[ic]; t < sp[ic + 1]; t++)
-          out.items.emplace_back(oc, TermBuilder::Term{st[4 * t], st[4 * t + 1], st[4 * t + 2], st[4 * t + 3]});
-      }
-      out.upload(*sc_terms[s]);
+    std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
+    upload_vec(src_flux, sflux);
+    if (tslot >= 0) {
+      // solid cells of -1..n+2 with a fluid neighbour there: their temperature
+      // is the mean of those neighbours after each stage
+      const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+      const long hy = nx + 6, hz = (long)(nx + 6) * (ny + 6);
+      auto pt = [&](int i, int j, int k) { return L.prtype[(i + 2) + (j + 2) * hy + (k + 2) * hz]; };
+      std::vector<int32_t> fix;
+      for (int k = -1; k <= nz + 2; k++)
+        for (int j = -1; j <= ny + 2; j++)
+          for (int i = -1; i <= nx + 2; i++) {
+            if (pt(i, j, k) <= 0) continue;
+            const bool any = (i > -1 && pt(i - 1, j, k) <= 0) || (i < nx + 2 && pt(i + 1, j, k) <= 0) ||
+                             (j > -1 && pt(i, j - 1, k) <= 0) || (j < ny + 2 && pt(i, j + 1, k) <= 0) ||
+                             (k > -1 && pt(i, j, k - 1) <= 0) || (k < nz + 2 && pt(i, j, k + 1) <= 0);
+            if (any) fix.insert(fix.end(), {i, j, k});
+          }
+      tfix_n = (int)(fix.size() / 3);
+      upload_vec(tfix_ijk, fix);
     }
-    for (long key : uni.keys) scode[key] |= CODE_CORRECTED;
-  }
-  upload_vec(code_s, scode);
-  tr.mark(" scalar terms");
-
-  // immersed-boundary lists
-  for (int c = 0; c < 4; c++) {
-    ib_n[c] = L.ib_n[c];
-    std::vector<int32_t> tg(L.ib_ijk[c], L.ib_ijk[c] + 3 * (size_t)ib_n[c]);
-    std::vector<int32_t> pt(L.ib_ptr[c], L.ib_ptr[c] + (ib_n[c] ? ib_n[c] + 1 : 0));
-    if (pt.empty()) pt.push_back(0);
-    int nt = pt.back();
-    std::vector<int32_t> sr(L.ib_src_ijk[c], L.ib_src_ijk[c] + 3 * (size_t)nt);
-    std::vector<double> cf(L.ib_coef[c], L.ib_coef[c] + nt);
-    if (c == 3 && ib_n[3] > 0) {
-      // dependency levels of the sequential ScalFlIBPoints loop
-      std::unordered_map<long, int> wlev, rlev;
-      std::vector<int> lev(ib_n[3]);
-      for (int p = 0; p < ib_n[3]; p++) {
-        long tk = lin(tg[3 * p], tg[3 * p + 1], tg[3 * p + 2]);
-        int lv = 0;
-        auto bump = [&](std::unordered_map<long, int> &m, long key) {
-          auto it = m.find(key);
-          if (it != m.end()) lv = std::max(lv, it->second + 1);
-        };
-        for (int t = pt[p]; t < pt[p + 1]; t++) bump(wlev, lin(sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]));
-        bump(rlev, tk);
-        bump(wlev, tk);
-        lev[p] = lv;
-        wlev[tk] = std::max(wlev.count(tk) ? wlev[tk] : 0, lv);
-        for (int t = pt[p]; t < pt[p + 1]; t++) {
-          long sk = lin(sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]);
-          rlev[sk] = std::max(rlev.count(sk) ? rlev[sk] : 0, lv);
+    for (int s = 0; s < nsc; s++) sc_terms.emplace_back(new DevTerms);
+    if (nsc > 0) {
+      std::vector<TermBuilder> tbs;
+      for (int s = 0; s < nsc; s++) tbs.emplace_back(ntot);
+      auto interior = [&](int i, int j, int k) {
+        return i >= 1 && i <= cfg.nx && j >= 1 && j <= cfg.ny && k >= 1 && k <= cfg.nz;
+      };
+      for (int s = 0; s < nsc; s++) {
+        TermBuilder &tb = tbs[s];
+        for (int d = 0; d < 3; d++)
+          for (int pass = 0; pass < 2; pass++)
+            for (int p = 0; p < L.scfl_n[d]; p++) {
+              int q[3] = {L.scfl_ijk[d][3 * p], L.scfl_ijk[d][3 * p + 1], L.scfl_ijk[d][3 * p + 2]};
+              int t[3] = {q[0], q[1], q[2]};
+              if (pass == 1) t[d] += 1;
+              if (!interior(t[0], t[1], t[2])) continue;
+              tb.add(lin(t[0], t[1], t[2]), t[0], t[1], t[2],
+                     {(pass == 0 ? 0 : 1) | (d << 2), (int32_t)lin(q[0], q[1], q[2]), 0, 0});
+            }
+        for (int p = 0; p < L.src_n; p++) {
+          if (L.src_scalar[p] != s) continue;
+          int q[3] = {L.src_ijk[3 * p], L.src_ijk[3 * p + 1], L.src_ijk[3 * p + 2]};
+          tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {2, p, 0, 0});
+        }
+        for (int p = 0; p < wmp_n; p++) {
+          int q[3] = {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]};
+          tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {3, p, 0, 0});
         }
       }
-      int nl = *std::max_element(lev.begin(), lev.end()) + 1;
-      std::vector<int> order(ib_n[3]);
-      for (int p = 0; p < ib_n[3]; p++) order[p] = p;
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
-      std::vector<int32_t> tg2, pt2{0}, sr2;
-      std::vector<double> cf2;
-      visc_levels.assign(nl + 1, 0);
-      for (int q = 0; q < ib_n[3]; q++) {
-        int p = order[q];
-        tg2.insert(tg2.end(), {tg[3 * p], tg[3 * p + 1], tg[3 * p + 2]});
-        for (int t = pt[p]; t < pt[p + 1]; t++) {
-          sr2.insert(sr2.end(), {sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]});
-          cf2.push_back(cf[t]);
+      // every scalar lists the union of corrected cells (same cell order)
+      TermBuilder uni(ntot);
+      for (int s = 0; s < nsc; s++)
+        for (size_t q = 0; q < tbs[s].cells.size(); q++) {
+          auto &cc = tbs[s].cells[q];
+          uni.cell_of(tbs[s].keys[q], cc[0], cc[1], cc[2]);
         }
-        pt2.push_back((int32_t)cf2.size());
-        visc_levels[lev[p] + 1] = q + 1;
+      for (int s = 0; s < nsc; s++) {
+        std::vector<int32_t> sp, st;
+        tbs[s].csr(sp, st);
+        TermBuilder out(ntot);
+        for (size_t q = 0; q < uni.cells.size(); q++) {
+          const auto &cc = uni.cells[q];
+          const long key = uni.keys[q];
+          const int oc = out.cell_of(key, cc[0], cc[1], cc[2]);
+          const int ic = tbs[s].find(key);
+          if (ic < 0) continue;
+          for (int t = sp[ic]; t < sp[ic + 1]; t++)
+            out.items.emplace_back(oc, TermBuilder::Term{st[4 * t], st[4 * t + 1], st[4 * t + 2], st[4 * t + 3]});
+        }
+        out.upload(*sc_terms[s]);
       }
-      for (int l = 1; l <= nl; l++) visc_levels[l] = std::max(visc_levels[l], visc_levels[l - 1]);
-      tg = tg2;
-      pt = pt2;
-      sr = sr2;
-      cf = cf2;
+      for (long key : uni.keys) scode[key] |= CODE_CORRECTED;
     }
-    upload_vec(ib_tgt[c], tg);
-    upload_vec(ib_ptr[c], pt);
-    upload_vec(ib_src[c], sr);
-    upload_vec(ib_coef[c], cf);
-  }
-  ib_tmp.alloc((size_t)std::max(ib_n[0] + ib_n[1] + ib_n[2], 1) * sizeof(double));
-  tr.mark(" ib lists");
+    upload_vec(code_s, scode);
+  });
+  tasks.emplace_back([&] {
+    // immersed-boundary lists
+    for (int c = 0; c < 4; c++) {
+      ib_n[c] = L.ib_n[c];
+      std::vector<int32_t> tg(L.ib_ijk[c], L.ib_ijk[c] + 3 * (size_t)ib_n[c]);
+      std::vector<int32_t> pt(L.ib_ptr[c], L.ib_ptr[c] + (ib_n[c] ? ib_n[c] + 1 : 0));
+      if (pt.empty()) pt.push_back(0);
+      int nt = pt.back();
+      std::vector<int32_t> sr(L.ib_src_ijk[c], L.ib_src_ijk[c] + 3 * (size_t)nt);
+      std::vector<double> cf(L.ib_coef[c], L.ib_coef[c] + nt);
+      if (c == 3 && ib_n[3] > 0) {
+        // dependency levels of the sequential ScalFlIBPoints loop
+        std::unordered_map<long, int> wlev, rlev;
+        std::vector<int> lev(ib_n[3]);
+        for (int p = 0; p < ib_n[3]; p++) {
+          long tk = lin(tg[3 * p], tg[3 * p + 1], tg[3 * p + 2]);
+          int lv = 0;
+          auto bump = [&](std::unordered_map<long, int> &m, long key) {
+            auto it = m.find(key);
+            if (it != m.end()) lv = std::max(lv, it->second + 1);
+          };
+          for (int t = pt[p]; t < pt[p + 1]; t++) bump(wlev, lin(sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]));
+          bump(rlev, tk);
+          bump(wlev, tk);
+          lev[p] = lv;
+          wlev[tk] = std::max(wlev.count(tk) ? wlev[tk] : 0, lv);
+          for (int t = pt[p]; t < pt[p + 1]; t++) {
+            long sk = lin(sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]);
+            rlev[sk] = std::max(rlev.count(sk) ? rlev[sk] : 0, lv);
+          }
+        }
+        int nl = *std::max_element(lev.begin(), lev.end()) + 1;
+        std::vector<int> order(ib_n[3]);
+        for (int p = 0; p < ib_n[3]; p++) order[p] = p;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
+        std::vector<int32_t> tg2, pt2{0}, sr2;
+        std::vector<double> cf2;
+        visc_levels.assign(nl + 1, 0);
+        for (int q = 0; q < ib_n[3]; q++) {
+          int p = order[q];
+          tg2.insert(tg2.end(), {tg[3 * p], tg[3 * p + 1], tg[3 * p + 2]});
+          for (int t = pt[p]; t < pt[p + 1]; t++) {
+            sr2.insert(sr2.end(), {sr[3 * t], sr[3 * t + 1], sr[3 * t + 2]});
+            cf2.push_back(cf[t]);
+          }
+          pt2.push_back((int32_t)cf2.size());
+          visc_levels[lev[p] + 1] = q + 1;
+        }
+        for (int l = 1; l <= nl; l++) visc_levels[l] = std::max(visc_levels[l], visc_levels[l - 1]);
+        tg = tg2;
+        pt = pt2;
+        sr = sr2;
+        cf = cf2;
+      }
+      upload_vec(ib_tgt[c], tg);
+      upload_vec(ib_ptr[c], pt);
+      upload_vec(ib_src[c], sr);
+      upload_vec(ib_coef[c], cf);
+    }
+    ib_tmp.alloc((size_t)std::max(ib_n[0] + ib_n[1] + ib_n[2], 1) * sizeof(double));
+  });
+  tasks.emplace_back([&] {
+    // IBMassSources gather terms (src/tsteps.f90:490-531) + Bound_Q fold targets
+    if (cfg.ib_mass_sources && L.ib_n[0] + L.ib_n[1] + L.ib_n[2] > 0) {
+      TermBuilder tb(ntot);
+      for (int c = 0; c < 3; c++)
+        for (int p = 0; p < L.ib_n[c]; p++) {
+          int q[3] = {L.ib_ijk[c][3 * p], L.ib_ijk[c][3 * p + 1], L.ib_ijk[c][3 * p + 2]};
+          int32_t src = (int32_t)lin(q[0], q[1], q[2]);
+          tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {c, src, q[c], 0});
+          int r[3] = {q[0], q[1], q[2]};
+          r[c] += 1;
+          tb.add(lin(r[0], r[1], r[2]), r[0], r[1], r[2], {c | (1 << 2), src, q[c] + 1, 0});
+        }
+      // fold destinations must be rewritten every stage (their pre-fold value
+      // is their own direct sum, zero when they have none)
+      const int N[3] = {cfg.nx, cfg.ny, cfg.nz};
+      std::vector<std::array<int, 3>> targets(tb.cells.begin(), tb.cells.end());
+      // Bound_Q reads a ghost cell only when exactly one of its coordinates is
+      // outside the interior (src/boundaries.f90:2094-2192)
+      for (auto &q : targets) {
+        int nghost = 0, a = -1;
+        for (int b = 0; b < 3; b++)
+          if (q[b] < 1 || q[b] > N[b]) {
+            nghost++;
+            a = b;
+          }
+        if (nghost != 1) continue;
+        bool per = cfg.btype[2 * a] == ELMM_BC_PERIODIC;
+        std::array<int, 3> d = q;
+        if (q[a] <= 0) d[a] = per ? N[a] : 1;
+        else d[a] = per ? 1 : N[a];
+        tb.cell_of(lin(d[0], d[1], d[2]), d[0], d[1], d[2]);
+      }
+      tb.upload(q_terms);
+      // rows (j, k) and 64-cell segments of x where Q may be non-zero: the
+      // listed interior cells (everything else stays zero)
+      std::vector<uint32_t> qm((size_t)cfg.ny * cfg.nz, 0);
+      for (auto &q : tb.cells)
+        if (q[0] >= 1 && q[0] <= cfg.nx && q[1] >= 1 && q[1] <= cfg.ny && q[2] >= 1 && q[2] <= cfg.nz)
+          qm[(size_t)(q[1] - 1) + (size_t)cfg.ny * (q[2] - 1)] |= 1u << ((q[0] - 1) >> 6);
+      upload_vec(qmask, qm);
+    }
+    for (int c = 0; c < 3; c++) {
+      null_n[c] = L.null_n[c];
+      std::vector<int32_t> v(L.null_ijk[c], L.null_ijk[c] + 3 * (size_t)null_n[c]);
+      upload_vec(null_ijk[c], v);
+    }
+  });
+  run_parallel(tasks, device);
+  tr.mark(" lists");
   setup_ghost_delta();
   tr.mark(" ghost delta");
-  // IBMassSources gather terms (src/tsteps.f90:490-531) + Bound_Q fold targets
-  if (cfg.ib_mass_sources && has_ib()) {
-    TermBuilder tb(ntot);
-    for (int c = 0; c < 3; c++)
-      for (int p = 0; p < ib_n[c]; p++) {
-        int q[3] = {L.ib_ijk[c][3 * p], L.ib_ijk[c][3 * p + 1], L.ib_ijk[c][3 * p + 2]};
-        int32_t src = (int32_t)lin(q[0], q[1], q[2]);
-        tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {c, src, q[c], 0});
-        int r[3] = {q[0], q[1], q[2]};
-        r[c] += 1;
-        tb.add(lin(r[0], r[1], r[2]), r[0], r[1], r[2], {c | (1 << 2), src, q[c] + 1, 0});
-      }
-    // fold destinations must be rewritten every stage (their pre-fold value
-    // is their own direct sum, zero when they have none)
-    const int N[3] = {cfg.nx, cfg.ny, cfg.nz};
-    std::vector<std::array<int, 3>> targets(tb.cells.begin(), tb.cells.end());
-    // Bound_Q reads a ghost cell only when exactly one of its coordinates is
-    // outside the interior (src/boundaries.f90:2094-2192)
-    for (auto &q : targets) {
-      int nghost = 0, a = -1;
-      for (int b = 0; b < 3; b++)
-        if (q[b] < 1 || q[b] > N[b]) {
-          nghost++;
-          a = b;
-        }
-      if (nghost != 1) continue;
-      bool per = cfg.btype[2 * a] == ELMM_BC_PERIODIC;
-      std::array<int, 3> d = q;
-      if (q[a] <= 0) d[a] = per ? N[a] : 1;
-      else d[a] = per ? 1 : N[a];
-      tb.cell_of(lin(d[0], d[1], d[2]), d[0], d[1], d[2]);
-    }
-    tb.upload(q_terms);
-    // rows (j, k) and 64-cell segments of x where Q may be non-zero: the
-    // listed interior cells (everything else stays zero)
-    std::vector<uint32_t> qm((size_t)cfg.ny * cfg.nz, 0);
-    for (auto &q : tb.cells)
-      if (q[0] >= 1 && q[0] <= cfg.nx && q[1] >= 1 && q[1] <= cfg.ny && q[2] >= 1 && q[2] <= cfg.nz)
-        qm[(size_t)(q[1] - 1) + (size_t)cfg.ny * (q[2] - 1)] |= 1u << ((q[0] - 1) >> 6);
-    upload_vec(qmask, qm);
-  }
-  for (int c = 0; c < 3; c++) {
-    null_n[c] = L.null_n[c];
-    std::vector<int32_t> v(L.null_ijk[c], L.null_ijk[c] + 3 * (size_t)null_n[c]);
-    upload_vec(null_ijk[c], v);
-  }
-  tr.mark(" mass sources");
 }
 
 template <typename T>
@@ -1749,7 +1797,6 @@ std::map<std::tuple<int, int, int, int, int>, TileChoice> g_tune_cache;
 
 template <typename T>
 void Engine<T>::autotune() {
-  HostOpLock host;  // events created and destroyed
   tuned = true;
   const auto key = std::make_tuple((int)sizeof(T), cfg.nx, cfg.ny, cfg.nz, nsc);
   TileChoice c{mom_tile, zchunk, g.sgs_zchunk, zchunk_scalar};
@@ -1764,8 +1811,11 @@ void Engine<T>::autotune() {
   }
   if (!hit) {
     hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    {
+      HostOpLock host;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+    }
     auto time_of = [&](auto launch) {
       launch();
       HIPCHK(hipEventRecord(e0, stream));
@@ -1828,8 +1878,11 @@ void Engine<T>::autotune() {
         }
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    {
+      HostOpLock host;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune_cache[key] = c;
   }

@@ -68,6 +68,7 @@ bool task_ready(Request &r) { return r.task.wait_for(std::chrono::seconds(0)) ==
 
 struct elmm_server {
   size_t budget = 0, used = 0;
+  int device = 0;  // current at elmm_server_create; the workers run on it
   int max_active = 0, chunk = 0;
   int64_t next_id = 1;
   std::deque<std::unique_ptr<Request>> queue;
@@ -185,6 +186,7 @@ int elmm_server_create(int64_t hbm_budget, int32_t max_active, int32_t chunk_ste
   s->budget = (size_t)hbm_budget;
   s->max_active = max_active;
   s->chunk = chunk_steps;
+  s->device = elmm::current_device();
   *out = s.release();
   return 0;
 }
@@ -265,7 +267,11 @@ int elmm_server_progress(elmm_server *s, int32_t *n_active, int32_t *n_queued) {
           continue;
         }
         Request *rp = &r;
-        r.task = std::async(std::launch::async, [rp] { return retire(*rp); });
+        const int dev = s->device;
+        r.task = std::async(std::launch::async, [rp, dev] {
+          elmm::use_device(dev);
+          return retire(*rp);
+        });
         s->retiring.push_back(std::move(s->active[a]));
         s->active.erase(s->active.begin() + a);
         continue;
@@ -303,8 +309,11 @@ int elmm_server_progress(elmm_server *s, int32_t *n_active, int32_t *n_queued) {
     s->queue.pop_front();
     s->used += r->footprint;
     Request *rp = r.get();
-    const int chunk = s->chunk;
-    r->task = std::async(std::launch::async, [rp, chunk] { return admit(chunk, *rp); });
+    const int chunk = s->chunk, dev = s->device;
+    r->task = std::async(std::launch::async, [rp, chunk, dev] {
+      elmm::use_device(dev);
+      return admit(chunk, *rp);
+    });
     s->preparing.push_back(std::move(r));
   }
   if (n_active) *n_active = (int32_t)(s->preparing.size() + s->active.size() + s->retiring.size());
