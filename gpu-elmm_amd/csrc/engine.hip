@@ -1378,37 +1378,50 @@ void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool
   if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) throw ArgError("extents must be positive");
   HIPCHK(hipStreamSynchronize(stream));
   const size_t es = is64 ? sizeof(double) : sizeof(T);
-  std::vector<double> stage(g.ntot);
-  std::vector<T> staget(is64 ? 0 : g.ntot);
-  if (is64) {
-    HIPCHK(hipMemcpy(stage.data(), d, g.ntot * es, hipMemcpyDeviceToHost));
-  } else {
-    HIPCHK(hipMemcpy(staget.data(), d, g.ntot * es, hipMemcpyDeviceToHost));
-    for (long q = 0; q < g.ntot; q++) stage[q] = (double)staget[q];
-  }
   const int n[3] = {cfg.nx, cfg.ny, cfg.nz};
   int lo[3], hi[3];
   for (int a = 0; a < 3; a++) {
     lo[a] = std::max(lb, -2);
     hi[a] = std::min(lb + ext[a] - 1, n[a] + 3);
   }
+  // An upload of the whole box -2..n+3 replaces every element any kernel
+  // reads or writes; the rest of the allocation (row padding) is never
+  // written after the allocation's zero fill.  Such an upload starts from
+  // zeros instead of the field's current contents.
+  const bool whole = up && lo[0] == -2 && lo[1] == -2 && lo[2] == -2 && hi[0] == n[0] + 3 && hi[1] == n[1] + 3 &&
+                     hi[2] == n[2] + 3;
+  // host image of the allocation: FP64 for Phi and Q, the field type else
+  std::vector<double> stage(is64 ? g.ntot : 0);
+  std::vector<T> staget(is64 ? 0 : g.ntot);
+  double *st64 = stage.data();
+  void *himg = is64 ? (void *)stage.data() : (void *)staget.data();
+  if (!whole) HIPCHK(hipMemcpy(himg, d, g.ntot * es, hipMemcpyDeviceToHost));
+  // rows (j, k) of the box, split over threads
   const long hy = ext[0], hz = (long)ext[0] * ext[1];
-  for (int k = lo[2]; k <= hi[2]; k++)
-    for (int j = lo[1]; j <= hi[1]; j++)
-      for (int i = lo[0]; i <= hi[0]; i++) {
-        long h = (i - lb) + (j - lb) * hy + (k - lb) * hz;
-        long p = gix(g, i, j, k);
+  const int nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
+  const int nth = (int)std::min<long>(8, std::max<long>(1, (long)nj * nk * (hi[0] - lo[0] + 1) / 262144));
+  if (nj > 0 && nk > 0 && hi[0] >= lo[0]) run_tasks(nth, [&](int t) {
+    for (int r = t; r < nj * nk; r += nth) {
+      const int j = lo[1] + r % nj, k = lo[2] + r / nj;
+      const long h0 = (lo[0] - lb) + (j - lb) * hy + (k - lb) * hz;
+      const long p0 = gix(g, lo[0], j, k);
+      const int w = hi[0] - lo[0] + 1;
+      if (is64) {
         if (up)
-          stage[p] = host[h];
+          std::memcpy(st64 + p0, host + h0, w * sizeof(double));
         else
-          host[h] = stage[p];
+          std::memcpy(host + h0, st64 + p0, w * sizeof(double));
+      } else {
+        T *f = staget.data() + p0;
+        if (up)
+          for (int i = 0; i < w; i++) f[i] = (T)host[h0 + i];
+        else
+          for (int i = 0; i < w; i++) host[h0 + i] = (double)f[i];
       }
-  if (up) {
-    const void *src = stage.data();
-    if (!is64) {
-      for (long q = 0; q < g.ntot; q++) staget[q] = (T)stage[q];
-      src = staget.data();
     }
+  });
+  if (up) {
+    const void *src = himg;
     HIPCHK(hipMemcpy(d, src, g.ntot * es, hipMemcpyHostToDevice));
     // scalar buffers rotate: both copies start from the uploaded state so
     // ghost cells that no fill touches keep the uploaded value
