@@ -126,6 +126,7 @@ std::string validate_config(const elmm_config &c) {
   if (c.sgs_model != 0 && c.sgs_model != 1) return "sgs_model must be 0 or 1";
   if (c.nscalars < 0 || c.nscalars > 8) return "nscalars must be in 0..8";
   if (c.precision != ELMM_FP64 && c.precision != ELMM_FP32_REFINED) return "unknown precision";
+  if (c.enable_moisture && !c.enable_buoyancy) return "moisture needs enable_buoyancy (the temperature)";
   if (c.variable_dt) {
     if (!(c.cfl_target > 0) || !(c.dt_max > 0)) return "variable time steps need cfl_target > 0 and dt_max > 0";
   } else if (!(c.dt > 0)) {

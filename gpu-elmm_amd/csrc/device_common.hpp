@@ -116,8 +116,10 @@ struct StageConst {
   double coef;        // 2*RK_alpha(stage)*dt
   double A, Ax, Ay, Az;     // -coef, -coef/dxmin, ...
   double prgx, prgy;  // uniform pressure gradient (0 = off)
-  // buoyancy (BuoyancyForce, 2nd order): W += bA2 (T(k+1) + T(k)) - bATref
-  int buoy;
+  // buoyancy (BuoyancyForce, 2nd order): W += bA2 (T(k+1) + T(k)) - bATref;
+  // moist: the virtual temperature T (1 + 0.61 q) in the reference's
+  // two-pass order (odd k first)
+  int buoy, moist;
   double bA2, bATref;
 };
 

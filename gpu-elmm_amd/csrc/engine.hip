@@ -364,9 +364,13 @@ struct Engine : EngineIface {
   std::vector<double> zw_copy;
   Geo g{};
   Grid1D grid;
-  int nsc = 0;  // scalar slots: the passive scalars, then the temperature (buoyancy)
+  int nsc = 0;  // scalar slots: the passive scalars, the temperature (buoyancy), the moisture
   int tslot = -1;  // slot of the temperature, -1 without buoyancy
+  int mslot = -1;  // slot of the moisture, -1 without it
   Geo gtemp{};     // g with the temperature's boundary conditions
+  Geo gmoist{};    // g with the moisture's boundary conditions
+  // g with the boundary conditions of scalar slot s
+  const Geo &gscal(int s) const { return s == tslot ? gtemp : s == mslot ? gmoist : g; }
   DevBuf tfix_ijk;  // solid cells whose temperature is the mean of fluid neighbours
   int tfix_n = 0;
   // fields
@@ -1300,8 +1304,10 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     zw_copy.assign(cfg.zw, cfg.zw + cfg.nz + 1);
     cfg.zw = zw_copy.data();
   }
-  nsc = cfg.nscalars + (cfg.enable_buoyancy ? 1 : 0);
+  if (cfg.enable_moisture && !cfg.enable_buoyancy) throw ArgError("moisture needs enable_buoyancy");
+  nsc = cfg.nscalars + (cfg.enable_buoyancy ? 1 : 0) + (cfg.enable_moisture ? 1 : 0);
   tslot = cfg.enable_buoyancy ? cfg.nscalars : -1;
+  mslot = cfg.enable_moisture ? cfg.nscalars + 1 : -1;
   dt_cur = cfg.variable_dt ? cfg.dt_max : cfg.dt;
   time = cfg.start_time;
   SetupTrace tr;
@@ -1325,6 +1331,18 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     for (int s = 0; s < 6; s++) {
       gtemp.sbt[s] = cfg.temp_btype[s];
       gtemp.side_scal[s] = cfg.side_temp[s];
+    }
+  }
+  if (cfg.enable_moisture) {
+    for (int s = 0; s < 4; s++)
+      if (cfg.moist_btype[s] != ELMM_BC_PERIODIC) throw ArgError("the moisture must be periodic in x and y");
+    for (int s = 4; s < 6; s++)
+      if (cfg.moist_btype[s] != ELMM_BC_DIRICHLET && cfg.moist_btype[s] != ELMM_BC_NEUMANN)
+        throw ArgError("moisture z boundaries must be Dirichlet or Neumann");
+    gmoist = g;
+    for (int s = 0; s < 6; s++) {
+      gmoist.sbt[s] = cfg.moist_btype[s];
+      gmoist.side_scal[s] = cfg.side_moist[s];
     }
   }
   tr.mark("geometry");
@@ -1373,6 +1391,7 @@ void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool
     default:
       if (field >= ELMM_F_SCALAR && field < ELMM_F_SCALAR + cfg.nscalars) d = s_cur[field - ELMM_F_SCALAR];
       if (field == ELMM_F_TEMPERATURE && tslot >= 0) d = s_cur[tslot];
+      if (field == ELMM_F_MOISTURE && mslot >= 0) d = s_cur[mslot];
   }
   if (!d) throw ArgError("unknown field id " + std::to_string(field));
   if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) throw ArgError("extents must be positive");
@@ -1429,6 +1448,8 @@ void Engine<T>::xfer(int field, int lb, const int32_t ext[3], double *host, bool
       HIPCHK(hipMemcpy(s_nxt[field - ELMM_F_SCALAR], src, g.ntot * es, hipMemcpyHostToDevice));
     if (field == ELMM_F_TEMPERATURE && tslot >= 0)
       HIPCHK(hipMemcpy(s_nxt[tslot], src, g.ntot * es, hipMemcpyHostToDevice));
+    if (field == ELMM_F_MOISTURE && mslot >= 0)
+      HIPCHK(hipMemcpy(s_nxt[mslot], src, g.ntot * es, hipMemcpyHostToDevice));
   }
 }
 
@@ -1596,6 +1617,7 @@ StageConst Engine<T>::stage_const(int st) const {
     // A = grav_acc / temperature_ref, A2 = A / 2, and A temperature_ref
     const double A = cfg.grav_acc / cfg.temperature_ref;
     sc.buoy = 1;
+    sc.moist = cfg.enable_moisture ? 1 : 0;
     sc.bA2 = A / 2.0;
     sc.bATref = A * cfg.temperature_ref;
   }
@@ -1613,7 +1635,7 @@ void Engine<T>::stage(int st, bool last) {
   if (fused_momentum) {
     const uint8_t *codes[3] = {code[0].as<uint8_t>(), code[1].as<uint8_t>(), code[2].as<uint8_t>()};
     timed(ELMM_T_MOMENTUM, [&] {
-      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), tslot >= 0 ? s_cur[tslot] : nullptr, ustar, U2p,
+      launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), tslot >= 0 ? s_cur[tslot] : nullptr, mslot >= 0 ? s_cur[mslot] : nullptr, ustar, U2p,
                             codes, sc, zchunk, mom_tile,
                                  stream);
     });
@@ -1658,11 +1680,14 @@ void Engine<T>::stage(int st, bool last) {
                              mask[2].as<uint8_t>(), sc, sc_terms[s]->view(), src_flux.as<double>(),
                              wmp_div.as<double>(), stream);
         });
-      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<T>(s == tslot ? gtemp : g, s_nxt[s], stream); });
+      timed(ELMM_T_GHOST, [&] { launch_bound_scalar<T>(gscal(s), s_nxt[s], stream); });
       if (s == tslot && tfix_n)
         timed(ELMM_T_SCALAR_CORR, [&] {
           launch_temp_solid_fix<T>(g, tfix_ijk.as<int32_t>(), tfix_n, prtype.as<int32_t>(), s_nxt[s], stream);
         });
+      if (s == mslot)
+        timed(ELMM_T_SCALAR_CORR,
+              [&] { launch_moist_fill<T>(g, code_s.as<uint8_t>(), s_nxt[s], cfg.moisture_ref, stream); });
       std::swap(s_cur[s], s_nxt[s]);
     }
   }
@@ -1713,7 +1738,7 @@ void Engine<T>::init() {
     launch_tdiff_init(g, Visc.as<T>(), TDiff.as<T>(), cfg.nu, cfg.nu_scalar, stream);
     launch_bound_visc<T>(g, TDiff.as<T>(), stream);
   }
-  for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(s == tslot ? gtemp : g, s_cur[s], stream);
+  for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(gscal(s), s_cur[s], stream);
   if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, wmpdev(), cfg.nu, Visc.as<T>(), stream);
   if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
@@ -1852,7 +1877,7 @@ void Engine<T>::autotune() {
       for (int t : tiles)
         for (int ch : chunks) {
           const float ms = time_of([&] {
-            launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), tslot >= 0 ? s_cur[tslot] : nullptr, ustar,
+            launch_momentum_zm<T>(g, u, v, w, Visc.as<T>(), Pr.as<T>(), tslot >= 0 ? s_cur[tslot] : nullptr, mslot >= 0 ? s_cur[mslot] : nullptr, ustar,
                                   U2p, codes, sc0, ch, t, stream);
           });
           if (ms < best) {

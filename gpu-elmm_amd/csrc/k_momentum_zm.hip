@@ -9,7 +9,7 @@ size_t momentum_zm_lds_bytes(int elem) { return (size_t)NSLOT * Tile<32, 8>::PS 
 
 bool momentum_zm_supported(const Geo &g) { return !g.variable_z; }
 
-template void launch_momentum_zm<double>(const Geo &, const double *, const double *, const double *, const double *,
+template void launch_momentum_zm<double>(const Geo &, const double *, const double *, const double *, const double *, const double *,
                                          const double *, const double *, double *const[3], double *const[3],
                                          const uint8_t *const[3], StageConst, int, int, hipStream_t);
 }  // namespace elmm

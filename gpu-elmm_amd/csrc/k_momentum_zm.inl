@@ -52,7 +52,8 @@ __device__ inline T m4(T a, T b, T c, T d) {
 template <typename T>
 struct MomZArgs {
   const T *U, *V, *W, *Visc, *Pr;
-  const T *Temp;  // temperature (buoyancy), stage start
+  const T *Temp;   // temperature (buoyancy), stage start
+  const T *Moist;  // moisture (moist buoyancy), stage start
   T *Us[3];
   T *U2[3];
   const uint8_t *code[3];
@@ -255,6 +256,12 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
     ct0 = a.Temp[pc + (long)kbeg * sz];
     ct1 = a.Temp[pc + (long)(kbeg + 1) * sz];
   }
+  // moisture at k and k+1 (moist buoyancy)
+  T cq0 = 0, cq1 = 0;
+  if (sc.moist) {
+    cq0 = a.Moist[pc + (long)kbeg * sz];
+    cq1 = a.Moist[pc + (long)(kbeg + 1) * sz];
+  }
   uint8_t ccd[3];
   T cus[3];
 #pragma unroll
@@ -289,6 +296,8 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
       }
       T nt = 0;
       if (sc.buoy) nt = a.Temp[pc + (long)(k + 2) * sz];
+      T nq = 0;
+      if (sc.moist) nq = a.Moist[pc + (long)(k + 2) * sz];
       const long q = pc + (long)(k + 1) * sz;
       const T nprx = a.Pr[q + 1], npry = a.Pr[q + g.sy], nprz = a.Pr[q + sz];
       uint8_t ncd[3];
@@ -391,7 +400,19 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
             }
             w = adv(dX, dY, dZ);
           }
-          if (sc.buoy) w = w + (T)sc.bA2 * (ct1 + ct0) - (T)sc.bATref;
+          if (sc.buoy) {
+            if (sc.moist) {
+              // W(k) gets Tv(k) in the pass over odd k and Tv(k+1) in the
+              // pass over even k (and the reverse for even k)
+              const T tv0 = ct0 * ((T)1 + (T)0.61 * cq0), tv1 = ct1 * ((T)1 + (T)0.61 * cq1);
+              if (k & 1)
+                w = ((w + (T)sc.bA2 * tv0) - (T)sc.bATref) + (T)sc.bA2 * tv1;
+              else
+                w = ((w + (T)sc.bA2 * tv1) + (T)sc.bA2 * tv0) - (T)sc.bATref;
+            } else {
+              w = w + (T)sc.bA2 * (ct1 + ct0) - (T)sc.bATref;
+            }
+          }
           const T f0 = cw[3];
           w = w + qa(at(n1, 1, 0) + at(n0, 1, 0) + at(n1, 0, 0) + at(n0, 0, 0)) * (at(w0, 1, 0) - f0) * rxq;
           w = w - qa(at(n1, 0, 0) + at(n0, 0, 0) + at(n1, -1, 0) + at(n0, -1, 0)) * (f0 - at(w0, -1, 0)) * rxq;
@@ -425,6 +446,8 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
       cw[6] = ow;
       ct0 = ct1;
       ct1 = nt;
+      cq0 = cq1;
+      cq1 = nq;
       cpr = cprz;
       cprx = nprx;
       cpry = npry;
@@ -476,10 +499,11 @@ void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, 
 // asking for it runs 32x8).  The engine's autotuner times them per grid.
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, const T *Temp,
-                        T *const Us[3], T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk,
-                        int tile, hipStream_t st) {
+                        const T *Moist, T *const Us[3], T *const U2[3], const uint8_t *const code[3], StageConst sc,
+                        int kchunk, int tile, hipStream_t st) {
   MomZArgs<T> a;
   a.Temp = Temp;
+  a.Moist = Moist;
   a.U = U;
   a.V = V;
   a.W = W;

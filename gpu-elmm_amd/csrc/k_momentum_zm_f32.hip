@@ -7,7 +7,7 @@
 
 namespace elmm {
 
-template void launch_momentum_zm<float>(const Geo &, const float *, const float *, const float *, const float *,
+template void launch_momentum_zm<float>(const Geo &, const float *, const float *, const float *, const float *, const float *,
                                         const float *, const float *, float *const[3], float *const[3], const uint8_t *const[3],
                                         StageConst, int, int, hipStream_t);
 

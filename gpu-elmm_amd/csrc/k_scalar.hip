@@ -481,6 +481,20 @@ void launch_temp_solid_fix(const Geo &g, const int32_t *ijk, int n, const int32_
   if (n > 0) hipLaunchKernelGGL(k_temp_solid_fix<Real>, dim3(nblk(n, 256)), dim3(256), 0, st, g, ijk, n, prtype, T);
 }
 
+// The moisture in solid cells after its stage (src/scalars.f90:208):
+// moisture_ref wherever Prtype > 0 in -2..n+3 (the scalar code's CODE_SOLID,
+// set over that box; the row padding has code 0).
+template <typename Real>
+__global__ void k_moist_fill(long n, const uint8_t *__restrict__ code, Real *__restrict__ q, Real ref) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n && (code[p] & CODE_SOLID)) q[p] = ref;
+}
+
+template <typename Real>
+void launch_moist_fill(const Geo &g, const uint8_t *code, Real *q, double ref, hipStream_t st) {
+  hipLaunchKernelGGL(k_moist_fill<Real>, dim3(nblk(g.ntot, 256)), dim3(256), 0, st, g.ntot, code, q, (Real)ref);
+}
+
 #define ELMM_INST_SCALAR(R)                                                                                        \
   template void launch_scalar<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *,     \
                                  const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *, StageConst,   \
@@ -492,6 +506,7 @@ void launch_temp_solid_fix(const Geo &g, const int32_t *ijk, int n, const int32_
                                  hipStream_t);                                                                    \
   template void launch_tdiff_init<R>(const Geo &, const R *, R *, double, double, hipStream_t);                 \
   template void launch_temp_solid_fix<R>(const Geo &, const int32_t *, int, const int32_t *, R *, hipStream_t);  \
+  template void launch_moist_fill<R>(const Geo &, const uint8_t *, R *, double, hipStream_t);                      \
   template void launch_scalar_zm<R>(const Geo &, const R *, const R *, const R *, const R *, const R *, R *, R *,  \
                                     const uint8_t *, const uint8_t *, const uint8_t *, const uint8_t *, StageConst,\
                                     int, hipStream_t);

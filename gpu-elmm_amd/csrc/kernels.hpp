@@ -122,6 +122,9 @@ void launch_mg_coarse(const MgLevel &L, const MgCoarse &C, const double *az, con
 
 template <typename R>
 void launch_temp_solid_fix(const Geo &g, const int32_t *ijk, int n, const int32_t *prtype, R *T, hipStream_t st);
+// Moisture = ref in the solid cells (scalar code CODE_SOLID) of -2..n+3
+template <typename R>
+void launch_moist_fill(const Geo &g, const uint8_t *code, R *q, double ref, hipStream_t st);
 template <typename T>
 void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, WMUstarPoints wp, double nu, T *Visc,
                      hipStream_t st);
@@ -134,8 +137,8 @@ size_t momentum_zm_lds_bytes(int elem);
 enum { kMomTile32x8 = 0, kMomTile64x4 = 1, kMomTile64x8 = 3 };
 template <typename T>
 void launch_momentum_zm(const Geo &g, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, const T *Temp,
-                        T *const Us[3], T *const U2[3], const uint8_t *const code[3], StageConst sc, int kchunk,
-                        int tile, hipStream_t st);
+                        const T *Moist, T *const Us[3], T *const U2[3], const uint8_t *const code[3], StageConst sc,
+                        int kchunk, int tile, hipStream_t st);
 template <typename T>
 void launch_momentum_corr(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr,
                           T *Ustar, T *U2, const uint8_t *code, StageConst sc, TermList tl, const double *fluxp,

@@ -151,7 +151,8 @@ int elmm_footprint(const elmm_config *cfg, int64_t *bytes) {
   const long M = cfg->nx / 2 + 1;
   const int es = cfg->precision == ELMM_FP32_REFINED && cfg->refine_iters > 0 ? 4 : 8;
   const long Mp = es == 8 ? (M + 7) / 8 * 8 : (M + 15) / 16 * 16;  // 128-byte rows
-  const int nslots = cfg->nscalars + (cfg->enable_buoyancy ? 1 : 0);  // the temperature is one more slot
+  // the temperature and the moisture are one more slot each
+  const int nslots = cfg->nscalars + (cfg->enable_buoyancy ? 1 : 0) + (cfg->enable_moisture ? 1 : 0);
   int64_t b = (int64_t)ntot * (12 * e + 3 * e * nslots + 3 * 8 + 7 + 4 + 1);
   if (cfg->poisson_solver == ELMM_POISSON_MG) {
     // multigrid: phi and rhs per level (x and y halved: at most 4/3 of the

@@ -20,7 +20,7 @@ WE, EA, SO, NO, BO, TO = range(6)
 NOSLIP, FREESLIP, PERIODIC, DIRICHLET, NEUMANN = 1, 2, 3, 4, 5
 FP64, FP32_REFINED = 0, 1
 POISSON_FFT, POISSON_MG = 2, 3
-F_U, F_V, F_W, F_PR, F_VISC, F_TDIFF, F_PHI, F_Q, F_TEMPERATURE, F_SCALAR = 0, 1, 2, 3, 4, 5, 6, 7, 15, 16
+F_U, F_V, F_W, F_PR, F_VISC, F_TDIFF, F_PHI, F_Q, F_MOISTURE, F_TEMPERATURE, F_SCALAR = 0, 1, 2, 3, 4, 5, 6, 7, 14, 15, 16
 
 
 class Config(C.Structure):
@@ -50,6 +50,8 @@ class Config(C.Structure):
         ("autotune", C.c_int32), ("mg_cycles_max", C.c_int32),
         ("enable_buoyancy", C.c_int32), ("grav_acc", C.c_double), ("temperature_ref", C.c_double),
         ("temp_btype", C.c_int32 * 6), ("side_temp", C.c_double * 6),
+        ("enable_moisture", C.c_int32), ("moist_btype", C.c_int32 * 6), ("moist_pad", C.c_int32),
+        ("moisture_ref", C.c_double), ("side_moist", C.c_double * 6),
         ("reserved", C.c_int32 * 2),
     ]
 

@@ -55,6 +55,11 @@ module elmm_gpu
     real(c_double) :: grav_acc, temperature_ref
     integer(c_int32_t) :: temp_btype(6)
     real(c_double) :: side_temp(6)
+    integer(c_int32_t) :: enable_moisture
+    integer(c_int32_t) :: moist_btype(6)
+    integer(c_int32_t) :: moist_pad
+    real(c_double) :: moisture_ref
+    real(c_double) :: side_moist(6)
     integer(c_int32_t) :: reserved(2)
   end type
 

@@ -122,6 +122,10 @@ program elmm_host
   cfg%temp_btype = [ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_NEUMANN, &
                     ELMM_BC_NEUMANN]
   cfg%side_temp = 0
+  cfg%enable_moisture = 0; cfg%moist_pad = 0; cfg%moisture_ref = 0.01_c_double
+  cfg%moist_btype = [ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_PERIODIC, ELMM_BC_NEUMANN, &
+                     ELMM_BC_NEUMANN]
+  cfg%side_moist = 0
   cfg%reserved = 0
 
   allocate(cub(max(1, n_cuboids)), src(max(1, n_sources)))

@@ -58,6 +58,7 @@ enum { ELMM_POISSON_FFT = 2, ELMM_POISSON_MG = 3 };
 enum {
   ELMM_F_U = 0, ELMM_F_V = 1, ELMM_F_W = 2, ELMM_F_PR = 3,
   ELMM_F_VISC = 4, ELMM_F_TDIFF = 5, ELMM_F_PHI = 6, ELMM_F_Q = 7,
+  ELMM_F_MOISTURE = 14,    /* with enable_moisture */
   ELMM_F_TEMPERATURE = 15, /* with enable_buoyancy */
   ELMM_F_SCALAR = 16      /* + scalar index (0-based) */
 };
@@ -136,6 +137,18 @@ typedef struct elmm_config {
   int32_t temp_btype[6];     /* TempBtype (periodic x/y; Dirichlet or Neumann
                                 in z)                                        */
   double  side_temp[6];      /* sideTemp (value or gradient)                  */
+  int32_t enable_moisture;   /* 1 (with enable_buoyancy): moisture as an
+                                active scalar: its RK3 stage (src/scalars.f90:
+                                202-208), Moisture = moisture_ref in solid
+                                cells after it, and the virtual temperature
+                                T (1 + 0.61 q) in the buoyancy force
+                                (src/dynamics.f90:620-641, 695-710, 766-770);
+                                no liquid water                               */
+  int32_t moist_btype[6];    /* MoistBtype (periodic x/y; Dirichlet or Neumann
+                                in z)                                        */
+  int32_t moist_pad;         /* (alignment; 0)                                */
+  double  moisture_ref;      /* moisture_ref                                  */
+  double  side_moist[6];     /* sideMoist (value or gradient)                 */
   int32_t reserved[2];
 } elmm_config;
 

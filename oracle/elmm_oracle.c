@@ -239,8 +239,9 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
   o->Q = field(o);
   o->Phi = field(o);
   o->RHS = field(o);
-  /* scalar slots: the passive scalars, then the temperature (buoyancy) */
-  int ns = cfg->nscalars + (cfg->enable_buoyancy ? 1 : 0);
+  /* scalar slots: the passive scalars, then the temperature (buoyancy),
+     then the moisture */
+  int ns = cfg->nscalars + (cfg->enable_buoyancy ? 1 : 0) + (cfg->enable_buoyancy && cfg->enable_moisture ? 1 : 0);
   o->S = xcalloc(ns + 1, sizeof(double *));
   o->S2 = xcalloc(ns + 1, sizeof(double *));
   o->Sadv = xcalloc(ns + 1, sizeof(double *));
@@ -316,7 +317,8 @@ void oracle_destroy(oracle *o) {
                     o->dzW, o->U, o->V, o->W, o->U2, o->V2, o->W2, o->Us, o->Vs, o->Ws, o->Pr, o->Visc,
                     o->TDiff, o->Q, o->Phi, o->RHS};
   for (size_t i = 0; i < sizeof arrs / sizeof arrs[0]; i++) free(arrs[i]);
-  for (int s = 0; s < o->c.nscalars + (o->c.enable_buoyancy ? 1 : 0); s++) {
+  for (int s = 0; s < o->c.nscalars + (o->c.enable_buoyancy ? 1 : 0) + (o->c.enable_buoyancy && o->c.enable_moisture ? 1 : 0);
+       s++) {
     free(o->S[s]);
     free(o->S2[s]);
     free(o->Sadv[s]);
@@ -366,6 +368,7 @@ static double *field_ptr(oracle *o, int32_t f) {
     default:
       if (f >= ELMM_F_SCALAR && f < ELMM_F_SCALAR + o->c.nscalars) return o->S[f - ELMM_F_SCALAR];
       if (f == ELMM_F_TEMPERATURE && o->c.enable_buoyancy) return o->S[o->c.nscalars];
+      if (f == ELMM_F_MOISTURE && o->c.enable_buoyancy && o->c.enable_moisture) return o->S[o->c.nscalars + 1];
       if (f >= ORACLE_F_SADV && f < ORACLE_F_SADV + o->c.nscalars) return o->Sadv[f - ORACLE_F_SADV];
   }
   return NULL;
@@ -630,6 +633,9 @@ static void bound_scalar_bc(oracle *o, double *a, const int *bt, const double *s
    sideTemp; src/scalarboundaries.f90:296-308): the same rules. */
 static void bound_scalar(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.scal_btype, o->c.side_scal); }
 static void bound_temperature(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.temp_btype, o->c.side_temp); }
+/* BoundMoisture, src/scalarboundaries.f90:311-323: the same rules with
+   MoistBtype and sideMoist */
+static void bound_moisture(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.moist_btype, o->c.side_moist); }
 
 /* BoundViscosity, src/scalarboundaries.f90:326-431 */
 static void bound_visc(oracle *o, double *a) {
@@ -1587,10 +1593,29 @@ static void momentum_diffusion_vz(oracle *o) {
 
 /* Convection (advection_method 4 on uniform grids, 2 on stretched z; explicit
    diffusion), src/dynamics.f90:230-499 */
-/* BuoyancyForce with buoyancy only, 2nd order (src/dynamics.f90:608-700):
-   W += A/2 (T(k+1) + T(k)) - A T_ref, A = grav_acc / temperature_ref. */
+/* BuoyancyForce, 2nd order (src/dynamics.f90:608-700), A = grav_acc /
+   temperature_ref.  Buoyancy only: W += A/2 (T(k+1) + T(k)) - A T_ref.
+   With moisture (apply_moist, src/dynamics.f90:695-710): the virtual
+   temperature Tv = T (1 + 0.61 q) (theta_v, 766-770) is scattered to the two
+   W faces of each cell in two passes over k (odd k first, then even), each
+   W(k) += A/2 Tv(k) - A T_ref and W(k-1) += A/2 Tv(k) for k up to Wnz+1. */
+static double theta_v(oracle *o, const double *T, const double *q, int i, int j, int k) {
+  return A3(o, T, i, j, k) * (1.0 + 0.61 * A3(o, q, i, j, k));
+}
 static void buoyancy_force(oracle *o, double *Ws, const double *T) {
   const double A = o->c.grav_acc / o->c.temperature_ref, A2 = A / 2.0;
+  if (o->c.enable_moisture) {
+    const double *q = o->S[o->c.nscalars + 1];
+    for (int start = 1; start <= 2; start++)
+      for (int k = start; k <= o->Wn[2] + 1; k += 2)
+        for (int j = 1; j <= o->Wn[1]; j++)
+          for (int i = 1; i <= o->Wn[0]; i++) {
+            const double tv = theta_v(o, T, q, i, j, k);
+            A3(o, Ws, i, j, k) = A3(o, Ws, i, j, k) + A2 * tv - A * o->c.temperature_ref;
+            A3(o, Ws, i, j, k - 1) = A3(o, Ws, i, j, k - 1) + A2 * tv;
+          }
+    return;
+  }
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= o->Wn[2]; k++)
     for (int j = 1; j <= o->Wn[1]; j++)
@@ -1832,14 +1857,20 @@ static void scalar_diffusion(oracle *o, double *Sadv, const double *S) {
 
 /* ScalarRK3 stage for passive scalars, src/scalars.f90:88-299, 364-379 */
 /* One scalar's RK3 stage (the `stage` procedure of ScalarRK3,
-   src/scalars.f90:233-299): slot sc (the temperature when temp: its
+   src/scalars.f90:233-299): slot sc of kind 0 (passive), 1 (the temperature:
+   its boundary conditions, no point sources) or 2 (the moisture: its
    boundary conditions, no point sources). */
-static void scalar_stage(oracle *o, int sc, int stage, double dt, int temp) {
-  double *S = o->S[sc], *S2 = o->S2[sc], *Sadv = o->Sadv[sc];
-  if (temp)
+static void bound_kind(oracle *o, double *S, int kind) {
+  if (kind == 1)
     bound_temperature(o, S);
+  else if (kind == 2)
+    bound_moisture(o, S);
   else
     bound_scalar(o, S);
+}
+static void scalar_stage(oracle *o, int sc, int stage, double dt, int kind) {
+  double *S = o->S[sc], *S2 = o->S2[sc], *Sadv = o->Sadv[sc];
+  bound_kind(o, S, kind);
   if (stage > 0) {
     double a = RK_rho[stage] * dt;
 #pragma omp parallel for
@@ -1857,7 +1888,7 @@ static void scalar_stage(oracle *o, int sc, int stage, double dt, int temp) {
   /* ScalarExtra / TemperatureExtra: volume (point) sources, then the WM
      divergence term (src/scalars.f90:301-331, 364-379) */
   for (int p = 0; p < o->src_n; p++) {
-    if (temp || o->src_sc[p] != sc) continue;
+    if (kind || o->src_sc[p] != sc) continue;
     const int32_t *q = &o->src_ijk[3 * p];
     A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->src_flux[p];
   }
@@ -1870,10 +1901,7 @@ static void scalar_stage(oracle *o, int sc, int stage, double dt, int temp) {
   for (long p = 0; p < o->ntot; p++) S2[p] = S2[p] + Sadv[p] * b;
 #pragma omp parallel for
   for (long p = 0; p < o->ntot; p++) S[p] = S[p] + S2[p];
-  if (temp)
-    bound_temperature(o, S);
-  else
-    bound_scalar(o, S);
+  bound_kind(o, S, kind);
 }
 
 /* The temperature in solid cells after its stage (src/scalars.f90:131-207):
@@ -1899,8 +1927,18 @@ static void temperature_solid_fix(oracle *o, double *T) {
       }
 }
 
+/* The moisture in solid cells after its stage (src/scalars.f90:208):
+   moisture_ref over the whole array -2..n+3. */
+static void moisture_solid_fill(oracle *o, double *q) {
+  for (int k = -2; k <= o->nz + 3; k++)
+    for (int j = -2; j <= o->ny + 3; j++)
+      for (int i = -2; i <= o->nx + 3; i++)
+        if (A3(o, o->Prtype, i, j, k) > 0) A3(o, q, i, j, k) = o->c.moisture_ref;
+}
+
 /* ScalarRK3, src/scalars.f90:88-299: DivergenceWM, the temperature stage
-   and its solid-cell values (buoyancy), then the passive scalars. */
+   and its solid-cell values (buoyancy), the moisture stage and its solid
+   cells, then the passive scalars. */
 static void scalar_rk3(oracle *o, int stage, double dt) {
   const int nsc = o->c.nscalars, buoy = o->c.enable_buoyancy != 0;
   if (nsc == 0 && !buoy) return;
@@ -1908,6 +1946,10 @@ static void scalar_rk3(oracle *o, int stage, double dt) {
   if (buoy) {
     scalar_stage(o, nsc, stage, dt, 1);
     temperature_solid_fix(o, o->S[nsc]);
+  }
+  if (buoy && o->c.enable_moisture) {
+    scalar_stage(o, nsc + 1, stage, dt, 2);
+    moisture_solid_fill(o, o->S[nsc + 1]);
   }
   for (int sc = 0; sc < nsc; sc++) scalar_stage(o, sc, stage, dt, 0);
 }
@@ -2786,6 +2828,7 @@ int oracle_init(oracle *o) {
     bound_visc(o, o->TDiff);
   }
   if (o->c.enable_buoyancy) bound_temperature(o, o->S[o->c.nscalars]);
+  if (o->c.enable_buoyancy && o->c.enable_moisture) bound_moisture(o, o->S[o->c.nscalars + 1]);
   for (int s = 0; s < o->c.nscalars; s++) bound_scalar(o, o->S[s]);
   wm_visc_points(o, 1);
   wm_uvw_fluxes(o);
@@ -2975,6 +3018,12 @@ void oracle_op_buoyancy(oracle *o) {
 }
 void oracle_op_temp_fix(oracle *o) {
   if (o->c.enable_buoyancy) temperature_solid_fix(o, o->S[o->c.nscalars]);
+}
+void oracle_op_moist_fill(oracle *o) {
+  if (o->c.enable_buoyancy && o->c.enable_moisture) moisture_solid_fill(o, o->S[o->c.nscalars + 1]);
+}
+void oracle_op_bound_moisture(oracle *o) {
+  if (o->c.enable_buoyancy && o->c.enable_moisture) bound_moisture(o, o->S[o->c.nscalars + 1]);
 }
 void oracle_op_bound_temperature(oracle *o) {
   if (o->c.enable_buoyancy) bound_temperature(o, o->S[o->c.nscalars]);

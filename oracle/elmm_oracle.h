@@ -71,6 +71,8 @@ void oracle_op_wm_visc(oracle *o);              /* ComputeViscsWM (u*, Visc)    
 void oracle_op_buoyancy(oracle *o);             /* BuoyancyForce on Wstar       */
 void oracle_op_temp_fix(oracle *o);             /* temperature in solid cells   */
 void oracle_op_bound_temperature(oracle *o);    /* BoundTemperature             */
+void oracle_op_moist_fill(oracle *o);           /* moisture in solid cells      */
+void oracle_op_bound_moisture(oracle *o);       /* BoundMoisture                */
 int32_t oracle_op_poisson_mg(oracle *o, const double *rhs_interior, double *phi_interior, double *res);
 void oracle_mg_tables(int nz, double ax, double ay, const double *az, const double *cz, double *den, double *e);
 int32_t oracle_wm_ustar(oracle *o, double *out); /* WMPoints%ustar -> out (n)   */

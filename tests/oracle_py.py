@@ -53,7 +53,8 @@ def _load():
         lib.oracle_op_poisson_mg.argtypes = [C.c_void_p, pd, pd, pd]
         lib.oracle_op_poisson_mg.restype = C.c_int32
         lib.oracle_op_wm_visc.argtypes = [C.c_void_p]
-        for name in ("oracle_op_buoyancy", "oracle_op_temp_fix", "oracle_op_bound_temperature"):
+        for name in ("oracle_op_buoyancy", "oracle_op_temp_fix", "oracle_op_bound_temperature",
+                     "oracle_op_moist_fill", "oracle_op_bound_moisture"):
             getattr(lib, name).argtypes = [C.c_void_p]
         lib.oracle_wm_ustar.argtypes = [C.c_void_p, pd]
         lib.oracle_wm_ustar.restype = C.c_int32
@@ -179,6 +180,12 @@ class Oracle:
 
     def op_bound_temperature(self):
         _lib.oracle_op_bound_temperature(self._h)
+
+    def op_moist_fill(self):
+        _lib.oracle_op_moist_fill(self._h)
+
+    def op_bound_moisture(self):
+        _lib.oracle_op_bound_moisture(self._h)
 
     def op_scalar_extra(self, sc=0):
         _lib.oracle_op_scalar_extra(self._h, sc)
