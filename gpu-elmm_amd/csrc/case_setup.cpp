@@ -127,6 +127,17 @@ std::string validate_config(const elmm_config &c) {
   if (c.nscalars < 0 || c.nscalars > 8) return "nscalars must be in 0..8";
   if (c.precision != ELMM_FP64 && c.precision != ELMM_FP32_REFINED) return "unknown precision";
   if (c.enable_moisture && !c.enable_buoyancy) return "moisture needs enable_buoyancy (the temperature)";
+  // A wall with wall-model points (no-slip, wall model on) takes a Dirichlet
+  // or flux temperature into its points (prescribed surface temperature or
+  // heat flux: the Monin-Obukhov laws, src/wallmodels.f90:592-622, 2086-2118)
+  // and resets the temperature's ghost rule there; those laws are not built,
+  // so the temperature must be Neumann on such walls (neutral law, the
+  // Neumann ghost rule kept).
+  if (c.enable_buoyancy && c.wall_model)
+    for (int s = ELMM_WE; s <= ELMM_TO; s++)
+      if (c.btype[s] == ELMM_BC_NOSLIP && c.temp_btype[s] != ELMM_BC_NEUMANN)
+        return "the temperature must be Neumann on walls with wall-model points (side " + std::to_string(s) +
+               "; the Monin-Obukhov wall laws are not built)";
   if (c.variable_dt) {
     if (!(c.cfl_target > 0) || !(c.dt_max > 0)) return "variable time steps need cfl_target > 0 and dt_max > 0";
   } else if (!(c.dt > 0)) {

@@ -22,15 +22,26 @@ from oracle_py import F_WSTAR, Oracle
 OBST = [(8.0, 16.0, 8.0, 16.0, 0.0, 6.0), (22.0, 30.0, 18.0, 28.0, 0.0, 9.0, 0.02)]
 
 
+T_BOT, T_TOP = 293.0, 299.0
+
+
+def temperature_bcs(nz, dz=1.0):
+    """Periodic x/y; the bottom (a wall with wall-model points, where the
+    temperature has to be Neumann) with the gradient of the linear profile
+    from T_BOT at z = 0 to T_TOP at z = nz dz; the free-slip top Dirichlet
+    T_TOP."""
+    return dict(temp_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.DIRICHLET],
+                side_temp=[0, 0, 0, 0, (T_TOP - T_BOT) / (nz * dz), T_TOP])
+
+
 def buoyant_config(nx=32, ny=32, nz=16, **kw):
     args = dict(dx=1.0, dt=0.05, nscalars=1, enable_buoyancy=1, grav_acc=9.81, temperature_ref=295.0,
-                temp_btype=[capi.PERIODIC] * 4 + [capi.DIRICHLET, capi.DIRICHLET],
-                side_temp=[0, 0, 0, 0, 293.0, 299.0])
+                **temperature_bcs(nz))
     args.update(kw)
     return cases.uniform_config(nx, ny, nz, **args)
 
 
-def temperature_field(cfg, seed=3, amp=0.2, t_bot=293.0, t_top=299.0):
+def temperature_field(cfg, seed=3, amp=0.2, t_bot=T_BOT, t_top=T_TOP):
     """Linear profile between the Dirichlet values at the walls (z = 0 and
     z = nz dz) plus seeded noise; ghosted, lower bound -2."""
     shape = (cfg.nx + 6, cfg.ny + 6, cfg.nz + 6)
@@ -142,6 +153,15 @@ def test_hydrostatic_check_catches_a_sign_error():
     want = cfg.dz * A * ((T[:, :, 1:] + T[:, :, :-1]) / 2 - cfg.temperature_ref)
     assert np.abs(np.diff(Pr, axis=2) - want).max() > 1e-3
     o.close()
+
+
+def test_dirichlet_temperature_on_a_wall_model_wall_is_rejected():
+    """A Dirichlet temperature on the no-slip bottom (wall-model points)
+    would switch the reference to the Monin-Obukhov law there."""
+    cfg = buoyant_config(temp_btype=[capi.PERIODIC] * 4 + [capi.DIRICHLET, capi.DIRICHLET],
+                         side_temp=[0, 0, 0, 0, T_BOT, T_TOP])
+    with pytest.raises(capi.ElmmError, match="Neumann on walls with wall-model points"):
+        capi.Case(cfg)
 
 
 # ------------------------------------------------------------------ GPU

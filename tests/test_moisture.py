@@ -21,16 +21,14 @@ import pytest
 from elmm_amd import capi, cases
 from helpers import compare_state, velocity_rel_l2
 from oracle_py import F_WSTAR, Oracle
-from test_buoyancy import OBST, temperature_field
+from test_buoyancy import OBST, temperature_bcs, temperature_field
 
 Q_BOT, Q_TOP = 0.012, 0.008
 
 
 def moist_config(nx=32, ny=32, nz=16, **kw):
     args = dict(dx=1.0, dt=0.05, nscalars=1, enable_buoyancy=1, grav_acc=9.81, temperature_ref=295.0,
-                temp_btype=[capi.PERIODIC] * 4 + [capi.DIRICHLET, capi.DIRICHLET],
-                side_temp=[0, 0, 0, 0, 293.0, 299.0],
-                enable_moisture=1, moisture_ref=0.01,
+                **temperature_bcs(nz), enable_moisture=1, moisture_ref=0.01,
                 moist_btype=[capi.PERIODIC] * 4 + [capi.DIRICHLET, capi.NEUMANN],
                 side_moist=[0, 0, 0, 0, Q_BOT, 0.0])
     args.update(kw)
