@@ -225,27 +225,38 @@ def cpu_baseline(wl, cfg, case, fields, threads, warmup=1, steps=5):
     the host cores: after the initial projection, `warmup` untimed steps,
     then `steps` timed steps (reduced-step sample of the workload), with the
     OpenMP threads bound to cores (OMP_PROC_BIND=close, OMP_PLACES=cores; set
-    before the oracle library and its OpenMP runtime are loaded)."""
+    before the oracle library and its OpenMP runtime are loaded).
+
+    Binding pins the calling thread to the first core when the first parallel
+    region starts; the process's CPU mask is put back afterwards, since every
+    thread started later (the engine's set-up threads, the server's workers)
+    inherits the mask of the thread that starts it."""
+    allowed = os.sched_getaffinity(0)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ["OMP_PROC_BIND"] = "close"
     os.environ["OMP_PLACES"] = "cores"
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import Oracle
-    o = Oracle(cfg, case)
-    cases.upload_initial(o, cfg, fields)
-    o.init()
-    if warmup:
-        o.step(warmup)
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
+    try:
+        o = Oracle(cfg, case)
+        cases.upload_initial(o, cfg, fields)
+        o.init()
+        if warmup:
+            o.step(warmup)
+        t0 = time.perf_counter()
+        o.step(steps)
+        dt = time.perf_counter() - t0
+        o.close()
+    finally:
+        os.sched_setaffinity(0, allowed)
+        for k in ("OMP_PROC_BIND", "OMP_PLACES"):
+            os.environ.pop(k, None)
     cells = cfg.nx * cfg.ny * cfg.nz
-    o.close()
     return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
             "sample": f"reduced-step: {steps} timed steps after {warmup} warm-up step(s) of {wl.name} "
                       f"({cfg.nx}x{cfg.ny}x{cfg.nz}) following the initial projection; oracle C/OpenMP FP64, "
                       f"{dt:.1f} s, {dt / steps:.2f} s per step",
-            "nproc": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)), "cpu_model": _cpu_model(),
+            "nproc": os.cpu_count(), "cpus_allowed": len(allowed), "cpu_model": _cpu_model(),
             "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}}
 
 
