@@ -335,7 +335,7 @@ static void point_sources(const elmm_config &c, const Grid1D &g, CaseData &cd, i
 }
 
 int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc, const elmm_point_source *src,
-               CaseData &cd) {
+               CaseData &cd, bool device) {
   std::string err = validate_config(c);
   if (!err.empty()) return fail(ELMM_E_ARG, err);
   if (c.stretched_z && ncub > 0) return fail(ELMM_E_ARG, "solid bodies on stretched z grids are not built in this version");
@@ -348,12 +348,15 @@ int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc,
   cd.ntot = cd.sz * (c.nz + 6);
   component_extents(c, cd.n);
   Grid1D g = make_grid(c);
-  cd.prtype.assign(cd.ntot, 0);
-  cd.utype.assign(cd.ntot, 0);
-  cd.vtype.assign(cd.ntot, 0);
-  cd.wtype.assign(cd.ntot, 0);
+  const bool on_device = device && ncub > 0;
+  if (!on_device) {  // the device builder writes every element
+    cd.prtype.assign(cd.ntot, 0);
+    cd.utype.assign(cd.ntot, 0);
+    cd.vtype.assign(cd.ntot, 0);
+    cd.wtype.assign(cd.ntot, 0);
+  }
   if (ncub > 0) {
-    int rc = build_solids(c, g, ncub, cub, cd);
+    int rc = on_device ? build_solids_device(c, g, ncub, cub, cd) : build_solids(c, g, ncub, cub, cd);
     if (rc) return rc;
   }
   if (c.wall_model) {
@@ -363,7 +366,7 @@ int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc,
     outer_pr(c, g, cd);
     remove_duplicate_wm_points(c, cd.wmp);
   }
-  init_scalar_masks(c, cd);
+  if (!(on_device && c.wall_model)) init_scalar_masks(c, cd);  // else built with the solids
   point_sources(c, g, cd, nsrc, src);
   return 0;
 }
@@ -436,14 +439,14 @@ int64_t case_list_bytes(const elmm_case *c) {
 }
 }  // namespace elmm
 
-extern "C" int elmm_case_build(const elmm_config *cfg, int32_t n_cuboids, const elmm_cuboid *cuboids,
-                               int32_t n_sources, const elmm_point_source *sources, elmm_case **out) {
+static int case_build(const elmm_config *cfg, int32_t n_cuboids, const elmm_cuboid *cuboids, int32_t n_sources,
+                      const elmm_point_source *sources, elmm_case **out, bool device) {
   if (!cfg || !out || (n_cuboids > 0 && !cuboids) || (n_sources > 0 && !sources))
     return elmm::fail(ELMM_E_ARG, "null argument to elmm_case_build");
   *out = nullptr;
   try {
     auto *c = new elmm_case;
-    int rc = elmm::build_case(*cfg, n_cuboids, cuboids, n_sources, sources, c->data);
+    int rc = elmm::build_case(*cfg, n_cuboids, cuboids, n_sources, sources, c->data, device);
     if (rc) {
       delete c;
       return rc;
@@ -453,6 +456,16 @@ extern "C" int elmm_case_build(const elmm_config *cfg, int32_t n_cuboids, const 
   } catch (const std::bad_alloc &) {
     return elmm::fail(ELMM_E_NOMEM, "out of host memory in elmm_case_build");
   }
+}
+
+extern "C" int elmm_case_build(const elmm_config *cfg, int32_t n_cuboids, const elmm_cuboid *cuboids,
+                               int32_t n_sources, const elmm_point_source *sources, elmm_case **out) {
+  return case_build(cfg, n_cuboids, cuboids, n_sources, sources, out, false);
+}
+
+extern "C" int elmm_case_build_device(const elmm_config *cfg, int32_t n_cuboids, const elmm_cuboid *cuboids,
+                                      int32_t n_sources, const elmm_point_source *sources, elmm_case **out) {
+  return case_build(cfg, n_cuboids, cuboids, n_sources, sources, out, true);
 }
 
 extern "C" int elmm_case_view(const elmm_case *c, elmm_lists *view) {

@@ -55,14 +55,19 @@ Grid1D make_grid(const elmm_config &c);
 void component_extents(const elmm_config &c, int n[3][3]);
 std::string validate_config(const elmm_config &c);
 void remove_duplicate_wm_points(const elmm_config &c, WMList &L);
+// device: the solid-body masks and lists on the current HIP device
+// (build_solids_device) instead of the host (build_solids); the same result.
 int build_case(const elmm_config &c, int ncub, const elmm_cuboid *cub, int nsrc, const elmm_point_source *src,
-               CaseData &cd);
+               CaseData &cd, bool device = false);
 void view_case(const CaseData &cd, elmm_lists &v);
 // Estimated device bytes of a case's surface lists (elmm_create's copies).
 int64_t case_list_bytes(const elmm_case *c);
 // Solid bodies (axis-aligned cuboids): masks, IB lists, solid-body WM points,
-// nullified points.  Defined in solids.cpp.
+// nullified points.  Defined in solids.hip.
 int build_solids(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cuboid *cub, CaseData &cd);
+// The same on the current HIP device, and the scalar face masks and lists
+// when the wall model is on (defined in k_solids.hip).
+int build_solids_device(const elmm_config &c, const Grid1D &g, int ncub, const elmm_cuboid *cub, CaseData &cd);
 // Runs f(0) .. f(n-1) on n threads (f(0) on the caller's) and joins them; an
 // exception in a task is rethrown after every task has finished.
 void run_tasks(int n, const std::function<void(int)> &f);

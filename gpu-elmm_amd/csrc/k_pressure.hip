@@ -298,9 +298,10 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(PoissonPlan<R> pp, int in
 constexpr int ZSEG = 16;
 constexpr int kSolveZMaxNz = 1024;  // 112 KiB of LDS at the maximum
 
-__host__ __device__ inline size_t solve_z_lds(int nz) {
+__host__ __device__ constexpr size_t solve_z_lds(int nz) {
   return (size_t)(3 * ((nz + ZSEG - 1) / ZSEG) * 64 + 2 * nz) * sizeof(double);
 }
+static_assert(solve_z_lds(kSolveZMaxNz) <= 160 * 1024, "the z solve's LDS at the largest nz exceeds a CU's 160 KiB");
 
 template <typename R, bool PIPE>
 __global__ void __launch_bounds__(64) k_solve_z(PoissonPlan<R> pp) {

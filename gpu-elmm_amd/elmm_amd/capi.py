@@ -122,6 +122,8 @@ class PointSource(C.Structure):
 EXPORTS = {
     "elmm_case_build": (C.c_int, [C.POINTER(Config), C.c_int32, C.POINTER(Cuboid), C.c_int32,
                                   C.POINTER(PointSource), C.POINTER(C.c_void_p)]),
+    "elmm_case_build_device": (C.c_int, [C.POINTER(Config), C.c_int32, C.POINTER(Cuboid), C.c_int32,
+                                         C.POINTER(PointSource), C.POINTER(C.c_void_p)]),
     "elmm_case_view": (C.c_int, [C.c_void_p, C.POINTER(Lists)]),
     "elmm_case_free": (None, [C.c_void_p]),
     "elmm_create": (C.c_int, [C.POINTER(Config), C.POINTER(Lists), C.POINTER(C.c_void_p)]),
@@ -234,16 +236,18 @@ def _np(ptr, n, dtype):
 
 
 class Case:
-    """Masks and point lists of one case (elmm_case_build)."""
+    """Masks and point lists of one case (elmm_case_build; device=True:
+    elmm_case_build_device, the solid-body part built on the current GPU)."""
 
-    def __init__(self, cfg, cuboids=(), sources=()):
+    def __init__(self, cfg, cuboids=(), sources=(), device=False):
         lib = load()
         self.cfg = cfg
         cub = (Cuboid * max(1, len(cuboids)))(*[Cuboid(*c) for c in cuboids])
         src = (PointSource * max(1, len(sources)))(
             *[PointSource(s[0], s[1], s[2], s[3], int(s[4]) if len(s) > 4 else 0, 0) for s in sources])
         h = C.c_void_p()
-        _check(lib.elmm_case_build(C.byref(cfg), len(cuboids), cub, len(sources), src, C.byref(h)))
+        build = lib.elmm_case_build_device if device else lib.elmm_case_build
+        _check(build(C.byref(cfg), len(cuboids), cub, len(sources), src, C.byref(h)))
         self._h = h
         self.view = Lists()
         _check(lib.elmm_case_view(h, C.byref(self.view)))

@@ -238,6 +238,21 @@ int  elmm_case_build(const elmm_config *cfg,
                      int32_t n_cuboids, const elmm_cuboid *cuboids,
                      int32_t n_sources, const elmm_point_source *sources,
                      elmm_case **out);
+/* elmm_case_build with the solid-body masks and lists (FindInsideCells,
+ * FindNeighbouringCells, InitImBoundaries, GetSolidBodiesWM(_UVW),
+ * SetNullifiedPoints, the scalar part of InitWMMasks) built on the calling
+ * thread's current HIP device; the result is the same as elmm_case_build's,
+ * byte for byte and in the same order, and lives in host memory like it.
+ *   replaces: as elmm_case_build; FindInsideCells (src/solidbodies.f90:
+ *             75-210), FindNeighbouringCells (src/immersedboundary.f90:
+ *             1475-1506), InitImBoundaries / TVelIBPoint_Create
+ *             (1650-1747, 690-1020), GetSolidBodiesWM(_UVW) (14-308),
+ *             InitWMMasks (src/wallmodels.f90:911-1113) and
+ *             SetNullifiedPoints (src/initial.f90:4861-4962) */
+int  elmm_case_build_device(const elmm_config *cfg,
+                            int32_t n_cuboids, const elmm_cuboid *cuboids,
+                            int32_t n_sources, const elmm_point_source *sources,
+                            elmm_case **out);
 int  elmm_case_view(const elmm_case *c, elmm_lists *view);
 void elmm_case_free(elmm_case *c);
 
