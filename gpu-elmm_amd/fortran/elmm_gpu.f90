@@ -13,7 +13,7 @@ module elmm_gpu
 
   private
   public :: elmm_config, elmm_cuboid, elmm_point_source, elmm_diag
-  public :: elmm_case_build, elmm_case_free, elmm_create_case, elmm_destroy, elmm_init, elmm_step
+  public :: elmm_case_build, elmm_case_build_device, elmm_case_free, elmm_create_case, elmm_destroy, elmm_init, elmm_step
   public :: elmm_upload, elmm_download, elmm_divergence
   public :: elmm_put, elmm_get, elmm_check, elmm_error_message
   public :: ELMM_WE, ELMM_EA, ELMM_SO, ELMM_NO, ELMM_BO, ELMM_TO
@@ -82,6 +82,18 @@ module elmm_gpu
   interface
     integer(c_int) function elmm_case_build(cfg, n_cuboids, cuboids, n_sources, sources, case) &
         bind(C, name="elmm_case_build")
+      import
+      type(elmm_config), intent(in) :: cfg
+      integer(c_int32_t), value :: n_cuboids
+      type(elmm_cuboid), intent(in) :: cuboids(*)
+      integer(c_int32_t), value :: n_sources
+      type(elmm_point_source), intent(in) :: sources(*)
+      type(c_ptr), intent(out) :: case
+    end function
+
+    ! the same case, its solid-body masks and lists built on the GPU
+    integer(c_int) function elmm_case_build_device(cfg, n_cuboids, cuboids, n_sources, sources, case) &
+        bind(C, name="elmm_case_build_device")
       import
       type(elmm_config), intent(in) :: cfg
       integer(c_int32_t), value :: n_cuboids

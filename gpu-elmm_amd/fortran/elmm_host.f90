@@ -15,7 +15,8 @@
 ! reference's zgrid.txt (src/initial.f90:254-279); it sets nz and a stretched
 ! z grid (which needs advection_method = 2).
 !   &time_stepping dt, nsteps, report_every
-!   &obstacles   n_cuboids, cuboids(7, n) = xmin xmax ymin ymax zmin zmax z0
+!   &obstacles   n_cuboids, cuboids(7, n) = xmin xmax ymin ymax zmin zmax z0,
+!                device_lists (masks and lists built on the GPU; default .true.)
 !   &point_sources n_sources, sources(5, n) = x y z flux scalar(1-based)
 !   &io          init_file, out_file, u0
 ! init_file (optional): stream of float64 U, V, W (lb -2), Pr (lb -1),
@@ -33,6 +34,7 @@ program elmm_host
   real(c_double) :: c_smag = 0.122_c_double
   real(c_double) :: z0_bottom = 0.01_c_double, dt = 0.05_c_double, u0 = 1
   integer :: nsteps = 10, report_every = 5, n_cuboids = 0, n_sources = 0
+  logical :: device_lists = .true.
   integer, parameter :: maxobj = 256
   real(c_double) :: cuboids(7, maxobj) = 0, sources(5, maxobj) = 0
   character(len=512) :: init_file = "", out_file = "", zgrid_file = ""
@@ -43,7 +45,7 @@ program elmm_host
   namelist /physics/ nu, nu_scalar, c_smag, z0_bottom, z0_top, top_bc, nscalars, precision, advection_method, &
                      pr_gradient_x
   namelist /time_stepping/ dt, nsteps, report_every
-  namelist /obstacles/ n_cuboids, cuboids
+  namelist /obstacles/ n_cuboids, cuboids, device_lists
   namelist /point_sources/ n_sources, sources
   namelist /io/ init_file, out_file, u0
 
@@ -137,8 +139,13 @@ program elmm_host
     src(i) = elmm_point_source(sources(1, i), sources(2, i), sources(3, i), sources(4, i), &
                                int(sources(5, i)) - 1, 0)
   end do
-  call elmm_check(elmm_case_build(cfg, int(n_cuboids, c_int32_t), cub, int(n_sources, c_int32_t), src, case), &
-                  "elmm_case_build")
+  if (device_lists) then
+    call elmm_check(elmm_case_build_device(cfg, int(n_cuboids, c_int32_t), cub, int(n_sources, c_int32_t), &
+                                           src, case), "elmm_case_build_device")
+  else
+    call elmm_check(elmm_case_build(cfg, int(n_cuboids, c_int32_t), cub, int(n_sources, c_int32_t), src, case), &
+                    "elmm_case_build")
+  end if
   call elmm_check(elmm_create_case(cfg, case, h), "elmm_create")
 
   ! the reference's bounds: velocities and scalars -2:n+3, pressure -1:n+2
