@@ -22,13 +22,14 @@ def test_interoperable_sizes_match_header():
                      C.sizeof(capi.Diag)]
 
 
-def _namelist(path, n, nz, steps, cuboids=(), sources=(), init="", out="", precision=0):
+def _namelist(path, n, nz, steps, cuboids=(), sources=(), init="", out="", precision=0, device_lists=None):
     lines = [f"&grid nx={n}, ny={n}, nz={nz}, dx=1.0, dy=1.0, dz=1.0 /",
              f"&physics nscalars=1, precision={precision} /",
              f"&time_stepping dt=0.05, nsteps={steps}, report_every={steps} /"]
     if cuboids:
         vals = ", ".join(f"{v:.17g}" for c in cuboids for v in (list(c) + [0.0] * (7 - len(c))))
-        lines.append(f"&obstacles n_cuboids={len(cuboids)}, cuboids={vals} /")
+        dl = "" if device_lists is None else f", device_lists={'.true.' if device_lists else '.false.'}"
+        lines.append(f"&obstacles n_cuboids={len(cuboids)}, cuboids={vals}{dl} /")
     if sources:
         vals = ", ".join(f"{v:.17g}" for s in sources for v in (s[0], s[1], s[2], s[3], s[4] + 1))
         lines.append(f"&point_sources n_sources={len(sources)}, sources={vals} /")
@@ -38,14 +39,18 @@ def _namelist(path, n, nz, steps, cuboids=(), sources=(), init="", out="", preci
 
 
 @needs_host
-def test_host_reports_library_errors(tmp_path):
+@pytest.mark.parametrize("device_lists", [None, False])
+def test_host_reports_library_errors(tmp_path, device_lists):
     """A bad case (solid with empty extent) stops the host with the
-    library's message (error_stop semantics of the reference)."""
+    library's message (error_stop semantics of the reference), from the
+    device case builder (the default) and the host one; the cuboids are
+    checked before any device work, so no GPU is needed."""
     nml = tmp_path / "bad.nml"
-    _namelist(nml, 16, 8, 1, cuboids=[(5.0, 2.0, 1.0, 4.0, 0.0, 3.0)])
+    _namelist(nml, 16, 8, 1, cuboids=[(5.0, 2.0, 1.0, 4.0, 0.0, 3.0)], device_lists=device_lists)
     r = subprocess.run([HOST, str(nml)], capture_output=True, text=True)
     assert r.returncode != 0
-    assert "elmm_case_build failed" in r.stdout and "cuboid" in r.stdout
+    entry = "elmm_case_build" if device_lists is False else "elmm_case_build_device"
+    assert f"{entry} failed" in r.stdout and "cuboid" in r.stdout
 
 
 OBSTACLES = [(8.0, 16.0, 8.0, 16.0, 0.0, 6.0), (20.0, 26.0, 18.0, 30.0, 0.0, 10.0, 0.01)]
