@@ -91,6 +91,11 @@ void component_extents(const elmm_config &c, int n[3][3]) {
   n[2][2] = c.btype[ELMM_TO] == ELMM_BC_PERIODIC ? c.nz : c.nz - 1;
 }
 
+bool mo_wall(const elmm_config &c, int s) {
+  return c.enable_buoyancy && c.wall_model && c.btype[s] == ELMM_BC_NOSLIP &&
+         (c.temp_btype[s] == ELMM_BC_DIRICHLET || c.temp_btype[s] == ELMM_BC_CONSTFLUX);
+}
+
 std::string validate_config(const elmm_config &c) {
   if (c.nx < 4 || c.ny < 4 || c.nz < 4) return "grid must be at least 4 cells in every direction";
   if (!(c.dx > 0) || !(c.dy > 0)) return "dx and dy must be positive";
@@ -129,15 +134,19 @@ std::string validate_config(const elmm_config &c) {
   if (c.enable_moisture && !c.enable_buoyancy) return "moisture needs enable_buoyancy (the temperature)";
   // A wall with wall-model points (no-slip, wall model on) takes a Dirichlet
   // or flux temperature into its points (prescribed surface temperature or
-  // heat flux: the Monin-Obukhov laws, src/wallmodels.f90:592-622, 2086-2118)
-  // and resets the temperature's ghost rule there; those laws are not built,
-  // so the temperature must be Neumann on such walls (neutral law, the
-  // Neumann ghost rule kept).
-  if (c.enable_buoyancy && c.wall_model)
-    for (int s = ELMM_WE; s <= ELMM_TO; s++)
-      if (c.btype[s] == ELMM_BC_NOSLIP && c.temp_btype[s] != ELMM_BC_NEUMANN)
-        return "the temperature must be Neumann on walls with wall-model points (side " + std::to_string(s) +
-               "; the Monin-Obukhov wall laws are not built)";
+  // heat flux: the Monin-Obukhov laws, src/wallmodels.f90:592-622,
+  // 2086-2118); a flux temperature exists only there.  The moist variant of
+  // those laws (surface moisture and virtual heat flux) is not built.
+  if (c.enable_buoyancy)
+    for (int s = ELMM_BO; s <= ELMM_TO; s++) {
+      const bool wm_wall = c.wall_model && c.btype[s] == ELMM_BC_NOSLIP;
+      if (c.temp_btype[s] == ELMM_BC_CONSTFLUX && !wm_wall)
+        return "a temperature flux (ELMM_BC_CONSTFLUX) needs a no-slip wall with the wall model (side " +
+               std::to_string(s) + ")";
+      if (mo_wall(c, s) && c.enable_moisture)
+        return "a Dirichlet or flux temperature on a wall with wall-model points needs the moist Monin-Obukhov "
+               "law with moisture (side " + std::to_string(s) + "), which is not built";
+    }
   if (c.variable_dt) {
     if (!(c.cfl_target > 0) || !(c.dt_max > 0)) return "variable time steps need cfl_target > 0 and dt_max > 0";
   } else if (!(c.dt > 0)) {
@@ -154,6 +163,24 @@ struct UVWPoint {
   double z0;
 };
 
+// Thermal class of the wall-model points of domain side s (the velocity
+// lists' direction dir equals the side): a prescribed surface temperature
+// (Dirichlet) or heat flux (CONSTFLUX, positive upward: the top wall's is
+// -sideTemp(To)) on a Monin-Obukhov wall, else neutral
+// (src/wallmodels.f90:592-602, 652-662, 821-831, 872-882).
+void wall_thermal(const elmm_config &c, int s, int8_t &tc, double &tv) {
+  tc = ELMM_WM_NEUTRAL;
+  tv = 0;
+  if (s < ELMM_BO || !mo_wall(c, s)) return;
+  if (c.temp_btype[s] == ELMM_BC_DIRICHLET) {
+    tc = ELMM_WM_SURFACE_TEMP;
+    tv = c.side_temp[s];
+  } else {
+    tc = ELMM_WM_SURFACE_FLUX;
+    tv = s == ELMM_BO ? c.side_temp[s] : -c.side_temp[s];
+  }
+}
+
 // GetOutsideBoundariesWM_UVW helper, src/wallmodels.f90:640-904
 void outer_uvw(const elmm_config &c, const Grid1D &g, CaseData &cd, int comp) {
   const int *n = cd.n[comp];
@@ -163,10 +190,10 @@ void outer_uvw(const elmm_config &c, const Grid1D &g, CaseData &cd, int comp) {
   auto Y = [&](int j) { return comp == 1 ? g.at(g.yV, j) : g.at(g.yPr, j); };
   auto Z = [&](int k) { return comp == 2 ? g.at(g.zW, k) : g.at(g.zPr, k); };
   auto add = [&](int dir, int i, int j, int k, double dx, double dy, double dz, double z0) {
-    auto &L = cd.wm[comp * 6 + dir];
-    L.ijk.insert(L.ijk.end(), {i, j, k});
-    L.dist.insert(L.dist.end(), {dx, dy, dz});
-    L.z0.push_back(z0);
+    int8_t tc;
+    double tv;
+    wall_thermal(c, dir, tc, tv);
+    cd.wm[comp * 6 + dir].push(i, j, k, dx, dy, dz, z0, tc, tv);
   };
   auto t = [&](int i, int j, int k) { return T[cd.lin(i, j, k)]; };
   if (c.btype[ELMM_WE] == ELMM_BC_NOSLIP)
@@ -201,39 +228,40 @@ void outer_uvw(const elmm_config &c, const Grid1D &g, CaseData &cd, int comp) {
 // GetOutsideBoundariesWM_Pr, src/wallmodels.f90:467-637
 void outer_pr(const elmm_config &c, const Grid1D &g, CaseData &cd) {
   const int nx = c.nx, ny = c.ny, nz = c.nz;
-  auto add = [&](int i, int j, int k, double dx, double dy, double dz, double z0) {
-    cd.wmp.ijk.insert(cd.wmp.ijk.end(), {i, j, k});
-    cd.wmp.dist.insert(cd.wmp.dist.end(), {dx, dy, dz});
-    cd.wmp.z0.push_back(z0);
+  auto add = [&](int side, int i, int j, int k, double dx, double dy, double dz, double z0) {
+    int8_t tc;
+    double tv;
+    wall_thermal(c, side, tc, tv);
+    cd.wmp.push(i, j, k, dx, dy, dz, z0, tc, tv);
   };
   auto t = [&](int i, int j, int k) { return cd.prtype[cd.lin(i, j, k)]; };
   if (c.btype[ELMM_WE] == ELMM_BC_NOSLIP)
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny; j++)
-        if (t(1, j, k) == 0) add(1, j, k, g.at(g.xU, 0) - g.at(g.xPr, 1), 0, 0, c.z0_wall[ELMM_WE]);
+        if (t(1, j, k) == 0) add(ELMM_WE, 1, j, k, g.at(g.xU, 0) - g.at(g.xPr, 1), 0, 0, c.z0_wall[ELMM_WE]);
   if (c.btype[ELMM_EA] == ELMM_BC_NOSLIP)
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny; j++)
         if (t(nx, j, k) == 0)
-          add(nx, j, k, g.at(g.xU, cd.n[0][0] + 1) - g.at(g.xPr, nx), 0, 0, c.z0_wall[ELMM_EA]);
+          add(ELMM_EA, nx, j, k, g.at(g.xU, cd.n[0][0] + 1) - g.at(g.xPr, nx), 0, 0, c.z0_wall[ELMM_EA]);
   if (c.btype[ELMM_SO] == ELMM_BC_NOSLIP)
     for (int k = 1; k <= nz; k++)
       for (int i = 1; i <= nx; i++)
-        if (t(i, 1, k) == 0) add(i, 1, k, 0, g.at(g.yV, 0) - g.at(g.yPr, 1), 0, c.z0_wall[ELMM_SO]);
+        if (t(i, 1, k) == 0) add(ELMM_SO, i, 1, k, 0, g.at(g.yV, 0) - g.at(g.yPr, 1), 0, c.z0_wall[ELMM_SO]);
   if (c.btype[ELMM_NO] == ELMM_BC_NOSLIP)
     for (int k = 1; k <= nz; k++)
       for (int i = 1; i <= nx; i++)
         if (t(i, ny, k) == 0)
-          add(i, ny, k, 0, g.at(g.yV, cd.n[1][1] + 1) - g.at(g.yPr, ny), 0, c.z0_wall[ELMM_NO]);
+          add(ELMM_NO, i, ny, k, 0, g.at(g.yV, cd.n[1][1] + 1) - g.at(g.yPr, ny), 0, c.z0_wall[ELMM_NO]);
   if (c.btype[ELMM_BO] == ELMM_BC_NOSLIP)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++)
-        if (t(i, j, 1) == 0) add(i, j, 1, 0, 0, g.at(g.zW, 0) - g.at(g.zPr, 1), c.z0_wall[ELMM_BO]);
+        if (t(i, j, 1) == 0) add(ELMM_BO, i, j, 1, 0, 0, g.at(g.zW, 0) - g.at(g.zPr, 1), c.z0_wall[ELMM_BO]);
   if (c.btype[ELMM_TO] == ELMM_BC_NOSLIP)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++)
         if (t(i, j, nz) == 0)
-          add(i, j, nz, 0, 0, g.at(g.zW, cd.n[2][2] + 1) - g.at(g.zPr, nz), c.z0_wall[ELMM_TO]);
+          add(ELMM_TO, i, j, nz, 0, 0, g.at(g.zW, cd.n[2][2] + 1) - g.at(g.zPr, nz), c.z0_wall[ELMM_TO]);
 }
 
 }  // namespace
@@ -268,9 +296,8 @@ void remove_duplicate_wm_points(const elmm_config &c, WMList &L) {
       if (L.ijk[3 * r] == L.ijk[3 * p] && L.ijk[3 * r + 1] == L.ijk[3 * p + 1] && L.ijk[3 * r + 2] == L.ijk[3 * p + 2])
         continue;
     }
-    out.ijk.insert(out.ijk.end(), {L.ijk[3 * p], L.ijk[3 * p + 1], L.ijk[3 * p + 2]});
-    out.dist.insert(out.dist.end(), {L.dist[3 * p], L.dist[3 * p + 1], L.dist[3 * p + 2]});
-    out.z0.push_back(L.z0[p]);
+    out.push(L.ijk[3 * p], L.ijk[3 * p + 1], L.ijk[3 * p + 2], L.dist[3 * p], L.dist[3 * p + 1], L.dist[3 * p + 2],
+             L.z0[p], L.tclass[p], L.tval[p]);
   }
   L = std::move(out);
 }
@@ -384,6 +411,12 @@ void view_case(const CaseData &cd, elmm_lists &v) {
     v.wm_dist[l] = cd.wm[l].dist.data();
     v.wm_z0[l] = cd.wm[l].z0.data();
   }
+  for (int l = 0; l < 18; l++) {
+    v.wm_tclass[l] = cd.wm[l].tclass.data();
+    v.wm_tval[l] = cd.wm[l].tval.data();
+  }
+  v.wmp_tclass = cd.wmp.tclass.data();
+  v.wmp_tval = cd.wmp.tval.data();
   v.wmp_n = (int32_t)cd.wmp.z0.size();
   v.wmp_ijk = cd.wmp.ijk.data();
   v.wmp_dist = cd.wmp.dist.data();

@@ -25,7 +25,24 @@ struct WMList {
   std::vector<int32_t> ijk;  // 3 per point
   std::vector<double> dist;  // 3 per point
   std::vector<double> z0;
+  std::vector<int8_t> tclass;  // ELMM_WM_* per point
+  std::vector<double> tval;    // surface temperature or heat flux per point
+  // a point with the neutral thermal class
+  void push(int i, int j, int k, double dx, double dy, double dz, double z, int8_t tc = 0, double tv = 0) {
+    ijk.insert(ijk.end(), {i, j, k});
+    dist.insert(dist.end(), {dx, dy, dz});
+    z0.push_back(z);
+    tclass.push_back(tc);
+    tval.push_back(tv);
+  }
 };
+
+// The bottom (s = ELMM_BO) or top (ELMM_TO) wall takes the temperature into
+// its wall-model points (a prescribed surface temperature or heat flux, the
+// Monin-Obukhov laws) and the temperature's ghost rule there becomes the
+// mirror (TempBtype = 0): buoyancy, the wall model, a no-slip wall and a
+// Dirichlet or flux temperature (src/wallmodels.f90:574-625, 636-680).
+bool mo_wall(const elmm_config &c, int s);
 
 struct IBList {
   std::vector<int32_t> ijk;      // target, 3 per point

@@ -384,19 +384,31 @@ struct Engine : EngineIface {
   DevBuf coords;
   // wall model
   DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
+  DevBuf wm_tclass, wm_tval, wm_tflux, wmp_tclass, wmp_tval, wmp_tflux, wmp_af;
+  bool mo_any = false;  // some wall-model point has a thermal class (buoyancy)
   DevBuf width_tab, width_idx, sgs_rdz, sgs_rz2, sgs_rdzf, sgs_rz2f;
   SgsWidth sgs_width{nullptr, nullptr, 0};
   int wm_n = 0;
   DevBuf wmp_ijk, wmp_div, wmp_dist, wmp_z0, wmp_logdz0, wmp_ustar;
   int wmp_n = 0;
+  MOParams mo_params(const DevBuf &tc, const DevBuf &tv, const DevBuf &tf) const {
+    return MOParams{cfg.enable_buoyancy ? 1 : 0, tc.as<int8_t>(), tv.as<double>(), tf.as<double>(), cfg.grav_acc,
+                    cfg.temperature_ref, cfg.nu_scalar};
+  }
+  // the temperature's wall source tflux * af goes to src_flux[src_n + p]
   WMUstarPoints wmpdev() const {
     return WMUstarPoints{wmp_n,
                          wmp_ijk.as<int32_t>(),
                          wmp_dist.as<double>(),
                          wmp_z0.as<double>(),
                          wmp_logdz0.as<double>(),
-                         wmp_ustar.as<double>()};
+                         wmp_ustar.as<double>(),
+                         mo_params(wmp_tclass, wmp_tval, wmp_tflux),
+                         wmp_af.as<double>(),
+                         tslot >= 0 ? src_flux.as<double>() + src_n : nullptr};
   }
+  int src_n = 0;
+  const T *temp_cur() const { return tslot >= 0 ? s_cur[tslot] : nullptr; }
   DevTerms mom_terms[3];
   std::vector<std::unique_ptr<DevTerms>> sc_terms;
   DevBuf src_flux;
@@ -464,7 +476,8 @@ struct Engine : EngineIface {
                     wm_ustar.as<double>(),
                     wm_fluxp.as<double>(),
                     wm_fluxm.as<double>(),
-                    wm_logdz0.as<double>()};
+                    wm_logdz0.as<double>(),
+                    mo_params(wm_tclass, wm_tval, wm_tflux)};
   }
   bool has_ib() const { return ib_n[0] + ib_n[1] + ib_n[2] > 0; }
 
@@ -982,7 +995,8 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   };
   // wall-model points of the 18 momentum lists, concatenated in list order
   std::vector<int32_t> wijk, wlist;
-  std::vector<double> wdist, wz0, wustar;
+  std::vector<double> wdist, wz0, wustar, wtval, wtflux;
+  std::vector<int8_t> wtclass;
   std::vector<int> list_off(19, 0);
   for (int l = 0; l < 18; l++) {
     list_off[l] = (int)wlist.size();
@@ -992,8 +1006,18 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       wdist.insert(wdist.end(), {L.wm_dist[l][3 * p], L.wm_dist[l][3 * p + 1], L.wm_dist[l][3 * p + 2]});
       wz0.push_back(L.wm_z0[l][p]);
       wustar.push_back(1.0);  // WMpointUVW%ustar = 1 (src/wallmodels.f90:79)
+      // thermal class; temperature_flux = sideTemp on a flux wall, else 0
+      const int8_t tc = L.wm_tclass[l] ? L.wm_tclass[l][p] : 0;
+      const double tv = L.wm_tclass[l] ? L.wm_tval[l][p] : 0.0;
+      wtclass.push_back(tc);
+      wtval.push_back(tv);
+      wtflux.push_back(tc == ELMM_WM_SURFACE_FLUX ? tv : 0.0);
+      if (tc != ELMM_WM_NEUTRAL && cfg.enable_buoyancy) mo_any = true;
     }
   }
+  upload_vec(wm_tclass, wtclass);
+  upload_vec(wm_tval, wtval);
+  upload_vec(wm_tflux, wtflux);
   list_off[18] = (int)wlist.size();
   wm_n = (int)wlist.size();
   std::vector<double> wlog(wm_n, 0.0);
@@ -1088,8 +1112,50 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       upload_vec(wmp_z0, pz);
       upload_vec(wmp_logdz0, plog);
       upload_vec(wmp_ustar, pu);
+      // thermal class, value, persistent heat flux and ComputeAreaFactor
+      // (src/wallmodels.f90:170-203; norm2 as the square root of the sum of
+      // squares): the wall area over the cell volume from the dominant
+      // component of the wall normal
+      std::vector<int8_t> tc(wmp_n, 0);
+      std::vector<double> tv(wmp_n, 0.0), tf(wmp_n, 0.0), af(wmp_n, 0.0);
+      const double dxm = g.dxmin, dym = g.dymin, dzm = g.dzmin;
+      for (int p = 0; p < wmp_n; p++) {
+        if (L.wmp_tclass) {
+          tc[p] = L.wmp_tclass[p];
+          tv[p] = L.wmp_tval[p];
+          tf[p] = tc[p] == ELMM_WM_SURFACE_FLUX ? tv[p] : 0.0;
+          if (tc[p] != ELMM_WM_NEUTRAL && cfg.enable_buoyancy) mo_any = true;
+        }
+        const double *d = &pd[3 * p];
+        if (-d[0] == 0 && -d[1] == 0 && -d[2] == 0) continue;
+        const double V = dxm * dym * dzm;
+        const double nrm = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double n0 = std::fabs(-d[0] / nrm), n1 = std::fabs(-d[1] / nrm), n2 = std::fabs(-d[2] / nrm);
+        double on, dd;
+        if (n0 > n1 && n0 > n2) {
+          on = n0;
+          dd = std::sqrt(dym * dzm);
+        } else if (n1 > n0 && n1 > n2) {
+          on = n1;
+          dd = std::sqrt(dxm * dzm);
+        } else {
+          on = n2;
+          dd = std::sqrt(dxm * dym);
+        }
+        const double l = dd / on;
+        const double S = dd * l;
+        af[p] = S / V;
+      }
+      upload_vec(wmp_tclass, tc);
+      upload_vec(wmp_tval, tv);
+      upload_vec(wmp_tflux, tf);
+      upload_vec(wmp_af, af);
     }
+    // point sources, then (with buoyancy) one slot per wall-model point for
+    // the temperature's wall source written by k_wm_ustar every stage
+    src_n = L.src_n;
     std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
+    if (tslot >= 0) sflux.resize(src_n + wmp_n, 0.0);
     upload_vec(src_flux, sflux);
     if (tslot >= 0) {
       // solid cells of -1..n+2 with a fluid neighbour there: their temperature
@@ -1136,6 +1202,9 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
         }
         for (int p = 0; p < wmp_n; p++) {
           int q[3] = {L.wmp_ijk[3 * p], L.wmp_ijk[3 * p + 1], L.wmp_ijk[3 * p + 2]};
+          // TemperatureExtra: the surface heat flux times the area factor,
+          // then the divergence term (src/scalars.f90:322-328)
+          if (s == tslot) tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {2, L.src_n + p, 0, 0});
           tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {3, p, 0, 0});
         }
       }
@@ -1325,11 +1394,14 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     for (int s = 0; s < 4; s++)
       if (cfg.temp_btype[s] != ELMM_BC_PERIODIC) throw ArgError("the temperature must be periodic in x and y");
     for (int s = 4; s < 6; s++)
-      if (cfg.temp_btype[s] != ELMM_BC_DIRICHLET && cfg.temp_btype[s] != ELMM_BC_NEUMANN)
-        throw ArgError("temperature z boundaries must be Dirichlet or Neumann");
+      if (cfg.temp_btype[s] != ELMM_BC_DIRICHLET && cfg.temp_btype[s] != ELMM_BC_NEUMANN &&
+          !(cfg.temp_btype[s] == ELMM_BC_CONSTFLUX && mo_wall(cfg, s)))
+        throw ArgError("temperature z boundaries must be Dirichlet, Neumann, or a flux on a wall-model wall");
     gtemp = g;
     for (int s = 0; s < 6; s++) {
-      gtemp.sbt[s] = cfg.temp_btype[s];
+      // a wall that takes the temperature into its wall-model points has
+      // TempBtype = 0: the mirror ghost rule (src/wallmodels.f90:618-620)
+      gtemp.sbt[s] = mo_wall(cfg, s) ? 0 : cfg.temp_btype[s];
       gtemp.side_scal[s] = cfg.side_temp[s];
     }
   }
@@ -1574,14 +1646,15 @@ void Engine<T>::subgrid_stresses(bool cfl) {
   // ComputeViscsWM: WMPoints%ustar (the viscosity it writes is overwritten
   // by SubgridModel)
   if (cfg.wall_model && wmp_n)
-    timed(ELMM_T_WALL, [&] { launch_wm_ustar<T>(g, u, v, w, wmpdev(), cfg.nu, (T *)nullptr, stream); });
+    timed(ELMM_T_WALL, [&] { launch_wm_ustar<T>(g, u, v, w, temp_cur(), wmpdev(), cfg.nu, (T *)nullptr, stream); });
   // constant dt with the CFL check: the step's CFL number and the divergence
   // test, on the device (graph-safe)
   if (cfl && !cfg.variable_dt)
     timed(ELMM_T_REDUCE, [&] {
       launch_cfl_monitor(cfl_partial.as<double>(), cfl_blocks, dt_cur, cfg.cfl_max, mon, stream);
     });
-  if (cfg.wall_model && wm_n) timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream); });
+  if (cfg.wall_model && wm_n)
+    timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, temp_cur(), wmdev(), cfg.nu, g.variable_z, stream); });
   timed(ELMM_T_GHOST, [&] { launch_bound_visc<T>(g, Visc.as<T>(), stream); });
   if (ib_n[3])
     timed(ELMM_T_IB, [&] {
@@ -1739,8 +1812,8 @@ void Engine<T>::init() {
     launch_bound_visc<T>(g, TDiff.as<T>(), stream);
   }
   for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(gscal(s), s_cur[s], stream);
-  if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, wmpdev(), cfg.nu, Visc.as<T>(), stream);
-  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, wmdev(), cfg.nu, g.variable_z, stream);
+  if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, temp_cur(), wmpdev(), cfg.nu, Visc.as<T>(), stream);
+  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, temp_cur(), wmdev(), cfg.nu, g.variable_z, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));

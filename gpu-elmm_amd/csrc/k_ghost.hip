@@ -224,6 +224,10 @@ __device__ inline ScalRule scal_rule(const Geo &g, int axis, int m) {
     r.src = low ? 1 - m : 2 * n + 1 - m;
     return r;
   }
+  if (bt == 0) {  // TempBtype 0 (a Monin-Obukhov wall): the mirror
+    r.src = low ? 1 - m : 2 * n + 1 - m;
+    return r;
+  }
   // Neumann: ghost = arr(1) - side*mult*d (low), arr(n) + side*mult*d (high)
   r.kind = 1;
   r.src = low ? 1 : n;

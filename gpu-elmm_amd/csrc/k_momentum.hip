@@ -407,9 +407,169 @@ __device__ inline double wm_rough_ustar(double ustar, double vel, double dist, d
   return ustar;
 }
 
+// exp(x) with fixed operations, the same as the oracle's (host and device
+// libm differ by an ulp): x = k ln2 + r, k = rint(x / ln2), r = (x - k
+// ln2_hi) - k ln2_lo (ln2_hi of 33 significant bits), |r| <= ln2/2; exp(r)
+// by its Taylor polynomial to r^13/13! in Horner form; times 2^k (exact).
+// The reduction of W. J. Cody and W. Waite (1980).
+__device__ inline double det_exp(double x) {
+  const double inv_ln2 = 0x1.71547652b82fep+0;
+  const double ln2_hi = 0x1.62e42fee00000p-1, ln2_lo = 0x1.a39ef35793c76p-33;
+  const double k = rint(x * inv_ln2);
+  const double r = (x - k * ln2_hi) - k * ln2_lo;
+  double p = 1.0 / 6227020800.0;
+  p = 1.0 / 479001600.0 + r * p;
+  p = 1.0 / 39916800.0 + r * p;
+  p = 1.0 / 3628800.0 + r * p;
+  p = 1.0 / 362880.0 + r * p;
+  p = 1.0 / 40320.0 + r * p;
+  p = 1.0 / 5040.0 + r * p;
+  p = 1.0 / 720.0 + r * p;
+  p = 1.0 / 120.0 + r * p;
+  p = 1.0 / 24.0 + r * p;
+  p = 1.0 / 6.0 + r * p;
+  p = 0.5 + r * p;
+  p = 1.0 + r * p;
+  p = 1.0 + r * p;
+  return ldexp(p, (int)k);
+}
+
+// atan(x) with fixed operations (the oracle's): 1/a above 1, the pi/6 shift
+// above tan(pi/12), then the Taylor series to t^31/31.
+__device__ inline double det_atan(double x) {
+  const double pi2 = 0x1.921fb54442d18p+0, pi6 = 0x1.0c152382d7365p-1;
+  const double sqrt3 = 0x1.bb67ae8584caap+0, tan12 = 0x1.126145e9ecd58p-2;
+  double a = fabs(x), off = 0;
+  bool inv = false;
+  if (a > 1) {
+    a = 1 / a;
+    inv = true;
+  }
+  if (a > tan12) {
+    a = (a * sqrt3 - 1) / (a + sqrt3);
+    off = pi6;
+  }
+  const double z = a * a;
+  double p = -1.0 / 31.0;
+  p = 1.0 / 29.0 + z * p;
+  p = -1.0 / 27.0 + z * p;
+  p = 1.0 / 25.0 + z * p;
+  p = -1.0 / 23.0 + z * p;
+  p = 1.0 / 21.0 + z * p;
+  p = -1.0 / 19.0 + z * p;
+  p = 1.0 / 17.0 + z * p;
+  p = -1.0 / 15.0 + z * p;
+  p = 1.0 / 13.0 + z * p;
+  p = -1.0 / 11.0 + z * p;
+  p = 1.0 / 9.0 + z * p;
+  p = -1.0 / 7.0 + z * p;
+  p = 1.0 / 5.0 + z * p;
+  p = -1.0 / 3.0 + z * p;
+  p = 1.0 + z * p;
+  double r = off + a * p;
+  if (inv) r = pi2 - r;
+  return x < 0 ? -r : r;
+}
+
+// x**y (x >= 0, y > 0) as exp(y log x); 0 for x = 0
+__device__ inline double det_pow(double x, double y) { return x == 0 ? 0.0 : det_exp(y * det_log(x)); }
+
+// Monin-Obukhov laws, src/wallmodels.f90:1546-1716 (pi = acos(-1.0_knd),
+// k_U = 0.4, k_T = 0.47; the flux law's eps = 1e-3, 1E-5, 1E-4 and the
+// Dirichlet law's 1.1 are default-real literals)
+__device__ inline double mo_psi_unstable(double x) {
+  const double c = (1 + x) / 2.0;
+  return (det_log(((1 + x * x) / 2.0) * (c * c)) - 2.0 * det_atan(x)) + 0x1.921fb54442d18p+1 / 2;
+}
+
+__device__ inline double psim_mo(double zeta) {
+  return zeta < 0 ? mo_psi_unstable(det_pow(1 - 15.0 * zeta, 0.25)) : -(4.8 * zeta);
+}
+
+__device__ inline double psim_mo_mod(double zeta, double kU) {
+  return zeta < 0 ? mo_psi_unstable(det_pow(1 - (6.0 * zeta) * kU, 0.25)) : -(3 * det_pow(zeta * kU, 5.0 / 6.0));
+}
+
+__device__ inline double psih_mo_mod(double zeta, double kU) {
+  if (zeta < 0) {
+    const double x = det_pow(1 - (6.0 * zeta) * kU, 0.25);
+    return 2.0 * det_log((1 + x * x) / 2.0);
+  }
+  return -(2.5 * det_pow(zeta * kU, 4.0 / 5.0));
+}
+
+__device__ inline double mo_flux_ustar(double vel, double dist, double ustar, double z0, double tflux, double tref,
+                                       double g, double nu) {
+  const double eps = (double)1e-3f, yplcrit = 11.225;
+  if (dist <= z0) {
+    if (nu > 0)
+      return (dist * ustar / nu) < yplcrit ? sqrt(nu * vel / dist)
+                                           : vel / (det_log(fabs(ustar * dist / nu)) / 0.4 + 5.2);
+    return 0;
+  }
+  double zL = 0, Psi = 0;
+  for (int i = 1;; i++) {
+    const double ustar2 = ustar;
+    double m = dist / z0 - Psi;
+    if (!(m > (double)1e-5f)) m = (double)1e-5f;
+    double t = vel * 0.4 / det_log(m);
+    if (!(t > 0)) t = 0;
+    ustar = ustar + (t - ustar) / 2;
+    if (ustar < (double)1e-4f) {
+      zL = -10000;
+    } else {
+      const double zL2 = (dist * ((0.4 * (g / tref)) * tflux)) / (-((ustar * ustar) * ustar));
+      zL = zL + (zL2 - zL) / 2;
+    }
+    Psi = psim_mo(zL);
+    double den = fabs(ustar);
+    if (!(den > 1e-3)) den = 1e-3;
+    if (fabs(ustar - ustar2) / den < eps) break;
+    if (i >= 50) {
+      ustar = 0;
+      break;
+    }
+  }
+  return ustar;
+}
+
+// WM_MO_Dirichlet_ustar_tfl with z0H = z0 (logz0 = log(dist/z0), host libm)
+__device__ inline void mo_dirichlet(double &ustar, double &tflux, double dist, double z0, double logz0, double vel,
+                                    double tempdif, double nu, double kappa, double g, double tref) {
+  const double eps = 1e-3, yplcrit = 11.225, kU = 0.4, kT = 0.47;
+  ustar = wm_rough_ustar(ustar, vel, dist, z0, nu, logz0);
+  double dist_plus = 1.7976931348623157e308;
+  if (nu > 0) {
+    const double ustar_lam = sqrt(nu * vel / dist);
+    dist_plus = dist * ustar_lam / nu;
+  }
+  if (dist <= z0 || dist_plus < (double)1.1f * yplcrit) {
+    tflux = -((kappa * tempdif) / dist);
+    return;
+  }
+  double zL = 0, zL0 = -10000;
+  int i = 0;
+  for (;;) {
+    i = i + 1;
+    zL = -(((dist * g) * tflux) / ((kU * tref) * ((ustar * ustar) * ustar)));
+    const double psi_m = psim_mo_mod(zL, kU), psi_h = psih_mo_mod(zL, kU);
+    ustar = (vel * kU) / (logz0 - psi_m);
+    tflux = -(((tempdif * ustar) * kT) / (logz0 - psi_h));
+    double den = fabs(zL);
+    if (!(den > 1e-4)) den = 1e-4;
+    if (i > 1 && fabs(zL - zL0) / den < eps) break;
+    if (i >= 50 || zL > 10000) break;
+    zL0 = zL;
+  }
+  if (i >= 50 || zL > 10000) {
+    ustar = sqrt(nu * vel / dist);
+    tflux = -((kappa * tempdif) / dist);
+  }
+}
+
 template <typename T>
 __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict__ V_, const T *__restrict__ W_,
-                            WMPoints wm, double nu, int variable_z) {
+                            const T *__restrict__ Temp, WMPoints wm, double nu, int variable_z) {
   // surface lists: evaluated in FP64 whatever the field type
   auto U = [&](long q) { return (double)U_[q]; };
   auto V = [&](long q) { return (double)V_[q]; };
@@ -446,8 +606,21 @@ __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict
   const double vel = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
   double ustar = wm.ustar[p];
   const double z0 = wm.z0[p];
-  if (vel != 0)
+  if (z0 > 0 && wm.mo.buoy && wm.mo.tclass[p] == ELMM_WM_SURFACE_TEMP) {
+    // WM_MO_DirichletStress with the temperature at the velocity point
+    // (local_value, src/wallmodels.f90:2340-2353)
+    const long off = comp == 0 ? 1 : comp == 1 ? sy : sz;
+    const double temp = ((double)Temp[q + off] + (double)Temp[q]) / 2;
+    double tf = wm.mo.tflux[p];
+    mo_dirichlet(ustar, tf, dist, z0, wm.logdz0[p], vel, temp - wm.mo.tval[p], nu, wm.mo.kappa, wm.mo.g,
+                 wm.mo.tref);
+    wm.mo.tflux[p] = tf;
+  } else if (z0 > 0 && wm.mo.buoy && wm.mo.tflux[p] > 0) {
+    // WM_MO_FluxStress
+    ustar = vel != 0 ? mo_flux_ustar(vel, dist, ustar, z0, wm.mo.tflux[p], wm.mo.tref, wm.mo.g, nu) : 0.0;
+  } else if (vel != 0) {
     ustar = z0 > 0 ? wm_rough_ustar(ustar, vel, dist, z0, nu, wm.logdz0[p]) : wm_flat_ustar(ustar, vel, dist, nu);
+  }
   if (ustar < 0) ustar = 0;
   wm.ustar[p] = ustar;
   const double eps = 1.4901161193847656e-08;  // sqrt(epsilon(1._knd))
@@ -475,10 +648,10 @@ __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict
 }
 
 template <typename T>
-void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,
-                      hipStream_t st) {
+void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMPoints wm, double nu,
+                      int variable_z, hipStream_t st) {
   if (wm.n == 0) return;
-  hipLaunchKernelGGL(k_wm_fluxes<T>, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, wm, nu, variable_z);
+  hipLaunchKernelGGL(k_wm_fluxes<T>, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, Temp, wm, nu, variable_z);
 }
 
 // ComputeViscsWM (src/wallmodels.f90:2050-2152; neutral, wall velocity 0,
@@ -490,7 +663,7 @@ void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints
 // keeps only u*.  FP64 whatever the field type.
 template <typename T>
 __global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict__ V_, const T *__restrict__ W_,
-                           WMUstarPoints wp, double nu, T *__restrict__ Visc) {
+                           const T *__restrict__ Temp, WMUstarPoints wp, double nu, T *__restrict__ Visc) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wp.n) return;
   const int xi = wp.ijk[3 * p], yj = wp.ijk[3 * p + 1], zk = wp.ijk[3 * p + 2];
@@ -513,7 +686,32 @@ __global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict_
     t1 = t1 - dot * d1 / dd;
     t2 = t2 - dot * d2 / dd;
     const double vel = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    if (z0 > 0) {
+    if (z0 > 0 && wp.mo.buoy && wp.mo.tclass[p] == ELMM_WM_SURFACE_TEMP) {
+      // WM_MO_Dirichlet (src/wallmodels.f90:1858-1884)
+      double tf = wp.mo.tflux[p];
+      mo_dirichlet(ustar, tf, dist, z0, wp.logdz0[p], vel, (double)Temp[q] - wp.mo.tval[p], nu, wp.mo.kappa,
+                   wp.mo.g, wp.mo.tref);
+      wp.mo.tflux[p] = tf;
+      if (ustar < 0) ustar = 0;
+      if (vel > 0 || tf != 0) {
+        const double a = ustar * ustar * dist / vel;
+        visc = a > nu ? a : nu;
+      } else {
+        visc = nu > 0 ? nu : 0.0;
+      }
+    } else if (z0 > 0 && wp.mo.buoy && wp.mo.tflux[p] > 0) {
+      // WM_MO_FLUX (src/wallmodels.f90:1792-1832; gravity replaced by the
+      // cosine of the wall normal with the vertical, as the reference does)
+      if (vel != 0) {
+        const double g_cos = fabs(d2) / dist;
+        ustar = mo_flux_ustar(vel, dist, ustar, z0, wp.mo.tflux[p], wp.mo.tref, g_cos, nu);
+        if (ustar < 0) ustar = 0;
+        visc = ustar * ustar * dist / vel > nu ? ustar * ustar * dist / vel : (nu > 0 ? nu : 0.0);
+      } else {
+        ustar = 0;
+        visc = nu > 0 ? nu : 0.0;
+      }
+    } else if (z0 > 0) {
       if (vel != 0) ustar = wm_rough_ustar(ustar, vel, dist, z0, nu, wp.logdz0[p]);
       if (ustar < 0) ustar = 0;
       if (vel > 0 && ustar * ustar * dist / vel > nu)
@@ -531,13 +729,15 @@ __global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict_
   }
   wp.ustar[p] = ustar;
   if (Visc) Visc[q] = (T)visc;
+  // TemperatureExtra's surface source (src/scalars.f90:322-326)
+  if (wp.heat) wp.heat[p] = wp.mo.tflux[p] * wp.af[p];
 }
 
 template <typename T>
-void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, WMUstarPoints wp, double nu, T *Visc,
-                     hipStream_t st) {
+void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMUstarPoints wp, double nu,
+                     T *Visc, hipStream_t st) {
   if (wp.n == 0) return;
-  hipLaunchKernelGGL(k_wm_ustar<T>, dim3(nblk(wp.n, 256)), dim3(256), 0, st, g, U, V, W, wp, nu, Visc);
+  hipLaunchKernelGGL(k_wm_ustar<T>, dim3(nblk(wp.n, 256)), dim3(256), 0, st, g, U, V, W, Temp, wp, nu, Visc);
 }
 
 // ------------------------------------------------------ CD4 advection
@@ -1119,10 +1319,10 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
 #define ELMM_INST_MOMENTUM(T)                                                                                       \
   template void launch_sgs<T>(const Geo &, const T *, const T *, const T *, T *, double, double, double,           \
                               SgsWidth, int, double *, int, hipStream_t);                                          \
-  template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, WMPoints, double, int,           \
+  template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, const T *, WMPoints, double, int, \
                                     hipStream_t);                                                                  \
-  template void launch_wm_ustar<T>(const Geo &, const T *, const T *, const T *, WMUstarPoints, double, T *,      \
-                                   hipStream_t);                                                                   \
+  template void launch_wm_ustar<T>(const Geo &, const T *, const T *, const T *, const T *, WMUstarPoints, double, \
+                                   T *, hipStream_t);                                                              \
   template void launch_momentum<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *, T *,   \
                                    T *, const uint8_t *, StageConst, hipStream_t);                                 \
   template void launch_momentum_corr3<T>(const Geo &, const T *, const T *, const T *, const T *, const T *,      \

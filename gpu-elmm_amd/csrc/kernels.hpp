@@ -5,6 +5,7 @@
 #include <algorithm>
 
 #include "device_common.hpp"
+#include "elmm_gpu.h"
 
 namespace elmm {
 
@@ -26,6 +27,18 @@ template <typename T> void launch_bound_visc(const Geo &g, T *F, hipStream_t st)
 template <typename T> void launch_bound_scalar(const Geo &g, T *F, hipStream_t st);
 template <typename T> void launch_bound_pr(const Geo &g, T *P, hipStream_t st);
 
+// Thermal state of wall-model points for the Monin-Obukhov laws
+// (src/wallmodels.f90:1546-1966): with buoyancy, a point of class
+// ELMM_WM_SURFACE_TEMP (surface temperature tval) takes the Dirichlet law,
+// one whose persistent heat flux tflux is positive the flux law.
+struct MOParams {
+  int buoy;                // 0: neutral laws only
+  const int8_t *tclass;    // ELMM_WM_* per point
+  const double *tval;      // surface temperature per point
+  double *tflux;           // WMPoint%temperature_flux, in/out
+  double g, tref, kappa;   // grav_acc, temperature_ref, molecular diffusivity
+};
+
 // Wall-model points of the 18 momentum lists, concatenated (SoA).
 struct WMPoints {
   int n;
@@ -37,6 +50,7 @@ struct WMPoints {
   double *fluxp;           // out: flux scaled to the cell beyond the face
   double *fluxm;           // out: to the cell before it (= fluxp on uniform z)
   const double *logdz0;    // log(|dist|/z0) per point (static, from the host libm)
+  MOParams mo;             // Monin-Obukhov laws (with buoyancy)
 };
 
 // Scalar wall-model points (WMPoints of the reference).
@@ -54,6 +68,9 @@ struct WMUstarPoints {
   const double *z0;
   const double *logdz0;   // log(|dist| / z0) where z0 > 0 (host libm)
   double *ustar;
+  MOParams mo;
+  const double *af;       // WMPoint%area_factor
+  double *heat;           // out: tflux * af (the temperature's wall source)
 };
 
 // IB list on the device (CSR).
@@ -91,8 +108,8 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
                 double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, int cfl_blocks,
                 hipStream_t st);
 template <typename T>
-void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, WMPoints wm, double nu, int variable_z,
-                      hipStream_t st);
+void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMPoints wm, double nu,
+                      int variable_z, hipStream_t st);
 // Multigrid level (k_multigrid.hip): dense nx x ny x nz arrays, x fastest.
 struct MgLevel {
   int nx, ny, nz;
@@ -126,8 +143,8 @@ void launch_temp_solid_fix(const Geo &g, const int32_t *ijk, int n, const int32_
 template <typename R>
 void launch_moist_fill(const Geo &g, const uint8_t *code, R *q, double ref, hipStream_t st);
 template <typename T>
-void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, WMUstarPoints wp, double nu, T *Visc,
-                     hipStream_t st);
+void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMUstarPoints wp, double nu,
+                     T *Visc, hipStream_t st);
 template <typename T>
 void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *Ustar,
                      T *U2, const uint8_t *code, StageConst sc, hipStream_t st);

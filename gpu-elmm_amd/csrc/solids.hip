@@ -141,9 +141,7 @@ std::string solid_wm_pr(const sg::Geom &G, const std::vector<sg::Body> &bodies, 
         const int r = sg::wm_pr_point(G, bodies.data(), cd.prtype.data(), i, j, k, min_wall_distance, dv, &z0);
         if (r < 0) return sg::solid_error(-r);
         if (r == 0) continue;
-        cd.wmp.ijk.insert(cd.wmp.ijk.end(), {i, j, k});
-        cd.wmp.dist.insert(cd.wmp.dist.end(), {dv[0], dv[1], dv[2]});
-        cd.wmp.z0.push_back(z0);
+        cd.wmp.push(i, j, k, dv[0], dv[1], dv[2], z0);  // solid bodies: neutral
       }
   return std::string();
 }
@@ -162,10 +160,7 @@ std::string solid_wm_uvw(const sg::Geom &G, const std::vector<sg::Body> &bodies,
                                        min_wall_distance, dv, &z0);
           if (r < 0) return sg::solid_error(-r);
           if (r == 0) continue;
-          WMList &L = cd.wm[comp * 6 + dir];
-          L.ijk.insert(L.ijk.end(), {i, j, k});
-          L.dist.insert(L.dist.end(), {dv[0], dv[1], dv[2]});
-          L.z0.push_back(z0);
+          cd.wm[comp * 6 + dir].push(i, j, k, dv[0], dv[1], dv[2], z0);
         }
       }
   return std::string();

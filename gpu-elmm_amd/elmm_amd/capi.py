@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_PKG, "lib", "libelmm_hip.so")
 
 # sides and boundary codes (src/parameters.f90:233, 260-261)
 WE, EA, SO, NO, BO, TO = range(6)
-NOSLIP, FREESLIP, PERIODIC, DIRICHLET, NEUMANN = 1, 2, 3, 4, 5
+NOSLIP, FREESLIP, PERIODIC, DIRICHLET, NEUMANN, CONSTFLUX = 1, 2, 3, 4, 5, 6
+WM_NEUTRAL, WM_SURFACE_TEMP, WM_SURFACE_FLUX = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
 POISSON_FFT, POISSON_MG = 2, 3
 F_U, F_V, F_W, F_PR, F_VISC, F_TDIFF, F_PHI, F_Q, F_MOISTURE, F_TEMPERATURE, F_SCALAR = 0, 1, 2, 3, 4, 5, 6, 7, 14, 15, 16
@@ -72,6 +73,8 @@ class Lists(C.Structure):
         ("ib_src_ijk", _pi32 * 4), ("ib_coef", _pd * 4),
         ("null_n", C.c_int32 * 3), ("null_ijk", _pi32 * 3),
         ("src_n", C.c_int32), ("src_ijk", _pi32), ("src_scalar", _pi32), ("src_flux", _pd),
+        ("wm_tclass", C.POINTER(C.c_int8) * 18), ("wm_tval", _pd * 18),
+        ("wmp_tclass", C.POINTER(C.c_int8)), ("wmp_tval", _pd),
     ]
 
 
@@ -274,6 +277,17 @@ class Case:
         n = v.wm_n[l]
         return (_np(v.wm_ijk[l], 3 * n, np.int32).reshape(n, 3), _np(v.wm_dist[l], 3 * n, np.float64).reshape(n, 3),
                 _np(v.wm_z0[l], n, np.float64))
+
+    def wm_thermal(self, l):
+        """Thermal class and value of the points of momentum list l (-1: the
+        scalar wall-model points)."""
+        v = self.view
+        n = v.wmp_n if l < 0 else v.wm_n[l]
+        tc = v.wmp_tclass if l < 0 else v.wm_tclass[l]
+        tv = v.wmp_tval if l < 0 else v.wm_tval[l]
+        if not tc:
+            return np.zeros(n, np.int8), np.zeros(n)
+        return _np(tc, n, np.int8), _np(tv, n, np.float64)
 
     def wmp_list(self):
         v = self.view

@@ -38,7 +38,17 @@ enum {
   ELMM_BC_FREESLIP = 2,
   ELMM_BC_PERIODIC = 3,
   ELMM_BC_DIRICHLET = 4,
-  ELMM_BC_NEUMANN = 5
+  ELMM_BC_NEUMANN = 5,
+  ELMM_BC_CONSTFLUX = 6   /* temperature only: a surface heat flux (K m/s) on a
+                             wall with wall-model points (Monin-Obukhov law) */
+};
+
+/* Thermal class of a wall-model point (WMPoint%prescribed_temperature,
+ * %temperature_flux; src/wallmodels.f90:592-602, 821-831): */
+enum {
+  ELMM_WM_NEUTRAL = 0,    /* no surface heat flux: the neutral wall laws    */
+  ELMM_WM_SURFACE_TEMP = 1, /* prescribed surface temperature (tval, K)       */
+  ELMM_WM_SURFACE_FLUX = 2  /* prescribed surface heat flux (tval, K m/s)     */
 };
 
 /* Precision forms of the step. */
@@ -190,6 +200,13 @@ typedef struct elmm_lists {
   const int32_t *src_ijk;
   const int32_t *src_scalar;     /* 0-based scalar index                      */
   const double  *src_flux;       /* flux / cell volume                        */
+
+  /* thermal class (ELMM_WM_*) and value of each wall-model point; null:
+     every point neutral */
+  const int8_t  *wm_tclass[18];
+  const double  *wm_tval[18];
+  const int8_t  *wmp_tclass;
+  const double  *wmp_tval;
 } elmm_lists;
 
 /* Per-call diagnostics of elmm_step. */

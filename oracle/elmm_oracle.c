@@ -33,12 +33,15 @@ static const double RK_beta[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
 static const double RK_rho[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
 
 enum { We = 0, Ea = 1, So = 2, No = 3, Bo = 4, To = 5 };
-enum { NOSLIP = 1, FREESLIP = 2, PERIODIC = 3, DIRICHLET = 4, NEUMANN = 5 };
+enum { NOSLIP = 1, FREESLIP = 2, PERIODIC = 3, DIRICHLET = 4, NEUMANN = 5, CONSTFLUX = 6 };
 
 typedef struct {
   int n;
   int32_t *ijk;
   double *dist, *z0, *ustar, *fluxp, *fluxm;
+  int8_t *tclass;  /* ELMM_WM_* (prescribed_temperature, temperature_flux) */
+  double *tval;    /* surface temperature or heat flux */
+  double *tflux;   /* WMPoint%temperature_flux, persistent */
 } wm_list;
 
 struct oracle {
@@ -58,6 +61,8 @@ struct oracle {
   wm_list wm[18];
   wm_list wmp;
   double *wmp_div;
+  double *wmp_af;   /* WMPoint%area_factor */
+  int temp_bt[6];   /* TempBtype after GetOutsideBoundariesWM (0 on MO walls) */
   int scfl_n[3];
   int32_t *scfl_ijk[3];
   int ib_n[4];
@@ -155,8 +160,18 @@ static void init_grid(oracle *o) {
 /* ------------------------------------------------------------ creation --- */
 static double *field(oracle *o) { return xcalloc(o->ntot, sizeof(double)); }
 
-static void copy_wm(wm_list *d, int n, const int32_t *ijk, const double *dist, const double *z0) {
+static void copy_wm(wm_list *d, int n, const int32_t *ijk, const double *dist, const double *z0, const int8_t *tc,
+                    const double *tv) {
   d->n = n;
+  d->tclass = xcalloc((size_t)n + 1, sizeof(int8_t));
+  d->tval = xcalloc((size_t)n + 1, sizeof(double));
+  d->tflux = xcalloc((size_t)n + 1, sizeof(double));
+  for (int p = 0; p < n && tc; p++) {
+    d->tclass[p] = tc[p];
+    d->tval[p] = tv[p];
+    /* temperature_flux = sideTemp for a flux wall, else 0 (wallmodels.f90:592-596) */
+    d->tflux[p] = tc[p] == ELMM_WM_SURFACE_FLUX ? tv[p] : 0.0;
+  }
   d->ijk = xcalloc(3 * (size_t)n, sizeof(int32_t));
   d->dist = xcalloc(3 * (size_t)n, sizeof(double));
   d->z0 = xcalloc((size_t)n, sizeof(double));
@@ -169,6 +184,30 @@ static void copy_wm(wm_list *d, int n, const int32_t *ijk, const double *dist, c
     memcpy(d->z0, z0, (size_t)n * sizeof(double));
   }
   for (int p = 0; p < n; p++) d->ustar[p] = 1.0; /* WMpointUVW%ustar = 1 (wallmodels.f90:79) */
+}
+
+/* ComputeAreaFactor, src/wallmodels.f90:170-203: the wall area over the
+   cell volume from the dominant component of the point's wall normal
+   (norm2 as the square root of the sum of squares). */
+static double area_factor(const oracle *o, const double *d) {
+  if (-d[0] == 0 && -d[1] == 0 && -d[2] == 0) return 0;
+  const double V = o->dxmin * o->dymin * o->dzmin;
+  const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double n0 = fabs(-d[0] / nrm), n1 = fabs(-d[1] / nrm), n2 = fabs(-d[2] / nrm);
+  double on, dd;
+  if (n0 > n1 && n0 > n2) {
+    on = n0;
+    dd = sqrt(o->dymin * o->dzmin);
+  } else if (n1 > n0 && n1 > n2) {
+    on = n1;
+    dd = sqrt(o->dxmin * o->dzmin);
+  } else {
+    on = n2;
+    dd = sqrt(o->dxmin * o->dymin);
+  }
+  const double l = dd / on;
+  const double S = dd * l;
+  return S / V;
 }
 
 oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
@@ -262,9 +301,21 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
     o->scflmask[d] = xcalloc(o->ntot, 1);
     memcpy(o->scflmask[d], L->scfl_mask[d], o->ntot);
   }
-  for (int l = 0; l < 18; l++) copy_wm(&o->wm[l], L->wm_n[l], L->wm_ijk[l], L->wm_dist[l], L->wm_z0[l]);
-  copy_wm(&o->wmp, L->wmp_n, L->wmp_ijk, L->wmp_dist, L->wmp_z0);
+  for (int l = 0; l < 18; l++)
+    copy_wm(&o->wm[l], L->wm_n[l], L->wm_ijk[l], L->wm_dist[l], L->wm_z0[l], L->wm_tclass[l], L->wm_tval[l]);
+  copy_wm(&o->wmp, L->wmp_n, L->wmp_ijk, L->wmp_dist, L->wmp_z0, L->wmp_tclass, L->wmp_tval);
   o->wmp_div = xcalloc(L->wmp_n + 1, sizeof(double));
+  o->wmp_af = xcalloc(L->wmp_n + 1, sizeof(double));
+  for (int p = 0; p < L->wmp_n; p++) o->wmp_af[p] = area_factor(o, &o->wmp.dist[3 * p]);
+  /* GetOutsideBoundariesWM sets TempBtype = 0 (the mirror ghost rule) on a
+     no-slip wall with a Dirichlet or flux temperature (wallmodels.f90:618-620,
+     678-680) */
+  for (int q = 0; q < 6; q++) {
+    o->temp_bt[q] = o->c.temp_btype[q];
+    if (q >= Bo && o->c.enable_buoyancy && o->c.wall_model && o->c.btype[q] == NOSLIP &&
+        (o->c.temp_btype[q] == DIRICHLET || o->c.temp_btype[q] == CONSTFLUX))
+      o->temp_bt[q] = 0;
+  }
   for (int d = 0; d < 3; d++) {
     o->scfl_n[d] = L->scfl_n[d];
     o->scfl_ijk[d] = xcalloc(3 * (size_t)L->scfl_n[d] + 1, sizeof(int32_t));
@@ -303,6 +354,9 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
 }
 
 static void free_wm(wm_list *l) {
+  free(l->tclass);
+  free(l->tval);
+  free(l->tflux);
   free(l->ijk);
   free(l->dist);
   free(l->z0);
@@ -338,6 +392,7 @@ void oracle_destroy(oracle *o) {
   for (int l = 0; l < 18; l++) free_wm(&o->wm[l]);
   free_wm(&o->wmp);
   free(o->wmp_div);
+  free(o->wmp_af);
   for (int c = 0; c < 4; c++) {
     free(o->ib_ijk[c]);
     free(o->ib_ptr[c]);
@@ -612,6 +667,10 @@ static void bound_scalar_bc(oracle *o, double *a, const int *bt, const double *s
         S_(i, j, 0) = S_(i, j, 1) - side[Bo] * C1D(o->dzW, 0);
         S_(i, j, -1) = S_(i, j, 1) - side[Bo] * (C1D(o->zPr, 1) - C1D(o->zPr, -1));
         S_(i, j, -2) = S_(i, j, 1) - side[Bo] * (C1D(o->zPr, 1) - C1D(o->zPr, -2));
+      } else if (bt[Bo] == 0) { /* any other type: the mirror (scalarboundaries.f90:233-239) */
+        S_(i, j, -2) = S_(i, j, 3);
+        S_(i, j, -1) = S_(i, j, 2);
+        S_(i, j, 0) = S_(i, j, 1);
       }
       if (bt[To] == DIRICHLET) {
         S_(i, j, nz + 1) = side[To] - (S_(i, j, nz) - side[To]);
@@ -624,6 +683,10 @@ static void bound_scalar_bc(oracle *o, double *a, const int *bt, const double *s
         S_(i, j, nz + 1) = S_(i, j, nz) + side[To] * C1D(o->dzW, nz + 1);
         S_(i, j, nz + 2) = S_(i, j, nz) + side[To] * (C1D(o->zPr, nz + 2) - C1D(o->zPr, nz));
         S_(i, j, nz + 3) = S_(i, j, nz) + side[To] * (C1D(o->zPr, nz + 3) - C1D(o->zPr, nz));
+      } else if (bt[To] == 0) { /* the mirror (scalarboundaries.f90:271-277) */
+        S_(i, j, nz + 1) = S_(i, j, nz);
+        S_(i, j, nz + 2) = S_(i, j, nz - 1);
+        S_(i, j, nz + 3) = S_(i, j, nz - 2);
       }
     }
 #undef S_
@@ -632,7 +695,7 @@ static void bound_scalar_bc(oracle *o, double *a, const int *bt, const double *s
 /* BoundScalar (ScalBtype, sideScal) and BoundTemperature (TempBtype,
    sideTemp; src/scalarboundaries.f90:296-308): the same rules. */
 static void bound_scalar(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.scal_btype, o->c.side_scal); }
-static void bound_temperature(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.temp_btype, o->c.side_temp); }
+static void bound_temperature(oracle *o, double *a) { bound_scalar_bc(o, a, o->temp_bt, o->c.side_temp); }
 /* BoundMoisture, src/scalarboundaries.f90:311-323: the same rules with
    MoistBtype and sideMoist */
 static void bound_moisture(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.moist_btype, o->c.side_moist); }
@@ -953,6 +1016,218 @@ static double wm_rough_ustar(double ustar, double vel, double dist, double z0, d
   return ustar;
 }
 
+/* exp(x) with fixed operations, evaluated identically by the device (host
+   and device libm differ by an ulp): x = k ln2 + r with k = rint(x / ln2)
+   and r = (x - k ln2_hi) - k ln2_lo (ln2_hi of 33 significant bits, so
+   k ln2_hi is exact), |r| <= ln2/2; exp(r) by its Taylor polynomial to
+   r^13/13! (truncation below 4e-18 relative) in Horner form; times 2^k
+   (exact).  The reduction of W. J. Cody and W. Waite (1980).  Within a few
+   ulps of libm (pinned in tests). */
+double oracle_det_exp(double x) {
+  const double inv_ln2 = 0x1.71547652b82fep+0;
+  const double ln2_hi = 0x1.62e42fee00000p-1, ln2_lo = 0x1.a39ef35793c76p-33;
+  const double k = rint(x * inv_ln2);
+  const double r = (x - k * ln2_hi) - k * ln2_lo;
+  double p = 1.0 / 6227020800.0;
+  p = 1.0 / 479001600.0 + r * p;
+  p = 1.0 / 39916800.0 + r * p;
+  p = 1.0 / 3628800.0 + r * p;
+  p = 1.0 / 362880.0 + r * p;
+  p = 1.0 / 40320.0 + r * p;
+  p = 1.0 / 5040.0 + r * p;
+  p = 1.0 / 720.0 + r * p;
+  p = 1.0 / 120.0 + r * p;
+  p = 1.0 / 24.0 + r * p;
+  p = 1.0 / 6.0 + r * p;
+  p = 0.5 + r * p;
+  p = 1.0 + r * p;
+  p = 1.0 + r * p;
+  return ldexp(p, (int)k);
+}
+
+/* atan(x) with fixed operations, the same on the device: |x| > 1 by
+   atan(a) = pi/2 - atan(1/a); a > tan(pi/12) by atan(a) = pi/6 +
+   atan((a sqrt3 - 1) / (a + sqrt3)); then |t| <= tan(pi/12) = 0.268 by the
+   Taylor series t sum_{n=0..15} (-1)^n t^2n / (2n+1) (truncation below
+   1e-19). */
+double oracle_det_atan(double x) {
+  const double pi2 = 0x1.921fb54442d18p+0, pi6 = 0x1.0c152382d7365p-1;
+  const double sqrt3 = 0x1.bb67ae8584caap+0, tan12 = 0x1.126145e9ecd58p-2;
+  double a = fabs(x), off = 0;
+  int inv = 0;
+  if (a > 1) {
+    a = 1 / a;
+    inv = 1;
+  }
+  if (a > tan12) {
+    a = (a * sqrt3 - 1) / (a + sqrt3);
+    off = pi6;
+  }
+  const double z = a * a;
+  double p = -1.0 / 31.0;
+  p = 1.0 / 29.0 + z * p;
+  p = -1.0 / 27.0 + z * p;
+  p = 1.0 / 25.0 + z * p;
+  p = -1.0 / 23.0 + z * p;
+  p = 1.0 / 21.0 + z * p;
+  p = -1.0 / 19.0 + z * p;
+  p = 1.0 / 17.0 + z * p;
+  p = -1.0 / 15.0 + z * p;
+  p = 1.0 / 13.0 + z * p;
+  p = -1.0 / 11.0 + z * p;
+  p = 1.0 / 9.0 + z * p;
+  p = -1.0 / 7.0 + z * p;
+  p = 1.0 / 5.0 + z * p;
+  p = -1.0 / 3.0 + z * p;
+  p = 1.0 + z * p;
+  double r = off + a * p;
+  if (inv) r = pi2 - r;
+  return x < 0 ? -r : r;
+}
+
+/* x**y for x >= 0, y > 0: exp(y log x) by the two functions above (0 for
+   x = 0) */
+double oracle_det_pow(double x, double y) {
+  if (x == 0) return 0;
+  return oracle_det_exp(y * oracle_det_log(x));
+}
+
+/* ------------------------------------------------ Monin-Obukhov laws --- */
+static const double MO_PI = 0x1.921fb54442d18p+1; /* pi = acos(-1.0_knd), parameters.f90:43 */
+
+/* PsiM_MO, src/wallmodels.f90:1546-1556 */
+static double psim_mo(double zeta) {
+  if (zeta < 0) {
+    const double x = oracle_det_pow(1 - 15.0 * zeta, 0.25);
+    const double c = (1 + x) / 2.0;
+    return (oracle_det_log(((1 + x * x) / 2.0) * (c * c)) - 2.0 * oracle_det_atan(x)) + MO_PI / 2;
+  }
+  return -(4.8 * zeta);
+}
+
+/* PsiM_MO_mod, src/wallmodels.f90:1572-1582 */
+static double psim_mo_mod(double zeta, double kU) {
+  if (zeta < 0) {
+    const double x = oracle_det_pow(1 - (6.0 * zeta) * kU, 0.25);
+    const double c = (1 + x) / 2.0;
+    return (oracle_det_log(((1 + x * x) / 2.0) * (c * c)) - 2.0 * oracle_det_atan(x)) + MO_PI / 2;
+  }
+  return -(3 * oracle_det_pow(zeta * kU, 5.0 / 6.0));
+}
+
+/* PsiH_MO_mod, src/wallmodels.f90:1585-1595 */
+static double psih_mo_mod(double zeta, double kU) {
+  if (zeta < 0) {
+    const double x = oracle_det_pow(1 - (6.0 * zeta) * kU, 0.25);
+    return 2.0 * oracle_det_log((1 + x * x) / 2.0);
+  }
+  return -(2.5 * oracle_det_pow(zeta * kU, 4.0 / 5.0));
+}
+
+/* Obukhov_zL, src/wallmodels.f90:1598-1602 */
+static double obukhov_zl(double ustar, double tflux, double tref, double g, double z) {
+  return (z * ((0.4 * (g / tref)) * tflux)) / (-((ustar * ustar) * ustar));
+}
+
+/* WM_MO_Flux_ustar, src/wallmodels.f90:1606-1660 (its eps = 1e-3, 1E-5 and
+   1E-4 are default-real literals) */
+static double mo_flux_ustar(double vel, double dist, double ustar, double z0, double tflux, double tref, double g,
+                            double nu) {
+  const double eps = (double)1e-3f, yplcrit = 11.225;
+  if (dist <= z0) {
+    if (nu > 0) {
+      if ((dist * ustar / nu) < yplcrit)
+        ustar = sqrt(nu * vel / dist);
+      else
+        ustar = vel / (oracle_det_log(fabs(ustar * dist / nu)) / 0.4 + 5.2);
+    } else {
+      ustar = 0;
+    }
+    return ustar;
+  }
+  int i = 0;
+  double zL = 0, Psi = 0;
+  for (;;) {
+    i = i + 1;
+    const double ustar2 = ustar;
+    double m = dist / z0 - Psi;
+    if (!(m > (double)1e-5f)) m = (double)1e-5f;
+    double t = vel * 0.4 / oracle_det_log(m);
+    if (!(t > 0)) t = 0;
+    ustar = ustar + (t - ustar) / 2;
+    if (ustar < (double)1e-4f) {
+      zL = -10000;
+    } else {
+      const double zL2 = obukhov_zl(ustar, tflux, tref, g, dist);
+      zL = zL + (zL2 - zL) / 2;
+    }
+    Psi = psim_mo(zL);
+    double den = fabs(ustar);
+    if (!(den > 1e-3)) den = 1e-3;
+    if (fabs(ustar - ustar2) / den < eps) break;
+    if (i >= 50) {
+      ustar = 0;
+      break;
+    }
+  }
+  return ustar;
+}
+
+/* WM_MO_Dirichlet_ustar_tfl, src/wallmodels.f90:1664-1716 (k_T = 0.47 of
+   Zilitinkevich and Esau; 1.1 a default-real literal).  logz0 = log(dist/z0)
+   and logz0h = log(dist/z0H) (libm). */
+static void mo_dirichlet_ustar_tfl(double *ustar_io, double *tflux_io, double dist, double z0, double logz0,
+                                   double logz0h, double vel, double tempdif, double nu, double kappa, double g,
+                                   double tref) {
+  const double eps = 1e-3, yplcrit = 11.225, kU = 0.4, kT = 0.47;
+  double ustar = wm_rough_ustar(*ustar_io, vel, dist, z0, nu), tflux = *tflux_io;
+  double dist_plus;
+  if (nu > 0) {
+    const double ustar_lam = sqrt(nu * vel / dist);
+    dist_plus = dist * ustar_lam / nu;
+  } else {
+    dist_plus = DBL_MAX;
+  }
+  if (dist <= z0 || dist_plus < (double)1.1f * yplcrit) {
+    tflux = -((kappa * tempdif) / dist);
+  } else {
+    double zL = 0, zL0 = -10000;
+    int i = 0;
+    for (;;) {
+      i = i + 1;
+      zL = -(((dist * g) * tflux) / ((kU * tref) * ((ustar * ustar) * ustar)));
+      const double psi_m = psim_mo_mod(zL, kU), psi_h = psih_mo_mod(zL, kU);
+      ustar = (vel * kU) / (logz0 - psi_m);
+      tflux = -(((tempdif * ustar) * kT) / (logz0h - psi_h));
+      double den = fabs(zL);
+      if (!(den > 1e-4)) den = 1e-4;
+      if (i > 1 && fabs(zL - zL0) / den < eps) break;
+      if (i >= 50 || zL > 10000) break;
+      zL0 = zL;
+    }
+    if (i >= 50 || zL > 10000) {
+      ustar = sqrt(nu * vel / dist);
+      tflux = -((kappa * tempdif) / dist);
+    }
+  }
+  *ustar_io = ustar;
+  *tflux_io = tflux;
+}
+
+/* exported for the pins of the Monin-Obukhov laws */
+double oracle_psim_mo(double zeta) { return psim_mo(zeta); }
+double oracle_psim_mo_mod(double zeta) { return psim_mo_mod(zeta, 0.4); }
+double oracle_psih_mo_mod(double zeta) { return psih_mo_mod(zeta, 0.4); }
+double oracle_mo_flux_ustar(double vel, double dist, double ustar, double z0, double tflux, double tref, double g,
+                            double nu) {
+  return mo_flux_ustar(vel, dist, ustar, z0, tflux, tref, g, nu);
+}
+void oracle_mo_dirichlet(double *ustar, double *tflux, double dist, double z0, double vel, double tempdif, double nu,
+                         double kappa, double g, double tref) {
+  const double lz = log(dist / z0);
+  mo_dirichlet_ustar_tfl(ustar, tflux, dist, z0, lz, lz, vel, tempdif, nu, kappa, g, tref);
+}
+
 /* local_velocity, src/wallmodels.f90:2356-2399 */
 static void local_velocity(oracle *o, int comp, int i, int j, int k, double v[3]) {
   const double *U = o->U, *V = o->V, *W = o->W;
@@ -971,11 +1246,15 @@ static void local_velocity(oracle *o, int comp, int i, int j, int k, double v[3]
   }
 }
 
-/* ComputeUVWFluxesWM + fluxes, src/wallmodels.f90:2159-2337 (neutral,
-   no prescribed ustar, wall velocity 0). */
+/* ComputeUVWFluxesWM + fluxes, src/wallmodels.f90:2159-2337 (no prescribed
+   ustar, wall velocity 0): with buoyancy, a point with a prescribed surface
+   temperature takes the Monin-Obukhov Dirichlet law and one with a positive
+   surface heat flux the flux law, else the neutral laws. */
 static void wm_uvw_fluxes(oracle *o) {
   if (!o->c.wall_model) return;
   const double nu = o->c.nu;
+  const int buoy = o->c.enable_buoyancy;
+  const double *T = buoy ? o->S[o->c.nscalars] : NULL;
   const double eps = sqrt(2.220446049250313e-16);
   double drec[6] = {1 / o->dxmin, 1 / o->dxmin, 1 / o->dymin, 1 / o->dymin, 1 / o->dzmin, 1 / o->dzmin};
   for (int l = 0; l < 18; l++) {
@@ -990,7 +1269,23 @@ static void wm_uvw_fluxes(oracle *o) {
       double v = norm3(tan);
       double ustar = L->ustar[p];
       if (L->z0[p] > 0) {
-        if (v != 0) ustar = wm_rough_ustar(ustar, v, dist, L->z0[p], nu);
+        if (buoy && L->tclass[p] == ELMM_WM_SURFACE_TEMP) {
+          /* WM_MO_DirichletStress (wallmodels.f90:1922-1938) with the
+             temperature at the velocity point (local_value, 2340-2353) */
+          const long off = comp == 0 ? 1 : comp == 1 ? o->sy : o->sz;
+          const double temp = (T[IX(o, xi, yj, zk) + off] + T[IX(o, xi, yj, zk)]) / 2;
+          const double lz = log(dist / L->z0[p]);
+          mo_dirichlet_ustar_tfl(&ustar, &L->tflux[p], dist, L->z0[p], lz, lz, v, temp - L->tval[p], nu,
+                                 o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
+        } else if (buoy && L->tflux[p] > 0) {
+          /* WM_MO_FluxStress (1836-1854) */
+          if (v != 0)
+            ustar = mo_flux_ustar(v, dist, ustar, L->z0[p], L->tflux[p], o->c.temperature_ref, o->c.grav_acc, nu);
+          else
+            ustar = 0;
+        } else if (v != 0) {
+          ustar = wm_rough_ustar(ustar, v, dist, L->z0[p], nu);
+        }
       } else {
         if (v != 0) ustar = wm_flat_ustar(ustar, v, dist, nu);
       }
@@ -1094,17 +1389,22 @@ static void null_interior(oracle *o) {
     }
 }
 
-/* ComputeViscsWM, src/wallmodels.f90:2050-2152 (neutral: no buoyancy, no
-   prescribed u*, wall velocity 0, wallmodeltype 1): per scalar wall-model
-   point the cell-centred velocity from two-point averages (2069-2071), then
-   WMRoughVisc (1362-1379, via WMRoughStress 1336-1359) where z0 > 0, else
-   WMFlatVisc (1268-1304, via WMFlatStress 1243-1265).  WMPoints%ustar is the
+/* ComputeViscsWM, src/wallmodels.f90:2050-2152 (no prescribed u*, wall
+   velocity 0, wallmodeltype 1): per scalar wall-model point the
+   cell-centred velocity from two-point averages (2069-2071), then where
+   z0 > 0 with buoyancy the Monin-Obukhov Dirichlet law for a prescribed
+   surface temperature (which updates the point's temperature_flux) or the
+   flux law for a positive surface heat flux, else WMRoughVisc (1362-1379,
+   via WMRoughStress 1336-1359); where z0 = 0 WMFlatVisc (1268-1304, via
+   WMFlatStress 1243-1265).  WMPoints%ustar is the
    persistent state (the flat law iterates from it); the viscosity written
    at the point is returned in visc when write_visc is set (the caller in
    SubgridStresses lets SubgridModel overwrite it, as the reference does). */
 static void wm_visc_points(oracle *o, int write_visc) {
   if (!o->c.wall_model) return;
   const double nu = o->c.nu;
+  const int buoy = o->c.enable_buoyancy;
+  const double *T = buoy ? o->S[o->c.nscalars] : NULL;
   wm_list *L = &o->wmp;
   for (int p = 0; p < L->n; p++) {
     const int xi = L->ijk[3 * p], yj = L->ijk[3 * p + 1], zk = L->ijk[3 * p + 2];
@@ -1115,7 +1415,38 @@ static void wm_visc_points(oracle *o, int write_visc) {
     vel[1] = (A3(o, o->V, xi, yj, zk) + A3(o, o->V, xi, yj - 1, zk)) / 2.0;
     vel[2] = (A3(o, o->W, xi, yj, zk) + A3(o, o->W, xi, yj, zk - 1)) / 2.0;
     double ustar = L->ustar[p], visc;
-    if (L->z0[p] > 0) {
+    if (L->z0[p] > 0 && buoy && L->tclass[p] == ELMM_WM_SURFACE_TEMP) {
+      /* WM_MO_Dirichlet (wallmodels.f90:1858-1884) */
+      vel_and_dist(tan, &dist, vel, wallu, d);
+      const double v = norm3(tan);
+      const double lz = log(dist / L->z0[p]);
+      mo_dirichlet_ustar_tfl(&ustar, &L->tflux[p], dist, L->z0[p], lz, lz, v, A3(o, T, xi, yj, zk) - L->tval[p],
+                             nu, o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
+      if (ustar < 0) ustar = 0;
+      if (v > 0 || L->tflux[p] != 0) {
+        const double a = ustar * ustar * dist / v;
+        visc = a > nu ? a : nu;
+      } else {
+        visc = nu > 0 ? nu : 0;
+      }
+    } else if (L->z0[p] > 0 && buoy && L->tflux[p] > 0) {
+      /* WM_MO_FLUX (wallmodels.f90:1792-1832; gravity replaced by the
+         cosine of the wall normal with the vertical, as the reference does) */
+      vel_and_dist(tan, &dist, vel, wallu, d);
+      const double v = norm3(tan);
+      if (v != 0) {
+        const double g_cos = fabs(d[2]) / dist;
+        ustar = mo_flux_ustar(v, dist, ustar, L->z0[p], L->tflux[p], o->c.temperature_ref, g_cos, nu);
+        if (ustar < 0) ustar = 0;
+        if (ustar * ustar * dist / v > nu)
+          visc = ustar * ustar * dist / v;
+        else
+          visc = nu > 0 ? nu : 0;
+      } else {
+        ustar = 0;
+        visc = nu > 0 ? nu : 0;
+      }
+    } else if (L->z0[p] > 0) {
       vel_and_dist(tan, &dist, vel, wallu, d);
       const double v = norm3(tan);
       if (v != 0) ustar = wm_rough_ustar(ustar, v, dist, L->z0[p], nu);
@@ -1894,6 +2225,10 @@ static void scalar_stage(oracle *o, int sc, int stage, double dt, int kind) {
   }
   for (int p = 0; p < o->wmp.n; p++) {
     const int32_t *q = &o->wmp.ijk[3 * p];
+    /* TemperatureExtra: the point's surface heat flux times its area factor
+       first (scalars.f90:322-326) */
+    if (kind == 1)
+      A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp.tflux[p] * o->wmp_af[p];
     A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
   }
   double b = RK_beta[stage] * dt;

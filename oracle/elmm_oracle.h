@@ -61,6 +61,21 @@ double oracle_op_residual(oracle *o, const double *phi_ghosted, const double *rh
 /* the logarithm both the oracle and the GPU library use in the smooth-wall
    law (positive normal x) */
 double oracle_det_log(double x);
+/* exp, atan and x**y (x >= 0) with fixed operations, evaluated identically
+   by the GPU library (Cody-Waite exp, reduced Taylor atan) */
+double oracle_det_exp(double x);
+double oracle_det_atan(double x);
+double oracle_det_pow(double x, double y);
+/* Monin-Obukhov pieces (src/wallmodels.f90:1546-1716): PsiM_MO, the
+   modified PsiM/PsiH with k_U = 0.4, WM_MO_Flux_ustar, and
+   WM_MO_Dirichlet_ustar_tfl with z0H = z0 (ustar, tflux updated in place) */
+double oracle_psim_mo(double zeta);
+double oracle_psim_mo_mod(double zeta);
+double oracle_psih_mo_mod(double zeta);
+double oracle_mo_flux_ustar(double vel, double dist, double ustar, double z0, double tflux, double tref, double g,
+                            double nu);
+void oracle_mo_dirichlet(double *ustar, double *tflux, double dist, double z0, double vel, double tempdif, double nu,
+                         double kappa, double g, double tref);
 void oracle_op_wm_fluxes(oracle *o);            /* ComputeUVWFluxesWM           */
 void oracle_op_ib_momentum(oracle *o);          /* IBMomentum(U,V,W)            */
 void oracle_op_mass_sources(oracle *o);         /* IBMassSources + Bound_Q -> Q */

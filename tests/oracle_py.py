@@ -42,6 +42,15 @@ def _load():
         lib.oracle_op_residual.restype = C.c_double
         lib.oracle_det_log.argtypes = [C.c_double]
         lib.oracle_det_log.restype = C.c_double
+        for name in ("oracle_det_exp", "oracle_det_atan", "oracle_psim_mo", "oracle_psim_mo_mod",
+                     "oracle_psih_mo_mod"):
+            getattr(lib, name).argtypes = [C.c_double]
+            getattr(lib, name).restype = C.c_double
+        lib.oracle_det_pow.argtypes = [C.c_double, C.c_double]
+        lib.oracle_det_pow.restype = C.c_double
+        lib.oracle_mo_flux_ustar.argtypes = [C.c_double] * 8
+        lib.oracle_mo_flux_ustar.restype = C.c_double
+        lib.oracle_mo_dirichlet.argtypes = [pd, pd] + [C.c_double] * 8
         lib.oracle_op_wm_fluxes.argtypes = [C.c_void_p]
         lib.oracle_wm_list.argtypes = [C.c_void_p, C.c_int32, pd, pd]
         lib.oracle_wm_list.restype = C.c_int32
@@ -70,6 +79,19 @@ F_USTAR, F_VSTAR, F_WSTAR, F_RHS, F_SADV = 8, 9, 10, 11, 32
 def det_log(x):
     """The logarithm of the oracle's smooth-wall law."""
     return _load().oracle_det_log(float(x))
+
+
+def det_fn(name, *args):
+    """The oracle's fixed-operation exp, atan, pow and Monin-Obukhov
+    functions (oracle_<name>)."""
+    return getattr(_load(), "oracle_" + name)(*[float(a) for a in args])
+
+
+def mo_dirichlet(ustar, tflux, dist, z0, vel, tempdif, nu, kappa, g, tref):
+    """WM_MO_Dirichlet_ustar_tfl (z0H = z0): the updated (ustar, tflux)."""
+    u, t = C.c_double(ustar), C.c_double(tflux)
+    _load().oracle_mo_dirichlet(C.byref(u), C.byref(t), dist, z0, vel, tempdif, nu, kappa, g, tref)
+    return u.value, t.value
 
 
 class Oracle:

@@ -155,12 +155,12 @@ def test_hydrostatic_check_catches_a_sign_error():
     o.close()
 
 
-def test_dirichlet_temperature_on_a_wall_model_wall_is_rejected():
-    """A Dirichlet temperature on the no-slip bottom (wall-model points)
-    would switch the reference to the Monin-Obukhov law there."""
-    cfg = buoyant_config(temp_btype=[capi.PERIODIC] * 4 + [capi.DIRICHLET, capi.DIRICHLET],
-                         side_temp=[0, 0, 0, 0, T_BOT, T_TOP])
-    with pytest.raises(capi.ElmmError, match="Neumann on walls with wall-model points"):
+def test_temperature_flux_needs_a_wall_model_wall():
+    """A temperature flux (ELMM_BC_CONSTFLUX) exists only as the surface heat
+    flux of wall-model points; on the free-slip top it is rejected."""
+    cfg = buoyant_config(temp_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.CONSTFLUX],
+                         side_temp=[0, 0, 0, 0, 0.0, 0.1])
+    with pytest.raises(capi.ElmmError, match="needs a no-slip wall with the wall model"):
         capi.Case(cfg)
 
 
