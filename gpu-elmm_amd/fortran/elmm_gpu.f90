@@ -17,7 +17,7 @@ module elmm_gpu
   public :: elmm_upload, elmm_download, elmm_divergence
   public :: elmm_put, elmm_get, elmm_check, elmm_error_message
   public :: ELMM_WE, ELMM_EA, ELMM_SO, ELMM_NO, ELMM_BO, ELMM_TO
-  public :: ELMM_BC_NOSLIP, ELMM_BC_FREESLIP, ELMM_BC_PERIODIC, ELMM_BC_DIRICHLET, ELMM_BC_NEUMANN
+  public :: ELMM_BC_NOSLIP, ELMM_BC_FREESLIP, ELMM_BC_PERIODIC, ELMM_BC_DIRICHLET, ELMM_BC_NEUMANN, ELMM_BC_CONSTFLUX
   public :: ELMM_FP64, ELMM_FP32_REFINED, ELMM_POISSON_FFT
   public :: ELMM_F_U, ELMM_F_V, ELMM_F_W, ELMM_F_PR, ELMM_F_VISC, ELMM_F_TDIFF, ELMM_F_SCALAR
 
@@ -25,7 +25,8 @@ module elmm_gpu
   integer(c_int32_t), parameter :: ELMM_WE = 0, ELMM_EA = 1, ELMM_SO = 2, ELMM_NO = 3, ELMM_BO = 4, ELMM_TO = 5
   ! boundary codes, the reference's BC_* numbers (src/parameters.f90:249)
   integer(c_int32_t), parameter :: ELMM_BC_NOSLIP = 1, ELMM_BC_FREESLIP = 2, ELMM_BC_PERIODIC = 3, &
-                                   ELMM_BC_DIRICHLET = 4, ELMM_BC_NEUMANN = 5
+                                   ELMM_BC_DIRICHLET = 4, ELMM_BC_NEUMANN = 5, &
+                                   ELMM_BC_CONSTFLUX = 6
   integer(c_int32_t), parameter :: ELMM_FP64 = 0, ELMM_FP32_REFINED = 1
   integer(c_int32_t), parameter :: ELMM_POISSON_FFT = 2
   integer(c_int32_t), parameter :: ELMM_F_U = 0, ELMM_F_V = 1, ELMM_F_W = 2, ELMM_F_PR = 3, ELMM_F_VISC = 4, &

@@ -26,8 +26,8 @@ T_BOT, T_TOP = 293.0, 299.0
 
 
 def temperature_bcs(nz, dz=1.0):
-    """Periodic x/y; the bottom (a wall with wall-model points, where the
-    temperature has to be Neumann) with the gradient of the linear profile
+    """Periodic x/y; the bottom (a wall with wall-model points, neutral wall
+    laws with a Neumann temperature) with the gradient of the linear profile
     from T_BOT at z = 0 to T_TOP at z = nz dz; the free-slip top Dirichlet
     T_TOP."""
     return dict(temp_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.DIRICHLET],

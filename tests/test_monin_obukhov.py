@@ -200,6 +200,36 @@ def test_oracle_heat_flux_warms_the_wall_layer(where):
     o.close()
 
 
+@pytest.mark.parametrize("floor", ["warm", "neumann"])
+def test_oracle_warm_floor_heats_the_air(floor):
+    """Wind over a rough floor 2 K warmer than the air (uniform 295 K): the
+    Dirichlet law's upward heat flux warms the first layer within a step by
+    about flux dt / dz (the flux from the bulk relation, 0.47 u* 2 K /
+    (ln(z/z0) - Psi_H) ~ 0.03 K m/s); a zero-gradient floor leaves it."""
+    if floor == "warm":
+        bc = dict(temp_btype=[capi.PERIODIC] * 4 + [capi.DIRICHLET, capi.NEUMANN], side_temp=[0, 0, 0, 0, 297.0, 0])
+    else:
+        bc = dict(temp_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.NEUMANN], side_temp=[0] * 6)
+    cfg = buoyant_config(16, 16, 8, **bc)
+    o = Oracle(cfg, capi.Case(cfg))
+    shape = (cfg.nx + 6, cfg.ny + 6, cfg.nz + 6)
+    U = np.full(shape, 2.0, order="F")
+    zero = np.zeros(shape, order="F")
+    o.upload(capi.F_U, U)
+    for f in (capi.F_V, capi.F_W, capi.F_PR, capi.F_SCALAR):
+        o.upload(f, zero)
+    o.upload(capi.F_TEMPERATURE, np.full(shape, 295.0, order="F"))
+    o.init()
+    o.step(1)
+    T = o.download(capi.F_TEMPERATURE)[3:3 + cfg.nx, 3:3 + cfg.ny, 3:3 + cfg.nz]
+    rise = T[:, :, 0].mean() - 295.0
+    if floor == "warm":
+        assert 0.2 * 0.03 * cfg.dt < rise < 5 * 0.03 * cfg.dt, rise
+        assert abs(T[:, :, 1:].mean() - 295.0) < 0.1 * rise
+    else:
+        assert abs(rise) < 1e-9
+
+
 def test_moist_monin_obukhov_is_rejected():
     cfg = mo_config(enable_moisture=1, moisture_ref=0.01, moist_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN] * 2,
                     side_moist=[0] * 6)
