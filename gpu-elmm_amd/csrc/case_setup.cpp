@@ -91,6 +91,11 @@ void component_extents(const elmm_config &c, int n[3][3]) {
   n[2][2] = c.btype[ELMM_TO] == ELMM_BC_PERIODIC ? c.nz : c.nz - 1;
 }
 
+bool mo_moist_flux_wall(const elmm_config &c, int s) {
+  return c.enable_buoyancy && c.enable_moisture && c.wall_model && c.btype[s] == ELMM_BC_NOSLIP &&
+         c.moist_btype[s] == ELMM_BC_CONSTFLUX;
+}
+
 bool mo_wall(const elmm_config &c, int s) {
   return c.enable_buoyancy && c.wall_model && c.btype[s] == ELMM_BC_NOSLIP &&
          (c.temp_btype[s] == ELMM_BC_DIRICHLET || c.temp_btype[s] == ELMM_BC_CONSTFLUX);
@@ -135,17 +140,16 @@ std::string validate_config(const elmm_config &c) {
   // A wall with wall-model points (no-slip, wall model on) takes a Dirichlet
   // or flux temperature into its points (prescribed surface temperature or
   // heat flux: the Monin-Obukhov laws, src/wallmodels.f90:592-622,
-  // 2086-2118); a flux temperature exists only there.  The moist variant of
-  // those laws (surface moisture and virtual heat flux) is not built.
+  // 2086-2118); a flux temperature or moisture exists only there.
   if (c.enable_buoyancy)
     for (int s = ELMM_BO; s <= ELMM_TO; s++) {
       const bool wm_wall = c.wall_model && c.btype[s] == ELMM_BC_NOSLIP;
       if (c.temp_btype[s] == ELMM_BC_CONSTFLUX && !wm_wall)
         return "a temperature flux (ELMM_BC_CONSTFLUX) needs a no-slip wall with the wall model (side " +
                std::to_string(s) + ")";
-      if (mo_wall(c, s) && c.enable_moisture)
-        return "a Dirichlet or flux temperature on a wall with wall-model points needs the moist Monin-Obukhov "
-               "law with moisture (side " + std::to_string(s) + "), which is not built";
+      if (c.enable_moisture && c.moist_btype[s] == ELMM_BC_CONSTFLUX && !wm_wall)
+        return "a moisture flux (ELMM_BC_CONSTFLUX) needs a no-slip wall with the wall model (side " +
+               std::to_string(s) + ")";
     }
   if (c.variable_dt) {
     if (!(c.cfl_target > 0) || !(c.dt_max > 0)) return "variable time steps need cfl_target > 0 and dt_max > 0";
@@ -168,10 +172,19 @@ struct UVWPoint {
 // (Dirichlet) or heat flux (CONSTFLUX, positive upward: the top wall's is
 // -sideTemp(To)) on a Monin-Obukhov wall, else neutral
 // (src/wallmodels.f90:592-602, 652-662, 821-831, 872-882).
-void wall_thermal(const elmm_config &c, int s, int8_t &tc, double &tv) {
+void wall_thermal(const elmm_config &c, int s, int8_t &tc, double &tv, double &qv, double &qf) {
   tc = ELMM_WM_NEUTRAL;
   tv = 0;
-  if (s < ELMM_BO || !mo_wall(c, s)) return;
+  qv = 0;
+  qf = 0;
+  if (s < ELMM_BO) return;
+  // surface moisture (Dirichlet) or moisture flux (flux; the top's is
+  // -sideMoist(To)) of a no-slip wall-model wall (wallmodels.f90:604-612)
+  if (c.enable_buoyancy && c.enable_moisture && c.wall_model && c.btype[s] == ELMM_BC_NOSLIP) {
+    if (c.moist_btype[s] == ELMM_BC_DIRICHLET) qv = c.side_moist[s];
+    if (c.moist_btype[s] == ELMM_BC_CONSTFLUX) qf = s == ELMM_BO ? c.side_moist[s] : -c.side_moist[s];
+  }
+  if (!mo_wall(c, s)) return;
   if (c.temp_btype[s] == ELMM_BC_DIRICHLET) {
     tc = ELMM_WM_SURFACE_TEMP;
     tv = c.side_temp[s];
@@ -191,9 +204,9 @@ void outer_uvw(const elmm_config &c, const Grid1D &g, CaseData &cd, int comp) {
   auto Z = [&](int k) { return comp == 2 ? g.at(g.zW, k) : g.at(g.zPr, k); };
   auto add = [&](int dir, int i, int j, int k, double dx, double dy, double dz, double z0) {
     int8_t tc;
-    double tv;
-    wall_thermal(c, dir, tc, tv);
-    cd.wm[comp * 6 + dir].push(i, j, k, dx, dy, dz, z0, tc, tv);
+    double tv, qv, qf;
+    wall_thermal(c, dir, tc, tv, qv, qf);
+    cd.wm[comp * 6 + dir].push(i, j, k, dx, dy, dz, z0, tc, tv, qv, qf);
   };
   auto t = [&](int i, int j, int k) { return T[cd.lin(i, j, k)]; };
   if (c.btype[ELMM_WE] == ELMM_BC_NOSLIP)
@@ -230,9 +243,9 @@ void outer_pr(const elmm_config &c, const Grid1D &g, CaseData &cd) {
   const int nx = c.nx, ny = c.ny, nz = c.nz;
   auto add = [&](int side, int i, int j, int k, double dx, double dy, double dz, double z0) {
     int8_t tc;
-    double tv;
-    wall_thermal(c, side, tc, tv);
-    cd.wmp.push(i, j, k, dx, dy, dz, z0, tc, tv);
+    double tv, qv, qf;
+    wall_thermal(c, side, tc, tv, qv, qf);
+    cd.wmp.push(i, j, k, dx, dy, dz, z0, tc, tv, qv, qf);
   };
   auto t = [&](int i, int j, int k) { return cd.prtype[cd.lin(i, j, k)]; };
   if (c.btype[ELMM_WE] == ELMM_BC_NOSLIP)
@@ -297,7 +310,7 @@ void remove_duplicate_wm_points(const elmm_config &c, WMList &L) {
         continue;
     }
     out.push(L.ijk[3 * p], L.ijk[3 * p + 1], L.ijk[3 * p + 2], L.dist[3 * p], L.dist[3 * p + 1], L.dist[3 * p + 2],
-             L.z0[p], L.tclass[p], L.tval[p]);
+             L.z0[p], L.tclass[p], L.tval[p], L.qval[p], L.qflux[p]);
   }
   L = std::move(out);
 }
@@ -417,6 +430,12 @@ void view_case(const CaseData &cd, elmm_lists &v) {
   }
   v.wmp_tclass = cd.wmp.tclass.data();
   v.wmp_tval = cd.wmp.tval.data();
+  for (int l = 0; l < 18; l++) {
+    v.wm_qval[l] = cd.wm[l].qval.data();
+    v.wm_qflux[l] = cd.wm[l].qflux.data();
+  }
+  v.wmp_qval = cd.wmp.qval.data();
+  v.wmp_qflux = cd.wmp.qflux.data();
   v.wmp_n = (int32_t)cd.wmp.z0.size();
   v.wmp_ijk = cd.wmp.ijk.data();
   v.wmp_dist = cd.wmp.dist.data();

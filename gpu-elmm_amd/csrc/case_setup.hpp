@@ -27,13 +27,18 @@ struct WMList {
   std::vector<double> z0;
   std::vector<int8_t> tclass;  // ELMM_WM_* per point
   std::vector<double> tval;    // surface temperature or heat flux per point
-  // a point with the neutral thermal class
-  void push(int i, int j, int k, double dx, double dy, double dz, double z, int8_t tc = 0, double tv = 0) {
+  std::vector<double> qval;    // surface moisture per point
+  std::vector<double> qflux;   // initial moisture flux per point
+  // a point (by default neutral and dry)
+  void push(int i, int j, int k, double dx, double dy, double dz, double z, int8_t tc = 0, double tv = 0,
+            double qv = 0, double qf = 0) {
     ijk.insert(ijk.end(), {i, j, k});
     dist.insert(dist.end(), {dx, dy, dz});
     z0.push_back(z);
     tclass.push_back(tc);
     tval.push_back(tv);
+    qval.push_back(qv);
+    qflux.push_back(qf);
   }
 };
 
@@ -43,6 +48,10 @@ struct WMList {
 // mirror (TempBtype = 0): buoyancy, the wall model, a no-slip wall and a
 // Dirichlet or flux temperature (src/wallmodels.f90:574-625, 636-680).
 bool mo_wall(const elmm_config &c, int s);
+// The same wall takes a flux moisture into its points (MoistBtype = 0 there,
+// src/wallmodels.f90:627-629): moisture, the wall model, a no-slip wall and
+// a flux moisture.
+bool mo_moist_flux_wall(const elmm_config &c, int s);
 
 struct IBList {
   std::vector<int32_t> ijk;      // target, 3 per point

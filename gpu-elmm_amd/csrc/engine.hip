@@ -385,17 +385,20 @@ struct Engine : EngineIface {
   // wall model
   DevBuf wm_ijk, wm_list, wm_dist, wm_z0, wm_ustar, wm_fluxp, wm_fluxm, wm_logdz0;
   DevBuf wm_tclass, wm_tval, wm_tflux, wmp_tclass, wmp_tval, wmp_tflux, wmp_af;
+  DevBuf wm_qval, wm_qflux, wmp_qval, wmp_qflux;
   bool mo_any = false;  // some wall-model point has a thermal class (buoyancy)
   DevBuf width_tab, width_idx, sgs_rdz, sgs_rz2, sgs_rdzf, sgs_rz2f;
   SgsWidth sgs_width{nullptr, nullptr, 0};
   int wm_n = 0;
   DevBuf wmp_ijk, wmp_div, wmp_dist, wmp_z0, wmp_logdz0, wmp_ustar;
   int wmp_n = 0;
-  MOParams mo_params(const DevBuf &tc, const DevBuf &tv, const DevBuf &tf) const {
+  MOParams mo_params(const DevBuf &tc, const DevBuf &tv, const DevBuf &tf, const DevBuf &qv,
+                     const DevBuf &qf) const {
     return MOParams{cfg.enable_buoyancy ? 1 : 0, tc.as<int8_t>(), tv.as<double>(), tf.as<double>(), cfg.grav_acc,
-                    cfg.temperature_ref, cfg.nu_scalar};
+                    cfg.temperature_ref, cfg.nu_scalar, mslot >= 0 ? 1 : 0, qv.as<double>(), qf.as<double>()};
   }
-  // the temperature's wall source tflux * af goes to src_flux[src_n + p]
+  // the temperature's wall source tflux * af goes to src_flux[src_n + p], the
+  // moisture's qflux * af to src_flux[src_n + wmp_n + p]
   WMUstarPoints wmpdev() const {
     return WMUstarPoints{wmp_n,
                          wmp_ijk.as<int32_t>(),
@@ -403,12 +406,14 @@ struct Engine : EngineIface {
                          wmp_z0.as<double>(),
                          wmp_logdz0.as<double>(),
                          wmp_ustar.as<double>(),
-                         mo_params(wmp_tclass, wmp_tval, wmp_tflux),
+                         mo_params(wmp_tclass, wmp_tval, wmp_tflux, wmp_qval, wmp_qflux),
                          wmp_af.as<double>(),
-                         tslot >= 0 ? src_flux.as<double>() + src_n : nullptr};
+                         tslot >= 0 ? src_flux.as<double>() + src_n : nullptr,
+                         mslot >= 0 ? src_flux.as<double>() + src_n + wmp_n : nullptr};
   }
   int src_n = 0;
   const T *temp_cur() const { return tslot >= 0 ? s_cur[tslot] : nullptr; }
+  const T *moist_cur() const { return mslot >= 0 ? s_cur[mslot] : nullptr; }
   DevTerms mom_terms[3];
   std::vector<std::unique_ptr<DevTerms>> sc_terms;
   DevBuf src_flux;
@@ -477,7 +482,7 @@ struct Engine : EngineIface {
                     wm_fluxp.as<double>(),
                     wm_fluxm.as<double>(),
                     wm_logdz0.as<double>(),
-                    mo_params(wm_tclass, wm_tval, wm_tflux)};
+                    mo_params(wm_tclass, wm_tval, wm_tflux, wm_qval, wm_qflux)};
   }
   bool has_ib() const { return ib_n[0] + ib_n[1] + ib_n[2] > 0; }
 
@@ -995,7 +1000,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
   };
   // wall-model points of the 18 momentum lists, concatenated in list order
   std::vector<int32_t> wijk, wlist;
-  std::vector<double> wdist, wz0, wustar, wtval, wtflux;
+  std::vector<double> wdist, wz0, wustar, wtval, wtflux, wqval, wqflux;
   std::vector<int8_t> wtclass;
   std::vector<int> list_off(19, 0);
   for (int l = 0; l < 18; l++) {
@@ -1012,12 +1017,16 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       wtclass.push_back(tc);
       wtval.push_back(tv);
       wtflux.push_back(tc == ELMM_WM_SURFACE_FLUX ? tv : 0.0);
+      wqval.push_back(L.wm_qval[l] ? L.wm_qval[l][p] : 0.0);
+      wqflux.push_back(L.wm_qval[l] ? L.wm_qflux[l][p] : 0.0);
       if (tc != ELMM_WM_NEUTRAL && cfg.enable_buoyancy) mo_any = true;
     }
   }
   upload_vec(wm_tclass, wtclass);
   upload_vec(wm_tval, wtval);
   upload_vec(wm_tflux, wtflux);
+  upload_vec(wm_qval, wqval);
+  upload_vec(wm_qflux, wqflux);
   list_off[18] = (int)wlist.size();
   wm_n = (int)wlist.size();
   std::vector<double> wlog(wm_n, 0.0);
@@ -1117,7 +1126,7 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       // squares): the wall area over the cell volume from the dominant
       // component of the wall normal
       std::vector<int8_t> tc(wmp_n, 0);
-      std::vector<double> tv(wmp_n, 0.0), tf(wmp_n, 0.0), af(wmp_n, 0.0);
+      std::vector<double> tv(wmp_n, 0.0), tf(wmp_n, 0.0), af(wmp_n, 0.0), qv(wmp_n, 0.0), qf(wmp_n, 0.0);
       const double dxm = g.dxmin, dym = g.dymin, dzm = g.dzmin;
       for (int p = 0; p < wmp_n; p++) {
         if (L.wmp_tclass) {
@@ -1125,6 +1134,10 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
           tv[p] = L.wmp_tval[p];
           tf[p] = tc[p] == ELMM_WM_SURFACE_FLUX ? tv[p] : 0.0;
           if (tc[p] != ELMM_WM_NEUTRAL && cfg.enable_buoyancy) mo_any = true;
+        }
+        if (L.wmp_qval) {
+          qv[p] = L.wmp_qval[p];
+          qf[p] = L.wmp_qflux[p];
         }
         const double *d = &pd[3 * p];
         if (-d[0] == 0 && -d[1] == 0 && -d[2] == 0) continue;
@@ -1150,12 +1163,14 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
       upload_vec(wmp_tval, tv);
       upload_vec(wmp_tflux, tf);
       upload_vec(wmp_af, af);
+      upload_vec(wmp_qval, qv);
+      upload_vec(wmp_qflux, qf);
     }
     // point sources, then (with buoyancy) one slot per wall-model point for
     // the temperature's wall source written by k_wm_ustar every stage
     src_n = L.src_n;
     std::vector<double> sflux(L.src_flux, L.src_flux + L.src_n);
-    if (tslot >= 0) sflux.resize(src_n + wmp_n, 0.0);
+    if (tslot >= 0) sflux.resize(src_n + (mslot >= 0 ? 2 : 1) * (size_t)wmp_n, 0.0);
     upload_vec(src_flux, sflux);
     if (tslot >= 0) {
       // solid cells of -1..n+2 with a fluid neighbour there: their temperature
@@ -1205,6 +1220,8 @@ void Engine<T>::setup_lists(const elmm_lists &L) {
           // TemperatureExtra: the surface heat flux times the area factor,
           // then the divergence term (src/scalars.f90:322-328)
           if (s == tslot) tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {2, L.src_n + p, 0, 0});
+          // MoistureExtra: the moisture flux likewise (src/scalars.f90:350-356)
+          if (s == mslot) tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {2, L.src_n + wmp_n + p, 0, 0});
           tb.add(lin(q[0], q[1], q[2]), q[0], q[1], q[2], {3, p, 0, 0});
         }
       }
@@ -1409,11 +1426,14 @@ void Engine<T>::create(const elmm_config &c, const elmm_lists &L) {
     for (int s = 0; s < 4; s++)
       if (cfg.moist_btype[s] != ELMM_BC_PERIODIC) throw ArgError("the moisture must be periodic in x and y");
     for (int s = 4; s < 6; s++)
-      if (cfg.moist_btype[s] != ELMM_BC_DIRICHLET && cfg.moist_btype[s] != ELMM_BC_NEUMANN)
-        throw ArgError("moisture z boundaries must be Dirichlet or Neumann");
+      if (cfg.moist_btype[s] != ELMM_BC_DIRICHLET && cfg.moist_btype[s] != ELMM_BC_NEUMANN &&
+          !mo_moist_flux_wall(cfg, s))
+        throw ArgError("moisture z boundaries must be Dirichlet, Neumann, or a flux on a wall-model wall");
     gmoist = g;
     for (int s = 0; s < 6; s++) {
-      gmoist.sbt[s] = cfg.moist_btype[s];
+      // a flux moisture on a wall-model wall: MoistBtype = 0, the mirror
+      // (src/wallmodels.f90:627-629)
+      gmoist.sbt[s] = mo_moist_flux_wall(cfg, s) ? 0 : cfg.moist_btype[s];
       gmoist.side_scal[s] = cfg.side_moist[s];
     }
   }
@@ -1646,7 +1666,7 @@ void Engine<T>::subgrid_stresses(bool cfl) {
   // ComputeViscsWM: WMPoints%ustar (the viscosity it writes is overwritten
   // by SubgridModel)
   if (cfg.wall_model && wmp_n)
-    timed(ELMM_T_WALL, [&] { launch_wm_ustar<T>(g, u, v, w, temp_cur(), wmpdev(), cfg.nu, (T *)nullptr, stream); });
+    timed(ELMM_T_WALL, [&] { launch_wm_ustar<T>(g, u, v, w, temp_cur(), moist_cur(), wmpdev(), cfg.nu, (T *)nullptr, stream); });
   // constant dt with the CFL check: the step's CFL number and the divergence
   // test, on the device (graph-safe)
   if (cfl && !cfg.variable_dt)
@@ -1654,7 +1674,7 @@ void Engine<T>::subgrid_stresses(bool cfl) {
       launch_cfl_monitor(cfl_partial.as<double>(), cfl_blocks, dt_cur, cfg.cfl_max, mon, stream);
     });
   if (cfg.wall_model && wm_n)
-    timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, temp_cur(), wmdev(), cfg.nu, g.variable_z, stream); });
+    timed(ELMM_T_WALL, [&] { launch_wm_fluxes(g, u, v, w, temp_cur(), moist_cur(), wmdev(), cfg.nu, g.variable_z, stream); });
   timed(ELMM_T_GHOST, [&] { launch_bound_visc<T>(g, Visc.as<T>(), stream); });
   if (ib_n[3])
     timed(ELMM_T_IB, [&] {
@@ -1812,8 +1832,8 @@ void Engine<T>::init() {
     launch_bound_visc<T>(g, TDiff.as<T>(), stream);
   }
   for (int s = 0; s < nsc; s++) launch_bound_scalar<T>(gscal(s), s_cur[s], stream);
-  if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, temp_cur(), wmpdev(), cfg.nu, Visc.as<T>(), stream);
-  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, temp_cur(), wmdev(), cfg.nu, g.variable_z, stream);
+  if (cfg.wall_model && wmp_n) launch_wm_ustar<T>(g, u, v, w, temp_cur(), moist_cur(), wmpdev(), cfg.nu, Visc.as<T>(), stream);
+  if (cfg.wall_model) launch_wm_fluxes(g, u, v, w, temp_cur(), moist_cur(), wmdev(), cfg.nu, g.variable_z, stream);
   launch_bound_visc<T>(g, Visc.as<T>(), stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));

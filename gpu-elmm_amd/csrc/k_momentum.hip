@@ -567,9 +567,46 @@ __device__ inline void mo_dirichlet(double &ustar, double &tflux, double dist, d
   }
 }
 
+// WM_MO_Dirichlet_ustar_tfl_mfl (src/wallmodels.f90:1721-1784): the virtual
+// heat flux in the Obukhov length, the moisture flux with k_Q = k_T; laminar
+// branch dist <= z0 only; the fallback keeps the moisture flux
+__device__ inline void mo_dirichlet_moist(double &ustar, double &tflux, double &qflux, double dist, double z0,
+                                          double logz0, double vel, double temp, double surf_temp, double moist,
+                                          double surf_moist, double nu, double kappa, double g, double tref) {
+  const double eps = 1e-3, kU = 0.4, kT = 0.47, kQ = 0.47, e_coef = 0.61;
+  const double tempdif = temp - surf_temp, moistdif = moist - surf_moist;
+  ustar = wm_rough_ustar(ustar, vel, dist, z0, nu, logz0);
+  if (dist <= z0) {
+    tflux = -((kappa * tempdif) / dist);
+    qflux = -((kappa * moistdif) / dist);
+    return;
+  }
+  double zL = 0, zL0 = -10000;
+  int i = 0;
+  for (;;) {
+    i = i + 1;
+    const double virt = tflux * (1 + e_coef * moist) + (e_coef * temp) * qflux;
+    zL = -(((dist * g) * virt) / ((kU * tref) * ((ustar * ustar) * ustar)));
+    const double psi_m = psim_mo_mod(zL, kU), psi_h = psih_mo_mod(zL, kU), psi_e = psi_h;
+    ustar = (vel * kU) / (logz0 - psi_m);
+    tflux = -(((tempdif * ustar) * kT) / (logz0 - psi_h));
+    qflux = -(((moistdif * ustar) * kQ) / (logz0 - psi_e));
+    double den = fabs(zL);
+    if (!(den > 1e-4)) den = 1e-4;
+    if (i > 1 && fabs(zL - zL0) / den < eps) break;
+    if (i >= 50 || zL > 10000) break;
+    zL0 = zL;
+  }
+  if (i >= 50 || zL > 10000) {
+    ustar = sqrt(nu * vel / dist);
+    tflux = -((kappa * tempdif) / dist);
+  }
+}
+
 template <typename T>
 __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict__ V_, const T *__restrict__ W_,
-                            const T *__restrict__ Temp, WMPoints wm, double nu, int variable_z) {
+                            const T *__restrict__ Temp, const T *__restrict__ Moist, WMPoints wm, double nu,
+                            int variable_z) {
   // surface lists: evaluated in FP64 whatever the field type
   auto U = [&](long q) { return (double)U_[q]; };
   auto V = [&](long q) { return (double)V_[q]; };
@@ -612,8 +649,16 @@ __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict
     const long off = comp == 0 ? 1 : comp == 1 ? sy : sz;
     const double temp = ((double)Temp[q + off] + (double)Temp[q]) / 2;
     double tf = wm.mo.tflux[p];
-    mo_dirichlet(ustar, tf, dist, z0, wm.logdz0[p], vel, temp - wm.mo.tval[p], nu, wm.mo.kappa, wm.mo.g,
-                 wm.mo.tref);
+    if (wm.mo.moist) {  // WM_MO_DirichletStress_moist
+      double qf = wm.mo.qflux[p];
+      const double mq = ((double)Moist[q + off] + (double)Moist[q]) / 2;
+      mo_dirichlet_moist(ustar, tf, qf, dist, z0, wm.logdz0[p], vel, temp, wm.mo.tval[p], mq, wm.mo.qval[p], nu,
+                         wm.mo.kappa, wm.mo.g, wm.mo.tref);
+      wm.mo.qflux[p] = qf;
+    } else {
+      mo_dirichlet(ustar, tf, dist, z0, wm.logdz0[p], vel, temp - wm.mo.tval[p], nu, wm.mo.kappa, wm.mo.g,
+                   wm.mo.tref);
+    }
     wm.mo.tflux[p] = tf;
   } else if (z0 > 0 && wm.mo.buoy && wm.mo.tflux[p] > 0) {
     // WM_MO_FluxStress
@@ -648,10 +693,11 @@ __global__ void k_wm_fluxes(Geo g, const T *__restrict__ U_, const T *__restrict
 }
 
 template <typename T>
-void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMPoints wm, double nu,
-                      int variable_z, hipStream_t st) {
+void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, const T *Moist, WMPoints wm,
+                      double nu, int variable_z, hipStream_t st) {
   if (wm.n == 0) return;
-  hipLaunchKernelGGL(k_wm_fluxes<T>, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, Temp, wm, nu, variable_z);
+  hipLaunchKernelGGL(k_wm_fluxes<T>, dim3(nblk(wm.n, 256)), dim3(256), 0, st, g, U, V, W, Temp, Moist, wm, nu,
+                     variable_z);
 }
 
 // ComputeViscsWM (src/wallmodels.f90:2050-2152; neutral, wall velocity 0,
@@ -663,7 +709,8 @@ void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T 
 // keeps only u*.  FP64 whatever the field type.
 template <typename T>
 __global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict__ V_, const T *__restrict__ W_,
-                           const T *__restrict__ Temp, WMUstarPoints wp, double nu, T *__restrict__ Visc) {
+                           const T *__restrict__ Temp, const T *__restrict__ Moist, WMUstarPoints wp, double nu,
+                           T *__restrict__ Visc) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wp.n) return;
   const int xi = wp.ijk[3 * p], yj = wp.ijk[3 * p + 1], zk = wp.ijk[3 * p + 2];
@@ -689,8 +736,15 @@ __global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict_
     if (z0 > 0 && wp.mo.buoy && wp.mo.tclass[p] == ELMM_WM_SURFACE_TEMP) {
       // WM_MO_Dirichlet (src/wallmodels.f90:1858-1884)
       double tf = wp.mo.tflux[p];
-      mo_dirichlet(ustar, tf, dist, z0, wp.logdz0[p], vel, (double)Temp[q] - wp.mo.tval[p], nu, wp.mo.kappa,
-                   wp.mo.g, wp.mo.tref);
+      if (wp.mo.moist) {  // WM_MO_Dirichlet_moist (src/wallmodels.f90:1887-1919)
+        double qf = wp.mo.qflux[p];
+        mo_dirichlet_moist(ustar, tf, qf, dist, z0, wp.logdz0[p], vel, (double)Temp[q], wp.mo.tval[p],
+                           (double)Moist[q], wp.mo.qval[p], nu, wp.mo.kappa, wp.mo.g, wp.mo.tref);
+        wp.mo.qflux[p] = qf;
+      } else {
+        mo_dirichlet(ustar, tf, dist, z0, wp.logdz0[p], vel, (double)Temp[q] - wp.mo.tval[p], nu, wp.mo.kappa,
+                     wp.mo.g, wp.mo.tref);
+      }
       wp.mo.tflux[p] = tf;
       if (ustar < 0) ustar = 0;
       if (vel > 0 || tf != 0) {
@@ -731,13 +785,15 @@ __global__ void k_wm_ustar(Geo g, const T *__restrict__ U_, const T *__restrict_
   if (Visc) Visc[q] = (T)visc;
   // TemperatureExtra's surface source (src/scalars.f90:322-326)
   if (wp.heat) wp.heat[p] = wp.mo.tflux[p] * wp.af[p];
+  // MoistureExtra's (src/scalars.f90:350-354)
+  if (wp.mheat) wp.mheat[p] = wp.mo.qflux[p] * wp.af[p];
 }
 
 template <typename T>
-void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMUstarPoints wp, double nu,
-                     T *Visc, hipStream_t st) {
+void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, const T *Moist,
+                     WMUstarPoints wp, double nu, T *Visc, hipStream_t st) {
   if (wp.n == 0) return;
-  hipLaunchKernelGGL(k_wm_ustar<T>, dim3(nblk(wp.n, 256)), dim3(256), 0, st, g, U, V, W, Temp, wp, nu, Visc);
+  hipLaunchKernelGGL(k_wm_ustar<T>, dim3(nblk(wp.n, 256)), dim3(256), 0, st, g, U, V, W, Temp, Moist, wp, nu, Visc);
 }
 
 // ------------------------------------------------------ CD4 advection
@@ -1319,10 +1375,10 @@ void launch_mass_sources(const Geo &g, TermList tl, const T *U, const T *V, cons
 #define ELMM_INST_MOMENTUM(T)                                                                                       \
   template void launch_sgs<T>(const Geo &, const T *, const T *, const T *, T *, double, double, double,           \
                               SgsWidth, int, double *, int, hipStream_t);                                          \
-  template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, const T *, WMPoints, double, int, \
-                                    hipStream_t);                                                                  \
-  template void launch_wm_ustar<T>(const Geo &, const T *, const T *, const T *, const T *, WMUstarPoints, double, \
-                                   T *, hipStream_t);                                                              \
+  template void launch_wm_fluxes<T>(const Geo &, const T *, const T *, const T *, const T *, const T *, WMPoints,  \
+                                    double, int, hipStream_t);                                                     \
+  template void launch_wm_ustar<T>(const Geo &, const T *, const T *, const T *, const T *, const T *,             \
+                                   WMUstarPoints, double, T *, hipStream_t);                                       \
   template void launch_momentum<T>(const Geo &, int, const T *, const T *, const T *, const T *, const T *, T *,   \
                                    T *, const uint8_t *, StageConst, hipStream_t);                                 \
   template void launch_momentum_corr3<T>(const Geo &, const T *, const T *, const T *, const T *, const T *,      \

@@ -532,8 +532,10 @@ int build_on_device(const elmm_config &c, const Grid1D &g, int ncub, const elmm_
         if (J.wm) {
           from_out(J.wm->dist, J.a.dist, 3 * (size_t)J.n);
           from_out(J.wm->z0, J.a.z0, J.n);
-          J.wm->tclass.assign(J.n, 0);  // solid bodies: neutral
+          J.wm->tclass.assign(J.n, 0);  // solid bodies: neutral and dry
           J.wm->tval.assign(J.n, 0.0);
+          J.wm->qval.assign(J.n, 0.0);
+          J.wm->qflux.assign(J.n, 0.0);
         }
       }
     }

@@ -37,6 +37,9 @@ struct MOParams {
   const double *tval;      // surface temperature per point
   double *tflux;           // WMPoint%temperature_flux, in/out
   double g, tref, kappa;   // grav_acc, temperature_ref, molecular diffusivity
+  int moist;               // with moisture: the moist Dirichlet law
+  const double *qval;      // surface moisture per point
+  double *qflux;           // WMPoint%moisture_flux, in/out
 };
 
 // Wall-model points of the 18 momentum lists, concatenated (SoA).
@@ -71,6 +74,7 @@ struct WMUstarPoints {
   MOParams mo;
   const double *af;       // WMPoint%area_factor
   double *heat;           // out: tflux * af (the temperature's wall source)
+  double *mheat;          // out: qflux * af (the moisture's wall source)
 };
 
 // IB list on the device (CSR).
@@ -108,8 +112,8 @@ void launch_sgs(const Geo &g, const T *U, const T *V, const T *W, T *Visc, doubl
                 double filter_ratio, SgsWidth width, int sgs_model, double *cfl_partial, int cfl_blocks,
                 hipStream_t st);
 template <typename T>
-void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMPoints wm, double nu,
-                      int variable_z, hipStream_t st);
+void launch_wm_fluxes(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, const T *Moist, WMPoints wm,
+                      double nu, int variable_z, hipStream_t st);
 // Multigrid level (k_multigrid.hip): dense nx x ny x nz arrays, x fastest.
 struct MgLevel {
   int nx, ny, nz;
@@ -143,8 +147,8 @@ void launch_temp_solid_fix(const Geo &g, const int32_t *ijk, int n, const int32_
 template <typename R>
 void launch_moist_fill(const Geo &g, const uint8_t *code, R *q, double ref, hipStream_t st);
 template <typename T>
-void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, WMUstarPoints wp, double nu,
-                     T *Visc, hipStream_t st);
+void launch_wm_ustar(const Geo &g, const T *U, const T *V, const T *W, const T *Temp, const T *Moist,
+                     WMUstarPoints wp, double nu, T *Visc, hipStream_t st);
 template <typename T>
 void launch_momentum(const Geo &g, int comp, const T *U, const T *V, const T *W, const T *Visc, const T *Pr, T *Ustar,
                      T *U2, const uint8_t *code, StageConst sc, hipStream_t st);

@@ -75,6 +75,7 @@ class Lists(C.Structure):
         ("src_n", C.c_int32), ("src_ijk", _pi32), ("src_scalar", _pi32), ("src_flux", _pd),
         ("wm_tclass", C.POINTER(C.c_int8) * 18), ("wm_tval", _pd * 18),
         ("wmp_tclass", C.POINTER(C.c_int8)), ("wmp_tval", _pd),
+        ("wm_qval", _pd * 18), ("wm_qflux", _pd * 18), ("wmp_qval", _pd), ("wmp_qflux", _pd),
     ]
 
 
@@ -288,6 +289,17 @@ class Case:
         if not tc:
             return np.zeros(n, np.int8), np.zeros(n)
         return _np(tc, n, np.int8), _np(tv, n, np.float64)
+
+    def wm_moist(self, l):
+        """Surface moisture and initial moisture flux of the points of list l
+        (-1: the scalar wall-model points)."""
+        v = self.view
+        n = v.wmp_n if l < 0 else v.wm_n[l]
+        qv = v.wmp_qval if l < 0 else v.wm_qval[l]
+        qf = v.wmp_qflux if l < 0 else v.wm_qflux[l]
+        if not qv:
+            return np.zeros(n), np.zeros(n)
+        return _np(qv, n, np.float64), _np(qf, n, np.float64)
 
     def wmp_list(self):
         v = self.view

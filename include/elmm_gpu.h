@@ -39,8 +39,8 @@ enum {
   ELMM_BC_PERIODIC = 3,
   ELMM_BC_DIRICHLET = 4,
   ELMM_BC_NEUMANN = 5,
-  ELMM_BC_CONSTFLUX = 6   /* temperature only: a surface heat flux (K m/s) on a
-                             wall with wall-model points (Monin-Obukhov law) */
+  ELMM_BC_CONSTFLUX = 6   /* temperature and moisture: a surface flux on a wall
+                             with wall-model points (Monin-Obukhov law)     */
 };
 
 /* Thermal class of a wall-model point (WMPoint%prescribed_temperature,
@@ -207,6 +207,12 @@ typedef struct elmm_lists {
   const double  *wm_tval[18];
   const int8_t  *wmp_tclass;
   const double  *wmp_tval;
+  /* surface moisture (WMPoint%moisture) and initial moisture flux
+     (%moisture_flux) of each wall-model point; null: zero */
+  const double  *wm_qval[18];
+  const double  *wm_qflux[18];
+  const double  *wmp_qval;
+  const double  *wmp_qflux;
 } elmm_lists;
 
 /* Per-call diagnostics of elmm_step. */

@@ -42,6 +42,8 @@ typedef struct {
   int8_t *tclass;  /* ELMM_WM_* (prescribed_temperature, temperature_flux) */
   double *tval;    /* surface temperature or heat flux */
   double *tflux;   /* WMPoint%temperature_flux, persistent */
+  double *qval;    /* WMPoint%moisture (surface moisture) */
+  double *qflux;   /* WMPoint%moisture_flux, persistent */
 } wm_list;
 
 struct oracle {
@@ -63,6 +65,7 @@ struct oracle {
   double *wmp_div;
   double *wmp_af;   /* WMPoint%area_factor */
   int temp_bt[6];   /* TempBtype after GetOutsideBoundariesWM (0 on MO walls) */
+  int moist_bt[6];  /* MoistBtype after it (0 on walls with a moisture flux) */
   int scfl_n[3];
   int32_t *scfl_ijk[3];
   int ib_n[4];
@@ -161,8 +164,14 @@ static void init_grid(oracle *o) {
 static double *field(oracle *o) { return xcalloc(o->ntot, sizeof(double)); }
 
 static void copy_wm(wm_list *d, int n, const int32_t *ijk, const double *dist, const double *z0, const int8_t *tc,
-                    const double *tv) {
+                    const double *tv, const double *qv, const double *qf) {
   d->n = n;
+  d->qval = xcalloc((size_t)n + 1, sizeof(double));
+  d->qflux = xcalloc((size_t)n + 1, sizeof(double));
+  for (int p = 0; p < n && qv; p++) {
+    d->qval[p] = qv[p];
+    d->qflux[p] = qf[p];
+  }
   d->tclass = xcalloc((size_t)n + 1, sizeof(int8_t));
   d->tval = xcalloc((size_t)n + 1, sizeof(double));
   d->tflux = xcalloc((size_t)n + 1, sizeof(double));
@@ -302,8 +311,10 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
     memcpy(o->scflmask[d], L->scfl_mask[d], o->ntot);
   }
   for (int l = 0; l < 18; l++)
-    copy_wm(&o->wm[l], L->wm_n[l], L->wm_ijk[l], L->wm_dist[l], L->wm_z0[l], L->wm_tclass[l], L->wm_tval[l]);
-  copy_wm(&o->wmp, L->wmp_n, L->wmp_ijk, L->wmp_dist, L->wmp_z0, L->wmp_tclass, L->wmp_tval);
+    copy_wm(&o->wm[l], L->wm_n[l], L->wm_ijk[l], L->wm_dist[l], L->wm_z0[l], L->wm_tclass[l], L->wm_tval[l],
+            L->wm_qval[l], L->wm_qflux[l]);
+  copy_wm(&o->wmp, L->wmp_n, L->wmp_ijk, L->wmp_dist, L->wmp_z0, L->wmp_tclass, L->wmp_tval, L->wmp_qval,
+          L->wmp_qflux);
   o->wmp_div = xcalloc(L->wmp_n + 1, sizeof(double));
   o->wmp_af = xcalloc(L->wmp_n + 1, sizeof(double));
   for (int p = 0; p < L->wmp_n; p++) o->wmp_af[p] = area_factor(o, &o->wmp.dist[3 * p]);
@@ -315,6 +326,10 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
     if (q >= Bo && o->c.enable_buoyancy && o->c.wall_model && o->c.btype[q] == NOSLIP &&
         (o->c.temp_btype[q] == DIRICHLET || o->c.temp_btype[q] == CONSTFLUX))
       o->temp_bt[q] = 0;
+    o->moist_bt[q] = o->c.moist_btype[q];
+    if (q >= Bo && o->c.enable_buoyancy && o->c.enable_moisture && o->c.wall_model && o->c.btype[q] == NOSLIP &&
+        o->c.moist_btype[q] == CONSTFLUX)
+      o->moist_bt[q] = 0;
   }
   for (int d = 0; d < 3; d++) {
     o->scfl_n[d] = L->scfl_n[d];
@@ -354,6 +369,8 @@ oracle *oracle_create(const elmm_config *cfg, const elmm_lists *L) {
 }
 
 static void free_wm(wm_list *l) {
+  free(l->qval);
+  free(l->qflux);
   free(l->tclass);
   free(l->tval);
   free(l->tflux);
@@ -698,7 +715,7 @@ static void bound_scalar(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.scal
 static void bound_temperature(oracle *o, double *a) { bound_scalar_bc(o, a, o->temp_bt, o->c.side_temp); }
 /* BoundMoisture, src/scalarboundaries.f90:311-323: the same rules with
    MoistBtype and sideMoist */
-static void bound_moisture(oracle *o, double *a) { bound_scalar_bc(o, a, o->c.moist_btype, o->c.side_moist); }
+static void bound_moisture(oracle *o, double *a) { bound_scalar_bc(o, a, o->moist_bt, o->c.side_moist); }
 
 /* BoundViscosity, src/scalarboundaries.f90:326-431 */
 static void bound_visc(oracle *o, double *a) {
@@ -1214,6 +1231,55 @@ static void mo_dirichlet_ustar_tfl(double *ustar_io, double *tflux_io, double di
   *tflux_io = tflux;
 }
 
+/* WM_MO_Dirichlet_ustar_tfl_mfl, src/wallmodels.f90:1721-1784: the
+   Dirichlet law with the virtual heat flux (no saturation, e = 0.61) in the
+   Obukhov length and the moisture flux by the same stability function
+   (k_Q = k_T); its laminar branch is dist <= z0 only, and its fallback
+   resets u* and the heat flux but keeps the moisture flux. */
+static void mo_dirichlet_moist(double *ustar_io, double *tflux_io, double *qflux_io, double dist, double z0,
+                               double logz0, double logz0h, double vel, double temp, double surf_temp, double moist,
+                               double surf_moist, double nu, double kappa, double g, double tref) {
+  const double eps = 1e-3, kU = 0.4, kT = 0.47, kQ = 0.47, e_coef = 0.61;
+  const double tempdif = temp - surf_temp, moistdif = moist - surf_moist;
+  double ustar = wm_rough_ustar(*ustar_io, vel, dist, z0, nu), tflux = *tflux_io, qflux = *qflux_io;
+  if (dist <= z0) {
+    tflux = -((kappa * tempdif) / dist);
+    qflux = -((kappa * moistdif) / dist);
+  } else {
+    double zL = 0, zL0 = -10000;
+    int i = 0;
+    for (;;) {
+      i = i + 1;
+      const double virt = tflux * (1 + e_coef * moist) + (e_coef * temp) * qflux;
+      zL = -(((dist * g) * virt) / ((kU * tref) * ((ustar * ustar) * ustar)));
+      const double psi_m = psim_mo_mod(zL, kU), psi_h = psih_mo_mod(zL, kU), psi_e = psi_h;
+      ustar = (vel * kU) / (logz0 - psi_m);
+      tflux = -(((tempdif * ustar) * kT) / (logz0h - psi_h));
+      qflux = -(((moistdif * ustar) * kQ) / (logz0h - psi_e));
+      double den = fabs(zL);
+      if (!(den > 1e-4)) den = 1e-4;
+      if (i > 1 && fabs(zL - zL0) / den < eps) break;
+      if (i >= 50 || zL > 10000) break;
+      zL0 = zL;
+    }
+    if (i >= 50 || zL > 10000) {
+      ustar = sqrt(nu * vel / dist);
+      tflux = -((kappa * tempdif) / dist);
+    }
+  }
+  *ustar_io = ustar;
+  *tflux_io = tflux;
+  *qflux_io = qflux;
+}
+
+void oracle_mo_dirichlet_moist(double *ustar, double *tflux, double *qflux, double dist, double z0, double vel,
+                               double temp, double surf_temp, double moist, double surf_moist, double nu,
+                               double kappa, double g, double tref) {
+  const double lz = log(dist / z0);
+  mo_dirichlet_moist(ustar, tflux, qflux, dist, z0, lz, lz, vel, temp, surf_temp, moist, surf_moist, nu, kappa, g,
+                     tref);
+}
+
 /* exported for the pins of the Monin-Obukhov laws */
 double oracle_psim_mo(double zeta) { return psim_mo(zeta); }
 double oracle_psim_mo_mod(double zeta) { return psim_mo_mod(zeta, 0.4); }
@@ -1255,6 +1321,7 @@ static void wm_uvw_fluxes(oracle *o) {
   const double nu = o->c.nu;
   const int buoy = o->c.enable_buoyancy;
   const double *T = buoy ? o->S[o->c.nscalars] : NULL;
+  const double *Mq = buoy && o->c.enable_moisture ? o->S[o->c.nscalars + 1] : NULL;
   const double eps = sqrt(2.220446049250313e-16);
   double drec[6] = {1 / o->dxmin, 1 / o->dxmin, 1 / o->dymin, 1 / o->dymin, 1 / o->dzmin, 1 / o->dzmin};
   for (int l = 0; l < 18; l++) {
@@ -1275,8 +1342,13 @@ static void wm_uvw_fluxes(oracle *o) {
           const long off = comp == 0 ? 1 : comp == 1 ? o->sy : o->sz;
           const double temp = (T[IX(o, xi, yj, zk) + off] + T[IX(o, xi, yj, zk)]) / 2;
           const double lz = log(dist / L->z0[p]);
-          mo_dirichlet_ustar_tfl(&ustar, &L->tflux[p], dist, L->z0[p], lz, lz, v, temp - L->tval[p], nu,
-                                 o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
+          if (Mq) /* WM_MO_DirichletStress_moist (1941-1963) */
+            mo_dirichlet_moist(&ustar, &L->tflux[p], &L->qflux[p], dist, L->z0[p], lz, lz, v, temp, L->tval[p],
+                               (Mq[IX(o, xi, yj, zk) + off] + Mq[IX(o, xi, yj, zk)]) / 2, L->qval[p], nu,
+                               o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
+          else
+            mo_dirichlet_ustar_tfl(&ustar, &L->tflux[p], dist, L->z0[p], lz, lz, v, temp - L->tval[p], nu,
+                                   o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
         } else if (buoy && L->tflux[p] > 0) {
           /* WM_MO_FluxStress (1836-1854) */
           if (v != 0)
@@ -1405,6 +1477,7 @@ static void wm_visc_points(oracle *o, int write_visc) {
   const double nu = o->c.nu;
   const int buoy = o->c.enable_buoyancy;
   const double *T = buoy ? o->S[o->c.nscalars] : NULL;
+  const double *Mq = buoy && o->c.enable_moisture ? o->S[o->c.nscalars + 1] : NULL;
   wm_list *L = &o->wmp;
   for (int p = 0; p < L->n; p++) {
     const int xi = L->ijk[3 * p], yj = L->ijk[3 * p + 1], zk = L->ijk[3 * p + 2];
@@ -1420,8 +1493,13 @@ static void wm_visc_points(oracle *o, int write_visc) {
       vel_and_dist(tan, &dist, vel, wallu, d);
       const double v = norm3(tan);
       const double lz = log(dist / L->z0[p]);
-      mo_dirichlet_ustar_tfl(&ustar, &L->tflux[p], dist, L->z0[p], lz, lz, v, A3(o, T, xi, yj, zk) - L->tval[p],
-                             nu, o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
+      if (Mq) /* WM_MO_Dirichlet_moist (1887-1919) */
+        mo_dirichlet_moist(&ustar, &L->tflux[p], &L->qflux[p], dist, L->z0[p], lz, lz, v, A3(o, T, xi, yj, zk),
+                           L->tval[p], A3(o, Mq, xi, yj, zk), L->qval[p], nu, o->c.nu_scalar, o->c.grav_acc,
+                           o->c.temperature_ref);
+      else
+        mo_dirichlet_ustar_tfl(&ustar, &L->tflux[p], dist, L->z0[p], lz, lz, v, A3(o, T, xi, yj, zk) - L->tval[p],
+                               nu, o->c.nu_scalar, o->c.grav_acc, o->c.temperature_ref);
       if (ustar < 0) ustar = 0;
       if (v > 0 || L->tflux[p] != 0) {
         const double a = ustar * ustar * dist / v;
@@ -2229,6 +2307,9 @@ static void scalar_stage(oracle *o, int sc, int stage, double dt, int kind) {
        first (scalars.f90:322-326) */
     if (kind == 1)
       A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp.tflux[p] * o->wmp_af[p];
+    /* MoistureExtra: the moisture flux likewise (scalars.f90:350-354) */
+    if (kind == 2)
+      A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp.qflux[p] * o->wmp_af[p];
     A3(o, Sadv, q[0], q[1], q[2]) = A3(o, Sadv, q[0], q[1], q[2]) + o->wmp_div[p] * A3(o, S, q[0], q[1], q[2]);
   }
   double b = RK_beta[stage] * dt;

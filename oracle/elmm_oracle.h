@@ -76,6 +76,10 @@ double oracle_mo_flux_ustar(double vel, double dist, double ustar, double z0, do
                             double nu);
 void oracle_mo_dirichlet(double *ustar, double *tflux, double dist, double z0, double vel, double tempdif, double nu,
                          double kappa, double g, double tref);
+/* WM_MO_Dirichlet_ustar_tfl_mfl with z0H = z0 (ustar, tflux, qflux updated) */
+void oracle_mo_dirichlet_moist(double *ustar, double *tflux, double *qflux, double dist, double z0, double vel,
+                               double temp, double surf_temp, double moist, double surf_moist, double nu,
+                               double kappa, double g, double tref);
 void oracle_op_wm_fluxes(oracle *o);            /* ComputeUVWFluxesWM           */
 void oracle_op_ib_momentum(oracle *o);          /* IBMomentum(U,V,W)            */
 void oracle_op_mass_sources(oracle *o);         /* IBMassSources + Bound_Q -> Q */

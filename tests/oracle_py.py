@@ -51,6 +51,7 @@ def _load():
         lib.oracle_mo_flux_ustar.argtypes = [C.c_double] * 8
         lib.oracle_mo_flux_ustar.restype = C.c_double
         lib.oracle_mo_dirichlet.argtypes = [pd, pd] + [C.c_double] * 8
+        lib.oracle_mo_dirichlet_moist.argtypes = [pd, pd, pd] + [C.c_double] * 11
         lib.oracle_op_wm_fluxes.argtypes = [C.c_void_p]
         lib.oracle_wm_list.argtypes = [C.c_void_p, C.c_int32, pd, pd]
         lib.oracle_wm_list.restype = C.c_int32
@@ -85,6 +86,15 @@ def det_fn(name, *args):
     """The oracle's fixed-operation exp, atan, pow and Monin-Obukhov
     functions (oracle_<name>)."""
     return getattr(_load(), "oracle_" + name)(*[float(a) for a in args])
+
+
+def mo_dirichlet_moist(ustar, tflux, qflux, dist, z0, vel, temp, surf_temp, moist, surf_moist, nu, kappa, g, tref):
+    """WM_MO_Dirichlet_ustar_tfl_mfl (z0H = z0): the updated (ustar, tflux,
+    qflux)."""
+    u, t, q = C.c_double(ustar), C.c_double(tflux), C.c_double(qflux)
+    _load().oracle_mo_dirichlet_moist(C.byref(u), C.byref(t), C.byref(q), dist, z0, vel, temp, surf_temp, moist,
+                                      surf_moist, nu, kappa, g, tref)
+    return u.value, t.value, q.value
 
 
 def mo_dirichlet(ustar, tflux, dist, z0, vel, tempdif, nu, kappa, g, tref):

@@ -6,7 +6,13 @@ iteration) and a flux temperature (ELMM_BC_CONSTFLUX) as their surface heat
 flux (WM_MO_FLUX, WM_MO_FluxStress when positive); the temperature then gets
 the flux times the point's area factor (TemperatureExtra, src/scalars.f90:
 322-326) and the mirror ghost rule on that wall (TempBtype = 0,
-src/wallmodels.f90:618-620; scalarboundaries.f90:233-239, 271-277).
+src/wallmodels.f90:618-620; scalarboundaries.f90:233-239, 271-277).  With
+moisture, the Dirichlet law takes the virtual heat flux and returns the
+moisture flux too (WM_MO_Dirichlet(Stress)_moist,
+WM_MO_Dirichlet_ustar_tfl_mfl, src/wallmodels.f90:1721-1784, 1887-1963); a
+Dirichlet moisture is the points' surface moisture, a flux moisture their
+moisture flux (and the mirror ghost rule), added like the heat
+(MoistureExtra, src/scalars.f90:336-356).
 
 CPU pins of the oracle: its fixed-operation exp, atan and pow within a few
 ulps of libm; the stability functions against their textbook formulas (libm);
@@ -17,9 +23,12 @@ relations at the Obukhov length they imply, unstable for a warm surface and
 stable for a cold one, and the molecular flux in the laminar branch; a
 heated floor under fluid at rest warms its first layer by q dt / dz per step
 and nothing else (a lost area factor, a sign error of the top wall or a
-wrong ghost rule fails it).  GPU: bit-identical to the oracle (FP64) with a
-warm rough floor, a heat-flux floor and a cooled rough top, with obstacles,
-after the initial tail and 1 and 3 steps; the precision form within 1e-5.
+wrong ghost rule fails it); the moist law's three fluxes against the bulk
+relations at the virtual flux's Obukhov length; a moisture-flux floor wets
+its first layer by m dt / dz.  GPU: bit-identical to the oracle (FP64) with a
+warm rough floor, a heat-flux floor, a cooled rough top, and with moisture (a
+wetter surface, a moisture-flux floor), with obstacles, after the initial
+tail and 1 and 3 steps; the precision form within 1e-5.
 """
 import math
 
@@ -28,7 +37,7 @@ import pytest
 
 from elmm_amd import capi, cases
 from helpers import compare_state, velocity_rel_l2
-from oracle_py import Oracle, det_fn, mo_dirichlet
+from oracle_py import Oracle, det_fn, mo_dirichlet, mo_dirichlet_moist
 from test_buoyancy import OBST, T_BOT, T_TOP, buoyant_config, temperature_field
 
 NU = cases.NU_DEFAULT
@@ -230,33 +239,96 @@ def test_oracle_warm_floor_heats_the_air(floor):
         assert abs(rise) < 1e-9
 
 
-def test_moist_monin_obukhov_is_rejected():
-    cfg = mo_config(enable_moisture=1, moisture_ref=0.01, moist_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN] * 2,
-                    side_moist=[0] * 6)
-    with pytest.raises(capi.ElmmError, match="moist Monin-Obukhov"):
+@pytest.mark.parametrize("tempdif,moistdif", [(-2.0, -0.004), (-0.5, 0.003), (1.0, -0.002)])
+def test_moist_dirichlet_law_bulk_relations(tempdif, moistdif):
+    """WM_MO_Dirichlet_ustar_tfl_mfl: (u*, heat flux, moisture flux)
+    consistent with the bulk relations at the Obukhov length of the virtual
+    heat flux tflux (1 + 0.61 q) + 0.61 T qflux; a drier surface (moistdif >
+    0) gives a downward moisture flux."""
+    g, tref, vel, dist, z0, T, q = 9.81, 295.0, 2.0, 0.5, 0.01, 295.0, 0.01
+    u, tf, qf = 1.0, 0.0, 0.0
+    for _ in range(3):
+        u, tf, qf = mo_dirichlet_moist(u, tf, qf, dist, z0, vel, T, T - tempdif, q, q - moistdif, NU, KAPPA, g,
+                                       tref)
+    L = math.log(dist / z0)
+    virt = tf * (1 + 0.61 * q) + 0.61 * T * qf
+    zl = -dist * g * virt / (0.4 * tref * u ** 3)
+    u_want = vel * 0.4 / (L - _psim_mod(zl))
+    tf_want = -tempdif * u_want * 0.47 / (L - _psih_mod(zl))
+    qf_want = -moistdif * u_want * 0.47 / (L - _psih_mod(zl))
+    for got, want in ((u, u_want), (tf, tf_want), (qf, qf_want)):
+        assert abs(got - want) <= 5e-3 * abs(want), (got, want)
+    assert (qf > 0) == (moistdif < 0)
+
+
+def test_oracle_moisture_flux_wets_the_wall_layer():
+    """Fluid at rest, uniform temperature and moisture, a moisture flux m on
+    the floor: after one step the first layer's moisture has risen by
+    m dt / dz, up to molecular diffusion within the step."""
+    m = 1e-4
+    cfg = buoyant_config(16, 16, 8, temp_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.NEUMANN],
+                         side_temp=[0] * 6, enable_moisture=1, moisture_ref=0.01,
+                         moist_btype=[capi.PERIODIC] * 4 + [capi.CONSTFLUX, capi.NEUMANN],
+                         side_moist=[0, 0, 0, 0, m, 0])
+    o = Oracle(cfg, capi.Case(cfg))
+    shape = (cfg.nx + 6, cfg.ny + 6, cfg.nz + 6)
+    zero = np.zeros(shape, order="F")
+    for f in (capi.F_U, capi.F_V, capi.F_W, capi.F_PR, capi.F_SCALAR):
+        o.upload(f, zero)
+    o.upload(capi.F_TEMPERATURE, np.full(shape, 295.0, order="F"))
+    o.upload(capi.F_MOISTURE, np.full(shape, 0.01, order="F"))
+    o.init()
+    o.step(1)
+    q = o.download(capi.F_MOISTURE)[3:3 + cfg.nx, 3:3 + cfg.ny, 3:3 + cfg.nz]
+    dq = m * cfg.dt / cfg.dz
+    tol = 10 * KAPPA * dq * cfg.dt / cfg.dz ** 2 + 1e-15
+    assert np.abs(q[:, :, 0] - (0.01 + dq)).max() <= tol
+    assert np.abs(q[:, :, 1:] - 0.01).max() <= tol
+    o.close()
+
+
+def test_moisture_flux_needs_a_wall_model_wall():
+    cfg = mo_config(enable_moisture=1, moisture_ref=0.01,
+                    moist_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.CONSTFLUX], side_moist=[0, 0, 0, 0, 0, 1e-4])
+    with pytest.raises(capi.ElmmError, match="moisture flux"):
         capi.Case(cfg)
 
 
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
-@pytest.mark.parametrize("bottom,top", [("temp", "slip"), ("flux", "slip"), ("flux", "wall")])
-def test_gpu_monin_obukhov_bit_identical_to_oracle(require_gpu, bottom, top):
-    cfg = mo_config(48, 32, 16, bottom=bottom, top=top)
+@pytest.mark.parametrize("bottom,top,moist", [("temp", "slip", None), ("flux", "slip", None), ("flux", "wall", None),
+                                              ("temp", "slip", "dirichlet"), ("temp", "wall", "flux")])
+def test_gpu_monin_obukhov_bit_identical_to_oracle(require_gpu, bottom, top, moist):
+    kw = {}
+    if moist:
+        mb = capi.DIRICHLET if moist == "dirichlet" else capi.CONSTFLUX
+        mv = 0.014 if moist == "dirichlet" else 2e-4
+        kw = dict(enable_moisture=1, moisture_ref=0.01, moist_btype=[capi.PERIODIC] * 4 + [mb, capi.NEUMANN],
+                  side_moist=[0, 0, 0, 0, mv, 0])
+    cfg = mo_config(48, 32, 16, bottom=bottom, top=top, **kw)
     case = capi.Case(cfg, cuboids=OBST, sources=[(4.5, 4.5, 0.5, 1.0, 0)])
     f = cases.initial_fields(cfg, u0=2.0, amp=0.3, seed=41)
     T = temperature_field(cfg, seed=42)
     o, s = Oracle(cfg, case), capi.Solver(cfg, case)
     s32 = capi.Solver(capi.with_precision(cfg, capi.FP32_REFINED), case)
+    qf = None
+    if moist:
+        g = np.random.default_rng(43)
+        qf = np.asfortranarray(0.01 + g.uniform(-0.001, 0.001, T.shape))
     for x in (o, s, s32):
         cases.upload_initial(x, cfg, f)
         x.upload(capi.F_TEMPERATURE, T)
+        if moist:
+            x.upload(capi.F_MOISTURE, qf)
         x.init()
     fields = ["U", "V", "W", "Pr", "Visc", "TDiff", "S0"]
+    fids = [capi.F_TEMPERATURE] + ([capi.F_MOISTURE] if moist else [])
 
     def same():
         err = compare_state(s, o, cfg, fields)
         assert max(err.values()) == 0.0, err
-        assert np.array_equal(s.download(capi.F_TEMPERATURE), o.download(capi.F_TEMPERATURE))
+        for fid in fids:
+            assert np.array_equal(s.download(fid), o.download(fid)), fid
         assert np.array_equal(s.wm_ustar(), o.wm_ustar())
 
     same()
@@ -267,7 +339,7 @@ def test_gpu_monin_obukhov_bit_identical_to_oracle(require_gpu, bottom, top):
         same()
         l2 = velocity_rel_l2(s32, o, cfg)
         t32 = np.abs(s32.download(capi.F_TEMPERATURE) - o.download(capi.F_TEMPERATURE)).max() / T_TOP
-        print(f"MO {bottom}/{top} after {n} more steps: precision form velocity L2 {l2:.2e}, temperature {t32:.2e}",
+        print(f"MO {bottom}/{top}/{moist} after {n} more steps: precision form velocity L2 {l2:.2e}, temperature {t32:.2e}",
               flush=True)
         assert l2 <= 1e-5 and t32 <= 1e-5, (l2, t32)
     for x in (o, s, s32):
