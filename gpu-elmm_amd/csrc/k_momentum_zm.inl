@@ -137,7 +137,10 @@ __device__ inline void finish_cell(const StageConst &sc, int C, T w, T f0, T dpr
 // (launch bounds, in waves per SIMD: 4-byte fields 4 for 512-thread blocks
 // (128 VGPRs, two blocks per CU) and 3 for 256-thread blocks (three blocks,
 // as many as their LDS allows); 8-byte fields 2)
-template <typename T, int TX, int TY, bool FAST>
+// BUOY: 0 no buoyancy, 1 the temperature's, 2 with the virtual temperature
+// (moisture); a compile-time choice, so the case without buoyancy keeps the
+// registers of the temperature and moisture columns
+template <typename T, int TX, int TY, bool FAST, int BUOY>
 __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 : 3) : 2)
     k_momentum_zm(Geo g, MomZArgs<T> a, StageConst sc, int kchunk) {
   constexpr bool F32 = sizeof(T) == 4;
@@ -252,13 +255,13 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
     cpry = a.Pr[pc + (long)kbeg * sz + g.sy], cprz = a.Pr[pc + (long)(kbeg + 1) * sz];
   // temperature at k and k+1 of the column (buoyancy only)
   T ct0 = 0, ct1 = 0;
-  if (sc.buoy) {
+  if constexpr (BUOY >= 1) {
     ct0 = a.Temp[pc + (long)kbeg * sz];
     ct1 = a.Temp[pc + (long)(kbeg + 1) * sz];
   }
   // moisture at k and k+1 (moist buoyancy)
   T cq0 = 0, cq1 = 0;
-  if (sc.moist) {
+  if constexpr (BUOY == 2) {
     cq0 = a.Moist[pc + (long)kbeg * sz];
     cq1 = a.Moist[pc + (long)(kbeg + 1) * sz];
   }
@@ -295,9 +298,9 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
         ow = a.W[o];
       }
       T nt = 0;
-      if (sc.buoy) nt = a.Temp[pc + (long)(k + 2) * sz];
+      if constexpr (BUOY >= 1) nt = a.Temp[pc + (long)(k + 2) * sz];
       T nq = 0;
-      if (sc.moist) nq = a.Moist[pc + (long)(k + 2) * sz];
+      if constexpr (BUOY == 2) nq = a.Moist[pc + (long)(k + 2) * sz];
       const long q = pc + (long)(k + 1) * sz;
       const T nprx = a.Pr[q + 1], npry = a.Pr[q + g.sy], nprz = a.Pr[q + sz];
       uint8_t ncd[3];
@@ -400,8 +403,8 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
             }
             w = adv(dX, dY, dZ);
           }
-          if (sc.buoy) {
-            if (sc.moist) {
+          if constexpr (BUOY >= 1) {
+            if constexpr (BUOY == 2) {
               // W(k) gets Tv(k) in the pass over odd k and Tv(k+1) in the
               // pass over even k (and the reverse for even k)
               const T tv0 = ct0 * ((T)1 + (T)0.61 * cq0), tv1 = ct1 * ((T)1 + (T)0.61 * cq1);
@@ -474,12 +477,21 @@ __global__ void __launch_bounds__(TX *TY, sizeof(T) == 4 ? (TX * TY == 512 ? 4 :
 
 
 namespace {
+template <typename T, int TX, int TY, bool FAST, int BUOY>
+void launch_tile_b(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
+  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
+  if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY, FAST, BUOY>, lds);
+  dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
+  hipLaunchKernelGGL((k_momentum_zm<T, TX, TY, FAST, BUOY>), gr, b, lds, st, g, a, sc, kchunk);
+}
 template <typename T, int TX, int TY, bool FAST>
 void launch_tile_f(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
-  const size_t lds = (size_t)NSLOT * Tile<TX, TY>::PS * sizeof(T);
-  if (lds > 65536) (void)lds_limit_raise((const void *)k_momentum_zm<T, TX, TY, FAST>, lds);
-  dim3 b(TX * TY), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL((k_momentum_zm<T, TX, TY, FAST>), gr, b, lds, st, g, a, sc, kchunk);
+  if (!sc.buoy)
+    launch_tile_b<T, TX, TY, FAST, 0>(g, a, sc, kchunk, st);
+  else if (!sc.moist)
+    launch_tile_b<T, TX, TY, FAST, 1>(g, a, sc, kchunk, st);
+  else
+    launch_tile_b<T, TX, TY, FAST, 2>(g, a, sc, kchunk, st);
 }
 template <typename T, int TX, int TY>
 void launch_tile(const Geo &g, const MomZArgs<T> &a, StageConst sc, int kchunk, hipStream_t st) {
