@@ -3,9 +3,11 @@ the type arrays, the IB lists, the solid-body wall-model points, the
 nullified points and the scalar face masks and lists, equal byte for byte and
 in order to the host builder (elmm_case_build), which the solids tests pin
 against point-in-box tests, binary dilation and erosion, interpolation of
-wall-linear fields and closed-form wall distances.  Cases: cuboids on a
-periodic grid, cuboids across the periodic x and y boundaries, a no-slip top
-with a rough wall and a body touching it, the wall model off, a non-unit
+wall-linear fields and closed-form wall distances; with them the wall-model
+points' thermal classes, surface moisture and moisture fluxes.  Cases:
+cuboids on a periodic grid, cuboids across the periodic x and y boundaries,
+a rough no-slip top and a body touching it with a heat- and moisture-flux
+floor and a Dirichlet top, the wall model off, a non-unit
 spacing with cuboid faces off the grid, dispersion scenarios of 0.3-2.5 M
 cells with 30-140 bodies, the 512x512x128 urban case with 64 bodies; no
 bodies; and an invalid cuboid, rejected with the host builder's message."""
@@ -27,7 +29,14 @@ def _cases():
     wrap = [(-3.0, 5.0, 10.0, 20.0, 0.0, 6.0), (26.0, 35.0, -2.0, 4.0, 0.0, 4.0, 0.05),
             (12.0, 14.0, 12.0, 14.0, 0.0, 15.5)]
     out.append(("wrap", cfg, wrap, []))
-    top = cases.uniform_config(32, 24, 16, dx=1.0, dt=0.05, nscalars=1, top=capi.NOSLIP)
+    # no-slip top; buoyancy with a heat-flux floor and a Dirichlet top, and
+    # moisture with a flux floor: wall points of both walls carry thermal
+    # classes, surface moisture and moisture fluxes
+    top = cases.uniform_config(32, 24, 16, dx=1.0, dt=0.05, nscalars=1, top=capi.NOSLIP, enable_buoyancy=1,
+                               temp_btype=[capi.PERIODIC] * 4 + [capi.CONSTFLUX, capi.DIRICHLET],
+                               side_temp=[0, 0, 0, 0, 0.1, 296.0], enable_moisture=1, moisture_ref=0.01,
+                               moist_btype=[capi.PERIODIC] * 4 + [capi.CONSTFLUX, capi.DIRICHLET],
+                               side_moist=[0, 0, 0, 0, 1e-4, 0.008])
     top.z0_wall[5] = 0.02
     out.append(("noslip top", top, [(0.0, 6.0, 4.0, 10.0, 0.0, 5.0), (20.0, 32.0, 18.0, 24.0, 0.0, 8.0, 0.03),
                                     (10.0, 14.0, 10.0, 14.0, 12.0, 16.0)], [(15.5, 12.5, 0.5, 1.0, 0)]))

@@ -174,7 +174,11 @@ def _case_arrays(case):
     out = list(case.masks())
     for l in range(18):
         out.extend(case.wm_list(l))
+        out.extend(case.wm_thermal(l))
+        out.extend(case.wm_moist(l))
     out.extend(case.wmp_list())
+    out.extend(case.wm_thermal(-1))
+    out.extend(case.wm_moist(-1))
     for c in range(4):
         out.extend(case.ib_list(c))
     for c in range(3):
