@@ -43,6 +43,7 @@ namespace elmm {
 sg::Geom make_geom(const elmm_config &c, const Grid1D &g, const CaseData &cd);
 int make_bodies(int ncub, const elmm_cuboid *cub, std::vector<sg::Body> &bodies);
 double solid_delta(const Grid1D &g);
+void body_thermal(const elmm_config &c, const sg::Body &B, int8_t &tc, double &tv, double &qf);
 
 namespace {
 
@@ -128,6 +129,7 @@ struct ListArgs {
   const int32_t *poff, *toff;   // fill pass: inclusive sums of the counts
   int32_t *ijk, *ptr, *src;     // fill pass outputs
   double *coef, *dist, *z0;
+  int32_t *bid;                 // wall-model points: the body
   uint8_t *mask;                // scalar face mask (fill pass, every cell)
 };
 
@@ -141,6 +143,7 @@ __global__ void k_solid_list(sg::Geom G, const sg::Body *bodies, ListArgs a, lon
   int np = 0, nt = 0, err = 0;
   sg::Tap taps[6];
   double dv[3] = {0, 0, 0}, z0 = 0;
+  int bid = 0;
   bool face_solid = false;
   if (in) switch (a.kind) {
       case K_IBV:
@@ -157,14 +160,15 @@ __global__ void k_solid_list(sg::Geom G, const sg::Body *bodies, ListArgs a, lon
         break;
       case K_WMP:
         if (a.T[p] < 0) {
-          const int r = sg::wm_pr_point(G, bodies, a.T, c.i, c.j, c.k, a.mwd, dv, &z0);
+          const int r = sg::wm_pr_point(G, bodies, a.T, c.i, c.j, c.k, a.mwd, dv, &z0, &bid);
           if (r < 0) err = -r;
           else np = r;
         }
         break;
       case K_WMUVW:
         if (a.T[p] < 0) {
-          const int r = sg::wm_uvw_dir(G, bodies, a.T, c.i, c.j, c.k, a.dir, a.cx, a.cy, a.cz, a.mwd, dv, &z0);
+          const int r =
+              sg::wm_uvw_dir(G, bodies, a.T, c.i, c.j, c.k, a.dir, a.cx, a.cy, a.cz, a.mwd, dv, &z0, &bid);
           if (r < 0) err = -r;
           else np = r;
         }
@@ -205,6 +209,7 @@ __global__ void k_solid_list(sg::Geom G, const sg::Body *bodies, ListArgs a, lon
     a.dist[3 * q + 1] = dv[1];
     a.dist[3 * q + 2] = dv[2];
     a.z0[q] = z0;
+    a.bid[q] = bid;
   }
 }
 
@@ -475,6 +480,7 @@ int build_on_device(const elmm_config &c, const Grid1D &g, int ncub, const elmm_
       if (J.wm) {
         J.a.dist = out.take<double>(3 * (size_t)J.n);
         J.a.z0 = out.take<double>(J.n);
+        J.a.bid = out.take<int32_t>(J.n);
       }
     }
     if (pass == 0) out.allocate();
@@ -532,10 +538,14 @@ int build_on_device(const elmm_config &c, const Grid1D &g, int ncub, const elmm_
         if (J.wm) {
           from_out(J.wm->dist, J.a.dist, 3 * (size_t)J.n);
           from_out(J.wm->z0, J.a.z0, J.n);
-          J.wm->tclass.assign(J.n, 0);  // solid bodies: neutral and dry
-          J.wm->tval.assign(J.n, 0.0);
+          std::vector<int32_t> bid;
+          from_out(bid, J.a.bid, J.n);
+          J.wm->tclass.resize(J.n);
+          J.wm->tval.resize(J.n);
           J.wm->qval.assign(J.n, 0.0);
-          J.wm->qflux.assign(J.n, 0.0);
+          J.wm->qflux.resize(J.n);
+          for (int q = 0; q < J.n; q++)
+            body_thermal(c, bodies[bid[q] - 1], J.wm->tclass[q], J.wm->tval[q], J.wm->qflux[q]);
         }
       }
     }

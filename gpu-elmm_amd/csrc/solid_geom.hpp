@@ -45,6 +45,7 @@ struct Body {
   int id;  // numofbody, 1-based
   double z0;
   Plane pl[6];
+  double tflux, qflux;  // surface heat and moisture flux (rough bodies)
 };
 
 struct Tap {
@@ -108,6 +109,8 @@ SG_HD Body make_body(int id, double xmin, double xmax, double ymin, double ymax,
   Body B;
   B.id = id;
   B.z0 = z0;
+  B.tflux = 0;
+  B.qflux = 0;
   B.pl[0] = make_plane(xmin, 0, 0, -1, 0, 0);
   B.pl[1] = make_plane(xmax, 0, 0, 1, 0, 0);
   B.pl[2] = make_plane(0, ymin, 0, 0, -1, 0);
@@ -581,7 +584,7 @@ SG_HD int scalfl_ib_point(const Geom &G, const Body *bodies, const int32_t *prty
 // solid face neighbour's body.  Returns 1 with the distance vector and z0, 0
 // when no face neighbour is solid, or a negative error code.
 SG_HD int wm_pr_point(const Geom &G, const Body *bodies, const int32_t *prtype, int i, int j, int k,
-                      double min_wall_distance, double dv[3], double *z0) {
+                      double min_wall_distance, double dv[3], double *z0, int *bid) {
   const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
   const double x = G.XP(i), y = G.YP(j), z = G.ZP(k);
   double dist = DBL_MAX, nx_ = 0, ny_ = 0, nz_ = 0;
@@ -606,6 +609,7 @@ SG_HD int wm_pr_point(const Geom &G, const Body *bodies, const int32_t *prtype, 
   if (d == 0) return -E_WM_ZERO;
   if (d < min_wall_distance) return -E_WM_CLOSE;
   *z0 = bodies[body - 1].z0 > 0 ? bodies[body - 1].z0 : 0.0;
+  *bid = body;
   return 1;
 }
 
@@ -615,7 +619,7 @@ SG_HD int wm_pr_point(const Geom &G, const Body *bodies, const int32_t *prtype, 
 // code.
 SG_HD int wm_uvw_dir(const Geom &G, const Body *bodies, const int32_t *T, int i, int j, int k, int dir,
                      const double *cx, const double *cy, const double *cz, double min_wall_distance, double dv[3],
-                     double *z0) {
+                     double *z0, int *bid) {
   const int dvec[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
   const int m = dvec[dir][0], n = dvec[dir][1], o = dvec[dir][2];
   const int t = T[G.lin(i + m, j + n, k + o)];
@@ -650,6 +654,7 @@ SG_HD int wm_uvw_dir(const Geom &G, const Body *bodies, const int32_t *T, int i,
   }
   if (norm3(dv[0], dv[1], dv[2]) < min_wall_distance) return -E_UVW_CLOSE;
   *z0 = SB.z0 > 0 ? SB.z0 : 0.0;
+  *bid = t;
   return 1;
 }
 

@@ -62,8 +62,27 @@ int make_bodies(int ncub, const elmm_cuboid *cub, std::vector<sg::Body> &bodies)
     if (!(q.xmin < q.xmax && q.ymin < q.ymax && q.zmin < q.zmax) || !(q.z0 >= 0))
       return fail(ELMM_E_ARG, "cuboid " + std::to_string(b) + ": empty extent or negative z0");
     bodies.push_back(sg::make_body(b + 1, q.xmin, q.xmax, q.ymin, q.ymax, q.zmin, q.zmax, q.z0));
+    // a rough body's surface fluxes (SolidBody%rough: z0 > 0)
+    if (q.z0 > 0) {
+      bodies.back().tflux = q.temperature_flux;
+      bodies.back().qflux = q.moisture_flux;
+    }
   }
   return 0;
+}
+
+// Thermal class, value and moisture flux of a solid body's wall-model point:
+// its heat flux with buoyancy (the flux class when non-zero), its moisture
+// flux with moisture (src/immersedboundary.f90:111-125, 260-285)
+void body_thermal(const elmm_config &c, const sg::Body &B, int8_t &tc, double &tv, double &qf) {
+  tc = ELMM_WM_NEUTRAL;
+  tv = 0;
+  qf = 0;
+  if (c.enable_buoyancy && B.tflux != 0) {
+    tc = ELMM_WM_SURFACE_FLUX;
+    tv = B.tflux;
+  }
+  if (c.enable_buoyancy && c.enable_moisture) qf = B.qflux;
 }
 
 // Inside-cell tolerance and minimum wall distance: (dxmin dymin dzmin)^(1/3) / 20
@@ -138,10 +157,15 @@ std::string solid_wm_pr(const sg::Geom &G, const std::vector<sg::Body> &bodies, 
       for (int i = 1; i <= G.nx; i++) {
         if (cd.prtype[cd.lin(i, j, k)] >= 0) continue;
         double dv[3], z0 = 0;
-        const int r = sg::wm_pr_point(G, bodies.data(), cd.prtype.data(), i, j, k, min_wall_distance, dv, &z0);
+        int bid = 0;
+        const int r =
+            sg::wm_pr_point(G, bodies.data(), cd.prtype.data(), i, j, k, min_wall_distance, dv, &z0, &bid);
         if (r < 0) return sg::solid_error(-r);
         if (r == 0) continue;
-        cd.wmp.push(i, j, k, dv[0], dv[1], dv[2], z0);  // solid bodies: neutral
+        int8_t tc;
+        double tv, qf;
+        body_thermal(cd.cfg, bodies[bid - 1], tc, tv, qf);
+        cd.wmp.push(i, j, k, dv[0], dv[1], dv[2], z0, tc, tv, 0.0, qf);
       }
   return std::string();
 }
@@ -156,11 +180,15 @@ std::string solid_wm_uvw(const sg::Geom &G, const std::vector<sg::Body> &bodies,
         if (T[cd.lin(i, j, k)] >= 0) continue;
         for (int dir = 0; dir < 6; dir++) {
           double dv[3], z0 = 0;
+          int bid = 0;
           const int r = sg::wm_uvw_dir(G, bodies.data(), T.data(), i, j, k, dir, cx.data(), cy.data(), cz.data(),
-                                       min_wall_distance, dv, &z0);
+                                       min_wall_distance, dv, &z0, &bid);
           if (r < 0) return sg::solid_error(-r);
           if (r == 0) continue;
-          cd.wm[comp * 6 + dir].push(i, j, k, dv[0], dv[1], dv[2], z0);
+          int8_t tc;
+          double tv, qf;
+          body_thermal(cd.cfg, bodies[bid - 1], tc, tv, qf);
+          cd.wm[comp * 6 + dir].push(i, j, k, dv[0], dv[1], dv[2], z0, tc, tv, 0.0, qf);
         }
       }
   return std::string();

@@ -114,7 +114,8 @@ class Tiles(C.Structure):
 
 class Cuboid(C.Structure):
     _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
-                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double), ("z0", C.c_double)]
+                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double), ("z0", C.c_double),
+                ("temperature_flux", C.c_double), ("moisture_flux", C.c_double)]
 
 
 class PointSource(C.Structure):

@@ -66,6 +66,7 @@ module elmm_gpu
 
   type, bind(C) :: elmm_cuboid
     real(c_double) :: xmin, xmax, ymin, ymax, zmin, zmax, z0
+    real(c_double) :: temperature_flux = 0, moisture_flux = 0
   end type
 
   type, bind(C) :: elmm_point_source

@@ -16,6 +16,7 @@
 ! z grid (which needs advection_method = 2).
 !   &time_stepping dt, nsteps, report_every
 !   &obstacles   n_cuboids, cuboids(7, n) = xmin xmax ymin ymax zmin zmax z0,
+!                cuboid_fluxes(2, n) = surface heat and moisture flux (default 0),
 !                device_lists (masks and lists built on the GPU; default .true.)
 !   &point_sources n_sources, sources(5, n) = x y z flux scalar(1-based)
 !   &io          init_file, out_file, u0
@@ -36,7 +37,7 @@ program elmm_host
   integer :: nsteps = 10, report_every = 5, n_cuboids = 0, n_sources = 0
   logical :: device_lists = .true.
   integer, parameter :: maxobj = 256
-  real(c_double) :: cuboids(7, maxobj) = 0, sources(5, maxobj) = 0
+  real(c_double) :: cuboids(7, maxobj) = 0, sources(5, maxobj) = 0, cuboid_fluxes(2, maxobj) = 0
   character(len=512) :: init_file = "", out_file = "", zgrid_file = ""
   integer :: advection_method = 4
   real(c_double) :: z0_top = 0, pr_gradient_x = 0, rtmp
@@ -45,7 +46,7 @@ program elmm_host
   namelist /physics/ nu, nu_scalar, c_smag, z0_bottom, z0_top, top_bc, nscalars, precision, advection_method, &
                      pr_gradient_x
   namelist /time_stepping/ dt, nsteps, report_every
-  namelist /obstacles/ n_cuboids, cuboids, device_lists
+  namelist /obstacles/ n_cuboids, cuboids, cuboid_fluxes, device_lists
   namelist /point_sources/ n_sources, sources
   namelist /io/ init_file, out_file, u0
 
@@ -65,7 +66,7 @@ program elmm_host
   end if
   if (trim(path) == "--sizes") then
     ! interoperable sizes, checked against the C header by the test suite
-    write(*, '(a,4(1x,i0))') "sizes", c_sizeof(cfg), c_sizeof(elmm_cuboid(0, 0, 0, 0, 0, 0, 0)), &
+    write(*, '(a,4(1x,i0))') "sizes", c_sizeof(cfg), c_sizeof(elmm_cuboid(0, 0, 0, 0, 0, 0, 0, 0, 0)), &
         c_sizeof(elmm_point_source(0, 0, 0, 0, 0, 0)), c_sizeof(diag)
     stop
   end if
@@ -133,7 +134,7 @@ program elmm_host
   allocate(cub(max(1, n_cuboids)), src(max(1, n_sources)))
   do i = 1, n_cuboids
     cub(i) = elmm_cuboid(cuboids(1, i), cuboids(2, i), cuboids(3, i), cuboids(4, i), cuboids(5, i), &
-                         cuboids(6, i), cuboids(7, i))
+                         cuboids(6, i), cuboids(7, i), cuboid_fluxes(1, i), cuboid_fluxes(2, i))
   end do
   do i = 1, n_sources
     src(i) = elmm_point_source(sources(1, i), sources(2, i), sources(3, i), sources(4, i), &

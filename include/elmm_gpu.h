@@ -235,12 +235,18 @@ typedef struct elmm_diag {
 typedef struct elmm_case elmm_case;
 typedef struct elmm_handle elmm_handle;
 
-/* Axis-aligned solid cuboid, metres: xmin,xmax,ymin,ymax,zmin,zmax, and the
- * roughness length z0 of its walls (0 = smooth wall law).  Equivalent to a
- * POLYHEDRON of six planes in the reference's obstacle file
- * (src/geometricshapes.f90:2655-2760) wrapped as SolidBody(z0=z0)
- * (src/solidbodies.f90:439-454). */
-typedef struct elmm_cuboid { double xmin, xmax, ymin, ymax, zmin, zmax, z0; } elmm_cuboid;
+/* Axis-aligned solid cuboid, metres: xmin,xmax,ymin,ymax,zmin,zmax, the
+ * roughness length z0 of its walls (0 = smooth wall law), and the surface
+ * heat flux (K m/s) and moisture flux of its walls (rough walls only; with
+ * buoyancy and moisture).  Equivalent to a POLYHEDRON of six planes in the
+ * reference's obstacle file (src/geometricshapes.f90:2655-2760) wrapped as
+ * SolidBody(z0, temperature_flux, moisture_flux) (src/solidbodies.f90:
+ * 439-454; the fluxes reach its wall-model points, src/immersedboundary.f90:
+ * 111-125, 260-285). */
+typedef struct elmm_cuboid {
+  double xmin, xmax, ymin, ymax, zmin, zmax, z0;
+  double temperature_flux, moisture_flux;
+} elmm_cuboid;
 
 /* Scalar point source: position (m), flux (units/s), scalar index. */
 typedef struct elmm_point_source { double x, y, z, flux; int32_t scalar; int32_t pad; } elmm_point_source;

@@ -38,7 +38,7 @@ def _cases():
                                moist_btype=[capi.PERIODIC] * 4 + [capi.CONSTFLUX, capi.DIRICHLET],
                                side_moist=[0, 0, 0, 0, 1e-4, 0.008])
     top.z0_wall[5] = 0.02
-    out.append(("noslip top", top, [(0.0, 6.0, 4.0, 10.0, 0.0, 5.0), (20.0, 32.0, 18.0, 24.0, 0.0, 8.0, 0.03),
+    out.append(("noslip top", top, [(0.0, 6.0, 4.0, 10.0, 0.0, 5.0), (20.0, 32.0, 18.0, 24.0, 0.0, 8.0, 0.03, 0.2, 3e-5),
                                     (10.0, 14.0, 10.0, 14.0, 12.0, 16.0)], [(15.5, 12.5, 0.5, 1.0, 0)]))
     out.append(("no wall model", cases.uniform_config(32, 32, 16, dx=1.0, dt=0.05, nscalars=1, wall_model=0), CUB, []))
     dx07 = cases.uniform_config(40, 36, 20, dx=0.7, dt=0.05, nscalars=1)

@@ -163,6 +163,24 @@ def test_wall_points_take_the_thermal_class():
     assert (tcu == capi.WM_SURFACE_FLUX).sum() > 500
 
 
+HEATED = [(8.0, 16.0, 8.0, 16.0, 0.0, 6.0), (22.0, 30.0, 18.0, 28.0, 0.0, 9.0, 0.02, 0.2, 3e-5)]
+
+
+def test_rough_body_carries_its_surface_fluxes():
+    """A rough cuboid's heat and moisture flux reach its wall-model points
+    (the flux class); a smooth one's points stay neutral."""
+    cfg = buoyant_config(enable_moisture=1, moisture_ref=0.01, moist_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN] * 2,
+                         side_moist=[0] * 6)
+    case = capi.Case(cfg, cuboids=HEATED + [(2.0, 5.0, 24.0, 28.0, 0.0, 4.0, 0.0, 0.5, 1e-4)])
+    _, _, z0 = case.wmp_list()
+    tc, tv = case.wm_thermal(-1)
+    _, qf = case.wm_moist(-1)
+    body = z0 == 0.02  # the heated body's points (the floor's z0 is 0.01)
+    assert body.sum() > 50
+    assert np.all(tc[body] == capi.WM_SURFACE_FLUX) and np.all(tv[body] == 0.2) and np.all(qf[body] == 3e-5)
+    assert np.all(tc[~body] == capi.WM_NEUTRAL) and np.all(qf[~body] == 0)
+
+
 @pytest.mark.parametrize("where", ["bottom", "top"])
 def test_oracle_heat_flux_warms_the_wall_layer(where):
     """Fluid at rest, uniform temperature, a surface heat flux q on one wall
@@ -297,7 +315,8 @@ def test_moisture_flux_needs_a_wall_model_wall():
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
 @pytest.mark.parametrize("bottom,top,moist", [("temp", "slip", None), ("flux", "slip", None), ("flux", "wall", None),
-                                              ("temp", "slip", "dirichlet"), ("temp", "wall", "flux")])
+                                              ("temp", "slip", "dirichlet"), ("temp", "wall", "flux"),
+                                              ("body", "slip", "flux")])
 def test_gpu_monin_obukhov_bit_identical_to_oracle(require_gpu, bottom, top, moist):
     kw = {}
     if moist:
@@ -305,8 +324,10 @@ def test_gpu_monin_obukhov_bit_identical_to_oracle(require_gpu, bottom, top, moi
         mv = 0.014 if moist == "dirichlet" else 2e-4
         kw = dict(enable_moisture=1, moisture_ref=0.01, moist_btype=[capi.PERIODIC] * 4 + [mb, capi.NEUMANN],
                   side_moist=[0, 0, 0, 0, mv, 0])
-    cfg = mo_config(48, 32, 16, bottom=bottom, top=top, **kw)
-    case = capi.Case(cfg, cuboids=OBST, sources=[(4.5, 4.5, 0.5, 1.0, 0)])
+    if bottom == "body":  # a neutral floor, a heated wet rough cuboid
+        kw.update(temp_btype=[capi.PERIODIC] * 4 + [capi.NEUMANN, capi.DIRICHLET], side_temp=[0, 0, 0, 0, 0.0, T_TOP])
+    cfg = mo_config(48, 32, 16, bottom="temp", top=top, **kw)
+    case = capi.Case(cfg, cuboids=HEATED if bottom == "body" else OBST, sources=[(4.5, 4.5, 0.5, 1.0, 0)])
     f = cases.initial_fields(cfg, u0=2.0, amp=0.3, seed=41)
     T = temperature_field(cfg, seed=42)
     o, s = Oracle(cfg, case), capi.Solver(cfg, case)
