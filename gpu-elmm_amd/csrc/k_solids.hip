@@ -57,21 +57,6 @@ struct DevError {
     if (e_ != hipSuccess) throw DevError{std::string(#x) + ": " + hipGetErrorString(e_)}; \
   } while (0)
 
-// Device allocations of one build, freed together.
-struct Arena {
-  std::vector<void *> ptrs;
-  template <class T>
-  T *get(size_t n) {
-    void *q = nullptr;
-    SCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
-    ptrs.push_back(q);
-    return static_cast<T *>(q);
-  }
-  ~Arena() {
-    for (void *q : ptrs) (void)hipFree(q);
-  }
-};
-
 struct Cell {
   int i, j, k;
 };
